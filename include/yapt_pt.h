@@ -1,0 +1,169 @@
+/* C ABI of the MI355X spectral path-tracing core.
+ *
+ * One handle = one path integrator instance bound to one GPU, the role of
+ * YAPT::PathIntegratorSubStage (Source/Renderer/inc/Renderer/Shared/RenderPipeline/PathTracer/
+ * PathIntegratorSubStage.h:17-72), which PathTraceStage constructs by type
+ * (Source/Renderer/src/Shared/RenderPipeline/PathTracer/PathTraceStage.cpp:40-54) and drives
+ * through sceneChanged / update / getOutput. Every call returns 0 on success and a negative
+ * YPT_E* code on failure; ypt_last_error() describes the last failure of the calling thread.
+ * The caller owns host buffers; the library owns device buffers. A handle is used by one thread
+ * at a time.
+ */
+#ifndef YAPT_PT_H
+#define YAPT_PT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YPT_OK 0
+#define YPT_E_INVALID (-1)   /* bad argument or state */
+#define YPT_E_DEVICE (-2)    /* HIP runtime error */
+#define YPT_E_MEMORY (-3)    /* device allocation failed */
+#define YPT_E_NOSCENE (-4)   /* render before set_scene / set_frame */
+
+/* integrator types: RendererVarsList.h:24 option order of "RenderPipeline" */
+#define YPT_INTEGRATOR_BPT 0
+#define YPT_INTEGRATOR_NEE 1  /* BPT_WITH_NEE */
+#define YPT_INTEGRATOR_BDPT 2
+
+/* frame flags */
+#define YPT_FLAG_DISABLE_JITTER (1u << 1)  /* CONSTANTS_FLAG_DISABLE_TEXEL_JITTER */
+#define YPT_FLAG_WHITE_FURNACE (1u << 8)   /* WHITE_FURNACE_TEST as a runtime switch */
+
+/* material mask bits (MaterialInternal.h / materialsCommon.hlsl:8-10) */
+#define YPT_MATERIAL_TWO_SIDED (1u << 0)
+#define YPT_MATERIAL_DISPERSION (1u << 1)
+
+/* Material parameters as the reference's MaterialParameters (Source/Renderer/inc/Renderer/Shared/
+ * MaterialInternal.h:41-125); the library applies the validation of MaterialInternal.cpp:27-43 and
+ * the packing clamps of BindlessMaterialManager.cpp:70-100. Colours are linear RGB. */
+typedef struct ypt_material {
+  float albedo[3];
+  float transparency;
+  float specular[3];
+  float metalness;
+  float absorption[3];
+  float dielectric_ior;
+  float emissive[3];
+  float roughness;
+  float anisotropy;
+  float anisotropy_rotation;
+  uint32_t material_mask;
+  float specular_amount;
+  float clear_coat_amount;
+  float clear_coat_ior;
+  float clear_coat_roughness;
+  float sheen_tint[3];
+  float sheen_roughness;
+  float sheen_amount;
+  float thin_film_thickness_nm;
+  float emission_focus;
+  float alpha_cutoff;
+  float cauchy_coeffs[2];
+  float subsurface_radius[3];
+  float subsurface_amount;
+  float subsurface_radius_scale;
+} ypt_material;
+
+/* Scene: world-space triangle soup (render-object transforms already applied, the
+ * SceneData::objToWorld of PathIntegratorSubStage::sceneChanged), one record per instance
+ * (render object x submesh) owning a contiguous triangle range. Mesh lights are the instances
+ * whose material is emissive (LightManager.cpp:145). */
+typedef struct ypt_scene_desc {
+  const float* positions;       /* 3 * vertex_count */
+  const float* normals;         /* 3 * vertex_count, or NULL for area-weighted vertex normals */
+  const float* tangents;        /* 4 * vertex_count (xyz, handedness w), or NULL: tangent (1, 0, 0) as for
+                                   meshes without a tangent buffer (hitShadersCommon.hlsl:198) */
+  uint32_t vertex_count;
+  const uint32_t* indices;      /* 3 * triangle_count */
+  uint32_t triangle_count;
+  const uint32_t* instance_first_triangle;
+  const uint32_t* instance_triangle_count;
+  const uint32_t* instance_material;
+  uint32_t instance_count;
+  const ypt_material* materials;
+  uint32_t material_count;
+} ypt_scene_desc;
+
+/* Per-frame constants: the fields of RaytraceConstantData (raytraceCommonResources.hlsl:8-31) the
+ * host fills in RaytraceCommonResources::update (RaytraceCommonResources.cpp:64-140), the film
+ * merge parameters (mergeNewSamples.comp:7-19) and the integrator parameters. Matrices are
+ * row-major and act on column vectors (out = M * v). */
+typedef struct ypt_frame_desc {
+  uint32_t width, height;
+  float uv_to_view[16];
+  float view_to_uv[16];
+  float world_to_view[16];
+  float view_to_world[16];
+  float aperture_radius;
+  float focus_distance;
+  float forward_z_sign;
+  float env_intensity_scale;
+  uint32_t max_depth;            /* TraceRay segments per path */
+  uint32_t regularize_after;     /* RegularizeAfterVertices (default 8) */
+  uint32_t flags;                /* YPT_FLAG_* */
+  float firefly_clamp_scale;     /* FireflyClampScale (default 12, 0 disables) */
+  float firefly_clamp_relax_power;
+  uint32_t kill_nan_samples;
+} ypt_frame_desc;
+
+typedef struct ypt_integrator ypt_integrator;
+
+/* PathTraceStage.cpp:40-54 (construct by type) + PathIntegratorSubStage::initialize */
+int ypt_create(int integrator_type, int device, ypt_integrator** out);
+/* PathIntegratorSubStage::shutdown + delete */
+int ypt_destroy(ypt_integrator* h);
+/* PathIntegratorSubStage::sceneChanged (PathIntegratorSubStage.h:62-71) + the acceleration-structure
+ * build of PathTraceStage::beforeExecute (PathTraceStage.cpp:215-251). */
+int ypt_set_scene(ypt_integrator* h, const ypt_scene_desc* scene);
+/* RaytraceCommonResources::update (RaytraceCommonResources.cpp:64-140) + onRenderResolutionChanged.
+ * A change of resolution clears the film. */
+int ypt_set_frame(ypt_integrator* h, const ypt_frame_desc* frame);
+/* clearAccumulatedSamples (clearAccumulatedSamples.comp:1-41; PathTraceStage.cpp:158-161) */
+int ypt_clear_film(ypt_integrator* h);
+/* PathIntegratorSubStage::update + executeRaytrace + mergeNewSamples, for Sobol sample indices
+ * [sample_begin, sample_begin + sample_count): one pass per index, each merged into the film.
+ * stream: a hipStream_t or NULL for the library's stream. Asynchronous; see ypt_synchronize. */
+int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, void* stream);
+int ypt_synchronize(ypt_integrator* h);
+/* getOutput(COLOR): merged film, W*H float4 (X, Y, Z, 1) into host memory; counts: W*H uint32 or NULL */
+int ypt_get_film(ypt_integrator* h, float* xyzw, uint32_t* counts);
+/* last pass before merging: W*H float4 (X, Y, Z, hero wavelength) */
+int ypt_get_last_samples(ypt_integrator* h, float* xyzl);
+/* getOutput(MATERIAL_PARAMS0/1) of the last pass: first-hit features, W*H x 12 floats
+ * (normal xyz, albedo xyz, roughness, ior, metalness, transparency, depth, hit) */
+int ypt_get_first_hit_features(ypt_integrator* h, float* out);
+/* statistics of the last ypt_render, 16 doubles:
+ *  [0] extension rays, [1] shadow rays, [6]/[7] BVH nodes visited / triangles tested by extension
+ *      rays, [8]/[9] the same for shadow rays   (when ypt_set_option("count_traversal", 1));
+ *  [2] extension, [3] shadow, [4] shading, [11] path start, [12] film merge kernel time in ms and
+ *      [5] the whole render in ms                (when ypt_set_option("profile", 1));
+ *  [10] extension launches. */
+int ypt_get_stats(ypt_integrator* h, double* out16);
+/* tuning knobs: "bvh_width" (2 or 8), "block_size" (64..1024), "count_traversal" (0/1),
+ * "sort_by_material" (0/1) */
+int ypt_set_option(ypt_integrator* h, const char* name, int64_t value);
+
+/* Closest-hit queries against the scene's BVH on the GPU (the TraceRay / RayQuery the integrators
+ * use). rays: n x 8 floats (origin xyz, tmin, direction xyz, tmax). Outputs per ray: t, triangle
+ * index (0xFFFFFFFF on a miss) and barycentrics b1, b2 (may be NULL). */
+int ypt_trace_closest(ypt_integrator* h, const float* rays, uint32_t n, float* t, uint32_t* tri, float* b12);
+
+/* Scrambled sampler values on the GPU: for each (seed, dimension) pair the value of
+ * randomSampler.hlsl getRandomSampleFloat for Sobol sample `sample_index`. */
+int ypt_sample_dims(ypt_integrator* h, uint32_t sample_index, const uint32_t* seeds, const uint32_t* dims,
+                    uint32_t n, float* out);
+
+/* Sobol point of a sample index: 256 uint32 dimensions (RaytraceCommonResources.cpp:623-627). */
+int ypt_sobol_point(uint32_t sample_index, uint32_t* out256);
+
+const char* ypt_last_error(void);
+const char* ypt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,461 @@
+// CPU checker for the spectral path-tracing core. TEST INFRASTRUCTURE ONLY: tests/, the smoke check
+// and the CPU-baseline leg of bench.py load it; the product never does.
+//
+// A scalar restatement of the reference's per-pixel integrator loop, one path per thread as the
+// DXR ray-generation shader runs it (Source/Renderer/shaders/raytracing/rayGenPrimaryRays.hlsl:10-145),
+// with the shadow ray of each explicit light connection traced at once (rayHitDefault.hlsl:83-111:
+// the committed closest hit must be the sampled light triangle), followed by the film merge of
+// mergeNewSamples.comp:173-265. Surface and light arithmetic (layered BSDF, spectra, sampler,
+// camera) comes from the shared FP32 core under yapt_amd/csrc/core, compiled here by g++ with
+// -ffp-contract=off; what this file owns is independent of the GPU path:
+//   * closest hit by brute force over all triangles, or through its own median-split BVH
+//     (both give the minimum (t, triangle id) of the shared watertight test);
+//   * the path loop and the per-pixel film merge (clamp against the previous pass's film);
+//   * Sobol points straight from the generator matrices.
+// Parity unpinned against the reference itself (no per-sample vectors exist); pinned by the tests
+// listed in the repository's design notes.
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../include/yapt_pt.h"
+#include "../yapt_amd/csrc/core/intersect.h"
+#include "../yapt_amd/csrc/core/shade.h"
+#include "../yapt_amd/csrc/runtime/luts.h"
+#include "../yapt_amd/csrc/runtime/prepare.h"
+#include "../yapt_amd/csrc/runtime/rgb2spec.h"
+#include "../yapt_amd/csrc/data/cie_tables.inc"
+#include "../yapt_amd/csrc/data/sobol_matrices.inc"
+
+using namespace ypt;
+
+namespace {
+
+std::string g_err;
+
+struct Tables {
+  std::vector<float> msl;
+  Rgb2Spec r2s;
+  std::vector<float> rgb2specFull;
+  Tables() {
+    msl.resize(MSL_TOTAL);
+    generate_ms_luts(msl.data());
+    r2s.init(YPT_CIE_X_D, YPT_CIE_Y_D, YPT_CIE_Z_D, YPT_CIE_D65_D, YPT_SRGB_XYZ_TO_RGB_D, YPT_SRGB_RGB_TO_XYZ_D);
+  }
+};
+Tables& tables() {
+  static Tables t;
+  return t;
+}
+
+// Sobol point of a sample index: RandUtility.inl:33-45 for dimensions 0..255
+void sobol_point(uint32_t index, uint32_t out[256]) {
+  for (uint32_t d = 0; d < 256; ++d) {
+    uint32_t v = 0, idx = index;
+    for (uint32_t i = d * 32; idx != 0; idx >>= 1, ++i)
+      if (idx & 1) v ^= YPT_SOBOL_MATRICES[i];
+    out[d] = v;
+  }
+}
+
+// ------------------------------------------------------------------ closest hit
+struct CpuBvhNode {
+  float lo[3], hi[3];
+  uint32_t left, right;   // inner: child indices
+  uint32_t first, count;  // leaf: triangle range in `order` (count > 0)
+};
+
+struct Geometry {
+  const PreparedScene* S = nullptr;
+  bool useBvh = false;
+  std::vector<CpuBvhNode> nodes;
+  std::vector<uint32_t> order;
+
+  void tri(uint32_t t, vec3& a, vec3& b, vec3& c) const {
+    const float* p = S->pos.data();
+    const uint32_t* ix = S->idx.data();
+    a = load3(p, ix[3 * t]); b = load3(p, ix[3 * t + 1]); c = load3(p, ix[3 * t + 2]);
+  }
+
+  void build(const PreparedScene& s, bool bvh) {
+    S = &s;
+    useBvh = bvh;
+    nodes.clear();
+    order.clear();
+    if (!bvh) return;
+    uint32_t n = (uint32_t)s.triInst.size();
+    order.resize(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::vector<float> cent(3 * n);
+    for (uint32_t t = 0; t < n; ++t) {
+      vec3 a, b, c;
+      tri(t, a, b, c);
+      for (int k = 0; k < 3; ++k) cent[3 * t + k] = (a[k] + b[k] + c[k]) / 3.f;
+    }
+    struct Item { uint32_t node, first, count; };
+    std::vector<Item> st;
+    nodes.push_back(CpuBvhNode());
+    st.push_back({0, 0, n});
+    while (!st.empty()) {
+      Item it = st.back();
+      st.pop_back();
+      CpuBvhNode nd;
+      for (int k = 0; k < 3; ++k) { nd.lo[k] = FLT_MAX; nd.hi[k] = -FLT_MAX; }
+      for (uint32_t i = it.first; i < it.first + it.count; ++i) {
+        vec3 v[3];
+        tri(order[i], v[0], v[1], v[2]);
+        for (int j = 0; j < 3; ++j)
+          for (int k = 0; k < 3; ++k) { nd.lo[k] = std::min(nd.lo[k], v[j][k]); nd.hi[k] = std::max(nd.hi[k], v[j][k]); }
+      }
+      nd.left = nd.right = 0;
+      nd.first = it.first;
+      nd.count = it.count;
+      if (it.count > 4) {  // median split on the widest centroid axis
+        float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (uint32_t i = it.first; i < it.first + it.count; ++i)
+          for (int k = 0; k < 3; ++k) { clo[k] = std::min(clo[k], cent[3 * order[i] + k]); chi[k] = std::max(chi[k], cent[3 * order[i] + k]); }
+        int ax = 0;
+        for (int k = 1; k < 3; ++k) if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+        uint32_t mid = it.first + it.count / 2;
+        std::nth_element(order.begin() + it.first, order.begin() + mid, order.begin() + it.first + it.count,
+                         [&](uint32_t a, uint32_t b) { return cent[3 * a + ax] < cent[3 * b + ax] || (cent[3 * a + ax] == cent[3 * b + ax] && a < b); });
+        uint32_t l = (uint32_t)nodes.size();
+        nodes.push_back(CpuBvhNode());
+        nodes.push_back(CpuBvhNode());
+        nd.left = l; nd.right = l + 1; nd.count = 0;
+        st.push_back({l, it.first, mid - it.first});
+        st.push_back({l + 1, mid, it.first + it.count - mid});
+      }
+      nodes[it.node] = nd;
+    }
+  }
+
+  // minimum (t, id) over triangles hit in (tmin, tmax]
+  bool closest(vec3 o, vec3 d, float tmin, float tmax, Hit& h, uint64_t* work) const {
+    RayPre rp = ray_precompute(o, d);
+    float best = tmax;
+    uint32_t bid = NO_HIT_ID;
+    float bb1 = 0.f, bb2 = 0.f;
+    auto test = [&](uint32_t t) {
+      vec3 a, b, c;
+      tri(t, a, b, c);
+      float tt, b1, b2;
+      if (intersect_triangle(rp, a, b, c, tmin, best, tt, b1, b2) && hit_closer(tt, t, best, bid)) {
+        best = tt; bid = t; bb1 = b1; bb2 = b2;
+      }
+    };
+    if (!useBvh) {
+      uint32_t n = (uint32_t)S->triInst.size();
+      for (uint32_t t = 0; t < n; ++t) test(t);
+      if (work) work[1] += n;
+    } else {
+      vec3 inv(1.f / d.x, 1.f / d.y, 1.f / d.z);
+      uint32_t stack[128];
+      int sp = 0;
+      stack[sp++] = 0;
+      while (sp) {
+        const CpuBvhNode& nd = nodes[stack[--sp]];
+        if (work) work[0]++;
+        float tn;
+        if (!slab_test(vec3(nd.lo[0], nd.lo[1], nd.lo[2]), vec3(nd.hi[0], nd.hi[1], nd.hi[2]), o, inv, tmin, best, tn)) continue;
+        if (nd.count) {
+          for (uint32_t i = nd.first; i < nd.first + nd.count; ++i) test(order[i]);
+          if (work) work[1] += nd.count;
+        } else {
+          if (sp + 2 > 128) throw std::runtime_error("oracle BVH stack overflow");
+          stack[sp++] = nd.right;
+          stack[sp++] = nd.left;
+        }
+      }
+    }
+    h.tri = bid;
+    h.t = bid == NO_HIT_ID ? 0.f : best;
+    h.b1 = bb1;
+    h.b2 = bb2;
+    return bid != NO_HIT_ID;
+  }
+};
+
+struct Film {
+  uint32_t w = 0, h = 0;
+  std::vector<float> accum;   // 2 layers x npix x 3
+  std::vector<float> merged;  // npix x 3
+  std::vector<uint32_t> counts;
+  uint32_t samples = 0;
+};
+
+// mergeNewSamples.comp:173-265 for one pass, clamp against the film before this pass
+void merge_pass(Film& F, const std::vector<float>& sampleXyz, const ypt_frame_desc& fd) {
+  size_t npix = (size_t)F.w * F.h;
+  std::vector<float> prevMerged = F.merged;
+  std::vector<uint32_t> prevCounts = F.counts;
+  for (uint32_t y = 0; y < F.h; ++y)
+    for (uint32_t x = 0; x < F.w; ++x) {
+      size_t p = (size_t)y * F.w + x;
+      float c[3] = {sampleXyz[3 * p], sampleXyz[3 * p + 1], sampleXyz[3 * p + 2]};
+      float alpha = 1.f;
+      if (fd.kill_nan_samples && (c[0] != c[0] || c[1] != c[1] || c[2] != c[2])) { c[0] = c[1] = c[2] = 0.f; alpha = 0.f; }
+      uint32_t newS = (uint32_t)alpha;
+      uint32_t oldS = prevCounts[p];
+      if (newS > 0) {
+        float scale = fd.firefly_clamp_scale;
+        if (!(scale <= 0.f || oldS == 0) && !(c[0] <= 0.f && c[1] <= 0.f && c[2] <= 0.f)) {
+          float ref[3] = {0.f, 0.f, 0.f};
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+              int lx = (int)x + dx, ly = (int)y + dy;
+              if (lx < 0 || ly < 0 || lx >= (int)F.w || ly >= (int)F.h) continue;
+              size_t q = (size_t)ly * F.w + lx;
+              if (prevCounts[q] == 0) continue;
+              for (int k = 0; k < 3; ++k) {
+                float m = prevMerged[3 * q + k] > 0.f ? prevMerged[3 * q + k] : 0.f;
+                ref[k] = ref[k] > m ? ref[k] : m;
+              }
+            }
+          float lvl = ref[0] > ref[1] ? ref[0] : ref[1];
+          lvl = lvl > ref[2] ? lvl : ref[2];
+          if (lvl > 0.f) {
+            for (int k = 0; k < 3; ++k) ref[k] = ref[k] > lvl * 0.05f ? ref[k] : lvl * 0.05f;
+            float relax = ypow((float)oldS > 1.f ? (float)oldS : 1.f, fd.firefly_clamp_relax_power);
+            float s = 1.f;
+            for (int k = 0; k < 3; ++k) {
+              float allowed = ref[k] * scale * relax;
+              if (c[k] > allowed) s = s < allowed / c[k] ? s : allowed / c[k];
+            }
+            for (int k = 0; k < 3; ++k) c[k] = c[k] * s;
+          }
+        }
+        F.counts[p] = newS + oldS;
+        uint32_t layer = F.samples % 2;
+        uint32_t old = oldS / 2;
+        double total = (double)(newS + old);
+        double db = old / total, sb = newS / total;
+        float* a = &F.accum[(layer * npix + p) * 3];
+        for (int k = 0; k < 3; ++k) a[k] = (float)(a[k] * db + sb * c[k]);
+      }
+      float sum[3] = {F.accum[p * 3], F.accum[p * 3 + 1], F.accum[p * 3 + 2]};
+      uint32_t targets = F.samples + 1 < 2 ? F.samples + 1 : 2;
+      for (uint32_t k = 1; k < targets; ++k)
+        for (int j = 0; j < 3; ++j) sum[j] = sum[j] + F.accum[(k * npix + p) * 3 + j];
+      for (int j = 0; j < 3; ++j) F.merged[3 * p + j] = sum[j] / (float)targets;
+    }
+  F.samples++;
+}
+
+struct Ctx {
+  PreparedScene scene;
+  Geometry geo;
+  FrameConstants fc;
+  SceneView sv;
+  bool nee = true;
+};
+
+void make_view(Ctx& c) {
+  Tables& T = tables();
+  c.sv.pos = c.scene.pos.data();
+  c.sv.nrm = c.scene.nrm.data();
+  c.sv.tan = c.scene.tan.empty() ? nullptr : c.scene.tan.data();
+  c.sv.idx = c.scene.idx.data();
+  c.sv.triInst = c.scene.triInst.data();
+  c.sv.inst = c.scene.inst.data();
+  c.sv.mats = c.scene.mats.data();
+  c.sv.lights = c.scene.lights.empty() ? nullptr : c.scene.lights.data();
+  c.sv.rgb2spec = T.rgb2specFull.empty() ? nullptr : T.rgb2specFull.data();
+  c.sv.ntris = (uint32_t)c.scene.triInst.size();
+  c.sv.ninst = (uint32_t)c.scene.inst.size();
+  c.sv.spec.cie_xyz4 = YPT_CIE_XYZ;
+  c.sv.spec.d65 = YPT_CIE_D65;
+  c.sv.luts.base = T.msl.data();
+}
+
+// rayGenPrimaryRays with rayHitDefault / rayMissEnvironment / inline shadow query
+vec3 trace_pixel(const Ctx& c, const uint32_t* point, uint32_t px, uint32_t py, float& lambdaOut, uint64_t* work,
+                 FirstHitFeatures* fh) {
+  PathState ps;
+  start_path(c.fc, point, px, py, ps);
+  for (uint32_t i = 0; i < c.fc.maxRayDepth; ++i) {
+    if (ps.rayState != RAY_ALIVE) break;
+    Hit h;
+    c.geo.closest(ps.origin, ps.dir, RAY_MIN_T, __builtin_inff(), h, work);
+    if (work) work[2]++;
+    ShadowRay sr;
+    sr.valid = false;
+    if (h.tri == NO_HIT_ID) shade_miss(c.sv, c.fc, ps);
+    else shade_hit(c.sv, c.fc, point, c.nee, ps, h, sr, i == 0 ? fh : nullptr);
+    if (sr.valid) {
+      Hit sh;
+      c.geo.closest(sr.origin, sr.dir, RAY_MIN_T, sr.tmax, sh, work);
+      if (work) work[3]++;
+      if (sh.tri == sr.targetTri) ps.totalLight = ps.totalLight + sr.contrib;
+    }
+    after_segment(point, ps);
+  }
+  lambdaOut = ps.lambda;
+  return path_xyz(c.sv, ps);
+}
+
+int setup(Ctx& c, const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integrator, int useBvh) {
+  if (integrator != YPT_INTEGRATOR_NEE && integrator != YPT_INTEGRATOR_BPT) { g_err = "oracle: integrator not supported"; return YPT_E_INVALID; }
+  prepare_scene(*sd, tables().r2s, c.scene);
+  prepare_frame(*fd, c.scene, (uint32_t)integrator, c.fc);
+  if (fd->env_intensity_scale != 0.f && !(fd->flags & YPT_FLAG_WHITE_FURNACE) && tables().rgb2specFull.empty())
+    tables().r2s.full_table(tables().rgb2specFull, std::max(1u, std::thread::hardware_concurrency()));
+  c.nee = integrator == YPT_INTEGRATOR_NEE;
+  c.geo.build(c.scene, useBvh != 0);
+  make_view(c);
+  return YPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_err.c_str(); }
+
+int oracle_get_ms_luts(float* out) {
+  std::memcpy(out, tables().msl.data(), MSL_TOTAL * sizeof(float));
+  return MSL_TOTAL;
+}
+
+int oracle_sobol_point(uint32_t index, uint32_t* out256) {
+  sobol_point(index, out256);
+  return YPT_OK;
+}
+
+int oracle_sample_dims(uint32_t sample_index, const uint32_t* seeds, const uint32_t* dims, uint32_t n, float* out) {
+  uint32_t pt[256];
+  sobol_point(sample_index, pt);
+  for (uint32_t i = 0; i < n; ++i) out[i] = sample_dim(pt, dims[i], seeds[i]);
+  return YPT_OK;
+}
+
+// rgb -> (c0, c1, c2) through the same grid lookup the library uses
+int oracle_rgb_coeffs(const float* rgb, float* out) {
+  tables().r2s.coeffs(rgb, out);
+  return YPT_OK;
+}
+
+int oracle_trace_closest(const ypt_scene_desc* sd, const float* rays, uint32_t n, int useBvh, float* t, uint32_t* tri, float* b12) {
+  try {
+    Ctx c;
+    prepare_scene(*sd, tables().r2s, c.scene);
+    c.geo.build(c.scene, useBvh != 0);
+    for (uint32_t i = 0; i < n; ++i) {
+      const float* r = rays + (size_t)i * 8;
+      Hit h;
+      c.geo.closest(vec3(r[0], r[1], r[2]), vec3(r[4], r[5], r[6]), r[3], r[7], h, nullptr);
+      t[i] = h.t; tri[i] = h.tri;
+      if (b12) { b12[2 * i] = h.tri == NO_HIT_ID ? 0.f : h.b1; b12[2 * i + 1] = h.tri == NO_HIT_ID ? 0.f : h.b2; }
+    }
+    return YPT_OK;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return YPT_E_INVALID;
+  }
+}
+
+// Renders passes [sample_begin, sample_begin + sample_count) and merges them into a film.
+// Optional outputs: film (W*H*4: X, Y, Z, 1), counts (W*H), last pass samples (W*H*4: X, Y, Z,
+// hero wavelength), first-hit features of the last pass (W*H*12). pixel_stride > 1 renders only every
+// pixel_stride-th pixel of every pass (a bounded CPU sample; the film is then partial).
+// stats[0] = wall seconds, [1] = path samples, [2] = extension rays, [3] = shadow rays,
+// [4] = BVH nodes visited, [5] = triangles tested.
+int oracle_render(const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integrator, uint32_t sample_begin,
+                  uint32_t sample_count, int threads, int useBvh, uint32_t pixel_stride, float* film, uint32_t* counts,
+                  float* last, float* features, double* stats) {
+  try {
+    Ctx c;
+    int rc = setup(c, sd, fd, integrator, useBvh);
+    if (rc) return rc;
+    uint32_t W = fd->width, H = fd->height;
+    size_t npix = (size_t)W * H;
+    uint32_t stride = pixel_stride > 1 ? pixel_stride : 1;
+    size_t traced = (npix + stride - 1) / stride;
+    Film F;
+    F.w = W; F.h = H;
+    F.accum.assign(2 * npix * 3, 0.f);
+    F.merged.assign(npix * 3, 0.f);
+    F.counts.assign(npix, 0);
+    std::vector<float> xyz(npix * 3, 0.f), lam(npix, 0.f);
+    std::vector<FirstHitFeatures> fh(features ? npix : 0);
+    unsigned nt = threads > 0 ? (unsigned)threads : std::max(1u, std::thread::hardware_concurrency());
+    const uint32_t TILE = 16;
+    uint32_t tilesX = (W + TILE - 1) / TILE, tilesY = (H + TILE - 1) / TILE;
+    std::vector<std::array<uint64_t, 4>> work(nt);
+    for (auto& w : work) w = {0, 0, 0, 0};
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t k = 0; k < sample_count; ++k) {
+      uint32_t point[256];
+      sobol_point(sample_begin + k, point);
+      if (features)
+        for (auto& f : fh) std::memset(&f, 0, sizeof(f));
+      std::atomic<uint32_t> next{0};
+      std::vector<std::thread> th;
+      for (unsigned ti = 0; ti < nt; ++ti)
+        th.emplace_back([&, ti]() {
+          uint64_t* wk = stats ? work[ti].data() : nullptr;
+          while (true) {
+            uint32_t tile = next.fetch_add(1);
+            if (tile >= tilesX * tilesY) break;
+            uint32_t tx = tile % tilesX, ty = tile / tilesX;
+            for (uint32_t y = ty * TILE; y < std::min(H, ty * TILE + TILE); ++y)
+              for (uint32_t x = tx * TILE; x < std::min(W, tx * TILE + TILE); ++x) {
+                size_t p = (size_t)y * W + x;
+                if (stride > 1 && (p % stride) != 0) continue;
+                float l;
+                vec3 v = trace_pixel(c, point, x, y, l, wk, features ? &fh[p] : nullptr);
+                xyz[3 * p] = v.x; xyz[3 * p + 1] = v.y; xyz[3 * p + 2] = v.z;
+                lam[p] = l;
+              }
+          }
+        });
+      for (auto& x : th) x.join();
+      merge_pass(F, xyz, *fd);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (film)
+      for (size_t p = 0; p < npix; ++p) {
+        film[4 * p] = F.merged[3 * p]; film[4 * p + 1] = F.merged[3 * p + 1]; film[4 * p + 2] = F.merged[3 * p + 2]; film[4 * p + 3] = 1.f;
+      }
+    if (counts) std::memcpy(counts, F.counts.data(), npix * sizeof(uint32_t));
+    if (last)
+      for (size_t p = 0; p < npix; ++p) {
+        last[4 * p] = xyz[3 * p]; last[4 * p + 1] = xyz[3 * p + 1]; last[4 * p + 2] = xyz[3 * p + 2]; last[4 * p + 3] = lam[p];
+      }
+    if (features)
+      for (size_t p = 0; p < npix; ++p) {
+        float* f = features + 12 * p;
+        const FirstHitFeatures& a = fh[p];
+        if (!a.hit) { std::memset(f, 0, 12 * sizeof(float)); continue; }
+        f[0] = a.normal.x; f[1] = a.normal.y; f[2] = a.normal.z;
+        f[3] = a.albedo.x; f[4] = a.albedo.y; f[5] = a.albedo.z;
+        f[6] = a.roughness; f[7] = a.ior; f[8] = a.metalness; f[9] = a.transparency; f[10] = a.depth; f[11] = 1.f;
+      }
+    if (stats) {
+      uint64_t tot[4] = {0, 0, 0, 0};
+      for (auto& w : work) for (int k = 0; k < 4; ++k) tot[k] += w[k];
+      stats[0] = std::chrono::duration<double>(t1 - t0).count();
+      stats[1] = (double)traced * sample_count;
+      stats[2] = (double)tot[2];
+      stats[3] = (double)tot[3];
+      stats[4] = (double)tot[0];
+      stats[5] = (double)tot[1];
+      stats[6] = (double)nt;
+    }
+    return YPT_OK;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return YPT_E_INVALID;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+"""ctypes binding of the CPU checker (liboracle.so). Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_LIB = None
+
+
+def load():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("liboracle.so is not built; run oracle/build.py")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    FP = ctypes.POINTER(ctypes.c_float)
+    UP = ctypes.POINTER(ctypes.c_uint32)
+    lib.oracle_render.argtypes = [P, P, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_uint32, FP, UP, FP, FP, ctypes.POINTER(ctypes.c_double)]
+    lib.oracle_trace_closest.argtypes = [P, FP, ctypes.c_uint32, ctypes.c_int, FP, UP, FP]
+    lib.oracle_sample_dims.argtypes = [ctypes.c_uint32, UP, UP, ctypes.c_uint32, FP]
+    lib.oracle_sobol_point.argtypes = [ctypes.c_uint32, UP]
+    lib.oracle_get_ms_luts.argtypes = [FP]
+    lib.oracle_rgb_coeffs.argtypes = [FP, FP]
+    lib.oracle_last_error.restype = ctypes.c_char_p
+    _LIB = lib
+    return lib
+
+
+def _f(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _u(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle error %d: %s" % (rc, load().oracle_last_error().decode()))
+
+
+def render(scene, frame, integrator, sample_begin, sample_count, threads=0, use_bvh=True, pixel_stride=1,
+           features=False):
+    lib = load()
+    W, H = frame.width, frame.height
+    film = np.zeros((H, W, 4), np.float32)
+    counts = np.zeros((H, W), np.uint32)
+    last = np.zeros((H, W, 4), np.float32)
+    feat = np.zeros((H, W, 12), np.float32) if features else None
+    stats = np.zeros(8, np.float64)
+    _check(lib.oracle_render(ctypes.addressof(scene.desc), ctypes.addressof(frame), int(integrator), int(sample_begin),
+                             int(sample_count), int(threads), int(bool(use_bvh)), int(pixel_stride), _f(film), _u(counts),
+                             _f(last), None if feat is None else _f(feat),
+                             stats.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    out = dict(film=film, counts=counts, last=last, stats=stats)
+    if features:
+        out["features"] = feat
+    return out
+
+
+def trace_closest(scene, rays, use_bvh=False):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = len(rays)
+    t = np.zeros(n, np.float32)
+    tri = np.zeros(n, np.uint32)
+    b = np.zeros((n, 2), np.float32)
+    _check(load().oracle_trace_closest(ctypes.addressof(scene.desc), _f(rays), n, int(bool(use_bvh)), _f(t), _u(tri), _f(b)))
+    return t, tri, b
+
+
+def sample_dims(sample_index, seeds, dims):
+    seeds = np.ascontiguousarray(seeds, np.uint32)
+    dims = np.ascontiguousarray(dims, np.uint32)
+    out = np.zeros(len(seeds), np.float32)
+    _check(load().oracle_sample_dims(int(sample_index), _u(seeds), _u(dims), len(seeds), _f(out)))
+    return out
+
+
+def sobol_point(index):
+    out = np.zeros(256, np.uint32)
+    _check(load().oracle_sobol_point(int(index), _u(out)))
+    return out
+
+
+def ms_luts():
+    out = np.zeros(16384, np.float32)
+    n = load().oracle_get_ms_luts(_f(out))
+    return out[:n]
+
+
+def rgb_coeffs(rgb):
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    out = np.zeros(3, np.float32)
+    _check(load().oracle_rgb_coeffs(_f(rgb), _f(out)))
+    return out

@@ -1,0 +1,233 @@
+"""ctypes binding of the C ABI declared in include/yapt_pt.h.
+
+The shared library ``libyapt_pt.so`` is built in-tree by ``yapt_amd.build``. Loading fails loudly
+when it is missing: there is no CPU fallback for the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libyapt_pt.so")
+
+YPT_INTEGRATOR_BPT = 0
+YPT_INTEGRATOR_NEE = 1
+YPT_INTEGRATOR_BDPT = 2
+YPT_FLAG_DISABLE_JITTER = 1 << 1
+YPT_FLAG_WHITE_FURNACE = 1 << 8
+YPT_MATERIAL_TWO_SIDED = 1 << 0
+YPT_MATERIAL_DISPERSION = 1 << 1
+
+F3 = ctypes.c_float * 3
+F16 = ctypes.c_float * 16
+
+
+class Material(ctypes.Structure):
+    """ypt_material: the reference's MaterialParameters (MaterialInternal.h:41-125)."""
+    _fields_ = [
+        ("albedo", F3), ("transparency", ctypes.c_float),
+        ("specular", F3), ("metalness", ctypes.c_float),
+        ("absorption", F3), ("dielectric_ior", ctypes.c_float),
+        ("emissive", F3), ("roughness", ctypes.c_float),
+        ("anisotropy", ctypes.c_float), ("anisotropy_rotation", ctypes.c_float),
+        ("material_mask", ctypes.c_uint32), ("specular_amount", ctypes.c_float),
+        ("clear_coat_amount", ctypes.c_float), ("clear_coat_ior", ctypes.c_float),
+        ("clear_coat_roughness", ctypes.c_float), ("sheen_tint", F3),
+        ("sheen_roughness", ctypes.c_float), ("sheen_amount", ctypes.c_float),
+        ("thin_film_thickness_nm", ctypes.c_float), ("emission_focus", ctypes.c_float),
+        ("alpha_cutoff", ctypes.c_float), ("cauchy_coeffs", ctypes.c_float * 2),
+        ("subsurface_radius", F3), ("subsurface_amount", ctypes.c_float),
+        ("subsurface_radius_scale", ctypes.c_float),
+    ]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("positions", ctypes.POINTER(ctypes.c_float)), ("normals", ctypes.POINTER(ctypes.c_float)),
+        ("tangents", ctypes.POINTER(ctypes.c_float)), ("vertex_count", ctypes.c_uint32), ("indices", ctypes.POINTER(ctypes.c_uint32)),
+        ("triangle_count", ctypes.c_uint32), ("instance_first_triangle", ctypes.POINTER(ctypes.c_uint32)),
+        ("instance_triangle_count", ctypes.POINTER(ctypes.c_uint32)),
+        ("instance_material", ctypes.POINTER(ctypes.c_uint32)), ("instance_count", ctypes.c_uint32),
+        ("materials", ctypes.POINTER(Material)), ("material_count", ctypes.c_uint32),
+    ]
+
+
+class FrameDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+        ("uv_to_view", F16), ("view_to_uv", F16), ("world_to_view", F16), ("view_to_world", F16),
+        ("aperture_radius", ctypes.c_float), ("focus_distance", ctypes.c_float),
+        ("forward_z_sign", ctypes.c_float), ("env_intensity_scale", ctypes.c_float),
+        ("max_depth", ctypes.c_uint32), ("regularize_after", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+        ("firefly_clamp_scale", ctypes.c_float), ("firefly_clamp_relax_power", ctypes.c_float),
+        ("kill_nan_samples", ctypes.c_uint32),
+    ]
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _uptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+class SceneArrays:
+    """Keeps the numpy arrays a SceneDesc points into alive."""
+
+    def __init__(self, positions, normals, indices, inst_first, inst_count, inst_material, materials, tangents=None):
+        self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        self.tangents = None if tangents is None else np.ascontiguousarray(tangents, dtype=np.float32).reshape(-1, 4)
+        self.indices = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+        self.inst_first = np.ascontiguousarray(inst_first, dtype=np.uint32)
+        self.inst_count = np.ascontiguousarray(inst_count, dtype=np.uint32)
+        self.inst_material = np.ascontiguousarray(inst_material, dtype=np.uint32)
+        self.materials = (Material * len(materials))(*materials)
+        self.desc = SceneDesc(
+            _fptr(self.positions), None if self.normals is None else _fptr(self.normals),
+            None if self.tangents is None else _fptr(self.tangents), len(self.positions),
+            _uptr(self.indices), len(self.indices), _uptr(self.inst_first), _uptr(self.inst_count),
+            _uptr(self.inst_material), len(self.inst_first), self.materials, len(materials))
+
+    @property
+    def triangle_count(self):
+        return len(self.indices)
+
+
+class YptError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load():
+    """Loads libyapt_pt.so (built by yapt_amd.build). Raises when it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise YptError("libyapt_pt.so is not built; run yapt_amd.build.build() (no CPU fallback exists)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    lib.ypt_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]
+    lib.ypt_destroy.argtypes = [P]
+    lib.ypt_set_scene.argtypes = [P, ctypes.POINTER(SceneDesc)]
+    lib.ypt_set_frame.argtypes = [P, ctypes.POINTER(FrameDesc)]
+    lib.ypt_clear_film.argtypes = [P]
+    lib.ypt_render.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, P]
+    lib.ypt_synchronize.argtypes = [P]
+    lib.ypt_get_film.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint32)]
+    lib.ypt_get_last_samples.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
+    lib.ypt_get_first_hit_features.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
+    lib.ypt_get_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+    lib.ypt_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_int64]
+    lib.ypt_trace_closest.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
+                                      ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_float)]
+    lib.ypt_sample_dims.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                    ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]
+    lib.ypt_sobol_point.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+    lib.ypt_last_error.restype = ctypes.c_char_p
+    lib.ypt_version.restype = ctypes.c_char_p
+    _LIB = lib
+    return lib
+
+
+EXPORTED_SYMBOLS = [
+    "ypt_create", "ypt_destroy", "ypt_set_scene", "ypt_set_frame", "ypt_clear_film", "ypt_render",
+    "ypt_synchronize", "ypt_get_film", "ypt_get_last_samples", "ypt_get_first_hit_features", "ypt_get_stats",
+    "ypt_set_option", "ypt_trace_closest", "ypt_sample_dims", "ypt_sobol_point", "ypt_last_error", "ypt_version",
+]
+
+
+def check(rc):
+    if rc != 0:
+        raise YptError("yapt_pt error %d: %s" % (rc, load().ypt_last_error().decode()))
+
+
+class Integrator:
+    """One path-integrator instance on one GPU (the role of PathIntegratorSubStage)."""
+
+    def __init__(self, integrator_type=YPT_INTEGRATOR_NEE, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.ypt_create(int(integrator_type), int(device), ctypes.byref(h)))
+        self.h = h
+        self.width = self.height = 0
+        self._scene = None
+
+    def close(self):
+        if self.h:
+            self.lib.ypt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_option(self, name, value):
+        check(self.lib.ypt_set_option(self.h, name.encode(), int(value)))
+
+    def set_scene(self, scene):
+        check(self.lib.ypt_set_scene(self.h, ctypes.byref(scene.desc)))
+        self._scene = scene
+
+    def set_frame(self, frame):
+        check(self.lib.ypt_set_frame(self.h, ctypes.byref(frame)))
+        self.width, self.height = frame.width, frame.height
+
+    def clear_film(self):
+        check(self.lib.ypt_clear_film(self.h))
+
+    def render(self, sample_begin, sample_count, stream=None):
+        check(self.lib.ypt_render(self.h, int(sample_begin), int(sample_count), stream))
+
+    def synchronize(self):
+        check(self.lib.ypt_synchronize(self.h))
+
+    def film(self, with_counts=False):
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        cnt = np.zeros((self.height, self.width), np.uint32) if with_counts else None
+        check(self.lib.ypt_get_film(self.h, _fptr(out), None if cnt is None else _uptr(cnt)))
+        return (out, cnt) if with_counts else out
+
+    def last_samples(self):
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        check(self.lib.ypt_get_last_samples(self.h, _fptr(out)))
+        return out
+
+    def first_hit_features(self):
+        out = np.zeros((self.height, self.width, 12), np.float32)
+        check(self.lib.ypt_get_first_hit_features(self.h, _fptr(out)))
+        return out
+
+    def stats(self):
+        out = np.zeros(16, np.float64)
+        check(self.lib.ypt_get_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        tri = np.zeros(n, np.uint32)
+        b = np.zeros((n, 2), np.float32)
+        check(self.lib.ypt_trace_closest(self.h, _fptr(rays), n, _fptr(t), _uptr(tri), _fptr(b)))
+        return t, tri, b
+
+    def sample_dims(self, sample_index, seeds, dims):
+        seeds = np.ascontiguousarray(seeds, np.uint32)
+        dims = np.ascontiguousarray(dims, np.uint32)
+        out = np.zeros(len(seeds), np.float32)
+        check(self.lib.ypt_sample_dims(self.h, int(sample_index), _uptr(seeds), _uptr(dims), len(seeds), _fptr(out)))
+        return out
+
+
+def sobol_point(index):
+    out = np.zeros(256, np.uint32)
+    check(load().ypt_sobol_point(int(index), _uptr(out)))
+    return out

@@ -1,0 +1,51 @@
+"""In-tree build of the gfx950 library (libyapt_pt.so).
+
+Flags: -ffp-contract=off and IEEE-correct fp32 division/sqrt so device arithmetic matches the host
+build bit for bit; packed-fp32 instructions disabled; -O3 for gfx950 only.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc", "ypt_lib.hip")
+OUT = os.path.join(HERE, "libyapt_pt.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    "--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def sources():
+    out = []
+    for d, _, files in os.walk(os.path.join(HERE, "csrc")):
+        for f in files:
+            if f.endswith((".h", ".hip", ".inc")):
+                out.append(os.path.join(d, f))
+    out.append(os.path.join(ROOT, "include", "yapt_pt.h"))
+    return out
+
+
+def build(verbose=False):
+    cmd = [HIPCC] + HIP_FLAGS + ["-o", OUT + ".tmp", SRC]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    errs = "\n".join(l for l in res.stderr.splitlines() if "not a recognized feature" not in l)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed (%d):\n%s" % (res.returncode, errs))
+    if verbose and errs.strip():
+        print(errs)
+    os.replace(OUT + ".tmp", OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,111 @@
+// Shared scalar/vector vocabulary of the path-tracing core.
+//
+// Every function here is compiled twice: by hipcc for gfx950 device code and by g++ for the host
+// (scene preparation, LUT integration, and the CPU checker). Both builds use -ffp-contract=off and
+// IEEE-correct division and square root, so a function evaluated on the same inputs returns the
+// same bits on both sides. Operation order follows the HLSL the functions restate.
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define YPT_HD __host__ __device__ __forceinline__
+#define YPT_DEVICE_CODE 1
+#else
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#define YPT_HD inline
+#endif
+
+#include <stdint.h>
+
+namespace ypt {
+
+static constexpr float PI = 3.14159265f;        // shaders/common/commonMath.hlsl:5
+static constexpr float INVPI = 0.31830988f;     // :6
+static constexpr float PI_OVER_2 = 1.57079633f; // :7
+static constexpr float PI_OVER_4 = 0.785398163f;// :8
+
+YPT_HD float as_float(uint32_t u) { float f; __builtin_memcpy(&f, &u, 4); return f; }
+YPT_HD uint32_t as_uint(float f) { uint32_t u; __builtin_memcpy(&u, &f, 4); return u; }
+YPT_HD int32_t as_int(float f) { return (int32_t)as_uint(f); }
+YPT_HD float as_float_i(int32_t i) { return as_float((uint32_t)i); }
+
+YPT_HD float fmin_(float a, float b) { return a < b ? a : b; }   // HLSL min on non-NaN operands
+YPT_HD float fmax_(float a, float b) { return a > b ? a : b; }
+YPT_HD float clampf(float x, float lo, float hi) { return fmin_(fmax_(x, lo), hi); }
+YPT_HD float saturate(float x) { return clampf(x, 0.f, 1.f); }
+YPT_HD float sqr(float x) { return x * x; }
+YPT_HD float lerpf(float a, float b, float t) { return a + t * (b - a); }
+YPT_HD float signf(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+YPT_HD bool is_nan(float x) { return x != x; }
+YPT_HD bool is_inf(float x) { return (as_uint(x) & 0x7fffffffu) == 0x7f800000u; }
+YPT_HD bool is_finite(float x) { return (as_uint(x) & 0x7f800000u) != 0x7f800000u; }
+YPT_HD float safe_div(float n, float d) { return d == 0.f ? 0.f : n / d; }   // common.hlsl:13-22
+YPT_HD float safe_divisor(float x) { return x == 0.f ? 1.f : x; }             // common.hlsl:4
+
+struct float2_ { float x, y; };
+
+struct vec2 {
+  float x, y;
+  YPT_HD vec2() : x(0.f), y(0.f) {}
+  YPT_HD vec2(float a, float b) : x(a), y(b) {}
+};
+
+struct vec3 {
+  float x, y, z;
+  YPT_HD vec3() : x(0.f), y(0.f), z(0.f) {}
+  YPT_HD explicit vec3(float s) : x(s), y(s), z(s) {}
+  YPT_HD vec3(float a, float b, float c) : x(a), y(b), z(c) {}
+  YPT_HD float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+  YPT_HD float& at(int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+YPT_HD vec3 operator+(vec3 a, vec3 b) { return vec3(a.x + b.x, a.y + b.y, a.z + b.z); }
+YPT_HD vec3 operator-(vec3 a, vec3 b) { return vec3(a.x - b.x, a.y - b.y, a.z - b.z); }
+YPT_HD vec3 operator*(vec3 a, vec3 b) { return vec3(a.x * b.x, a.y * b.y, a.z * b.z); }
+YPT_HD vec3 operator/(vec3 a, vec3 b) { return vec3(a.x / b.x, a.y / b.y, a.z / b.z); }
+YPT_HD vec3 operator*(vec3 a, float s) { return vec3(a.x * s, a.y * s, a.z * s); }
+YPT_HD vec3 operator*(float s, vec3 a) { return vec3(s * a.x, s * a.y, s * a.z); }
+YPT_HD vec3 operator/(vec3 a, float s) { return vec3(a.x / s, a.y / s, a.z / s); }
+YPT_HD vec3 operator-(vec3 a) { return vec3(-a.x, -a.y, -a.z); }
+YPT_HD float dot(vec3 a, vec3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+YPT_HD vec3 cross(vec3 a, vec3 b) {
+  return vec3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+YPT_HD float length(vec3 a) { return sqrtf(dot(a, a)); }
+YPT_HD vec3 normalize(vec3 a) { return a / length(a); }
+YPT_HD vec3 vabs(vec3 a) { return vec3(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
+YPT_HD vec3 vmin(vec3 a, vec3 b) { return vec3(fmin_(a.x, b.x), fmin_(a.y, b.y), fmin_(a.z, b.z)); }
+YPT_HD vec3 vmax(vec3 a, vec3 b) { return vec3(fmax_(a.x, b.x), fmax_(a.y, b.y), fmax_(a.z, b.z)); }
+YPT_HD bool is_zero(vec3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+YPT_HD float max3(vec3 a) { return fmax_(fmax_(a.x, a.y), a.z); }
+YPT_HD vec3 reflect(vec3 i, vec3 n) { return i - (2.f * dot(n, i)) * n; }   // HLSL reflect
+// HLSL refract(i, n, eta): zero vector on total internal reflection.
+YPT_HD vec3 refract(vec3 i, vec3 n, float eta) {
+  float ni = dot(n, i);
+  float k = 1.f - eta * eta * (1.f - ni * ni);
+  if (k < 0.f) return vec3(0.f);
+  return eta * i - (eta * ni + sqrtf(k)) * n;
+}
+
+// Row-major 3x3: rows r0, r1, r2 (HLSL float3x3(r0, r1, r2)).
+struct mat3 {
+  vec3 r0, r1, r2;
+  YPT_HD mat3() {}
+  YPT_HD mat3(vec3 a, vec3 b, vec3 c) : r0(a), r1(b), r2(c) {}
+};
+YPT_HD vec3 mul(const mat3& m, vec3 v) { return vec3(dot(m.r0, v), dot(m.r1, v), dot(m.r2, v)); }
+// row vector times matrix: v.x * r0 + v.y * r1 + v.z * r2
+YPT_HD vec3 mul(vec3 v, const mat3& m) { return (v.x * m.r0 + v.y * m.r1) + v.z * m.r2; }
+
+// Row-major 4x4 (HLSL float4x4 as uploaded from glm column-major with the HLSL default packing).
+struct mat4 {
+  float m[16];  // m[row * 4 + col]
+};
+YPT_HD void mul4(const mat4& M, const float v[4], float out[4]) {
+  for (int r = 0; r < 4; ++r)
+    out[r] = ((M.m[r * 4 + 0] * v[0] + M.m[r * 4 + 1] * v[1]) + M.m[r * 4 + 2] * v[2]) + M.m[r * 4 + 3] * v[3];
+}
+
+}  // namespace ypt

@@ -1,0 +1,469 @@
+// Per-vertex work of the unidirectional integrators (BPT and BPT_WITH_NEE).
+//
+// Restates shaders/raytracing/rayGenPrimaryRays.hlsl:10-145 (path start, bounce loop, Russian
+// roulette), rayHitDefault.hlsl:27-585 (explicit light connection, MIS, surface evaluation and next
+// direction), rayMissEnvironment.hlsl:13-31, hitShadersCommon.hlsl:138-160 (transmittance, ray
+// origin offset), lightSampling.hlsl:31-228,360-370 (light pick, light sampling, density conversion),
+// raytraceCommonResources.hlsl:108-156 (thin-lens camera) and payload.hlsl (volume stack).
+//
+// The shadow ray of an explicit light connection is returned to the caller instead of being traced
+// here, so that one driver can trace it at once and another can batch all of them in a separate pass:
+// the contribution and the order in which it reaches totalLight are the same either way.
+#pragma once
+#include "scene.h"
+#include "sampler.h"
+
+namespace ypt {
+
+static constexpr float RAY_MIN_T = 5e-4f;  // DEFAULT_RAY_MIN_T
+static constexpr uint32_t RAY_ALIVE = 0, RAY_TERMINATED = 1, RAY_CANCELLED = 2;
+static constexpr uint32_t RS_SECONDARY_TERMINATED = 1u << 0;
+static constexpr uint32_t RS_REGULARIZE = 1u << 1;
+static constexpr uint32_t RS_DELTA = 1u << 2;
+static constexpr uint32_t VOLUME_NONE = 0xFFFFFFFFu;
+static constexpr int MAX_VOLUMES = 4;
+static constexpr uint32_t NO_HIT = 0xFFFFFFFFu;
+
+struct PathState {
+  Spectrum throughput, totalLight;
+  vec3 origin, dir;
+  float lambda;
+  float pdfThisRay;
+  uint32_t dim, seed, flags, pathLength, rayState, nvol;
+  uint32_t furnaceInstance;  // WHITE_FURNACE_TEST: first object the path hit (rayIndex in the reference)
+  float volIor[MAX_VOLUMES];
+  Spectrum volAbs[MAX_VOLUMES];
+  uint32_t volMat[MAX_VOLUMES];
+};
+
+struct Hit {
+  float t, b1, b2;  // b1, b2: weights of the second and third vertex (DXR barycentrics)
+  uint32_t tri;     // NO_HIT on a miss
+};
+
+struct ShadowRay {
+  vec3 origin, dir;
+  float tmax;
+  uint32_t targetTri;  // closest hit must be this triangle for the light to count
+  Spectrum contrib;    // added to totalLight when visible
+  bool valid;
+};
+
+// first-hit features (hitShadersCommon.hlsl:94-136), filled when requested
+struct FirstHitFeatures {
+  vec3 normal;
+  vec3 albedo;
+  float roughness, ior, metalness, transparency, anisotropy, depth;
+  uint32_t hit;
+};
+
+// ---------------------------------------------------------------- volume stack (payload.hlsl)
+YPT_HD float current_ior(const PathState& p) { return p.nvol != 0 ? p.volIor[p.nvol - 1] : 1.0f; }
+YPT_HD float previous_ior(const PathState& p) { return p.nvol > 1 ? p.volIor[p.nvol - 2] : 1.0f; }
+YPT_HD Spectrum current_absorption(const PathState& p) { return p.nvol != 0 ? p.volAbs[p.nvol - 1] : Spectrum(0.f); }
+YPT_HD void entered_volume(PathState& p, float ior, const Spectrum& absorption, uint32_t mat) {
+  p.nvol = p.nvol + 1 < (uint32_t)MAX_VOLUMES ? p.nvol + 1 : (uint32_t)MAX_VOLUMES;
+  p.volIor[p.nvol - 1] = ior;
+  p.volAbs[p.nvol - 1] = absorption;
+  p.volMat[p.nvol - 1] = mat;
+}
+YPT_HD void exited_volume(PathState& p, uint32_t mat) {
+  if (p.nvol == 0) return;
+  int index = -1;
+  for (int i = (int)p.nvol - 1; i >= 0; --i)
+    if (p.volMat[i] == mat) { index = i; break; }
+  if (index < 0) index = (int)p.nvol - 1;
+  for (uint32_t i = (uint32_t)index; i + 1 < p.nvol; ++i) {
+    p.volIor[i] = p.volIor[i + 1];
+    p.volAbs[i] = p.volAbs[i + 1];
+    p.volMat[i] = p.volMat[i + 1];
+  }
+  --p.nvol;
+}
+
+// "A Fast and Robust Method for Avoiding Self-Intersection", Ray Tracing Gems ch. 6 (hitShadersCommon.hlsl:147-160)
+YPT_HD vec3 spawn_ray_origin(vec3 position, vec3 geometryNormal, vec3 rayDir) {
+  const float originThreshold = 1.0f / 32.0f;
+  const float floatScale = 1.0f / 65536.0f;
+  const float intScale = 256.0f;
+  vec3 n = dot(rayDir, geometryNormal) < 0.f ? -geometryNormal : geometryNormal;
+  int ox = (int)(intScale * n.x), oy = (int)(intScale * n.y), oz = (int)(intScale * n.z);
+  int sx = position.x < 0.f ? -ox : ox;
+  int sy = position.y < 0.f ? -oy : oy;
+  int sz = position.z < 0.f ? -oz : oz;
+  float px = as_float_i(as_int(position.x) + sx);
+  float py = as_float_i(as_int(position.y) + sy);
+  float pz = as_float_i(as_int(position.z) + sz);
+  return vec3(fabsf(position.x) < originThreshold ? position.x + floatScale * n.x : px,
+              fabsf(position.y) < originThreshold ? position.y + floatScale * n.y : py,
+              fabsf(position.z) < originThreshold ? position.z + floatScale * n.z : pz);
+}
+
+YPT_HD float weight_mis(float a, float b) {  // power heuristic
+  float a2 = a * a, b2 = b * b;
+  return a2 / (a2 + b2);
+}
+YPT_HD float solid_angle_to_area(vec3 v, vec3 n) {
+  float invDistSqr = 1.f / dot(v, v);
+  float absDot = fabsf(dot(n, v * sqrtf(invDistSqr)));
+  return absDot * invDistSqr;
+}
+YPT_HD float area_to_solid_angle(vec3 v, vec3 n) { return 1.f / solid_angle_to_area(v, n); }
+
+// ---------------------------------------------------------------- camera
+YPT_HD vec3 xform_point(const mat4& M, vec3 p) {
+  float v[4] = {p.x, p.y, p.z, 1.f}, o[4];
+  mul4(M, v, o);
+  return vec3(o[0], o[1], o[2]);
+}
+YPT_HD vec3 xform_dir(const mat4& M, vec3 d) {
+  float v[4] = {d.x, d.y, d.z, 0.f}, o[4];
+  mul4(M, v, o);
+  return vec3(o[0], o[1], o[2]);
+}
+
+// rayGenPrimaryRays.hlsl:13-39 and generateCameraRay (raytraceCommonResources.hlsl:115-137)
+YPT_HD void start_path(const FrameConstants& fc, const uint32_t* point, uint32_t px, uint32_t py, PathState& ps) {
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0];
+  uint32_t rayIndex = py * W + px;
+  RandomSampler rand;
+  rand.init(point, 0u, rayIndex);
+  float lambda = hero_wavelength(rand.r1());
+  float uvx = (float)px * fc.targetTexDimensions[2] + fc.rayUVOffset[0];
+  float uvy = (float)py * fc.targetTexDimensions[3] + fc.rayUVOffset[1];
+  float jx, jy;
+  rand.texel_offset(jx, jy);
+  float lens[2];
+  rand.r2(lens);
+  float ox = 0.f, oy = 0.f;
+  if ((fc.flags & FRAME_FLAG_DISABLE_JITTER) == 0) { ox = jx - 0.5f; oy = jy - 0.5f; }
+  ox *= fc.targetTexDimensions[2];
+  oy *= fc.targetTexDimensions[3];
+  float nearp[4] = {uvx + ox, uvy + oy, 0.0f, 1.0f}, v[4];
+  mul4(fc.uvToView, nearp, v);
+  vec3 pinhole(v[0] / v[3], v[1] / v[3], v[2] / v[3]);
+  vec3 focusPoint = pinhole * safe_div(fc.lensParams[1], fabsf(pinhole.z));
+  float dx, dy;
+  sample_concentric_disk(lens[0], lens[1], dx, dy);
+  vec3 lensPos(dx * fc.lensParams[0], dy * fc.lensParams[0], 0.f);
+  ps.origin = xform_point(fc.viewToWorld, lensPos);
+  ps.dir = normalize(xform_dir(fc.viewToWorld, focusPoint - lensPos));
+  ps.throughput.set(1.f);
+  ps.totalLight.set(0.f);
+  ps.rayState = RAY_ALIVE;
+  ps.pathLength = 0;
+  ps.nvol = 0;
+  for (int i = 0; i < MAX_VOLUMES; ++i) { ps.volMat[i] = VOLUME_NONE; ps.volIor[i] = 1.f; ps.volAbs[i].set(0.f); }
+  ps.flags = 0;
+  ps.furnaceInstance = VOLUME_NONE;
+  ps.pdfThisRay = 0.f;
+  ps.dim = rand.dim;
+  ps.seed = rand.seed;
+  ps.lambda = lambda;
+}
+
+// ---------------------------------------------------------------- miss
+YPT_HD Spectrum rgb_unbounded_spectrum(const SceneView& S, vec3 rgb, const Lambdas& L);
+
+YPT_HD void shade_miss(const SceneView& S, const FrameConstants& fc, PathState& ps) {
+  Lambdas L = spectral_lambdas(ps.lambda);
+  vec3 color;
+  if (fc.flags & FRAME_FLAG_WHITE_FURNACE) color = vec3(1.f);
+  else color = vabs(ps.dir) * fc.envIntensityScale;  // getSkyBoxColor with no environment texture
+  ps.totalLight = ps.totalLight + ps.throughput * rgb_unbounded_spectrum(S, color, L);
+  ps.rayState = RAY_TERMINATED;
+}
+
+// RGB -> coefficients via the 64^3 table (only needed for non-grey colours on the device).
+YPT_HD void rgb_coeffs(const SceneView& S, vec3 v, float c[3]) {
+  if (v.x == v.y && v.x == v.z) { c[0] = 0.f; c[1] = 0.f; c[2] = sigmoid_inv(v.x); return; }
+  const int R = 64;
+  int i0[3], i1[3]; float f[3];
+  for (int a = 0; a < 3; ++a) lut1_index(lut1_coord(v[a], R), R, i0[a], i1[a], f[a]);
+  for (int k = 0; k < 3; ++k) {
+    float cz[2];
+    for (int zz = 0; zz < 2; ++zz) {
+      int z = zz ? i1[2] : i0[2];
+      float cy[2];
+      for (int yy = 0; yy < 2; ++yy) {
+        int y = yy ? i1[1] : i0[1];
+        float a = S.rgb2spec[((z * R + y) * R + i0[0]) * 4 + k];
+        float b = S.rgb2spec[((z * R + y) * R + i1[0]) * 4 + k];
+        cy[yy] = a + f[0] * (b - a);
+      }
+      cz[zz] = cy[0] + f[1] * (cy[1] - cy[0]);
+    }
+    c[k] = cz[0] + f[2] * (cz[1] - cz[0]);
+  }
+}
+
+YPT_HD Spectrum rgb_unbounded_spectrum(const SceneView& S, vec3 values, const Lambdas& L) {  // setFromRGBUnbounded
+  float m = fmax_(fmax_(values.x, values.y), values.z);
+  float scale = 2.f * m;
+  vec3 nv = scale == 0.f ? vec3(0.f) : values / scale;
+  SpectralColor col;
+  if (nv.x == nv.y && nv.x == nv.z) { col.c[0] = 0.f; col.c[1] = 0.f; col.c[2] = sigmoid_inv(nv.x); }
+  else if (S.rgb2spec) rgb_coeffs(S, nv, col.c);
+  else { col.c[0] = 0.f; col.c[1] = 0.f; col.c[2] = as_float(0x7fc00000u); }  // table missing: NaN marks the sample
+  col.scale = scale;
+  return spectrum_unbounded(col, L, S.spec);
+}
+
+// ---------------------------------------------------------------- lights
+// sampleLightIndex / lightSelectionPDF (lightSampling.hlsl:31-42)
+YPT_HD uint32_t sample_light_index(const LightGPU* lights, uint32_t count, float u, float& pickPdf) {
+  float scaled = u * (float)count;
+  uint32_t index = (uint32_t)scaled;
+  if (index > count - 1) index = count - 1;
+  const LightGPU& e = lights[index];
+  uint32_t chosen = (scaled - (float)index) < e.aliasThreshold ? index : e.aliasIndex;
+  pickPdf = lights[chosen].selectionPdf;
+  return chosen;
+}
+
+struct ExplicitLightSample {
+  Spectrum emission;
+  vec3 toLight;
+  float distance;
+  float pdf;
+  uint32_t tri;
+};
+
+// sampleExplicitLight for mesh lights (rayHitDefault.hlsl:33-82) with sampleLight (lightSampling.hlsl:134-197)
+YPT_HD void sample_explicit_light(const SceneView& S, const FrameConstants& fc, vec3 pos, const float r[4], const Lambdas& L,
+                                  ExplicitLightSample& out) {
+  float pickPdf;
+  uint32_t li = sample_light_index(S.lights, fc.lightCount, r[3], pickPdf);
+  const LightGPU& light = S.lights[li];
+  uint32_t primCount = light.triCount;
+  uint32_t prim = (uint32_t)fmin_((float)primCount * r[2], (float)(primCount - 1u));
+  float bx = r[0], by = r[1];
+  if (bx + by > 1.f) { bx = 1.f - bx; by = 1.f - by; }
+  float b0 = 1.f - bx - by;
+  uint32_t tri = light.firstTri + prim;
+  vec3 p1, p2, p3;
+  tri_vertices(S, tri, p1, p2, p3);
+  vec3 gn = cross(p2 - p1, p3 - p1);
+  float gl = length(gn);
+  gn = gn / gl;
+  float area = gl * 0.5f;
+  vec3 lp = b0 * p1 + bx * p2 + by * p3;
+  const MaterialGPU& m = S.mats[light.material];
+  bool twoSided = surface_two_sided(m.flags);
+  Spectrum radiance = spectrum_unbounded(m.emissive, L, S.spec);
+  float pdfPos = 1.f / ((float)primCount * area);
+  vec3 toLight = lp - pos;
+  float len = length(toLight);
+  float cosEmit = -dot(gn, toLight / len);
+  out.emission = radiance * emission_profile(cosEmit, m.emissionFocus);
+  out.pdf = pickPdf * pdfPos * area_to_solid_angle(toLight, gn);
+  out.toLight = toLight / len;
+  out.distance = len;
+  out.tri = tri;
+  if (!twoSided && dot(toLight, gn) >= 0.f) { out.emission.set(0.f); out.pdf = 0.f; }
+}
+
+// calculateExplicitLightConnectionPDF (rayHitDefault.hlsl:139-169)
+YPT_HD float light_connection_pdf(const SceneView& S, vec3 fromToLight, uint32_t tri) {
+  const InstanceGPU& in = S.inst[S.triInst[tri]];
+  float pick = in.lightSelectionPdf;
+  vec3 p1, p2, p3;
+  tri_vertices(S, tri, p1, p2, p3);
+  vec3 gn = cross(p2 - p1, p3 - p1);
+  float gl = length(gn);
+  gn = gn / gl;
+  float area = gl * 0.5f;
+  float posPdf = 1.f / ((float)in.triCount * area);
+  return posPdf * pick * area_to_solid_angle(fromToLight, gn);
+}
+
+// ---------------------------------------------------------------- closest hit
+// rayHitDefault (rayHitDefault.hlsl:464-585) with evaluateSurfaceAndGenerateNextSampleDirection (:316-435)
+// and connectToExplicitLight (:204-257). `nee` selects BPT_WITH_NEE.
+YPT_HD void shade_hit(const SceneView& S, const FrameConstants& fc, const uint32_t* point, bool nee, PathState& ps,
+                      const Hit& h, ShadowRay& sr, FirstHitFeatures* fh) {
+  sr.valid = false;
+  Lambdas L = spectral_lambdas(ps.lambda);
+  uint32_t instId = S.triInst[h.tri];
+  const InstanceGPU& inst = S.inst[instId];
+  uint32_t matId = inst.material;
+  const MaterialGPU& m = S.mats[matId];
+  float b0 = 1.f - h.b1 - h.b2;
+  uint32_t i0 = S.idx[3 * h.tri], i1 = S.idx[3 * h.tri + 1], i2 = S.idx[3 * h.tri + 2];
+  vec3 p1 = load3(S.pos, i0), p2 = load3(S.pos, i1), p3 = load3(S.pos, i2);
+  vec3 geometryNormal = normalize(cross(p2 - p1, p3 - p1));
+  vec3 normal = normalize(b0 * load3(S.nrm, i0) + h.b1 * load3(S.nrm, i1) + h.b2 * load3(S.nrm, i2));
+  vec3 tangent(1.f, 0.f, 0.f);  // meshHasValidTangents ? fetchMeshTangent : (1, 0, 0)
+  if (S.tan) {
+    const float* t0 = S.tan + 4 * i0;
+    const float* t1 = S.tan + 4 * i1;
+    const float* t2 = S.tan + 4 * i2;
+    vec3 a(t0[0] * t0[3], t0[1] * t0[3], t0[2] * t0[3]);
+    vec3 b(t1[0] * t1[3], t1[1] * t1[3], t1[2] * t1[3]);
+    vec3 c(t2[0] * t2[3], t2[1] * t2[3], t2[2] * t2[3]);
+    tangent = normalize(b0 * a + h.b1 * b + h.b2 * c);
+  }
+  vec3 rayDir = normalize(ps.dir);
+  bool frontFace = dot(ps.dir, cross(p2 - p1, p3 - p1)) < 0.f;
+
+  if (ps.pathLength >= fc.regularizeAfterVertices) ps.flags |= RS_REGULARIZE;
+  bool regularizePath = (ps.flags & RS_REGULARIZE) != 0;
+  SurfaceDef sd = make_surface(m, geometryNormal, normal, tangent, -rayDir, true, regularizePath, L, S.spec);
+  vec3 hitPos = ps.origin + ps.dir * h.t;
+
+  if (ps.pathLength == 0 && fh) {  // writeMaterialParamsForFirstBounce
+    float cw = sd.clearCoatAmount, sw = sd.sheenAmount, bw = sd.specularAmount;
+    float wsum = cw + sw + bw;
+    if (wsum == 0.f) wsum = 1.f;
+    cw /= wsum; sw /= wsum; bw /= wsum;
+    vec3 n = sd.baseNormal;
+    if (sd.clearCoatAmount > 0.f) n = n + sd.coatingNormal;
+    fh->normal = normalize(n);
+    vec3 em(m.emissiveRGB[0], m.emissiveRGB[1], m.emissiveRGB[2]);
+    vec3 al(m.albedoRGB[0], m.albedoRGB[1], m.albedoRGB[2]);
+    vec3 sh(m.sheenRGB[0], m.sheenRGB[1], m.sheenRGB[2]);
+    fh->albedo = em / (vec3(1.f) + em) + al + sh * sd.sheenAmount;
+    fh->roughness = sd.roughness * bw + sd.clearCoatRoughness * cw + sd.sheenRoughness * sw;
+    fh->ior = sd.dielectricIOR * sd.specularAmount + sd.clearCoatIOR * sd.clearCoatAmount;
+    fh->anisotropy = sd.anisotropy;
+    fh->metalness = sd.metalness;
+    vec3 vp = xform_point(fc.worldToView, hitPos);
+    fh->depth = vp.z * fc.lensParams[2];
+    fh->transparency = sd.transparency;
+    fh->hit = 1u;
+  }
+
+  // handleRayStartInsideMedium (hitShadersCommon.hlsl:213-222)
+  if (dot(rayDir, geometryNormal) > 0.f && ps.nvol == 0 && !surface_two_sided(sd.flags))
+    entered_volume(ps, sd.dielectricIOR, sd.absorption, matId);
+  Spectrum absorb = current_absorption(ps);
+  if (!absorb.all_equal(0.f)) {
+    Spectrum tr;
+    for (int i = 0; i < 4; ++i) tr.s[i] = yexp(-absorb.s[i] * h.t);
+    ps.throughput = ps.throughput * tr;
+  }
+
+  if (!sd.emissive.all_equal(0.f)) {
+    float wMIS = 1.f;
+    if (nee && fc.lightCount > 0 && ps.pdfThisRay > 0.f && (ps.flags & RS_DELTA) == 0) {
+      float lightPdf = light_connection_pdf(S, hitPos - ps.origin, h.tri);
+      wMIS = weight_mis(ps.pdfThisRay, lightPdf);
+    }
+    float side = dot(sd.geometryNormal, -rayDir) < 0.f ? 0.f : 1.f;
+    float profile = 1.f;
+    if (sd.emissionFocus > 0.f) {
+      vec3 gnWS = normalize(sd.geometryNormal);
+      profile = emission_profile(dot(gnWS, -normalize(ps.dir)), sd.emissionFocus);
+    }
+    ps.totalLight = ps.totalLight + ps.throughput * sd.emissive * wMIS * side * profile;
+    ps.rayState = RAY_TERMINATED;
+    return;
+  }
+
+  if (fc.flags & FRAME_FLAG_WHITE_FURNACE) {  // handleWhiteFurnaceTest (rayHitDefault.hlsl:437-459)
+    if (ps.furnaceInstance == VOLUME_NONE) {
+      ps.furnaceInstance = instId;
+    } else if (ps.furnaceInstance != instId) {
+      Spectrum white = rgb_unbounded_spectrum(S, vec3(1.f), L);
+      ps.totalLight = ps.totalLight + ps.throughput * white;
+      ps.rayState = RAY_TERMINATED;
+      return;
+    }
+  }
+
+  RandomSampler rand;
+  rand.point = point; rand.dim = ps.dim; rand.seed = ps.seed;
+  float rb[4];
+  rand.r4(rb);
+  vec3 wo = -rayDir;
+  SurfaceContext ctx;
+  make_surface_context(ctx, sd, current_ior(ps), previous_ior(ps), wo, frontFace, L, false);
+  vec3 normalWS = normalize(sd.geometryNormal);
+
+  bool hasLights = nee ? fc.lightCount > 0 : false;
+  if (hasLights && !is_delta_distribution(ctx.pre.ax, ctx.pre.ay, sd)) {
+    float rl[4];
+    rand.r4(rl);
+    ExplicitLightSample ls;
+    sample_explicit_light(S, fc, hitPos, rl, L, ls);
+    if (!ls.emission.all_equal(0.f) && ls.pdf > 0.f) {
+      vec3 toLightOS = normalize(ls.toLight);
+      SurfaceEval ev = evaluate_surface(ctx, S.luts, wo, toLightOS, EVAL_NONE);
+      Spectrum w = ev.weight;
+      if ((ev.transmission & TRANSMISSION_DISPERSED) != 0 && (ps.flags & RS_SECONDARY_TERMINATED) == 0) w.terminate_secondary();
+      if (!w.all_equal(0.f)) {
+        sr.origin = spawn_ray_origin(hitPos, normalWS, ls.toLight);
+        sr.dir = ls.toLight;
+        sr.tmax = ls.distance + RAY_MIN_T;
+        sr.targetTri = ls.tri;
+        float wMIS = weight_mis(ls.pdf, ev.pdf);
+        w = (w / ls.pdf) * wMIS;
+        sr.contrib = ps.throughput * w * ls.emission;
+        sr.valid = true;
+      }
+    }
+  }
+
+  int layer = select_layer(rb[0], ctx.prob);
+  bool isDiffuse = true, isDelta = false;
+  vec3 wi(0.f);
+  if (layer != LAYER_SUBSURFACE) wi = sample_layer_direction(ctx, S.luts, layer, rb[1], rb[2], rb[3], isDiffuse, isDelta);
+  bool local = !is_zero(wi) && layer != LAYER_SUBSURFACE;
+  SurfaceEval ev;
+  ev.weight.set(0.f);
+  ev.pdf = 0.f;
+  ev.transmission = TRANSMISSION_NONE;
+  if (local) {
+    ev = evaluate_surface(ctx, S.luts, wo, wi, isDelta ? EVAL_TREAT_AS_DELTA : EVAL_NONE);
+    if (isDiffuse) ps.flags |= RS_REGULARIZE;
+  }
+  ps.pdfThisRay = ev.pdf;
+  if (isDelta) ps.flags |= RS_DELTA;
+  else ps.flags &= ~RS_DELTA;
+  if (ev.pdf > 0.f) {
+    if (ev.transmission != TRANSMISSION_NONE) {
+      if ((ev.transmission & TRANSMISSION_DISPERSED) != 0) {
+        if ((ps.flags & RS_SECONDARY_TERMINATED) == 0) ev.weight.terminate_secondary();
+        ps.flags |= RS_SECONDARY_TERMINATED;
+      }
+      bool entered = (ev.transmission & TRANSMISSION_ENTERED) != 0;
+      bool exited = (ev.transmission & TRANSMISSION_EXITED) != 0;
+      if (entered != exited) {
+        if (exited) exited_volume(ps, matId);
+        else entered_volume(ps, sd.dielectricIOR, sd.absorption, matId);
+      }
+    }
+    ev.weight = ev.weight / ev.pdf;
+  }
+  vec3 nextDir(0.f), nextOrigin = hitPos;
+  if (!is_zero(wi)) {
+    vec3 nWorld = normalize(sd.geometryNormal);
+    nextDir = normalize(wi);
+    nextOrigin = spawn_ray_origin(hitPos, nWorld, nextDir);
+  }
+  ps.throughput = ps.throughput * ev.weight;
+  ps.dim = rand.dim;
+  if (is_zero(nextDir) || ps.pdfThisRay == 0.0f) {
+    ps.rayState = RAY_CANCELLED;
+  } else {
+    ps.dir = nextDir;
+    ps.origin = nextOrigin;
+  }
+}
+
+// rayGenPrimaryRays.hlsl:72-101: after each traced segment, count it and apply Russian roulette.
+YPT_HD void after_segment(const uint32_t* point, PathState& ps) {
+  ps.pathLength += 1;
+  if (ps.rayState != RAY_ALIVE) return;  // roulette on a finished path changes nothing it outputs
+  RandomSampler r;
+  r.point = point; r.dim = ps.dim; r.seed = ps.seed;
+  if (ps.throughput.russian_roulette(r.r1())) { ps.rayState = RAY_TERMINATED; return; }
+  ps.dim = r.dim;
+}
+
+// float4(totalLight.ToXYZ(), 1)
+YPT_HD vec3 path_xyz(const SceneView& S, const PathState& ps) {
+  return spectrum_to_xyz(ps.totalLight, spectral_lambdas(ps.lambda), S.spec);
+}
+
+}  // namespace ypt

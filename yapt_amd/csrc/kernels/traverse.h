@@ -1,0 +1,183 @@
+// BVH traversal on gfx950: one ray per lane, short traversal stack in LDS with a global-memory
+// overflow, both children's boxes fetched in one 64-byte load (BVH2) or eight in 256 bytes (BVH8),
+// leaves intersected as soon as their box is hit. Closest hit = minimum (t, triangle id); the
+// shadow query stops at the first triangle that is closer than the light triangle.
+#pragma once
+#include "../core/intersect.h"
+#include "../runtime/bvh.h"
+
+namespace ypt {
+
+struct TraversalCounters {
+  uint32_t nodes;
+  uint32_t tris;
+};
+
+struct TravStack {
+  uint32_t* lds;      // [depth][block] in LDS
+  uint32_t* ovf;      // [depth_ovf][stride] in global memory
+  uint32_t block;     // lanes per block
+  uint32_t lane;      // threadIdx.x
+  uint32_t ldsDepth;
+  uint32_t ovfStride; // total threads of the launch
+  uint32_t gid;       // global thread id
+  uint32_t sp;
+  __device__ void push(uint32_t v) {
+    if (sp < ldsDepth) lds[sp * block + lane] = v;
+    else ovf[(size_t)(sp - ldsDepth) * ovfStride + gid] = v;
+    ++sp;
+  }
+  __device__ uint32_t pop() {
+    --sp;
+    if (sp < ldsDepth) return lds[sp * block + lane];
+    return ovf[(size_t)(sp - ldsDepth) * ovfStride + gid];
+  }
+};
+
+__device__ __forceinline__ TriRecord load_tri(const TriRecord* tris, uint32_t k) {
+  const float4* p = reinterpret_cast<const float4*>(tris + k);
+  float4 a = p[0], b = p[1], c = p[2];
+  TriRecord r;
+  r.v0[0] = a.x; r.v0[1] = a.y; r.v0[2] = a.z; r.id = __float_as_uint(a.w);
+  r.v1[0] = b.x; r.v1[1] = b.y; r.v1[2] = b.z;
+  r.v2[0] = c.x; r.v2[1] = c.y; r.v2[2] = c.z;
+  return r;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void leaf_closest(const TriRecord* tris, uint32_t ref, const RayPre& rp, float tmin, float& best,
+                                             uint32_t& bestId, float& bb1, float& bb2, TraversalCounters& cnt) {
+  uint32_t first = ref & 0x07FFFFFFu, count = ((ref >> 27) & 15u) + 1u;
+  for (uint32_t k = first; k < first + count; ++k) {
+    TriRecord r = load_tri(tris, k);
+    if (COUNT) cnt.tris++;
+    float t, b1, b2;
+    if (intersect_triangle(rp, vec3(r.v0[0], r.v0[1], r.v0[2]), vec3(r.v1[0], r.v1[1], r.v1[2]), vec3(r.v2[0], r.v2[1], r.v2[2]),
+                           tmin, best, t, b1, b2) &&
+        hit_closer(t, r.id, best, bestId)) {
+      best = t; bestId = r.id; bb1 = b1; bb2 = b2;
+    }
+  }
+}
+
+// true when a triangle other than `target` is hit closer than (targetT, target)
+template <bool COUNT>
+__device__ __forceinline__ bool leaf_occluded(const TriRecord* tris, uint32_t ref, const RayPre& rp, float tmin, float targetT,
+                                              uint32_t target, TraversalCounters& cnt) {
+  uint32_t first = ref & 0x07FFFFFFu, count = ((ref >> 27) & 15u) + 1u;
+  for (uint32_t k = first; k < first + count; ++k) {
+    TriRecord r = load_tri(tris, k);
+    if (COUNT) cnt.tris++;
+    if (r.id == target) continue;
+    float t, b1, b2;
+    if (intersect_triangle(rp, vec3(r.v0[0], r.v0[1], r.v0[2]), vec3(r.v1[0], r.v1[1], r.v1[2]), vec3(r.v2[0], r.v2[1], r.v2[2]),
+                           tmin, targetT, t, b1, b2) &&
+        hit_closer(t, r.id, targetT, target))
+      return true;
+  }
+  return false;
+}
+
+// MODE 0: closest hit. MODE 1: occlusion test against (targetT, target).
+template <int MODE, bool COUNT>
+__device__ bool traverse_bvh2(const Bvh2Node* nodes, const TriRecord* tris, vec3 o, vec3 d, float tmin, float tmax,
+                              TravStack& st, float& bestT, uint32_t& bestId, float& b1, float& b2, uint32_t target,
+                              TraversalCounters& cnt) {
+  RayPre rp = ray_precompute(o, d);
+  vec3 inv(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  float best = tmax;
+  uint32_t bid = NO_HIT_ID;
+  st.sp = 0;
+  uint32_t cur = 0;
+  while (true) {
+    const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+    float4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
+    if (COUNT) cnt.nodes++;
+    float tn0, tn1;
+    uint32_t c0 = __float_as_uint(qc.x), c1 = __float_as_uint(qc.y);
+    bool h0 = c0 != BVH_EMPTY && slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), o, inv, tmin, best, tn0);
+    bool h1 = c1 != BVH_EMPTY && slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), o, inv, tmin, best, tn1);
+    if (h0 && (c0 & BVH_LEAF_BIT)) {
+      if (MODE == 0) leaf_closest<COUNT>(tris, c0, rp, tmin, best, bid, b1, b2, cnt);
+      else if (leaf_occluded<COUNT>(tris, c0, rp, tmin, tmax, target, cnt)) return true;
+      h0 = false;
+    }
+    if (h1 && (c1 & BVH_LEAF_BIT)) {
+      if (MODE == 0) leaf_closest<COUNT>(tris, c1, rp, tmin, best, bid, b1, b2, cnt);
+      else if (leaf_occluded<COUNT>(tris, c1, rp, tmin, tmax, target, cnt)) return true;
+      h1 = false;
+    }
+    if (h0 && h1) {
+      if (tn1 < tn0) { uint32_t t = c0; c0 = c1; c1 = t; }
+      st.push(c1);
+      cur = c0;
+    } else if (h0) {
+      cur = c0;
+    } else if (h1) {
+      cur = c1;
+    } else {
+      if (st.sp == 0) break;
+      cur = st.pop();
+    }
+  }
+  bestT = best;
+  bestId = bid;
+  return bid != NO_HIT_ID;
+}
+
+// BVH8: test all eight boxes of a node, intersect hit leaves, push hit inner children far to near.
+template <int MODE, bool COUNT>
+__device__ bool traverse_bvh8(const Bvh8Node* nodes, const TriRecord* tris, vec3 o, vec3 d, float tmin, float tmax,
+                              TravStack& st, float& bestT, uint32_t& bestId, float& b1, float& b2, uint32_t target,
+                              TraversalCounters& cnt) {
+  RayPre rp = ray_precompute(o, d);
+  vec3 inv(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  float best = tmax;
+  uint32_t bid = NO_HIT_ID;
+  st.sp = 0;
+  uint32_t cur = 0;
+  while (true) {
+    const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+    if (COUNT) cnt.nodes++;
+    float4 lx0 = np[0], lx1 = np[1], hx0 = np[2], hx1 = np[3];
+    float4 ly0 = np[4], ly1 = np[5], hy0 = np[6], hy1 = np[7];
+    float4 lz0 = np[8], lz1 = np[9], hz0 = np[10], hz1 = np[11];
+    uint4 ch0 = reinterpret_cast<const uint4*>(np)[12], ch1 = reinterpret_cast<const uint4*>(np)[13];
+    float lx[8] = {lx0.x, lx0.y, lx0.z, lx0.w, lx1.x, lx1.y, lx1.z, lx1.w};
+    float hx[8] = {hx0.x, hx0.y, hx0.z, hx0.w, hx1.x, hx1.y, hx1.z, hx1.w};
+    float ly[8] = {ly0.x, ly0.y, ly0.z, ly0.w, ly1.x, ly1.y, ly1.z, ly1.w};
+    float hy[8] = {hy0.x, hy0.y, hy0.z, hy0.w, hy1.x, hy1.y, hy1.z, hy1.w};
+    float lz[8] = {lz0.x, lz0.y, lz0.z, lz0.w, lz1.x, lz1.y, lz1.z, lz1.w};
+    float hz[8] = {hz0.x, hz0.y, hz0.z, hz0.w, hz1.x, hz1.y, hz1.z, hz1.w};
+    uint32_t ch[8] = {ch0.x, ch0.y, ch0.z, ch0.w, ch1.x, ch1.y, ch1.z, ch1.w};
+    // hit inner children, sorted near to far by insertion (at most 8)
+    uint32_t hc[8];
+    float ht[8];
+    int nh = 0;
+    for (int s = 0; s < 8; ++s) {
+      if (ch[s] == BVH_EMPTY) continue;
+      float tn;
+      if (!slab_test(vec3(lx[s], ly[s], lz[s]), vec3(hx[s], hy[s], hz[s]), o, inv, tmin, best, tn)) continue;
+      if (ch[s] & BVH_LEAF_BIT) {
+        if (MODE == 0) leaf_closest<COUNT>(tris, ch[s], rp, tmin, best, bid, b1, b2, cnt);
+        else if (leaf_occluded<COUNT>(tris, ch[s], rp, tmin, tmax, target, cnt)) return true;
+        continue;
+      }
+      int p = nh++;
+      while (p > 0 && ht[p - 1] > tn) { ht[p] = ht[p - 1]; hc[p] = hc[p - 1]; --p; }
+      ht[p] = tn; hc[p] = ch[s];
+    }
+    if (nh > 0) {
+      for (int k = nh - 1; k >= 1; --k) st.push(hc[k]);
+      cur = hc[0];
+    } else {
+      if (st.sp == 0) break;
+      cur = st.pop();
+    }
+  }
+  bestT = best;
+  bestId = bid;
+  return bid != NO_HIT_ID;
+}
+
+}  // namespace ypt

@@ -1,0 +1,360 @@
+// Wavefront kernels of the unidirectional integrators (BPT, BPT_WITH_NEE) for gfx950.
+//
+// The reference runs each path start-to-finish in one DXR ray-generation thread
+// (rayGenPrimaryRays.hlsl:10-145). Here a pass over one Sobol sample index is split into kernels
+// that each do one kind of work for all live paths, with path state in HBM as structure-of-arrays:
+//   start   - camera rays for every pixel, ray queue = all pixels
+//   extend  - closest-hit traversal of the ray queue (hit records per path slot)
+//   shade   - surface/miss evaluation, light sample, next direction, Russian roulette; appends
+//             shadow rays and surviving paths to two compacted queues (wave ballot + one atomic
+//             per wave)
+//   shadow  - occlusion queries of the shadow queue, adds visible light contributions
+//   merge   - XYZ of each path, film running mean with the firefly clamp (mergeNewSamples.comp)
+// Queue order is whatever the atomics produce; no result depends on it because every path only
+// touches its own slot.
+#pragma once
+#include "../core/shade.h"
+#include "traverse.h"
+
+namespace ypt {
+
+struct PathSoA {
+  float4* orgLambda;   // origin xyz, hero wavelength
+  float4* dirPdf;      // direction xyz, pdfThisRay
+  float4* thr;         // throughput
+  float4* light;       // totalLight
+  uint4* st0;          // dim, seed, flags, pathLength
+  uint4* st1;          // rayState, nvol, furnaceInstance, -
+  float4* volIor;      // 4 iors
+  uint4* volMat;       // 4 material ids
+  float4* volAbs;      // 4 spectra per slot: [slot * 4 + k]
+};
+
+struct Queues {
+  uint32_t* ray[2];
+  uint32_t* counters;  // [0], [1]: ray queue lengths, [2]: shadow queue length, [3..]: stats
+  float4* hit;         // per slot: t, b1, b2, tri (bits)
+  float4* sOrgTmax;    // shadow queue
+  float4* sDirTri;
+  float4* sContrib;
+  uint32_t* sSlot;
+  uint32_t* ovf;       // traversal stack overflow
+  uint32_t ovfStride;
+  unsigned long long* workCounters;  // [0] nodes, [1] tris (extension), [2] nodes, [3] tris (shadow)
+};
+
+struct FilmBuffers {
+  float4* accum;       // [2][npix]
+  float4* merged[2];   // double buffer: read previous, write current
+  uint32_t* counts[2];
+  float* variance;
+  float4* lastSample;  // X, Y, Z, hero wavelength
+  float* features;     // 12 floats per pixel (first-hit features), optional
+};
+
+struct DevScene {
+  SceneView sv;
+  const Bvh2Node* nodes2;
+  const Bvh8Node* nodes8;
+  const TriRecord* tris;
+};
+
+__device__ __forceinline__ void load_path(const PathSoA& P, uint32_t s, PathState& ps) {
+  float4 a = P.orgLambda[s], b = P.dirPdf[s], t = P.thr[s], l = P.light[s];
+  uint4 u0 = P.st0[s], u1 = P.st1[s];
+  ps.origin = vec3(a.x, a.y, a.z);
+  ps.lambda = a.w;
+  ps.dir = vec3(b.x, b.y, b.z);
+  ps.pdfThisRay = b.w;
+  ps.throughput = Spectrum(t.x, t.y, t.z, t.w);
+  ps.totalLight = Spectrum(l.x, l.y, l.z, l.w);
+  ps.dim = u0.x; ps.seed = u0.y; ps.flags = u0.z; ps.pathLength = u0.w;
+  ps.rayState = u1.x; ps.nvol = u1.y; ps.furnaceInstance = u1.z;
+  if (ps.nvol > 0) {
+    float4 io = P.volIor[s];
+    uint4 vm = P.volMat[s];
+    ps.volIor[0] = io.x; ps.volIor[1] = io.y; ps.volIor[2] = io.z; ps.volIor[3] = io.w;
+    ps.volMat[0] = vm.x; ps.volMat[1] = vm.y; ps.volMat[2] = vm.z; ps.volMat[3] = vm.w;
+    for (int k = 0; k < 4; ++k) {
+      float4 ab = P.volAbs[(size_t)s * 4 + k];
+      ps.volAbs[k] = Spectrum(ab.x, ab.y, ab.z, ab.w);
+    }
+  } else {
+    for (int k = 0; k < 4; ++k) { ps.volIor[k] = 1.f; ps.volMat[k] = VOLUME_NONE; ps.volAbs[k].set(0.f); }
+  }
+}
+
+__device__ __forceinline__ void store_path(const PathSoA& P, uint32_t s, const PathState& ps, bool volumes) {
+  P.orgLambda[s] = make_float4(ps.origin.x, ps.origin.y, ps.origin.z, ps.lambda);
+  P.dirPdf[s] = make_float4(ps.dir.x, ps.dir.y, ps.dir.z, ps.pdfThisRay);
+  P.thr[s] = make_float4(ps.throughput.s[0], ps.throughput.s[1], ps.throughput.s[2], ps.throughput.s[3]);
+  P.light[s] = make_float4(ps.totalLight.s[0], ps.totalLight.s[1], ps.totalLight.s[2], ps.totalLight.s[3]);
+  P.st0[s] = make_uint4(ps.dim, ps.seed, ps.flags, ps.pathLength);
+  P.st1[s] = make_uint4(ps.rayState, ps.nvol, ps.furnaceInstance, 0u);
+  if (volumes) {
+    P.volIor[s] = make_float4(ps.volIor[0], ps.volIor[1], ps.volIor[2], ps.volIor[3]);
+    P.volMat[s] = make_uint4(ps.volMat[0], ps.volMat[1], ps.volMat[2], ps.volMat[3]);
+    for (int k = 0; k < 4; ++k)
+      P.volAbs[(size_t)s * 4 + k] = make_float4(ps.volAbs[k].s[0], ps.volAbs[k].s[1], ps.volAbs[k].s[2], ps.volAbs[k].s[3]);
+  }
+}
+
+// Reserve one queue entry per lane with pred set: one atomic per wave, lane offsets from the ballot.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+  unsigned long long mask = __ballot(pred ? 1 : 0);
+  if (mask == 0ull) return 0u;
+  uint32_t lane = __lane_id();
+  uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  int leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = (uint32_t)__shfl((int)base, leader);
+  return base + prefix;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__global__ void k_start(FrameConstants fc, const uint32_t* point, PathSoA P, Queues Q, FilmBuffers F, uint32_t npix,
+                        int writeFeatures) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) { Q.counters[0] = npix; Q.counters[1] = 0; Q.counters[2] = 0; }
+  if (i >= npix) return;
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0];
+  PathState ps;
+  start_path(fc, point, i % W, i / W, ps);
+  store_path(P, i, ps, false);
+  Q.ray[0][i] = i;
+  if (writeFeatures)
+    for (int k = 0; k < 12; ++k) F.features[(size_t)i * 12 + k] = 0.f;
+}
+
+template <int WIDTH, bool COUNT>
+__global__ void k_extend(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t ldsDepth) {
+  extern __shared__ uint32_t lds[];
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) { Q.counters[qsel ^ 1] = 0; Q.counters[2] = 0; }
+  uint32_t n = Q.counters[qsel];
+  TraversalCounters cnt = {0u, 0u};
+  if (i < n) {
+    uint32_t slot = Q.ray[qsel][i];
+    float4 a = P.orgLambda[slot], b = P.dirPdf[slot];
+    TravStack st;
+    st.lds = lds; st.ovf = Q.ovf; st.block = blockDim.x; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
+    st.ovfStride = Q.ovfStride; st.gid = i; st.sp = 0;
+    float t = 0.f, b1 = 0.f, b2 = 0.f;
+    uint32_t tri = NO_HIT_ID;
+    vec3 o(a.x, a.y, a.z), d(b.x, b.y, b.z);
+    if (WIDTH == 2) traverse_bvh2<0, COUNT>(S.nodes2, S.tris, o, d, RAY_MIN_T, __builtin_inff(), st, t, tri, b1, b2, 0u, cnt);
+    else traverse_bvh8<0, COUNT>(S.nodes8, S.tris, o, d, RAY_MIN_T, __builtin_inff(), st, t, tri, b1, b2, 0u, cnt);
+    Q.hit[slot] = make_float4(t, b1, b2, __uint_as_float(tri));
+  }
+  if (COUNT) {
+    unsigned long long nn = wave_sum(cnt.nodes), nt = wave_sum(cnt.tris), nr = wave_sum(i < n ? 1ull : 0ull);
+    if (__lane_id() == 0 && (nn | nt)) {
+      atomicAdd(&Q.workCounters[0], nn); atomicAdd(&Q.workCounters[1], nt); atomicAdd(&Q.workCounters[4], nr);
+    }
+  }
+}
+
+template <bool NEE>
+__global__ void k_shade(FrameConstants fc, DevScene S, const uint32_t* point, PathSoA P, Queues Q, FilmBuffers F, int qsel,
+                        int writeFeatures) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = Q.counters[qsel];
+  if (i >= n) return;  // whole waves past the end leave together; the last wave keeps its live lanes
+  uint32_t slot = Q.ray[qsel][i];
+  PathState ps;
+  load_path(P, slot, ps);
+  uint32_t nvolIn = ps.nvol;
+  float4 hv = Q.hit[slot];
+  Hit h;
+  h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.tri = __float_as_uint(hv.w);
+  ShadowRay sr;
+  sr.valid = false;
+  FirstHitFeatures fh;
+  fh.hit = 0u;
+  bool first = ps.pathLength == 0;
+  if (h.tri == NO_HIT_ID) shade_miss(S.sv, fc, ps);
+  else shade_hit(S.sv, fc, point, NEE, ps, h, sr, (writeFeatures && first) ? &fh : nullptr);
+  after_segment(point, ps);
+  bool cont = ps.rayState == RAY_ALIVE && ps.pathLength < fc.maxRayDepth;
+  store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);
+  if (writeFeatures && first && fh.hit) {
+    float* f = F.features + (size_t)slot * 12;
+    f[0] = fh.normal.x; f[1] = fh.normal.y; f[2] = fh.normal.z;
+    f[3] = fh.albedo.x; f[4] = fh.albedo.y; f[5] = fh.albedo.z;
+    f[6] = fh.roughness; f[7] = fh.ior; f[8] = fh.metalness; f[9] = fh.transparency; f[10] = fh.depth; f[11] = 1.f;
+  }
+  uint32_t si = wave_append(&Q.counters[2], sr.valid);
+  if (sr.valid) {
+    Q.sOrgTmax[si] = make_float4(sr.origin.x, sr.origin.y, sr.origin.z, sr.tmax);
+    Q.sDirTri[si] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, __uint_as_float(sr.targetTri));
+    Q.sContrib[si] = make_float4(sr.contrib.s[0], sr.contrib.s[1], sr.contrib.s[2], sr.contrib.s[3]);
+    Q.sSlot[si] = slot;
+  }
+  uint32_t ri = wave_append(&Q.counters[qsel ^ 1], cont);
+  if (cont) Q.ray[qsel ^ 1][ri] = slot;
+}
+
+template <int WIDTH, bool COUNT>
+__global__ void k_shadow(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth) {
+  extern __shared__ uint32_t lds[];
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = Q.counters[2];
+  TraversalCounters cnt = {0u, 0u};
+  if (i < n) {
+    float4 a = Q.sOrgTmax[i], b = Q.sDirTri[i];
+    vec3 o(a.x, a.y, a.z), d(b.x, b.y, b.z);
+    uint32_t target = __float_as_uint(b.w);
+    // the light triangle itself first: the connection counts only if it is the closest hit
+    RayPre rp = ray_precompute(o, d);
+    vec3 p1, p2, p3;
+    tri_vertices(S.sv, target, p1, p2, p3);
+    float tt, b1, b2;
+    bool visible = false;
+    if (intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2)) {
+      TravStack st;
+      st.lds = lds; st.ovf = Q.ovf; st.block = blockDim.x; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
+      st.ovfStride = Q.ovfStride; st.gid = i; st.sp = 0;
+      float bt; uint32_t bid; float c1, c2;
+      bool occluded = WIDTH == 2 ? traverse_bvh2<1, COUNT>(S.nodes2, S.tris, o, d, RAY_MIN_T, tt, st, bt, bid, c1, c2, target, cnt)
+                                 : traverse_bvh8<1, COUNT>(S.nodes8, S.tris, o, d, RAY_MIN_T, tt, st, bt, bid, c1, c2, target, cnt);
+      visible = !occluded;
+    }
+    if (visible) {
+      uint32_t slot = Q.sSlot[i];
+      float4 c = Q.sContrib[i];
+      float4 l = P.light[slot];
+      P.light[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w + c.w);
+    }
+  }
+  if (COUNT) {
+    unsigned long long nn = wave_sum(cnt.nodes), nt = wave_sum(cnt.tris), nr = wave_sum(i < n ? 1ull : 0ull);
+    if (__lane_id() == 0 && (nr | nt)) {
+      atomicAdd(&Q.workCounters[2], nn); atomicAdd(&Q.workCounters[3], nt); atomicAdd(&Q.workCounters[5], nr);
+    }
+  }
+}
+
+struct MergeParams {
+  uint32_t width, height;
+  uint32_t sampleCount;       // passes merged so far (g_mergeParams.sampleCount)
+  uint32_t accumTargets;      // ACCUMULATION_TARGET_COUNT = 2
+  float fireflyClampScale;
+  float fireflyClampRelaxPower;
+  uint32_t killNaN;
+  uint32_t cur;               // which merged/count buffer is written this pass
+};
+
+// clampFireflySample (mergeNewSamples.comp:122-170), reading the previous pass's merged film
+__device__ vec3 clamp_firefly(const MergeParams& mp, const FilmBuffers& F, int x, int y, vec3 c, uint32_t accumulated) {
+  float scale = mp.fireflyClampScale;
+  if (scale <= 0.f || accumulated == 0) return c;
+  if (c.x <= 0.f && c.y <= 0.f && c.z <= 0.f) return c;
+  const float4* prev = F.merged[mp.cur ^ 1];
+  const uint32_t* pc = F.counts[mp.cur ^ 1];
+  vec3 ref(0.f);
+  for (int dy = -1; dy <= 1; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) {
+      int lx = x + dx, ly = y + dy;
+      if (lx < 0 || ly < 0 || lx >= (int)mp.width || ly >= (int)mp.height) continue;
+      size_t q = (size_t)ly * mp.width + lx;
+      if (pc[q] == 0) continue;
+      float4 m = prev[q];
+      ref = vmax(ref, vmax(vec3(m.x, m.y, m.z), vec3(0.f)));
+    }
+  float refLevel = fmax_(fmax_(ref.x, ref.y), ref.z);
+  if (refLevel <= 0.f) return c;
+  ref = vmax(ref, vec3(refLevel * 0.05f));
+  float relax = ypow(fmax_(1.f, (float)accumulated), mp.fireflyClampRelaxPower);
+  vec3 allowed = ref * scale * relax;
+  float s = 1.f;
+  if (c.x > allowed.x) s = fmin_(s, allowed.x / c.x);
+  if (c.y > allowed.y) s = fmin_(s, allowed.y / c.y);
+  if (c.z > allowed.z) s = fmin_(s, allowed.z / c.z);
+  return c * s;
+}
+
+// path XYZ + mergeNewSamples (mergeNewSamples.comp:173-265) with per-pixel sample counts
+__global__ void k_merge(DevScene S, PathSoA P, FilmBuffers F, MergeParams mp) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= mp.width * mp.height) return;
+  int x = (int)(p % mp.width), y = (int)(p / mp.width);
+  float4 l = P.light[p];
+  float lam = P.orgLambda[p].w;
+  PathState ps;
+  ps.totalLight = Spectrum(l.x, l.y, l.z, l.w);
+  ps.lambda = lam;
+  vec3 xyz = path_xyz(S.sv, ps);
+  F.lastSample[p] = make_float4(xyz.x, xyz.y, xyz.z, lam);
+  float alpha = 1.f;
+  if (mp.killNaN != 0 && (is_nan(xyz.x) || is_nan(xyz.y) || is_nan(xyz.z))) { xyz = vec3(0.f); alpha = 0.f; }
+  uint32_t newSamples = (uint32_t)alpha;
+  uint32_t oldSamples = F.counts[mp.cur ^ 1][p];
+  uint32_t countOut = oldSamples;
+  if (newSamples > 0) {
+    xyz = clamp_firefly(mp, F, x, y, xyz, oldSamples);
+    countOut = newSamples + oldSamples;
+  }
+  F.counts[mp.cur][p] = countOut;
+  size_t npix = (size_t)mp.width * mp.height;
+  uint32_t layer = mp.sampleCount % mp.accumTargets;
+  if (newSamples > 0) {
+    uint32_t old = oldSamples / mp.accumTargets;
+    float4 dst = F.accum[layer * npix + p];
+    double total = (double)(newSamples + old);
+    double destBlend = old / total, srcBlend = newSamples / total;
+    F.accum[layer * npix + p] = make_float4((float)(dst.x * destBlend + srcBlend * xyz.x), (float)(dst.y * destBlend + srcBlend * xyz.y),
+                                            (float)(dst.z * destBlend + srcBlend * xyz.z), 1.f);
+  }
+  float4 r0 = F.accum[p];
+  vec3 sum(r0.x, r0.y, r0.z);
+  float cmp = ylog(1.f + sum.y);
+  float variance = 0.f;
+  uint32_t targets = mp.sampleCount + 1 < mp.accumTargets ? mp.sampleCount + 1 : mp.accumTargets;
+  for (uint32_t k = 1; k < targets; ++k) {
+    float4 r = F.accum[k * npix + p];
+    sum = sum + vec3(r.x, r.y, r.z);
+    float lum = ylog(1.f + r.y);
+    float d = fabsf(cmp - lum);
+    variance += d * d;
+  }
+  sum = sum / (float)targets;
+  variance /= (float)targets;
+  F.merged[mp.cur][p] = make_float4(sum.x, sum.y, sum.z, 1.f);
+  F.variance[p] = mp.sampleCount == 0 ? 0.f : variance;
+}
+
+// ---------------------------------------------------------------- query kernels used by tests
+template <int WIDTH>
+__global__ void k_trace_closest(DevScene S, const float* rays, uint32_t n, float* outT, uint32_t* outTri, float* outB,
+                                uint32_t* ovf, uint32_t ldsDepth) {
+  extern __shared__ uint32_t lds[];
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* r = rays + (size_t)i * 8;
+  TravStack st;
+  st.lds = lds; st.ovf = ovf; st.block = blockDim.x; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
+  st.ovfStride = n; st.gid = i; st.sp = 0;
+  TraversalCounters cnt = {0u, 0u};
+  float t = 0.f, b1 = 0.f, b2 = 0.f;
+  uint32_t tri = NO_HIT_ID;
+  vec3 o(r[0], r[1], r[2]), d(r[4], r[5], r[6]);
+  if (WIDTH == 2) traverse_bvh2<0, false>(S.nodes2, S.tris, o, d, r[3], r[7], st, t, tri, b1, b2, 0u, cnt);
+  else traverse_bvh8<0, false>(S.nodes8, S.tris, o, d, r[3], r[7], st, t, tri, b1, b2, 0u, cnt);
+  outT[i] = tri == NO_HIT_ID ? 0.f : t;
+  outTri[i] = tri;
+  outB[2 * i] = tri == NO_HIT_ID ? 0.f : b1;
+  outB[2 * i + 1] = tri == NO_HIT_ID ? 0.f : b2;
+}
+
+__global__ void k_sample_dims(const uint32_t* point, const uint32_t* seeds, const uint32_t* dims, uint32_t n, float* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = sample_dim(point, dims[i], seeds[i]);
+}
+
+}  // namespace ypt

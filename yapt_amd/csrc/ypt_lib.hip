@@ -1,0 +1,631 @@
+// libyapt_pt: C ABI over the gfx950 wavefront path tracer (see include/yapt_pt.h).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/yapt_pt.h"
+#include "core/shade.h"
+#include "kernels/wavefront.h"
+#include "runtime/bvh.h"
+#include "runtime/luts.h"
+#include "runtime/prepare.h"
+#include "runtime/rgb2spec.h"
+
+#include "data/cie_tables.inc"
+#include "data/sobol_matrices.inc"
+
+using namespace ypt;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+struct HipError : std::runtime_error {
+  int code;
+  HipError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                          \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      throw HipError(e_ == hipErrorOutOfMemory ? YPT_E_MEMORY : YPT_E_DEVICE,                    \
+                     std::string(#expr) + ": " + hipGetErrorString(e_));                          \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const T* src, size_t count, hipStream_t s) {
+    if (count > n) alloc(count);
+    if (count) HIP_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void zero(hipStream_t s) {
+    if (n) HIP_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+// Process-wide host tables: directional-albedo LUTs and the RGB->spectrum fit (both deterministic).
+struct HostTables {
+  std::vector<float> msl;
+  Rgb2Spec r2s;
+  HostTables() {
+    msl.resize(MSL_TOTAL);
+    generate_ms_luts(msl.data());
+    r2s.init(YPT_CIE_X_D, YPT_CIE_Y_D, YPT_CIE_Z_D, YPT_CIE_D65_D, YPT_SRGB_XYZ_TO_RGB_D, YPT_SRGB_RGB_TO_XYZ_D);
+  }
+};
+HostTables& host_tables() {
+  static HostTables t;
+  return t;
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+struct ypt_integrator {
+  int type = YPT_INTEGRATOR_NEE;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool ownStream = false;
+
+  // scene
+  PreparedScene scene;
+  BvhResult bvh;
+  bool hasScene = false;
+  DevBuf<float> dPos, dNrm, dTan, dMsl, dCie, dD65, dRgb2spec;
+  DevBuf<uint32_t> dIdx, dTriInst;
+  DevBuf<InstanceGPU> dInst;
+  DevBuf<MaterialGPU> dMats;
+  DevBuf<LightGPU> dLights;
+  DevBuf<Bvh2Node> dNodes2;
+  DevBuf<Bvh8Node> dNodes8;
+  DevBuf<TriRecord> dTris;
+  uint32_t stackBound2 = BVH_MAX_DEPTH, stackBound8 = 8 * BVH_MAX_DEPTH;
+
+  // frame
+  ypt_frame_desc frameDesc;
+  FrameConstants fc;
+  bool hasFrame = false;
+  uint32_t width = 0, height = 0;
+
+  // wavefront state
+  DevBuf<float4> pOrg, pDir, pThr, pLight, pVolIor, pVolAbs, hit, sOrg, sDir, sContrib;
+  DevBuf<uint4> pSt0, pSt1, pVolMat;
+  DevBuf<uint32_t> rayQ0, rayQ1, counters, sSlot, ovf, sobolPoints;
+  DevBuf<unsigned long long> work;
+  uint32_t ovfStride = 0;
+
+  // film
+  DevBuf<float4> accum, merged0, merged1, lastSample;
+  DevBuf<uint32_t> counts0, counts1;
+  DevBuf<float> variance, features;
+  uint32_t filmSamples = 0;  // passes merged since the last clear
+  uint32_t filmCur = 0;
+
+  // options
+  int bvhWidth = 2;
+  int blockSize = 128;
+  int ldsDepth = 24;
+  int countTraversal = 0;
+  int writeFeatures = 0;
+
+  // stats of the last render
+  double stats[16] = {0};
+  hipEvent_t ev[8] = {};
+  bool profile = false;
+  std::vector<hipEvent_t> evPool;  // profile mode: one event per kernel boundary, read after the render
+  size_t evUsed = 0;
+  hipEvent_t next_event() {
+    if (evUsed == evPool.size()) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreate(&e));
+      evPool.push_back(e);
+    }
+    return evPool[evUsed++];
+  }
+
+  DevScene dev_scene() const {
+    DevScene s;
+    s.sv.pos = dPos.p; s.sv.nrm = dNrm.p; s.sv.idx = dIdx.p; s.sv.triInst = dTriInst.p;
+    s.sv.tan = scene.tan.empty() ? nullptr : dTan.p;
+    s.sv.inst = dInst.p; s.sv.mats = dMats.p; s.sv.lights = dLights.p;
+    s.sv.rgb2spec = dRgb2spec.n ? dRgb2spec.p : nullptr;
+    s.sv.ntris = (uint32_t)scene.triInst.size();
+    s.sv.ninst = (uint32_t)scene.inst.size();
+    s.sv.spec.cie_xyz4 = dCie.p;
+    s.sv.spec.d65 = dD65.p;
+    s.sv.luts.base = dMsl.p;
+    s.nodes2 = dNodes2.p; s.nodes8 = dNodes8.p; s.tris = dTris.p;
+    return s;
+  }
+  PathSoA path_soa() const {
+    PathSoA P;
+    P.orgLambda = pOrg.p; P.dirPdf = pDir.p; P.thr = pThr.p; P.light = pLight.p;
+    P.st0 = pSt0.p; P.st1 = pSt1.p; P.volIor = pVolIor.p; P.volMat = pVolMat.p; P.volAbs = pVolAbs.p;
+    return P;
+  }
+  Queues queues() const {
+    Queues Q;
+    Q.ray[0] = rayQ0.p; Q.ray[1] = rayQ1.p; Q.counters = counters.p; Q.hit = hit.p;
+    Q.sOrgTmax = sOrg.p; Q.sDirTri = sDir.p; Q.sContrib = sContrib.p; Q.sSlot = sSlot.p;
+    Q.ovf = ovf.p; Q.ovfStride = ovfStride; Q.workCounters = work.p;
+    return Q;
+  }
+  FilmBuffers film() const {
+    FilmBuffers F;
+    F.accum = accum.p; F.merged[0] = merged0.p; F.merged[1] = merged1.p;
+    F.counts[0] = counts0.p; F.counts[1] = counts1.p; F.variance = variance.p; F.lastSample = lastSample.p;
+    F.features = features.p;
+    return F;
+  }
+};
+
+namespace {
+
+void upload_constant_tables(ypt_integrator* h) {
+  HostTables& T = host_tables();
+  h->dMsl.upload(T.msl.data(), T.msl.size(), h->stream);
+  h->dCie.upload(YPT_CIE_XYZ, 4 * YPT_CIE_RES, h->stream);
+  h->dD65.upload(YPT_CIE_D65, YPT_CIE_RES, h->stream);
+}
+
+void ensure_frame_buffers(ypt_integrator* h, uint32_t w, uint32_t hgt) {
+  size_t npix = (size_t)w * hgt;
+  if (h->width == w && h->height == hgt && h->pOrg.n == npix) return;
+  h->pOrg.alloc(npix); h->pDir.alloc(npix); h->pThr.alloc(npix); h->pLight.alloc(npix);
+  h->pSt0.alloc(npix); h->pSt1.alloc(npix); h->pVolIor.alloc(npix); h->pVolMat.alloc(npix); h->pVolAbs.alloc(npix * 4);
+  h->hit.alloc(npix);
+  h->rayQ0.alloc(npix); h->rayQ1.alloc(npix);
+  h->sOrg.alloc(npix); h->sDir.alloc(npix); h->sContrib.alloc(npix); h->sSlot.alloc(npix);
+  h->counters.alloc(16);
+  h->work.alloc(8);
+  h->accum.alloc(2 * npix); h->merged0.alloc(npix); h->merged1.alloc(npix); h->lastSample.alloc(npix);
+  h->counts0.alloc(npix); h->counts1.alloc(npix); h->variance.alloc(npix);
+  h->features.alloc(npix * 12);
+  h->width = w;
+  h->height = hgt;
+  h->ovf.release();
+  h->ovfStride = 0;
+}
+
+void ensure_overflow(ypt_integrator* h) {
+  size_t npix = (size_t)h->width * h->height;
+  uint32_t bound = h->bvhWidth == 8 ? h->stackBound8 : h->stackBound2;
+  size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
+  size_t need = depth * npix;
+  if (h->ovf.n >= need && h->ovfStride == npix) return;
+  h->ovf.alloc(need);
+  h->ovfStride = (uint32_t)npix;
+}
+
+void clear_film(ypt_integrator* h) {
+  h->accum.zero(h->stream); h->merged0.zero(h->stream); h->merged1.zero(h->stream);
+  h->counts0.zero(h->stream); h->counts1.zero(h->stream); h->variance.zero(h->stream);
+  h->features.zero(h->stream); h->lastSample.zero(h->stream);
+  h->filmSamples = 0;
+  h->filmCur = 0;
+}
+
+// longest root-to-leaf push count of the BVH8 (bounds the traversal stack)
+uint32_t bvh8_stack_bound(const std::vector<Bvh8Node>& nodes) {
+  if (nodes.empty()) return 1;
+  std::vector<uint32_t> depth(nodes.size(), 0);
+  uint32_t best = 0;
+  for (size_t i = 0; i < nodes.size(); ++i) {  // children have larger indices (breadth-first emission)
+    uint32_t inner = 0;
+    for (int s = 0; s < 8; ++s)
+      if (nodes[i].child[s] != BVH_EMPTY && !(nodes[i].child[s] & BVH_LEAF_BIT)) inner++;
+    uint32_t here = depth[i] + (inner > 0 ? inner - 1 : 0);
+    best = std::max(best, here);
+    for (int s = 0; s < 8; ++s) {
+      uint32_t c = nodes[i].child[s];
+      if (c != BVH_EMPTY && !(c & BVH_LEAF_BIT)) depth[c] = std::max(depth[c], here);
+    }
+  }
+  return best + 1;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const HipError& e) {
+    return fail(e.code, e.what());
+  } catch (const std::invalid_argument& e) {
+    return fail(YPT_E_INVALID, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(YPT_E_MEMORY, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(YPT_E_INVALID, e.what());
+  }
+}
+
+template <int WIDTH>
+void launch_extend(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, int q, uint32_t grid) {
+  size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
+  if (h->countTraversal)
+    hipLaunchKernelGGL((k_extend<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, (uint32_t)h->ldsDepth);
+  else
+    hipLaunchKernelGGL((k_extend<WIDTH, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, (uint32_t)h->ldsDepth);
+}
+template <int WIDTH>
+void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, uint32_t grid) {
+  size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
+  if (h->countTraversal)
+    hipLaunchKernelGGL((k_shadow<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, (uint32_t)h->ldsDepth);
+  else
+    hipLaunchKernelGGL((k_shadow<WIDTH, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, (uint32_t)h->ldsDepth);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ypt_last_error(void) { return g_error.c_str(); }
+const char* ypt_version(void) { return "yapt_pt 0.1 (gfx950 wavefront)"; }
+
+int ypt_sobol_point(uint32_t sample_index, uint32_t* out256) {
+  if (!out256) return fail(YPT_E_INVALID, "null output");
+  for (uint32_t d = 0; d < YPT_SOBOL_POINT_DIMS; ++d) out256[d] = sobol_uint(YPT_SOBOL_MATRICES, sample_index, d);
+  return YPT_OK;
+}
+
+int ypt_create(int integrator_type, int device, ypt_integrator** out) {
+  return guarded([&]() -> int {
+    if (!out) return fail(YPT_E_INVALID, "null handle pointer");
+    if (integrator_type < YPT_INTEGRATOR_BPT || integrator_type > YPT_INTEGRATOR_BDPT)
+      return fail(YPT_E_INVALID, "unknown integrator type " + std::to_string(integrator_type));
+    if (integrator_type == YPT_INTEGRATOR_BDPT) return fail(YPT_E_INVALID, "BDPT integrator is not available in this build");
+    int n = 0;
+    HIP_CHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(YPT_E_INVALID, "device index out of range");
+    HIP_CHECK(hipSetDevice(device));
+    std::unique_ptr<ypt_integrator> h(new ypt_integrator());
+    h->type = integrator_type;
+    h->device = device;
+    HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->ownStream = true;
+    for (auto& e : h->ev) HIP_CHECK(hipEventCreate(&e));
+    upload_constant_tables(h.get());
+    *out = h.release();
+    return YPT_OK;
+  });
+}
+
+int ypt_destroy(ypt_integrator* h) {
+  if (!h) return YPT_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto& e : h->ev) if (e) (void)hipEventDestroy(e);
+  for (auto& e : h->evPool) (void)hipEventDestroy(e);
+  if (h->ownStream && h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return YPT_OK;
+}
+
+int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
+  if (!h || !name) return fail(YPT_E_INVALID, "null argument");
+  std::string n(name);
+  if (n == "bvh_width") {
+    if (value != 2 && value != 8) return fail(YPT_E_INVALID, "bvh_width must be 2 or 8");
+    h->bvhWidth = (int)value;
+  } else if (n == "block_size") {
+    if (value < 64 || value > 1024 || (value % 64) != 0) return fail(YPT_E_INVALID, "block_size must be a multiple of 64 in [64, 1024]");
+    h->blockSize = (int)value;
+  } else if (n == "lds_stack_depth") {
+    if (value < 1 || value > 64) return fail(YPT_E_INVALID, "lds_stack_depth must be in [1, 64]");
+    h->ldsDepth = (int)value;
+    h->ovf.release();
+  } else if (n == "count_traversal") {
+    h->countTraversal = value != 0;
+  } else if (n == "first_hit_features") {
+    h->writeFeatures = value != 0;
+  } else if (n == "profile") {
+    h->profile = value != 0;
+  } else {
+    return fail(YPT_E_INVALID, "unknown option " + n);
+  }
+  return YPT_OK;
+}
+
+int ypt_set_scene(ypt_integrator* h, const ypt_scene_desc* d) {
+  if (!h || !d) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    HIP_CHECK(hipSetDevice(h->device));
+    PreparedScene ps;
+    prepare_scene(*d, host_tables().r2s, ps);
+    BvhResult bvh;
+    BvhBuilder().build(ps.pos.data(), ps.idx.data(), (uint32_t)ps.triInst.size(), bvh);
+    h->scene = std::move(ps);
+    h->bvh = std::move(bvh);
+    h->stackBound2 = h->bvh.stats.maxDepth + 1;
+    h->stackBound8 = bvh8_stack_bound(h->bvh.nodes8);
+    hipStream_t s = h->stream;
+    h->dPos.upload(h->scene.pos.data(), h->scene.pos.size(), s);
+    h->dNrm.upload(h->scene.nrm.data(), h->scene.nrm.size(), s);
+    if (!h->scene.tan.empty()) h->dTan.upload(h->scene.tan.data(), h->scene.tan.size(), s);
+    h->dIdx.upload(h->scene.idx.data(), h->scene.idx.size(), s);
+    h->dTriInst.upload(h->scene.triInst.data(), h->scene.triInst.size(), s);
+    h->dInst.upload(h->scene.inst.data(), h->scene.inst.size(), s);
+    h->dMats.upload(h->scene.mats.data(), h->scene.mats.size(), s);
+    if (!h->scene.lights.empty()) h->dLights.upload(h->scene.lights.data(), h->scene.lights.size(), s);
+    else { LightGPU dummy; std::memset(&dummy, 0, sizeof(dummy)); h->dLights.upload(&dummy, 1, s); }
+    h->dNodes2.upload(h->bvh.nodes2.data(), h->bvh.nodes2.size(), s);
+    h->dNodes8.upload(h->bvh.nodes8.data(), h->bvh.nodes8.size(), s);
+    h->dTris.upload(h->bvh.tris.data(), h->bvh.tris.size(), s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    h->hasScene = true;
+    h->ovf.release();
+    if (h->hasFrame) {
+      prepare_frame(h->frameDesc, h->scene, (uint32_t)h->type, h->fc);
+      clear_film(h);
+    }
+    return YPT_OK;
+  });
+}
+
+int ypt_set_frame(ypt_integrator* h, const ypt_frame_desc* f) {
+  if (!h || !f) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (!h->hasScene) return fail(YPT_E_NOSCENE, "set_scene must come before set_frame");
+    HIP_CHECK(hipSetDevice(h->device));
+    FrameConstants fc;
+    prepare_frame(*f, h->scene, (uint32_t)h->type, fc);
+    if (f->env_intensity_scale != 0.f && !(f->flags & YPT_FLAG_WHITE_FURNACE) && h->dRgb2spec.n == 0) {
+      std::vector<float> table;
+      host_tables().r2s.full_table(table, std::max(1u, std::thread::hardware_concurrency()));
+      h->dRgb2spec.upload(table.data(), table.size(), h->stream);
+    }
+    bool resized = f->width != h->width || f->height != h->height;
+    ensure_frame_buffers(h, f->width, f->height);
+    h->frameDesc = *f;
+    h->fc = fc;
+    h->hasFrame = true;
+    if (resized) clear_film(h);
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    return YPT_OK;
+  });
+}
+
+int ypt_clear_film(ypt_integrator* h) {
+  if (!h) return fail(YPT_E_INVALID, "null handle");
+  return guarded([&]() -> int {
+    if (!h->hasFrame) return fail(YPT_E_NOSCENE, "no frame set");
+    HIP_CHECK(hipSetDevice(h->device));
+    clear_film(h);
+    return YPT_OK;
+  });
+}
+
+int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, void* stream) {
+  if (!h) return fail(YPT_E_INVALID, "null handle");
+  return guarded([&]() -> int {
+    if (!h->hasScene || !h->hasFrame) return fail(YPT_E_NOSCENE, "render needs a scene and a frame");
+    HIP_CHECK(hipSetDevice(h->device));
+    if (stream) h->stream = (hipStream_t)stream;
+    ensure_overflow(h);
+    hipStream_t s = h->stream;
+    uint32_t npix = h->width * h->height;
+    // Sobol points of all passes, one upload
+    std::vector<uint32_t> pts((size_t)sample_count * YPT_SOBOL_POINT_DIMS);
+    for (uint32_t k = 0; k < sample_count; ++k)
+      for (uint32_t d = 0; d < YPT_SOBOL_POINT_DIMS; ++d)
+        pts[(size_t)k * YPT_SOBOL_POINT_DIMS + d] = sobol_uint(YPT_SOBOL_MATRICES, sample_begin + k, d);
+    if (h->sobolPoints.n < pts.size()) h->sobolPoints.alloc(pts.size());
+    h->sobolPoints.upload(pts.data(), pts.size(), s);
+    if (h->countTraversal) h->work.zero(s);
+    DevScene S = h->dev_scene();
+    PathSoA P = h->path_soa();
+    Queues Q = h->queues();
+    FilmBuffers F = h->film();
+    uint32_t B = (uint32_t)h->blockSize;
+    uint32_t grid = blocks_for(npix, B);
+    // profile mode: events on this stream around every kernel, read once the render has finished
+    struct Span { hipEvent_t a, b; int kind; };  // kind 0 extend, 1 shade, 2 shadow, 3 start, 4 merge
+    std::vector<Span> spans;
+    h->evUsed = 0;
+    auto mark = [&]() -> hipEvent_t {
+      hipEvent_t e = h->next_event();
+      HIP_CHECK(hipEventRecord(e, s));
+      return e;
+    };
+    hipEvent_t eAll0 = nullptr;
+    if (h->profile) eAll0 = mark();
+    for (uint32_t k = 0; k < sample_count; ++k) {
+      const uint32_t* point = h->sobolPoints.p + (size_t)k * YPT_SOBOL_POINT_DIMS;
+      hipEvent_t e0 = h->profile ? mark() : nullptr;
+      hipLaunchKernelGGL(k_start, dim3(blocks_for(npix, 256)), dim3(256), 0, s, h->fc, point, P, Q, F, npix, h->writeFeatures);
+      if (h->profile) spans.push_back({e0, mark(), 3});
+      int q = 0;
+      for (uint32_t b = 0; b < h->fc.maxRayDepth; ++b) {
+        hipEvent_t a0 = h->profile ? mark() : nullptr;
+        if (h->bvhWidth == 8) launch_extend<8>(h, S, P, Q, q, grid);
+        else launch_extend<2>(h, S, P, Q, q, grid);
+        hipEvent_t a1 = h->profile ? mark() : nullptr;
+        if (h->type == YPT_INTEGRATOR_NEE)
+          hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(B), 0, s, h->fc, S, point, P, Q, F, q, h->writeFeatures);
+        else
+          hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(B), 0, s, h->fc, S, point, P, Q, F, q, h->writeFeatures);
+        hipEvent_t a2 = h->profile ? mark() : nullptr;
+        if (h->profile) { spans.push_back({a0, a1, 0}); spans.push_back({a1, a2, 1}); }
+        if (h->type == YPT_INTEGRATOR_NEE) {
+          if (h->bvhWidth == 8) launch_shadow<8>(h, S, P, Q, grid);
+          else launch_shadow<2>(h, S, P, Q, grid);
+          if (h->profile) spans.push_back({a2, mark(), 2});
+        }
+        q ^= 1;
+      }
+      MergeParams mp;
+      mp.width = h->width; mp.height = h->height; mp.sampleCount = h->filmSamples; mp.accumTargets = 2;
+      mp.fireflyClampScale = h->frameDesc.firefly_clamp_scale;
+      mp.fireflyClampRelaxPower = h->frameDesc.firefly_clamp_relax_power;
+      mp.killNaN = h->frameDesc.kill_nan_samples;
+      mp.cur = h->filmCur;
+      hipEvent_t m0 = h->profile ? mark() : nullptr;
+      hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp);
+      if (h->profile) spans.push_back({m0, mark(), 4});
+      h->filmSamples++;
+      h->filmCur ^= 1;
+    }
+    HIP_CHECK(hipGetLastError());
+    std::memset(h->stats, 0, sizeof(h->stats));
+    if (h->profile) {
+      hipEvent_t eAll1 = mark();
+      HIP_CHECK(hipEventSynchronize(eAll1));
+      double t[5] = {0, 0, 0, 0, 0};
+      for (const Span& sp : spans) {
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, sp.a, sp.b));
+        t[sp.kind] += ms;
+      }
+      float all = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&all, eAll0, eAll1));
+      h->stats[2] = t[0]; h->stats[3] = t[2]; h->stats[4] = t[1]; h->stats[5] = all;
+      h->stats[11] = t[3]; h->stats[12] = t[4];
+    }
+    h->stats[10] = (double)h->fc.maxRayDepth * sample_count;  // extension launches
+    return YPT_OK;
+  });
+}
+
+int ypt_synchronize(ypt_integrator* h) {
+  if (!h) return fail(YPT_E_INVALID, "null handle");
+  return guarded([&]() -> int {
+    HIP_CHECK(hipSetDevice(h->device));
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    return YPT_OK;
+  });
+}
+
+int ypt_get_stats(ypt_integrator* h, double* out16) {
+  if (!h || !out16) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    HIP_CHECK(hipSetDevice(h->device));
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (h->countTraversal && h->work.n) {
+      unsigned long long w[8] = {0};
+      HIP_CHECK(hipMemcpy(w, h->work.p, sizeof(w), hipMemcpyDeviceToHost));
+      h->stats[6] = (double)w[0]; h->stats[7] = (double)w[1]; h->stats[8] = (double)w[2]; h->stats[9] = (double)w[3];
+      h->stats[0] = (double)w[4]; h->stats[1] = (double)w[5];
+    }
+    std::memcpy(out16, h->stats, sizeof(h->stats));
+    return YPT_OK;
+  });
+}
+
+int ypt_get_film(ypt_integrator* h, float* xyzw, uint32_t* counts) {
+  if (!h || !xyzw) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (!h->hasFrame) return fail(YPT_E_NOSCENE, "no frame");
+    HIP_CHECK(hipSetDevice(h->device));
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    size_t npix = (size_t)h->width * h->height;
+    const float4* m = (h->filmCur ^ 1) ? h->merged1.p : h->merged0.p;  // last written
+    const uint32_t* c = (h->filmCur ^ 1) ? h->counts1.p : h->counts0.p;
+    HIP_CHECK(hipMemcpy(xyzw, m, npix * sizeof(float4), hipMemcpyDeviceToHost));
+    if (counts) HIP_CHECK(hipMemcpy(counts, c, npix * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return YPT_OK;
+  });
+}
+
+int ypt_get_last_samples(ypt_integrator* h, float* xyzl) {
+  if (!h || !xyzl) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (!h->hasFrame) return fail(YPT_E_NOSCENE, "no frame");
+    HIP_CHECK(hipSetDevice(h->device));
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    HIP_CHECK(hipMemcpy(xyzl, h->lastSample.p, (size_t)h->width * h->height * sizeof(float4), hipMemcpyDeviceToHost));
+    return YPT_OK;
+  });
+}
+
+int ypt_get_first_hit_features(ypt_integrator* h, float* out) {
+  if (!h || !out) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (!h->hasFrame) return fail(YPT_E_NOSCENE, "no frame");
+    HIP_CHECK(hipSetDevice(h->device));
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    HIP_CHECK(hipMemcpy(out, h->features.p, (size_t)h->width * h->height * 12 * sizeof(float), hipMemcpyDeviceToHost));
+    return YPT_OK;
+  });
+}
+
+int ypt_trace_closest(ypt_integrator* h, const float* rays, uint32_t n, float* t, uint32_t* tri, float* b12) {
+  if (!h || !rays || !t || !tri) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (!h->hasScene) return fail(YPT_E_NOSCENE, "no scene");
+    if (n == 0) return YPT_OK;
+    HIP_CHECK(hipSetDevice(h->device));
+    DevBuf<float> dr, dt, db;
+    DevBuf<uint32_t> dtri, dovf;
+    hipStream_t s = h->stream;
+    dr.upload(rays, (size_t)n * 8, s);
+    dt.alloc(n); db.alloc(2 * (size_t)n); dtri.alloc(n);
+    uint32_t bound = h->bvhWidth == 8 ? h->stackBound8 : h->stackBound2;
+    size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
+    dovf.alloc(depth * n);
+    DevScene S = h->dev_scene();
+    size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
+    if (h->bvhWidth == 8)
+      hipLaunchKernelGGL(k_trace_closest<8>, dim3(blocks_for(n, h->blockSize)), dim3(h->blockSize), lds, s, S, dr.p, n, dt.p, dtri.p, db.p, dovf.p, (uint32_t)h->ldsDepth);
+    else
+      hipLaunchKernelGGL(k_trace_closest<2>, dim3(blocks_for(n, h->blockSize)), dim3(h->blockSize), lds, s, S, dr.p, n, dt.p, dtri.p, db.p, dovf.p, (uint32_t)h->ldsDepth);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipMemcpy(t, dt.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(tri, dtri.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (b12) HIP_CHECK(hipMemcpy(b12, db.p, 2 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    return YPT_OK;
+  });
+}
+
+int ypt_sample_dims(ypt_integrator* h, uint32_t sample_index, const uint32_t* seeds, const uint32_t* dims, uint32_t n, float* out) {
+  if (!h || !seeds || !dims || !out) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (n == 0) return YPT_OK;
+    HIP_CHECK(hipSetDevice(h->device));
+    uint32_t pt[YPT_SOBOL_POINT_DIMS];
+    ypt_sobol_point(sample_index, pt);
+    DevBuf<uint32_t> dp, ds, dd;
+    DevBuf<float> dout;
+    hipStream_t s = h->stream;
+    dp.upload(pt, YPT_SOBOL_POINT_DIMS, s);
+    ds.upload(seeds, n, s);
+    dd.upload(dims, n, s);
+    dout.alloc(n);
+    hipLaunchKernelGGL(k_sample_dims, dim3(blocks_for(n, 256)), dim3(256), 0, s, dp.p, ds.p, dd.p, n, dout.p);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipMemcpy(out, dout.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return YPT_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,332 @@
+"""Synthetic scenes of the benchmark configurations, as world-space triangle soups.
+
+* ``cornell_box``: 5 wall quads + 2 axis-aligned boxes + 1 quad light (36 triangles), diffuse
+  principled materials (PLASTIC preset, specularAmount 0, roughness 1), RGB-white emitter whose
+  spectrum is D65-shaped (spectralDistribution.hlsl:140-180).
+* ``random_shapes``: seeded spheres / boxes / tori (1:1:1) tessellated to ~1M triangles inside a
+  20-unit cube, roughness ~ U(0.05, 1), metalness ~ Bernoulli(0.3), 10% glass (GLASS preset with
+  the BK7 Cauchy coefficients and dispersion), lit by quad lights above the cube. The reference's own
+  scene family (pythonScripts/yapt/random_shapes.py) needs glTF assets that are not available; this
+  generator follows its material families and presets instead.
+Material presets follow MaterialProxy.cpp:15-52 and the defaults MaterialInternal.h:41-75.
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+from .camera import Camera
+
+
+# ------------------------------------------------------------------ materials
+def blank_material():
+    """MaterialParameters() defaults (MaterialInternal.h:41-75)."""
+    m = _lib.Material()
+    m.albedo[:] = (1.0, 1.0, 1.0)
+    m.transparency = 0.0
+    m.specular[:] = (1.0, 1.0, 1.0)
+    m.metalness = 1.0
+    m.absorption[:] = (0.0, 0.0, 0.0)
+    m.dielectric_ior = 1.5
+    m.emissive[:] = (0.0, 0.0, 0.0)
+    m.roughness = 0.1
+    m.anisotropy = 0.0
+    m.anisotropy_rotation = 0.0
+    m.material_mask = 0
+    m.specular_amount = 1.0
+    m.clear_coat_amount = 0.0
+    m.clear_coat_ior = 1.5
+    m.clear_coat_roughness = 0.1
+    m.sheen_tint[:] = (1.0, 1.0, 1.0)
+    m.sheen_roughness = 0.5
+    m.sheen_amount = 0.0
+    m.thin_film_thickness_nm = 0.0
+    m.emission_focus = 0.0
+    m.alpha_cutoff = 0.0
+    m.cauchy_coeffs[:] = (1.5046, 0.00420)
+    m.subsurface_radius[:] = (1.0, 0.2, 0.1)
+    m.subsurface_amount = 0.0
+    m.subsurface_radius_scale = 1.0
+    return m
+
+
+def _pow22(c):
+    return tuple(float(v) for v in np.power(np.asarray(c, np.float32), np.float32(2.2)))
+
+
+PRESETS = {
+    # albedo, transparency, specular, metalness, ior, roughness  (MaterialProxy.cpp:22-48)
+    "METAL_GOLD": (_pow22((0.9451, 0.7294, 0.37255)), 0.0, _pow22((1.0, 0.97255, 0.73333)), 1.0, 1.5, 0.0),
+    "METAL_SILVER": (_pow22((0.9607, 0.9490, 0.9176)), 0.0, _pow22((1.0, 1.0, 1.0)), 1.0, 1.5, 0.0),
+    "METAL_ALUMINIUM": (_pow22((0.9137, 0.9137, 0.9137)), 0.0, _pow22((0.9686, 0.98039, 0.9882)), 1.0, 1.5, 0.0),
+    "METAL_BRASS": (_pow22((0.903, 0.744, 0.473)), 0.0, _pow22((0.975, 0.958, 0.918)), 1.0, 1.5, 0.0),
+    "METAL_COPPER": (_pow22((0.92549, 0.68627, 0.50196)), 0.0, _pow22((0.9960, 0.945098, 0.8196)), 1.0, 1.5, 0.0),
+    "GLASS": ((1.0, 1.0, 1.0), 1.0, (1.0, 1.0, 1.0), 0.0, 1.5, 0.0),
+    "PLASTIC": ((1.0, 1.0, 1.0), 0.0, (1.0, 1.0, 1.0), 0.0, 1.5, 0.0),
+}
+
+
+def preset_material(name):
+    m = blank_material()
+    if name == "BLANK":
+        return m
+    albedo, transparency, specular, metalness, ior, roughness = PRESETS[name]
+    m.albedo[:] = albedo
+    m.transparency = transparency
+    m.specular[:] = specular
+    m.metalness = metalness
+    m.dielectric_ior = ior
+    m.roughness = roughness
+    return m
+
+
+# ------------------------------------------------------------------ geometry helpers
+def arbitrary_tangents(normals):
+    """Unit tangents orthogonal to each normal, handedness 1 (the role of
+    MeshUtility.calculate_arbitrary_tangents, pythonScripts/yapt/mesh_utils.py:69-85, which returns a
+    zero vector for normals such as (-1, 0, 0); here the axis least aligned with n is crossed instead)."""
+    n = np.asarray(normals, np.float64).reshape(-1, 3)
+    a = np.abs(n)
+    axis = np.zeros_like(n)
+    k = np.argmin(a, axis=1)
+    axis[np.arange(len(n)), k] = 1.0
+    t = np.cross(axis, n)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    return np.concatenate([t, np.ones((len(n), 1))], axis=1).astype(np.float32)
+
+
+class MeshBuilder:
+    def __init__(self):
+        self.pos, self.nrm, self.idx = [], [], []
+        self.inst_first, self.inst_count, self.inst_mat = [], [], []
+        self.materials = []
+        self.nverts = 0
+        self.ntris = 0
+
+    def material(self, m):
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def add(self, positions, normals, indices, material):
+        positions = np.asarray(positions, np.float32).reshape(-1, 3)
+        normals = np.asarray(normals, np.float32).reshape(-1, 3)
+        indices = np.asarray(indices, np.uint32).reshape(-1, 3)
+        self.pos.append(positions)
+        self.nrm.append(normals)
+        self.idx.append(indices + self.nverts)
+        self.inst_first.append(self.ntris)
+        self.inst_count.append(len(indices))
+        self.inst_mat.append(material)
+        self.nverts += len(positions)
+        self.ntris += len(indices)
+
+    def add_many(self, positions, normals, indices, materials):
+        """positions/normals: (n_inst, V, 3) sharing one index template (T, 3)."""
+        n, V = positions.shape[0], positions.shape[1]
+        T = len(indices)
+        off = (self.nverts + np.arange(n, dtype=np.uint64) * V)[:, None, None]
+        self.pos.append(positions.reshape(-1, 3).astype(np.float32))
+        self.nrm.append(normals.reshape(-1, 3).astype(np.float32))
+        self.idx.append((indices[None, :, :].astype(np.uint64) + off).reshape(-1, 3).astype(np.uint32))
+        self.inst_first.extend((self.ntris + np.arange(n) * T).tolist())
+        self.inst_count.extend([T] * n)
+        self.inst_mat.extend([int(m) for m in materials])
+        self.nverts += n * V
+        self.ntris += n * T
+
+    def quad(self, corner, u, v, material):
+        """Quad corner, corner+u, corner+u+v, corner+v; front side along cross(u, v)."""
+        c = np.asarray(corner, np.float32)
+        u = np.asarray(u, np.float32)
+        v = np.asarray(v, np.float32)
+        n = np.cross(u, v)
+        n = n / np.linalg.norm(n)
+        p = np.stack([c, c + u, c + u + v, c + v])
+        self.add(p, np.tile(n, (4, 1)), [[0, 1, 2], [0, 2, 3]], material)
+
+    def arrays(self):
+        nrm = np.concatenate(self.nrm)
+        return _lib.SceneArrays(np.concatenate(self.pos), nrm, np.concatenate(self.idx),
+                                self.inst_first, self.inst_count, self.inst_mat, self.materials,
+                                tangents=arbitrary_tangents(nrm))
+
+
+def box_template():
+    """Unit cube [-1, 1]^3, 24 vertices (flat faces), outward winding."""
+    P, N, I = [], [], []
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            n = np.zeros(3, np.float32)
+            n[axis] = sgn
+            a, b = (axis + 1) % 3, (axis + 2) % 3
+            u = np.zeros(3, np.float32)
+            v = np.zeros(3, np.float32)
+            u[a], v[b] = 2.0, 2.0
+            if np.dot(np.cross(u, v), n) < 0:
+                u, v = v, u
+            c = n - u / 2 - v / 2
+            base = len(P)
+            P += [c, c + u, c + u + v, c + v]
+            N += [n] * 4
+            I += [[base, base + 1, base + 2], [base, base + 2, base + 3]]
+    return np.array(P, np.float32), np.array(N, np.float32), np.array(I, np.uint32)
+
+
+def sphere_template(stacks=20, slices=20):
+    P, N = [], []
+    for i in range(stacks + 1):
+        th = math.pi * i / stacks
+        for j in range(slices):
+            ph = 2 * math.pi * j / slices
+            n = (math.sin(th) * math.cos(ph), math.cos(th), math.sin(th) * math.sin(ph))
+            P.append(n)
+            N.append(n)
+    I = []
+    for i in range(stacks):
+        for j in range(slices):
+            a = i * slices + j
+            b = i * slices + (j + 1) % slices
+            c = (i + 1) * slices + j
+            d = (i + 1) * slices + (j + 1) % slices
+            if i != 0:
+                I.append([a, b, c])
+            if i != stacks - 1:
+                I.append([b, d, c])
+    P, N, I = np.array(P, np.float32), np.array(N, np.float32), np.array(I, np.uint32)
+    return _fix_winding(P, N, I)
+
+
+def torus_template(major=32, minor=12, R=1.0, r=0.35):
+    P, N = [], []
+    for i in range(major):
+        u = 2 * math.pi * i / major
+        for j in range(minor):
+            v = 2 * math.pi * j / minor
+            cx, cz = R * math.cos(u), R * math.sin(u)
+            n = (math.cos(v) * math.cos(u), math.sin(v), math.cos(v) * math.sin(u))
+            P.append((cx + r * n[0], r * n[1], cz + r * n[2]))
+            N.append(n)
+    I = []
+    for i in range(major):
+        for j in range(minor):
+            a = i * minor + j
+            b = ((i + 1) % major) * minor + j
+            c = i * minor + (j + 1) % minor
+            d = ((i + 1) % major) * minor + (j + 1) % minor
+            I.append([a, b, d])
+            I.append([a, d, c])
+    P, N, I = np.array(P, np.float32), np.array(N, np.float32), np.array(I, np.uint32)
+    return _fix_winding(P, N, I)
+
+
+def _fix_winding(P, N, I):
+    """Orient every triangle so that cross(p1 - p0, p2 - p0) agrees with the vertex normals."""
+    e = np.cross(P[I[:, 1]] - P[I[:, 0]], P[I[:, 2]] - P[I[:, 0]])
+    flip = np.einsum("ij,ij->i", e, N[I].sum(axis=1)) < 0
+    I = I.copy()
+    I[flip] = I[flip][:, [0, 2, 1]]
+    return P, N, I
+
+
+def random_rotations(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+    return R
+
+
+def hsv_to_rgb(h, s, v):
+    i = int(h * 6.0) % 6
+    f = h * 6.0 - int(h * 6.0)
+    p, q, t = v * (1 - s), v * (1 - s * f), v * (1 - s * (1 - f))
+    return [(v, t, p), (q, v, p), (p, v, t), (p, q, v), (t, p, v), (v, p, q)][i]
+
+
+# ------------------------------------------------------------------ scenes
+def cornell_box():
+    """36-triangle Cornell-style box (configuration: 256x256, 16 spp, depth 8)."""
+    b = MeshBuilder()
+
+    def diffuse(rgb):
+        m = preset_material("PLASTIC")
+        m.albedo[:] = rgb
+        m.specular_amount = 0.0
+        m.roughness = 1.0
+        return b.material(m)
+
+    white = diffuse((0.73, 0.73, 0.73))
+    red = diffuse((0.63, 0.065, 0.05))
+    green = diffuse((0.14, 0.45, 0.091))
+    lm = preset_material("PLASTIC")
+    lm.albedo[:] = (0.78, 0.78, 0.78)
+    lm.specular_amount = 0.0
+    lm.roughness = 1.0
+    lm.emissive[:] = (15.0, 15.0, 15.0)
+    light = b.material(lm)
+    # room x in [-1, 1], y in [0, 2], z in [-1, 1]; fronts face the interior
+    b.quad((-1, 0, 1), (2, 0, 0), (0, 0, -2), white)     # floor, +y
+    b.quad((-1, 2, -1), (2, 0, 0), (0, 0, 2), white)     # ceiling, -y
+    b.quad((-1, 0, -1), (2, 0, 0), (0, 2, 0), white)     # back, +z
+    b.quad((-1, 0, 1), (0, 0, -2), (0, 2, 0), red)       # left, +x
+    b.quad((1, 0, -1), (0, 0, 2), (0, 2, 0), green)      # right, -x
+    P, N, I = box_template()
+    for centre, half in (((-0.35, 0.6, -0.3), (0.3, 0.6, 0.3)), ((0.4, 0.3, 0.35), (0.3, 0.3, 0.3))):
+        b.add(P * np.asarray(half, np.float32) + np.asarray(centre, np.float32), N, I, white)
+    b.quad((-0.25, 1.98, -0.25), (0.5, 0, 0), (0, 0, 0.5), light)  # light, -y
+    cam = Camera(eye=(0.0, 1.0, 3.9), target=(0.0, 1.0, 0.0), fovy=math.pi * 0.3)
+    return b.arrays(), cam
+
+
+def random_shapes(n_shapes=2000, seed=2, cube=20.0, n_lights=4, light_emission=80.0, sphere_res=(20, 20),
+                  torus_res=(32, 12)):
+    """~1M-triangle random-shapes scene (n_shapes=2000, seed 2)."""
+    rng = np.random.default_rng(seed)
+    b = MeshBuilder()
+    templates = [sphere_template(*sphere_res), box_template(), torus_template(*torus_res)]
+    kinds = np.arange(n_shapes) % 3
+    centres = rng.uniform(-cube / 2, cube / 2, size=(n_shapes, 3)).astype(np.float32)
+    scales = rng.uniform(0.4, 1.1, size=n_shapes).astype(np.float32)
+    rots = random_rotations(rng, n_shapes).astype(np.float32)
+    mats = []
+    for _ in range(n_shapes):
+        u = rng.random()
+        if u < 0.1:
+            m = preset_material("GLASS")
+            m.material_mask = _lib.YPT_MATERIAL_DISPERSION
+            m.roughness = float(rng.uniform(0.0, 0.3))
+        else:
+            metal = rng.random() < 0.3
+            m = preset_material("METAL_SILVER" if metal else "PLASTIC")
+            rgb = hsv_to_rgb(rng.random(), rng.uniform(0.3, 0.9), rng.uniform(0.4, 0.95))
+            m.albedo[:] = tuple(round(float(c) * 63.0) / 63.0 for c in rgb)  # on the colour-table grid
+            m.roughness = float(rng.uniform(0.05, 1.0))
+        mats.append(b.material(m))
+    mats = np.array(mats)
+    for k, (P, N, I) in enumerate(templates):
+        sel = np.nonzero(kinds == k)[0]
+        pos = np.einsum("nij,vj->nvi", rots[sel], P) * scales[sel, None, None] + centres[sel, None, :]
+        nrm = np.einsum("nij,vj->nvi", rots[sel], N)
+        b.add_many(pos, nrm, I, mats[sel])
+    fm = preset_material("PLASTIC")
+    fm.albedo[:] = (0.6, 0.6, 0.6)
+    fm.roughness = 0.8
+    floor = b.material(fm)
+    ext = cube * 3.0
+    b.quad((-ext / 2, -cube / 2 - 1.0, ext / 2), (ext, 0, 0), (0, 0, -ext), floor)  # floor, +y
+    lm = preset_material("PLASTIC")
+    lm.albedo[:] = (0.5, 0.5, 0.5)
+    lm.specular_amount = 0.0
+    lm.roughness = 1.0
+    lm.emissive[:] = (light_emission,) * 3
+    light = b.material(lm)
+    side = cube * 0.4
+    for i in range(n_lights):
+        cx = (-0.5 + (i % 2)) * cube * 0.5 - side / 2
+        cz = (-0.5 + (i // 2)) * cube * 0.5 - side / 2
+        b.quad((cx, cube * 0.6, cz), (side, 0, 0), (0, 0, side), light)  # facing -y
+    cam = Camera(eye=(0.0, cube * 0.35, cube * 1.35), target=(0.0, -cube * 0.1, 0.0), fovy=math.pi * 0.3)
+    return b.arrays(), cam
