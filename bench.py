@@ -26,7 +26,7 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=64)
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=16)
     p.add_argument("--width", type=int, default=1920)
     p.add_argument("--height", type=int, default=1080)
     p.add_argument("--depth", type=int, default=12)
@@ -35,6 +35,7 @@ def parse():
     p.add_argument("--bvh-width", type=int, default=0, help="2 or 8; 0 = autotuned choice")
     p.add_argument("--block-size", type=int, default=0)
     p.add_argument("--lds-depth", type=int, default=0)
+    p.add_argument("--passes-in-flight", type=int, default=0, help="sample passes per wavefront batch; 0 = tuned")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the checker sample")
     p.add_argument("--no-cpu", action="store_true")
     return p.parse_args()
@@ -78,9 +79,11 @@ def main():
     width = a.bvh_width or cfg.get("bvh_width", 2)
     block = a.block_size or cfg.get("block_size", 128)
     lds = a.lds_depth or cfg.get("lds_stack_depth", 24)
+    pif = a.passes_in_flight or cfg.get("passes_in_flight", 0)
     it.set_option("bvh_width", width)
     it.set_option("block_size", block)
     it.set_option("lds_stack_depth", lds)
+    it.set_option("passes_in_flight", pif)
     t1 = time.time()
     it.set_scene(scene)
     it.set_frame(frame)
@@ -150,7 +153,8 @@ def main():
         "data": "synthetic (seeded random-shapes scene, seed 2)",
         "config": {"workload": "%s random-shapes %d shapes / %d tris, %dx%d, depth %d, 1 spp per step" %
                                (a.integrator.upper(), a.shapes, scene.triangle_count, a.width, a.height, a.depth),
-                   "bvh_width": width, "block_size": block, "lds_stack_depth": lds},
+                   "bvh_width": width, "block_size": block, "lds_stack_depth": lds,
+                   "passes_in_flight": int(st_prof[13])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_extend<%d> (closest-hit traversal)" % width,

@@ -1,0 +1,35 @@
+"""Command-line autotuner for the benchmark workloads.
+
+    python -m yapt_amd.autotune [--out PATH] [--width 1920 --height 1080 --depth 12 --shapes 2000]
+
+Sweeps BVH width, threads per block, LDS traversal-stack depth and passes in flight for the
+random-shapes NEE workload and stores the fastest configuration (yapt_amd/tuned.json by default).
+"""
+import argparse
+
+from . import _lib, camera, scenes, tuning
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--out", default=None)
+    p.add_argument("--width", type=int, default=1920)
+    p.add_argument("--height", type=int, default=1080)
+    p.add_argument("--depth", type=int, default=12)
+    p.add_argument("--shapes", type=int, default=2000)
+    p.add_argument("--integrator", default="nee", choices=["nee", "bpt"])
+    p.add_argument("--passes", type=int, default=16)
+    a = p.parse_args()
+    scene, cam = scenes.random_shapes(n_shapes=a.shapes, seed=2)
+    frame = camera.make_frame(cam, a.width, a.height, max_depth=a.depth)
+    it = _lib.Integrator(_lib.YPT_INTEGRATOR_NEE if a.integrator == "nee" else _lib.YPT_INTEGRATOR_BPT, 0)
+    it.set_scene(scene)
+    it.set_frame(frame)
+    key = "%s-shapes%d-%dx%d-d%d" % (a.integrator, a.shapes, a.width, a.height, a.depth)
+    best = tuning.autotune(it, key, passes=a.passes, out=a.out)
+    print("best", key, best)
+    it.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -117,14 +117,17 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
-__global__ void k_start(FrameConstants fc, const uint32_t* point, PathSoA P, Queues Q, FilmBuffers F, uint32_t npix,
-                        int writeFeatures) {
+// Slots [p * npix, (p + 1) * npix) hold the paths of pass p of the batch (several sample indices are
+// in flight at once so that every bounce launch carries enough rays to hide traversal latency).
+__global__ void k_start(FrameConstants fc, const uint32_t* points, PathSoA P, Queues Q, FilmBuffers F, uint32_t npix,
+                        uint32_t nslots, int writeFeatures) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) { Q.counters[0] = npix; Q.counters[1] = 0; Q.counters[2] = 0; }
-  if (i >= npix) return;
+  if (i == 0) { Q.counters[0] = nslots; Q.counters[1] = 0; Q.counters[2] = 0; }
+  if (i >= nslots) return;
   uint32_t W = (uint32_t)fc.targetTexDimensions[0];
+  uint32_t pass = i / npix, pix = i - pass * npix;
   PathState ps;
-  start_path(fc, point, i % W, i / W, ps);
+  start_path(fc, points + (size_t)pass * YPT_SOBOL_POINT_DIMS, pix % W, pix / W, ps);
   store_path(P, i, ps, false);
   Q.ray[0][i] = i;
   if (writeFeatures)
@@ -160,12 +163,13 @@ __global__ void k_extend(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t lds
 }
 
 template <bool NEE>
-__global__ void k_shade(FrameConstants fc, DevScene S, const uint32_t* point, PathSoA P, Queues Q, FilmBuffers F, int qsel,
-                        int writeFeatures) {
+__global__ void k_shade(FrameConstants fc, DevScene S, const uint32_t* points, uint32_t npix, PathSoA P, Queues Q,
+                        FilmBuffers F, int qsel, int writeFeatures) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t n = Q.counters[qsel];
   if (i >= n) return;  // whole waves past the end leave together; the last wave keeps its live lanes
   uint32_t slot = Q.ray[qsel][i];
+  const uint32_t* point = points + (size_t)(slot / npix) * YPT_SOBOL_POINT_DIMS;
   PathState ps;
   load_path(P, slot, ps);
   uint32_t nvolIn = ps.nvol;
@@ -241,6 +245,7 @@ __global__ void k_shadow(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth) {
 
 struct MergeParams {
   uint32_t width, height;
+  uint32_t slotBase;          // first path slot of the pass being merged
   uint32_t sampleCount;       // passes merged so far (g_mergeParams.sampleCount)
   uint32_t accumTargets;      // ACCUMULATION_TARGET_COUNT = 2
   float fireflyClampScale;
@@ -283,8 +288,8 @@ __global__ void k_merge(DevScene S, PathSoA P, FilmBuffers F, MergeParams mp) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= mp.width * mp.height) return;
   int x = (int)(p % mp.width), y = (int)(p / mp.width);
-  float4 l = P.light[p];
-  float lam = P.orgLambda[p].w;
+  float4 l = P.light[mp.slotBase + p];
+  float lam = P.orgLambda[mp.slotBase + p].w;
   PathState ps;
   ps.totalLight = Spectrum(l.x, l.y, l.z, l.w);
   ps.lambda = lam;

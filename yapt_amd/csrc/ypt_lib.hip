@@ -128,6 +128,8 @@ struct ypt_integrator {
   DevBuf<float> variance, features;
   uint32_t filmSamples = 0;  // passes merged since the last clear
   uint32_t filmCur = 0;
+  size_t slots = 0;          // path slots allocated
+  uint32_t lastPassBase = 0; // slot offset of the last rendered pass (first-hit features)
 
   // options
   int bvhWidth = 2;
@@ -135,6 +137,7 @@ struct ypt_integrator {
   int ldsDepth = 24;
   int countTraversal = 0;
   int writeFeatures = 0;
+  int passesInFlight = 0;  // 0 = automatic
 
   // stats of the last render
   double stats[16] = {0};
@@ -198,31 +201,49 @@ void upload_constant_tables(ypt_integrator* h) {
 
 void ensure_frame_buffers(ypt_integrator* h, uint32_t w, uint32_t hgt) {
   size_t npix = (size_t)w * hgt;
-  if (h->width == w && h->height == hgt && h->pOrg.n == npix) return;
-  h->pOrg.alloc(npix); h->pDir.alloc(npix); h->pThr.alloc(npix); h->pLight.alloc(npix);
-  h->pSt0.alloc(npix); h->pSt1.alloc(npix); h->pVolIor.alloc(npix); h->pVolMat.alloc(npix); h->pVolAbs.alloc(npix * 4);
-  h->hit.alloc(npix);
-  h->rayQ0.alloc(npix); h->rayQ1.alloc(npix);
-  h->sOrg.alloc(npix); h->sDir.alloc(npix); h->sContrib.alloc(npix); h->sSlot.alloc(npix);
+  if (h->width == w && h->height == hgt && h->accum.n == 2 * npix) return;
   h->counters.alloc(16);
   h->work.alloc(8);
   h->accum.alloc(2 * npix); h->merged0.alloc(npix); h->merged1.alloc(npix); h->lastSample.alloc(npix);
   h->counts0.alloc(npix); h->counts1.alloc(npix); h->variance.alloc(npix);
-  h->features.alloc(npix * 12);
   h->width = w;
   h->height = hgt;
+  h->slots = 0;
+  h->ovf.release();
+  h->ovfStride = 0;
+}
+
+// path-state and queue buffers for `slots` paths in flight
+void ensure_slots(ypt_integrator* h, size_t slots) {
+  if (h->slots >= slots) return;
+  h->pOrg.alloc(slots); h->pDir.alloc(slots); h->pThr.alloc(slots); h->pLight.alloc(slots);
+  h->pSt0.alloc(slots); h->pSt1.alloc(slots); h->pVolIor.alloc(slots); h->pVolMat.alloc(slots); h->pVolAbs.alloc(slots * 4);
+  h->hit.alloc(slots);
+  h->rayQ0.alloc(slots); h->rayQ1.alloc(slots);
+  h->sOrg.alloc(slots); h->sDir.alloc(slots); h->sContrib.alloc(slots); h->sSlot.alloc(slots);
+  h->features.alloc(slots * 12);
+  h->slots = slots;
   h->ovf.release();
   h->ovfStride = 0;
 }
 
 void ensure_overflow(ypt_integrator* h) {
-  size_t npix = (size_t)h->width * h->height;
+  size_t slots = h->slots;
   uint32_t bound = h->bvhWidth == 8 ? h->stackBound8 : h->stackBound2;
   size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
-  size_t need = depth * npix;
-  if (h->ovf.n >= need && h->ovfStride == npix) return;
+  size_t need = depth * slots;
+  if (h->ovf.n >= need && h->ovfStride == slots) return;
   h->ovf.alloc(need);
-  h->ovfStride = (uint32_t)npix;
+  h->ovfStride = (uint32_t)slots;
+}
+
+// passes in flight: enough slots (~32M paths, ~10 GB of path state) that the deep, nearly empty
+// bounce launches of one batch are shared by many passes
+uint32_t passes_in_flight(const ypt_integrator* h, uint32_t count) {
+  size_t npix = (size_t)h->width * h->height;
+  uint32_t g = h->passesInFlight > 0 ? (uint32_t)h->passesInFlight : (uint32_t)std::max<size_t>(1, (32u << 20) / npix);
+  g = std::min<uint32_t>(g, 16u);
+  return std::max(1u, std::min(g, count));
 }
 
 void clear_film(ypt_integrator* h) {
@@ -347,6 +368,9 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
     h->countTraversal = value != 0;
   } else if (n == "first_hit_features") {
     h->writeFeatures = value != 0;
+  } else if (n == "passes_in_flight") {
+    if (value < 0 || value > 16) return fail(YPT_E_INVALID, "passes_in_flight must be in [0, 16] (0 = automatic)");
+    h->passesInFlight = (int)value;
   } else if (n == "profile") {
     h->profile = value != 0;
   } else {
@@ -430,9 +454,11 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
     if (!h->hasScene || !h->hasFrame) return fail(YPT_E_NOSCENE, "render needs a scene and a frame");
     HIP_CHECK(hipSetDevice(h->device));
     if (stream) h->stream = (hipStream_t)stream;
-    ensure_overflow(h);
     hipStream_t s = h->stream;
     uint32_t npix = h->width * h->height;
+    uint32_t G = passes_in_flight(h, sample_count);
+    ensure_slots(h, (size_t)npix * G);
+    ensure_overflow(h);
     // Sobol points of all passes, one upload
     std::vector<uint32_t> pts((size_t)sample_count * YPT_SOBOL_POINT_DIMS);
     for (uint32_t k = 0; k < sample_count; ++k)
@@ -446,7 +472,6 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
     Queues Q = h->queues();
     FilmBuffers F = h->film();
     uint32_t B = (uint32_t)h->blockSize;
-    uint32_t grid = blocks_for(npix, B);
     // profile mode: events on this stream around every kernel, read once the render has finished
     struct Span { hipEvent_t a, b; int kind; };  // kind 0 extend, 1 shade, 2 shadow, 3 start, 4 merge
     std::vector<Span> spans;
@@ -458,10 +483,14 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
     };
     hipEvent_t eAll0 = nullptr;
     if (h->profile) eAll0 = mark();
-    for (uint32_t k = 0; k < sample_count; ++k) {
+    for (uint32_t k = 0; k < sample_count; k += G) {
+      uint32_t g = std::min(G, sample_count - k);
+      uint32_t nslots = npix * g;
+      uint32_t grid = blocks_for(nslots, B);
       const uint32_t* point = h->sobolPoints.p + (size_t)k * YPT_SOBOL_POINT_DIMS;
       hipEvent_t e0 = h->profile ? mark() : nullptr;
-      hipLaunchKernelGGL(k_start, dim3(blocks_for(npix, 256)), dim3(256), 0, s, h->fc, point, P, Q, F, npix, h->writeFeatures);
+      hipLaunchKernelGGL(k_start, dim3(blocks_for(nslots, 256)), dim3(256), 0, s, h->fc, point, P, Q, F, npix, nslots,
+                         h->writeFeatures);
       if (h->profile) spans.push_back({e0, mark(), 3});
       int q = 0;
       for (uint32_t b = 0; b < h->fc.maxRayDepth; ++b) {
@@ -470,9 +499,9 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
         else launch_extend<2>(h, S, P, Q, q, grid);
         hipEvent_t a1 = h->profile ? mark() : nullptr;
         if (h->type == YPT_INTEGRATOR_NEE)
-          hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(B), 0, s, h->fc, S, point, P, Q, F, q, h->writeFeatures);
+          hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
         else
-          hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(B), 0, s, h->fc, S, point, P, Q, F, q, h->writeFeatures);
+          hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
         hipEvent_t a2 = h->profile ? mark() : nullptr;
         if (h->profile) { spans.push_back({a0, a1, 0}); spans.push_back({a1, a2, 1}); }
         if (h->type == YPT_INTEGRATOR_NEE) {
@@ -482,17 +511,21 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
         }
         q ^= 1;
       }
-      MergeParams mp;
-      mp.width = h->width; mp.height = h->height; mp.sampleCount = h->filmSamples; mp.accumTargets = 2;
-      mp.fireflyClampScale = h->frameDesc.firefly_clamp_scale;
-      mp.fireflyClampRelaxPower = h->frameDesc.firefly_clamp_relax_power;
-      mp.killNaN = h->frameDesc.kill_nan_samples;
-      mp.cur = h->filmCur;
-      hipEvent_t m0 = h->profile ? mark() : nullptr;
-      hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp);
-      if (h->profile) spans.push_back({m0, mark(), 4});
-      h->filmSamples++;
-      h->filmCur ^= 1;
+      for (uint32_t p = 0; p < g; ++p) {  // passes merge in sample order
+        MergeParams mp;
+        mp.width = h->width; mp.height = h->height; mp.slotBase = p * npix;
+        mp.sampleCount = h->filmSamples; mp.accumTargets = 2;
+        mp.fireflyClampScale = h->frameDesc.firefly_clamp_scale;
+        mp.fireflyClampRelaxPower = h->frameDesc.firefly_clamp_relax_power;
+        mp.killNaN = h->frameDesc.kill_nan_samples;
+        mp.cur = h->filmCur;
+        hipEvent_t m0 = h->profile ? mark() : nullptr;
+        hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp);
+        if (h->profile) spans.push_back({m0, mark(), 4});
+        h->filmSamples++;
+        h->filmCur ^= 1;
+      }
+      h->lastPassBase = (g - 1) * npix;
     }
     HIP_CHECK(hipGetLastError());
     std::memset(h->stats, 0, sizeof(h->stats));
@@ -510,7 +543,8 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
       h->stats[2] = t[0]; h->stats[3] = t[2]; h->stats[4] = t[1]; h->stats[5] = all;
       h->stats[11] = t[3]; h->stats[12] = t[4];
     }
-    h->stats[10] = (double)h->fc.maxRayDepth * sample_count;  // extension launches
+    h->stats[10] = (double)h->fc.maxRayDepth * ((sample_count + G - 1) / G);  // extension launches
+    h->stats[13] = G;
     return YPT_OK;
   });
 }
@@ -572,7 +606,8 @@ int ypt_get_first_hit_features(ypt_integrator* h, float* out) {
     if (!h->hasFrame) return fail(YPT_E_NOSCENE, "no frame");
     HIP_CHECK(hipSetDevice(h->device));
     HIP_CHECK(hipStreamSynchronize(h->stream));
-    HIP_CHECK(hipMemcpy(out, h->features.p, (size_t)h->width * h->height * 12 * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(out, h->features.p + (size_t)h->lastPassBase * 12, (size_t)h->width * h->height * 12 * sizeof(float),
+                        hipMemcpyDeviceToHost));
     return YPT_OK;
   });
 }

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(HERE, "tuned.json")
 ROOT = os.path.dirname(HERE)
 
-DEFAULT = {"bvh_width": 2, "block_size": 128, "lds_stack_depth": 24}
+DEFAULT = {"bvh_width": 2, "block_size": 64, "lds_stack_depth": 16, "passes_in_flight": 0}
 
 
 def _load():
@@ -42,29 +42,31 @@ def measured_traffic(kernel_key):
     return None if v is None else v.get("bytes_per_launch")
 
 
-def autotune(integrator, scene, frame, key, passes=4, widths=(2, 8), blocks=(64, 128, 256), lds_depths=(16, 24, 32),
-             verbose=True):
-    """Times `passes` full passes for every configuration and stores the fastest under `key`."""
+def autotune(integrator, key, passes=16, widths=(2, 8), blocks=(64, 128), lds_depths=(8, 16, 24), flights=(8, 16),
+             verbose=True, out=None):
+    """Times `passes` full passes (scene and frame already set) for every configuration of BVH width,
+    threads per block, LDS stack depth and passes in flight; stores the fastest under `key`."""
     results = []
-    for w in widths:
-        for b in blocks:
-            for d in lds_depths:
-                integrator.set_option("bvh_width", w)
-                integrator.set_option("block_size", b)
-                integrator.set_option("lds_stack_depth", d)
-                integrator.render(0, 1)
-                integrator.synchronize()
-                t = time.perf_counter()
-                integrator.render(1, passes)
-                integrator.synchronize()
-                dt = (time.perf_counter() - t) / passes
-                results.append({"config": {"bvh_width": w, "block_size": b, "lds_stack_depth": d}, "ms_per_pass": dt * 1e3})
-                if verbose:
-                    print("autotune %s w=%d block=%d lds=%d: %.2f ms/pass" % (key, w, b, d, dt * 1e3), flush=True)
+    for g in flights:
+        for w in widths:
+            for b in blocks:
+                for d in lds_depths:
+                    cfg = {"bvh_width": w, "block_size": b, "lds_stack_depth": d, "passes_in_flight": g}
+                    for k, v in cfg.items():
+                        integrator.set_option(k, v)
+                    integrator.render(0, 2)
+                    integrator.synchronize()
+                    t = time.perf_counter()
+                    integrator.render(2, passes)
+                    integrator.synchronize()
+                    dt = (time.perf_counter() - t) / passes
+                    results.append({"config": cfg, "ms_per_pass": dt * 1e3})
+                    if verbose:
+                        print("autotune %s %s: %.2f ms/pass" % (key, cfg, dt * 1e3), flush=True)
     best = min(results, key=lambda r: r["ms_per_pass"])
     store = _load()
     store[key] = {"config": best["config"], "ms_per_pass": best["ms_per_pass"], "all": results}
-    with open(TUNED, "w") as f:
+    with open(out or TUNED, "w") as f:
         json.dump(store, f, indent=1, sort_keys=True)
     for k, v in best["config"].items():
         integrator.set_option(k, v)
