@@ -459,3 +459,56 @@ int oracle_render(const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integr
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ single-function evaluation for tests
+// fn selects the function; in: n rows of 8 floats, out: n rows of 4 floats.
+//  0 d_ggx(wm.xyz, ax, ay)            1 pdf_wm_ggx(wo.xyz, wm.xyz, ax, ay)
+//  2 sample_wm_ggx(wo.xyz, ax, ay, u1, u2) -> wm   3 fresnel_dielectric(eta, cos)
+//  4 fresnel_conductor(n, k, cos)     5 ysin  6 ycos  7 yexp  8 ylog  9 ypow(x, y)
+// 10 sample_cosine_hemisphere(u1, u2) 11 sigmoid(x) 12 sigmoid_inv(x) 13 hash_pair(a, b) (uint bits)
+// 14 owen_scramble_fast(v, seed) (uint bits)  15 uint_to_float01(v) (uint bits in)
+// 16 ss_dir(cos, r)  17 ss_avg(r)  18 ssms_dir(eta, cos, r)  19 sheen_dir(cos, r)  20 tr_dir(eta, cos, r)
+// 21 emission_profile(cos, n)
+extern "C" int oracle_eval(int fn, const float* in, uint32_t n, float* out) {
+  MSLuts L;
+  L.base = tables().msl.data();
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* a = in + 8 * (size_t)i;
+    float* o = out + 4 * (size_t)i;
+    o[0] = o[1] = o[2] = o[3] = 0.f;
+    switch (fn) {
+      case 0: o[0] = d_ggx(vec3(a[0], a[1], a[2]), a[3], a[4]); break;
+      case 1: o[0] = pdf_wm_ggx(vec3(a[0], a[1], a[2]), vec3(a[3], a[4], a[5]), a[6], a[7]); break;
+      case 2: { vec3 w = sample_wm_ggx(vec3(a[0], a[1], a[2]), a[3], a[4], a[5], a[6]); o[0] = w.x; o[1] = w.y; o[2] = w.z; break; }
+      case 3: o[0] = fresnel_dielectric(a[0], a[1]); break;
+      case 4: o[0] = fresnel_conductor(a[0], a[1], a[2]); break;
+      case 5: o[0] = ysin(a[0]); break;
+      case 6: o[0] = ycos(a[0]); break;
+      case 7: o[0] = yexp(a[0]); break;
+      case 8: o[0] = ylog(a[0]); break;
+      case 9: o[0] = ypow(a[0], a[1]); break;
+      case 10: { vec3 w = sample_cosine_hemisphere(a[0], a[1]); o[0] = w.x; o[1] = w.y; o[2] = w.z; break; }
+      case 11: o[0] = sigmoid(a[0]); break;
+      case 12: o[0] = sigmoid_inv(a[0]); break;
+      case 13: { uint32_t r = hash_pair(as_uint(a[0]), as_uint(a[1])); o[0] = as_float(r); break; }
+      case 14: { uint32_t r = owen_scramble_fast(as_uint(a[0]), as_uint(a[1])); o[0] = as_float(r); break; }
+      case 15: o[0] = uint_to_float01(as_uint(a[0])); break;
+      case 16: o[0] = L.ss_dir(a[0], a[1]); break;
+      case 17: o[0] = L.ss_avg(a[0]); break;
+      case 18: o[0] = L.ssms_dir(a[0], a[1], a[2]); break;
+      case 19: o[0] = L.sheen_dir(a[0], a[1]); break;
+      case 20: o[0] = directional_albedo_translucent(L, a[0], a[1], a[2]); break;
+      case 21: o[0] = emission_profile(a[0], a[1]); break;
+      default: g_err = "unknown function"; return YPT_E_INVALID;
+    }
+  }
+  return YPT_OK;
+}
+
+// the colour that a coefficient triplet reproduces under D65 (SpectralUtility.cpp:455-476), linear sRGB
+extern "C" int oracle_rgb_reproduce(const float* coeffs, float* rgb_out) {
+  double c[3] = {coeffs[0], coeffs[1], coeffs[2]}, r[3];
+  tables().r2s.reproduce(c, r);
+  for (int k = 0; k < 3; ++k) rgb_out[k] = (float)r[k];
+  return YPT_OK;
+}

@@ -99,3 +99,33 @@ def rgb_coeffs(rgb):
     out = np.zeros(3, np.float32)
     _check(load().oracle_rgb_coeffs(_f(rgb), _f(out)))
     return out
+
+
+FN = {"d_ggx": 0, "pdf_wm_ggx": 1, "sample_wm_ggx": 2, "fresnel_dielectric": 3, "fresnel_conductor": 4, "sin": 5,
+      "cos": 6, "exp": 7, "log": 8, "pow": 9, "sample_cosine_hemisphere": 10, "sigmoid": 11, "sigmoid_inv": 12,
+      "hash_pair": 13, "owen_scramble": 14, "uint_to_float01": 15, "ss_dir": 16, "ss_avg": 17, "ssms_dir": 18,
+      "sheen_dir": 19, "tr_dir": 20, "emission_profile": 21}
+
+
+def evaluate(name, args):
+    """Evaluates one shared-core function over rows of arguments (up to 8 floats per row)."""
+    lib = load()
+    lib.oracle_eval.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]
+    a = np.zeros((len(args), 8), np.float32)
+    arr = np.asarray(args)
+    if arr.dtype == np.uint32:
+        arr = arr.view(np.float32)
+    arr = arr.astype(np.float32, copy=False).reshape(len(args), -1)
+    a[:, :arr.shape[1]] = arr
+    out = np.zeros((len(args), 4), np.float32)
+    _check(lib.oracle_eval(FN[name], _f(a), len(a), _f(out)))
+    return out
+
+
+def rgb_reproduce(coeffs):
+    lib = load()
+    lib.oracle_rgb_reproduce.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    c = np.ascontiguousarray(coeffs, np.float32)
+    out = np.zeros(3, np.float32)
+    _check(lib.oracle_rgb_reproduce(_f(c), _f(out)))
+    return out
