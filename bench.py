@@ -132,11 +132,14 @@ def main():
         per_sample = probe["stats"][0] / max(1.0, probe["stats"][1])
         target = a.cpu_seconds
         want = max(1.0, target / max(per_sample, 1e-9))
-        stride = int(max(1, min(4096, (a.width * a.height) / want)))
-        res = oracle_lib.render(scene, frame, integ, a.warmup, 1, threads=threads, use_bvh=True, pixel_stride=stride)
+        npix = a.width * a.height
+        stride = int(max(1, min(4096, npix / want)))
+        passes = int(max(1, min(a.steps, round(want / npix)))) if stride == 1 else 1
+        res = oracle_lib.render(scene, frame, integ, a.warmup, passes, threads=threads, use_bvh=True, pixel_stride=stride)
         cpu = {"value": res["stats"][1] / res["stats"][0], "unit": "path samples/s", "cores": threads, "kind": "port",
-               "sample": "every %d-th pixel of one %dx%d pass (sample index %d), %d paths, %.1f s on %d threads of %s" %
-                         (stride, a.width, a.height, a.warmup, int(res["stats"][1]), res["stats"][0], threads, cpu_model())}
+               "sample": "every %d-th pixel of %d %dx%d pass(es) from sample index %d, %d paths, %.1f s on %d threads of %s" %
+                         (stride, passes, a.width, a.height, a.warmup, int(res["stats"][1]), res["stats"][0], threads,
+                          cpu_model())}
 
     out = {
         "metric": "path samples/s (NEE, random shapes ~1M tris, 1920x1080, depth 12)",
