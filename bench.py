@@ -76,14 +76,17 @@ def main():
 
     it = _lib.Integrator(integ, 0)
     cfg = tuning.best_config("%s-shapes%d-%dx%d-d%d" % (a.integrator, a.shapes, a.width, a.height, a.depth))
-    width = a.bvh_width or cfg.get("bvh_width", 2)
-    block = a.block_size or cfg.get("block_size", 128)
-    lds = a.lds_depth or cfg.get("lds_stack_depth", 24)
-    pif = a.passes_in_flight or cfg.get("passes_in_flight", 0)
-    it.set_option("bvh_width", width)
-    it.set_option("block_size", block)
-    it.set_option("lds_stack_depth", lds)
-    it.set_option("passes_in_flight", pif)
+    if a.bvh_width:
+        cfg["bvh_width"] = a.bvh_width
+    if a.block_size:
+        cfg["block_size"] = a.block_size
+    if a.lds_depth:
+        cfg["lds_stack_depth"] = a.lds_depth
+    if a.passes_in_flight:
+        cfg["passes_in_flight"] = a.passes_in_flight
+    for k, v in cfg.items():
+        it.set_option(k, v)
+    width = cfg["bvh_width"]
     t1 = time.time()
     it.set_scene(scene)
     it.set_frame(frame)
@@ -156,11 +159,10 @@ def main():
         "data": "synthetic (seeded random-shapes scene, seed 2)",
         "config": {"workload": "%s random-shapes %d shapes / %d tris, %dx%d, depth %d, 1 spp per step" %
                                (a.integrator.upper(), a.shapes, scene.triangle_count, a.width, a.height, a.depth),
-                   "bvh_width": width, "block_size": block, "lds_stack_depth": lds,
-                   "passes_in_flight": int(st_prof[13])},
+                   "launch": dict(cfg, passes_in_flight=int(st_prof[13]))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_extend<%d> (closest-hit traversal)" % width,
+                     "kernel": "%s<%d> (closest-hit traversal)" % ("k_extend_p" if cfg.get("persistent", 1) else "k_extend", width),
                      "bytes_per_launch": ext_bytes_per_launch, "ms_per_launch": ext_ms_per_launch},
         "cpu_baseline": cpu,
         "detail": {

@@ -143,8 +143,13 @@ int ypt_get_first_hit_features(ypt_integrator* h, float* out);
  *      [5] the whole render in ms                (when ypt_set_option("profile", 1));
  *  [10] extension launches. */
 int ypt_get_stats(ypt_integrator* h, double* out16);
-/* tuning knobs: "bvh_width" (2 or 8), "block_size" (64..1024), "count_traversal" (0/1),
- * "sort_by_material" (0/1) */
+/* Launch and diagnostics knobs (no counterpart in the reference, whose traversal is the driver's):
+ *  "bvh_width" 2 | 8, "block_size" 64..1024 (multiple of 64), "lds_stack_depth" 1..64,
+ *  "passes_in_flight" 0..16 (0 = automatic), "persistent" 0/1 (traversal kernels loop over their
+ *  queue), "persistent_waves" 1..64 (waves per CU of those launches), "refill_threshold" 1..64
+ *  (idle lanes before a wave fetches new rays), "shade_waves" 0..1024 (0 = one workgroup per queue
+ *  entry), "count_traversal" 0/1, "first_hit_features" 0/1, "profile" 0/1.
+ *  None of them changes a result bit. */
 int ypt_set_option(ypt_integrator* h, const char* name, int64_t value);
 
 /* Closest-hit queries against the scene's BVH on the GPU (the TraceRay / RayQuery the integrators

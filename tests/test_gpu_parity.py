@@ -72,9 +72,11 @@ def test_closest_hit_bitexact_shapes(ylib, oracle, gpu_nee, shapes_small, width)
     assert np.array_equal(t.view(np.uint32), t2.view(np.uint32))
 
 
-def _compare_render(ylib, oracle, scene, frame, integrator, passes, begin=0, width=2):
+def _compare_render(ylib, oracle, scene, frame, integrator, passes, begin=0, width=2, options=None):
     it = ylib.Integrator(integrator, 0)
     it.set_option("bvh_width", width)
+    for k, v in (options or {}).items():
+        it.set_option(k, v)
     it.set_scene(scene)
     it.set_frame(frame)
     it.render(begin, passes)
@@ -105,3 +107,33 @@ def test_render_parity_shapes_nee(ylib, oracle, shapes_small):
     frame = camera.make_frame(cam, 80, 60, max_depth=12)
     _compare_render(ylib, oracle, scene, frame, 1, passes=2, begin=5)
     _compare_render(ylib, oracle, scene, frame, 1, passes=1, begin=9, width=8)
+
+
+@pytest.mark.parametrize("opts", [
+    {"persistent": 0},
+    {"persistent": 1, "persistent_waves": 1, "refill_threshold": 1},
+    {"persistent": 1, "persistent_waves": 2, "refill_threshold": 64, "block_size": 192},
+    {"persistent": 1, "persistent_waves": 32, "refill_threshold": 16, "passes_in_flight": 3},
+])
+def test_launch_shapes_give_the_same_film(ylib, oracle, shapes_small, opts):
+    # persistent or one-workgroup-per-entry launches, any refill threshold, block size or number of
+    # passes in flight: every path touches only its own slot, so the films agree bit for bit
+    from yapt_amd import camera
+    scene, cam = shapes_small
+    frame = camera.make_frame(cam, 83, 61, max_depth=12)
+
+    def film(o, width):
+        it = ylib.Integrator(1, 0)
+        it.set_option("bvh_width", width)
+        for k, v in o.items():
+            it.set_option(k, v)
+        it.set_scene(scene)
+        it.set_frame(frame)
+        it.render(2, 5)
+        f = it.film()
+        it.close()
+        return f
+
+    for width in (2, 8):
+        base = film({"persistent": 0, "passes_in_flight": 1}, width)
+        assert np.array_equal(film(opts, width).view(np.uint32), base.view(np.uint32)), (width, opts)

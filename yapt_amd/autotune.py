@@ -2,8 +2,8 @@
 
     python -m yapt_amd.autotune [--out PATH] [--width 1920 --height 1080 --depth 12 --shapes 2000]
 
-Sweeps BVH width, threads per block, LDS traversal-stack depth and passes in flight for the
-random-shapes NEE workload and stores the fastest configuration (yapt_amd/tuned.json by default).
+Coordinate-descent search over the launch options (yapt_amd.tuning.SPACE) for the random-shapes
+workload; stores the fastest configuration (yapt_amd/tuned.json by default).
 """
 import argparse
 
@@ -19,6 +19,7 @@ def main():
     p.add_argument("--shapes", type=int, default=2000)
     p.add_argument("--integrator", default="nee", choices=["nee", "bpt"])
     p.add_argument("--passes", type=int, default=16)
+    p.add_argument("--rounds", type=int, default=2)
     a = p.parse_args()
     scene, cam = scenes.random_shapes(n_shapes=a.shapes, seed=2)
     frame = camera.make_frame(cam, a.width, a.height, max_depth=a.depth)
@@ -26,7 +27,7 @@ def main():
     it.set_scene(scene)
     it.set_frame(frame)
     key = "%s-shapes%d-%dx%d-d%d" % (a.integrator, a.shapes, a.width, a.height, a.depth)
-    best = tuning.autotune(it, key, passes=a.passes, out=a.out)
+    best = tuning.autotune(it, key, passes=a.passes, rounds=a.rounds, out=a.out)
     print("best", key, best)
     it.close()
 

@@ -78,106 +78,147 @@ __device__ __forceinline__ bool leaf_occluded(const TriRecord* tris, uint32_t re
   return false;
 }
 
-// MODE 0: closest hit. MODE 1: occlusion test against (targetT, target).
+// MODE 0: closest hit. MODE 1: occlusion test against (target t, target id).
+// Traversal state of one ray; a step visits one node (and intersects the leaves it hits), so a
+// lane can hand its ray back and take a new one between steps.
+struct RayTrav {
+  RayPre rp;
+  vec3 o, inv;
+  float tmin;
+  float best;         // closest t so far (MODE 0) or the target triangle's t (MODE 1)
+  float b1, b2;
+  uint32_t bid;       // closest triangle so far (MODE 0) or the target triangle (MODE 1)
+  uint32_t cur;       // node to visit next
+  uint32_t occluded;  // MODE 1 result
+};
+
+__device__ __forceinline__ void trav_init(RayTrav& r, vec3 o, vec3 d, float tmin, float tmax, uint32_t target) {
+  r.rp = ray_precompute(o, d);
+  r.o = o;
+  r.inv = vec3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  r.tmin = tmin;
+  r.best = tmax;
+  r.b1 = 0.f;
+  r.b2 = 0.f;
+  r.bid = target;
+  r.cur = 0;
+  r.occluded = 0;
+}
+
+// BVH2: both child boxes in one 64-byte node; hit leaves are intersected at once, the nearer inner
+// child is visited next and the farther one pushed. Returns true when the ray is finished.
 template <int MODE, bool COUNT>
-__device__ bool traverse_bvh2(const Bvh2Node* nodes, const TriRecord* tris, vec3 o, vec3 d, float tmin, float tmax,
-                              TravStack& st, float& bestT, uint32_t& bestId, float& b1, float& b2, uint32_t target,
-                              TraversalCounters& cnt) {
-  RayPre rp = ray_precompute(o, d);
-  vec3 inv(1.f / d.x, 1.f / d.y, 1.f / d.z);
-  float best = tmax;
-  uint32_t bid = NO_HIT_ID;
-  st.sp = 0;
-  uint32_t cur = 0;
-  while (true) {
-    const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-    float4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
-    if (COUNT) cnt.nodes++;
-    float tn0, tn1;
-    uint32_t c0 = __float_as_uint(qc.x), c1 = __float_as_uint(qc.y);
-    bool h0 = c0 != BVH_EMPTY && slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), o, inv, tmin, best, tn0);
-    bool h1 = c1 != BVH_EMPTY && slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), o, inv, tmin, best, tn1);
-    if (h0 && (c0 & BVH_LEAF_BIT)) {
-      if (MODE == 0) leaf_closest<COUNT>(tris, c0, rp, tmin, best, bid, b1, b2, cnt);
-      else if (leaf_occluded<COUNT>(tris, c0, rp, tmin, tmax, target, cnt)) return true;
-      h0 = false;
-    }
-    if (h1 && (c1 & BVH_LEAF_BIT)) {
-      if (MODE == 0) leaf_closest<COUNT>(tris, c1, rp, tmin, best, bid, b1, b2, cnt);
-      else if (leaf_occluded<COUNT>(tris, c1, rp, tmin, tmax, target, cnt)) return true;
-      h1 = false;
-    }
-    if (h0 && h1) {
-      if (tn1 < tn0) { uint32_t t = c0; c0 = c1; c1 = t; }
-      st.push(c1);
-      cur = c0;
-    } else if (h0) {
-      cur = c0;
-    } else if (h1) {
-      cur = c1;
-    } else {
-      if (st.sp == 0) break;
-      cur = st.pop();
-    }
+__device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriRecord* tris, RayTrav& r, TravStack& st,
+                                          TraversalCounters& cnt) {
+  const float4* np = reinterpret_cast<const float4*>(nodes + r.cur);
+  float4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
+  if (COUNT) cnt.nodes++;
+  float tn0, tn1;
+  uint32_t c0 = __float_as_uint(qc.x), c1 = __float_as_uint(qc.y);
+  bool h0 = c0 != BVH_EMPTY && slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), r.o, r.inv, r.tmin, r.best, tn0);
+  bool h1 = c1 != BVH_EMPTY && slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), r.o, r.inv, r.tmin, r.best, tn1);
+  if (h0 && (c0 & BVH_LEAF_BIT)) {
+    if (MODE == 0) leaf_closest<COUNT>(tris, c0, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
+    else if (leaf_occluded<COUNT>(tris, c0, r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
+    h0 = false;
   }
-  bestT = best;
-  bestId = bid;
-  return bid != NO_HIT_ID;
+  if (h1 && (c1 & BVH_LEAF_BIT)) {
+    if (MODE == 0) leaf_closest<COUNT>(tris, c1, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
+    else if (leaf_occluded<COUNT>(tris, c1, r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
+    h1 = false;
+  }
+  if (h0 && h1) {
+    if (tn1 < tn0) { uint32_t t = c0; c0 = c1; c1 = t; }
+    st.push(c1);
+    r.cur = c0;
+  } else if (h0) {
+    r.cur = c0;
+  } else if (h1) {
+    r.cur = c1;
+  } else {
+    if (st.sp == 0) return true;
+    r.cur = st.pop();
+  }
+  return false;
 }
 
 // BVH8: test all eight boxes of a node, intersect hit leaves, push hit inner children far to near.
 template <int MODE, bool COUNT>
+__device__ __forceinline__ bool step_bvh8(const Bvh8Node* nodes, const TriRecord* tris, RayTrav& r, TravStack& st,
+                                          TraversalCounters& cnt) {
+  const float4* np = reinterpret_cast<const float4*>(nodes + r.cur);
+  if (COUNT) cnt.nodes++;
+  float4 lx0 = np[0], lx1 = np[1], hx0 = np[2], hx1 = np[3];
+  float4 ly0 = np[4], ly1 = np[5], hy0 = np[6], hy1 = np[7];
+  float4 lz0 = np[8], lz1 = np[9], hz0 = np[10], hz1 = np[11];
+  uint4 ch0 = reinterpret_cast<const uint4*>(np)[12], ch1 = reinterpret_cast<const uint4*>(np)[13];
+  float lx[8] = {lx0.x, lx0.y, lx0.z, lx0.w, lx1.x, lx1.y, lx1.z, lx1.w};
+  float hx[8] = {hx0.x, hx0.y, hx0.z, hx0.w, hx1.x, hx1.y, hx1.z, hx1.w};
+  float ly[8] = {ly0.x, ly0.y, ly0.z, ly0.w, ly1.x, ly1.y, ly1.z, ly1.w};
+  float hy[8] = {hy0.x, hy0.y, hy0.z, hy0.w, hy1.x, hy1.y, hy1.z, hy1.w};
+  float lz[8] = {lz0.x, lz0.y, lz0.z, lz0.w, lz1.x, lz1.y, lz1.z, lz1.w};
+  float hz[8] = {hz0.x, hz0.y, hz0.z, hz0.w, hz1.x, hz1.y, hz1.z, hz1.w};
+  uint32_t ch[8] = {ch0.x, ch0.y, ch0.z, ch0.w, ch1.x, ch1.y, ch1.z, ch1.w};
+  // hit inner children, sorted near to far by insertion (at most 8)
+  uint32_t hc[8];
+  float ht[8];
+  int nh = 0;
+  for (int s = 0; s < 8; ++s) {
+    if (ch[s] == BVH_EMPTY) continue;
+    float tn;
+    if (!slab_test(vec3(lx[s], ly[s], lz[s]), vec3(hx[s], hy[s], hz[s]), r.o, r.inv, r.tmin, r.best, tn)) continue;
+    if (ch[s] & BVH_LEAF_BIT) {
+      if (MODE == 0) leaf_closest<COUNT>(tris, ch[s], r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
+      else if (leaf_occluded<COUNT>(tris, ch[s], r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
+      continue;
+    }
+    int p = nh++;
+    while (p > 0 && ht[p - 1] > tn) { ht[p] = ht[p - 1]; hc[p] = hc[p - 1]; --p; }
+    ht[p] = tn; hc[p] = ch[s];
+  }
+  if (nh > 0) {
+    for (int k = nh - 1; k >= 1; --k) st.push(hc[k]);
+    r.cur = hc[0];
+  } else {
+    if (st.sp == 0) return true;
+    r.cur = st.pop();
+  }
+  return false;
+}
+
+template <int WIDTH, int MODE, bool COUNT>
+__device__ __forceinline__ bool trav_step(const Bvh2Node* nodes2, const Bvh8Node* nodes8, const TriRecord* tris, RayTrav& r,
+                                          TravStack& st, TraversalCounters& cnt) {
+  if (WIDTH == 2) return step_bvh2<MODE, COUNT>(nodes2, tris, r, st, cnt);
+  return step_bvh8<MODE, COUNT>(nodes8, tris, r, st, cnt);
+}
+
+// Whole traversal of one ray. MODE 0 returns whether anything was hit (closest in bestT/bestId);
+// MODE 1 returns whether a triangle other than `target` lies closer than (tmax, target).
+template <int MODE, bool COUNT>
+__device__ bool traverse_bvh2(const Bvh2Node* nodes, const TriRecord* tris, vec3 o, vec3 d, float tmin, float tmax,
+                              TravStack& st, float& bestT, uint32_t& bestId, float& b1, float& b2, uint32_t target,
+                              TraversalCounters& cnt) {
+  RayTrav r;
+  trav_init(r, o, d, tmin, tmax, MODE == 0 ? NO_HIT_ID : target);
+  st.sp = 0;
+  while (!step_bvh2<MODE, COUNT>(nodes, tris, r, st, cnt)) {}
+  if (MODE == 1) return r.occluded != 0;
+  bestT = r.best; bestId = r.bid; b1 = r.b1; b2 = r.b2;
+  return r.bid != NO_HIT_ID;
+}
+
+template <int MODE, bool COUNT>
 __device__ bool traverse_bvh8(const Bvh8Node* nodes, const TriRecord* tris, vec3 o, vec3 d, float tmin, float tmax,
                               TravStack& st, float& bestT, uint32_t& bestId, float& b1, float& b2, uint32_t target,
                               TraversalCounters& cnt) {
-  RayPre rp = ray_precompute(o, d);
-  vec3 inv(1.f / d.x, 1.f / d.y, 1.f / d.z);
-  float best = tmax;
-  uint32_t bid = NO_HIT_ID;
+  RayTrav r;
+  trav_init(r, o, d, tmin, tmax, MODE == 0 ? NO_HIT_ID : target);
   st.sp = 0;
-  uint32_t cur = 0;
-  while (true) {
-    const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-    if (COUNT) cnt.nodes++;
-    float4 lx0 = np[0], lx1 = np[1], hx0 = np[2], hx1 = np[3];
-    float4 ly0 = np[4], ly1 = np[5], hy0 = np[6], hy1 = np[7];
-    float4 lz0 = np[8], lz1 = np[9], hz0 = np[10], hz1 = np[11];
-    uint4 ch0 = reinterpret_cast<const uint4*>(np)[12], ch1 = reinterpret_cast<const uint4*>(np)[13];
-    float lx[8] = {lx0.x, lx0.y, lx0.z, lx0.w, lx1.x, lx1.y, lx1.z, lx1.w};
-    float hx[8] = {hx0.x, hx0.y, hx0.z, hx0.w, hx1.x, hx1.y, hx1.z, hx1.w};
-    float ly[8] = {ly0.x, ly0.y, ly0.z, ly0.w, ly1.x, ly1.y, ly1.z, ly1.w};
-    float hy[8] = {hy0.x, hy0.y, hy0.z, hy0.w, hy1.x, hy1.y, hy1.z, hy1.w};
-    float lz[8] = {lz0.x, lz0.y, lz0.z, lz0.w, lz1.x, lz1.y, lz1.z, lz1.w};
-    float hz[8] = {hz0.x, hz0.y, hz0.z, hz0.w, hz1.x, hz1.y, hz1.z, hz1.w};
-    uint32_t ch[8] = {ch0.x, ch0.y, ch0.z, ch0.w, ch1.x, ch1.y, ch1.z, ch1.w};
-    // hit inner children, sorted near to far by insertion (at most 8)
-    uint32_t hc[8];
-    float ht[8];
-    int nh = 0;
-    for (int s = 0; s < 8; ++s) {
-      if (ch[s] == BVH_EMPTY) continue;
-      float tn;
-      if (!slab_test(vec3(lx[s], ly[s], lz[s]), vec3(hx[s], hy[s], hz[s]), o, inv, tmin, best, tn)) continue;
-      if (ch[s] & BVH_LEAF_BIT) {
-        if (MODE == 0) leaf_closest<COUNT>(tris, ch[s], rp, tmin, best, bid, b1, b2, cnt);
-        else if (leaf_occluded<COUNT>(tris, ch[s], rp, tmin, tmax, target, cnt)) return true;
-        continue;
-      }
-      int p = nh++;
-      while (p > 0 && ht[p - 1] > tn) { ht[p] = ht[p - 1]; hc[p] = hc[p - 1]; --p; }
-      ht[p] = tn; hc[p] = ch[s];
-    }
-    if (nh > 0) {
-      for (int k = nh - 1; k >= 1; --k) st.push(hc[k]);
-      cur = hc[0];
-    } else {
-      if (st.sp == 0) break;
-      cur = st.pop();
-    }
-  }
-  bestT = best;
-  bestId = bid;
-  return bid != NO_HIT_ID;
+  while (!step_bvh8<MODE, COUNT>(nodes, tris, r, st, cnt)) {}
+  if (MODE == 1) return r.occluded != 0;
+  bestT = r.best; bestId = r.bid; b1 = r.b1; b2 = r.b2;
+  return r.bid != NO_HIT_ID;
 }
 
 }  // namespace ypt

@@ -122,7 +122,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 __global__ void k_start(FrameConstants fc, const uint32_t* points, PathSoA P, Queues Q, FilmBuffers F, uint32_t npix,
                         uint32_t nslots, int writeFeatures) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) { Q.counters[0] = nslots; Q.counters[1] = 0; Q.counters[2] = 0; }
+  if (i == 0) { Q.counters[0] = nslots; Q.counters[1] = 0; Q.counters[2] = 0; Q.counters[4] = 0; }
   if (i >= nslots) return;
   uint32_t W = (uint32_t)fc.targetTexDimensions[0];
   uint32_t pass = i / npix, pix = i - pass * npix;
@@ -162,45 +162,55 @@ __global__ void k_extend(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t lds
   }
 }
 
+// Grid-stride over the ray queue: a launch sized for the device, not for the slot count, so the
+// short queues of deep bounces cost little. The loop bound is uniform across each workgroup, so
+// every wave reaches the queue appends with all of its lanes.
 template <bool NEE>
 __global__ void k_shade(FrameConstants fc, DevScene S, const uint32_t* points, uint32_t npix, PathSoA P, Queues Q,
                         FilmBuffers F, int qsel, int writeFeatures) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t n = Q.counters[qsel];
-  if (i >= n) return;  // whole waves past the end leave together; the last wave keeps its live lanes
-  uint32_t slot = Q.ray[qsel][i];
-  const uint32_t* point = points + (size_t)(slot / npix) * YPT_SOBOL_POINT_DIMS;
-  PathState ps;
-  load_path(P, slot, ps);
-  uint32_t nvolIn = ps.nvol;
-  float4 hv = Q.hit[slot];
-  Hit h;
-  h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.tri = __float_as_uint(hv.w);
-  ShadowRay sr;
-  sr.valid = false;
-  FirstHitFeatures fh;
-  fh.hit = 0u;
-  bool first = ps.pathLength == 0;
-  if (h.tri == NO_HIT_ID) shade_miss(S.sv, fc, ps);
-  else shade_hit(S.sv, fc, point, NEE, ps, h, sr, (writeFeatures && first) ? &fh : nullptr);
-  after_segment(point, ps);
-  bool cont = ps.rayState == RAY_ALIVE && ps.pathLength < fc.maxRayDepth;
-  store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);
-  if (writeFeatures && first && fh.hit) {
-    float* f = F.features + (size_t)slot * 12;
-    f[0] = fh.normal.x; f[1] = fh.normal.y; f[2] = fh.normal.z;
-    f[3] = fh.albedo.x; f[4] = fh.albedo.y; f[5] = fh.albedo.z;
-    f[6] = fh.roughness; f[7] = fh.ior; f[8] = fh.metalness; f[9] = fh.transparency; f[10] = fh.depth; f[11] = 1.f;
+  if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;  // fetch counter of the next extension launch
+  const uint32_t n = Q.counters[qsel];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    ShadowRay sr;
+    sr.valid = false;
+    bool cont = false;
+    uint32_t slot = 0;
+    if (i < n) {
+      slot = Q.ray[qsel][i];
+      const uint32_t* point = points + (size_t)(slot / npix) * YPT_SOBOL_POINT_DIMS;
+      PathState ps;
+      load_path(P, slot, ps);
+      uint32_t nvolIn = ps.nvol;
+      float4 hv = Q.hit[slot];
+      Hit h;
+      h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.tri = __float_as_uint(hv.w);
+      FirstHitFeatures fh;
+      fh.hit = 0u;
+      bool first = ps.pathLength == 0;
+      if (h.tri == NO_HIT_ID) shade_miss(S.sv, fc, ps);
+      else shade_hit(S.sv, fc, point, NEE, ps, h, sr, (writeFeatures && first) ? &fh : nullptr);
+      after_segment(point, ps);
+      cont = ps.rayState == RAY_ALIVE && ps.pathLength < fc.maxRayDepth;
+      store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);
+      if (writeFeatures && first && fh.hit) {
+        float* f = F.features + (size_t)slot * 12;
+        f[0] = fh.normal.x; f[1] = fh.normal.y; f[2] = fh.normal.z;
+        f[3] = fh.albedo.x; f[4] = fh.albedo.y; f[5] = fh.albedo.z;
+        f[6] = fh.roughness; f[7] = fh.ior; f[8] = fh.metalness; f[9] = fh.transparency; f[10] = fh.depth; f[11] = 1.f;
+      }
+    }
+    uint32_t si = wave_append(&Q.counters[2], sr.valid);
+    if (sr.valid) {
+      Q.sOrgTmax[si] = make_float4(sr.origin.x, sr.origin.y, sr.origin.z, sr.tmax);
+      Q.sDirTri[si] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, __uint_as_float(sr.targetTri));
+      Q.sContrib[si] = make_float4(sr.contrib.s[0], sr.contrib.s[1], sr.contrib.s[2], sr.contrib.s[3]);
+      Q.sSlot[si] = slot;
+    }
+    uint32_t ri = wave_append(&Q.counters[qsel ^ 1], cont);
+    if (cont) Q.ray[qsel ^ 1][ri] = slot;
   }
-  uint32_t si = wave_append(&Q.counters[2], sr.valid);
-  if (sr.valid) {
-    Q.sOrgTmax[si] = make_float4(sr.origin.x, sr.origin.y, sr.origin.z, sr.tmax);
-    Q.sDirTri[si] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, __uint_as_float(sr.targetTri));
-    Q.sContrib[si] = make_float4(sr.contrib.s[0], sr.contrib.s[1], sr.contrib.s[2], sr.contrib.s[3]);
-    Q.sSlot[si] = slot;
-  }
-  uint32_t ri = wave_append(&Q.counters[qsel ^ 1], cont);
-  if (cont) Q.ray[qsel ^ 1][ri] = slot;
 }
 
 template <int WIDTH, bool COUNT>
@@ -237,6 +247,130 @@ __global__ void k_shadow(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth) {
   }
   if (COUNT) {
     unsigned long long nn = wave_sum(cnt.nodes), nt = wave_sum(cnt.tris), nr = wave_sum(i < n ? 1ull : 0ull);
+    if (__lane_id() == 0 && (nr | nt)) {
+      atomicAdd(&Q.workCounters[2], nn); atomicAdd(&Q.workCounters[3], nt); atomicAdd(&Q.workCounters[5], nr);
+    }
+  }
+}
+
+// Persistent traversal: a launch of a few waves per SIMD whose lanes take rays from the queue
+// through a fetch counter. A lane whose ray is finished idles until at least `refill` lanes of its
+// wave are idle; then the wave takes that many new rays with one atomic. Waves stay busy however
+// unevenly long the rays of a bounce are, and a bounce with few rays costs a few hundred waves, not
+// one workgroup per path slot. Fetch counter: counters[4] (extension), counters[5] (shadow).
+struct WaveFetch {
+  uint32_t* counter;
+  uint32_t n;
+  uint32_t refill;
+  bool drained;
+  // returns the queue index for this lane (>= n when none), or UINT32_MAX when the wave did not fetch
+  __device__ __forceinline__ uint32_t next(bool idle) {
+    if (drained) return 0xFFFFFFFFu;
+    unsigned long long need = __ballot(idle ? 1 : 0);
+    uint32_t k = (uint32_t)__popcll(need);
+    if (k < refill) return 0xFFFFFFFFu;
+    uint32_t lane = __lane_id();
+    int leader = __ffsll((long long)need) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, k);
+    base = (uint32_t)__shfl((int)base, leader);
+    if (base + k >= n) drained = true;
+    if (!idle) return 0xFFFFFFFFu;
+    return base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+  }
+};
+
+template <int WIDTH, bool COUNT>
+__global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t ldsDepth, uint32_t refill) {
+  extern __shared__ uint32_t lds[];
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid == 0) { Q.counters[qsel ^ 1] = 0; Q.counters[2] = 0; Q.counters[5] = 0; }
+  WaveFetch wf;
+  wf.counter = &Q.counters[4]; wf.n = Q.counters[qsel]; wf.refill = refill; wf.drained = false;
+  TravStack st;
+  st.lds = lds; st.ovf = Q.ovf; st.block = blockDim.x; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
+  st.ovfStride = Q.ovfStride; st.gid = gid; st.sp = 0;
+  TraversalCounters cnt = {0u, 0u};
+  unsigned long long nrays = 0;
+  RayTrav r;
+  bool has = false;
+  uint32_t slot = 0;
+  while (true) {
+    uint32_t idx = wf.next(!has);
+    if (!has && idx < wf.n) {
+      slot = Q.ray[qsel][idx];
+      float4 a = P.orgLambda[slot], b = P.dirPdf[slot];
+      trav_init(r, vec3(a.x, a.y, a.z), vec3(b.x, b.y, b.z), RAY_MIN_T, __builtin_inff(), NO_HIT_ID);
+      st.sp = 0;
+      has = true;
+      if (COUNT) nrays++;
+    }
+    if (__ballot(has ? 1 : 0) == 0ull) {
+      if (wf.drained) break;
+      continue;
+    }
+    if (has && trav_step<WIDTH, 0, COUNT>(S.nodes2, S.nodes8, S.tris, r, st, cnt)) {
+      Q.hit[slot] = make_float4(r.best, r.b1, r.b2, __uint_as_float(r.bid));
+      has = false;
+    }
+  }
+  if (COUNT) {
+    unsigned long long nn = wave_sum(cnt.nodes), nt = wave_sum(cnt.tris), nr = wave_sum(nrays);
+    if (__lane_id() == 0 && (nr | nt)) {
+      atomicAdd(&Q.workCounters[0], nn); atomicAdd(&Q.workCounters[1], nt); atomicAdd(&Q.workCounters[4], nr);
+    }
+  }
+}
+
+template <int WIDTH, bool COUNT>
+__global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, uint32_t refill) {
+  extern __shared__ uint32_t lds[];
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  WaveFetch wf;
+  wf.counter = &Q.counters[5]; wf.n = Q.counters[2]; wf.refill = refill; wf.drained = false;
+  TravStack st;
+  st.lds = lds; st.ovf = Q.ovf; st.block = blockDim.x; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
+  st.ovfStride = Q.ovfStride; st.gid = gid; st.sp = 0;
+  TraversalCounters cnt = {0u, 0u};
+  unsigned long long nrays = 0;
+  RayTrav r;
+  bool has = false;
+  uint32_t cur = 0;
+  while (true) {
+    uint32_t idx = wf.next(!has);
+    if (!has && idx < wf.n) {
+      if (COUNT) nrays++;
+      float4 a = Q.sOrgTmax[idx], b = Q.sDirTri[idx];
+      vec3 o(a.x, a.y, a.z), d(b.x, b.y, b.z);
+      uint32_t target = __float_as_uint(b.w);
+      // the light triangle itself first: the connection counts only if it is the closest hit
+      RayPre rp = ray_precompute(o, d);
+      vec3 p1, p2, p3;
+      tri_vertices(S.sv, target, p1, p2, p3);
+      float tt, b1, b2;
+      if (intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2)) {
+        trav_init(r, o, d, RAY_MIN_T, tt, target);
+        st.sp = 0;
+        has = true;
+        cur = idx;
+      }
+    }
+    if (__ballot(has ? 1 : 0) == 0ull) {
+      if (wf.drained) break;
+      continue;
+    }
+    if (has && trav_step<WIDTH, 1, COUNT>(S.nodes2, S.nodes8, S.tris, r, st, cnt)) {
+      if (!r.occluded) {
+        uint32_t slot = Q.sSlot[cur];
+        float4 c = Q.sContrib[cur];
+        float4 l = P.light[slot];
+        P.light[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w + c.w);
+      }
+      has = false;
+    }
+  }
+  if (COUNT) {
+    unsigned long long nn = wave_sum(cnt.nodes), nt = wave_sum(cnt.tris), nr = wave_sum(nrays);
     if (__lane_id() == 0 && (nr | nt)) {
       atomicAdd(&Q.workCounters[2], nn); atomicAdd(&Q.workCounters[3], nt); atomicAdd(&Q.workCounters[5], nr);
     }

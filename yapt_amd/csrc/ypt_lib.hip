@@ -138,6 +138,11 @@ struct ypt_integrator {
   int countTraversal = 0;
   int writeFeatures = 0;
   int passesInFlight = 0;  // 0 = automatic
+  int persistent = 1;      // traversal/shading kernels loop over their queue (see work_grid)
+  int persistentWaves = 32;
+  int refill = 16;         // idle lanes of a wave before it fetches new rays
+  int shadeWaves = 64;     // waves per CU of the shading launch (0: one workgroup per queue entry)
+  int cus = 256;           // compute units of the device
 
   // stats of the last render
   double stats[16] = {0};
@@ -227,14 +232,18 @@ void ensure_slots(ypt_integrator* h, size_t slots) {
   h->ovfStride = 0;
 }
 
+// traversal-stack overflow: one column per thread that can traverse. A thread's id is below the
+// queue length (one workgroup per entry, rounded up to a workgroup of at most 1024) or below the
+// persistent launch size.
 void ensure_overflow(ypt_integrator* h) {
-  size_t slots = h->slots;
+  size_t stride = h->slots + 1024;
+  if (h->persistent) stride = std::min(stride, (size_t)h->cus * h->persistentWaves * 64);
   uint32_t bound = h->bvhWidth == 8 ? h->stackBound8 : h->stackBound2;
   size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
-  size_t need = depth * slots;
-  if (h->ovf.n >= need && h->ovfStride == slots) return;
+  size_t need = depth * stride;
+  if (h->ovf.n >= need && h->ovfStride == stride) return;
   h->ovf.alloc(need);
-  h->ovfStride = (uint32_t)slots;
+  h->ovfStride = (uint32_t)stride;
 }
 
 // passes in flight: enough slots (~32M paths, ~10 GB of path state) that the deep, nearly empty
@@ -288,21 +297,54 @@ int guarded(F&& f) {
   }
 }
 
+// launch shape of the traversal and shading kernels: one workgroup per queue entry, or (persistent)
+// at most `persistentWaves` waves per CU that loop over the queue
+uint32_t work_grid(const ypt_integrator* h, uint32_t nslots) {
+  uint32_t B = (uint32_t)h->blockSize;
+  uint32_t g = blocks_for(nslots, B);
+  if (!h->persistent) return g;
+  uint32_t maxb = std::max<uint32_t>(1u, (uint32_t)((size_t)h->cus * h->persistentWaves * 64 / B));
+  return std::min(g, maxb);
+}
+
+// shading: grid-stride over the queue with `shadeWaves` waves per CU, or one workgroup per entry
+uint32_t shade_grid(const ypt_integrator* h, uint32_t nslots) {
+  uint32_t B = (uint32_t)h->blockSize;
+  uint32_t g = blocks_for(nslots, B);
+  if (h->shadeWaves == 0) return g;
+  uint32_t maxb = std::max<uint32_t>(1u, (uint32_t)((size_t)h->cus * h->shadeWaves * 64 / B));
+  return std::min(g, maxb);
+}
+
 template <int WIDTH>
 void launch_extend(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, int q, uint32_t grid) {
   size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
-  if (h->countTraversal)
-    hipLaunchKernelGGL((k_extend<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, (uint32_t)h->ldsDepth);
-  else
-    hipLaunchKernelGGL((k_extend<WIDTH, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, (uint32_t)h->ldsDepth);
+  uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
+  if (h->persistent) {
+    if (h->countTraversal)
+      hipLaunchKernelGGL((k_extend_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r);
+    else
+      hipLaunchKernelGGL((k_extend_p<WIDTH, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r);
+  } else if (h->countTraversal) {
+    hipLaunchKernelGGL((k_extend<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d);
+  } else {
+    hipLaunchKernelGGL((k_extend<WIDTH, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d);
+  }
 }
 template <int WIDTH>
 void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, uint32_t grid) {
   size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
-  if (h->countTraversal)
-    hipLaunchKernelGGL((k_shadow<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, (uint32_t)h->ldsDepth);
-  else
-    hipLaunchKernelGGL((k_shadow<WIDTH, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, (uint32_t)h->ldsDepth);
+  uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
+  if (h->persistent) {
+    if (h->countTraversal)
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+    else
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+  } else if (h->countTraversal) {
+    hipLaunchKernelGGL((k_shadow<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d);
+  } else {
+    hipLaunchKernelGGL((k_shadow<WIDTH, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d);
+  }
 }
 
 }  // namespace
@@ -331,6 +373,8 @@ int ypt_create(int integrator_type, int device, ypt_integrator** out) {
     std::unique_ptr<ypt_integrator> h(new ypt_integrator());
     h->type = integrator_type;
     h->device = device;
+    HIP_CHECK(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (h->cus < 1) h->cus = 1;
     HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     h->ownStream = true;
     for (auto& e : h->ev) HIP_CHECK(hipEventCreate(&e));
@@ -373,6 +417,17 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
     h->passesInFlight = (int)value;
   } else if (n == "profile") {
     h->profile = value != 0;
+  } else if (n == "persistent") {
+    h->persistent = value != 0;
+  } else if (n == "persistent_waves") {
+    if (value < 1 || value > 64) return fail(YPT_E_INVALID, "persistent_waves must be in [1, 64] (waves per CU)");
+    h->persistentWaves = (int)value;
+  } else if (n == "shade_waves") {
+    if (value < 0 || value > 1024) return fail(YPT_E_INVALID, "shade_waves must be in [0, 1024] (0 = one workgroup per entry)");
+    h->shadeWaves = (int)value;
+  } else if (n == "refill_threshold") {
+    if (value < 1 || value > 64) return fail(YPT_E_INVALID, "refill_threshold must be in [1, 64] (lanes of a wave)");
+    h->refill = (int)value;
   } else {
     return fail(YPT_E_INVALID, "unknown option " + n);
   }
@@ -486,7 +541,8 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
     for (uint32_t k = 0; k < sample_count; k += G) {
       uint32_t g = std::min(G, sample_count - k);
       uint32_t nslots = npix * g;
-      uint32_t grid = blocks_for(nslots, B);
+      uint32_t grid = work_grid(h, nslots);
+      uint32_t sgrid = shade_grid(h, nslots);
       const uint32_t* point = h->sobolPoints.p + (size_t)k * YPT_SOBOL_POINT_DIMS;
       hipEvent_t e0 = h->profile ? mark() : nullptr;
       hipLaunchKernelGGL(k_start, dim3(blocks_for(nslots, 256)), dim3(256), 0, s, h->fc, point, P, Q, F, npix, nslots,
@@ -499,9 +555,9 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
         else launch_extend<2>(h, S, P, Q, q, grid);
         hipEvent_t a1 = h->profile ? mark() : nullptr;
         if (h->type == YPT_INTEGRATOR_NEE)
-          hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
+          hipLaunchKernelGGL(k_shade<true>, dim3(sgrid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
         else
-          hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
+          hipLaunchKernelGGL(k_shade<false>, dim3(sgrid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
         hipEvent_t a2 = h->profile ? mark() : nullptr;
         if (h->profile) { spans.push_back({a0, a1, 0}); spans.push_back({a1, a2, 1}); }
         if (h->type == YPT_INTEGRATOR_NEE) {

@@ -1,8 +1,10 @@
 """Launch-configuration autotuner and the store of tuned choices.
 
-The autotuner sweeps BVH width (2 vs 8), threads per block and LDS traversal-stack depth for a
-workload, timing whole passes, and records the fastest configuration in ``tuned.json`` next to this
-file. ``best_config`` returns the stored choice (or defaults).
+The autotuner searches BVH width (2 vs 8), threads per block, LDS traversal-stack depth, passes in
+flight (queue depth), persistent vs one-workgroup-per-ray launches with their waves per CU and refill
+threshold, and the shading launch size for a workload, timing whole passes, and records the fastest
+configuration in ``tuned.json`` next to this file. ``best_config`` returns the stored choice (or
+defaults).
 """
 import json
 import os
@@ -12,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(HERE, "tuned.json")
 ROOT = os.path.dirname(HERE)
 
-DEFAULT = {"bvh_width": 2, "block_size": 64, "lds_stack_depth": 16, "passes_in_flight": 0}
+DEFAULT = {"bvh_width": 2, "block_size": 64, "lds_stack_depth": 16, "passes_in_flight": 0, "persistent": 1,
+           "persistent_waves": 32, "refill_threshold": 16, "shade_waves": 64}
 
 
 def _load():
@@ -42,32 +45,69 @@ def measured_traffic(kernel_key):
     return None if v is None else v.get("bytes_per_launch")
 
 
-def autotune(integrator, key, passes=16, widths=(2, 8), blocks=(64, 128), lds_depths=(8, 16, 24), flights=(8, 16),
-             verbose=True, out=None):
-    """Times `passes` full passes (scene and frame already set) for every configuration of BVH width,
-    threads per block, LDS stack depth and passes in flight; stores the fastest under `key`."""
+SPACE = {
+    "bvh_width": (2, 8),
+    "block_size": (64, 128, 256),
+    "lds_stack_depth": (8, 12, 16, 24),
+    "passes_in_flight": (4, 8, 16),
+    "persistent": (0, 1),
+    "persistent_waves": (8, 16, 24, 32),
+    "refill_threshold": (4, 8, 16, 32, 64),
+    "shade_waves": (0, 16, 32, 64, 128),
+}
+
+
+def _time(integrator, cfg, passes):
+    for k, v in cfg.items():
+        integrator.set_option(k, v)
+    integrator.render(0, 2)
+    integrator.synchronize()
+    t = time.perf_counter()
+    integrator.render(2, passes)
+    integrator.synchronize()
+    return (time.perf_counter() - t) / passes * 1e3
+
+
+def autotune(integrator, key, passes=16, space=None, rounds=2, start=None, verbose=True, out=None):
+    """Coordinate descent over the launch options: starting from `start` (the stored or default
+    configuration), each option in turn is set to every candidate of `space` with the others fixed,
+    timing `passes` full passes (scene and frame already set), and the fastest value is kept;
+    `rounds` sweeps over all options. Stores the best configuration under `key` and applies it."""
+    space = space or SPACE
+    cur = dict(start or best_config(key))
+    for k in space:
+        cur.setdefault(k, space[k][0])
     results = []
-    for g in flights:
-        for w in widths:
-            for b in blocks:
-                for d in lds_depths:
-                    cfg = {"bvh_width": w, "block_size": b, "lds_stack_depth": d, "passes_in_flight": g}
-                    for k, v in cfg.items():
-                        integrator.set_option(k, v)
-                    integrator.render(0, 2)
-                    integrator.synchronize()
-                    t = time.perf_counter()
-                    integrator.render(2, passes)
-                    integrator.synchronize()
-                    dt = (time.perf_counter() - t) / passes
-                    results.append({"config": cfg, "ms_per_pass": dt * 1e3})
-                    if verbose:
-                        print("autotune %s %s: %.2f ms/pass" % (key, cfg, dt * 1e3), flush=True)
-    best = min(results, key=lambda r: r["ms_per_pass"])
+    seen = {}
+
+    def measure(cfg):
+        sig = tuple(sorted(cfg.items()))
+        if sig not in seen:
+            ms = _time(integrator, cfg, passes)
+            seen[sig] = ms
+            results.append({"config": dict(cfg), "ms_per_pass": ms})
+            if verbose:
+                print("autotune %s %s: %.3f ms/pass" % (key, cfg, ms), flush=True)
+        return seen[sig]
+
+    best_ms = measure(cur)
+    for _ in range(rounds):
+        changed = False
+        for name, cands in space.items():
+            for v in cands:
+                if v == cur[name]:
+                    continue
+                cfg = dict(cur)
+                cfg[name] = v
+                ms = measure(cfg)
+                if ms < best_ms * 0.995:
+                    best_ms, cur, changed = ms, cfg, True
+        if not changed:
+            break
     store = _load()
-    store[key] = {"config": best["config"], "ms_per_pass": best["ms_per_pass"], "all": results}
+    store[key] = {"config": cur, "ms_per_pass": best_ms, "all": results}
     with open(out or TUNED, "w") as f:
         json.dump(store, f, indent=1, sort_keys=True)
-    for k, v in best["config"].items():
+    for k, v in cur.items():
         integrator.set_option(k, v)
-    return best
+    return {"config": cur, "ms_per_pass": best_ms}
