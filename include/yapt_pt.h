@@ -108,6 +108,9 @@ typedef struct ypt_frame_desc {
   float firefly_clamp_scale;     /* FireflyClampScale (default 12, 0 disables) */
   float firefly_clamp_relax_power;
   uint32_t kill_nan_samples;
+  uint32_t light_path_vertices;  /* BDPT: vertices per light subpath incl. the light, 2..16
+                                    (maxVerticesPerLightPath, BidirectionalPathIntegratorSubStage.cpp:482);
+                                    0 = 16 */
 } ypt_frame_desc;
 
 typedef struct ypt_integrator ypt_integrator;

@@ -30,6 +30,7 @@
 #include "../include/yapt_pt.h"
 #include "../yapt_amd/csrc/core/intersect.h"
 #include "../yapt_amd/csrc/core/shade.h"
+#include "../yapt_amd/csrc/core/bdpt.h"
 #include "../yapt_amd/csrc/runtime/luts.h"
 #include "../yapt_amd/csrc/runtime/prepare.h"
 #include "../yapt_amd/csrc/runtime/rgb2spec.h"
@@ -303,8 +304,128 @@ vec3 trace_pixel(const Ctx& c, const uint32_t* point, uint32_t px, uint32_t py, 
   return path_xyz(c.sv, ps);
 }
 
+// ---------------------------------------------------------------- bidirectional integrator
+struct CpuTracer {
+  const Geometry* geo;
+  uint64_t* work;
+  bool closest(vec3 o, vec3 d, float tmin, float tmax, Hit& h) {
+    if (work) work[2]++;
+    return geo->closest(o, d, tmin, tmax, h, work);
+  }
+  bool hits_target(vec3 o, vec3 d, float tmin, float tmax, uint32_t tri) {
+    if (work) work[3]++;
+    Hit h;
+    geo->closest(o, d, tmin, tmax, h, work);
+    return h.tri == tri;
+  }
+  bool unoccluded(vec3 o, vec3 d, float tmin, float tmax) {
+    if (work) work[3]++;
+    Hit h;
+    return !geo->closest(o, d, tmin, tmax, h, work);
+  }
+};
+
+template <class F>
+void parallel_for(uint32_t n, unsigned nt, F&& f) {
+  std::atomic<uint32_t> next{0};
+  std::vector<std::thread> th;
+  for (unsigned ti = 0; ti < nt; ++ti)
+    th.emplace_back([&, ti]() {
+      while (true) {
+        uint32_t i = next.fetch_add(1);
+        if (i >= n) break;
+        f(i, ti);
+      }
+    });
+  for (auto& x : th) x.join();
+}
+
+// One BDPT pass in the order of the reference's dispatches: light subpaths, camera subpaths per 8x8
+// workgroup of permuted texels, light tracing onto the lens, then the per-texel sum of those splats
+// in light-subpath order. Only every `stride`-th workgroup is traced when stride > 1.
+size_t bdpt_pass(const Ctx& c, const uint32_t* point, const std::vector<uint32_t>& perm, unsigned nt, uint32_t stride,
+                 std::vector<float>& xyz, std::vector<float>& lam, std::vector<FirstHitFeatures>* fh,
+                 std::vector<std::array<uint64_t, 4>>* work) {
+  const FrameConstants& fc = c.fc;
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1], npix = W * H;
+  uint32_t npaths = fc.lightPathsPerDim[0] * fc.lightPathsPerDim[1];
+  std::vector<LightNodePacked> nodes((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
+  std::vector<uint32_t> counts(npaths, 0);
+  std::vector<float> lambdas(npaths, 0.f);
+  auto wk = [&](unsigned ti) -> uint64_t* { return work ? (*work)[ti].data() : nullptr; };
+  parallel_for(npaths, nt, [&](uint32_t p, unsigned ti) {
+    CpuTracer tr{&c.geo, wk(ti)};
+    counts[p] = trace_light_path(c.sv, fc, point, p, tr, &nodes[(size_t)p * BDPT_MAX_LIGHT_VERTICES], lambdas[p]);
+  });
+  std::vector<uint32_t> headers;
+  for (uint32_t p = 0; p < npaths; ++p)
+    if (counts[p] > 0) headers.push_back(p);
+  uint32_t hc = (uint32_t)headers.size();
+  uint32_t gcx = (W + BDPT_TILE - 1) / BDPT_TILE, gcy = (H + BDPT_TILE - 1) / BDPT_TILE;
+  std::atomic<size_t> traced{0};
+  parallel_for(gcx * gcy, nt, [&](uint32_t g, unsigned ti) {
+    if (stride > 1 && g % stride != 0) return;
+    uint32_t gx = g % gcx, gy = g / gcx;
+    float lambda;
+    const LightNodePacked* nd = nullptr;
+    uint32_t count = 0;
+    if (hc > 0) {
+      uint32_t p = headers[pick_light_path(point, g, hc)];
+      lambda = lambdas[p];
+      nd = &nodes[(size_t)p * BDPT_MAX_LIGHT_VERTICES];
+      count = counts[p];
+    } else {
+      uint32_t q = perm[(gy * BDPT_TILE) * W + gx * BDPT_TILE];
+      lambda = fallback_wavelength(fc, point, q & 0xFFFFu, q >> 16);
+    }
+    CpuTracer tr{&c.geo, wk(ti)};
+    size_t n = 0;
+    for (uint32_t t = 0; t < BDPT_TILE * BDPT_TILE; ++t) {
+      uint32_t dx = gx * BDPT_TILE + t % BDPT_TILE, dy = gy * BDPT_TILE + t / BDPT_TILE;
+      if (dx >= W || dy >= H) continue;
+      uint32_t q = perm[dy * W + dx];
+      uint32_t px = q & 0xFFFFu, py = q >> 16;
+      size_t pix = (size_t)py * W + px;
+      FirstHitFeatures* f = fh ? &(*fh)[pix] : nullptr;
+      Spectrum L = trace_camera_path(c.sv, fc, point, px, py, lambda, nd, count, hc == 0 && t == 0, tr, f);
+      vec3 v = spectrum_to_xyz(L, spectral_lambdas(lambda), c.sv.spec);
+      xyz[3 * pix] = v.x; xyz[3 * pix + 1] = v.y; xyz[3 * pix + 2] = v.z;
+      lam[pix] = lambda;
+      ++n;
+    }
+    traced += n;
+  });
+  uint32_t per = fc.lightPathVertices - 1;
+  std::vector<SplatRecord> recs((size_t)npaths * per);
+  for (auto& r : recs) r.texel = npix;
+  std::vector<uint32_t> recCount(npaths, 0);
+  parallel_for(hc, nt, [&](uint32_t h, unsigned ti) {
+    uint32_t p = headers[h];
+    CpuTracer tr{&c.geo, wk(ti)};
+    splat_light_path(c.sv, fc, point, h, hc, lambdas[p], &nodes[(size_t)p * BDPT_MAX_LIGHT_VERTICES], counts[p], tr,
+                     &recs[(size_t)h * per]);
+    recCount[h] = counts[p] > 0 ? counts[p] - 1 : 0;
+  });
+  std::vector<float> acc((size_t)npix * 3, 0.f);
+  std::vector<uint8_t> has(npix, 0);
+  for (uint32_t h = 0; h < hc; ++h)
+    for (uint32_t j = 0; j < recCount[h]; ++j) {
+      const SplatRecord& r = recs[(size_t)h * per + j];
+      if (r.texel >= npix) continue;
+      acc[3 * r.texel] += r.xyz.x; acc[3 * r.texel + 1] += r.xyz.y; acc[3 * r.texel + 2] += r.xyz.z;
+      has[r.texel] = 1;
+    }
+  for (uint32_t p = 0; p < npix; ++p)
+    if (has[p] && (acc[3 * p] != 0.f || acc[3 * p + 1] != 0.f || acc[3 * p + 2] != 0.f))
+      for (int k = 0; k < 3; ++k) xyz[3 * p + k] = xyz[3 * p + k] + acc[3 * p + k];
+  return traced;
+}
+
 int setup(Ctx& c, const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integrator, int useBvh) {
-  if (integrator != YPT_INTEGRATOR_NEE && integrator != YPT_INTEGRATOR_BPT) { g_err = "oracle: integrator not supported"; return YPT_E_INVALID; }
+  if (integrator != YPT_INTEGRATOR_NEE && integrator != YPT_INTEGRATOR_BPT && integrator != YPT_INTEGRATOR_BDPT) {
+    g_err = "oracle: integrator not supported";
+    return YPT_E_INVALID;
+  }
   prepare_scene(*sd, tables().r2s, c.scene);
   prepare_frame(*fd, c.scene, (uint32_t)integrator, c.fc);
   if (fd->env_intensity_scale != 0.f && !(fd->flags & YPT_FLAG_WHITE_FURNACE) && tables().rgb2specFull.empty())
@@ -392,12 +513,23 @@ int oracle_render(const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integr
     uint32_t tilesX = (W + TILE - 1) / TILE, tilesY = (H + TILE - 1) / TILE;
     std::vector<std::array<uint64_t, 4>> work(nt);
     for (auto& w : work) w = {0, 0, 0, 0};
+    std::vector<uint32_t> perm;
+    size_t bdptTraced = 0;
+    if (integrator == YPT_INTEGRATOR_BDPT) {
+      perm.resize(npix);
+      texel_permutation(W, H, 0x59415054ull, perm.data());
+    }
     auto t0 = std::chrono::steady_clock::now();
     for (uint32_t k = 0; k < sample_count; ++k) {
       uint32_t point[256];
       sobol_point(sample_begin + k, point);
       if (features)
         for (auto& f : fh) std::memset(&f, 0, sizeof(f));
+      if (integrator == YPT_INTEGRATOR_BDPT) {
+        bdptTraced += bdpt_pass(c, point, perm, nt, stride, xyz, lam, features ? &fh : nullptr, stats ? &work : nullptr);
+        merge_pass(F, xyz, *fd);
+        continue;
+      }
       std::atomic<uint32_t> next{0};
       std::vector<std::thread> th;
       for (unsigned ti = 0; ti < nt; ++ti)
@@ -444,7 +576,7 @@ int oracle_render(const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integr
       uint64_t tot[4] = {0, 0, 0, 0};
       for (auto& w : work) for (int k = 0; k < 4; ++k) tot[k] += w[k];
       stats[0] = std::chrono::duration<double>(t1 - t0).count();
-      stats[1] = (double)traced * sample_count;
+      stats[1] = integrator == YPT_INTEGRATOR_BDPT ? (double)bdptTraced : (double)traced * sample_count;
       stats[2] = (double)tot[2];
       stats[3] = (double)tot[3];
       stats[4] = (double)tot[0];
@@ -468,7 +600,8 @@ int oracle_render(const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integr
 // 10 sample_cosine_hemisphere(u1, u2) 11 sigmoid(x) 12 sigmoid_inv(x) 13 hash_pair(a, b) (uint bits)
 // 14 owen_scramble_fast(v, seed) (uint bits)  15 uint_to_float01(v) (uint bits in)
 // 16 ss_dir(cos, r)  17 ss_avg(r)  18 ssms_dir(eta, cos, r)  19 sheen_dir(cos, r)  20 tr_dir(eta, cos, r)
-// 21 emission_profile(cos, n)
+// 21 emission_profile(cos, n)  22 f32_to_f16(x) (uint bits out)  23 f16_to_f32(h bits)
+// 24 accumulate_ri(sum, nom, denom, connectable)  25 pdf_ratio(nom, denom)
 extern "C" int oracle_eval(int fn, const float* in, uint32_t n, float* out) {
   MSLuts L;
   L.base = tables().msl.data();
@@ -499,6 +632,10 @@ extern "C" int oracle_eval(int fn, const float* in, uint32_t n, float* out) {
       case 19: o[0] = L.sheen_dir(a[0], a[1]); break;
       case 20: o[0] = directional_albedo_translucent(L, a[0], a[1], a[2]); break;
       case 21: o[0] = emission_profile(a[0], a[1]); break;
+      case 22: o[0] = as_float(f32_to_f16(a[0])); break;
+      case 23: o[0] = f16_to_f32(as_uint(a[0])); break;
+      case 24: o[0] = accumulate_ri(a[0], a[1], a[2], a[3] != 0.f); break;
+      case 25: o[0] = pdf_ratio(a[0], a[1]); break;
       default: g_err = "unknown function"; return YPT_E_INVALID;
     }
   }

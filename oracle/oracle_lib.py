@@ -104,7 +104,8 @@ def rgb_coeffs(rgb):
 FN = {"d_ggx": 0, "pdf_wm_ggx": 1, "sample_wm_ggx": 2, "fresnel_dielectric": 3, "fresnel_conductor": 4, "sin": 5,
       "cos": 6, "exp": 7, "log": 8, "pow": 9, "sample_cosine_hemisphere": 10, "sigmoid": 11, "sigmoid_inv": 12,
       "hash_pair": 13, "owen_scramble": 14, "uint_to_float01": 15, "ss_dir": 16, "ss_avg": 17, "ssms_dir": 18,
-      "sheen_dir": 19, "tr_dir": 20, "emission_profile": 21}
+      "sheen_dir": 19, "tr_dir": 20, "emission_profile": 21, "f32_to_f16": 22, "f16_to_f32": 23, "accumulate_ri": 24,
+      "pdf_ratio": 25}
 
 
 def evaluate(name, args):

@@ -61,7 +61,7 @@ class FrameDesc(ctypes.Structure):
         ("forward_z_sign", ctypes.c_float), ("env_intensity_scale", ctypes.c_float),
         ("max_depth", ctypes.c_uint32), ("regularize_after", ctypes.c_uint32), ("flags", ctypes.c_uint32),
         ("firefly_clamp_scale", ctypes.c_float), ("firefly_clamp_relax_power", ctypes.c_float),
-        ("kill_nan_samples", ctypes.c_uint32),
+        ("kill_nan_samples", ctypes.c_uint32), ("light_path_vertices", ctypes.c_uint32),
     ]
 
 

@@ -65,7 +65,7 @@ class Camera:
 
 
 def make_frame(camera, width, height, max_depth=8, env_intensity_scale=0.0, regularize_after=8, flags=0,
-               firefly_clamp_scale=12.0, firefly_clamp_relax_power=0.5, kill_nan_samples=1):
+               firefly_clamp_scale=12.0, firefly_clamp_relax_power=0.5, kill_nan_samples=1, light_path_vertices=16):
     """ypt_frame_desc for a camera; defaults are the reference's renderer variables (RendererVarsList.h:21-42)."""
     m = camera.matrices(width, height)
     f = _lib.FrameDesc()
@@ -82,4 +82,5 @@ def make_frame(camera, width, height, max_depth=8, env_intensity_scale=0.0, regu
     f.firefly_clamp_scale = firefly_clamp_scale
     f.firefly_clamp_relax_power = firefly_clamp_relax_power
     f.kill_nan_samples = kill_nan_samples
+    f.light_path_vertices = light_path_vertices
     return f

@@ -1,0 +1,830 @@
+// Bidirectional path tracing with all-strategy MIS: light subpaths, camera subpaths connected to
+// every vertex of one light subpath, and light tracing onto the lens (s >= 2, t = 1).
+//
+// Restates shaders/raytracing/BDPTCommon.hlsl:26-1223 (node layout, BDPTRayState,
+// evaluateSurfaceAndPDFs, calculateCameraDirectionalPDF, calculatePDFRatio, accumulateRISum,
+// calculateConnectingLightNodeWeightsAndPDFs, calculateMISWeight*, calculateLightPathRadiance),
+// BDPTTraceLightPath.comp:6-362, BDPTTraceCameraPath.comp:40-432 and
+// BDPTCalculatePureLightPaths.comp:8-206 for mesh lights. Energy compensation is the reciprocal
+// (Kulla-Conty) form (RECIPROCAL_MULTISCATTER).
+//
+// The tracing functions are templates over a Tracer with
+//   bool closest(vec3 o, vec3 d, float tmin, float tmax, Hit& h)         closest (t, id) in [tmin, tmax]
+//   bool hits_target(vec3 o, vec3 d, float tmin, float tmax, uint32_t tri) the closest hit is `tri`
+//   bool unoccluded(vec3 o, vec3 d, float tmin, float tmax)              nothing is hit in [tmin, tmax]
+// so that the GPU kernels and the CPU checker run the same vertex arithmetic over their own BVHs.
+//
+// Environment light subpaths (environmentSelectionPDF = 1 when a scene has no lights) and punctual
+// lights are not part of this build: light subpaths start on mesh lights only, and camera subpaths
+// that escape are weighted as the only strategy that finds the environment.
+#pragma once
+#include "shade.h"
+
+namespace ypt {
+
+static constexpr uint32_t BDPT_MAX_LIGHT_VERTICES = 16;  // MAX_LIGHT_PATH_VERTICES_HARD_LIMIT
+static constexpr uint32_t BDPT_TILE = 8;                  // THREAD_GROUP_DIM_SIZE_CAM_RAY = TEXELS_PER_LIGHTPATH
+static constexpr float BDPT_MAX_PDF_RATIO_SUM = 1e16f;
+static constexpr float BDPT_VISIBILITY_EPS = 0.001f;
+
+// LIGHT_PATH_NODE_FLAG_* (BDPTCommon.hlsl:14-28)
+static constexpr uint32_t LN_SECONDARY_TERMINATED = 1u << 0;
+static constexpr uint32_t LN_FRONT_FACE = 1u << 1;
+static constexpr uint32_t LN_DELTA = 1u << 3;
+static constexpr uint32_t LN_REGULARIZED = 1u << 4;
+
+// ---------------------------------------------------------------- half precision of the packed normal
+// f32tof16 / f16tof32 with round-to-nearest-even; the packed node keeps its normal at this precision.
+YPT_HD uint32_t f32_to_f16(float f) {
+  uint32_t x = as_uint(f);
+  uint32_t sign = (x >> 16) & 0x8000u;
+  uint32_t e = (x >> 23) & 0xFFu;
+  uint32_t m = x & 0x7FFFFFu;
+  if (e == 0xFFu) return sign | 0x7C00u | (m ? 0x200u : 0u);
+  int ex = (int)e - 127 + 15;
+  if (ex >= 31) return sign | 0x7C00u;
+  if (ex <= 0) {
+    if (ex < -10) return sign;
+    m |= 0x800000u;
+    uint32_t shift = (uint32_t)(14 - ex);
+    uint32_t hm = m >> shift;
+    uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (hm & 1u))) hm++;
+    return sign | hm;
+  }
+  uint32_t h = sign | ((uint32_t)ex << 10) | (m >> 13);
+  uint32_t rem = m & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+  return h;
+}
+YPT_HD float f16_to_f32(uint32_t h) {
+  uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+  if (e == 0) {
+    float v = (float)m * 5.9604644775390625e-8f;  // m * 2^-24, exact
+    return sign ? -v : v;
+  }
+  if (e == 31) return as_float(sign | 0x7F800000u | (m << 13));
+  return as_float(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+// ---------------------------------------------------------------- light subpath nodes
+// LightPathNodePacked (BDPTCommon.hlsl:37-44, 300-335): 64 bytes here (no next-index link: the
+// vertices of a subpath are stored in order, light first).
+struct LightNodePacked {
+  float radiance[4];
+  float pos[3];
+  uint32_t tri;        // global triangle id (instance + primitive)
+  float b1, b2;        // barycentrics; (-1, -1) for the light origin
+  float pdfFwd, pdfBwd;
+  uint32_t nxy;        // f16 normal x | y << 16
+  uint32_t nzFlags;    // f16 normal z | flags << 16
+  float iorPrev, iorCur;
+};
+
+struct LightNode {     // LightPathNode (:46-60)
+  Spectrum radiance;
+  vec3 pos, nrm;
+  uint32_t tri;
+  float b1, b2, pdfFwd, pdfBwd, iorPrev, iorCur;
+  uint32_t flags;
+};
+
+YPT_HD LightNodePacked pack_node(const LightNode& n) {
+  LightNodePacked p;
+  for (int i = 0; i < 4; ++i) p.radiance[i] = n.radiance.s[i];
+  p.pos[0] = n.pos.x; p.pos[1] = n.pos.y; p.pos[2] = n.pos.z;
+  p.tri = n.tri;
+  p.b1 = n.b1; p.b2 = n.b2;
+  p.pdfFwd = n.pdfFwd; p.pdfBwd = n.pdfBwd;
+  p.nxy = f32_to_f16(n.nrm.x) | (f32_to_f16(n.nrm.y) << 16);
+  p.nzFlags = f32_to_f16(n.nrm.z) | (n.flags << 16);
+  p.iorPrev = n.iorPrev; p.iorCur = n.iorCur;
+  return p;
+}
+YPT_HD LightNode unpack_node(const LightNodePacked& p) {
+  LightNode n;
+  n.radiance = Spectrum(p.radiance[0], p.radiance[1], p.radiance[2], p.radiance[3]);
+  n.pos = vec3(p.pos[0], p.pos[1], p.pos[2]);
+  n.nrm = vec3(f16_to_f32(p.nxy & 0xFFFFu), f16_to_f32(p.nxy >> 16), f16_to_f32(p.nzFlags & 0xFFFFu));
+  n.flags = p.nzFlags >> 16;
+  n.tri = p.tri;
+  n.b1 = p.b1; n.b2 = p.b2;
+  n.pdfFwd = p.pdfFwd; n.pdfBwd = p.pdfBwd;
+  n.iorPrev = p.iorPrev; n.iorCur = p.iorCur;
+  return n;
+}
+
+// ---------------------------------------------------------------- BDPTRayState (:74-186)
+struct BdptState {
+  float ior[MAX_VOLUMES];
+  Spectrum absorption[MAX_VOLUMES];
+  uint32_t mat[MAX_VOLUMES];
+  uint32_t n;
+  uint32_t flags;  // RS_SECONDARY_TERMINATED | RS_REGULARIZE
+  YPT_HD void init() {
+    n = 0;
+    flags = 0;
+    for (int i = 0; i < MAX_VOLUMES; ++i) { ior[i] = 1.f; absorption[i].set(0.f); mat[i] = VOLUME_NONE; }
+  }
+  YPT_HD Spectrum get_absorption() const { return n != 0 ? absorption[n - 1] : Spectrum(0.f); }
+  YPT_HD float current_ior() const { return n != 0 ? ior[n - 1] : 1.0f; }
+  YPT_HD float previous_ior() const { return n > 1 ? ior[n - 2] : 1.0f; }
+  YPT_HD void entered(float i, const Spectrum& a, uint32_t m) {
+    n = n + 1 < (uint32_t)MAX_VOLUMES ? n + 1 : (uint32_t)MAX_VOLUMES;
+    ior[n - 1] = i;
+    absorption[n - 1] = a;
+    mat[n - 1] = m;
+  }
+  YPT_HD void exited(uint32_t m) {
+    if (n == 0) return;
+    int index = -1;
+    for (int i = (int)n - 1; i >= 0; --i)
+      if (mat[i] == m) { index = i; break; }
+    if (index < 0) index = (int)n - 1;
+    for (uint32_t i = (uint32_t)index; i + 1 < n; ++i) {
+      ior[i] = ior[i + 1];
+      absorption[i] = absorption[i + 1];
+      mat[i] = mat[i + 1];
+    }
+    --n;
+  }
+};
+
+// ---------------------------------------------------------------- surfaces at path vertices
+struct TriGeometry {
+  vec3 p1, p2, p3;
+  vec3 geometryNormal, normal, tangent;
+};
+
+// positions, geometry normal, interpolated shading normal and tangent (fillSurfaceDefRGB's fetches)
+YPT_HD void fetch_triangle(const SceneView& S, uint32_t tri, float b1, float b2, TriGeometry& g) {
+  float b0 = 1.f - b1 - b2;
+  uint32_t i0 = S.idx[3 * tri], i1 = S.idx[3 * tri + 1], i2 = S.idx[3 * tri + 2];
+  g.p1 = load3(S.pos, i0); g.p2 = load3(S.pos, i1); g.p3 = load3(S.pos, i2);
+  g.geometryNormal = normalize(cross(g.p2 - g.p1, g.p3 - g.p1));
+  g.normal = normalize(b0 * load3(S.nrm, i0) + b1 * load3(S.nrm, i1) + b2 * load3(S.nrm, i2));
+  g.tangent = vec3(1.f, 0.f, 0.f);
+  if (S.tan) {
+    const float* t0 = S.tan + 4 * i0;
+    const float* t1 = S.tan + 4 * i1;
+    const float* t2 = S.tan + 4 * i2;
+    vec3 a(t0[0] * t0[3], t0[1] * t0[3], t0[2] * t0[3]);
+    vec3 b(t1[0] * t1[3], t1[1] * t1[3], t1[2] * t1[3]);
+    vec3 c(t2[0] * t2[3], t2[1] * t2[3], t2[2] * t2[3]);
+    g.tangent = normalize(b0 * a + b1 * b + b2 * c);
+  }
+}
+
+struct BdptVertex {
+  SurfaceContext ctx;
+  vec3 pos, wo, normalWS;
+  uint32_t tri, matId;
+  float b1, b2, t;
+  bool frontFace;
+};
+
+// makePathHit + calculateCommonSurfaceParams (BDPTCommon.hlsl:490-505, 637-665): the surface hit by
+// a ray, two-sided orientation, regularisation, medium entry of rays that start inside, and the
+// sampling context (reciprocal energy compensation).
+YPT_HD void bdpt_surface(const SceneView& S, BdptState& rs, const Hit& h, vec3 rayOrigin, vec3 rayDir, const Lambdas& L,
+                         BdptVertex& v, const MaterialGPU** matOut) {
+  TriGeometry g;
+  fetch_triangle(S, h.tri, h.b1, h.b2, g);
+  uint32_t matId = S.inst[S.triInst[h.tri]].material;
+  const MaterialGPU& m = S.mats[matId];
+  v.tri = h.tri;
+  v.matId = matId;
+  v.b1 = h.b1;
+  v.b2 = h.b2;
+  v.t = h.t;
+  v.pos = rayOrigin + rayDir * h.t;
+  v.wo = normalize(-rayDir);
+  v.frontFace = dot(rayDir, cross(g.p2 - g.p1, g.p3 - g.p1)) < 0.f;
+  bool reg = (rs.flags & RS_REGULARIZE) != 0;
+  SurfaceDef sd = make_surface(m, g.geometryNormal, g.normal, g.tangent, v.wo, true, reg, L, S.spec);
+  // handleRayStartInsideMedium (hitShadersCommon.hlsl:213-222)
+  if (dot(-v.wo, sd.geometryNormal) > 0.f && rs.n == 0 && !surface_two_sided(sd.flags))
+    rs.entered(sd.dielectricIOR, sd.absorption, matId);
+  make_surface_context(v.ctx, sd, rs.current_ior(), rs.previous_ior(), v.wo, v.frontFace, L, true);
+  v.normalWS = normalize(sd.geometryNormal);
+  if (matOut) *matOut = &m;
+}
+
+YPT_HD Spectrum transmittance(float t, const Spectrum& a) {
+  Spectrum tr;
+  for (int i = 0; i < 4; ++i) tr.s[i] = yexp(-a.s[i] * t);
+  return tr;
+}
+
+struct ScatterEval {
+  Spectrum weight;
+  float pdfFwd, pdfBwd;
+  bool dispersed;
+};
+
+// evaluateSurfaceAndPDFs (BDPTCommon.hlsl:386-442): forward weight and pdf, the pdf of the reversed
+// pair (the surface data recomputed for wi, IORs swapped on transmitted pairs), then the volume
+// and dispersion bookkeeping of the event.
+YPT_HD ScatterEval eval_and_pdfs(const SceneView& S, const SurfaceContext& ctx, BdptState& rs, uint32_t matId, vec3 wo,
+                                 vec3 wi, bool frontFace, bool fromLight, bool treatAsDelta) {
+  ScatterEval r;
+  r.dispersed = false;
+  uint32_t flags = EVAL_NONE;
+  if (fromLight) flags |= EVAL_LIGHT_PATH;
+  if (treatAsDelta) flags |= EVAL_TREAT_AS_DELTA;
+  SurfaceEval fwd = evaluate_surface(ctx, S.luts, wo, wi, flags);
+  r.weight = fwd.weight;
+  r.pdfFwd = fwd.pdf;
+  {
+    const SurfaceDef& sd = ctx.def;
+    bool transmittedPair = dot(wo, sd.geometryNormal) * dot(wi, sd.geometryNormal) < 0.f;
+    bool frontRev = transmittedPair ? !frontFace : frontFace;
+    SurfaceContext rev;
+    make_surface_context(rev, sd, rs.current_ior(), rs.previous_ior(), wi, frontRev, ctx.lambdas, true);
+    if (transmittedPair) {
+      rev.pre.fromIOR = ctx.pre.toIOR;
+      rev.pre.toIOR = ctx.pre.fromIOR;
+    }
+    r.pdfBwd = evaluate_surface(rev, S.luts, wi, wo, EVAL_PDF_ONLY).pdf;
+  }
+  if (fwd.transmission != TRANSMISSION_NONE) {
+    if ((fwd.transmission & TRANSMISSION_DISPERSED) != 0) {
+      r.dispersed = (rs.flags & RS_SECONDARY_TERMINATED) == 0;
+      rs.flags |= RS_SECONDARY_TERMINATED;
+    }
+    if ((fwd.transmission & TRANSMISSION_EXITED) != 0) rs.exited(matId);
+    else if ((fwd.transmission & TRANSMISSION_ENTERED) != 0) rs.entered(ctx.def.dielectricIOR, ctx.def.absorption, matId);
+  }
+  return r;
+}
+
+// sampleScatteringEvent (scatteringSample.hlsl:30-63) without subsurface: a zero direction is
+// SCATTERING_EVENT_NONE.
+YPT_HD vec3 sample_scattering(const SceneView& S, const SurfaceContext& ctx, const float r[4], bool& isDiffuse, bool& isDelta) {
+  int layer = select_layer(r[0], ctx.prob);
+  isDiffuse = true;
+  isDelta = false;
+  if (layer == LAYER_SUBSURFACE) return vec3(0.f);
+  return sample_layer_direction(ctx, S.luts, layer, r[1], r[2], r[3], isDiffuse, isDelta);
+}
+
+// ---------------------------------------------------------------- MIS bookkeeping
+YPT_HD float pdf_ratio(float nom, float denom) {  // calculatePDFRatio (:548-553)
+  nom = nom < 0.f ? 1.f : nom;
+  denom = denom < 0.f ? 1.f : denom;
+  return fmin_(safe_div(nom, denom), BDPT_MAX_PDF_RATIO_SUM);
+}
+YPT_HD float accumulate_ri(float riSum, float nom, float denom, bool connectable = true) {  // accumulateRISum (:555-566)
+  float ri = pdf_ratio(nom, denom);
+  bool acc = connectable && nom > 0.f && denom > 0.f;
+  riSum = ri * riSum;
+  if (acc) riSum += ri;
+  return fmin_(riSum, BDPT_MAX_PDF_RATIO_SUM);
+}
+
+// calculateCameraDirectionalPDF (:527-542): directional density of the camera toward a point, with
+// one cosine left out (it cancels against the camera's geometry term).
+YPT_HD float camera_dir_pdf(const FrameConstants& fc, vec3 camToNodeWS) {
+  float a[4] = {0.f, 0.f, 0.f, 1.f}, b[4] = {1.f, 1.f, 0.f, 1.f}, mn[4], mx[4];
+  mul4(fc.uvToView, a, mn);
+  mul4(fc.uvToView, b, mx);
+  float wn = mn[3], wx = mx[3];
+  for (int i = 0; i < 4; ++i) { mn[i] = mn[i] / wn; mx[i] = mx[i] / wx; }
+  float ex = fabsf(mx[0] - mn[0]), ey = fabsf(mx[1] - mn[1]);
+  float planeArea = safe_div(ex * ey, mn[2] * mn[2]);
+  vec3 vs = xform_dir(fc.worldToView, camToNodeWS);
+  float cosTheta = fabsf(vs.z) * (1.f / sqrtf(fmax_(0.0001f, dot(vs, vs))));
+  return safe_div(fc.targetTexDimensions[0] * fc.targetTexDimensions[1], planeArea * cosTheta * cosTheta * cosTheta);
+}
+
+// pdfForSamplingLight (lightSampling.hlsl:191-228) for a point on a mesh-light triangle
+YPT_HD void pdf_sampling_light(const SceneView& S, uint32_t tri, vec3 lightNormal, vec3 towards, float& pick, float& posPdf,
+                               float& dirPdf) {
+  const InstanceGPU& in = S.inst[S.triInst[tri]];
+  const MaterialGPU& m = S.mats[in.material];
+  pick = (1.f - 0.f) * in.lightSelectionPdf;  // (1 - environmentSelectionPDF) with mesh lights present
+  bool twoSided = surface_two_sided(m.flags);
+  vec3 p1, p2, p3;
+  tri_vertices(S, tri, p1, p2, p3);
+  float area = length(cross(p2 - p1, p3 - p1)) * 0.5f;
+  posPdf = 1.f / ((float)in.triCount * area);
+  float cosTheta = dot(lightNormal, towards);
+  if (twoSided) dirPdf = pdf_cosine_power_sphere(cosTheta, m.emissionFocus);
+  else dirPdf = cosTheta >= 0.f ? pdf_cosine_power_hemisphere(cosTheta, m.emissionFocus) : 0.f;
+}
+
+// pdfEnvironmentPosition (lightSampling.hlsl:73-76): planar density of the disk behind the world bounds
+YPT_HD float pdf_environment_position(const FrameConstants& fc) {
+  vec3 lo(fc.worldBoundsMin[0], fc.worldBoundsMin[1], fc.worldBoundsMin[2]);
+  vec3 hi(fc.worldBoundsMax[0], fc.worldBoundsMax[1], fc.worldBoundsMax[2]);
+  vec3 c = (hi + lo) * 0.5f;
+  vec3 ext = hi - c;
+  return safe_div(1.f, PI * dot(ext, ext));
+}
+
+// ---------------------------------------------------------------- light subpath
+YPT_HD uint32_t camera_dim_offset(const FrameConstants& fc) { return fc.lightPathVertices * 5u + 12u; }      // :226-227
+YPT_HD uint32_t pure_lens_dim_offset(const FrameConstants& fc) { return fc.lightPathVertices * 5u + 9u; }    // :229
+
+// traceLightRays (BDPTTraceLightPath.comp:6-362) for light subpath `pathIndex`: writes the vertices,
+// light first, to out[0 .. count) and returns count (0 when the light sample failed).
+template <class Tracer>
+YPT_HD uint32_t trace_light_path(const SceneView& S, const FrameConstants& fc, const uint32_t* point, uint32_t pathIndex,
+                                 Tracer& tr, LightNodePacked* out, float& lambdaOut) {
+  RandomSampler rand;
+  rand.init(point, 0u, pathIndex);
+  float lambda = hero_wavelength(rand.r1());
+  lambdaOut = lambda;
+  Lambdas L = spectral_lambdas(lambda);
+  float r0[4], r1[4];
+  rand.r4(r0);
+  rand.r4(r1);
+  if (fc.lightCount == 0) return 0;
+
+  // sampleLightOrEnv + sampleLight (lightSampling.hlsl:122-189)
+  float pick;
+  uint32_t li = sample_light_index(S.lights, fc.lightCount, r1[3], pick);
+  const LightGPU& light = S.lights[li];
+  uint32_t primCount = light.triCount;
+  uint32_t prim = (uint32_t)fmin_((float)primCount * r0[2], (float)(primCount - 1u));
+  float bx = r0[0], by = r0[1];
+  if (bx + by > 1.f) { bx = 1.f - bx; by = 1.f - by; }
+  uint32_t tri = light.firstTri + prim;
+  TriGeometry g;
+  fetch_triangle(S, tri, bx, by, g);
+  vec3 gn = cross(g.p2 - g.p1, g.p3 - g.p1);
+  float gl = length(gn);
+  gn = gn / gl;
+  float area = gl * 0.5f;
+  float b0 = 1.f - bx - by;
+  vec3 lightPos = b0 * g.p1 + bx * g.p2 + by * g.p3;
+  vec3 tangent = normalize(g.tangent);
+  const MaterialGPU& lm = S.mats[light.material];
+  bool twoSided = surface_two_sided(lm.flags);
+  float focus = lm.emissionFocus;
+  vec3 local = twoSided ? sample_cosine_power_sphere(r1[0], r1[1], focus) : sample_cosine_power_hemisphere(r1[0], r1[1], focus);
+  mat3 tanToWS = basis_transform(gn, tangent);
+  Spectrum radiance = spectrum_unbounded(lm.emissive, L, S.spec);
+  float pdfPos = 1.f / ((float)primCount * area);
+  float pdfDir = twoSided ? pdf_cosine_power_sphere(local.y, focus) : pdf_cosine_power_hemisphere(local.y, focus);
+  float profile = emission_profile(local.y, focus);
+  vec3 lightDir = mul(local, tanToWS);
+  if (radiance.all_equal(0.f) || pdfPos == 0.f || pdfDir == 0.f || pick == 0.f) return 0;
+  float maxRadiance = radiance.max_value();
+
+  LightNode nodes[2];
+  nodes[0].radiance = radiance;
+  nodes[0].pos = lightPos;
+  nodes[0].nrm = gn;
+  nodes[0].tri = tri;
+  nodes[0].b1 = -1.f;
+  nodes[0].b2 = -1.f;
+  nodes[0].pdfFwd = pdfPos * pick;
+  nodes[0].pdfBwd = 0.f;
+  nodes[0].iorPrev = 1.f;
+  nodes[0].iorCur = 1.f;
+  nodes[0].flags = 0;
+  nodes[1] = nodes[0];
+  nodes[1].pdfBwd = 0.f;
+  float pdfForward = pdfDir;
+  vec3 rayDir = lightDir;
+  vec3 rayOrigin = spawn_ray_origin(lightPos, gn, lightDir);
+  radiance = radiance * profile * fabsf(dot(gn, lightDir)) / (pick * pdfPos * pdfDir);
+
+  uint32_t stored = 0, nodeCount = 1;
+  bool previousWasDelta = false;
+  BdptState rs;
+  rs.init();
+  for (uint32_t i = 1; i < fc.lightPathVertices; ++i) {
+    float rs0[4];
+    rand.r4(rs0);
+    Hit h;
+    if (!tr.closest(rayOrigin, rayDir, RAY_MIN_T, __builtin_inff(), h)) break;
+    uint32_t ci = i & 1u, pi = (i - 1u) & 1u;
+    if (i - 1u >= fc.regularizeAfterVertices) rs.flags |= RS_REGULARIZE;
+    uint32_t flags = 0;
+    BdptVertex v;
+    bdpt_surface(S, rs, h, rayOrigin, rayDir, L, v, nullptr);
+    if (v.frontFace) flags |= LN_FRONT_FACE;
+    if (rs.flags & RS_SECONDARY_TERMINATED) flags |= LN_SECONDARY_TERMINATED;
+    if (rs.flags & RS_REGULARIZE) flags |= LN_REGULARIZED;
+    if (is_delta_distribution(v.ctx.pre.ax, v.ctx.pre.ay, v.ctx.def)) flags |= LN_DELTA;
+    float curFwd = previousWasDelta ? -1.f : pdfForward * solid_angle_to_area(v.pos - rayOrigin, v.normalWS);
+    Spectrum ab = rs.get_absorption();
+    if (!ab.all_equal(0.f)) radiance = radiance * transmittance(h.t, ab);
+    nodes[ci].radiance = radiance;
+    nodes[ci].pos = v.pos;
+    nodes[ci].nrm = v.normalWS;
+    nodes[ci].tri = h.tri;
+    nodes[ci].b1 = h.b1;
+    nodes[ci].b2 = h.b2;
+    nodes[ci].pdfFwd = curFwd;
+    nodes[ci].iorPrev = rs.previous_ior();
+    nodes[ci].iorCur = rs.current_ior();
+    nodes[ci].flags = flags;
+
+    bool isDiffuse, isDelta;
+    vec3 wi = sample_scattering(S, v.ctx, rs0, isDiffuse, isDelta);
+    if (is_zero(wi)) break;
+    if (isDiffuse) rs.flags |= RS_REGULARIZE;
+    ScatterEval ev = eval_and_pdfs(S, v.ctx, rs, v.matId, v.wo, wi, v.frontFace, true, isDelta);
+    pdfForward = ev.pdfFwd;
+    Spectrum sampleWeight = ev.weight * safe_div(1.f, pdfForward);
+    nodes[pi].pdfBwd = isDelta ? -1.f : ev.pdfBwd * solid_angle_to_area(rayOrigin - v.pos, nodes[pi].nrm);
+    out[stored++] = pack_node(nodes[pi]);
+    ++nodeCount;
+    radiance = radiance * sampleWeight;
+    if (ev.dispersed) radiance.terminate_secondary();
+    if (radiance.russian_roulette(rand.r1(), maxRadiance)) break;
+    if (radiance.all_equal(0.f)) break;
+    rayDir = normalize(wi);
+    rayOrigin = spawn_ray_origin(v.pos, v.normalWS, rayDir);
+    previousWasDelta = isDelta;
+  }
+  out[stored++] = pack_node(nodes[(nodeCount - 1u) & 1u]);
+  return stored;
+}
+
+// ---------------------------------------------------------------- connections
+// calculateConnectingLightNodeWeightsAndPDFs (:701-875) for a mesh-light origin (index 0) or a
+// surface vertex; false when the connection carries nothing.
+YPT_HD bool connect_light_node(const SceneView& S, const LightNode& n, uint32_t index, vec3 prevPos, vec3 nextPos,
+                               bool secondaryTerminated, const Lambdas& L, ScatterEval& ev, float& pdfFwdMIS) {
+  ev.weight.set(0.f);
+  ev.pdfFwd = 0.f;
+  ev.pdfBwd = 0.f;
+  ev.dispersed = false;
+  const MaterialGPU& m = S.mats[S.inst[S.triInst[n.tri]].material];
+  bool frontFace = (n.flags & LN_FRONT_FACE) != 0;
+  if (index == 0) {
+    bool twoSided = surface_two_sided(m.flags);
+    vec3 fromLight = nextPos - n.pos;
+    if (dot(fromLight, n.nrm) <= 0.f && !twoSided) { pdfFwdMIS = 0.f; return false; }
+    float pdf = n.pdfFwd;
+    ev.pdfFwd = pdf * area_to_solid_angle(-fromLight, n.nrm);
+    float cosEmit = dot(n.nrm, normalize(fromLight));
+    ev.weight.set(safe_div(emission_profile(cosEmit, m.emissionFocus), ev.pdfFwd));
+    pdfFwdMIS = twoSided ? pdf_cosine_power_sphere(cosEmit, m.emissionFocus)
+                         : (cosEmit >= 0.f ? pdf_cosine_power_hemisphere(cosEmit, m.emissionFocus) : 0.f);
+    return true;
+  }
+  vec3 wi = normalize(nextPos - n.pos);
+  vec3 wo = normalize(prevPos - n.pos);
+  TriGeometry g;
+  fetch_triangle(S, n.tri, n.b1, n.b2, g);
+  // no two-sided re-orientation here, as in the reference
+  SurfaceDef sd = make_surface(m, g.geometryNormal, g.normal, g.tangent, wo, false, (n.flags & LN_REGULARIZED) != 0, L, S.spec);
+  BdptState rs;
+  rs.init();
+  rs.ior[0] = n.iorPrev;
+  rs.ior[1] = n.iorCur;
+  rs.n = 2;
+  bool already = secondaryTerminated || (n.flags & LN_SECONDARY_TERMINATED) != 0;
+  rs.flags = already ? RS_SECONDARY_TERMINATED : 0u;
+  SurfaceContext ctx;
+  make_surface_context(ctx, sd, rs.current_ior(), rs.previous_ior(), wo, frontFace, L, true);
+  ev = eval_and_pdfs(S, ctx, rs, VOLUME_NONE, wo, wi, frontFace, true, false);
+  if (ev.dispersed) ev.weight.terminate_secondary();
+  pdfFwdMIS = ev.pdfFwd;
+  if (ev.pdfBwd == 0.f || ev.weight.all_equal(0.f)) return false;
+  return true;
+}
+
+struct CameraVertexCtx {  // CameraVertexContext (:877-896)
+  Spectrum throughput;
+  vec3 pos, normalWS, wo, wi;
+  vec3 prevPos, prevNrm;
+  float pdfRatioSum;
+  float prevFwdMIS, curFwdMIS;
+  uint32_t matId;
+  bool frontFace;
+};
+
+struct LightVertexCtx {  // LightVertexContext (:898-906)
+  vec3 prevPos, prevNrm;
+  float prevFwdMIS;
+  float riSum;
+  uint32_t prevFlags;
+};
+
+struct ConnectionPdfs {
+  float fwdCam, bwdCam, fwdLight, bwdLight;
+};
+
+// calculateMISWeight (:918-1007)
+YPT_HD float mis_connection(const SceneView& S, uint32_t camIndex, const CameraVertexCtx& cv, uint32_t lightIndex,
+                            const LightVertexCtx& lv, const LightNode& n, const ConnectionPdfs& p) {
+  float riCam = cv.pdfRatioSum;
+  if (camIndex > 0) {
+    float pb = p.bwdCam * solid_angle_to_area(cv.prevPos - cv.pos, cv.prevNrm);
+    riCam = accumulate_ri(riCam, pb, cv.prevFwdMIS);
+  }
+  riCam = accumulate_ri(riCam, p.fwdLight * solid_angle_to_area(cv.pos - n.pos, cv.normalWS), cv.curFwdMIS);
+  float riLight = lv.riSum;
+  if (lightIndex == 0) {
+    vec3 camToLight = n.pos - cv.pos;
+    float len = length(camToLight);
+    vec3 lightToCam = (-camToLight) / len;
+    float pick, posPdf, dirPdf;
+    pdf_sampling_light(S, n.tri, n.nrm, lightToCam, pick, posPdf, dirPdf);
+    float toLight = p.fwdCam * solid_angle_to_area(camToLight, n.nrm);
+    riLight = pdf_ratio(toLight, pick * posPdf);
+  } else {
+    float pbl = p.bwdLight * solid_angle_to_area(lv.prevPos - n.pos, lv.prevNrm);
+    riLight = accumulate_ri(riLight, pbl, lv.prevFwdMIS);
+    riLight = accumulate_ri(riLight, p.fwdCam * solid_angle_to_area(n.pos - cv.pos, n.nrm), n.pdfFwd);
+  }
+  return 1.f / (1.f + riCam + riLight);
+}
+
+// calculateMISWeightOnCameraPathHittingLight (:1009-1036)
+YPT_HD float mis_camera_hits_light(const SceneView& S, uint32_t camIndex, uint32_t tri, const CameraVertexCtx& cv) {
+  if (camIndex == 0) return 1.f;
+  vec3 towardsPrev = cv.prevPos - cv.pos;
+  vec3 towardsPrevDir = normalize(towardsPrev);
+  float pick, posPdf, dirPdf;
+  pdf_sampling_light(S, tri, cv.normalWS, towardsPrevDir, pick, posPdf, dirPdf);
+  float riCam = cv.pdfRatioSum;
+  riCam = accumulate_ri(riCam, dirPdf * solid_angle_to_area(towardsPrev, cv.prevNrm), cv.prevFwdMIS);
+  riCam = accumulate_ri(riCam, pick * posPdf, cv.curFwdMIS);
+  return 1.f / (1.f + riCam);
+}
+
+// calculateMISWeightOnCameraPathMissing (:1040-1058), environment selection probability 0
+YPT_HD float mis_camera_misses(const FrameConstants& fc, uint32_t camIndex, const CameraVertexCtx& cv, vec3 rayDir) {
+  if (camIndex == 0) return 1.f;
+  float riCam = cv.pdfRatioSum;
+  float pdfEnvToLast = pdf_environment_position(fc) * fabsf(dot(cv.prevNrm, rayDir));
+  riCam = accumulate_ri(riCam, pdfEnvToLast, cv.prevFwdMIS);
+  float pdfEnvDirection = 0.f;
+  riCam = accumulate_ri(riCam, pdfEnvDirection, cv.curFwdMIS);
+  return 1.f / (1.f + riCam);
+}
+
+// calculateLightPathRadiance (:1060-1185); rs is the camera subpath's state, taken by value.
+template <class Tracer>
+YPT_HD Spectrum connection_radiance(const SceneView& S, uint32_t camIndex, const CameraVertexCtx& cv,
+                                    const SurfaceContext& camCtx, const LightNode& n, uint32_t lightIndex,
+                                    const LightVertexCtx& lv, BdptState rs, const Lambdas& L, Tracer& tr) {
+  Spectrum zero(0.f);
+  bool camTerminated = (rs.flags & RS_SECONDARY_TERMINATED) != 0;
+  bool lightTerminated = (n.flags & LN_SECONDARY_TERMINATED) != 0;
+  if (lightTerminated) rs.flags |= RS_SECONDARY_TERMINATED;
+  ConnectionPdfs p;
+  ScatterEval ec = eval_and_pdfs(S, camCtx, rs, cv.matId, cv.wo, cv.wi, cv.frontFace, false, false);
+  if (ec.dispersed) ec.weight.terminate_secondary();
+  if (ec.pdfFwd == 0.f || ec.weight.all_equal(0.f)) return zero;
+  Spectrum wCam = ec.weight;
+  p.fwdCam = ec.pdfFwd;
+  p.bwdCam = ec.pdfBwd;
+  ScatterEval el;
+  if (!connect_light_node(S, n, lightIndex, lv.prevPos, cv.pos, (rs.flags & RS_SECONDARY_TERMINATED) != 0, L, el, p.fwdLight))
+    return zero;
+  Spectrum wLight = el.weight;
+  p.bwdLight = el.pdfBwd;
+  vec3 toNode = n.pos - cv.pos;
+  float len = length(toNode);
+  vec3 dir = toNode / len;
+  vec3 o = spawn_ray_origin(cv.pos, cv.normalWS, dir);
+  if (!tr.hits_target(o, dir, RAY_MIN_T, RAY_MIN_T + len + BDPT_VISIBILITY_EPS, n.tri)) return zero;
+  float mis = mis_connection(S, camIndex, cv, lightIndex, lv, n, p);
+  float geom = 1.f;
+  if (lightIndex > 0) geom = 1.f / (len * len);
+  Spectrum ab = rs.get_absorption();
+  if (!ab.all_equal(0.f)) wCam = wCam * transmittance(len, ab);
+  Spectrum r = wLight * wCam * cv.throughput * n.radiance * geom * mis;
+  if (camTerminated && lightTerminated) r.undo_secondary_rescale();
+  return r;
+}
+
+// ---------------------------------------------------------------- camera subpath
+// traceCameraRays (BDPTTraceCameraPath.comp:40-432) for texel (px, py), connecting every vertex to the
+// nodes[0 .. nodeCount) of the light subpath its workgroup picked. extraDim: the workgroup's first
+// thread drew the fallback wavelength from its own sampler when there were no light subpaths.
+template <class Tracer>
+YPT_HD Spectrum trace_camera_path(const SceneView& S, const FrameConstants& fc, const uint32_t* point, uint32_t px, uint32_t py,
+                                  float lambda, const LightNodePacked* nodes, uint32_t nodeCount, bool extraDim, Tracer& tr,
+                                  FirstHitFeatures* fh) {
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0];
+  RandomSampler rand;
+  rand.init(point, camera_dim_offset(fc), py * W + px);
+  float lens[2];
+  rand.r2(lens);
+  if (extraDim) rand.dim += 1u;
+  Lambdas L = spectral_lambdas(lambda);
+  float jx, jy;
+  rand.texel_offset(jx, jy);
+  vec3 rayOrigin, rayDir;
+  camera_ray(fc, px, py, jx, jy, lens[0], lens[1], rayOrigin, rayDir);
+  BdptState rs;
+  rs.init();
+  CameraVertexCtx cv;
+  cv.throughput.set(1.f);
+  cv.pdfRatioSum = 0.f;
+  cv.prevFwdMIS = 0.f;
+  cv.curFwdMIS = 0.f;
+  cv.prevPos = vec3(0.f);
+  cv.prevNrm = vec3(0.f);
+  float pdfForwardCamera = camera_dir_pdf(fc, rayDir);
+  float deferredCam = 0.f;
+  Spectrum total(0.f);
+  for (uint32_t k = 0; k < fc.maxRayDepth; ++k) {
+    Hit h;
+    bool hit = tr.closest(rayOrigin, rayDir, RAY_MIN_T, __builtin_inff(), h);
+    float rs0[4];
+    rand.r4(rs0);
+    if (!hit) {
+      cv.curFwdMIS = pdfForwardCamera;
+      float w = mis_camera_misses(fc, k, cv, rayDir);
+      vec3 sky = (fc.flags & FRAME_FLAG_WHITE_FURNACE) ? vec3(1.f) : vabs(rayDir) * fc.envIntensityScale;
+      total = total + cv.throughput * rgb_unbounded_spectrum(S, sky, L) * w;
+      break;
+    }
+    if (k >= fc.regularizeAfterVertices) rs.flags |= RS_REGULARIZE;
+    BdptVertex v;
+    const MaterialGPU* m = nullptr;
+    bdpt_surface(S, rs, h, rayOrigin, rayDir, L, v, &m);
+    if (k == 0 && fh) first_hit_features(fc, *m, v.ctx.def, v.pos, *fh);
+    cv.pos = v.pos;
+    cv.frontFace = v.frontFace;
+    cv.wo = v.wo;
+    cv.matId = v.matId;
+    cv.normalWS = v.normalWS;
+    bool delta = is_delta_distribution(v.ctx.pre.ax, v.ctx.pre.ay, v.ctx.def);
+    cv.curFwdMIS = pdfForwardCamera < 0.f ? -1.f : pdfForwardCamera * solid_angle_to_area(v.pos - rayOrigin, v.normalWS);
+    Spectrum ab = rs.get_absorption();
+    if (!ab.all_equal(0.f)) cv.throughput = cv.throughput * transmittance(h.t, ab);
+    const SurfaceDef& sd = v.ctx.def;
+    if (!sd.emissive.all_equal(0.f)) {
+      float w = mis_camera_hits_light(S, k, h.tri, cv);
+      if (surface_two_sided(sd.flags) || dot(sd.geometryNormal, v.wo) > 0.f) {
+        float profile = emission_profile(dot(v.normalWS, -rayDir), sd.emissionFocus);
+        total = total + cv.throughput * sd.emissive * w * profile;
+      }
+      break;
+    } else if (!delta) {
+      LightVertexCtx lv;
+      lv.riSum = 0.f;
+      lv.prevFlags = 0;
+      lv.prevPos = vec3(0.f);
+      lv.prevNrm = vec3(0.f);
+      lv.prevFwdMIS = 0.f;
+      float deferredLight = 0.f;
+      for (uint32_t j = 0; j < nodeCount; ++j) {
+        LightNode n = unpack_node(nodes[j]);
+        if ((n.flags & LN_DELTA) == 0) {
+          vec3 toNode = n.pos - v.pos;
+          float dist = length(toNode);
+          cv.wi = normalize(toNode / dist);
+          total = total + connection_radiance(S, k, cv, v.ctx, n, j, lv, rs, L, tr);
+        }
+        lv.riSum = deferredLight;
+        deferredLight = accumulate_ri(deferredLight, n.pdfBwd, n.pdfFwd);
+        lv.prevNrm = n.nrm;
+        lv.prevPos = n.pos;
+        lv.prevFwdMIS = n.pdfFwd;
+        lv.prevFlags = n.flags;
+      }
+    }
+    bool isDiffuse, isDelta;
+    vec3 wi = sample_scattering(S, v.ctx, rs0, isDiffuse, isDelta);
+    if (is_zero(wi)) break;
+    if (isDiffuse) rs.flags |= RS_REGULARIZE;
+    ScatterEval ev = eval_and_pdfs(S, v.ctx, rs, v.matId, v.wo, wi, v.frontFace, false, isDelta);
+    Spectrum sampleWeight = ev.weight * safe_div(1.f, ev.pdfFwd);
+    if (ev.dispersed) cv.throughput.terminate_secondary();
+    if (cv.throughput.russian_roulette(rand.r1())) break;
+    if (k > 0) {
+      float pb = isDelta ? -1.f : ev.pdfBwd * solid_angle_to_area(rayOrigin - v.pos, cv.prevNrm);
+      deferredCam = accumulate_ri(deferredCam, pb, cv.prevFwdMIS);
+      cv.pdfRatioSum = deferredCam;
+    }
+    cv.prevNrm = v.normalWS;
+    cv.prevPos = v.pos;
+    cv.prevFwdMIS = cv.curFwdMIS;
+    pdfForwardCamera = isDelta ? -1.f : ev.pdfFwd;
+    cv.throughput = cv.throughput * sampleWeight;
+    rayDir = normalize(wi);
+    rayOrigin = spawn_ray_origin(v.pos, v.normalWS, rayDir);
+  }
+  return total;
+}
+
+// light subpath the camera workgroup `group` connects to (BDPTTraceCameraPath.comp:75-83)
+YPT_HD uint32_t pick_light_path(const uint32_t* point, uint32_t group, uint32_t headerCount) {
+  RandomSampler r;
+  r.init(point, 0u, group);
+  float u = r.r1();
+  return (uint32_t)fmin_(floorf(u * (float)headerCount), (float)(headerCount - 1u));
+}
+
+// wavelength of a workgroup with no light subpaths: the first thread's sampler after the lens sample
+YPT_HD float fallback_wavelength(const FrameConstants& fc, const uint32_t* point, uint32_t px, uint32_t py) {
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0];
+  RandomSampler rand;
+  rand.init(point, camera_dim_offset(fc), py * W + px);
+  rand.dim += 2u;
+  return hero_wavelength(rand.r1());
+}
+
+// ---------------------------------------------------------------- light tracing onto the lens
+// projectThroughLensToUV (raytraceCommonResources.hlsl:139-156)
+YPT_HD bool project_to_uv(const FrameConstants& fc, vec3 lensPosVS, vec3 posWS, float& u, float& v) {
+  vec3 posVS = xform_point(fc.worldToView, posWS);
+  vec3 lensToPos = posVS - lensPosVS;
+  float forwardDist = lensToPos.z * fc.lensParams[2];
+  if (forwardDist <= 0.f) { u = 0.f; v = 0.f; return false; }
+  vec3 focus = lensPosVS + lensToPos * (fc.lensParams[1] / forwardDist);
+  float in[4] = {focus.x, focus.y, focus.z, 1.f}, o[4];
+  mul4(fc.viewToUV, in, o);
+  u = o[0] / o[3];
+  v = o[1] / o[3];
+  return u >= 0.f && v >= 0.f && u <= 1.f && v <= 1.f;
+}
+
+struct SplatRecord {
+  uint32_t texel;  // npix when the vertex adds nothing
+  vec3 xyz;
+};
+
+// calculatePureLightPaths (BDPTCalculatePureLightPaths.comp:107-206) for the light subpath in header
+// slot `header`: one record per vertex 1 .. count-1 into rec[0 .. count-1).
+template <class Tracer>
+YPT_HD void splat_light_path(const SceneView& S, const FrameConstants& fc, const uint32_t* point, uint32_t header,
+                             uint32_t headerCount, float lambda, const LightNodePacked* nodes, uint32_t count, Tracer& tr,
+                             SplatRecord* rec) {
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
+  uint32_t npix = W * H;
+  for (uint32_t i = 1; i < count; ++i) rec[i - 1].texel = npix;
+  if (count <= 1) return;
+  Lambdas L = spectral_lambdas(lambda);
+  RandomSampler lr;
+  lr.init(point, pure_lens_dim_offset(fc), header);
+  float u2[2];
+  lr.r2(u2);
+  float dx, dy;
+  sample_concentric_disk(u2[0], u2[1], dx, dy);
+  vec3 lensPosVS(dx * fc.lensParams[0], dy * fc.lensParams[0], 0.f);
+  vec3 lensPosWS = xform_point(fc.viewToWorld, lensPosVS);
+  LightNode n0 = unpack_node(nodes[0]);
+  float riSum = 0.f;
+  float deferred = accumulate_ri(0.f, n0.pdfBwd, n0.pdfFwd);
+  vec3 prevPos = n0.pos, prevNrm = n0.nrm;
+  float prevFwd = n0.pdfFwd;
+  for (uint32_t i = 1; i < count; ++i) {
+    LightNode n = unpack_node(nodes[i]);
+    float u, v;
+    if (project_to_uv(fc, lensPosVS, n.pos, u, v) && (n.flags & LN_DELTA) == 0) {
+      ScatterEval ev;
+      float unused;
+      if (connect_light_node(S, n, i, prevPos, lensPosWS, false, L, ev, unused) && !ev.weight.all_equal(0.f)) {
+        // checkVisibility (:70-101): from the vertex to the lens point, t in [0.01, distance]
+        vec3 fromTo = lensPosWS - n.pos;
+        float len = sqrtf(dot(fromTo, fromTo));
+        bool visible = len < 0.0001f || tr.unoccluded(n.pos, fromTo / len, 0.01f, len);
+        if (visible) {
+          float pbPrev = ev.pdfBwd * solid_angle_to_area(prevPos - n.pos, prevNrm);
+          vec3 camToNode = n.pos - lensPosWS;
+          float pdfCam = camera_dir_pdf(fc, camToNode);
+          float ri = accumulate_ri(riSum, pbPrev, prevFwd);
+          ri = accumulate_ri(ri, pdfCam * solid_angle_to_area(camToNode, n.nrm), n.pdfFwd);
+          float mis = 1.f / (1.f + ri);
+          float geom = 1.f / fmax_(0.0001f, dot(camToNode, camToNode));
+          float importance = safe_div(pdfCam, (float)headerCount);
+          Spectrum c = n.radiance * ev.weight * mis * geom * importance;
+          float su = fmin_(fmax_(u, 0.f), 1.f), sv = fmin_(fmax_(v, 0.f), 1.f);
+          uint32_t tx = (uint32_t)floorf(fmin_(su * (float)W, (float)(W - 1u)));
+          uint32_t ty = (uint32_t)floorf(fmin_(sv * (float)H, (float)(H - 1u)));
+          rec[i - 1].texel = ty * W + tx;
+          rec[i - 1].xyz = spectrum_to_xyz(c, L, S.spec);
+        }
+      }
+    }
+    riSum = deferred;
+    deferred = accumulate_ri(deferred, n.pdfBwd, n.pdfFwd);
+    prevPos = n.pos;
+    prevNrm = n.nrm;
+    prevFwd = n.pdfFwd;
+  }
+}
+
+// Texel permutation of the camera dispatch (BidirectionalPathIntegratorSubStage.cpp:549-591): every
+// 8x8 workgroup covers 64 texels spread over the image. Each position is swapped with a random one,
+// drawn from a seeded SplitMix64 stream here.
+inline void texel_permutation(uint32_t W, uint32_t H, uint64_t seed, uint32_t* out) {
+  uint32_t n = W * H;
+  for (uint32_t y = 0; y < H; ++y)
+    for (uint32_t x = 0; x < W; ++x) out[y * W + x] = (y << 16) | (x & 0xFFFFu);
+  uint64_t s = seed;
+  for (uint32_t i = 0; i < n; ++i) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    uint32_t j = (uint32_t)(z % n);
+    uint32_t t = out[i]; out[i] = out[j]; out[j] = t;
+  }
+}
+
+}  // namespace ypt

@@ -49,6 +49,10 @@ struct FrameConstants {
   uint32_t regularizeAfterVertices;
   uint32_t integrator;
   float lensParams[4];  // aperture radius, focus distance, view-space forward z sign, unused
+  // bidirectional integrator (BidirectionalPathTraceConstants, BDPTCommon.hlsl:30-38)
+  uint32_t lightPathVertices;  // maxVerticesPerLightPath
+  uint32_t lightPathsPerDim[2];
+  uint32_t pad;
 };
 
 struct SceneView {

@@ -122,19 +122,12 @@ YPT_HD vec3 xform_dir(const mat4& M, vec3 d) {
   return vec3(o[0], o[1], o[2]);
 }
 
-// rayGenPrimaryRays.hlsl:13-39 and generateCameraRay (raytraceCommonResources.hlsl:115-137)
-YPT_HD void start_path(const FrameConstants& fc, const uint32_t* point, uint32_t px, uint32_t py, PathState& ps) {
-  uint32_t W = (uint32_t)fc.targetTexDimensions[0];
-  uint32_t rayIndex = py * W + px;
-  RandomSampler rand;
-  rand.init(point, 0u, rayIndex);
-  float lambda = hero_wavelength(rand.r1());
+// generateCameraRay (raytraceCommonResources.hlsl:115-137): thin-lens ray through texel (px, py),
+// texel jitter (jx, jy) and lens sample (l0, l1)
+YPT_HD void camera_ray(const FrameConstants& fc, uint32_t px, uint32_t py, float jx, float jy, float l0, float l1,
+                       vec3& origin, vec3& dir) {
   float uvx = (float)px * fc.targetTexDimensions[2] + fc.rayUVOffset[0];
   float uvy = (float)py * fc.targetTexDimensions[3] + fc.rayUVOffset[1];
-  float jx, jy;
-  rand.texel_offset(jx, jy);
-  float lens[2];
-  rand.r2(lens);
   float ox = 0.f, oy = 0.f;
   if ((fc.flags & FRAME_FLAG_DISABLE_JITTER) == 0) { ox = jx - 0.5f; oy = jy - 0.5f; }
   ox *= fc.targetTexDimensions[2];
@@ -144,10 +137,24 @@ YPT_HD void start_path(const FrameConstants& fc, const uint32_t* point, uint32_t
   vec3 pinhole(v[0] / v[3], v[1] / v[3], v[2] / v[3]);
   vec3 focusPoint = pinhole * safe_div(fc.lensParams[1], fabsf(pinhole.z));
   float dx, dy;
-  sample_concentric_disk(lens[0], lens[1], dx, dy);
+  sample_concentric_disk(l0, l1, dx, dy);
   vec3 lensPos(dx * fc.lensParams[0], dy * fc.lensParams[0], 0.f);
-  ps.origin = xform_point(fc.viewToWorld, lensPos);
-  ps.dir = normalize(xform_dir(fc.viewToWorld, focusPoint - lensPos));
+  origin = xform_point(fc.viewToWorld, lensPos);
+  dir = normalize(xform_dir(fc.viewToWorld, focusPoint - lensPos));
+}
+
+// rayGenPrimaryRays.hlsl:13-39
+YPT_HD void start_path(const FrameConstants& fc, const uint32_t* point, uint32_t px, uint32_t py, PathState& ps) {
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0];
+  uint32_t rayIndex = py * W + px;
+  RandomSampler rand;
+  rand.init(point, 0u, rayIndex);
+  float lambda = hero_wavelength(rand.r1());
+  float jx, jy;
+  rand.texel_offset(jx, jy);
+  float lens[2];
+  rand.r2(lens);
+  camera_ray(fc, px, py, jx, jy, lens[0], lens[1], ps.origin, ps.dir);
   ps.throughput.set(1.f);
   ps.totalLight.set(0.f);
   ps.rayState = RAY_ALIVE;
@@ -277,6 +284,30 @@ YPT_HD float light_connection_pdf(const SceneView& S, vec3 fromToLight, uint32_t
   return posPdf * pick * area_to_solid_angle(fromToLight, gn);
 }
 
+// writeMaterialParamsForFirstBounce (hitShadersCommon.hlsl:94-136), unpacked
+YPT_HD void first_hit_features(const FrameConstants& fc, const MaterialGPU& m, const SurfaceDef& sd, vec3 hitPos,
+                               FirstHitFeatures& f) {
+  float cw = sd.clearCoatAmount, sw = sd.sheenAmount, bw = sd.specularAmount;
+  float wsum = cw + sw + bw;
+  if (wsum == 0.f) wsum = 1.f;
+  cw /= wsum; sw /= wsum; bw /= wsum;
+  vec3 n = sd.baseNormal;
+  if (sd.clearCoatAmount > 0.f) n = n + sd.coatingNormal;
+  f.normal = normalize(n);
+  vec3 em(m.emissiveRGB[0], m.emissiveRGB[1], m.emissiveRGB[2]);
+  vec3 al(m.albedoRGB[0], m.albedoRGB[1], m.albedoRGB[2]);
+  vec3 sh(m.sheenRGB[0], m.sheenRGB[1], m.sheenRGB[2]);
+  f.albedo = em / (vec3(1.f) + em) + al + sh * sd.sheenAmount;
+  f.roughness = sd.roughness * bw + sd.clearCoatRoughness * cw + sd.sheenRoughness * sw;
+  f.ior = sd.dielectricIOR * sd.specularAmount + sd.clearCoatIOR * sd.clearCoatAmount;
+  f.anisotropy = sd.anisotropy;
+  f.metalness = sd.metalness;
+  vec3 vp = xform_point(fc.worldToView, hitPos);
+  f.depth = vp.z * fc.lensParams[2];
+  f.transparency = sd.transparency;
+  f.hit = 1u;
+}
+
 // ---------------------------------------------------------------- closest hit
 // rayHitDefault (rayHitDefault.hlsl:464-585) with evaluateSurfaceAndGenerateNextSampleDirection (:316-435)
 // and connectToExplicitLight (:204-257). `nee` selects BPT_WITH_NEE.
@@ -311,27 +342,7 @@ YPT_HD void shade_hit(const SceneView& S, const FrameConstants& fc, const uint32
   SurfaceDef sd = make_surface(m, geometryNormal, normal, tangent, -rayDir, true, regularizePath, L, S.spec);
   vec3 hitPos = ps.origin + ps.dir * h.t;
 
-  if (ps.pathLength == 0 && fh) {  // writeMaterialParamsForFirstBounce
-    float cw = sd.clearCoatAmount, sw = sd.sheenAmount, bw = sd.specularAmount;
-    float wsum = cw + sw + bw;
-    if (wsum == 0.f) wsum = 1.f;
-    cw /= wsum; sw /= wsum; bw /= wsum;
-    vec3 n = sd.baseNormal;
-    if (sd.clearCoatAmount > 0.f) n = n + sd.coatingNormal;
-    fh->normal = normalize(n);
-    vec3 em(m.emissiveRGB[0], m.emissiveRGB[1], m.emissiveRGB[2]);
-    vec3 al(m.albedoRGB[0], m.albedoRGB[1], m.albedoRGB[2]);
-    vec3 sh(m.sheenRGB[0], m.sheenRGB[1], m.sheenRGB[2]);
-    fh->albedo = em / (vec3(1.f) + em) + al + sh * sd.sheenAmount;
-    fh->roughness = sd.roughness * bw + sd.clearCoatRoughness * cw + sd.sheenRoughness * sw;
-    fh->ior = sd.dielectricIOR * sd.specularAmount + sd.clearCoatIOR * sd.clearCoatAmount;
-    fh->anisotropy = sd.anisotropy;
-    fh->metalness = sd.metalness;
-    vec3 vp = xform_point(fc.worldToView, hitPos);
-    fh->depth = vp.z * fc.lensParams[2];
-    fh->transparency = sd.transparency;
-    fh->hit = 1u;
-  }
+  if (ps.pathLength == 0 && fh) first_hit_features(fc, m, sd, hitPos, *fh);
 
   // handleRayStartInsideMedium (hitShadersCommon.hlsl:213-222)
   if (dot(rayDir, geometryNormal) > 0.f && ps.nvol == 0 && !surface_two_sided(sd.flags))

@@ -1,0 +1,226 @@
+// Kernels of the bidirectional integrator on gfx950. One pass (one Sobol sample index):
+//   light    - one light subpath per thread (BDPTTraceLightPath.comp), vertices stored in order
+//   headers  - the subpaths that exist, in index order (the reference reserves them with an atomic;
+//              a fixed order keeps every result reproducible)
+//   camera   - one camera subpath per thread; a 64-thread workgroup covers 64 permuted texels and
+//              connects all of them to one light subpath (BDPTTraceCameraPath.comp)
+//   splat    - light tracing onto a lens point: one record (texel, XYZ) per light vertex
+//              (BDPTCalculatePureLightPaths.comp)
+//   sort + accumulate - records sorted by texel with a stable radix sort, then summed per texel in
+//              record order and added to the camera sample (BDPTAccumulatePureLightPaths.comp),
+//              so the floating-point sum has one fixed order
+// Traversal is inline (closest hit, target visibility, occlusion) with the LDS stack of traverse.h.
+#pragma once
+#include "../core/bdpt.h"
+#include "wavefront.h"
+
+namespace ypt {
+
+template <int WIDTH>
+struct GpuTracer {
+  const Bvh2Node* nodes2;
+  const Bvh8Node* nodes8;
+  const TriRecord* tris;
+  SceneView sv;
+  TravStack st;
+  __device__ bool trace(int mode, vec3 o, vec3 d, float tmin, float tmax, uint32_t target, float& t, uint32_t& id, float& b1,
+                        float& b2) {
+    TraversalCounters c = {0u, 0u};
+    if (mode == 0) {
+      if (WIDTH == 2) return traverse_bvh2<0, false>(nodes2, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, c);
+      return traverse_bvh8<0, false>(nodes8, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, c);
+    }
+    if (WIDTH == 2) return traverse_bvh2<1, false>(nodes2, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, c);
+    return traverse_bvh8<1, false>(nodes8, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, c);
+  }
+  __device__ bool closest(vec3 o, vec3 d, float tmin, float tmax, Hit& h) {
+    float t = 0.f, b1 = 0.f, b2 = 0.f;
+    uint32_t id = NO_HIT_ID;
+    bool hit = trace(0, o, d, tmin, tmax, 0u, t, id, b1, b2);
+    h.t = hit ? t : 0.f;
+    h.tri = hit ? id : NO_HIT_ID;
+    h.b1 = hit ? b1 : 0.f;
+    h.b2 = hit ? b2 : 0.f;
+    return hit;
+  }
+  // closest hit is `tri`: `tri` is hit in [tmin, tmax] and nothing orders before it
+  __device__ bool hits_target(vec3 o, vec3 d, float tmin, float tmax, uint32_t tri) {
+    RayPre rp = ray_precompute(o, d);
+    vec3 p1, p2, p3;
+    tri_vertices(sv, tri, p1, p2, p3);
+    float tt, b1, b2;
+    if (!intersect_triangle(rp, p1, p2, p3, tmin, tmax, tt, b1, b2)) return false;
+    float t;
+    uint32_t id;
+    return !trace(1, o, d, tmin, tt, tri, t, id, b1, b2);
+  }
+  __device__ bool unoccluded(vec3 o, vec3 d, float tmin, float tmax) {
+    float t, b1, b2;
+    uint32_t id;
+    return !trace(1, o, d, tmin, tmax, NO_HIT_ID, t, id, b1, b2);
+  }
+};
+
+struct BdptBuffers {
+  LightNodePacked* nodes;  // [path][BDPT_MAX_LIGHT_VERTICES]
+  uint32_t* counts;        // vertices per light subpath (0: none)
+  float* lambdas;          // hero wavelength per light subpath
+  uint32_t* headers;       // existing subpaths in index order
+  uint32_t* headerCount;
+  const uint32_t* perm;    // texel permutation (y << 16 | x) per dispatch position
+  float4* camXYZ;          // per texel: XYZ of the sample, hero wavelength
+  uint32_t* recKeys;       // texel of each splat record (npix: none)
+  uint32_t* recVals;       // record index
+  float4* recXYZ;
+  uint32_t* ovf;
+  uint32_t ovfStride;
+  uint32_t npaths;
+  uint32_t recPerPath;     // light vertices per path - 1
+};
+
+template <int WIDTH>
+__device__ __forceinline__ GpuTracer<WIDTH> make_tracer(const DevScene& S, uint32_t* lds, const BdptBuffers& B, uint32_t gid,
+                                                        uint32_t ldsDepth) {
+  GpuTracer<WIDTH> tr;
+  tr.nodes2 = S.nodes2;
+  tr.nodes8 = S.nodes8;
+  tr.tris = S.tris;
+  tr.sv = S.sv;
+  tr.st.lds = lds;
+  tr.st.ovf = B.ovf;
+  tr.st.block = blockDim.x;
+  tr.st.lane = threadIdx.x;
+  tr.st.ldsDepth = ldsDepth;
+  tr.st.ovfStride = B.ovfStride;
+  tr.st.gid = gid;
+  tr.st.sp = 0;
+  return tr;
+}
+
+template <int WIDTH>
+__global__ __launch_bounds__(64) void k_bdpt_light(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, uint32_t ldsDepth) {
+  extern __shared__ uint32_t lds[];
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= B.npaths) return;
+  GpuTracer<WIDTH> tr = make_tracer<WIDTH>(S, lds, B, p, ldsDepth);
+  float lambda;
+  uint32_t c = trace_light_path(S.sv, fc, point, p, tr, B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES, lambda);
+  B.counts[p] = c;
+  B.lambdas[p] = lambda;
+}
+
+// one workgroup of 1024 threads: subpaths with vertices, compacted in index order
+__global__ void k_bdpt_headers(BdptBuffers B) {
+  __shared__ uint32_t sums[1024];
+  uint32_t t = threadIdx.x;
+  uint32_t chunk = (B.npaths + 1023u) / 1024u;
+  uint32_t begin = t * chunk;
+  uint32_t end = begin + chunk < B.npaths ? begin + chunk : B.npaths;
+  uint32_t c = 0;
+  for (uint32_t i = begin; i < end; ++i) c += B.counts[i] > 0u ? 1u : 0u;
+  sums[t] = c;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < 1024u; ++k) {
+      uint32_t v = sums[k];
+      sums[k] = run;
+      run += v;
+    }
+    *B.headerCount = run;
+  }
+  __syncthreads();
+  uint32_t o = sums[t];
+  for (uint32_t i = begin; i < end; ++i)
+    if (B.counts[i] > 0u) B.headers[o++] = i;
+}
+
+// 64 threads per workgroup = one 8x8 dispatch tile
+template <int WIDTH>
+__global__ __launch_bounds__(64) void k_bdpt_camera(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, float* features,
+                              uint32_t ldsDepth) {
+  extern __shared__ uint32_t lds[];
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
+  uint32_t gcx = (W + BDPT_TILE - 1u) / BDPT_TILE;
+  uint32_t g = blockIdx.x, t = threadIdx.x;
+  uint32_t gx = g % gcx, gy = g / gcx;
+  uint32_t dx = gx * BDPT_TILE + t % BDPT_TILE, dy = gy * BDPT_TILE + t / BDPT_TILE;
+  uint32_t hc = *B.headerCount;
+  float lambda;
+  const LightNodePacked* nodes = nullptr;
+  uint32_t count = 0;
+  if (hc > 0) {
+    uint32_t p = B.headers[pick_light_path(point, g, hc)];
+    lambda = B.lambdas[p];
+    nodes = B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES;
+    count = B.counts[p];
+  } else {
+    uint32_t q = B.perm[(gy * BDPT_TILE) * W + gx * BDPT_TILE];
+    lambda = fallback_wavelength(fc, point, q & 0xFFFFu, q >> 16);
+  }
+  if (dx >= W || dy >= H) return;
+  uint32_t q = B.perm[dy * W + dx];
+  uint32_t px = q & 0xFFFFu, py = q >> 16;
+  uint32_t pix = py * W + px;
+  GpuTracer<WIDTH> tr = make_tracer<WIDTH>(S, lds, B, g * blockDim.x + t, ldsDepth);
+  FirstHitFeatures fh;
+  fh.hit = 0u;
+  Spectrum L = trace_camera_path(S.sv, fc, point, px, py, lambda, nodes, count, hc == 0 && t == 0, tr, features ? &fh : nullptr);
+  vec3 xyz = spectrum_to_xyz(L, spectral_lambdas(lambda), S.sv.spec);
+  B.camXYZ[pix] = make_float4(xyz.x, xyz.y, xyz.z, lambda);
+  if (features) {
+    float* f = features + (size_t)pix * 12;
+    if (fh.hit) {
+      f[0] = fh.normal.x; f[1] = fh.normal.y; f[2] = fh.normal.z;
+      f[3] = fh.albedo.x; f[4] = fh.albedo.y; f[5] = fh.albedo.z;
+      f[6] = fh.roughness; f[7] = fh.ior; f[8] = fh.metalness; f[9] = fh.transparency; f[10] = fh.depth; f[11] = 1.f;
+    } else {
+      for (int k = 0; k < 12; ++k) f[k] = 0.f;
+    }
+  }
+}
+
+template <int WIDTH>
+__global__ __launch_bounds__(64) void k_bdpt_splat(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, uint32_t ldsDepth) {
+  extern __shared__ uint32_t lds[];
+  uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= B.npaths) return;
+  uint32_t npix = (uint32_t)fc.targetTexDimensions[0] * (uint32_t)fc.targetTexDimensions[1];
+  size_t base = (size_t)h * B.recPerPath;
+  uint32_t hc = *B.headerCount;
+  SplatRecord rec[BDPT_MAX_LIGHT_VERTICES - 1];
+  uint32_t n = 0;
+  if (h < hc) {
+    uint32_t p = B.headers[h];
+    n = B.counts[p];
+    GpuTracer<WIDTH> tr = make_tracer<WIDTH>(S, lds, B, h, ldsDepth);
+    splat_light_path(S.sv, fc, point, h, hc, B.lambdas[p], B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES, n, tr, rec);
+  }
+  for (uint32_t j = 0; j < B.recPerPath; ++j) {
+    bool valid = j + 1u < n && rec[j].texel < npix;
+    B.recKeys[base + j] = valid ? rec[j].texel : npix;
+    B.recVals[base + j] = (uint32_t)(base + j);
+    if (valid) B.recXYZ[base + j] = make_float4(rec[j].xyz.x, rec[j].xyz.y, rec[j].xyz.z, 0.f);
+  }
+}
+
+// sorted records: the first record of each texel sums its run in record order
+__global__ void k_bdpt_accum(const uint32_t* keys, const uint32_t* vals, const float4* recXYZ, uint32_t n, uint32_t npix,
+                             float4* camXYZ) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint32_t key = keys[k];
+  if (key >= npix) return;
+  if (k > 0 && keys[k - 1] == key) return;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  for (uint32_t j = k; j < n && keys[j] == key; ++j) {
+    float4 r = recXYZ[vals[j]];
+    sx += r.x; sy += r.y; sz += r.z;
+  }
+  if (sx != 0.f || sy != 0.f || sz != 0.f) {
+    float4 c = camXYZ[key];
+    camXYZ[key] = make_float4(c.x + sx, c.y + sy, c.z + sz, c.w);
+  }
+}
+
+}  // namespace ypt

@@ -418,16 +418,25 @@ __device__ vec3 clamp_firefly(const MergeParams& mp, const FilmBuffers& F, int x
 }
 
 // path XYZ + mergeNewSamples (mergeNewSamples.comp:173-265) with per-pixel sample counts
-__global__ void k_merge(DevScene S, PathSoA P, FilmBuffers F, MergeParams mp) {
+// sampleXYZ: XYZ and hero wavelength per texel (bidirectional integrator); null: XYZ of the path slots
+__global__ void k_merge(DevScene S, PathSoA P, FilmBuffers F, MergeParams mp, const float4* sampleXYZ) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= mp.width * mp.height) return;
   int x = (int)(p % mp.width), y = (int)(p / mp.width);
-  float4 l = P.light[mp.slotBase + p];
-  float lam = P.orgLambda[mp.slotBase + p].w;
-  PathState ps;
-  ps.totalLight = Spectrum(l.x, l.y, l.z, l.w);
-  ps.lambda = lam;
-  vec3 xyz = path_xyz(S.sv, ps);
+  vec3 xyz;
+  float lam;
+  if (sampleXYZ) {
+    float4 v = sampleXYZ[p];
+    xyz = vec3(v.x, v.y, v.z);
+    lam = v.w;
+  } else {
+    float4 l = P.light[mp.slotBase + p];
+    lam = P.orgLambda[mp.slotBase + p].w;
+    PathState ps;
+    ps.totalLight = Spectrum(l.x, l.y, l.z, l.w);
+    ps.lambda = lam;
+    xyz = path_xyz(S.sv, ps);
+  }
   F.lastSample[p] = make_float4(xyz.x, xyz.y, xyz.z, lam);
   float alpha = 1.f;
   if (mp.killNaN != 0 && (is_nan(xyz.x) || is_nan(xyz.y) || is_nan(xyz.z))) { xyz = vec3(0.f); alpha = 0.f; }

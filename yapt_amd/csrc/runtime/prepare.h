@@ -277,6 +277,12 @@ inline void prepare_frame(const ypt_frame_desc& f, const PreparedScene& s, uint3
   fc.lensParams[1] = std::max(1e-4f, f.focus_distance);
   fc.lensParams[2] = f.forward_z_sign;
   fc.lensParams[3] = 0.f;
+  uint32_t lv = f.light_path_vertices == 0 ? 16u : f.light_path_vertices;
+  if (lv < 2 || lv > 16) throw std::invalid_argument("light_path_vertices must be in [2, 16]");
+  fc.lightPathVertices = lv;
+  fc.lightPathsPerDim[0] = (f.width + 7u) / 8u;   // one light subpath per 8x8 texels (TEXELS_PER_LIGHTPATH)
+  fc.lightPathsPerDim[1] = (f.height + 7u) / 8u;
+  fc.pad = 0;
 }
 
 }  // namespace ypt

@@ -1,5 +1,6 @@
 // libyapt_pt: C ABI over the gfx950 wavefront path tracer (see include/yapt_pt.h).
 #include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
 
 #include <chrono>
 #include <cstdio>
@@ -12,6 +13,7 @@
 
 #include "../../include/yapt_pt.h"
 #include "core/shade.h"
+#include "kernels/bdpt_kernels.h"
 #include "kernels/wavefront.h"
 #include "runtime/bvh.h"
 #include "runtime/luts.h"
@@ -121,6 +123,15 @@ struct ypt_integrator {
   DevBuf<uint32_t> rayQ0, rayQ1, counters, sSlot, ovf, sobolPoints;
   DevBuf<unsigned long long> work;
   uint32_t ovfStride = 0;
+
+  // bidirectional integrator
+  DevBuf<LightNodePacked> bNodes;
+  DevBuf<uint32_t> bCounts, bHeaders, bHeaderCount, bPerm, bKeys, bVals, bKeysOut, bValsOut, bOvf;
+  DevBuf<float> bLambdas;
+  DevBuf<float4> bCamXYZ, bRecXYZ;
+  DevBuf<uint8_t> bSortTemp;
+  uint32_t bW = 0, bH = 0, bOvfStride = 0;
+  size_t bSortBytes = 0;
 
   // film
   DevBuf<float4> accum, merged0, merged1, lastSample;
@@ -347,6 +358,126 @@ void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
   }
 }
 
+
+// ---------------------------------------------------------------- bidirectional integrator
+void ensure_bdpt(ypt_integrator* h) {
+  const FrameConstants& fc = h->fc;
+  uint32_t W = h->width, H = h->height;
+  uint32_t npix = W * H;
+  uint32_t npaths = fc.lightPathsPerDim[0] * fc.lightPathsPerDim[1];
+  size_t nrec = (size_t)npaths * (BDPT_MAX_LIGHT_VERTICES - 1);
+  if (h->bW != W || h->bH != H) {
+    h->bNodes.alloc((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
+    h->bCounts.alloc(npaths); h->bLambdas.alloc(npaths); h->bHeaders.alloc(npaths); h->bHeaderCount.alloc(1);
+    h->bCamXYZ.alloc(npix);
+    h->bKeys.alloc(nrec); h->bVals.alloc(nrec); h->bKeysOut.alloc(nrec); h->bValsOut.alloc(nrec); h->bRecXYZ.alloc(nrec);
+    std::vector<uint32_t> perm(npix);
+    texel_permutation(W, H, 0x59415054ull, perm.data());
+    h->bPerm.upload(perm.data(), perm.size(), h->stream);
+    size_t bytes = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, h->bKeys.p, h->bKeysOut.p, h->bVals.p, h->bValsOut.p,
+                                                 (int)nrec, 0, 32, h->stream));
+    h->bSortTemp.alloc(bytes);
+    h->bSortBytes = bytes;
+    h->bW = W;
+    h->bH = H;
+    h->bOvf.release();
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+  }
+  uint32_t groups = ((W + BDPT_TILE - 1) / BDPT_TILE) * ((H + BDPT_TILE - 1) / BDPT_TILE);
+  uint32_t stride = groups * 64u;
+  uint32_t bound = h->bvhWidth == 8 ? h->stackBound8 : h->stackBound2;
+  size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
+  if (h->bOvf.n < depth * stride || h->bOvfStride != stride) {
+    h->bOvf.alloc(depth * stride);
+    h->bOvfStride = stride;
+  }
+  if (h->writeFeatures && h->features.n < (size_t)npix * 12) h->features.alloc((size_t)npix * 12);
+}
+
+template <int WIDTH>
+void bdpt_pass(ypt_integrator* h, const DevScene& S, const uint32_t* point, const BdptBuffers& B, uint32_t groups) {
+  hipStream_t s = h->stream;
+  const FrameConstants& fc = h->fc;
+  uint32_t d = (uint32_t)h->ldsDepth;
+  size_t lds = (size_t)h->ldsDepth * 64 * 4;
+  hipLaunchKernelGGL((k_bdpt_light<WIDTH>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
+  hipLaunchKernelGGL(k_bdpt_headers, dim3(1), dim3(1024), 0, s, B);
+  hipLaunchKernelGGL((k_bdpt_camera<WIDTH>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
+                     h->writeFeatures ? h->features.p : (float*)nullptr, d);
+  hipLaunchKernelGGL((k_bdpt_splat<WIDTH>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
+}
+
+void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count) {
+  ensure_bdpt(h);
+  hipStream_t s = h->stream;
+  const FrameConstants& fc = h->fc;
+  uint32_t W = h->width, H = h->height, npix = W * H;
+  uint32_t groups = ((W + BDPT_TILE - 1) / BDPT_TILE) * ((H + BDPT_TILE - 1) / BDPT_TILE);
+  std::vector<uint32_t> pts((size_t)sample_count * YPT_SOBOL_POINT_DIMS);
+  for (uint32_t k = 0; k < sample_count; ++k)
+    for (uint32_t dd = 0; dd < YPT_SOBOL_POINT_DIMS; ++dd)
+      pts[(size_t)k * YPT_SOBOL_POINT_DIMS + dd] = sobol_uint(YPT_SOBOL_MATRICES, sample_begin + k, dd);
+  if (h->sobolPoints.n < pts.size()) h->sobolPoints.alloc(pts.size());
+  h->sobolPoints.upload(pts.data(), pts.size(), s);
+  BdptBuffers B;
+  B.nodes = h->bNodes.p; B.counts = h->bCounts.p; B.lambdas = h->bLambdas.p; B.headers = h->bHeaders.p;
+  B.headerCount = h->bHeaderCount.p; B.perm = h->bPerm.p; B.camXYZ = h->bCamXYZ.p;
+  B.recKeys = h->bKeys.p; B.recVals = h->bVals.p; B.recXYZ = h->bRecXYZ.p;
+  B.ovf = h->bOvf.p; B.ovfStride = h->bOvfStride;
+  B.npaths = fc.lightPathsPerDim[0] * fc.lightPathsPerDim[1];
+  B.recPerPath = fc.lightPathVertices - 1;
+  uint32_t nrec = B.npaths * B.recPerPath;
+  int bits = 32 - __builtin_clz(npix);
+  {
+    size_t need = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, h->bKeys.p, h->bKeysOut.p, h->bVals.p, h->bValsOut.p, (int)nrec, 0,
+                                                 bits, s));
+    if (need > h->bSortBytes) {
+      h->bSortTemp.alloc(need);
+      h->bSortBytes = need;
+    }
+  }
+  DevScene S = h->dev_scene();
+  PathSoA P = h->path_soa();
+  FilmBuffers F = h->film();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  h->evUsed = 0;
+  if (h->profile) { e0 = h->next_event(); HIP_CHECK(hipEventRecord(e0, s)); }
+  for (uint32_t k = 0; k < sample_count; ++k) {
+    const uint32_t* point = h->sobolPoints.p + (size_t)k * YPT_SOBOL_POINT_DIMS;
+    if (h->bvhWidth == 8) bdpt_pass<8>(h, S, point, B, groups);
+    else bdpt_pass<2>(h, S, point, B, groups);
+    size_t bytes = h->bSortBytes;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(h->bSortTemp.p, bytes, h->bKeys.p, h->bKeysOut.p, h->bVals.p, h->bValsOut.p,
+                                                 (int)nrec, 0, bits, s));
+    hipLaunchKernelGGL(k_bdpt_accum, dim3(blocks_for(nrec, 256)), dim3(256), 0, s, h->bKeysOut.p, h->bValsOut.p, h->bRecXYZ.p,
+                       nrec, npix, h->bCamXYZ.p);
+    MergeParams mp;
+    mp.width = W; mp.height = H; mp.slotBase = 0;
+    mp.sampleCount = h->filmSamples; mp.accumTargets = 2;
+    mp.fireflyClampScale = h->frameDesc.firefly_clamp_scale;
+    mp.fireflyClampRelaxPower = h->frameDesc.firefly_clamp_relax_power;
+    mp.killNaN = h->frameDesc.kill_nan_samples;
+    mp.cur = h->filmCur;
+    hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp, (const float4*)h->bCamXYZ.p);
+    h->filmSamples++;
+    h->filmCur ^= 1;
+  }
+  HIP_CHECK(hipGetLastError());
+  h->lastPassBase = 0;
+  std::memset(h->stats, 0, sizeof(h->stats));
+  if (h->profile) {
+    e1 = h->next_event();
+    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float all = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&all, e0, e1));
+    h->stats[5] = all;
+  }
+  h->stats[13] = 1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -365,7 +496,6 @@ int ypt_create(int integrator_type, int device, ypt_integrator** out) {
     if (!out) return fail(YPT_E_INVALID, "null handle pointer");
     if (integrator_type < YPT_INTEGRATOR_BPT || integrator_type > YPT_INTEGRATOR_BDPT)
       return fail(YPT_E_INVALID, "unknown integrator type " + std::to_string(integrator_type));
-    if (integrator_type == YPT_INTEGRATOR_BDPT) return fail(YPT_E_INVALID, "BDPT integrator is not available in this build");
     int n = 0;
     HIP_CHECK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(YPT_E_INVALID, "device index out of range");
@@ -509,6 +639,10 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
     if (!h->hasScene || !h->hasFrame) return fail(YPT_E_NOSCENE, "render needs a scene and a frame");
     HIP_CHECK(hipSetDevice(h->device));
     if (stream) h->stream = (hipStream_t)stream;
+    if (h->type == YPT_INTEGRATOR_BDPT) {
+      render_bdpt(h, sample_begin, sample_count);
+      return YPT_OK;
+    }
     hipStream_t s = h->stream;
     uint32_t npix = h->width * h->height;
     uint32_t G = passes_in_flight(h, sample_count);
@@ -576,7 +710,7 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
         mp.killNaN = h->frameDesc.kill_nan_samples;
         mp.cur = h->filmCur;
         hipEvent_t m0 = h->profile ? mark() : nullptr;
-        hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp);
+        hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp, (const float4*)nullptr);
         if (h->profile) spans.push_back({m0, mark(), 4});
         h->filmSamples++;
         h->filmCur ^= 1;
