@@ -5,6 +5,10 @@ Workload (default): next-event-estimation path tracing of the ~1M-triangle rando
 path: camera rays, extension/shadow traversal, shading, film merge). Prints one JSON line with
 path samples/s, the roofline of the dominant kernel (BVH extension traversal, HBM-bound) and the
 CPU checker's throughput on a bounded sample of the same workload.
+
+--integrator bdpt: the bidirectional workload (the same scene plus 8 seeded blackbody emissive
+triangles, 1280x720, camera subpaths of depth 8, light subpaths of 9 vertices, i.e. depth 8); its
+dominant kernel is the camera-subpath kernel.
 """
 import argparse
 import json
@@ -27,11 +31,11 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=64)
     p.add_argument("--warmup", type=int, default=16)
-    p.add_argument("--width", type=int, default=1920)
-    p.add_argument("--height", type=int, default=1080)
-    p.add_argument("--depth", type=int, default=12)
+    p.add_argument("--width", type=int, default=0, help="0: 1920 (nee, bpt) or 1280 (bdpt)")
+    p.add_argument("--height", type=int, default=0, help="0: 1080 (nee, bpt) or 720 (bdpt)")
+    p.add_argument("--depth", type=int, default=0, help="0: 12 (nee, bpt) or 8 (bdpt)")
     p.add_argument("--shapes", type=int, default=2000)
-    p.add_argument("--integrator", default="nee", choices=["nee", "bpt"])
+    p.add_argument("--integrator", default="nee", choices=["nee", "bpt", "bdpt"])
     p.add_argument("--bvh-width", type=int, default=0, help="2 or 8; 0 = autotuned choice")
     p.add_argument("--block-size", type=int, default=0)
     p.add_argument("--lds-depth", type=int, default=0)
@@ -68,10 +72,15 @@ def main():
     import numpy as np
     from yapt_amd import _lib, camera, scenes, tuning
 
-    integ = _lib.YPT_INTEGRATOR_NEE if a.integrator == "nee" else _lib.YPT_INTEGRATOR_BPT
+    bdpt = a.integrator == "bdpt"
+    integ = {"nee": _lib.YPT_INTEGRATOR_NEE, "bpt": _lib.YPT_INTEGRATOR_BPT, "bdpt": _lib.YPT_INTEGRATOR_BDPT}[a.integrator]
+    a.width = a.width or (1280 if bdpt else 1920)
+    a.height = a.height or (720 if bdpt else 1080)
+    a.depth = a.depth or (8 if bdpt else 12)
+    light_vertices = 9 if bdpt else 16
     t0 = time.time()
-    scene, cam = scenes.random_shapes(n_shapes=a.shapes, seed=2)
-    frame = camera.make_frame(cam, a.width, a.height, max_depth=a.depth)
+    scene, cam = scenes.random_shapes(n_shapes=a.shapes, seed=2, emissive_triangles=8 if bdpt else 0)
+    frame = camera.make_frame(cam, a.width, a.height, max_depth=a.depth, light_path_vertices=light_vertices)
     t_scene = time.time() - t0
 
     it = _lib.Integrator(integ, 0)
@@ -119,11 +128,14 @@ def main():
     it.set_option("count_traversal", 0)
     launches = st_prof[10]
     ext_ms = st_prof[2]
-    ext_bytes = st_cnt[6] * NODE_BYTES[width] + st_cnt[7] * TRI_BYTES + st_cnt[0] * RAY_BYTES
+    if bdpt:   # camera-subpath kernel: traversal bytes of its extension and connection rays
+        ext_bytes = st_cnt[6] * NODE_BYTES[width] + st_cnt[7] * TRI_BYTES
+    else:
+        ext_bytes = st_cnt[6] * NODE_BYTES[width] + st_cnt[7] * TRI_BYTES + st_cnt[0] * RAY_BYTES
     ext_bytes_per_launch = ext_bytes / launches
     ext_ms_per_launch = ext_ms / launches
     achieved = ext_bytes_per_launch / (ext_ms_per_launch * 1e-3) / 1e9
-    traffic = tuning.measured_traffic("extend-%s" % a.integrator)
+    traffic = tuning.measured_traffic(("camera-%s" if bdpt else "extend-%s") % a.integrator)
 
     # CPU checker on a bounded sample of the same workload
     cpu = None
@@ -140,12 +152,26 @@ def main():
         passes = int(max(1, min(a.steps, round(want / npix)))) if stride == 1 else 1
         res = oracle_lib.render(scene, frame, integ, a.warmup, passes, threads=threads, use_bvh=True, pixel_stride=stride)
         cpu = {"value": res["stats"][1] / res["stats"][0], "unit": "path samples/s", "cores": threads, "kind": "port",
-               "sample": "every %d-th pixel of %d %dx%d pass(es) from sample index %d, %d paths, %.1f s on %d threads of %s" %
-                         (stride, passes, a.width, a.height, a.warmup, int(res["stats"][1]), res["stats"][0], threads,
-                          cpu_model())}
+               "sample": "every %d-th %s of %d %dx%d pass(es) from sample index %d, %d paths, %.1f s on %d threads of %s" %
+                         (stride, "8x8 camera workgroup" if bdpt else "pixel", passes, a.width, a.height, a.warmup,
+                          int(res["stats"][1]), res["stats"][0], threads, cpu_model())}
 
+    if bdpt:
+        metric = "path samples/s (BDPT, random shapes ~1M tris + 8 emissive triangles, 1280x720, depth 8)"
+        kernel = "k_bdpt_camera<%d> (camera subpaths with all connections)" % width
+        kernel_ms = {"camera": st_prof[2], "light": st_prof[3], "splat": st_prof[4], "headers": st_prof[11],
+                     "sort_sum_merge": st_prof[12], "render_total": st_prof[5]}
+        rays = {"camera_side": st_cnt[0], "light_side": st_cnt[1]}
+        trav = {"camera_nodes": st_cnt[6], "camera_tris": st_cnt[7], "light_nodes": st_cnt[8], "light_tris": st_cnt[9]}
+    else:
+        metric = "path samples/s (NEE, random shapes ~1M tris, 1920x1080, depth 12)"
+        kernel = "%s<%d> (closest-hit traversal)" % ("k_extend_p" if cfg.get("persistent", 1) else "k_extend", width)
+        kernel_ms = {"extend": ext_ms, "shade": st_prof[4], "shadow": st_prof[3], "start": st_prof[11],
+                     "merge": st_prof[12], "render_total": st_prof[5]}
+        rays = {"extension": st_cnt[0], "shadow": st_cnt[1]}
+        trav = {"ext_nodes": st_cnt[6], "ext_tris": st_cnt[7], "shadow_nodes": st_cnt[8], "shadow_tris": st_cnt[9]}
     out = {
-        "metric": "path samples/s (NEE, random shapes ~1M tris, 1920x1080, depth 12)",
+        "metric": metric,
         "value": value,
         "unit": "path samples/s",
         "n_gpus": 1,
@@ -161,16 +187,12 @@ def main():
                                (a.integrator.upper(), a.shapes, scene.triangle_count, a.width, a.height, a.depth),
                    "launch": dict(cfg, passes_in_flight=int(st_prof[13]))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "%s<%d> (closest-hit traversal)" % ("k_extend_p" if cfg.get("persistent", 1) else "k_extend", width),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
                      "bytes_per_launch": ext_bytes_per_launch, "ms_per_launch": ext_ms_per_launch},
         "cpu_baseline": cpu,
         "detail": {
             "scene_build_s": t_scene, "upload_and_bvh_s": t_upload,
-            "kernel_ms": {"extend": ext_ms, "shade": st_prof[4], "shadow": st_prof[3], "start": st_prof[11],
-                          "merge": st_prof[12], "render_total": st_prof[5]},
-            "rays": {"extension": st_cnt[0], "shadow": st_cnt[1]},
-            "traversal": {"ext_nodes": st_cnt[6], "ext_tris": st_cnt[7], "shadow_nodes": st_cnt[8], "shadow_tris": st_cnt[9]},
+            "kernel_ms": kernel_ms, "rays": rays, "traversal": trav,
             "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         },
     }

@@ -144,7 +144,10 @@ int ypt_get_first_hit_features(ypt_integrator* h, float* out);
  *      rays, [8]/[9] the same for shadow rays   (when ypt_set_option("count_traversal", 1));
  *  [2] extension, [3] shadow, [4] shading, [11] path start, [12] film merge kernel time in ms and
  *      [5] the whole render in ms                (when ypt_set_option("profile", 1));
- *  [10] extension launches. */
+ *  [10] extension launches.
+ * Bidirectional integrator: [0]/[1] rays traced by camera subpaths / by light subpaths and light
+ * tracing, [6]/[7] and [8]/[9] their nodes and triangles; [2] camera-subpath, [3] light-subpath,
+ * [4] light-tracing, [11] light-subpath compaction, [12] splat sort + sum + film merge times in ms. */
 int ypt_get_stats(ypt_integrator* h, double* out16);
 /* Launch and diagnostics knobs (no counterpart in the reference, whose traversal is the driver's):
  *  "bvh_width" 2 | 8, "block_size" 64..1024 (multiple of 64), "lds_stack_depth" 1..64,

@@ -16,22 +16,31 @@
 
 namespace ypt {
 
-template <int WIDTH>
+// COUNT: nodes visited, triangles tested and rays traced, for the algorithmic-byte figures
+template <int WIDTH, bool COUNT>
 struct GpuTracer {
   const Bvh2Node* nodes2;
   const Bvh8Node* nodes8;
   const TriRecord* tris;
   SceneView sv;
   TravStack st;
+  TraversalCounters cnt;
+  uint32_t rays;
   __device__ bool trace(int mode, vec3 o, vec3 d, float tmin, float tmax, uint32_t target, float& t, uint32_t& id, float& b1,
                         float& b2) {
-    TraversalCounters c = {0u, 0u};
+    if (COUNT) rays++;
     if (mode == 0) {
-      if (WIDTH == 2) return traverse_bvh2<0, false>(nodes2, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, c);
-      return traverse_bvh8<0, false>(nodes8, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, c);
+      if (WIDTH == 2) return traverse_bvh2<0, COUNT>(nodes2, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, cnt);
+      return traverse_bvh8<0, COUNT>(nodes8, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, cnt);
     }
-    if (WIDTH == 2) return traverse_bvh2<1, false>(nodes2, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, c);
-    return traverse_bvh8<1, false>(nodes8, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, c);
+    if (WIDTH == 2) return traverse_bvh2<1, COUNT>(nodes2, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, cnt);
+    return traverse_bvh8<1, COUNT>(nodes8, tris, o, d, tmin, tmax, st, t, id, b1, b2, target, cnt);
+  }
+  __device__ void flush(unsigned long long* w) {  // w[0] nodes, w[1] triangles, w[2] rays
+    if (!COUNT) return;
+    atomicAdd(&w[0], (unsigned long long)cnt.nodes);
+    atomicAdd(&w[1], (unsigned long long)cnt.tris);
+    atomicAdd(&w[2], (unsigned long long)rays);
   }
   __device__ bool closest(vec3 o, vec3 d, float tmin, float tmax, Hit& h) {
     float t = 0.f, b1 = 0.f, b2 = 0.f;
@@ -76,12 +85,16 @@ struct BdptBuffers {
   uint32_t ovfStride;
   uint32_t npaths;
   uint32_t recPerPath;     // light vertices per path - 1
+  unsigned long long* work;  // counting launches: [0..2] camera subpaths, [3..5] light subpaths and splats
 };
 
-template <int WIDTH>
-__device__ __forceinline__ GpuTracer<WIDTH> make_tracer(const DevScene& S, uint32_t* lds, const BdptBuffers& B, uint32_t gid,
-                                                        uint32_t ldsDepth) {
-  GpuTracer<WIDTH> tr;
+template <int WIDTH, bool COUNT>
+__device__ __forceinline__ GpuTracer<WIDTH, COUNT> make_tracer(const DevScene& S, uint32_t* lds, const BdptBuffers& B,
+                                                               uint32_t gid, uint32_t ldsDepth) {
+  GpuTracer<WIDTH, COUNT> tr;
+  tr.cnt.nodes = 0;
+  tr.cnt.tris = 0;
+  tr.rays = 0;
   tr.nodes2 = S.nodes2;
   tr.nodes8 = S.nodes8;
   tr.tris = S.tris;
@@ -97,16 +110,17 @@ __device__ __forceinline__ GpuTracer<WIDTH> make_tracer(const DevScene& S, uint3
   return tr;
 }
 
-template <int WIDTH>
+template <int WIDTH, bool COUNT>
 __global__ __launch_bounds__(64) void k_bdpt_light(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, uint32_t ldsDepth) {
   extern __shared__ uint32_t lds[];
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= B.npaths) return;
-  GpuTracer<WIDTH> tr = make_tracer<WIDTH>(S, lds, B, p, ldsDepth);
+  GpuTracer<WIDTH, COUNT> tr = make_tracer<WIDTH, COUNT>(S, lds, B, p, ldsDepth);
   float lambda;
   uint32_t c = trace_light_path(S.sv, fc, point, p, tr, B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES, lambda);
   B.counts[p] = c;
   B.lambdas[p] = lambda;
+  tr.flush(B.work + 3);
 }
 
 // one workgroup of 1024 threads: subpaths with vertices, compacted in index order
@@ -136,7 +150,7 @@ __global__ void k_bdpt_headers(BdptBuffers B) {
 }
 
 // 64 threads per workgroup = one 8x8 dispatch tile
-template <int WIDTH>
+template <int WIDTH, bool COUNT>
 __global__ __launch_bounds__(64) void k_bdpt_camera(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, float* features,
                               uint32_t ldsDepth) {
   extern __shared__ uint32_t lds[];
@@ -162,12 +176,13 @@ __global__ __launch_bounds__(64) void k_bdpt_camera(FrameConstants fc, DevScene 
   uint32_t q = B.perm[dy * W + dx];
   uint32_t px = q & 0xFFFFu, py = q >> 16;
   uint32_t pix = py * W + px;
-  GpuTracer<WIDTH> tr = make_tracer<WIDTH>(S, lds, B, g * blockDim.x + t, ldsDepth);
+  GpuTracer<WIDTH, COUNT> tr = make_tracer<WIDTH, COUNT>(S, lds, B, g * blockDim.x + t, ldsDepth);
   FirstHitFeatures fh;
   fh.hit = 0u;
   Spectrum L = trace_camera_path(S.sv, fc, point, px, py, lambda, nodes, count, hc == 0 && t == 0, tr, features ? &fh : nullptr);
   vec3 xyz = spectrum_to_xyz(L, spectral_lambdas(lambda), S.sv.spec);
   B.camXYZ[pix] = make_float4(xyz.x, xyz.y, xyz.z, lambda);
+  tr.flush(B.work);
   if (features) {
     float* f = features + (size_t)pix * 12;
     if (fh.hit) {
@@ -180,7 +195,7 @@ __global__ __launch_bounds__(64) void k_bdpt_camera(FrameConstants fc, DevScene 
   }
 }
 
-template <int WIDTH>
+template <int WIDTH, bool COUNT>
 __global__ __launch_bounds__(64) void k_bdpt_splat(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, uint32_t ldsDepth) {
   extern __shared__ uint32_t lds[];
   uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,8 +208,9 @@ __global__ __launch_bounds__(64) void k_bdpt_splat(FrameConstants fc, DevScene S
   if (h < hc) {
     uint32_t p = B.headers[h];
     n = B.counts[p];
-    GpuTracer<WIDTH> tr = make_tracer<WIDTH>(S, lds, B, h, ldsDepth);
+    GpuTracer<WIDTH, COUNT> tr = make_tracer<WIDTH, COUNT>(S, lds, B, h, ldsDepth);
     splat_light_path(S.sv, fc, point, h, hc, B.lambdas[p], B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES, n, tr, rec);
+    tr.flush(B.work + 3);
   }
   for (uint32_t j = 0; j < B.recPerPath; ++j) {
     bool valid = j + 1u < n && rec[j].texel < npix;

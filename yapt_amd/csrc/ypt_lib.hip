@@ -395,17 +395,30 @@ void ensure_bdpt(ypt_integrator* h) {
   if (h->writeFeatures && h->features.n < (size_t)npix * 12) h->features.alloc((size_t)npix * 12);
 }
 
-template <int WIDTH>
-void bdpt_pass(ypt_integrator* h, const DevScene& S, const uint32_t* point, const BdptBuffers& B, uint32_t groups) {
+// profile spans of one pass: light, headers, camera, splat (kinds 0..3)
+template <int WIDTH, bool COUNT>
+void bdpt_pass(ypt_integrator* h, const DevScene& S, const uint32_t* point, const BdptBuffers& B, uint32_t groups,
+               std::vector<hipEvent_t>* ev) {
   hipStream_t s = h->stream;
   const FrameConstants& fc = h->fc;
   uint32_t d = (uint32_t)h->ldsDepth;
   size_t lds = (size_t)h->ldsDepth * 64 * 4;
-  hipLaunchKernelGGL((k_bdpt_light<WIDTH>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
+  auto mark = [&]() {
+    if (!ev) return;
+    hipEvent_t e = h->next_event();
+    HIP_CHECK(hipEventRecord(e, s));
+    ev->push_back(e);
+  };
+  mark();
+  hipLaunchKernelGGL((k_bdpt_light<WIDTH, COUNT>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
+  mark();
   hipLaunchKernelGGL(k_bdpt_headers, dim3(1), dim3(1024), 0, s, B);
-  hipLaunchKernelGGL((k_bdpt_camera<WIDTH>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
+  mark();
+  hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
                      h->writeFeatures ? h->features.p : (float*)nullptr, d);
-  hipLaunchKernelGGL((k_bdpt_splat<WIDTH>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
+  mark();
+  hipLaunchKernelGGL((k_bdpt_splat<WIDTH, COUNT>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
+  mark();
 }
 
 void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count) {
@@ -443,11 +456,17 @@ void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count
   FilmBuffers F = h->film();
   hipEvent_t e0 = nullptr, e1 = nullptr;
   h->evUsed = 0;
+  B.work = h->work.p;
+  if (h->countTraversal) h->work.zero(s);
+  std::vector<std::vector<hipEvent_t>> spans;
   if (h->profile) { e0 = h->next_event(); HIP_CHECK(hipEventRecord(e0, s)); }
   for (uint32_t k = 0; k < sample_count; ++k) {
     const uint32_t* point = h->sobolPoints.p + (size_t)k * YPT_SOBOL_POINT_DIMS;
-    if (h->bvhWidth == 8) bdpt_pass<8>(h, S, point, B, groups);
-    else bdpt_pass<2>(h, S, point, B, groups);
+    std::vector<hipEvent_t>* ev = nullptr;
+    if (h->profile) { spans.emplace_back(); ev = &spans.back(); }
+    bool c = h->countTraversal != 0;
+    if (h->bvhWidth == 8) { if (c) bdpt_pass<8, true>(h, S, point, B, groups, ev); else bdpt_pass<8, false>(h, S, point, B, groups, ev); }
+    else { if (c) bdpt_pass<2, true>(h, S, point, B, groups, ev); else bdpt_pass<2, false>(h, S, point, B, groups, ev); }
     size_t bytes = h->bSortBytes;
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(h->bSortTemp.p, bytes, h->bKeys.p, h->bKeysOut.p, h->bVals.p, h->bValsOut.p,
                                                  (int)nrec, 0, bits, s));
@@ -461,6 +480,11 @@ void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count
     mp.killNaN = h->frameDesc.kill_nan_samples;
     mp.cur = h->filmCur;
     hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp, (const float4*)h->bCamXYZ.p);
+    if (ev) {
+      hipEvent_t e = h->next_event();
+      HIP_CHECK(hipEventRecord(e, s));
+      ev->push_back(e);
+    }
     h->filmSamples++;
     h->filmCur ^= 1;
   }
@@ -474,7 +498,14 @@ void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count
     float all = 0.f;
     HIP_CHECK(hipEventElapsedTime(&all, e0, e1));
     h->stats[5] = all;
+    // [2] camera subpaths, [3] light subpaths, [4] light tracing splats, [11] headers, [12] sort + sum + merge
+    for (const auto& v : spans) {
+      float t[5];
+      for (int i = 0; i < 5; ++i) HIP_CHECK(hipEventElapsedTime(&t[i], v[i], v[i + 1]));
+      h->stats[3] += t[0]; h->stats[11] += t[1]; h->stats[2] += t[2]; h->stats[4] += t[3]; h->stats[12] += t[4];
+    }
   }
+  h->stats[10] = sample_count;  // camera-subpath launches
   h->stats[13] = 1;
 }
 
@@ -756,8 +787,13 @@ int ypt_get_stats(ypt_integrator* h, double* out16) {
     if (h->countTraversal && h->work.n) {
       unsigned long long w[8] = {0};
       HIP_CHECK(hipMemcpy(w, h->work.p, sizeof(w), hipMemcpyDeviceToHost));
-      h->stats[6] = (double)w[0]; h->stats[7] = (double)w[1]; h->stats[8] = (double)w[2]; h->stats[9] = (double)w[3];
-      h->stats[0] = (double)w[4]; h->stats[1] = (double)w[5];
+      if (h->type == YPT_INTEGRATOR_BDPT) {  // camera subpaths | light subpaths and splats
+        h->stats[6] = (double)w[0]; h->stats[7] = (double)w[1]; h->stats[0] = (double)w[2];
+        h->stats[8] = (double)w[3]; h->stats[9] = (double)w[4]; h->stats[1] = (double)w[5];
+      } else {
+        h->stats[6] = (double)w[0]; h->stats[7] = (double)w[1]; h->stats[8] = (double)w[2]; h->stats[9] = (double)w[3];
+        h->stats[0] = (double)w[4]; h->stats[1] = (double)w[5];
+      }
     }
     std::memcpy(out16, h->stats, sizeof(h->stats));
     return YPT_OK;

@@ -11,6 +11,7 @@
 Material presets follow MaterialProxy.cpp:15-52 and the defaults MaterialInternal.h:41-75.
 """
 import math
+import os
 
 import numpy as np
 
@@ -281,9 +282,22 @@ def cornell_box():
     return b.arrays(), cam
 
 
+def blackbody_rgb(kelvin):
+    """Linear sRGB of a blackbody (Planck's law over 360-830 nm against the CIE 1931 observer), scaled
+    so that its largest component is 1."""
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "cie_1931_2deg_d65.npz"))
+    lam = d["lambda"].astype(np.float64) * 1e-9
+    h, c, k = 6.62607015e-34, 2.99792458e8, 1.380649e-23
+    planck = 2 * h * c * c / lam ** 5 / np.expm1(h * c / (lam * k * kelvin))
+    xyz = np.array([np.sum(planck * d[a]) for a in ("x", "y", "z")])
+    rgb = np.maximum(d["srgb_xyz_to_rgb"] @ xyz, 0.0)
+    return rgb / rgb.max()
+
+
 def random_shapes(n_shapes=2000, seed=2, cube=20.0, n_lights=4, light_emission=80.0, sphere_res=(20, 20),
-                  torus_res=(32, 12)):
-    """~1M-triangle random-shapes scene (n_shapes=2000, seed 2)."""
+                  torus_res=(32, 12), emissive_triangles=0, triangle_emission=200.0):
+    """~1M-triangle random-shapes scene (n_shapes=2000, seed 2). emissive_triangles > 0 adds that many
+    seeded two-sided emissive triangles (edge ~1.5) coloured as blackbodies of 2700-9000 K."""
     rng = np.random.default_rng(seed)
     b = MeshBuilder()
     templates = [sphere_template(*sphere_res), box_template(), torus_template(*torus_res)]
@@ -328,5 +342,24 @@ def random_shapes(n_shapes=2000, seed=2, cube=20.0, n_lights=4, light_emission=8
         cx = (-0.5 + (i % 2)) * cube * 0.5 - side / 2
         cz = (-0.5 + (i // 2)) * cube * 0.5 - side / 2
         b.quad((cx, cube * 0.6, cz), (side, 0, 0), (0, 0, side), light)  # facing -y
+    if emissive_triangles:
+        er = np.random.default_rng(seed + 1000)
+        for _ in range(emissive_triangles):
+            em = preset_material("PLASTIC")
+            em.albedo[:] = (0.5, 0.5, 0.5)
+            em.specular_amount = 0.0
+            em.roughness = 1.0
+            em.material_mask = _lib.YPT_MATERIAL_TWO_SIDED
+            em.emissive[:] = tuple(float(v) for v in blackbody_rgb(er.uniform(2700.0, 9000.0)) * triangle_emission)
+            mid = b.material(em)
+            c = er.uniform(-cube / 2, cube / 2, size=3)
+            e1, e2 = er.normal(size=3), er.normal(size=3)
+            e1 *= 1.5 / np.linalg.norm(e1)
+            e2 -= e1 * (e2 @ e1) / (e1 @ e1)
+            e2 *= 1.5 / np.linalg.norm(e2)
+            P = np.array([c, c + e1, c + e2], np.float32)
+            n = np.cross(e1, e2)
+            n /= np.linalg.norm(n)
+            b.add(P, np.tile(n, (3, 1)).astype(np.float32), np.array([[0, 1, 2]]), mid)
     cam = Camera(eye=(0.0, cube * 0.35, cube * 1.35), target=(0.0, -cube * 0.1, 0.0), fovy=math.pi * 0.3)
     return b.arrays(), cam
