@@ -174,6 +174,39 @@ int ypt_sobol_point(uint32_t sample_index, uint32_t* out256);
 const char* ypt_last_error(void);
 const char* ypt_version(void);
 
+/* ---- Resident scenes: built once, shared by any number of integrator handles and render jobs.
+ * The reference keeps one scene per renderer: the bindless mesh/material buffers and the
+ * acceleration structures of AccelerationStructureProvider (PathTraceStage.cpp:215-251,
+ * PathIntegratorSubStage::sceneChanged, PathIntegratorSubStage.h:62-71). */
+typedef struct ypt_scene ypt_scene;
+int ypt_scene_create(const ypt_scene_desc* scene, int device, ypt_scene** out);
+/* the device memory is released when no handle or job uses the scene any more */
+int ypt_scene_destroy(ypt_scene* scene);
+int ypt_scene_info(const ypt_scene* scene, uint64_t* device_bytes, uint32_t* triangles);
+/* like ypt_set_scene, without a copy or a BVH build */
+int ypt_set_resident_scene(ypt_integrator* h, ypt_scene* scene);
+
+/* ---- Render server: continuous batching of progressive render jobs (no counterpart in the
+ * reference, whose renderer renders one view; the jobs run the same integrators).
+ * Jobs are admitted in arrival order while at most max_active_jobs run and their estimated device
+ * memory stays within hbm_budget_bytes; each active job keeps its next chunk of passes in flight on
+ * its own stream, so jobs overlap on the device. A job's film depends only on its scene, frame and
+ * sample count (Sobol indices 0 .. spp-1), alone or in any mix. */
+typedef struct ypt_server ypt_server;
+int ypt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_active_jobs, ypt_server** out);
+int ypt_server_destroy(ypt_server* server);
+int ypt_server_submit(ypt_server* server, ypt_scene* scene, int integrator_type, const ypt_frame_desc* frame,
+                      uint32_t spp, uint64_t* job_id);
+/* YPT_OK when the job has finished, 1 when timeout_s (< 0: no limit) passed first */
+int ypt_server_wait(ypt_server* server, uint64_t job_id, double timeout_s);
+/* film of a finished job (W*H float4 X, Y, Z, 1; may be NULL) and times3 = submit, admit, finish in
+ * seconds of ypt_clock (may be NULL); the job is forgotten afterwards */
+int ypt_server_take(ypt_server* server, uint64_t job_id, float* xyzw, double* times3);
+/* [0] queued, [1] active, [2] completed, [3] device bytes of active jobs, [4] peak active, [5] budget */
+int ypt_server_stats(ypt_server* server, double* out6);
+/* the steady clock of the job times, in seconds */
+double ypt_clock(void);
+
 #ifdef __cplusplus
 }
 #endif

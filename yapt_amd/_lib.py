@@ -131,6 +131,18 @@ def load():
     lib.ypt_sobol_point.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
     lib.ypt_last_error.restype = ctypes.c_char_p
     lib.ypt_version.restype = ctypes.c_char_p
+    lib.ypt_scene_create.argtypes = [ctypes.POINTER(SceneDesc), ctypes.c_int, ctypes.POINTER(P)]
+    lib.ypt_scene_destroy.argtypes = [P]
+    lib.ypt_scene_info.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32)]
+    lib.ypt_set_resident_scene.argtypes = [P, P]
+    lib.ypt_server_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(P)]
+    lib.ypt_server_destroy.argtypes = [P]
+    lib.ypt_server_submit.argtypes = [P, P, ctypes.c_int, ctypes.POINTER(FrameDesc), ctypes.c_uint32,
+                                      ctypes.POINTER(ctypes.c_uint64)]
+    lib.ypt_server_wait.argtypes = [P, ctypes.c_uint64, ctypes.c_double]
+    lib.ypt_server_take.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]
+    lib.ypt_server_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+    lib.ypt_clock.restype = ctypes.c_double
     _LIB = lib
     return lib
 
@@ -139,6 +151,9 @@ EXPORTED_SYMBOLS = [
     "ypt_create", "ypt_destroy", "ypt_set_scene", "ypt_set_frame", "ypt_clear_film", "ypt_render",
     "ypt_synchronize", "ypt_get_film", "ypt_get_last_samples", "ypt_get_first_hit_features", "ypt_get_stats",
     "ypt_set_option", "ypt_trace_closest", "ypt_sample_dims", "ypt_sobol_point", "ypt_last_error", "ypt_version",
+    "ypt_scene_create", "ypt_scene_destroy", "ypt_scene_info", "ypt_set_resident_scene",
+    "ypt_server_create", "ypt_server_destroy", "ypt_server_submit", "ypt_server_wait", "ypt_server_take",
+    "ypt_server_stats", "ypt_clock",
 ]
 
 
@@ -173,7 +188,10 @@ class Integrator:
         check(self.lib.ypt_set_option(self.h, name.encode(), int(value)))
 
     def set_scene(self, scene):
-        check(self.lib.ypt_set_scene(self.h, ctypes.byref(scene.desc)))
+        if isinstance(scene, ResidentScene):
+            check(self.lib.ypt_set_resident_scene(self.h, scene.h))
+        else:
+            check(self.lib.ypt_set_scene(self.h, ctypes.byref(scene.desc)))
         self._scene = scene
 
     def set_frame(self, frame):
@@ -231,3 +249,80 @@ def sobol_point(index):
     out = np.zeros(256, np.uint32)
     check(load().ypt_sobol_point(int(index), _uptr(out)))
     return out
+
+
+class ResidentScene:
+    """A scene built once on a device (BVH and buffers) and shared by integrators and render jobs."""
+
+    def __init__(self, scene, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.ypt_scene_create(ctypes.byref(scene.desc), int(device), ctypes.byref(h)))
+        self.h = h
+        self.device = device
+        nbytes, ntris = ctypes.c_uint64(), ctypes.c_uint32()
+        check(self.lib.ypt_scene_info(self.h, ctypes.byref(nbytes), ctypes.byref(ntris)))
+        self.device_bytes, self.triangle_count = nbytes.value, ntris.value
+
+    def close(self):
+        if self.h:
+            self.lib.ypt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Render server: continuous batching of progressive jobs over resident scenes (C ABI ypt_server_*)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=64 << 30, max_active_jobs=8):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.ypt_server_create(int(device), int(hbm_budget_bytes), int(max_active_jobs), ctypes.byref(h)))
+        self.h = h
+        self._frames = {}
+
+    def submit(self, scene, integrator, frame, spp):
+        jid = ctypes.c_uint64()
+        check(self.lib.ypt_server_submit(self.h, scene.h, int(integrator), ctypes.byref(frame), int(spp), ctypes.byref(jid)))
+        self._frames[jid.value] = (frame.width, frame.height)
+        return jid.value
+
+    def wait(self, job, timeout=-1.0):
+        rc = self.lib.ypt_server_wait(self.h, int(job), float(timeout))
+        if rc == 1:
+            return False
+        check(rc)
+        return True
+
+    def take(self, job):
+        """Film (H x W x 4) and (submit, admit, finish) times of a finished job."""
+        w, h = self._frames.pop(job)
+        film = np.zeros((h, w, 4), np.float32)
+        times = np.zeros(3, np.float64)
+        check(self.lib.ypt_server_take(self.h, int(job), _fptr(film), times.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return film, times
+
+    def stats(self):
+        out = np.zeros(6, np.float64)
+        check(self.lib.ypt_server_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return {"queued": int(out[0]), "active": int(out[1]), "completed": int(out[2]), "reserved_bytes": out[3],
+                "peak_active": int(out[4]), "budget_bytes": out[5]}
+
+    def clock(self):
+        return self.lib.ypt_clock()
+
+    def close(self):
+        if self.h:
+            self.lib.ypt_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

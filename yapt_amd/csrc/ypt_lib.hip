@@ -2,8 +2,13 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/device/device_radix_sort.hpp>
 
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
+#include <deque>
+#include <thread>
+#include <unordered_map>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -57,6 +62,11 @@ struct DevBuf {
     HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
     n = count;
   }
+  // grow-only: keeps the buffer when it is large enough, so no hipFree happens on a size change
+  void ensure(size_t count) {
+    if (n >= count) return;
+    alloc(count);
+  }
   void upload(const T* src, size_t count, hipStream_t s) {
     if (count > n) alloc(count);
     if (count) HIP_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
@@ -91,17 +101,13 @@ uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b)
 
 }  // namespace
 
-struct ypt_integrator {
-  int type = YPT_INTEGRATOR_NEE;
+// A scene resident in device memory: host copy, BVH and device buffers. Integrator handles and the
+// render server share it (the reference's AccelerationStructureProvider + bindless scene buffers).
+struct ResidentScene {
   int device = 0;
-  hipStream_t stream = nullptr;
-  bool ownStream = false;
-
-  // scene
   PreparedScene scene;
   BvhResult bvh;
-  bool hasScene = false;
-  DevBuf<float> dPos, dNrm, dTan, dMsl, dCie, dD65, dRgb2spec;
+  DevBuf<float> dPos, dNrm, dTan;
   DevBuf<uint32_t> dIdx, dTriInst;
   DevBuf<InstanceGPU> dInst;
   DevBuf<MaterialGPU> dMats;
@@ -110,6 +116,23 @@ struct ypt_integrator {
   DevBuf<Bvh8Node> dNodes8;
   DevBuf<TriRecord> dTris;
   uint32_t stackBound2 = BVH_MAX_DEPTH, stackBound8 = 8 * BVH_MAX_DEPTH;
+  size_t device_bytes() const {
+    return dPos.n * 4 + dNrm.n * 4 + dTan.n * 4 + dIdx.n * 4 + dTriInst.n * 4 + dInst.n * sizeof(InstanceGPU) +
+           dMats.n * sizeof(MaterialGPU) + dLights.n * sizeof(LightGPU) + dNodes2.n * sizeof(Bvh2Node) +
+           dNodes8.n * sizeof(Bvh8Node) + dTris.n * sizeof(TriRecord);
+  }
+};
+
+struct ypt_integrator {
+  int type = YPT_INTEGRATOR_NEE;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool ownStream = false;
+
+  // scene (resident, possibly shared with other handles)
+  std::shared_ptr<ResidentScene> res;
+  bool hasScene = false;
+  DevBuf<float> dMsl, dCie, dD65, dRgb2spec;
 
   // frame
   ypt_frame_desc frameDesc;
@@ -172,16 +195,17 @@ struct ypt_integrator {
 
   DevScene dev_scene() const {
     DevScene s;
-    s.sv.pos = dPos.p; s.sv.nrm = dNrm.p; s.sv.idx = dIdx.p; s.sv.triInst = dTriInst.p;
-    s.sv.tan = scene.tan.empty() ? nullptr : dTan.p;
-    s.sv.inst = dInst.p; s.sv.mats = dMats.p; s.sv.lights = dLights.p;
+    const ResidentScene& r = *res;
+    s.sv.pos = r.dPos.p; s.sv.nrm = r.dNrm.p; s.sv.idx = r.dIdx.p; s.sv.triInst = r.dTriInst.p;
+    s.sv.tan = r.scene.tan.empty() ? nullptr : r.dTan.p;
+    s.sv.inst = r.dInst.p; s.sv.mats = r.dMats.p; s.sv.lights = r.dLights.p;
     s.sv.rgb2spec = dRgb2spec.n ? dRgb2spec.p : nullptr;
-    s.sv.ntris = (uint32_t)scene.triInst.size();
-    s.sv.ninst = (uint32_t)scene.inst.size();
+    s.sv.ntris = (uint32_t)r.scene.triInst.size();
+    s.sv.ninst = (uint32_t)r.scene.inst.size();
     s.sv.spec.cie_xyz4 = dCie.p;
     s.sv.spec.d65 = dD65.p;
     s.sv.luts.base = dMsl.p;
-    s.nodes2 = dNodes2.p; s.nodes8 = dNodes8.p; s.tris = dTris.p;
+    s.nodes2 = r.dNodes2.p; s.nodes8 = r.dNodes8.p; s.tris = r.dTris.p;
     return s;
   }
   PathSoA path_soa() const {
@@ -217,30 +241,24 @@ void upload_constant_tables(ypt_integrator* h) {
 
 void ensure_frame_buffers(ypt_integrator* h, uint32_t w, uint32_t hgt) {
   size_t npix = (size_t)w * hgt;
-  if (h->width == w && h->height == hgt && h->accum.n == 2 * npix) return;
-  h->counters.alloc(16);
-  h->work.alloc(8);
-  h->accum.alloc(2 * npix); h->merged0.alloc(npix); h->merged1.alloc(npix); h->lastSample.alloc(npix);
-  h->counts0.alloc(npix); h->counts1.alloc(npix); h->variance.alloc(npix);
+  h->counters.ensure(16);
+  h->work.ensure(8);
+  h->accum.ensure(2 * npix); h->merged0.ensure(npix); h->merged1.ensure(npix); h->lastSample.ensure(npix);
+  h->counts0.ensure(npix); h->counts1.ensure(npix); h->variance.ensure(npix);
   h->width = w;
   h->height = hgt;
-  h->slots = 0;
-  h->ovf.release();
-  h->ovfStride = 0;
 }
 
 // path-state and queue buffers for `slots` paths in flight
 void ensure_slots(ypt_integrator* h, size_t slots) {
   if (h->slots >= slots) return;
-  h->pOrg.alloc(slots); h->pDir.alloc(slots); h->pThr.alloc(slots); h->pLight.alloc(slots);
-  h->pSt0.alloc(slots); h->pSt1.alloc(slots); h->pVolIor.alloc(slots); h->pVolMat.alloc(slots); h->pVolAbs.alloc(slots * 4);
-  h->hit.alloc(slots);
-  h->rayQ0.alloc(slots); h->rayQ1.alloc(slots);
-  h->sOrg.alloc(slots); h->sDir.alloc(slots); h->sContrib.alloc(slots); h->sSlot.alloc(slots);
-  h->features.alloc(slots * 12);
+  h->pOrg.ensure(slots); h->pDir.ensure(slots); h->pThr.ensure(slots); h->pLight.ensure(slots);
+  h->pSt0.ensure(slots); h->pSt1.ensure(slots); h->pVolIor.ensure(slots); h->pVolMat.ensure(slots); h->pVolAbs.ensure(slots * 4);
+  h->hit.ensure(slots);
+  h->rayQ0.ensure(slots); h->rayQ1.ensure(slots);
+  h->sOrg.ensure(slots); h->sDir.ensure(slots); h->sContrib.ensure(slots); h->sSlot.ensure(slots);
+  h->features.ensure(slots * 12);
   h->slots = slots;
-  h->ovf.release();
-  h->ovfStride = 0;
 }
 
 // traversal-stack overflow: one column per thread that can traverse. A thread's id is below the
@@ -249,11 +267,9 @@ void ensure_slots(ypt_integrator* h, size_t slots) {
 void ensure_overflow(ypt_integrator* h) {
   size_t stride = h->slots + 1024;
   if (h->persistent) stride = std::min(stride, (size_t)h->cus * h->persistentWaves * 64);
-  uint32_t bound = h->bvhWidth == 8 ? h->stackBound8 : h->stackBound2;
+  uint32_t bound = h->bvhWidth == 8 ? h->res->stackBound8 : h->res->stackBound2;
   size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
-  size_t need = depth * stride;
-  if (h->ovf.n >= need && h->ovfStride == stride) return;
-  h->ovf.alloc(need);
+  h->ovf.ensure(depth * stride);
   h->ovfStride = (uint32_t)stride;
 }
 
@@ -359,6 +375,41 @@ void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
 }
 
 
+
+// prepare, build the BVH and upload: a scene resident on `device`
+std::shared_ptr<ResidentScene> make_resident(const ypt_scene_desc& d, int device, hipStream_t s) {
+  auto r = std::make_shared<ResidentScene>();
+  r->device = device;
+  prepare_scene(d, host_tables().r2s, r->scene);
+  BvhBuilder().build(r->scene.pos.data(), r->scene.idx.data(), (uint32_t)r->scene.triInst.size(), r->bvh);
+  r->stackBound2 = r->bvh.stats.maxDepth + 1;
+  r->stackBound8 = bvh8_stack_bound(r->bvh.nodes8);
+  const PreparedScene& ps = r->scene;
+  r->dPos.upload(ps.pos.data(), ps.pos.size(), s);
+  r->dNrm.upload(ps.nrm.data(), ps.nrm.size(), s);
+  if (!ps.tan.empty()) r->dTan.upload(ps.tan.data(), ps.tan.size(), s);
+  r->dIdx.upload(ps.idx.data(), ps.idx.size(), s);
+  r->dTriInst.upload(ps.triInst.data(), ps.triInst.size(), s);
+  r->dInst.upload(ps.inst.data(), ps.inst.size(), s);
+  r->dMats.upload(ps.mats.data(), ps.mats.size(), s);
+  if (!ps.lights.empty()) r->dLights.upload(ps.lights.data(), ps.lights.size(), s);
+  else { LightGPU dummy; std::memset(&dummy, 0, sizeof(dummy)); r->dLights.upload(&dummy, 1, s); }
+  r->dNodes2.upload(r->bvh.nodes2.data(), r->bvh.nodes2.size(), s);
+  r->dNodes8.upload(r->bvh.nodes8.data(), r->bvh.nodes8.size(), s);
+  r->dTris.upload(r->bvh.tris.data(), r->bvh.tris.size(), s);
+  HIP_CHECK(hipStreamSynchronize(s));
+  return r;
+}
+
+void attach_scene(ypt_integrator* h, std::shared_ptr<ResidentScene> r) {
+  h->res = std::move(r);
+  h->hasScene = true;
+  if (h->hasFrame) {
+    prepare_frame(h->frameDesc, h->res->scene, (uint32_t)h->type, h->fc);
+    clear_film(h);
+  }
+}
+
 // ---------------------------------------------------------------- bidirectional integrator
 void ensure_bdpt(ypt_integrator* h) {
   const FrameConstants& fc = h->fc;
@@ -367,32 +418,24 @@ void ensure_bdpt(ypt_integrator* h) {
   uint32_t npaths = fc.lightPathsPerDim[0] * fc.lightPathsPerDim[1];
   size_t nrec = (size_t)npaths * (BDPT_MAX_LIGHT_VERTICES - 1);
   if (h->bW != W || h->bH != H) {
-    h->bNodes.alloc((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
-    h->bCounts.alloc(npaths); h->bLambdas.alloc(npaths); h->bHeaders.alloc(npaths); h->bHeaderCount.alloc(1);
-    h->bCamXYZ.alloc(npix);
-    h->bKeys.alloc(nrec); h->bVals.alloc(nrec); h->bKeysOut.alloc(nrec); h->bValsOut.alloc(nrec); h->bRecXYZ.alloc(nrec);
+    h->bNodes.ensure((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
+    h->bCounts.ensure(npaths); h->bLambdas.ensure(npaths); h->bHeaders.ensure(npaths); h->bHeaderCount.ensure(1);
+    h->bCamXYZ.ensure(npix);
+    h->bKeys.ensure(nrec); h->bVals.ensure(nrec); h->bKeysOut.ensure(nrec); h->bValsOut.ensure(nrec); h->bRecXYZ.ensure(nrec);
     std::vector<uint32_t> perm(npix);
     texel_permutation(W, H, 0x59415054ull, perm.data());
     h->bPerm.upload(perm.data(), perm.size(), h->stream);
-    size_t bytes = 0;
-    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, h->bKeys.p, h->bKeysOut.p, h->bVals.p, h->bValsOut.p,
-                                                 (int)nrec, 0, 32, h->stream));
-    h->bSortTemp.alloc(bytes);
-    h->bSortBytes = bytes;
     h->bW = W;
     h->bH = H;
-    h->bOvf.release();
     HIP_CHECK(hipStreamSynchronize(h->stream));
   }
   uint32_t groups = ((W + BDPT_TILE - 1) / BDPT_TILE) * ((H + BDPT_TILE - 1) / BDPT_TILE);
   uint32_t stride = groups * 64u;
-  uint32_t bound = h->bvhWidth == 8 ? h->stackBound8 : h->stackBound2;
+  uint32_t bound = h->bvhWidth == 8 ? h->res->stackBound8 : h->res->stackBound2;
   size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
-  if (h->bOvf.n < depth * stride || h->bOvfStride != stride) {
-    h->bOvf.alloc(depth * stride);
-    h->bOvfStride = stride;
-  }
-  if (h->writeFeatures && h->features.n < (size_t)npix * 12) h->features.alloc((size_t)npix * 12);
+  h->bOvf.ensure(depth * stride);
+  h->bOvfStride = stride;
+  if (h->writeFeatures) h->features.ensure((size_t)npix * 12);
 }
 
 // profile spans of one pass: light, headers, camera, splat (kinds 0..3)
@@ -599,34 +642,7 @@ int ypt_set_scene(ypt_integrator* h, const ypt_scene_desc* d) {
   if (!h || !d) return fail(YPT_E_INVALID, "null argument");
   return guarded([&]() -> int {
     HIP_CHECK(hipSetDevice(h->device));
-    PreparedScene ps;
-    prepare_scene(*d, host_tables().r2s, ps);
-    BvhResult bvh;
-    BvhBuilder().build(ps.pos.data(), ps.idx.data(), (uint32_t)ps.triInst.size(), bvh);
-    h->scene = std::move(ps);
-    h->bvh = std::move(bvh);
-    h->stackBound2 = h->bvh.stats.maxDepth + 1;
-    h->stackBound8 = bvh8_stack_bound(h->bvh.nodes8);
-    hipStream_t s = h->stream;
-    h->dPos.upload(h->scene.pos.data(), h->scene.pos.size(), s);
-    h->dNrm.upload(h->scene.nrm.data(), h->scene.nrm.size(), s);
-    if (!h->scene.tan.empty()) h->dTan.upload(h->scene.tan.data(), h->scene.tan.size(), s);
-    h->dIdx.upload(h->scene.idx.data(), h->scene.idx.size(), s);
-    h->dTriInst.upload(h->scene.triInst.data(), h->scene.triInst.size(), s);
-    h->dInst.upload(h->scene.inst.data(), h->scene.inst.size(), s);
-    h->dMats.upload(h->scene.mats.data(), h->scene.mats.size(), s);
-    if (!h->scene.lights.empty()) h->dLights.upload(h->scene.lights.data(), h->scene.lights.size(), s);
-    else { LightGPU dummy; std::memset(&dummy, 0, sizeof(dummy)); h->dLights.upload(&dummy, 1, s); }
-    h->dNodes2.upload(h->bvh.nodes2.data(), h->bvh.nodes2.size(), s);
-    h->dNodes8.upload(h->bvh.nodes8.data(), h->bvh.nodes8.size(), s);
-    h->dTris.upload(h->bvh.tris.data(), h->bvh.tris.size(), s);
-    HIP_CHECK(hipStreamSynchronize(s));
-    h->hasScene = true;
-    h->ovf.release();
-    if (h->hasFrame) {
-      prepare_frame(h->frameDesc, h->scene, (uint32_t)h->type, h->fc);
-      clear_film(h);
-    }
+    attach_scene(h, make_resident(*d, h->device, h->stream));
     return YPT_OK;
   });
 }
@@ -637,7 +653,7 @@ int ypt_set_frame(ypt_integrator* h, const ypt_frame_desc* f) {
     if (!h->hasScene) return fail(YPT_E_NOSCENE, "set_scene must come before set_frame");
     HIP_CHECK(hipSetDevice(h->device));
     FrameConstants fc;
-    prepare_frame(*f, h->scene, (uint32_t)h->type, fc);
+    prepare_frame(*f, h->res->scene, (uint32_t)h->type, fc);
     if (f->env_intensity_scale != 0.f && !(f->flags & YPT_FLAG_WHITE_FURNACE) && h->dRgb2spec.n == 0) {
       std::vector<float> table;
       host_tables().r2s.full_table(table, std::max(1u, std::thread::hardware_concurrency()));
@@ -849,7 +865,7 @@ int ypt_trace_closest(ypt_integrator* h, const float* rays, uint32_t n, float* t
     hipStream_t s = h->stream;
     dr.upload(rays, (size_t)n * 8, s);
     dt.alloc(n); db.alloc(2 * (size_t)n); dtri.alloc(n);
-    uint32_t bound = h->bvhWidth == 8 ? h->stackBound8 : h->stackBound2;
+    uint32_t bound = h->bvhWidth == 8 ? h->res->stackBound8 : h->res->stackBound2;
     size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
     dovf.alloc(depth * n);
     DevScene S = h->dev_scene();
@@ -888,5 +904,349 @@ int ypt_sample_dims(ypt_integrator* h, uint32_t sample_index, const uint32_t* se
     return YPT_OK;
   });
 }
+
+
+// ---------------------------------------------------------------- resident scenes
+struct ypt_scene {
+  std::shared_ptr<ResidentScene> r;
+};
+
+int ypt_scene_create(const ypt_scene_desc* d, int device, ypt_scene** out) {
+  if (!d || !out) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    int n = 0;
+    HIP_CHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(YPT_E_INVALID, "device index out of range");
+    HIP_CHECK(hipSetDevice(device));
+    hipStream_t s;
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::unique_ptr<ypt_scene> sc(new ypt_scene());
+    try {
+      sc->r = make_resident(*d, device, s);
+    } catch (...) {
+      (void)hipStreamDestroy(s);
+      throw;
+    }
+    HIP_CHECK(hipStreamDestroy(s));
+    *out = sc.release();
+    return YPT_OK;
+  });
+}
+
+int ypt_scene_destroy(ypt_scene* sc) {
+  delete sc;  // device memory goes when the last handle using the scene lets go of it
+  return YPT_OK;
+}
+
+int ypt_scene_info(const ypt_scene* sc, uint64_t* device_bytes, uint32_t* triangles) {
+  if (!sc) return fail(YPT_E_INVALID, "null scene");
+  if (device_bytes) *device_bytes = sc->r->device_bytes();
+  if (triangles) *triangles = (uint32_t)sc->r->scene.triInst.size();
+  return YPT_OK;
+}
+
+int ypt_set_resident_scene(ypt_integrator* h, ypt_scene* sc) {
+  if (!h || !sc) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (sc->r->device != h->device) return fail(YPT_E_INVALID, "scene and integrator are on different devices");
+    HIP_CHECK(hipSetDevice(h->device));
+    attach_scene(h, sc->r);
+    return YPT_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- render server
+// Continuous batching of progressive render jobs over resident scenes. A worker thread admits
+// queued jobs in arrival order while the number of active jobs and their estimated device memory
+// fit, and keeps every active job's next chunk of passes in flight on that job's own stream, so
+// the kernels of different jobs overlap on the device. A finished job's film is copied out, its
+// handle goes back to a pool (buffers only grow, so no allocation or free runs while others render)
+// and the next queued job is admitted. A job's result depends only on its scene, frame and sample
+// count: it is the same alone or in any mix.
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct ServerJob {
+  uint64_t id = 0;
+  std::shared_ptr<ResidentScene> scene;
+  int type = 0;
+  ypt_frame_desc frame;
+  uint32_t spp = 0, chunk = 1, passesInFlight = 1;
+  uint32_t issued = 0;
+  uint64_t mem = 0;
+  ypt_integrator* h = nullptr;
+  hipEvent_t ev = nullptr;
+  bool inFlight = false;
+  std::atomic<int> state{0};  // 0 queued, 1 active, 2 done, 3 failed
+  std::string err;
+  double tSubmit = 0, tAdmit = 0, tFinish = 0;
+  std::vector<float> film;
+};
+
+// passes per batch and the path-state estimate of a job (path slots, queues, stack overflow, film)
+void plan_job(ServerJob& j, int cus) {
+  uint64_t npix = (uint64_t)j.frame.width * j.frame.height;
+  uint64_t bytes = npix * (16 * 2 + 16 * 2 + 4 * 2 + 4 + 16);
+  if (j.type == YPT_INTEGRATOR_BDPT) {
+    uint64_t npaths = ((j.frame.width + 7) / 8) * (uint64_t)((j.frame.height + 7) / 8);
+    bytes += npaths * BDPT_MAX_LIGHT_VERTICES * sizeof(LightNodePacked) + npaths * 15 * (4 * 4 + 16) + npix * (16 + 4) +
+             npaths * 64 * 48 * 4;
+    j.passesInFlight = 1;
+    j.chunk = std::min<uint32_t>(j.spp, 2);
+  } else {
+    uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(16, (8ull << 20) / std::max<uint64_t>(npix, 1)));
+    g = std::min(g, j.spp);
+    uint64_t slots = npix * g;
+    bytes += slots * (6 * 16 + 16 + 16 + 64 + 16 + 8 + 52 + 48) + (uint64_t)cus * 32 * 64 * 48 * 4;
+    j.passesInFlight = g;
+    j.chunk = g;
+  }
+  j.mem = bytes;
+}
+
+}  // namespace
+
+struct ypt_server {
+  int device = 0;
+  int cus = 256;
+  uint64_t budget = 0;
+  uint32_t maxActive = 1;
+  std::mutex m;
+  std::condition_variable cvDone;
+  std::deque<std::shared_ptr<ServerJob>> queued;
+  std::vector<std::shared_ptr<ServerJob>> active;
+  std::unordered_map<uint64_t, std::shared_ptr<ServerJob>> jobs;
+  std::vector<ypt_integrator*> pool[3];
+  uint64_t nextId = 1, memUsed = 0, completed = 0, peakActive = 0;
+  bool stop = false;
+  std::thread worker;
+
+  ypt_integrator* take_handle(int type) {
+    if (!pool[type].empty()) {
+      ypt_integrator* h = pool[type].back();
+      pool[type].pop_back();
+      return h;
+    }
+    ypt_integrator* h = nullptr;
+    if (ypt_create(type, device, &h) != YPT_OK) throw std::runtime_error(std::string("server: ") + ypt_last_error());
+    return h;
+  }
+
+  void start_job(ServerJob& j) {
+    j.h = take_handle(j.type);
+    ypt_integrator* h = j.h;
+    if (!j.ev) HIP_CHECK(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
+    if (h->res != j.scene) attach_scene(h, j.scene);
+    if (ypt_set_frame(h, &j.frame) != YPT_OK) throw std::runtime_error(std::string("server: ") + ypt_last_error());
+    if (ypt_set_option(h, "passes_in_flight", j.passesInFlight) != YPT_OK) throw std::runtime_error(ypt_last_error());
+    clear_film(h);
+  }
+
+  void finish_job(ServerJob& j) {
+    size_t npix = (size_t)j.frame.width * j.frame.height;
+    j.film.resize(npix * 4);
+    if (ypt_get_film(j.h, j.film.data(), nullptr) != YPT_OK) throw std::runtime_error(ypt_last_error());
+  }
+
+  void release(ServerJob& j) {
+    if (j.h) pool[j.type].push_back(j.h);
+    j.h = nullptr;
+    if (j.ev) (void)hipEventDestroy(j.ev);
+    j.ev = nullptr;
+  }
+
+  void run() {
+    (void)hipSetDevice(device);
+    while (true) {
+      std::vector<std::shared_ptr<ServerJob>> fresh, act;
+      {
+        std::lock_guard<std::mutex> lk(m);
+        if (stop) break;
+        while (!queued.empty() && active.size() < maxActive) {
+          auto j = queued.front();
+          if (!active.empty() && memUsed + j->mem > budget) break;  // wait for memory to free up
+          queued.pop_front();
+          memUsed += j->mem;
+          j->state = 1;
+          j->tAdmit = now_s();
+          active.push_back(j);
+          fresh.push_back(j);
+        }
+        peakActive = std::max<uint64_t>(peakActive, active.size());
+        act = active;
+      }
+      std::vector<std::shared_ptr<ServerJob>> ended;
+      for (auto& j : fresh) {
+        try {
+          start_job(*j);
+        } catch (const std::exception& e) {
+          j->err = e.what();
+          j->state = 3;
+          ended.push_back(j);
+        }
+      }
+      bool progress = !fresh.empty();
+      for (auto& j : act) {
+        if (j->state != 1) continue;
+        try {
+          if (j->inFlight) {
+            hipError_t q = hipEventQuery(j->ev);
+            if (q == hipErrorNotReady) continue;
+            HIP_CHECK(q);
+            j->inFlight = false;
+          }
+          if (j->issued < j->spp) {
+            uint32_t n = std::min(j->chunk, j->spp - j->issued);
+            if (ypt_render(j->h, j->issued, n, nullptr) != YPT_OK) throw std::runtime_error(ypt_last_error());
+            HIP_CHECK(hipEventRecord(j->ev, j->h->stream));
+            j->issued += n;
+            j->inFlight = true;
+          } else {
+            finish_job(*j);
+            j->tFinish = now_s();
+            j->state = 2;
+            ended.push_back(j);
+          }
+          progress = true;
+        } catch (const std::exception& e) {
+          j->err = e.what();
+          j->state = 3;
+          ended.push_back(j);
+        }
+      }
+      if (!ended.empty()) {
+        std::lock_guard<std::mutex> lk(m);
+        for (auto& j : ended) {
+          release(*j);
+          memUsed -= j->mem;
+          if (j->state == 2) completed++;
+          for (size_t i = 0; i < active.size(); ++i)
+            if (active[i] == j) { active.erase(active.begin() + i); break; }
+        }
+        cvDone.notify_all();
+      }
+      if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+  }
+};
+
+extern "C" {
+
+int ypt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_active_jobs, ypt_server** out) {
+  if (!out || max_active_jobs == 0) return fail(YPT_E_INVALID, "null output or zero active jobs");
+  return guarded([&]() -> int {
+    int n = 0;
+    HIP_CHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(YPT_E_INVALID, "device index out of range");
+    std::unique_ptr<ypt_server> s(new ypt_server());
+    s->device = device;
+    HIP_CHECK(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, device));
+    s->budget = hbm_budget_bytes;
+    s->maxActive = max_active_jobs;
+    ypt_server* raw = s.get();
+    s->worker = std::thread([raw]() { raw->run(); });
+    *out = s.release();
+    return YPT_OK;
+  });
+}
+
+int ypt_server_destroy(ypt_server* s) {
+  if (!s) return YPT_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    s->stop = true;
+  }
+  if (s->worker.joinable()) s->worker.join();
+  for (auto& kv : s->jobs) {
+    ServerJob& j = *kv.second;
+    if (j.h) {
+      (void)hipStreamSynchronize(j.h->stream);
+      s->release(j);
+    }
+  }
+  for (auto& p : s->pool)
+    for (ypt_integrator* h : p) ypt_destroy(h);
+  delete s;
+  return YPT_OK;
+}
+
+int ypt_server_submit(ypt_server* s, ypt_scene* sc, int integrator_type, const ypt_frame_desc* frame, uint32_t spp,
+                      uint64_t* job_id) {
+  if (!s || !sc || !frame || !job_id || spp == 0) return fail(YPT_E_INVALID, "null argument or zero samples");
+  if (integrator_type < YPT_INTEGRATOR_BPT || integrator_type > YPT_INTEGRATOR_BDPT) return fail(YPT_E_INVALID, "unknown integrator type");
+  if (sc->r->device != s->device) return fail(YPT_E_INVALID, "scene is on another device");
+  if (frame->width == 0 || frame->height == 0 || frame->width > 65535 || frame->height > 65535)
+    return fail(YPT_E_INVALID, "frame size out of range");
+  auto j = std::make_shared<ServerJob>();
+  j->scene = sc->r;
+  j->type = integrator_type;
+  j->frame = *frame;
+  j->spp = spp;
+  plan_job(*j, s->cus);
+  if (j->mem > s->budget) return fail(YPT_E_MEMORY, "job needs more device memory than the server's budget");
+  std::lock_guard<std::mutex> lk(s->m);
+  j->id = s->nextId++;
+  j->tSubmit = now_s();
+  s->jobs[j->id] = j;
+  s->queued.push_back(j);
+  *job_id = j->id;
+  return YPT_OK;
+}
+
+// waits up to timeout_s (< 0: forever); YPT_OK when the job has finished, 1 when it is still running
+int ypt_server_wait(ypt_server* s, uint64_t job_id, double timeout_s) {
+  if (!s) return fail(YPT_E_INVALID, "null server");
+  std::unique_lock<std::mutex> lk(s->m);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return fail(YPT_E_INVALID, "unknown job");
+  auto j = it->second;
+  auto done = [&]() { return j->state >= 2; };
+  if (timeout_s < 0) s->cvDone.wait(lk, done);
+  else if (!s->cvDone.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return 1;
+  if (j->state == 3) return fail(YPT_E_DEVICE, "job failed: " + j->err);
+  return YPT_OK;
+}
+
+// film of a finished job (W*H float4 X, Y, Z, 1) and its times: submit, admit, finish (seconds,
+// steady clock); the job is forgotten afterwards
+int ypt_server_take(ypt_server* s, uint64_t job_id, float* xyzw, double* times3) {
+  if (!s) return fail(YPT_E_INVALID, "null server");
+  std::lock_guard<std::mutex> lk(s->m);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return fail(YPT_E_INVALID, "unknown job");
+  ServerJob& j = *it->second;
+  if (j.state < 2) return fail(YPT_E_INVALID, "job has not finished");
+  if (j.state == 3) {
+    std::string e = j.err;
+    s->jobs.erase(it);
+    return fail(YPT_E_DEVICE, "job failed: " + e);
+  }
+  if (xyzw) std::memcpy(xyzw, j.film.data(), j.film.size() * sizeof(float));
+  if (times3) { times3[0] = j.tSubmit; times3[1] = j.tAdmit; times3[2] = j.tFinish; }
+  s->jobs.erase(it);
+  return YPT_OK;
+}
+
+// [0] queued, [1] active, [2] completed, [3] device bytes reserved by active jobs, [4] peak active,
+// [5] budget
+int ypt_server_stats(ypt_server* s, double* out6) {
+  if (!s || !out6) return fail(YPT_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(s->m);
+  out6[0] = (double)s->queued.size();
+  out6[1] = (double)s->active.size();
+  out6[2] = (double)s->completed;
+  out6[3] = (double)s->memUsed;
+  out6[4] = (double)s->peakActive;
+  out6[5] = (double)s->budget;
+  return YPT_OK;
+}
+
+double ypt_clock(void) { return now_s(); }
 
 }  // extern "C"
