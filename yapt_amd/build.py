@@ -1,4 +1,5 @@
-"""In-tree build of the gfx950 library (libyapt_pt.so).
+"""In-tree build of the gfx950 libraries: the path tracer (libyapt_pt.so) and the denoiser kernels
+(libyapt_denoise.so).
 
 Flags: -ffp-contract=off and IEEE-correct fp32 division/sqrt so device arithmetic matches the host
 build bit for bit; packed-fp32 instructions disabled; -O3 for gfx950 only.
@@ -11,6 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "ypt_lib.hip")
 OUT = os.path.join(HERE, "libyapt_pt.so")
+DENOISE_SRC = os.path.join(HERE, "csrc", "denoise.hip")
+DENOISE_OUT = os.path.join(HERE, "libyapt_denoise.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -33,8 +36,8 @@ def sources():
     return out
 
 
-def build(verbose=False):
-    cmd = [HIPCC] + HIP_FLAGS + ["-o", OUT + ".tmp", SRC]
+def _compile(src, out, verbose):
+    cmd = [HIPCC] + HIP_FLAGS + ["-o", out + ".tmp", src]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -43,8 +46,13 @@ def build(verbose=False):
         raise RuntimeError("hipcc failed (%d):\n%s" % (res.returncode, errs))
     if verbose and errs.strip():
         print(errs)
-    os.replace(OUT + ".tmp", OUT)
-    return OUT
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build(verbose=False):
+    _compile(DENOISE_SRC, DENOISE_OUT, verbose)
+    return _compile(SRC, OUT, verbose)
 
 
 if __name__ == "__main__":
