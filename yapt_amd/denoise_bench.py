@@ -1,0 +1,161 @@
+"""Train the denoiser on the tracer's own renders and judge it at 1920x1080.
+
+    python -m yapt_amd.denoise_bench [--scenes 48] [--iters 3000] [--out yapt_amd/data/denoiser_weights.npz]
+
+1. Dataset: seeded random-shapes scenes at 256x256, NEE, 4 spp inputs (film + first-hit features) and
+   1024 spp targets.
+2. Training: Adam, random 128x128 crops, batch 8, cosine learning-rate decay.
+3. Judgement on a held-out scene (the 1M-triangle random-shapes scene, seed 2, 1920x1080): relMSE
+   against a 4096-spp plain render, for plain 64 spp and for 4 spp + denoiser, with the wall time of
+   each (render, and render + denoise, on the GPU).
+4. The denoiser's conv kernels: achieved TFLOP/s of one 1080p forward pass (HIP events).
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def conv_flops(net, H, W):
+    """multiply-add FLOPs of one forward pass at H x W (2 per multiply-add)"""
+    total = 0
+    L = len(net.ch)
+    for l in range(L):
+        P = (H >> l) * (W >> l)
+        for c in net.enc[l]:
+            total += 2 * P * 9 * c.cin * c.cout
+    for l in range(L - 1):
+        P = (H >> l) * (W >> l)
+        for c in net.dec[l]:
+            total += 2 * P * 9 * c.cin * c.cout
+    total += 2 * H * W * 9 * net.out.cin * net.out.cout
+    return total
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--scenes", type=int, default=48)
+    p.add_argument("--res", type=int, default=256)
+    p.add_argument("--spp-ref", type=int, default=1024)
+    p.add_argument("--iters", type=int, default=3000)
+    p.add_argument("--batch", type=int, default=8)
+    p.add_argument("--crop", type=int, default=128)
+    p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--eval-width", type=int, default=1920)
+    p.add_argument("--eval-height", type=int, default=1080)
+    p.add_argument("--eval-ref-spp", type=int, default=4096)
+    p.add_argument("--out", default=os.path.join(HERE, "data", "denoiser_weights.npz"))
+    p.add_argument("--load", default=None, help="skip training and load these weights")
+    a = p.parse_args()
+    import torch
+    from . import _lib, camera, denoiser, scenes, tuning
+
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    net = denoiser.UNet(seed=0)
+    t_data = t_train = 0.0
+    hist = []
+    if a.load:
+        st = np.load(a.load)
+        layers = [{"w": st["w%d" % k], "b": st["b%d" % k]} for k in range(len(net.layers()))]
+        net.load_state({"layers": layers})
+    else:
+        t0 = time.time()
+        data = denoiser.make_dataset(a.scenes, res=a.res, spp_in=4, spp_ref=a.spp_ref, seed=100, log=log)
+        t_data = time.time() - t0
+        t0 = time.time()
+        hist = denoiser.train(net, data, iters=a.iters, batch=a.batch, crop=a.crop, lr=a.lr, seed=1, log=log)
+        torch.cuda.synchronize()
+        t_train = time.time() - t0
+        st = net.state()
+        np.savez(a.out, **{"w%d" % k: l["w"] for k, l in enumerate(st["layers"])},
+                 **{"b%d" % k: l["b"] for k, l in enumerate(st["layers"])}, channels=np.array(net.ch))
+        del data
+
+    # held-out judgement
+    scene, cam = scenes.random_shapes(n_shapes=2000, seed=2)
+    W, H = a.eval_width, a.eval_height
+    frame = camera.make_frame(cam, W, H, max_depth=12)
+    it = _lib.Integrator(_lib.YPT_INTEGRATOR_NEE, 0)
+    for k, v in tuning.best_config("nee-shapes2000-%dx%d-d12" % (W, H)).items():
+        it.set_option(k, v)
+    it.set_option("first_hit_features", 1)
+    it.set_scene(scene)
+    it.set_frame(frame)
+    it.render(0, 4)   # warm-up
+    it.synchronize()
+    log("reference %d spp" % a.eval_ref_spp)
+    it.clear_film()
+    for s0 in range(0, a.eval_ref_spp, 256):
+        it.render(10000 + s0, 256)
+        it.synchronize()
+    ref = it.film()
+
+    it.clear_film()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it.render(0, 64)
+    it.synchronize()
+    t_plain = time.perf_counter() - t0
+    plain = it.film()
+
+    it.clear_film()
+    x_noisy = x_feat = None
+    denoiser.denoise(net, torch.from_numpy(plain).cuda(), torch.from_numpy(it.first_hit_features()).cuda())  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it.render(0, 4)
+    it.synchronize()
+    t_render4 = time.perf_counter() - t0
+    noisy = it.film()
+    feats = it.first_hit_features()
+    t1 = time.perf_counter()
+    x_noisy = torch.from_numpy(noisy).cuda()
+    x_feat = torch.from_numpy(feats).cuda()
+    out = denoiser.denoise(net, x_noisy, x_feat)
+    torch.cuda.synchronize()
+    t_denoise = time.perf_counter() - t1
+    den = out.cpu().numpy()
+
+    # conv kernels: achieved TFLOP/s over one forward pass at the evaluation size (device time)
+    x = denoiser.pack_input(x_noisy[None], x_feat[None], net.multiple)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    net.forward(x)
+    e0.record()
+    reps = 5
+    for _ in range(reps):
+        net.forward(x)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    flops = conv_flops(net, x.shape[1], x.shape[2])
+
+    r_plain = denoiser.rel_mse(plain, ref)
+    r_noisy = denoiser.rel_mse(noisy, ref)
+    r_den = denoiser.rel_mse(den, ref)
+    t_den_total = t_render4 + t_denoise
+    res = {
+        "metric": "relMSE vs %d-spp plain render, 4 spp + denoiser vs plain 64 spp (%dx%d)" % (a.eval_ref_spp, W, H),
+        "relmse": {"plain_64spp": r_plain, "plain_4spp": r_noisy, "denoised_4spp": r_den},
+        "wall_s": {"plain_64spp": t_plain, "render_4spp": t_render4, "denoise": t_denoise, "denoised_4spp_total": t_den_total},
+        "speedup_vs_plain_64spp": t_plain / t_den_total,
+        "denoised_matches_64spp": r_den <= r_plain,
+        "gain_over_4spp": r_noisy / r_den,
+        "conv_roofline": {"bound": "mfma", "achieved": flops / (fwd_ms * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                          "frac": flops / (fwd_ms * 1e-3) / 1e12 / 2500.0, "forward_ms": fwd_ms, "flops": flops},
+        "training": {"scenes": a.scenes, "res": a.res, "spp_ref": a.spp_ref, "iters": a.iters, "batch": a.batch,
+                     "crop": a.crop, "data_s": t_data, "train_s": t_train,
+                     "loss_first_last": [hist[0][1], hist[-1][1]] if hist else None},
+        "channels": list(net.ch),
+    }
+    print(json.dumps(res))
+    it.close()
+
+
+if __name__ == "__main__":
+    main()
