@@ -171,3 +171,39 @@ def test_bdpt_first_hit_features_match_nee(ylib, cornell):
         out.append(it.first_hit_features())
         it.close()
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_graph_batches_match_eager(ylib, shapes_small, integrator):
+    # captured batches replayed several times, with eager renders in between, give the eager film
+    from yapt_amd import camera
+    scene, cam = shapes_small
+    frame = camera.make_frame(cam, 64, 48, max_depth=10)
+
+    def handle(graph):
+        it = ylib.Integrator(integrator, 0)
+        it.set_option("passes_in_flight", 8)
+        it.set_option("graph", graph)
+        it.set_scene(scene)
+        it.set_frame(frame)
+        return it
+
+    a = handle(0)
+    a.render(0, 20)
+    want, want_last = a.film(), a.last_samples()
+    a.close()
+    b = handle(1)
+    b.render(0, 8)
+    b.set_option("graph", 0)
+    b.render(8, 4)
+    b.set_option("graph", 1)
+    b.render(12, 8)
+    got = b.film()
+    b.close()
+    c = handle(1)
+    c.render(0, 20)             # batches of 8, 8 and 4: two graphs
+    got_c, got_c_last = c.film(), c.last_samples()
+    c.close()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(got_c.view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(got_c_last.view(np.uint32), want_last.view(np.uint32))

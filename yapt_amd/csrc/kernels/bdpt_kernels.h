@@ -150,8 +150,9 @@ __global__ void k_bdpt_headers(BdptBuffers B) {
 }
 
 // 64 threads per workgroup = one 8x8 dispatch tile
-template <int WIDTH, bool COUNT>
-__global__ __launch_bounds__(64) void k_bdpt_camera(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, float* features,
+// MINW: waves per SIMD the register allocation must allow (1: up to 512 VGPRs, 2: 256)
+template <int WIDTH, bool COUNT, int MINW>
+__global__ __launch_bounds__(64, MINW) void k_bdpt_camera(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, float* features,
                               uint32_t ldsDepth) {
   extern __shared__ uint32_t lds[];
   uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];

@@ -386,7 +386,18 @@ struct MergeParams {
   float fireflyClampRelaxPower;
   uint32_t killNaN;
   uint32_t cur;               // which merged/count buffer is written this pass
+  uint32_t pass;              // pass index within the batch (with a device-side base, see k_merge)
 };
+
+// device-side film bookkeeping of a captured batch: dyn[0] passes merged so far, dyn[1] current buffer
+__global__ void k_set_dyn(uint32_t* dyn, uint32_t samples, uint32_t cur) {
+  dyn[0] = samples;
+  dyn[1] = cur;
+}
+__global__ void k_advance_dyn(uint32_t* dyn, uint32_t g) {
+  dyn[0] += g;
+  dyn[1] ^= (g & 1u);
+}
 
 // clampFireflySample (mergeNewSamples.comp:122-170), reading the previous pass's merged film
 __device__ vec3 clamp_firefly(const MergeParams& mp, const FilmBuffers& F, int x, int y, vec3 c, uint32_t accumulated) {
@@ -418,10 +429,15 @@ __device__ vec3 clamp_firefly(const MergeParams& mp, const FilmBuffers& F, int x
 }
 
 // path XYZ + mergeNewSamples (mergeNewSamples.comp:173-265) with per-pixel sample counts
-// sampleXYZ: XYZ and hero wavelength per texel (bidirectional integrator); null: XYZ of the path slots
-__global__ void k_merge(DevScene S, PathSoA P, FilmBuffers F, MergeParams mp, const float4* sampleXYZ) {
+// sampleXYZ: XYZ and hero wavelength per texel (bidirectional integrator); null: XYZ of the path slots.
+// dyn (captured batches): sample count and current buffer come from the device, advanced by pass.
+__global__ void k_merge(DevScene S, PathSoA P, FilmBuffers F, MergeParams mp, const float4* sampleXYZ, const uint32_t* dyn) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= mp.width * mp.height) return;
+  if (dyn) {
+    mp.sampleCount = dyn[0] + mp.pass;
+    mp.cur = dyn[1] ^ (mp.pass & 1u);
+  }
   int x = (int)(p % mp.width), y = (int)(p / mp.width);
   vec3 xyz;
   float lam;

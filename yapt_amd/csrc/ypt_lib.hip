@@ -156,6 +156,17 @@ struct ypt_integrator {
   uint32_t bW = 0, bH = 0, bOvfStride = 0;
   size_t bSortBytes = 0;
 
+  // captured batches (hipGraph): one executable graph per batch size, recaptured when anything that
+  // is baked into its kernel arguments changes (fingerprint `key`)
+  struct BatchGraph {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint64_t key = 0;
+  };
+  BatchGraph graphs[2];
+  DevBuf<uint32_t> batchPoints, mergeDyn;
+  int useGraph = 0;
+
   // film
   DevBuf<float4> accum, merged0, merged1, lastSample;
   DevBuf<uint32_t> counts0, counts1;
@@ -176,6 +187,7 @@ struct ypt_integrator {
   int persistentWaves = 32;
   int refill = 16;         // idle lanes of a wave before it fetches new rays
   int shadeWaves = 64;     // waves per CU of the shading launch (0: one workgroup per queue entry)
+  int bdptWaves = 4;       // register budget of the camera-subpath kernel: waves per SIMD (1, 2, 3, 4, 6, 8)
   int cus = 256;           // compute units of the device
 
   // stats of the last render
@@ -195,6 +207,7 @@ struct ypt_integrator {
 
   DevScene dev_scene() const {
     DevScene s;
+    std::memset(&s, 0, sizeof(s));
     const ResidentScene& r = *res;
     s.sv.pos = r.dPos.p; s.sv.nrm = r.dNrm.p; s.sv.idx = r.dIdx.p; s.sv.triInst = r.dTriInst.p;
     s.sv.tan = r.scene.tan.empty() ? nullptr : r.dTan.p;
@@ -210,12 +223,14 @@ struct ypt_integrator {
   }
   PathSoA path_soa() const {
     PathSoA P;
+    std::memset(&P, 0, sizeof(P));
     P.orgLambda = pOrg.p; P.dirPdf = pDir.p; P.thr = pThr.p; P.light = pLight.p;
     P.st0 = pSt0.p; P.st1 = pSt1.p; P.volIor = pVolIor.p; P.volMat = pVolMat.p; P.volAbs = pVolAbs.p;
     return P;
   }
   Queues queues() const {
     Queues Q;
+    std::memset(&Q, 0, sizeof(Q));
     Q.ray[0] = rayQ0.p; Q.ray[1] = rayQ1.p; Q.counters = counters.p; Q.hit = hit.p;
     Q.sOrgTmax = sOrg.p; Q.sDirTri = sDir.p; Q.sContrib = sContrib.p; Q.sSlot = sSlot.p;
     Q.ovf = ovf.p; Q.ovfStride = ovfStride; Q.workCounters = work.p;
@@ -223,6 +238,7 @@ struct ypt_integrator {
   }
   FilmBuffers film() const {
     FilmBuffers F;
+    std::memset(&F, 0, sizeof(F));
     F.accum = accum.p; F.merged[0] = merged0.p; F.merged[1] = merged1.p;
     F.counts[0] = counts0.p; F.counts[1] = counts1.p; F.variance = variance.p; F.lastSample = lastSample.p;
     F.features = features.p;
@@ -376,6 +392,78 @@ void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
 
 
 
+// All launches of one batch of g passes for the unidirectional integrators, reading the Sobol
+// points from `point` and (when dyn is set) the film bookkeeping from the device, which the last
+// launch advances: the form that is captured into a graph.
+void launch_batch(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, const FilmBuffers& F,
+                  const uint32_t* point, uint32_t g, uint32_t* dyn) {
+  hipStream_t s = h->stream;
+  uint32_t npix = h->width * h->height, nslots = npix * g;
+  uint32_t grid = work_grid(h, nslots), sgrid = shade_grid(h, nslots), B = (uint32_t)h->blockSize;
+  hipLaunchKernelGGL(k_start, dim3(blocks_for(nslots, 256)), dim3(256), 0, s, h->fc, point, P, Q, F, npix, nslots,
+                     h->writeFeatures);
+  int q = 0;
+  for (uint32_t b = 0; b < h->fc.maxRayDepth; ++b) {
+    if (h->bvhWidth == 8) launch_extend<8>(h, S, P, Q, q, grid);
+    else launch_extend<2>(h, S, P, Q, q, grid);
+    if (h->type == YPT_INTEGRATOR_NEE) {
+      hipLaunchKernelGGL(k_shade<true>, dim3(sgrid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
+      if (h->bvhWidth == 8) launch_shadow<8>(h, S, P, Q, grid);
+      else launch_shadow<2>(h, S, P, Q, grid);
+    } else {
+      hipLaunchKernelGGL(k_shade<false>, dim3(sgrid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
+    }
+    q ^= 1;
+  }
+  for (uint32_t p = 0; p < g; ++p) {
+    MergeParams mp;
+    std::memset(&mp, 0, sizeof(mp));
+    mp.width = h->width; mp.height = h->height; mp.slotBase = p * npix;
+    mp.accumTargets = 2;
+    mp.fireflyClampScale = h->frameDesc.firefly_clamp_scale;
+    mp.fireflyClampRelaxPower = h->frameDesc.firefly_clamp_relax_power;
+    mp.killNaN = h->frameDesc.kill_nan_samples;
+    mp.pass = p;
+    hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp, (const float4*)nullptr, dyn);
+  }
+  hipLaunchKernelGGL(k_advance_dyn, dim3(1), dim3(1), 0, s, dyn, g);
+}
+
+uint64_t fnv(uint64_t hsh, const void* p, size_t n) {
+  const unsigned char* c = (const unsigned char*)p;
+  for (size_t i = 0; i < n; ++i) { hsh ^= c[i]; hsh *= 1099511628211ull; }
+  return hsh;
+}
+
+// executable graph of one batch of g passes: captured on first use, reused while nothing it bakes
+// in (launch shapes, buffer addresses, frame constants) changes
+hipGraphExec_t batch_graph(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, const FilmBuffers& F,
+                           uint32_t g, uint32_t G) {
+  uint64_t k = 1469598103934665603ull;
+  k = fnv(k, &h->fc, sizeof(h->fc));
+  k = fnv(k, &S, sizeof(S)); k = fnv(k, &P, sizeof(P)); k = fnv(k, &Q, sizeof(Q)); k = fnv(k, &F, sizeof(F));
+  int opts[14] = {h->type, h->bvhWidth, h->blockSize, h->ldsDepth, h->persistent, h->persistentWaves, h->refill,
+                  h->shadeWaves, h->writeFeatures, h->cus, (int)g, (int)h->width, (int)h->height, (int)h->frameDesc.kill_nan_samples};
+  k = fnv(k, opts, sizeof(opts));
+  float fp[2] = {h->frameDesc.firefly_clamp_scale, h->frameDesc.firefly_clamp_relax_power};
+  k = fnv(k, fp, sizeof(fp));
+  const void* ptrs[2] = {h->batchPoints.p, h->mergeDyn.p};
+  k = fnv(k, ptrs, sizeof(ptrs));
+  for (auto& bg : h->graphs)
+    if (bg.exec && bg.key == k) return bg.exec;
+  ypt_integrator::BatchGraph& bg = h->graphs[g == G ? 0 : 1];
+  if (bg.exec) { HIP_CHECK(hipGraphExecDestroy(bg.exec)); bg.exec = nullptr; }
+  if (bg.graph) { HIP_CHECK(hipGraphDestroy(bg.graph)); bg.graph = nullptr; }
+  HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  launch_batch(h, S, P, Q, F, h->batchPoints.p, g, h->mergeDyn.p);
+  hipGraph_t graph = nullptr;
+  HIP_CHECK(hipStreamEndCapture(h->stream, &graph));
+  bg.graph = graph;
+  HIP_CHECK(hipGraphInstantiate(&bg.exec, graph, nullptr, nullptr, 0));
+  bg.key = k;
+  return bg.exec;
+}
+
 // prepare, build the BVH and upload: a scene resident on `device`
 std::shared_ptr<ResidentScene> make_resident(const ypt_scene_desc& d, int device, hipStream_t s) {
   auto r = std::make_shared<ResidentScene>();
@@ -457,8 +545,24 @@ void bdpt_pass(ypt_integrator* h, const DevScene& S, const uint32_t* point, cons
   mark();
   hipLaunchKernelGGL(k_bdpt_headers, dim3(1), dim3(1024), 0, s, B);
   mark();
-  hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
-                     h->writeFeatures ? h->features.p : (float*)nullptr, d);
+  if (h->bdptWaves == 2)
+    hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 2>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
+                       h->writeFeatures ? h->features.p : (float*)nullptr, d);
+  else if (h->bdptWaves == 3)
+    hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 3>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
+                       h->writeFeatures ? h->features.p : (float*)nullptr, d);
+  else if (h->bdptWaves == 4)
+    hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 4>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
+                       h->writeFeatures ? h->features.p : (float*)nullptr, d);
+  else if (h->bdptWaves == 6)
+    hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 6>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
+                       h->writeFeatures ? h->features.p : (float*)nullptr, d);
+  else if (h->bdptWaves == 8)
+    hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 8>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
+                       h->writeFeatures ? h->features.p : (float*)nullptr, d);
+  else
+    hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 1>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
+                       h->writeFeatures ? h->features.p : (float*)nullptr, d);
   mark();
   hipLaunchKernelGGL((k_bdpt_splat<WIDTH, COUNT>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
   mark();
@@ -522,7 +626,8 @@ void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count
     mp.fireflyClampRelaxPower = h->frameDesc.firefly_clamp_relax_power;
     mp.killNaN = h->frameDesc.kill_nan_samples;
     mp.cur = h->filmCur;
-    hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp, (const float4*)h->bCamXYZ.p);
+    mp.pass = 0;
+    hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp, (const float4*)h->bCamXYZ.p, (const uint32_t*)nullptr);
     if (ev) {
       hipEvent_t e = h->next_event();
       HIP_CHECK(hipEventRecord(e, s));
@@ -594,6 +699,10 @@ int ypt_destroy(ypt_integrator* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (auto& e : h->ev) if (e) (void)hipEventDestroy(e);
   for (auto& e : h->evPool) (void)hipEventDestroy(e);
+  for (auto& g : h->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
   if (h->ownStream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return YPT_OK;
@@ -626,6 +735,11 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
   } else if (n == "persistent_waves") {
     if (value < 1 || value > 64) return fail(YPT_E_INVALID, "persistent_waves must be in [1, 64] (waves per CU)");
     h->persistentWaves = (int)value;
+  } else if (n == "graph") {
+    h->useGraph = value != 0;
+  } else if (n == "bdpt_waves") {
+    if (value < 1 || value > 8 || value == 5 || value == 7) return fail(YPT_E_INVALID, "bdpt_waves must be 1, 2, 3, 4, 6 or 8");
+    h->bdptWaves = (int)value;
   } else if (n == "shade_waves") {
     if (value < 0 || value > 1024) return fail(YPT_E_INVALID, "shade_waves must be in [0, 1024] (0 = one workgroup per entry)");
     h->shadeWaves = (int)value;
@@ -719,6 +833,25 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
     };
     hipEvent_t eAll0 = nullptr;
     if (h->profile) eAll0 = mark();
+    if (h->useGraph && !h->profile && !h->countTraversal) {
+      h->batchPoints.ensure((size_t)16 * YPT_SOBOL_POINT_DIMS);
+      h->mergeDyn.ensure(2);
+      hipLaunchKernelGGL(k_set_dyn, dim3(1), dim3(1), 0, s, h->mergeDyn.p, h->filmSamples, h->filmCur);
+      for (uint32_t k = 0; k < sample_count; k += G) {
+        uint32_t g = std::min(G, sample_count - k);
+        HIP_CHECK(hipMemcpyAsync(h->batchPoints.p, h->sobolPoints.p + (size_t)k * YPT_SOBOL_POINT_DIMS,
+                                 (size_t)g * YPT_SOBOL_POINT_DIMS * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipGraphLaunch(batch_graph(h, S, P, Q, F, g, G), s));
+        h->filmSamples += g;
+        h->filmCur ^= (g & 1u);
+        h->lastPassBase = (g - 1) * npix;
+      }
+      HIP_CHECK(hipGetLastError());
+      std::memset(h->stats, 0, sizeof(h->stats));
+      h->stats[10] = (double)h->fc.maxRayDepth * ((sample_count + G - 1) / G);
+      h->stats[13] = G;
+      return YPT_OK;
+    }
     for (uint32_t k = 0; k < sample_count; k += G) {
       uint32_t g = std::min(G, sample_count - k);
       uint32_t nslots = npix * g;
@@ -756,8 +889,9 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
         mp.fireflyClampRelaxPower = h->frameDesc.firefly_clamp_relax_power;
         mp.killNaN = h->frameDesc.kill_nan_samples;
         mp.cur = h->filmCur;
+        mp.pass = p;
         hipEvent_t m0 = h->profile ? mark() : nullptr;
-        hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp, (const float4*)nullptr);
+        hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, S, P, F, mp, (const float4*)nullptr, (const uint32_t*)nullptr);
         if (h->profile) spans.push_back({m0, mark(), 4});
         h->filmSamples++;
         h->filmCur ^= 1;
