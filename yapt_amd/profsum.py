@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 CSV output: per-kernel timing tables and HBM bytes per launch.
 
-    python -m yapt_amd.profsum stats  <kernel-trace dir> "<command>"         > summary.md
+    python -m yapt_amd.profsum stats  <kernel-trace dir> "<command>" [rows]  > summary.md
     python -m yapt_amd.profsum traffic <fetch dir> <write dir> <calib fetch dir> <calib write dir> \
            <kernel substring> <key> <calib MiB> [traffic.json [summary.md "<command>"...]]
 
@@ -91,7 +91,7 @@ def traffic_markdown(t, key, commands):
 
 def main(argv):
     if argv[0] == "stats":
-        sys.stdout.write(stats_markdown(argv[1], argv[2]))
+        sys.stdout.write(stats_markdown(argv[1], argv[2], int(argv[3]) if len(argv) > 3 else 12))
     elif argv[0] == "traffic":
         fetch_dir, write_dir, cfd, cwd, kern, key, mib = argv[1:8]
         path = argv[8] if len(argv) > 8 else os.path.join(os.path.dirname(os.path.dirname(
