@@ -207,3 +207,25 @@ def test_graph_batches_match_eager(ylib, shapes_small, integrator):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     assert np.array_equal(got_c.view(np.uint32), want.view(np.uint32))
     assert np.array_equal(got_c_last.view(np.uint32), want_last.view(np.uint32))
+
+
+def test_bdpt_wavefront_matches_megakernel(ylib, shapes_small):
+    # the same camera subpaths traced as one kernel per texel or as queues; only the order in which
+    # visible connections are added differs
+    from yapt_amd import camera
+    scene, cam = shapes_small
+    frame = camera.make_frame(cam, 72, 48, max_depth=8, light_path_vertices=8)
+    out = []
+    for wf in (0, 1):
+        it = ylib.Integrator(ylib.YPT_INTEGRATOR_BDPT, 0)
+        it.set_option("bdpt_wavefront", wf)
+        it.set_scene(scene)
+        it.set_frame(frame)
+        it.render(3, 2)
+        out.append((it.last_samples(), it.film()))
+        it.close()
+    (la, fa), (lb, fb) = out
+    assert np.array_equal(la[..., 3], lb[..., 3])
+    d = np.abs(la[..., :3].astype(np.float64) - lb[..., :3])
+    assert np.all(d <= 1e-5 * np.abs(la[..., :3]) + 1e-7), d.max()
+    assert np.allclose(fa, fb, rtol=1e-5, atol=1e-7)

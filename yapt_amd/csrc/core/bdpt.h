@@ -561,32 +561,42 @@ YPT_HD float mis_camera_misses(const FrameConstants& fc, uint32_t camIndex, cons
   return 1.f / (1.f + riCam);
 }
 
-// calculateLightPathRadiance (:1060-1185); rs is the camera subpath's state, taken by value.
-template <class Tracer>
-YPT_HD Spectrum connection_radiance(const SceneView& S, uint32_t camIndex, const CameraVertexCtx& cv,
-                                    const SurfaceContext& camCtx, const LightNode& n, uint32_t lightIndex,
-                                    const LightVertexCtx& lv, BdptState rs, const Lambdas& L, Tracer& tr) {
-  Spectrum zero(0.f);
+// A connection whose contribution is known but whose visibility is not yet tested: the ray from
+// the camera vertex must hit the light vertex's triangle first.
+struct PendingConnection {
+  Spectrum contrib;
+  vec3 origin, dir;
+  float tmax;
+  uint32_t tri;
+};
+
+// calculateLightPathRadiance (:1060-1185) up to its visibility ray; rs is the camera subpath's
+// state, taken by value. Returns false when the connection carries nothing. Visibility only gates
+// the result, so testing it after the MIS weight changes no bit of the contribution.
+YPT_HD bool connection_prepare(const SceneView& S, uint32_t camIndex, const CameraVertexCtx& cv, const SurfaceContext& camCtx,
+                               const LightNode& n, uint32_t lightIndex, const LightVertexCtx& lv, BdptState rs,
+                               const Lambdas& L, PendingConnection& out) {
   bool camTerminated = (rs.flags & RS_SECONDARY_TERMINATED) != 0;
   bool lightTerminated = (n.flags & LN_SECONDARY_TERMINATED) != 0;
   if (lightTerminated) rs.flags |= RS_SECONDARY_TERMINATED;
   ConnectionPdfs p;
   ScatterEval ec = eval_and_pdfs(S, camCtx, rs, cv.matId, cv.wo, cv.wi, cv.frontFace, false, false);
   if (ec.dispersed) ec.weight.terminate_secondary();
-  if (ec.pdfFwd == 0.f || ec.weight.all_equal(0.f)) return zero;
+  if (ec.pdfFwd == 0.f || ec.weight.all_equal(0.f)) return false;
   Spectrum wCam = ec.weight;
   p.fwdCam = ec.pdfFwd;
   p.bwdCam = ec.pdfBwd;
   ScatterEval el;
   if (!connect_light_node(S, n, lightIndex, lv.prevPos, cv.pos, (rs.flags & RS_SECONDARY_TERMINATED) != 0, L, el, p.fwdLight))
-    return zero;
+    return false;
   Spectrum wLight = el.weight;
   p.bwdLight = el.pdfBwd;
   vec3 toNode = n.pos - cv.pos;
   float len = length(toNode);
-  vec3 dir = toNode / len;
-  vec3 o = spawn_ray_origin(cv.pos, cv.normalWS, dir);
-  if (!tr.hits_target(o, dir, RAY_MIN_T, RAY_MIN_T + len + BDPT_VISIBILITY_EPS, n.tri)) return zero;
+  out.dir = toNode / len;
+  out.origin = spawn_ray_origin(cv.pos, cv.normalWS, out.dir);
+  out.tmax = RAY_MIN_T + len + BDPT_VISIBILITY_EPS;
+  out.tri = n.tri;
   float mis = mis_connection(S, camIndex, cv, lightIndex, lv, n, p);
   float geom = 1.f;
   if (lightIndex > 0) geom = 1.f / (len * len);
@@ -594,120 +604,181 @@ YPT_HD Spectrum connection_radiance(const SceneView& S, uint32_t camIndex, const
   if (!ab.all_equal(0.f)) wCam = wCam * transmittance(len, ab);
   Spectrum r = wLight * wCam * cv.throughput * n.radiance * geom * mis;
   if (camTerminated && lightTerminated) r.undo_secondary_rescale();
-  return r;
+  out.contrib = r;
+  return true;
 }
 
 // ---------------------------------------------------------------- camera subpath
-// traceCameraRays (BDPTTraceCameraPath.comp:40-432) for texel (px, py), connecting every vertex to the
-// nodes[0 .. nodeCount) of the light subpath its workgroup picked. extraDim: the workgroup's first
-// thread drew the fallback wavelength from its own sampler when there were no light subpaths.
-template <class Tracer>
-YPT_HD Spectrum trace_camera_path(const SceneView& S, const FrameConstants& fc, const uint32_t* point, uint32_t px, uint32_t py,
-                                  float lambda, const LightNodePacked* nodes, uint32_t nodeCount, bool extraDim, Tracer& tr,
-                                  FirstHitFeatures* fh) {
+// traceCameraRays (BDPTTraceCameraPath.comp:40-432) as a resumable state machine: camera_begin sets
+// up texel (px, py), camera_vertex consumes the traced ray's hit (or miss) and does one iteration of
+// the vertex loop, handing every connection with its visibility ray to `sink` in light-vertex order.
+struct CameraPath {
+  BdptState rs;
+  Spectrum throughput, total;
+  vec3 rayOrigin, rayDir;
+  vec3 prevPos, prevNrm;
+  float pdfRatioSum, prevFwdMIS;
+  float pdfForwardCamera, deferredCam;
+  float lambda;
+  uint32_t dim, seed;
+  uint32_t k;      // camera vertex index of the ray in flight
+  bool alive;
+};
+
+// extraDim: the workgroup's first thread drew the fallback wavelength from its own sampler when
+// there were no light subpaths (BDPTTraceCameraPath.comp:67-74)
+YPT_HD void camera_begin(const FrameConstants& fc, const uint32_t* point, uint32_t px, uint32_t py, float lambda, bool extraDim,
+                         CameraPath& c) {
   uint32_t W = (uint32_t)fc.targetTexDimensions[0];
   RandomSampler rand;
   rand.init(point, camera_dim_offset(fc), py * W + px);
   float lens[2];
   rand.r2(lens);
   if (extraDim) rand.dim += 1u;
-  Lambdas L = spectral_lambdas(lambda);
   float jx, jy;
   rand.texel_offset(jx, jy);
-  vec3 rayOrigin, rayDir;
-  camera_ray(fc, px, py, jx, jy, lens[0], lens[1], rayOrigin, rayDir);
-  BdptState rs;
-  rs.init();
+  camera_ray(fc, px, py, jx, jy, lens[0], lens[1], c.rayOrigin, c.rayDir);
+  c.rs.init();
+  c.throughput.set(1.f);
+  c.total.set(0.f);
+  c.prevPos = vec3(0.f);
+  c.prevNrm = vec3(0.f);
+  c.pdfRatioSum = 0.f;
+  c.prevFwdMIS = 0.f;
+  c.pdfForwardCamera = camera_dir_pdf(fc, c.rayDir);
+  c.deferredCam = 0.f;
+  c.lambda = lambda;
+  c.dim = rand.dim;
+  c.seed = rand.seed;
+  c.k = 0;
+  c.alive = fc.maxRayDepth > 0;
+}
+
+template <class Sink>
+YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const uint32_t* point, CameraPath& c, bool hit,
+                          const Hit& h, const LightNodePacked* nodes, uint32_t nodeCount, Sink& sink, FirstHitFeatures* fh) {
+  RandomSampler rand;
+  rand.point = point;
+  rand.dim = c.dim;
+  rand.seed = c.seed;
+  Lambdas L = spectral_lambdas(c.lambda);
+  const uint32_t k = c.k;
+  float rs0[4];
+  rand.r4(rs0);
+  c.alive = false;
   CameraVertexCtx cv;
-  cv.throughput.set(1.f);
-  cv.pdfRatioSum = 0.f;
-  cv.prevFwdMIS = 0.f;
-  cv.curFwdMIS = 0.f;
-  cv.prevPos = vec3(0.f);
-  cv.prevNrm = vec3(0.f);
-  float pdfForwardCamera = camera_dir_pdf(fc, rayDir);
-  float deferredCam = 0.f;
-  Spectrum total(0.f);
-  for (uint32_t k = 0; k < fc.maxRayDepth; ++k) {
-    Hit h;
-    bool hit = tr.closest(rayOrigin, rayDir, RAY_MIN_T, __builtin_inff(), h);
-    float rs0[4];
-    rand.r4(rs0);
-    if (!hit) {
-      cv.curFwdMIS = pdfForwardCamera;
-      float w = mis_camera_misses(fc, k, cv, rayDir);
-      vec3 sky = (fc.flags & FRAME_FLAG_WHITE_FURNACE) ? vec3(1.f) : vabs(rayDir) * fc.envIntensityScale;
-      total = total + cv.throughput * rgb_unbounded_spectrum(S, sky, L) * w;
-      break;
-    }
-    if (k >= fc.regularizeAfterVertices) rs.flags |= RS_REGULARIZE;
-    BdptVertex v;
-    const MaterialGPU* m = nullptr;
-    bdpt_surface(S, rs, h, rayOrigin, rayDir, L, v, &m);
-    if (k == 0 && fh) first_hit_features(fc, *m, v.ctx.def, v.pos, *fh);
-    cv.pos = v.pos;
-    cv.frontFace = v.frontFace;
-    cv.wo = v.wo;
-    cv.matId = v.matId;
-    cv.normalWS = v.normalWS;
-    bool delta = is_delta_distribution(v.ctx.pre.ax, v.ctx.pre.ay, v.ctx.def);
-    cv.curFwdMIS = pdfForwardCamera < 0.f ? -1.f : pdfForwardCamera * solid_angle_to_area(v.pos - rayOrigin, v.normalWS);
-    Spectrum ab = rs.get_absorption();
-    if (!ab.all_equal(0.f)) cv.throughput = cv.throughput * transmittance(h.t, ab);
-    const SurfaceDef& sd = v.ctx.def;
-    if (!sd.emissive.all_equal(0.f)) {
-      float w = mis_camera_hits_light(S, k, h.tri, cv);
-      if (surface_two_sided(sd.flags) || dot(sd.geometryNormal, v.wo) > 0.f) {
-        float profile = emission_profile(dot(v.normalWS, -rayDir), sd.emissionFocus);
-        total = total + cv.throughput * sd.emissive * w * profile;
-      }
-      break;
-    } else if (!delta) {
-      LightVertexCtx lv;
-      lv.riSum = 0.f;
-      lv.prevFlags = 0;
-      lv.prevPos = vec3(0.f);
-      lv.prevNrm = vec3(0.f);
-      lv.prevFwdMIS = 0.f;
-      float deferredLight = 0.f;
-      for (uint32_t j = 0; j < nodeCount; ++j) {
-        LightNode n = unpack_node(nodes[j]);
-        if ((n.flags & LN_DELTA) == 0) {
-          vec3 toNode = n.pos - v.pos;
-          float dist = length(toNode);
-          cv.wi = normalize(toNode / dist);
-          total = total + connection_radiance(S, k, cv, v.ctx, n, j, lv, rs, L, tr);
-        }
-        lv.riSum = deferredLight;
-        deferredLight = accumulate_ri(deferredLight, n.pdfBwd, n.pdfFwd);
-        lv.prevNrm = n.nrm;
-        lv.prevPos = n.pos;
-        lv.prevFwdMIS = n.pdfFwd;
-        lv.prevFlags = n.flags;
-      }
-    }
-    bool isDiffuse, isDelta;
-    vec3 wi = sample_scattering(S, v.ctx, rs0, isDiffuse, isDelta);
-    if (is_zero(wi)) break;
-    if (isDiffuse) rs.flags |= RS_REGULARIZE;
-    ScatterEval ev = eval_and_pdfs(S, v.ctx, rs, v.matId, v.wo, wi, v.frontFace, false, isDelta);
-    Spectrum sampleWeight = ev.weight * safe_div(1.f, ev.pdfFwd);
-    if (ev.dispersed) cv.throughput.terminate_secondary();
-    if (cv.throughput.russian_roulette(rand.r1())) break;
-    if (k > 0) {
-      float pb = isDelta ? -1.f : ev.pdfBwd * solid_angle_to_area(rayOrigin - v.pos, cv.prevNrm);
-      deferredCam = accumulate_ri(deferredCam, pb, cv.prevFwdMIS);
-      cv.pdfRatioSum = deferredCam;
-    }
-    cv.prevNrm = v.normalWS;
-    cv.prevPos = v.pos;
-    cv.prevFwdMIS = cv.curFwdMIS;
-    pdfForwardCamera = isDelta ? -1.f : ev.pdfFwd;
-    cv.throughput = cv.throughput * sampleWeight;
-    rayDir = normalize(wi);
-    rayOrigin = spawn_ray_origin(v.pos, v.normalWS, rayDir);
+  cv.throughput = c.throughput;
+  cv.prevPos = c.prevPos;
+  cv.prevNrm = c.prevNrm;
+  cv.pdfRatioSum = c.pdfRatioSum;
+  cv.prevFwdMIS = c.prevFwdMIS;
+  if (!hit) {
+    cv.curFwdMIS = c.pdfForwardCamera;
+    float w = mis_camera_misses(fc, k, cv, c.rayDir);
+    vec3 sky = (fc.flags & FRAME_FLAG_WHITE_FURNACE) ? vec3(1.f) : vabs(c.rayDir) * fc.envIntensityScale;
+    c.total = c.total + cv.throughput * rgb_unbounded_spectrum(S, sky, L) * w;
+    c.dim = rand.dim;
+    return;
   }
-  return total;
+  if (k >= fc.regularizeAfterVertices) c.rs.flags |= RS_REGULARIZE;
+  BdptVertex v;
+  const MaterialGPU* m = nullptr;
+  bdpt_surface(S, c.rs, h, c.rayOrigin, c.rayDir, L, v, &m);
+  if (k == 0 && fh) first_hit_features(fc, *m, v.ctx.def, v.pos, *fh);
+  cv.pos = v.pos;
+  cv.frontFace = v.frontFace;
+  cv.wo = v.wo;
+  cv.matId = v.matId;
+  cv.normalWS = v.normalWS;
+  bool delta = is_delta_distribution(v.ctx.pre.ax, v.ctx.pre.ay, v.ctx.def);
+  cv.curFwdMIS = c.pdfForwardCamera < 0.f ? -1.f : c.pdfForwardCamera * solid_angle_to_area(v.pos - c.rayOrigin, v.normalWS);
+  Spectrum ab = c.rs.get_absorption();
+  if (!ab.all_equal(0.f)) cv.throughput = cv.throughput * transmittance(h.t, ab);
+  const SurfaceDef& sd = v.ctx.def;
+  if (!sd.emissive.all_equal(0.f)) {
+    float w = mis_camera_hits_light(S, k, h.tri, cv);
+    if (surface_two_sided(sd.flags) || dot(sd.geometryNormal, v.wo) > 0.f) {
+      float profile = emission_profile(dot(v.normalWS, -c.rayDir), sd.emissionFocus);
+      c.total = c.total + cv.throughput * sd.emissive * w * profile;
+    }
+    c.throughput = cv.throughput;
+    c.dim = rand.dim;
+    return;
+  }
+  if (!delta) {
+    LightVertexCtx lv;
+    lv.riSum = 0.f;
+    lv.prevFlags = 0;
+    lv.prevPos = vec3(0.f);
+    lv.prevNrm = vec3(0.f);
+    lv.prevFwdMIS = 0.f;
+    float deferredLight = 0.f;
+    for (uint32_t j = 0; j < nodeCount; ++j) {
+      LightNode n = unpack_node(nodes[j]);
+      if ((n.flags & LN_DELTA) == 0) {
+        vec3 toNode = n.pos - v.pos;
+        float dist = length(toNode);
+        cv.wi = normalize(toNode / dist);
+        PendingConnection pc;
+        if (connection_prepare(S, k, cv, v.ctx, n, j, lv, c.rs, L, pc)) sink(c, pc);
+      }
+      lv.riSum = deferredLight;
+      deferredLight = accumulate_ri(deferredLight, n.pdfBwd, n.pdfFwd);
+      lv.prevNrm = n.nrm;
+      lv.prevPos = n.pos;
+      lv.prevFwdMIS = n.pdfFwd;
+      lv.prevFlags = n.flags;
+    }
+  }
+  c.throughput = cv.throughput;
+  bool isDiffuse, isDelta;
+  vec3 wi = sample_scattering(S, v.ctx, rs0, isDiffuse, isDelta);
+  if (is_zero(wi)) { c.dim = rand.dim; return; }
+  if (isDiffuse) c.rs.flags |= RS_REGULARIZE;
+  ScatterEval ev = eval_and_pdfs(S, v.ctx, c.rs, v.matId, v.wo, wi, v.frontFace, false, isDelta);
+  Spectrum sampleWeight = ev.weight * safe_div(1.f, ev.pdfFwd);
+  if (ev.dispersed) c.throughput.terminate_secondary();
+  if (c.throughput.russian_roulette(rand.r1())) { c.dim = rand.dim; return; }
+  if (k > 0) {
+    float pb = isDelta ? -1.f : ev.pdfBwd * solid_angle_to_area(c.rayOrigin - v.pos, c.prevNrm);
+    c.deferredCam = accumulate_ri(c.deferredCam, pb, c.prevFwdMIS);
+    c.pdfRatioSum = c.deferredCam;
+  }
+  c.prevNrm = v.normalWS;
+  c.prevPos = v.pos;
+  c.prevFwdMIS = cv.curFwdMIS;
+  c.pdfForwardCamera = isDelta ? -1.f : ev.pdfFwd;
+  c.throughput = c.throughput * sampleWeight;
+  c.rayDir = normalize(wi);
+  c.rayOrigin = spawn_ray_origin(v.pos, v.normalWS, c.rayDir);
+  c.dim = rand.dim;
+  c.k = k + 1;
+  c.alive = c.k < fc.maxRayDepth;
+}
+
+template <class Tracer>
+struct TraceNowSink {  // tests visibility at once and adds visible contributions in order
+  Tracer* tr;
+  YPT_HD void operator()(CameraPath& c, const PendingConnection& pc) {
+    if (tr->hits_target(pc.origin, pc.dir, RAY_MIN_T, pc.tmax, pc.tri)) c.total = c.total + pc.contrib;
+  }
+};
+
+// the whole camera subpath of texel (px, py) with inline tracing
+template <class Tracer>
+YPT_HD Spectrum trace_camera_path(const SceneView& S, const FrameConstants& fc, const uint32_t* point, uint32_t px, uint32_t py,
+                                  float lambda, const LightNodePacked* nodes, uint32_t nodeCount, bool extraDim, Tracer& tr,
+                                  FirstHitFeatures* fh) {
+  CameraPath c;
+  camera_begin(fc, point, px, py, lambda, extraDim, c);
+  TraceNowSink<Tracer> sink;
+  sink.tr = &tr;
+  while (c.alive) {
+    Hit h;
+    bool hit = tr.closest(c.rayOrigin, c.rayDir, RAY_MIN_T, __builtin_inff(), h);
+    camera_vertex(S, fc, point, c, hit, h, nodes, nodeCount, sink, c.k == 0 ? fh : nullptr);
+  }
+  return c.total;
 }
 
 // light subpath the camera workgroup `group` connects to (BDPTTraceCameraPath.comp:75-83)

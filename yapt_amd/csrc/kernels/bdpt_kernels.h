@@ -240,4 +240,179 @@ __global__ void k_bdpt_accum(const uint32_t* keys, const uint32_t* vals, const f
   }
 }
 
+// ---------------------------------------------------------------- camera subpaths as a wavefront
+// The camera subpaths of a pass as queues in HBM: extension rays go through the persistent
+// closest-hit kernel, each vertex is shaded by k_bdpt_cshade (surface, emission, every connection's
+// contribution and visibility ray, next direction), and the connection rays go through the
+// persistent occlusion kernel, which adds the visible ones to the path's total atomically.
+struct CamSoA {
+  float4* orgLambda;   // ray origin, hero wavelength
+  float4* dirPdf;      // ray direction, pdfForwardCamera
+  float4* thr;         // throughput
+  float4* total;       // radiance so far
+  uint4* st0;          // dim, seed, state flags, vertex index
+  uint4* st1;          // alive, volumes, light subpath (0xFFFFFFFF: none), its vertex count
+  float4* prev;        // previous vertex position, its forward pdf
+  float4* prevN;       // previous vertex normal, pdf ratio sum
+  float* deferred;     // deferred ratio sum of the camera subpath
+  float4* volIor;
+  uint4* volMat;
+  float4* volAbs;      // 4 per slot
+};
+
+__device__ __forceinline__ void load_cam(const CamSoA& C, uint32_t s, CameraPath& c, uint32_t& lp, uint32_t& count) {
+  float4 a = C.orgLambda[s], b = C.dirPdf[s], t = C.thr[s], l = C.total[s], pv = C.prev[s], pn = C.prevN[s];
+  uint4 u0 = C.st0[s], u1 = C.st1[s];
+  c.rayOrigin = vec3(a.x, a.y, a.z);
+  c.lambda = a.w;
+  c.rayDir = vec3(b.x, b.y, b.z);
+  c.pdfForwardCamera = b.w;
+  c.throughput = Spectrum(t.x, t.y, t.z, t.w);
+  c.total = Spectrum(l.x, l.y, l.z, l.w);
+  c.prevPos = vec3(pv.x, pv.y, pv.z);
+  c.prevFwdMIS = pv.w;
+  c.prevNrm = vec3(pn.x, pn.y, pn.z);
+  c.pdfRatioSum = pn.w;
+  c.deferredCam = C.deferred[s];
+  c.dim = u0.x; c.seed = u0.y; c.rs.flags = u0.z; c.k = u0.w;
+  c.alive = u1.x != 0u;
+  c.rs.n = u1.y;
+  lp = u1.z;
+  count = u1.w;
+  if (c.rs.n > 0) {
+    float4 io = C.volIor[s];
+    uint4 vm = C.volMat[s];
+    c.rs.ior[0] = io.x; c.rs.ior[1] = io.y; c.rs.ior[2] = io.z; c.rs.ior[3] = io.w;
+    c.rs.mat[0] = vm.x; c.rs.mat[1] = vm.y; c.rs.mat[2] = vm.z; c.rs.mat[3] = vm.w;
+    for (int k = 0; k < 4; ++k) {
+      float4 ab = C.volAbs[(size_t)s * 4 + k];
+      c.rs.absorption[k] = Spectrum(ab.x, ab.y, ab.z, ab.w);
+    }
+  } else {
+    for (int k = 0; k < 4; ++k) { c.rs.ior[k] = 1.f; c.rs.mat[k] = VOLUME_NONE; c.rs.absorption[k].set(0.f); }
+  }
+}
+
+__device__ __forceinline__ void store_cam(const CamSoA& C, uint32_t s, const CameraPath& c, uint32_t lp, uint32_t count,
+                                          bool volumes) {
+  C.orgLambda[s] = make_float4(c.rayOrigin.x, c.rayOrigin.y, c.rayOrigin.z, c.lambda);
+  C.dirPdf[s] = make_float4(c.rayDir.x, c.rayDir.y, c.rayDir.z, c.pdfForwardCamera);
+  C.thr[s] = make_float4(c.throughput.s[0], c.throughput.s[1], c.throughput.s[2], c.throughput.s[3]);
+  C.total[s] = make_float4(c.total.s[0], c.total.s[1], c.total.s[2], c.total.s[3]);
+  C.prev[s] = make_float4(c.prevPos.x, c.prevPos.y, c.prevPos.z, c.prevFwdMIS);
+  C.prevN[s] = make_float4(c.prevNrm.x, c.prevNrm.y, c.prevNrm.z, c.pdfRatioSum);
+  C.deferred[s] = c.deferredCam;
+  C.st0[s] = make_uint4(c.dim, c.seed, c.rs.flags, c.k);
+  C.st1[s] = make_uint4(c.alive ? 1u : 0u, c.rs.n, lp, count);
+  if (volumes) {
+    C.volIor[s] = make_float4(c.rs.ior[0], c.rs.ior[1], c.rs.ior[2], c.rs.ior[3]);
+    C.volMat[s] = make_uint4(c.rs.mat[0], c.rs.mat[1], c.rs.mat[2], c.rs.mat[3]);
+    for (int k = 0; k < 4; ++k)
+      C.volAbs[(size_t)s * 4 + k] = make_float4(c.rs.absorption[k].s[0], c.rs.absorption[k].s[1], c.rs.absorption[k].s[2],
+                                                c.rs.absorption[k].s[3]);
+  }
+}
+
+// one slot per dispatch position (dy * W + dx); the 8x8 dispatch tile picks one light subpath
+__global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuffers B, CamSoA C, Queues Q, float* features) {
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
+  uint32_t npix = W * H;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) { Q.counters[0] = npix; Q.counters[1] = 0; Q.counters[2] = 0; Q.counters[4] = 0; }
+  if (i >= npix) return;
+  uint32_t dy = i / W, dx = i - dy * W;
+  uint32_t gcx = (W + BDPT_TILE - 1u) / BDPT_TILE;
+  uint32_t gx = dx / BDPT_TILE, gy = dy / BDPT_TILE, g = gy * gcx + gx;
+  uint32_t t = (dy % BDPT_TILE) * BDPT_TILE + dx % BDPT_TILE;
+  uint32_t hc = *B.headerCount;
+  float lambda;
+  uint32_t lp = 0xFFFFFFFFu, count = 0;
+  if (hc > 0) {
+    lp = B.headers[pick_light_path(point, g, hc)];
+    lambda = B.lambdas[lp];
+    count = B.counts[lp];
+  } else {
+    uint32_t q0 = B.perm[(gy * BDPT_TILE) * W + gx * BDPT_TILE];
+    lambda = fallback_wavelength(fc, point, q0 & 0xFFFFu, q0 >> 16);
+  }
+  uint32_t q = B.perm[i];
+  CameraPath c;
+  camera_begin(fc, point, q & 0xFFFFu, q >> 16, lambda, hc == 0 && t == 0, c);
+  store_cam(C, i, c, lp, count, false);
+  Q.ray[0][i] = i;
+  if (features) {
+    float* f = features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * 12;
+    for (int k = 0; k < 12; ++k) f[k] = 0.f;
+  }
+}
+
+struct QueueSink {  // connection rays into the shadow queue, one atomic per wave and connection index
+  Queues Q;
+  uint32_t slot;
+  __device__ void operator()(CameraPath&, const PendingConnection& pc) {
+    uint32_t si = wave_append(&Q.counters[2], true);
+    Q.sOrgTmax[si] = make_float4(pc.origin.x, pc.origin.y, pc.origin.z, pc.tmax);
+    Q.sDirTri[si] = make_float4(pc.dir.x, pc.dir.y, pc.dir.z, __uint_as_float(pc.tri));
+    Q.sContrib[si] = make_float4(pc.contrib.s[0], pc.contrib.s[1], pc.contrib.s[2], pc.contrib.s[3]);
+    Q.sSlot[si] = slot;
+  }
+};
+
+// grid-stride over the ray queue with a loop bound uniform per workgroup (see k_shade)
+template <int MINW>
+__global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B,
+                                                          CamSoA C, Queues Q, int qsel, float* features) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;
+  const uint32_t n = Q.counters[qsel];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t W = (uint32_t)fc.targetTexDimensions[0];
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    bool cont = false;
+    uint32_t slot = 0;
+    if (i < n) {
+      slot = Q.ray[qsel][i];
+      CameraPath c;
+      uint32_t lp, count;
+      load_cam(C, slot, c, lp, count);
+      uint32_t nvolIn = c.rs.n;
+      float4 hv = Q.hit[slot];
+      Hit h;
+      h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.tri = __float_as_uint(hv.w);
+      bool hit = h.tri != NO_HIT_ID;
+      const LightNodePacked* nodes = lp != 0xFFFFFFFFu ? B.nodes + (size_t)lp * BDPT_MAX_LIGHT_VERTICES : nullptr;
+      QueueSink sink;
+      sink.Q = Q;
+      sink.slot = slot;
+      FirstHitFeatures fh;
+      fh.hit = 0u;
+      bool first = c.k == 0;
+      camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, sink, (features && first) ? &fh : nullptr);
+      store_cam(C, slot, c, lp, count, nvolIn > 0 || c.rs.n > 0);
+      cont = c.alive;
+      if (features && first && fh.hit) {
+        uint32_t q = B.perm[slot];
+        float* f = features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * 12;
+        f[0] = fh.normal.x; f[1] = fh.normal.y; f[2] = fh.normal.z;
+        f[3] = fh.albedo.x; f[4] = fh.albedo.y; f[5] = fh.albedo.z;
+        f[6] = fh.roughness; f[7] = fh.ior; f[8] = fh.metalness; f[9] = fh.transparency; f[10] = fh.depth; f[11] = 1.f;
+      }
+    }
+    uint32_t ri = wave_append(&Q.counters[qsel ^ 1], cont);
+    if (cont) Q.ray[qsel ^ 1][ri] = slot;
+  }
+}
+
+// XYZ of every camera subpath into its texel
+__global__ void k_bdpt_cfinish(FrameConstants fc, DevScene S, BdptBuffers B, CamSoA C) {
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W * H) return;
+  float4 l = C.total[i];
+  float lambda = C.orgLambda[i].w;
+  vec3 xyz = spectrum_to_xyz(Spectrum(l.x, l.y, l.z, l.w), spectral_lambdas(lambda), S.sv.spec);
+  uint32_t q = B.perm[i];
+  B.camXYZ[(q >> 16) * W + (q & 0xFFFFu)] = make_float4(xyz.x, xyz.y, xyz.z, lambda);
+}
+
 }  // namespace ypt

@@ -322,7 +322,9 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
   }
 }
 
-template <int WIDTH, bool COUNT>
+// ATOMIC: several shadow rays of one path slot in one launch (bidirectional connections), so the
+// contributions are added atomically
+template <int WIDTH, bool COUNT, bool ATOMIC = false>
 __global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, uint32_t refill) {
   extern __shared__ uint32_t lds[];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -363,8 +365,13 @@ __global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, u
       if (!r.occluded) {
         uint32_t slot = Q.sSlot[cur];
         float4 c = Q.sContrib[cur];
-        float4 l = P.light[slot];
-        P.light[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w + c.w);
+        if (ATOMIC) {
+          float* l = reinterpret_cast<float*>(P.light + slot);
+          atomicAdd(l, c.x); atomicAdd(l + 1, c.y); atomicAdd(l + 2, c.z); atomicAdd(l + 3, c.w);
+        } else {
+          float4 l = P.light[slot];
+          P.light[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w + c.w);
+        }
       }
       has = false;
     }

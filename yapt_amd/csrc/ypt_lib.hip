@@ -154,6 +154,12 @@ struct ypt_integrator {
   DevBuf<float4> bCamXYZ, bRecXYZ;
   DevBuf<uint8_t> bSortTemp;
   uint32_t bW = 0, bH = 0, bOvfStride = 0;
+  // camera subpaths as a wavefront: per-slot state and the connection-ray queue
+  DevBuf<float4> cOrg, cDir, cThr, cTotal, cPrev, cPrevN, cVolIor, cVolAbs, cSOrg, cSDir, cSContrib;
+  DevBuf<uint4> cSt0, cSt1, cVolMat;
+  DevBuf<float> cDeferred;
+  DevBuf<uint32_t> cSSlot;
+  int bdptWavefront = 1;
   size_t bSortBytes = 0;
 
   // captured batches (hipGraph): one executable graph per batch size, recaptured when anything that
@@ -258,7 +264,7 @@ void upload_constant_tables(ypt_integrator* h) {
 void ensure_frame_buffers(ypt_integrator* h, uint32_t w, uint32_t hgt) {
   size_t npix = (size_t)w * hgt;
   h->counters.ensure(16);
-  h->work.ensure(8);
+  h->work.ensure(16);
   h->accum.ensure(2 * npix); h->merged0.ensure(npix); h->merged1.ensure(npix); h->lastSample.ensure(npix);
   h->counts0.ensure(npix); h->counts1.ensure(npix); h->variance.ensure(npix);
   h->width = w;
@@ -524,6 +530,53 @@ void ensure_bdpt(ypt_integrator* h) {
   h->bOvf.ensure(depth * stride);
   h->bOvfStride = stride;
   if (h->writeFeatures) h->features.ensure((size_t)npix * 12);
+  if (h->bdptWavefront) {
+    h->cOrg.ensure(npix); h->cDir.ensure(npix); h->cThr.ensure(npix); h->cTotal.ensure(npix);
+    h->cPrev.ensure(npix); h->cPrevN.ensure(npix); h->cDeferred.ensure(npix);
+    h->cSt0.ensure(npix); h->cSt1.ensure(npix); h->cVolIor.ensure(npix); h->cVolMat.ensure(npix); h->cVolAbs.ensure((size_t)npix * 4);
+    size_t nsh = (size_t)npix * h->fc.lightPathVertices;
+    h->cSOrg.ensure(nsh); h->cSDir.ensure(nsh); h->cSContrib.ensure(nsh); h->cSSlot.ensure(nsh);
+    ensure_slots(h, npix);
+    ensure_overflow(h);
+  }
+}
+
+// camera subpaths of one pass as a wavefront: start, then per vertex a persistent closest-hit launch,
+// the shading launch (connections into the ray queue) and a persistent occlusion launch
+template <int WIDTH, bool COUNT>
+void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t* point, const BdptBuffers& B) {
+  hipStream_t s = h->stream;
+  const FrameConstants& fc = h->fc;
+  uint32_t npix = h->width * h->height;
+  CamSoA C;
+  C.orgLambda = h->cOrg.p; C.dirPdf = h->cDir.p; C.thr = h->cThr.p; C.total = h->cTotal.p;
+  C.st0 = h->cSt0.p; C.st1 = h->cSt1.p; C.prev = h->cPrev.p; C.prevN = h->cPrevN.p; C.deferred = h->cDeferred.p;
+  C.volIor = h->cVolIor.p; C.volMat = h->cVolMat.p; C.volAbs = h->cVolAbs.p;
+  PathSoA P;
+  std::memset(&P, 0, sizeof(P));
+  P.orgLambda = C.orgLambda; P.dirPdf = C.dirPdf; P.light = C.total;
+  Queues Q = h->queues();
+  Q.sOrgTmax = h->cSOrg.p; Q.sDirTri = h->cSDir.p; Q.sContrib = h->cSContrib.p; Q.sSlot = h->cSSlot.p;
+  Q.workCounters = h->work.p + 8;
+  float* feats = h->writeFeatures ? h->features.p : (float*)nullptr;
+  uint32_t Bk = (uint32_t)h->blockSize;
+  uint32_t pgrid = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(npix, Bk), (uint32_t)((size_t)h->cus * h->persistentWaves * 64 / Bk)));
+  uint32_t sgrid = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(npix, Bk), (uint32_t)((size_t)h->cus * h->persistentWaves * 64 / Bk)));
+  uint32_t cgrid = std::min<uint32_t>(blocks_for(npix, 64), (uint32_t)h->cus * 32u);
+  size_t lds = (size_t)h->ldsDepth * Bk * 4;
+  uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
+  hipLaunchKernelGGL(k_bdpt_cstart, dim3(blocks_for(npix, 256)), dim3(256), 0, s, fc, point, B, C, Q, feats);
+  int q = 0;
+  for (uint32_t k = 0; k < fc.maxRayDepth; ++k) {
+    hipLaunchKernelGGL((k_extend_p<WIDTH, COUNT>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r);
+    if (h->bdptWaves <= 2)
+      hipLaunchKernelGGL(k_bdpt_cshade<2>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, feats);
+    else
+      hipLaunchKernelGGL(k_bdpt_cshade<4>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, feats);
+    hipLaunchKernelGGL((k_shadow_p<WIDTH, COUNT, true>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r);
+    q ^= 1;
+  }
+  hipLaunchKernelGGL(k_bdpt_cfinish, dim3(blocks_for(npix, 256)), dim3(256), 0, s, fc, S, B, C);
 }
 
 // profile spans of one pass: light, headers, camera, splat (kinds 0..3)
@@ -545,7 +598,8 @@ void bdpt_pass(ypt_integrator* h, const DevScene& S, const uint32_t* point, cons
   mark();
   hipLaunchKernelGGL(k_bdpt_headers, dim3(1), dim3(1024), 0, s, B);
   mark();
-  if (h->bdptWaves == 2)
+  if (h->bdptWavefront) bdpt_camera_wavefront<WIDTH, COUNT>(h, S, point, B);
+  else if (h->bdptWaves == 2)
     hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 2>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
                        h->writeFeatures ? h->features.p : (float*)nullptr, d);
   else if (h->bdptWaves == 3)
@@ -737,6 +791,8 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
     h->persistentWaves = (int)value;
   } else if (n == "graph") {
     h->useGraph = value != 0;
+  } else if (n == "bdpt_wavefront") {
+    h->bdptWavefront = value != 0;
   } else if (n == "bdpt_waves") {
     if (value < 1 || value > 8 || value == 5 || value == 7) return fail(YPT_E_INVALID, "bdpt_waves must be 1, 2, 3, 4, 6 or 8");
     h->bdptWaves = (int)value;
@@ -938,7 +994,11 @@ int ypt_get_stats(ypt_integrator* h, double* out16) {
       unsigned long long w[8] = {0};
       HIP_CHECK(hipMemcpy(w, h->work.p, sizeof(w), hipMemcpyDeviceToHost));
       if (h->type == YPT_INTEGRATOR_BDPT) {  // camera subpaths | light subpaths and splats
-        h->stats[6] = (double)w[0]; h->stats[7] = (double)w[1]; h->stats[0] = (double)w[2];
+        unsigned long long v[16] = {0};
+        HIP_CHECK(hipMemcpy(v, h->work.p, sizeof(v), hipMemcpyDeviceToHost));
+        // megakernel [0..2]; wavefront extension [8, 9, 12] and connection [10, 11, 13] launches
+        h->stats[6] = (double)(v[0] + v[8] + v[10]); h->stats[7] = (double)(v[1] + v[9] + v[11]);
+        h->stats[0] = (double)(v[2] + v[12] + v[13]);
         h->stats[8] = (double)w[3]; h->stats[9] = (double)w[4]; h->stats[1] = (double)w[5];
       } else {
         h->stats[6] = (double)w[0]; h->stats[7] = (double)w[1]; h->stats[8] = (double)w[2]; h->stats[9] = (double)w[3];
