@@ -158,7 +158,8 @@ def main():
 
     if bdpt:
         metric = "path samples/s (BDPT, random shapes ~1M tris + 8 emissive triangles, 1280x720, depth 8)"
-        kernel = "k_bdpt_camera<%d> (camera subpaths with all connections)" % width
+        kernel = ("camera-subpath stage: k_extend_p<%d> + k_bdpt_cshade + k_shadow_p<%d> (extension and connection rays)" % (width, width)
+                  if cfg.get("bdpt_wavefront", 1) else "k_bdpt_camera<%d> (camera subpaths with all connections)" % width)
         kernel_ms = {"camera": st_prof[2], "light": st_prof[3], "splat": st_prof[4], "headers": st_prof[11],
                      "sort_sum_merge": st_prof[12], "render_total": st_prof[5]}
         rays = {"camera_side": st_cnt[0], "light_side": st_cnt[1]}

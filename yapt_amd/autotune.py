@@ -17,17 +17,22 @@ def main():
     p.add_argument("--height", type=int, default=1080)
     p.add_argument("--depth", type=int, default=12)
     p.add_argument("--shapes", type=int, default=2000)
-    p.add_argument("--integrator", default="nee", choices=["nee", "bpt"])
+    p.add_argument("--integrator", default="nee", choices=["nee", "bpt", "bdpt"])
     p.add_argument("--passes", type=int, default=16)
     p.add_argument("--rounds", type=int, default=2)
     a = p.parse_args()
-    scene, cam = scenes.random_shapes(n_shapes=a.shapes, seed=2)
-    frame = camera.make_frame(cam, a.width, a.height, max_depth=a.depth)
-    it = _lib.Integrator(_lib.YPT_INTEGRATOR_NEE if a.integrator == "nee" else _lib.YPT_INTEGRATOR_BPT, 0)
+    bdpt = a.integrator == "bdpt"
+    if bdpt and (a.width, a.height, a.depth) == (1920, 1080, 12):
+        a.width, a.height, a.depth = 1280, 720, 8   # the bidirectional workload
+    scene, cam = scenes.random_shapes(n_shapes=a.shapes, seed=2, emissive_triangles=8 if bdpt else 0)
+    frame = camera.make_frame(cam, a.width, a.height, max_depth=a.depth, light_path_vertices=9 if bdpt else 16)
+    integ = {"nee": _lib.YPT_INTEGRATOR_NEE, "bpt": _lib.YPT_INTEGRATOR_BPT, "bdpt": _lib.YPT_INTEGRATOR_BDPT}[a.integrator]
+    it = _lib.Integrator(integ, 0)
     it.set_scene(scene)
     it.set_frame(frame)
     key = "%s-shapes%d-%dx%d-d%d" % (a.integrator, a.shapes, a.width, a.height, a.depth)
-    best = tuning.autotune(it, key, passes=a.passes, rounds=a.rounds, out=a.out)
+    best = tuning.autotune(it, key, passes=a.passes, rounds=a.rounds, out=a.out,
+                           space=tuning.SPACE_BDPT if bdpt else None)
     print("best", key, best)
     it.close()
 

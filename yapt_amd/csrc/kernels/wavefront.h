@@ -165,9 +165,10 @@ __global__ void k_extend(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t lds
 // Grid-stride over the ray queue: a launch sized for the device, not for the slot count, so the
 // short queues of deep bounces cost little. The loop bound is uniform across each workgroup, so
 // every wave reaches the queue appends with all of its lanes.
-template <bool NEE>
-__global__ void k_shade(FrameConstants fc, DevScene S, const uint32_t* points, uint32_t npix, PathSoA P, Queues Q,
-                        FilmBuffers F, int qsel, int writeFeatures) {
+// 64-thread workgroups; MINW: waves per SIMD the register allocation must allow
+template <bool NEE, int MINW>
+__global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene S, const uint32_t* points, uint32_t npix,
+                                                    PathSoA P, Queues Q, FilmBuffers F, int qsel, int writeFeatures) {
   if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;  // fetch counter of the next extension launch
   const uint32_t n = Q.counters[qsel];
   const uint32_t stride = gridDim.x * blockDim.x;

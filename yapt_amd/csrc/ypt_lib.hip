@@ -193,6 +193,7 @@ struct ypt_integrator {
   int persistentWaves = 32;
   int refill = 16;         // idle lanes of a wave before it fetches new rays
   int shadeWaves = 64;     // waves per CU of the shading launch (0: one workgroup per queue entry)
+  int shadeRegWaves = 4;   // register budget of the shading kernel: 2, 4 or 8 waves per SIMD
   int bdptWaves = 4;       // register budget of the camera-subpath kernel: waves per SIMD (1, 2, 3, 4, 6, 8)
   int cus = 256;           // compute units of the device
 
@@ -356,13 +357,26 @@ uint32_t work_grid(const ypt_integrator* h, uint32_t nslots) {
   return std::min(g, maxb);
 }
 
-// shading: grid-stride over the queue with `shadeWaves` waves per CU, or one workgroup per entry
+// shading (64-thread workgroups): grid-stride over the queue with `shadeWaves` waves per CU, or one
+// workgroup per entry
 uint32_t shade_grid(const ypt_integrator* h, uint32_t nslots) {
-  uint32_t B = (uint32_t)h->blockSize;
-  uint32_t g = blocks_for(nslots, B);
+  uint32_t g = blocks_for(nslots, 64);
   if (h->shadeWaves == 0) return g;
-  uint32_t maxb = std::max<uint32_t>(1u, (uint32_t)((size_t)h->cus * h->shadeWaves * 64 / B));
-  return std::min(g, maxb);
+  return std::min(g, std::max<uint32_t>(1u, (uint32_t)h->cus * (uint32_t)h->shadeWaves));
+}
+
+void launch_shade(ypt_integrator* h, const DevScene& S, const uint32_t* point, uint32_t npix, const PathSoA& P, const Queues& Q,
+                  const FilmBuffers& F, int q, uint32_t sgrid) {
+  hipStream_t s = h->stream;
+  bool nee = h->type == YPT_INTEGRATOR_NEE;
+  int w = h->shadeRegWaves;
+#define YPT_SHADE(N, M) hipLaunchKernelGGL((k_shade<N, M>), dim3(sgrid), dim3(64), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures)
+  if (nee) {
+    if (w == 2) YPT_SHADE(true, 2); else if (w == 8) YPT_SHADE(true, 8); else YPT_SHADE(true, 4);
+  } else {
+    if (w == 2) YPT_SHADE(false, 2); else if (w == 8) YPT_SHADE(false, 8); else YPT_SHADE(false, 4);
+  }
+#undef YPT_SHADE
 }
 
 template <int WIDTH>
@@ -413,11 +427,11 @@ void launch_batch(ypt_integrator* h, const DevScene& S, const PathSoA& P, const 
     if (h->bvhWidth == 8) launch_extend<8>(h, S, P, Q, q, grid);
     else launch_extend<2>(h, S, P, Q, q, grid);
     if (h->type == YPT_INTEGRATOR_NEE) {
-      hipLaunchKernelGGL(k_shade<true>, dim3(sgrid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
+      launch_shade(h, S, point, npix, P, Q, F, q, sgrid);
       if (h->bvhWidth == 8) launch_shadow<8>(h, S, P, Q, grid);
       else launch_shadow<2>(h, S, P, Q, grid);
     } else {
-      hipLaunchKernelGGL(k_shade<false>, dim3(sgrid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
+      launch_shade(h, S, point, npix, P, Q, F, q, sgrid);
     }
     q ^= 1;
   }
@@ -448,8 +462,9 @@ hipGraphExec_t batch_graph(ypt_integrator* h, const DevScene& S, const PathSoA& 
   uint64_t k = 1469598103934665603ull;
   k = fnv(k, &h->fc, sizeof(h->fc));
   k = fnv(k, &S, sizeof(S)); k = fnv(k, &P, sizeof(P)); k = fnv(k, &Q, sizeof(Q)); k = fnv(k, &F, sizeof(F));
-  int opts[14] = {h->type, h->bvhWidth, h->blockSize, h->ldsDepth, h->persistent, h->persistentWaves, h->refill,
-                  h->shadeWaves, h->writeFeatures, h->cus, (int)g, (int)h->width, (int)h->height, (int)h->frameDesc.kill_nan_samples};
+  int opts[15] = {h->type, h->bvhWidth, h->blockSize, h->ldsDepth, h->persistent, h->persistentWaves, h->refill,
+                  h->shadeWaves, h->writeFeatures, h->cus, (int)g, (int)h->width, (int)h->height, (int)h->frameDesc.kill_nan_samples,
+                  h->shadeRegWaves};
   k = fnv(k, opts, sizeof(opts));
   float fp[2] = {h->frameDesc.firefly_clamp_scale, h->frameDesc.firefly_clamp_relax_power};
   k = fnv(k, fp, sizeof(fp));
@@ -793,6 +808,9 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
     h->useGraph = value != 0;
   } else if (n == "bdpt_wavefront") {
     h->bdptWavefront = value != 0;
+  } else if (n == "shade_reg_waves") {
+    if (value != 2 && value != 4 && value != 8) return fail(YPT_E_INVALID, "shade_reg_waves must be 2, 4 or 8");
+    h->shadeRegWaves = (int)value;
   } else if (n == "bdpt_waves") {
     if (value < 1 || value > 8 || value == 5 || value == 7) return fail(YPT_E_INVALID, "bdpt_waves must be 1, 2, 3, 4, 6 or 8");
     h->bdptWaves = (int)value;
@@ -924,10 +942,7 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
         if (h->bvhWidth == 8) launch_extend<8>(h, S, P, Q, q, grid);
         else launch_extend<2>(h, S, P, Q, q, grid);
         hipEvent_t a1 = h->profile ? mark() : nullptr;
-        if (h->type == YPT_INTEGRATOR_NEE)
-          hipLaunchKernelGGL(k_shade<true>, dim3(sgrid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
-        else
-          hipLaunchKernelGGL(k_shade<false>, dim3(sgrid), dim3(B), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
+        launch_shade(h, S, point, npix, P, Q, F, q, sgrid);
         hipEvent_t a2 = h->profile ? mark() : nullptr;
         if (h->profile) { spans.push_back({a0, a1, 0}); spans.push_back({a1, a2, 1}); }
         if (h->type == YPT_INTEGRATOR_NEE) {

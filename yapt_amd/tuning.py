@@ -15,12 +15,12 @@ TUNED = os.path.join(HERE, "tuned.json")
 ROOT = os.path.dirname(HERE)
 
 DEFAULT = {"bvh_width": 2, "block_size": 64, "lds_stack_depth": 16, "passes_in_flight": 0, "persistent": 1,
-           "persistent_waves": 32, "refill_threshold": 16, "shade_waves": 64}
+           "persistent_waves": 32, "refill_threshold": 16, "shade_waves": 64, "shade_reg_waves": 4}
 
 
-def _load():
+def _load(path=None):
     try:
-        with open(TUNED) as f:
+        with open(path or TUNED) as f:
             return json.load(f)
     except (OSError, ValueError):
         return {}
@@ -54,6 +54,17 @@ SPACE = {
     "persistent_waves": (8, 16, 24, 32),
     "refill_threshold": (4, 8, 16, 32, 64),
     "shade_waves": (0, 16, 32, 64, 128),
+    "shade_reg_waves": (2, 4, 8),
+}
+
+SPACE_BDPT = {
+    "bvh_width": (2, 8),
+    "bdpt_wavefront": (1, 0),
+    "bdpt_waves": (4, 2, 3, 6),
+    "block_size": (64, 128, 256),
+    "persistent_waves": (16, 24, 32),
+    "refill_threshold": (8, 16, 32),
+    "lds_stack_depth": (12, 16, 24),
 }
 
 
@@ -74,7 +85,7 @@ def autotune(integrator, key, passes=16, space=None, rounds=2, start=None, verbo
     timing `passes` full passes (scene and frame already set), and the fastest value is kept;
     `rounds` sweeps over all options. Stores the best configuration under `key` and applies it."""
     space = space or SPACE
-    cur = dict(start or best_config(key))
+    cur = {k: v for k, v in dict(start or best_config(key)).items() if k in space or k in DEFAULT}
     for k in space:
         cur.setdefault(k, space[k][0])
     results = []
@@ -104,7 +115,7 @@ def autotune(integrator, key, passes=16, space=None, rounds=2, start=None, verbo
                     best_ms, cur, changed = ms, cfg, True
         if not changed:
             break
-    store = _load()
+    store = _load(out)
     store[key] = {"config": cur, "ms_per_pass": best_ms, "all": results}
     with open(out or TUNED, "w") as f:
         json.dump(store, f, indent=1, sort_keys=True)
