@@ -197,6 +197,9 @@ int ypt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_active
 int ypt_server_destroy(ypt_server* server);
 int ypt_server_submit(ypt_server* server, ypt_scene* scene, int integrator_type, const ypt_frame_desc* frame,
                       uint32_t spp, uint64_t* job_id);
+/* ypt_set_option for the handles that run jobs of one integrator type (< 0: every type), from the
+ * next admitted job on */
+int ypt_server_set_option(ypt_server* server, int integrator_type, const char* name, int64_t value);
 /* YPT_OK when the job has finished, 1 when timeout_s (< 0: no limit) passed first */
 int ypt_server_wait(ypt_server* server, uint64_t job_id, double timeout_s);
 /* film of a finished job (W*H float4 X, Y, Z, 1; may be NULL) and times3 = submit, admit, finish in

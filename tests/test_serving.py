@@ -86,3 +86,31 @@ def test_admission_respects_the_active_limit(ylib, cornell):
     finally:
         srv.close()
         sc.close()
+
+
+@pytest.mark.gpu
+def test_server_launch_options_change_no_result_bit(ylib, shapes_small):
+    from yapt_amd import camera
+    sc = ylib.ResidentScene(shapes_small[0])
+    fr = camera.make_frame(shapes_small[1], 72, 40, max_depth=6)
+    films = []
+    for opts in ({}, {"bvh_width": 8, "persistent": 1, "persistent_waves": 8, "refill_threshold": 8,
+                      "shade_reg_waves": 2, "bdpt_wavefront": 0, "bdpt_waves": 4}):
+        srv = ylib.Server(0, 4 << 30, 2)
+        try:
+            for k, v in opts.items():
+                srv.set_option(k, v)
+            jobs = [srv.submit(sc, integ, fr, 3) for integ in (0, 1, 2)]
+            out = []
+            for j in jobs:
+                srv.wait(j)
+                out.append(srv.take(j)[0])
+            films.append(out)
+        finally:
+            srv.close()
+    sc.close()
+    (bpt_a, nee_a, bdpt_a), (bpt_b, nee_b, bdpt_b) = films
+    assert np.array_equal(bpt_a.view(np.uint32), bpt_b.view(np.uint32))
+    assert np.array_equal(nee_a.view(np.uint32), nee_b.view(np.uint32))
+    # bidirectional: the megakernel and the queued camera stage add visible connections in another order
+    assert np.allclose(bdpt_a, bdpt_b, rtol=1e-5, atol=1e-7)

@@ -140,6 +140,7 @@ def load():
     lib.ypt_server_submit.argtypes = [P, P, ctypes.c_int, ctypes.POINTER(FrameDesc), ctypes.c_uint32,
                                       ctypes.POINTER(ctypes.c_uint64)]
     lib.ypt_server_wait.argtypes = [P, ctypes.c_uint64, ctypes.c_double]
+    lib.ypt_server_set_option.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
     lib.ypt_server_take.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]
     lib.ypt_server_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     lib.ypt_clock.restype = ctypes.c_double
@@ -153,7 +154,7 @@ EXPORTED_SYMBOLS = [
     "ypt_set_option", "ypt_trace_closest", "ypt_sample_dims", "ypt_sobol_point", "ypt_last_error", "ypt_version",
     "ypt_scene_create", "ypt_scene_destroy", "ypt_scene_info", "ypt_set_resident_scene",
     "ypt_server_create", "ypt_server_destroy", "ypt_server_submit", "ypt_server_wait", "ypt_server_take",
-    "ypt_server_stats", "ypt_clock",
+    "ypt_server_stats", "ypt_clock", "ypt_server_set_option",
 ]
 
 
@@ -291,6 +292,9 @@ class Server:
         check(self.lib.ypt_server_submit(self.h, scene.h, int(integrator), ctypes.byref(frame), int(spp), ctypes.byref(jid)))
         self._frames[jid.value] = (frame.width, frame.height)
         return jid.value
+
+    def set_option(self, name, value, integrator=-1):
+        check(self.lib.ypt_server_set_option(self.h, int(integrator), name.encode(), int(value)))
 
     def wait(self, job, timeout=-1.0):
         rc = self.lib.ypt_server_wait(self.h, int(job), float(timeout))

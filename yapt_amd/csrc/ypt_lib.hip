@@ -1232,6 +1232,7 @@ struct ypt_server {
   std::vector<std::shared_ptr<ServerJob>> active;
   std::unordered_map<uint64_t, std::shared_ptr<ServerJob>> jobs;
   std::vector<ypt_integrator*> pool[3];
+  std::vector<std::pair<std::string, int64_t>> options[3];  // launch options per integrator type
   uint64_t nextId = 1, memUsed = 0, completed = 0, peakActive = 0;
   bool stop = false;
   std::thread worker;
@@ -1252,6 +1253,8 @@ struct ypt_server {
     ypt_integrator* h = j.h;
     if (!j.ev) HIP_CHECK(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
     if (h->res != j.scene) attach_scene(h, j.scene);
+    for (const auto& o : options[j.type])
+      if (ypt_set_option(h, o.first.c_str(), o.second) != YPT_OK) throw std::runtime_error(ypt_last_error());
     if (ypt_set_frame(h, &j.frame) != YPT_OK) throw std::runtime_error(std::string("server: ") + ypt_last_error());
     if (ypt_set_option(h, "passes_in_flight", j.passesInFlight) != YPT_OK) throw std::runtime_error(ypt_last_error());
     clear_film(h);
@@ -1405,6 +1408,27 @@ int ypt_server_submit(ypt_server* s, ypt_scene* sc, int integrator_type, const y
   s->jobs[j->id] = j;
   s->queued.push_back(j);
   *job_id = j->id;
+  return YPT_OK;
+}
+
+// launch option for the handles of one integrator type (< 0: all types), applied to jobs admitted later
+int ypt_server_set_option(ypt_server* s, int integrator_type, const char* name, int64_t value) {
+  if (!s || !name) return fail(YPT_E_INVALID, "null argument");
+  if (integrator_type > YPT_INTEGRATOR_BDPT) return fail(YPT_E_INVALID, "unknown integrator type");
+  {
+    // validated now on a handle that holds no device state, so a bad value fails here and not in a job
+    std::unique_ptr<ypt_integrator> probe(new ypt_integrator());
+    int rc = ypt_set_option(probe.get(), name, value);
+    if (rc != YPT_OK) return rc;
+  }
+  std::lock_guard<std::mutex> lk(s->m);
+  for (int t = 0; t < 3; ++t) {
+    if (integrator_type >= 0 && t != integrator_type) continue;
+    bool found = false;
+    for (auto& o : s->options[t])
+      if (o.first == name) { o.second = value; found = true; }
+    if (!found) s->options[t].emplace_back(name, value);
+  }
   return YPT_OK;
 }
 

@@ -22,7 +22,7 @@ INTEGRATORS = [0, 1, 1, 2]            # BPT : NEE : BDPT = 1 : 2 : 1
 SCENE_SHAPES = [20, 200, 2000, 10000]  # ~10k, ~100k, ~1M, ~5M triangles
 
 
-def make_requests(n, rate, seed=11, n_scenes=4):
+def make_requests(n, rate, seed=4, n_scenes=4):
     """n requests: Poisson arrival times (exponential gaps, mean 1/rate) and a uniform draw of
     scene, resolution, spp and integrator (the integrator list carries the 1:2:1 weights)."""
     rng = np.random.default_rng(seed)
@@ -77,7 +77,7 @@ def main():
     p.add_argument("--budget-gb", type=float, default=96.0)
     p.add_argument("--max-active", type=int, default=8)
     p.add_argument("--shapes", default=",".join(str(s) for s in SCENE_SHAPES))
-    p.add_argument("--seed", type=int, default=11)
+    p.add_argument("--seed", type=int, default=4)
     a = p.parse_args()
     from . import _lib, scenes as S
     shapes = [int(x) for x in a.shapes.split(",")]
@@ -89,6 +89,13 @@ def main():
         tris.append(sc.triangle_count)
         print("scene %d: %d triangles resident" % (k, sc.triangle_count), file=sys.stderr, flush=True)
     server = _lib.Server(0, int(a.budget_gb * (1 << 30)), a.max_active)
+    from . import tuning
+    skip = ("passes_in_flight",)   # the server sizes batches per job
+    for integ, key in ((0, "nee-shapes2000-1920x1080-d12"), (1, "nee-shapes2000-1920x1080-d12"),
+                       (2, "bdpt-shapes2000-1280x720-d8")):
+        for k, v in tuning.best_config(key).items():
+            if k not in skip:
+                server.set_option(k, v, integ)
     reqs = make_requests(a.jobs, a.rate, a.seed, len(shapes))
     t = time.perf_counter()
     recs = run(server, res_scenes, cams, reqs)
