@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--passes-in-flight", type=int, default=0, help="sample passes per wavefront batch; 0 = tuned")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the checker sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                   help="launch option over the tuned choice (repeatable)")
     return p.parse_args()
 
 
@@ -93,6 +95,9 @@ def main():
         cfg["lds_stack_depth"] = a.lds_depth
     if a.passes_in_flight:
         cfg["passes_in_flight"] = a.passes_in_flight
+    for o in a.option:
+        k, v = o.split("=", 1)
+        cfg[k.strip()] = int(v)
     for k, v in cfg.items():
         it.set_option(k, v)
     width = cfg["bvh_width"]

@@ -154,8 +154,15 @@ int ypt_get_stats(ypt_integrator* h, double* out16);
  *  "passes_in_flight" 0..16 (0 = automatic), "persistent" 0/1 (traversal kernels loop over their
  *  queue), "persistent_waves" 1..64 (waves per CU of those launches), "refill_threshold" 1..64
  *  (idle lanes before a wave fetches new rays), "shade_waves" 0..1024 (0 = one workgroup per queue
- *  entry), "count_traversal" 0/1, "first_hit_features" 0/1, "profile" 0/1.
- *  None of them changes a result bit. */
+ *  entry), "shade_reg_waves" 2 | 4 | 8 (register budget of the shading kernel, in waves per SIMD),
+ *  "graph" 0/1 (replay each batch of passes as a captured hipGraph), "count_traversal" 0/1,
+ *  "first_hit_features" 0/1, "profile" 0/1.
+ *  Bidirectional integrator: "bdpt_wavefront" 0/1 (camera subpaths as queued kernels rather than
+ *  one kernel per texel), "bdpt_waves" 1 | 2 | 3 | 4 | 6 | 8 (register budget of the camera kernels;
+ *  the queued shading kernel takes 1, 2 or 4).
+ *  None of them changes a result bit of the unidirectional integrators. With "bdpt_wavefront" on,
+ *  the visible connections of a texel are added atomically, in an order that can vary from run to
+ *  run (agreement to about 1e-6 relative). */
 int ypt_set_option(ypt_integrator* h, const char* name, int64_t value);
 
 /* Closest-hit queries against the scene's BVH on the GPU (the TraceRay / RayQuery the integrators

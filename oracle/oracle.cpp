@@ -387,7 +387,8 @@ size_t bdpt_pass(const Ctx& c, const uint32_t* point, const std::vector<uint32_t
       uint32_t px = q & 0xFFFFu, py = q >> 16;
       size_t pix = (size_t)py * W + px;
       FirstHitFeatures* f = fh ? &(*fh)[pix] : nullptr;
-      Spectrum L = trace_camera_path(c.sv, fc, point, px, py, lambda, nd, count, hc == 0 && t == 0, tr, f);
+      NodeSurfacesOnTheFly surfs;
+      Spectrum L = trace_camera_path(c.sv, fc, point, px, py, lambda, nd, count, surfs, hc == 0 && t == 0, tr, f);
       vec3 v = spectrum_to_xyz(L, spectral_lambdas(lambda), c.sv.spec);
       xyz[3 * pix] = v.x; xyz[3 * pix + 1] = v.y; xyz[3 * pix + 2] = v.z;
       lam[pix] = lambda;
@@ -402,7 +403,8 @@ size_t bdpt_pass(const Ctx& c, const uint32_t* point, const std::vector<uint32_t
   parallel_for(hc, nt, [&](uint32_t h, unsigned ti) {
     uint32_t p = headers[h];
     CpuTracer tr{&c.geo, wk(ti)};
-    splat_light_path(c.sv, fc, point, h, hc, lambdas[p], &nodes[(size_t)p * BDPT_MAX_LIGHT_VERTICES], counts[p], tr,
+    NodeSurfacesOnTheFly surfs;
+    splat_light_path(c.sv, fc, point, h, hc, lambdas[p], &nodes[(size_t)p * BDPT_MAX_LIGHT_VERTICES], counts[p], surfs, tr,
                      &recs[(size_t)h * per]);
     recCount[h] = counts[p] > 0 ? counts[p] - 1 : 0;
   });

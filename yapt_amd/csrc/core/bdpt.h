@@ -126,25 +126,27 @@ struct BdptState {
     flags = 0;
     for (int i = 0; i < MAX_VOLUMES; ++i) { ior[i] = 1.f; absorption[i].set(0.f); mat[i] = VOLUME_NONE; }
   }
-  YPT_HD Spectrum get_absorption() const { return n != 0 ? absorption[n - 1] : Spectrum(0.f); }
-  YPT_HD float current_ior() const { return n != 0 ? ior[n - 1] : 1.0f; }
-  YPT_HD float previous_ior() const { return n > 1 ? ior[n - 2] : 1.0f; }
+  YPT_HD Spectrum get_absorption() const { return n != 0 ? sel_get(absorption, n - 1) : Spectrum(0.f); }
+  YPT_HD float current_ior() const { return n != 0 ? sel_get(ior, n - 1) : 1.0f; }
+  YPT_HD float previous_ior() const { return n > 1 ? sel_get(ior, n - 2) : 1.0f; }
   YPT_HD void entered(float i, const Spectrum& a, uint32_t m) {
     n = n + 1 < (uint32_t)MAX_VOLUMES ? n + 1 : (uint32_t)MAX_VOLUMES;
-    ior[n - 1] = i;
-    absorption[n - 1] = a;
-    mat[n - 1] = m;
+    sel_set(ior, n - 1, i);
+    sel_set(absorption, n - 1, a);
+    sel_set(mat, n - 1, m);
   }
+  // removes the innermost entry of material m (the innermost entry when m is not on the stack)
   YPT_HD void exited(uint32_t m) {
     if (n == 0) return;
-    int index = -1;
-    for (int i = (int)n - 1; i >= 0; --i)
-      if (mat[i] == m) { index = i; break; }
-    if (index < 0) index = (int)n - 1;
-    for (uint32_t i = (uint32_t)index; i + 1 < n; ++i) {
-      ior[i] = ior[i + 1];
-      absorption[i] = absorption[i + 1];
-      mat[i] = mat[i + 1];
+    uint32_t index = n - 1;
+#pragma unroll
+    for (int i = 0; i < MAX_VOLUMES; ++i) index = ((uint32_t)i < n && mat[i] == m) ? (uint32_t)i : index;
+#pragma unroll
+    for (int i = 0; i + 1 < MAX_VOLUMES; ++i) {
+      bool shift = (uint32_t)i >= index && (uint32_t)i + 1 < n;
+      ior[i] = pick(shift, ior[i + 1], ior[i]);
+      absorption[i] = pick(shift, absorption[i + 1], absorption[i]);
+      mat[i] = pick(shift, mat[i + 1], mat[i]);
     }
     --n;
   }
@@ -446,17 +448,66 @@ YPT_HD uint32_t trace_light_path(const SceneView& S, const FrameConstants& fc, c
 }
 
 // ---------------------------------------------------------------- connections
+// The surface of a light vertex as a connection sees it: material, shading frame and layer
+// probabilities, with wo towards the vertex before it. None of it depends on the connection's
+// other end, so it can be made once per light vertex and shared by every connection to it.
+struct LightNodeSurface {
+  SurfaceContext ctx;
+  vec3 wo;
+};
+
+YPT_HD BdptState light_node_state(const LightNode& n, bool secondaryTerminated) {
+  BdptState rs;
+  rs.init();
+  rs.ior[0] = n.iorPrev;
+  rs.ior[1] = n.iorCur;
+  rs.n = 2;
+  bool already = secondaryTerminated || (n.flags & LN_SECONDARY_TERMINATED) != 0;
+  rs.flags = already ? RS_SECONDARY_TERMINATED : 0u;
+  return rs;
+}
+
+// light vertex `n` (index > 0) reached from prevPos
+YPT_HD void light_node_surface(const SceneView& S, const LightNode& n, vec3 prevPos, const Lambdas& L, LightNodeSurface& out) {
+  const MaterialGPU& m = S.mats[S.inst[S.triInst[n.tri]].material];
+  bool frontFace = (n.flags & LN_FRONT_FACE) != 0;
+  out.wo = normalize(prevPos - n.pos);
+  TriGeometry g;
+  fetch_triangle(S, n.tri, n.b1, n.b2, g);
+  // no two-sided re-orientation here, as in the reference
+  SurfaceDef sd = make_surface(m, g.geometryNormal, g.normal, g.tangent, out.wo, false, (n.flags & LN_REGULARIZED) != 0, L,
+                               S.spec);
+  BdptState rs = light_node_state(n, false);
+  make_surface_context(out.ctx, sd, rs.current_ior(), rs.previous_ior(), out.wo, frontFace, L, true);
+}
+
+// Sources of the light-vertex surfaces: made for each connection, as the reference does, or read
+// from a table made once per vertex (k_bdpt_node_surfaces). Both give the same bits.
+struct NodeSurfacesOnTheFly {
+  LightNodeSurface tmp;
+  YPT_HD const LightNodeSurface* get(const SceneView& S, const LightNode& n, uint32_t, vec3 prevPos, const Lambdas& L) {
+    light_node_surface(S, n, prevPos, L, tmp);
+    return &tmp;
+  }
+};
+struct NodeSurfaceTable {
+  const LightNodeSurface* p;  // [vertex index] of one light subpath
+  YPT_HD const LightNodeSurface* get(const SceneView&, const LightNode&, uint32_t index, vec3, const Lambdas&) const {
+    return p + index;
+  }
+};
+
 // calculateConnectingLightNodeWeightsAndPDFs (:701-875) for a mesh-light origin (index 0) or a
-// surface vertex; false when the connection carries nothing.
-YPT_HD bool connect_light_node(const SceneView& S, const LightNode& n, uint32_t index, vec3 prevPos, vec3 nextPos,
+// surface vertex with its surface `ls`; false when the connection carries nothing.
+YPT_HD bool connect_light_node(const SceneView& S, const LightNode& n, uint32_t index, const LightNodeSurface* ls, vec3 nextPos,
                                bool secondaryTerminated, const Lambdas& L, ScatterEval& ev, float& pdfFwdMIS) {
   ev.weight.set(0.f);
   ev.pdfFwd = 0.f;
   ev.pdfBwd = 0.f;
   ev.dispersed = false;
-  const MaterialGPU& m = S.mats[S.inst[S.triInst[n.tri]].material];
   bool frontFace = (n.flags & LN_FRONT_FACE) != 0;
   if (index == 0) {
+    const MaterialGPU& m = S.mats[S.inst[S.triInst[n.tri]].material];
     bool twoSided = surface_two_sided(m.flags);
     vec3 fromLight = nextPos - n.pos;
     if (dot(fromLight, n.nrm) <= 0.f && !twoSided) { pdfFwdMIS = 0.f; return false; }
@@ -469,21 +520,8 @@ YPT_HD bool connect_light_node(const SceneView& S, const LightNode& n, uint32_t 
     return true;
   }
   vec3 wi = normalize(nextPos - n.pos);
-  vec3 wo = normalize(prevPos - n.pos);
-  TriGeometry g;
-  fetch_triangle(S, n.tri, n.b1, n.b2, g);
-  // no two-sided re-orientation here, as in the reference
-  SurfaceDef sd = make_surface(m, g.geometryNormal, g.normal, g.tangent, wo, false, (n.flags & LN_REGULARIZED) != 0, L, S.spec);
-  BdptState rs;
-  rs.init();
-  rs.ior[0] = n.iorPrev;
-  rs.ior[1] = n.iorCur;
-  rs.n = 2;
-  bool already = secondaryTerminated || (n.flags & LN_SECONDARY_TERMINATED) != 0;
-  rs.flags = already ? RS_SECONDARY_TERMINATED : 0u;
-  SurfaceContext ctx;
-  make_surface_context(ctx, sd, rs.current_ior(), rs.previous_ior(), wo, frontFace, L, true);
-  ev = eval_and_pdfs(S, ctx, rs, VOLUME_NONE, wo, wi, frontFace, true, false);
+  BdptState rs = light_node_state(n, secondaryTerminated);
+  ev = eval_and_pdfs(S, ls->ctx, rs, VOLUME_NONE, ls->wo, wi, frontFace, true, false);
   if (ev.dispersed) ev.weight.terminate_secondary();
   pdfFwdMIS = ev.pdfFwd;
   if (ev.pdfBwd == 0.f || ev.weight.all_equal(0.f)) return false;
@@ -574,8 +612,8 @@ struct PendingConnection {
 // state, taken by value. Returns false when the connection carries nothing. Visibility only gates
 // the result, so testing it after the MIS weight changes no bit of the contribution.
 YPT_HD bool connection_prepare(const SceneView& S, uint32_t camIndex, const CameraVertexCtx& cv, const SurfaceContext& camCtx,
-                               const LightNode& n, uint32_t lightIndex, const LightVertexCtx& lv, BdptState rs,
-                               const Lambdas& L, PendingConnection& out) {
+                               const LightNode& n, uint32_t lightIndex, const LightNodeSurface* ls, const LightVertexCtx& lv,
+                               BdptState rs, const Lambdas& L, PendingConnection& out) {
   bool camTerminated = (rs.flags & RS_SECONDARY_TERMINATED) != 0;
   bool lightTerminated = (n.flags & LN_SECONDARY_TERMINATED) != 0;
   if (lightTerminated) rs.flags |= RS_SECONDARY_TERMINATED;
@@ -587,7 +625,7 @@ YPT_HD bool connection_prepare(const SceneView& S, uint32_t camIndex, const Came
   p.fwdCam = ec.pdfFwd;
   p.bwdCam = ec.pdfBwd;
   ScatterEval el;
-  if (!connect_light_node(S, n, lightIndex, lv.prevPos, cv.pos, (rs.flags & RS_SECONDARY_TERMINATED) != 0, L, el, p.fwdLight))
+  if (!connect_light_node(S, n, lightIndex, ls, cv.pos, (rs.flags & RS_SECONDARY_TERMINATED) != 0, L, el, p.fwdLight))
     return false;
   Spectrum wLight = el.weight;
   p.bwdLight = el.pdfBwd;
@@ -654,9 +692,10 @@ YPT_HD void camera_begin(const FrameConstants& fc, const uint32_t* point, uint32
   c.alive = fc.maxRayDepth > 0;
 }
 
-template <class Sink>
+template <class Sink, class Surfaces>
 YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const uint32_t* point, CameraPath& c, bool hit,
-                          const Hit& h, const LightNodePacked* nodes, uint32_t nodeCount, Sink& sink, FirstHitFeatures* fh) {
+                          const Hit& h, const LightNodePacked* nodes, uint32_t nodeCount, Surfaces& surfs, Sink& sink,
+                          FirstHitFeatures* fh) {
   RandomSampler rand;
   rand.point = point;
   rand.dim = c.dim;
@@ -719,8 +758,9 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
         vec3 toNode = n.pos - v.pos;
         float dist = length(toNode);
         cv.wi = normalize(toNode / dist);
+        const LightNodeSurface* ls = j > 0 ? surfs.get(S, n, j, lv.prevPos, L) : nullptr;
         PendingConnection pc;
-        if (connection_prepare(S, k, cv, v.ctx, n, j, lv, c.rs, L, pc)) sink(c, pc);
+        if (connection_prepare(S, k, cv, v.ctx, n, j, ls, lv, c.rs, L, pc)) sink(c, pc);
       }
       lv.riSum = deferredLight;
       deferredLight = accumulate_ri(deferredLight, n.pdfBwd, n.pdfFwd);
@@ -765,10 +805,10 @@ struct TraceNowSink {  // tests visibility at once and adds visible contribution
 };
 
 // the whole camera subpath of texel (px, py) with inline tracing
-template <class Tracer>
+template <class Tracer, class Surfaces>
 YPT_HD Spectrum trace_camera_path(const SceneView& S, const FrameConstants& fc, const uint32_t* point, uint32_t px, uint32_t py,
-                                  float lambda, const LightNodePacked* nodes, uint32_t nodeCount, bool extraDim, Tracer& tr,
-                                  FirstHitFeatures* fh) {
+                                  float lambda, const LightNodePacked* nodes, uint32_t nodeCount, Surfaces& surfs, bool extraDim,
+                                  Tracer& tr, FirstHitFeatures* fh) {
   CameraPath c;
   camera_begin(fc, point, px, py, lambda, extraDim, c);
   TraceNowSink<Tracer> sink;
@@ -776,7 +816,7 @@ YPT_HD Spectrum trace_camera_path(const SceneView& S, const FrameConstants& fc, 
   while (c.alive) {
     Hit h;
     bool hit = tr.closest(c.rayOrigin, c.rayDir, RAY_MIN_T, __builtin_inff(), h);
-    camera_vertex(S, fc, point, c, hit, h, nodes, nodeCount, sink, c.k == 0 ? fh : nullptr);
+    camera_vertex(S, fc, point, c, hit, h, nodes, nodeCount, surfs, sink, c.k == 0 ? fh : nullptr);
   }
   return c.total;
 }
@@ -820,10 +860,10 @@ struct SplatRecord {
 
 // calculatePureLightPaths (BDPTCalculatePureLightPaths.comp:107-206) for the light subpath in header
 // slot `header`: one record per vertex 1 .. count-1 into rec[0 .. count-1).
-template <class Tracer>
+template <class Tracer, class Surfaces>
 YPT_HD void splat_light_path(const SceneView& S, const FrameConstants& fc, const uint32_t* point, uint32_t header,
-                             uint32_t headerCount, float lambda, const LightNodePacked* nodes, uint32_t count, Tracer& tr,
-                             SplatRecord* rec) {
+                             uint32_t headerCount, float lambda, const LightNodePacked* nodes, uint32_t count, Surfaces& surfs,
+                             Tracer& tr, SplatRecord* rec) {
   uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
   uint32_t npix = W * H;
   for (uint32_t i = 1; i < count; ++i) rec[i - 1].texel = npix;
@@ -848,7 +888,8 @@ YPT_HD void splat_light_path(const SceneView& S, const FrameConstants& fc, const
     if (project_to_uv(fc, lensPosVS, n.pos, u, v) && (n.flags & LN_DELTA) == 0) {
       ScatterEval ev;
       float unused;
-      if (connect_light_node(S, n, i, prevPos, lensPosWS, false, L, ev, unused) && !ev.weight.all_equal(0.f)) {
+      if (connect_light_node(S, n, i, surfs.get(S, n, i, prevPos, L), lensPosWS, false, L, ev, unused) &&
+          !ev.weight.all_equal(0.f)) {
         // checkVisibility (:70-101): from the vertex to the lens point, t in [0.01, distance]
         vec3 fromTo = lensPosWS - n.pos;
         float len = sqrtf(dot(fromTo, fromTo));

@@ -31,6 +31,26 @@ YPT_HD uint32_t as_uint(float f) { uint32_t u; __builtin_memcpy(&u, &f, 4); retu
 YPT_HD int32_t as_int(float f) { return (int32_t)as_uint(f); }
 YPT_HD float as_float_i(int32_t i) { return as_float((uint32_t)i); }
 
+// Element i of a small fixed-size array through unrolled value selects: a dynamic index into a
+// register array (or a select between element addresses) would put the whole array in scratch
+// memory. Values are copied unchanged. i must be below N. Types other than float and uint32_t
+// provide their own pick() next to their definition.
+YPT_HD float pick(bool c, float a, float b) { return c ? a : b; }
+YPT_HD uint32_t pick(bool c, uint32_t a, uint32_t b) { return c ? a : b; }
+
+template <class T, int N>
+YPT_HD T sel_get(const T (&a)[N], uint32_t i) {
+  T r = a[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k) r = pick((uint32_t)k == i, a[k], r);
+  return r;
+}
+template <class T, int N>
+YPT_HD void sel_set(T (&a)[N], uint32_t i, const T& v) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) a[k] = pick((uint32_t)k == i, v, a[k]);
+}
+
 YPT_HD float fmin_(float a, float b) { return a < b ? a : b; }   // HLSL min on non-NaN operands
 YPT_HD float fmax_(float a, float b) { return a > b ? a : b; }
 YPT_HD float clampf(float x, float lo, float hi) { return fmin_(fmax_(x, lo), hi); }

@@ -58,25 +58,27 @@ struct FirstHitFeatures {
 };
 
 // ---------------------------------------------------------------- volume stack (payload.hlsl)
-YPT_HD float current_ior(const PathState& p) { return p.nvol != 0 ? p.volIor[p.nvol - 1] : 1.0f; }
-YPT_HD float previous_ior(const PathState& p) { return p.nvol > 1 ? p.volIor[p.nvol - 2] : 1.0f; }
-YPT_HD Spectrum current_absorption(const PathState& p) { return p.nvol != 0 ? p.volAbs[p.nvol - 1] : Spectrum(0.f); }
+YPT_HD float current_ior(const PathState& p) { return p.nvol != 0 ? sel_get(p.volIor, p.nvol - 1) : 1.0f; }
+YPT_HD float previous_ior(const PathState& p) { return p.nvol > 1 ? sel_get(p.volIor, p.nvol - 2) : 1.0f; }
+YPT_HD Spectrum current_absorption(const PathState& p) { return p.nvol != 0 ? sel_get(p.volAbs, p.nvol - 1) : Spectrum(0.f); }
 YPT_HD void entered_volume(PathState& p, float ior, const Spectrum& absorption, uint32_t mat) {
   p.nvol = p.nvol + 1 < (uint32_t)MAX_VOLUMES ? p.nvol + 1 : (uint32_t)MAX_VOLUMES;
-  p.volIor[p.nvol - 1] = ior;
-  p.volAbs[p.nvol - 1] = absorption;
-  p.volMat[p.nvol - 1] = mat;
+  sel_set(p.volIor, p.nvol - 1, ior);
+  sel_set(p.volAbs, p.nvol - 1, absorption);
+  sel_set(p.volMat, p.nvol - 1, mat);
 }
+// removes the innermost entry of material mat (the innermost entry when mat is not on the stack)
 YPT_HD void exited_volume(PathState& p, uint32_t mat) {
   if (p.nvol == 0) return;
-  int index = -1;
-  for (int i = (int)p.nvol - 1; i >= 0; --i)
-    if (p.volMat[i] == mat) { index = i; break; }
-  if (index < 0) index = (int)p.nvol - 1;
-  for (uint32_t i = (uint32_t)index; i + 1 < p.nvol; ++i) {
-    p.volIor[i] = p.volIor[i + 1];
-    p.volAbs[i] = p.volAbs[i + 1];
-    p.volMat[i] = p.volMat[i + 1];
+  uint32_t index = p.nvol - 1;
+#pragma unroll
+  for (int i = 0; i < MAX_VOLUMES; ++i) index = ((uint32_t)i < p.nvol && p.volMat[i] == mat) ? (uint32_t)i : index;
+#pragma unroll
+  for (int i = 0; i + 1 < MAX_VOLUMES; ++i) {
+    bool shift = (uint32_t)i >= index && (uint32_t)i + 1 < p.nvol;
+    p.volIor[i] = pick(shift, p.volIor[i + 1], p.volIor[i]);
+    p.volAbs[i] = pick(shift, p.volAbs[i + 1], p.volAbs[i]);
+    p.volMat[i] = pick(shift, p.volMat[i + 1], p.volMat[i]);
   }
   --p.nvol;
 }

@@ -85,6 +85,15 @@ struct Spectrum {
   YPT_HD bool has_nan() const { return is_nan(s[0]) || is_nan(s[1]) || is_nan(s[2]) || is_nan(s[3]); }
 };
 
+YPT_HD Spectrum pick(bool c, const Spectrum& a, const Spectrum& b) {
+  Spectrum r;
+  for (int i = 0; i < 4; ++i) {
+    float x = a.s[i], y = b.s[i];
+    r.s[i] = c ? x : y;
+  }
+  return r;
+}
+
 YPT_HD Spectrum operator*(const Spectrum& a, const Spectrum& b) { return Spectrum(a.s[0] * b.s[0], a.s[1] * b.s[1], a.s[2] * b.s[2], a.s[3] * b.s[3]); }
 YPT_HD Spectrum operator/(const Spectrum& a, const Spectrum& b) { return Spectrum(a.s[0] / b.s[0], a.s[1] / b.s[1], a.s[2] / b.s[2], a.s[3] / b.s[3]); }
 YPT_HD Spectrum operator+(const Spectrum& a, const Spectrum& b) { return Spectrum(a.s[0] + b.s[0], a.s[1] + b.s[1], a.s[2] + b.s[2], a.s[3] + b.s[3]); }

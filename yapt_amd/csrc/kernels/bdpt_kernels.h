@@ -72,6 +72,7 @@ struct GpuTracer {
 
 struct BdptBuffers {
   LightNodePacked* nodes;  // [path][BDPT_MAX_LIGHT_VERTICES]
+  LightNodeSurface* surf;  // [path][BDPT_MAX_LIGHT_VERTICES]: surface of each vertex after the first
   uint32_t* counts;        // vertices per light subpath (0: none)
   float* lambdas;          // hero wavelength per light subpath
   uint32_t* headers;       // existing subpaths in index order
@@ -149,6 +150,19 @@ __global__ void k_bdpt_headers(BdptBuffers B) {
     if (B.counts[i] > 0u) B.headers[o++] = i;
 }
 
+// surface of every stored light vertex after the first, one thread per (subpath, vertex)
+__global__ void k_bdpt_node_surfaces(DevScene S, BdptBuffers B) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t p = i / BDPT_MAX_LIGHT_VERTICES, j = i % BDPT_MAX_LIGHT_VERTICES;
+  if (p >= B.npaths || j == 0u || j >= B.counts[p]) return;
+  const LightNodePacked* nodes = B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES;
+  LightNode n = unpack_node(nodes[j]);
+  LightNode prev = unpack_node(nodes[j - 1u]);
+  LightNodeSurface out;
+  light_node_surface(S.sv, n, prev.pos, spectral_lambdas(B.lambdas[p]), out);
+  B.surf[(size_t)p * BDPT_MAX_LIGHT_VERTICES + j] = out;
+}
+
 // 64 threads per workgroup = one 8x8 dispatch tile
 // MINW: waves per SIMD the register allocation must allow (1: up to 512 VGPRs, 2: 256)
 template <int WIDTH, bool COUNT, int MINW>
@@ -180,7 +194,10 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_camera(FrameConstants fc, Dev
   GpuTracer<WIDTH, COUNT> tr = make_tracer<WIDTH, COUNT>(S, lds, B, g * blockDim.x + t, ldsDepth);
   FirstHitFeatures fh;
   fh.hit = 0u;
-  Spectrum L = trace_camera_path(S.sv, fc, point, px, py, lambda, nodes, count, hc == 0 && t == 0, tr, features ? &fh : nullptr);
+  NodeSurfaceTable surfs;
+  surfs.p = nodes ? B.surf + (nodes - B.nodes) : nullptr;
+  Spectrum L = trace_camera_path(S.sv, fc, point, px, py, lambda, nodes, count, surfs, hc == 0 && t == 0, tr,
+                                 features ? &fh : nullptr);
   vec3 xyz = spectrum_to_xyz(L, spectral_lambdas(lambda), S.sv.spec);
   B.camXYZ[pix] = make_float4(xyz.x, xyz.y, xyz.z, lambda);
   tr.flush(B.work);
@@ -210,7 +227,9 @@ __global__ __launch_bounds__(64) void k_bdpt_splat(FrameConstants fc, DevScene S
     uint32_t p = B.headers[h];
     n = B.counts[p];
     GpuTracer<WIDTH, COUNT> tr = make_tracer<WIDTH, COUNT>(S, lds, B, h, ldsDepth);
-    splat_light_path(S.sv, fc, point, h, hc, B.lambdas[p], B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES, n, tr, rec);
+    NodeSurfaceTable surfs;
+    surfs.p = B.surf + (size_t)p * BDPT_MAX_LIGHT_VERTICES;
+    splat_light_path(S.sv, fc, point, h, hc, B.lambdas[p], B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES, n, surfs, tr, rec);
     tr.flush(B.work + 3);
   }
   for (uint32_t j = 0; j < B.recPerPath; ++j) {
@@ -313,14 +332,37 @@ __device__ __forceinline__ void store_cam(const CamSoA& C, uint32_t s, const Cam
   }
 }
 
-// one slot per dispatch position (dy * W + dx); the 8x8 dispatch tile picks one light subpath
+// Slot k of the camera wavefront holds dispatch position (dx, dy). Slots are in tile order: the
+// positions of one 8x8 dispatch tile are consecutive (fewer in a partial edge tile), so the lanes of
+// a wave mostly connect to one light subpath and read the same light vertices.
+__device__ __forceinline__ void slot_position(uint32_t W, uint32_t H, uint32_t k, uint32_t& dx, uint32_t& dy) {
+  uint32_t band = BDPT_TILE * W;  // positions in one row of tiles; every row but the last is full
+  uint32_t gy = k / band, r = k - gy * band;
+  uint32_t th = min((uint32_t)BDPT_TILE, H - gy * BDPT_TILE);
+  uint32_t gx = r / (BDPT_TILE * th);
+  r -= gx * BDPT_TILE * th;
+  uint32_t tw = min((uint32_t)BDPT_TILE, W - gx * BDPT_TILE);
+  uint32_t ly = r / tw;
+  dx = gx * BDPT_TILE + (r - ly * tw);
+  dy = gy * BDPT_TILE + ly;
+}
+
+__device__ __forceinline__ uint32_t slot_texel(const FrameConstants& fc, const BdptBuffers& B, uint32_t k) {
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
+  uint32_t dx, dy;
+  slot_position(W, H, k, dx, dy);
+  return B.perm[dy * W + dx];
+}
+
+// one slot per dispatch position; the 8x8 dispatch tile picks one light subpath
 __global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuffers B, CamSoA C, Queues Q, float* features) {
   uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
   uint32_t npix = W * H;
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) { Q.counters[0] = npix; Q.counters[1] = 0; Q.counters[2] = 0; Q.counters[4] = 0; }
   if (i >= npix) return;
-  uint32_t dy = i / W, dx = i - dy * W;
+  uint32_t dx, dy;
+  slot_position(W, H, i, dx, dy);
   uint32_t gcx = (W + BDPT_TILE - 1u) / BDPT_TILE;
   uint32_t gx = dx / BDPT_TILE, gy = dy / BDPT_TILE, g = gy * gcx + gx;
   uint32_t t = (dy % BDPT_TILE) * BDPT_TILE + dx % BDPT_TILE;
@@ -335,7 +377,7 @@ __global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuff
     uint32_t q0 = B.perm[(gy * BDPT_TILE) * W + gx * BDPT_TILE];
     lambda = fallback_wavelength(fc, point, q0 & 0xFFFFu, q0 >> 16);
   }
-  uint32_t q = B.perm[i];
+  uint32_t q = B.perm[dy * W + dx];
   CameraPath c;
   camera_begin(fc, point, q & 0xFFFFu, q >> 16, lambda, hc == 0 && t == 0, c);
   store_cam(C, i, c, lp, count, false);
@@ -349,7 +391,7 @@ __global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuff
 struct QueueSink {  // connection rays into the shadow queue, one atomic per wave and connection index
   Queues Q;
   uint32_t slot;
-  __device__ void operator()(CameraPath&, const PendingConnection& pc) {
+  __device__ __forceinline__ void operator()(CameraPath&, const PendingConnection& pc) {
     uint32_t si = wave_append(&Q.counters[2], true);
     Q.sOrgTmax[si] = make_float4(pc.origin.x, pc.origin.y, pc.origin.z, pc.tmax);
     Q.sDirTri[si] = make_float4(pc.dir.x, pc.dir.y, pc.dir.z, __uint_as_float(pc.tri));
@@ -384,14 +426,16 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
       QueueSink sink;
       sink.Q = Q;
       sink.slot = slot;
+      NodeSurfaceTable surfs;
+      surfs.p = nodes ? B.surf + (size_t)lp * BDPT_MAX_LIGHT_VERTICES : nullptr;
       FirstHitFeatures fh;
       fh.hit = 0u;
       bool first = c.k == 0;
-      camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, sink, (features && first) ? &fh : nullptr);
+      camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh);  // written below only when requested
       store_cam(C, slot, c, lp, count, nvolIn > 0 || c.rs.n > 0);
       cont = c.alive;
       if (features && first && fh.hit) {
-        uint32_t q = B.perm[slot];
+        uint32_t q = slot_texel(fc, B, slot);
         float* f = features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * 12;
         f[0] = fh.normal.x; f[1] = fh.normal.y; f[2] = fh.normal.z;
         f[3] = fh.albedo.x; f[4] = fh.albedo.y; f[5] = fh.albedo.z;
@@ -411,7 +455,7 @@ __global__ void k_bdpt_cfinish(FrameConstants fc, DevScene S, BdptBuffers B, Cam
   float4 l = C.total[i];
   float lambda = C.orgLambda[i].w;
   vec3 xyz = spectrum_to_xyz(Spectrum(l.x, l.y, l.z, l.w), spectral_lambdas(lambda), S.sv.spec);
-  uint32_t q = B.perm[i];
+  uint32_t q = slot_texel(fc, B, i);
   B.camXYZ[(q >> 16) * W + (q & 0xFFFFu)] = make_float4(xyz.x, xyz.y, xyz.z, lambda);
 }
 

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene 
       fh.hit = 0u;
       bool first = ps.pathLength == 0;
       if (h.tri == NO_HIT_ID) shade_miss(S.sv, fc, ps);
-      else shade_hit(S.sv, fc, point, NEE, ps, h, sr, (writeFeatures && first) ? &fh : nullptr);
+      else shade_hit(S.sv, fc, point, NEE, ps, h, sr, &fh);  // features written below only when requested
       after_segment(point, ps);
       cont = ps.rayState == RAY_ALIVE && ps.pathLength < fc.maxRayDepth;
       store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);

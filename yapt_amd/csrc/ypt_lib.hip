@@ -149,6 +149,7 @@ struct ypt_integrator {
 
   // bidirectional integrator
   DevBuf<LightNodePacked> bNodes;
+  DevBuf<LightNodeSurface> bSurf;  // surface of each stored light vertex, made once per pass
   DevBuf<uint32_t> bCounts, bHeaders, bHeaderCount, bPerm, bKeys, bVals, bKeysOut, bValsOut, bOvf;
   DevBuf<float> bLambdas;
   DevBuf<float4> bCamXYZ, bRecXYZ;
@@ -528,6 +529,7 @@ void ensure_bdpt(ypt_integrator* h) {
   size_t nrec = (size_t)npaths * (BDPT_MAX_LIGHT_VERTICES - 1);
   if (h->bW != W || h->bH != H) {
     h->bNodes.ensure((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
+    h->bSurf.ensure((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
     h->bCounts.ensure(npaths); h->bLambdas.ensure(npaths); h->bHeaders.ensure(npaths); h->bHeaderCount.ensure(1);
     h->bCamXYZ.ensure(npix);
     h->bKeys.ensure(nrec); h->bVals.ensure(nrec); h->bKeysOut.ensure(nrec); h->bValsOut.ensure(nrec); h->bRecXYZ.ensure(nrec);
@@ -584,7 +586,9 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
   int q = 0;
   for (uint32_t k = 0; k < fc.maxRayDepth; ++k) {
     hipLaunchKernelGGL((k_extend_p<WIDTH, COUNT>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r);
-    if (h->bdptWaves <= 2)
+    if (h->bdptWaves == 1)
+      hipLaunchKernelGGL(k_bdpt_cshade<1>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, feats);
+    else if (h->bdptWaves == 2)
       hipLaunchKernelGGL(k_bdpt_cshade<2>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, feats);
     else
       hipLaunchKernelGGL(k_bdpt_cshade<4>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, feats);
@@ -610,6 +614,7 @@ void bdpt_pass(ypt_integrator* h, const DevScene& S, const uint32_t* point, cons
   };
   mark();
   hipLaunchKernelGGL((k_bdpt_light<WIDTH, COUNT>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
+  hipLaunchKernelGGL(k_bdpt_node_surfaces, dim3(blocks_for((size_t)B.npaths * BDPT_MAX_LIGHT_VERTICES, 256)), dim3(256), 0, s, S, B);
   mark();
   hipLaunchKernelGGL(k_bdpt_headers, dim3(1), dim3(1024), 0, s, B);
   mark();
@@ -650,7 +655,7 @@ void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count
   if (h->sobolPoints.n < pts.size()) h->sobolPoints.alloc(pts.size());
   h->sobolPoints.upload(pts.data(), pts.size(), s);
   BdptBuffers B;
-  B.nodes = h->bNodes.p; B.counts = h->bCounts.p; B.lambdas = h->bLambdas.p; B.headers = h->bHeaders.p;
+  B.nodes = h->bNodes.p; B.surf = h->bSurf.p; B.counts = h->bCounts.p; B.lambdas = h->bLambdas.p; B.headers = h->bHeaders.p;
   B.headerCount = h->bHeaderCount.p; B.perm = h->bPerm.p; B.camXYZ = h->bCamXYZ.p;
   B.recKeys = h->bKeys.p; B.recVals = h->bVals.p; B.recXYZ = h->bRecXYZ.p;
   B.ovf = h->bOvf.p; B.ovfStride = h->bOvfStride;
@@ -1204,8 +1209,10 @@ void plan_job(ServerJob& j, int cus) {
   uint64_t bytes = npix * (16 * 2 + 16 * 2 + 4 * 2 + 4 + 16);
   if (j.type == YPT_INTEGRATOR_BDPT) {
     uint64_t npaths = ((j.frame.width + 7) / 8) * (uint64_t)((j.frame.height + 7) / 8);
-    bytes += npaths * BDPT_MAX_LIGHT_VERTICES * sizeof(LightNodePacked) + npaths * 15 * (4 * 4 + 16) + npix * (16 + 4) +
-             npaths * 64 * 48 * 4;
+    bytes += npaths * BDPT_MAX_LIGHT_VERTICES * (sizeof(LightNodePacked) + sizeof(LightNodeSurface)) + npaths * 15 * (4 * 4 + 16) +
+             npix * (16 + 4) + npaths * 64 * 48 * 4;
+    // camera subpaths as a wavefront: state per texel, and connection rays for every light vertex
+    bytes += npix * (16 * 9 + 4 + 16 * 4) + npix * BDPT_MAX_LIGHT_VERTICES * (16 * 3 + 4);
     j.passesInFlight = 1;
     j.chunk = std::min<uint32_t>(j.spp, 2);
   } else {

@@ -60,7 +60,7 @@ SPACE = {
 SPACE_BDPT = {
     "bvh_width": (2, 8),
     "bdpt_wavefront": (1, 0),
-    "bdpt_waves": (4, 2, 3, 6),
+    "bdpt_waves": (1, 2, 3, 4, 6),
     "block_size": (64, 128, 256),
     "persistent_waves": (16, 24, 32),
     "refill_threshold": (8, 16, 32),
