@@ -229,3 +229,25 @@ def test_bdpt_wavefront_matches_megakernel(ylib, shapes_small):
     d = np.abs(la[..., :3].astype(np.float64) - lb[..., :3])
     assert np.all(d <= 1e-5 * np.abs(la[..., :3]) + 1e-7), d.max()
     assert np.allclose(fa, fb, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.gpu
+def test_bdpt_light_stage_overlap_matches_one_stream(ylib, shapes_small):
+    # light subpaths of the next pass on a second stream (default) or everything on one stream
+    # (profile mode): with the per-texel camera kernel every sum has one order, so the films match bit
+    # for bit over several passes that reuse both light-subpath sets
+    from yapt_amd import camera
+    scene, cam = shapes_small
+    frame = camera.make_frame(cam, 64, 40, max_depth=6, light_path_vertices=6)
+    films = []
+    for prof in (0, 1):
+        it = ylib.Integrator(ylib.YPT_INTEGRATOR_BDPT, 0)
+        it.set_option("bdpt_wavefront", 0)
+        it.set_option("profile", prof)
+        it.set_scene(scene)
+        it.set_frame(frame)
+        it.render(0, 3)
+        it.render(3, 2)
+        films.append(it.film())
+        it.close()
+    assert np.array_equal(films[0].view(np.uint32), films[1].view(np.uint32))

@@ -147,11 +147,17 @@ struct ypt_integrator {
   DevBuf<unsigned long long> work;
   uint32_t ovfStride = 0;
 
-  // bidirectional integrator
-  DevBuf<LightNodePacked> bNodes;
-  DevBuf<LightNodeSurface> bSurf;  // surface of each stored light vertex, made once per pass
-  DevBuf<uint32_t> bCounts, bHeaders, bHeaderCount, bPerm, bKeys, bVals, bKeysOut, bValsOut, bOvf;
-  DevBuf<float> bLambdas;
+  // bidirectional integrator. The light subpaths of pass k+1 are traced on lightStream into one of
+  // two sets while the camera stage of pass k reads the other.
+  struct LightSet {
+    DevBuf<LightNodePacked> nodes;
+    DevBuf<LightNodeSurface> surf;  // surface of each stored light vertex, made once per pass
+    DevBuf<uint32_t> counts, headers, headerCount, ovf;
+    DevBuf<float> lambdas;
+  } lset[2];
+  hipStream_t lightStream = nullptr;
+  hipEvent_t lightDone[2] = {}, setFree[2] = {}, camReady = nullptr;
+  DevBuf<uint32_t> bPerm, bKeys, bVals, bKeysOut, bValsOut, bOvf;
   DevBuf<float4> bCamXYZ, bRecXYZ;
   DevBuf<uint8_t> bSortTemp;
   uint32_t bW = 0, bH = 0, bOvfStride = 0;
@@ -528,9 +534,11 @@ void ensure_bdpt(ypt_integrator* h) {
   uint32_t npaths = fc.lightPathsPerDim[0] * fc.lightPathsPerDim[1];
   size_t nrec = (size_t)npaths * (BDPT_MAX_LIGHT_VERTICES - 1);
   if (h->bW != W || h->bH != H) {
-    h->bNodes.ensure((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
-    h->bSurf.ensure((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
-    h->bCounts.ensure(npaths); h->bLambdas.ensure(npaths); h->bHeaders.ensure(npaths); h->bHeaderCount.ensure(1);
+    for (auto& L : h->lset) {
+      L.nodes.ensure((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
+      L.surf.ensure((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
+      L.counts.ensure(npaths); L.lambdas.ensure(npaths); L.headers.ensure(npaths); L.headerCount.ensure(1);
+    }
     h->bCamXYZ.ensure(npix);
     h->bKeys.ensure(nrec); h->bVals.ensure(nrec); h->bKeysOut.ensure(nrec); h->bValsOut.ensure(nrec); h->bRecXYZ.ensure(nrec);
     std::vector<uint32_t> perm(npix);
@@ -546,6 +554,15 @@ void ensure_bdpt(ypt_integrator* h) {
   size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
   h->bOvf.ensure(depth * stride);
   h->bOvfStride = stride;
+  for (auto& L : h->lset) L.ovf.ensure(depth * npaths);  // the light kernel's own stack overflow, stride npaths
+  if (!h->lightStream) {
+    HIP_CHECK(hipStreamCreateWithFlags(&h->lightStream, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+      HIP_CHECK(hipEventCreateWithFlags(&h->lightDone[i], hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&h->setFree[i], hipEventDisableTiming));
+    }
+    HIP_CHECK(hipEventCreateWithFlags(&h->camReady, hipEventDisableTiming));
+  }
   if (h->writeFeatures) h->features.ensure((size_t)npix * 12);
   if (h->bdptWavefront) {
     h->cOrg.ensure(npix); h->cDir.ensure(npix); h->cThr.ensure(npix); h->cTotal.ensure(npix);
@@ -598,26 +615,37 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
   hipLaunchKernelGGL(k_bdpt_cfinish, dim3(blocks_for(npix, 256)), dim3(256), 0, s, fc, S, B, C);
 }
 
-// profile spans of one pass: light, headers, camera, splat (kinds 0..3)
+void mark_span(ypt_integrator* h, hipStream_t s, std::vector<hipEvent_t>* ev) {
+  if (!ev) return;
+  hipEvent_t e = h->next_event();
+  HIP_CHECK(hipEventRecord(e, s));
+  ev->push_back(e);
+}
+
+// light subpaths of one pass on stream ls, into the set that B points to: trace, vertex surfaces,
+// headers (profile spans: light, headers)
 template <int WIDTH, bool COUNT>
-void bdpt_pass(ypt_integrator* h, const DevScene& S, const uint32_t* point, const BdptBuffers& B, uint32_t groups,
-               std::vector<hipEvent_t>* ev) {
+void bdpt_light_stage(ypt_integrator* h, const DevScene& S, const uint32_t* point, const BdptBuffers& B, hipStream_t ls,
+                      std::vector<hipEvent_t>* ev) {
+  const FrameConstants& fc = h->fc;
+  uint32_t d = (uint32_t)h->ldsDepth;
+  size_t lds = (size_t)h->ldsDepth * 64 * 4;
+  mark_span(h, ls, ev);
+  hipLaunchKernelGGL((k_bdpt_light<WIDTH, COUNT>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, ls, fc, S, point, B, d);
+  hipLaunchKernelGGL(k_bdpt_node_surfaces, dim3(blocks_for((size_t)B.npaths * BDPT_MAX_LIGHT_VERTICES, 256)), dim3(256), 0, ls, S, B);
+  mark_span(h, ls, ev);
+  hipLaunchKernelGGL(k_bdpt_headers, dim3(1), dim3(1024), 0, ls, B);
+}
+
+// camera subpaths and light tracing of one pass on the handle's stream (profile spans: camera, splat)
+template <int WIDTH, bool COUNT>
+void bdpt_camera_stage(ypt_integrator* h, const DevScene& S, const uint32_t* point, const BdptBuffers& B, uint32_t groups,
+                       std::vector<hipEvent_t>* ev) {
   hipStream_t s = h->stream;
   const FrameConstants& fc = h->fc;
   uint32_t d = (uint32_t)h->ldsDepth;
   size_t lds = (size_t)h->ldsDepth * 64 * 4;
-  auto mark = [&]() {
-    if (!ev) return;
-    hipEvent_t e = h->next_event();
-    HIP_CHECK(hipEventRecord(e, s));
-    ev->push_back(e);
-  };
-  mark();
-  hipLaunchKernelGGL((k_bdpt_light<WIDTH, COUNT>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
-  hipLaunchKernelGGL(k_bdpt_node_surfaces, dim3(blocks_for((size_t)B.npaths * BDPT_MAX_LIGHT_VERTICES, 256)), dim3(256), 0, s, S, B);
-  mark();
-  hipLaunchKernelGGL(k_bdpt_headers, dim3(1), dim3(1024), 0, s, B);
-  mark();
+  mark_span(h, s, ev);
   if (h->bdptWavefront) bdpt_camera_wavefront<WIDTH, COUNT>(h, S, point, B);
   else if (h->bdptWaves == 2)
     hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 2>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
@@ -637,9 +665,9 @@ void bdpt_pass(ypt_integrator* h, const DevScene& S, const uint32_t* point, cons
   else
     hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 1>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
                        h->writeFeatures ? h->features.p : (float*)nullptr, d);
-  mark();
+  mark_span(h, s, ev);
   hipLaunchKernelGGL((k_bdpt_splat<WIDTH, COUNT>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
-  mark();
+  mark_span(h, s, ev);
 }
 
 void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count) {
@@ -655,8 +683,7 @@ void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count
   if (h->sobolPoints.n < pts.size()) h->sobolPoints.alloc(pts.size());
   h->sobolPoints.upload(pts.data(), pts.size(), s);
   BdptBuffers B;
-  B.nodes = h->bNodes.p; B.surf = h->bSurf.p; B.counts = h->bCounts.p; B.lambdas = h->bLambdas.p; B.headers = h->bHeaders.p;
-  B.headerCount = h->bHeaderCount.p; B.perm = h->bPerm.p; B.camXYZ = h->bCamXYZ.p;
+  B.perm = h->bPerm.p; B.camXYZ = h->bCamXYZ.p;
   B.recKeys = h->bKeys.p; B.recVals = h->bVals.p; B.recXYZ = h->bRecXYZ.p;
   B.ovf = h->bOvf.p; B.ovfStride = h->bOvfStride;
   B.npaths = fc.lightPathsPerDim[0] * fc.lightPathsPerDim[1];
@@ -681,13 +708,47 @@ void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count
   if (h->countTraversal) h->work.zero(s);
   std::vector<std::vector<hipEvent_t>> spans;
   if (h->profile) { e0 = h->next_event(); HIP_CHECK(hipEventRecord(e0, s)); }
+  // Light subpaths of pass k+1 are traced on lightStream while the camera stage of pass k runs (not in
+  // profile mode, whose spans need one stream). Set k & 1 holds pass k's subpaths; it is free again
+  // once pass k's splat kernel has read it.
+  const bool overlap = !h->profile;
+  hipStream_t ls = overlap ? h->lightStream : s;
+  auto with_set = [&](uint32_t k, bool lightKernel) {
+    ypt_integrator::LightSet& L = h->lset[k & 1u];
+    BdptBuffers b = B;
+    b.nodes = L.nodes.p; b.surf = L.surf.p; b.counts = L.counts.p; b.lambdas = L.lambdas.p;
+    b.headers = L.headers.p; b.headerCount = L.headerCount.p;
+    if (lightKernel) { b.ovf = L.ovf.p; b.ovfStride = b.npaths; }
+    return b;
+  };
+  const bool cnt = h->countTraversal != 0;
+  auto light = [&](uint32_t k, std::vector<hipEvent_t>* ev) {
+    if (overlap && k >= 2) HIP_CHECK(hipStreamWaitEvent(ls, h->setFree[k & 1u], 0));
+    const uint32_t* point = h->sobolPoints.p + (size_t)k * YPT_SOBOL_POINT_DIMS;
+    BdptBuffers b = with_set(k, true);
+    if (h->bvhWidth == 8) { if (cnt) bdpt_light_stage<8, true>(h, S, point, b, ls, ev); else bdpt_light_stage<8, false>(h, S, point, b, ls, ev); }
+    else { if (cnt) bdpt_light_stage<2, true>(h, S, point, b, ls, ev); else bdpt_light_stage<2, false>(h, S, point, b, ls, ev); }
+    if (overlap) HIP_CHECK(hipEventRecord(h->lightDone[k & 1u], ls));
+  };
+  if (overlap) {  // the light stream starts after everything already queued on the handle's stream
+    HIP_CHECK(hipEventRecord(h->camReady, s));
+    HIP_CHECK(hipStreamWaitEvent(ls, h->camReady, 0));
+    if (sample_count > 0) light(0, nullptr);
+  }
   for (uint32_t k = 0; k < sample_count; ++k) {
     const uint32_t* point = h->sobolPoints.p + (size_t)k * YPT_SOBOL_POINT_DIMS;
     std::vector<hipEvent_t>* ev = nullptr;
     if (h->profile) { spans.emplace_back(); ev = &spans.back(); }
-    bool c = h->countTraversal != 0;
-    if (h->bvhWidth == 8) { if (c) bdpt_pass<8, true>(h, S, point, B, groups, ev); else bdpt_pass<8, false>(h, S, point, B, groups, ev); }
-    else { if (c) bdpt_pass<2, true>(h, S, point, B, groups, ev); else bdpt_pass<2, false>(h, S, point, B, groups, ev); }
+    if (overlap) {
+      if (k + 1 < sample_count) light(k + 1, nullptr);
+      HIP_CHECK(hipStreamWaitEvent(s, h->lightDone[k & 1u], 0));
+    } else {
+      light(k, ev);
+    }
+    BdptBuffers b = with_set(k, false);
+    if (h->bvhWidth == 8) { if (cnt) bdpt_camera_stage<8, true>(h, S, point, b, groups, ev); else bdpt_camera_stage<8, false>(h, S, point, b, groups, ev); }
+    else { if (cnt) bdpt_camera_stage<2, true>(h, S, point, b, groups, ev); else bdpt_camera_stage<2, false>(h, S, point, b, groups, ev); }
+    if (overlap) HIP_CHECK(hipEventRecord(h->setFree[k & 1u], s));
     size_t bytes = h->bSortBytes;
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(h->bSortTemp.p, bytes, h->bKeys.p, h->bKeysOut.p, h->bVals.p, h->bValsOut.p,
                                                  (int)nrec, 0, bits, s));
@@ -778,6 +839,15 @@ int ypt_destroy(ypt_integrator* h) {
     if (g.graph) (void)hipGraphDestroy(g.graph);
   }
   if (h->ownStream && h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->lightStream) {
+    (void)hipStreamSynchronize(h->lightStream);
+    for (int i = 0; i < 2; ++i) {
+      (void)hipEventDestroy(h->lightDone[i]);
+      (void)hipEventDestroy(h->setFree[i]);
+    }
+    (void)hipEventDestroy(h->camReady);
+    (void)hipStreamDestroy(h->lightStream);
+  }
   delete h;
   return YPT_OK;
 }
