@@ -381,7 +381,7 @@ __global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuff
   CameraPath c;
   camera_begin(fc, point, q & 0xFFFFu, q >> 16, lambda, hc == 0 && t == 0, c);
   store_cam(C, i, c, lp, count, false);
-  Q.ray[0][i] = i;
+  Q.ray[0][(q >> 16) * W + (q & 0xFFFFu)] = i;  // primary rays queued in screen order: coherent traversal
   if (features) {
     float* f = features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * 12;
     for (int k = 0; k < 12; ++k) f[k] = 0.f;
@@ -400,10 +400,12 @@ struct QueueSink {  // connection rays into the shadow queue, one atomic per wav
   }
 };
 
-// grid-stride over the ray queue with a loop bound uniform per workgroup (see k_shade)
+// grid-stride over the ray queue with a loop bound uniform per workgroup (see k_shade);
+// first: the first vertex, whose queue holds every slot; the slots are then shaded in slot (tile)
+// order rather than the screen order of the primary-ray queue
 template <int MINW>
 __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B,
-                                                          CamSoA C, Queues Q, int qsel, float* features) {
+                                                          CamSoA C, Queues Q, int qsel, int first, float* features) {
   if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;
   const uint32_t n = Q.counters[qsel];
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -413,7 +415,7 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
     bool cont = false;
     uint32_t slot = 0;
     if (i < n) {
-      slot = Q.ray[qsel][i];
+      slot = first ? i : Q.ray[qsel][i];
       CameraPath c;
       uint32_t lp, count;
       load_cam(C, slot, c, lp, count);

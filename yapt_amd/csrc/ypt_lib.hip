@@ -604,11 +604,11 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
   for (uint32_t k = 0; k < fc.maxRayDepth; ++k) {
     hipLaunchKernelGGL((k_extend_p<WIDTH, COUNT>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r);
     if (h->bdptWaves == 1)
-      hipLaunchKernelGGL(k_bdpt_cshade<1>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, feats);
+      hipLaunchKernelGGL(k_bdpt_cshade<1>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     else if (h->bdptWaves == 2)
-      hipLaunchKernelGGL(k_bdpt_cshade<2>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, feats);
+      hipLaunchKernelGGL(k_bdpt_cshade<2>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     else
-      hipLaunchKernelGGL(k_bdpt_cshade<4>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, feats);
+      hipLaunchKernelGGL(k_bdpt_cshade<4>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     hipLaunchKernelGGL((k_shadow_p<WIDTH, COUNT, true>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r);
     q ^= 1;
   }
