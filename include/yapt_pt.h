@@ -72,6 +72,24 @@ typedef struct ypt_material {
  * SceneData::objToWorld of PathIntegratorSubStage::sceneChanged), one record per instance
  * (render object x submesh) owning a contiguous triangle range. Mesh lights are the instances
  * whose material is emissive (LightManager.cpp:145). */
+/* Punctual lights (lightTypes.h:6-8) */
+#define YPT_LIGHT_POINT 1
+#define YPT_LIGHT_SPOT 2
+#define YPT_LIGHT_DIRECTIONAL 3
+
+/* A punctual light: PunctualLightParameters (LightManager.h:15-23), filled as
+ * LightManager::fillParameters does (LightManager.cpp:97-107): position and emission direction from
+ * the light's transform (direction = transform * (0, 0, -1)), intensity = color * max(0, intensity),
+ * and the cosines of the spot cone angles (setSpotConeAngles clamps outer to [0, pi/2] and inner to
+ * [0, outer], LightProxy.cpp:49-54). The direction is normalized here; a zero one becomes (0, 0, -1). */
+typedef struct ypt_punctual_light {
+  uint32_t type;                 /* YPT_LIGHT_* */
+  float position[3];             /* point, spot */
+  float direction[3];            /* spot, directional */
+  float intensity[3];
+  float cos_inner, cos_outer;    /* spot */
+} ypt_punctual_light;
+
 typedef struct ypt_scene_desc {
   const float* positions;       /* 3 * vertex_count */
   const float* normals;         /* 3 * vertex_count, or NULL for area-weighted vertex normals */
@@ -86,6 +104,8 @@ typedef struct ypt_scene_desc {
   uint32_t instance_count;
   const ypt_material* materials;
   uint32_t material_count;
+  const ypt_punctual_light* punctual_lights;  /* may be NULL when punctual_light_count is 0 */
+  uint32_t punctual_light_count;
 } ypt_scene_desc;
 
 /* Per-frame constants: the fields of RaytraceConstantData (raytraceCommonResources.hlsl:8-31) the

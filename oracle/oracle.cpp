@@ -430,7 +430,8 @@ int setup(Ctx& c, const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integr
   }
   prepare_scene(*sd, tables().r2s, c.scene);
   prepare_frame(*fd, c.scene, (uint32_t)integrator, c.fc);
-  if (fd->env_intensity_scale != 0.f && !(fd->flags & YPT_FLAG_WHITE_FURNACE) && tables().rgb2specFull.empty())
+  bool needTable = (fd->env_intensity_scale != 0.f && !(fd->flags & YPT_FLAG_WHITE_FURNACE)) || c.fc.punctualLightCount > 0;
+  if (needTable && tables().rgb2specFull.empty())
     tables().r2s.full_table(tables().rgb2specFull, std::max(1u, std::thread::hardware_concurrency()));
   c.nee = integrator == YPT_INTEGRATOR_NEE;
   c.geo.build(c.scene, useBvh != 0);

@@ -156,6 +156,28 @@ def test_bdpt_parity_shapes(ylib, oracle, shapes_small):
     _compare_render(ylib, oracle, scene, frame, ylib.YPT_INTEGRATOR_BDPT, passes=2, begin=4)
 
 
+@pytest.mark.parametrize("light_type,integrator", [("mixed", 0), ("mixed", 1), ("spot", 1), ("directional", 0)])
+def test_render_parity_punctual_lights(ylib, oracle, light_type, integrator):
+    # punctual lights sampled explicitly (BPT: punctual table only; NEE: all lights), occlusion rays
+    # without a target triangle
+    from yapt_amd import camera, scenes
+    scene, cam, env = scenes.punctual_lights(light_type, sphere_res=(12, 12))
+    frame = camera.make_frame(cam, 72, 44, max_depth=6, env_intensity_scale=env)
+    film = _compare_render(ylib, oracle, scene, frame, integrator, passes=3)
+    assert film[..., 1].max() > 0
+
+
+@pytest.mark.parametrize("light_type", ["mixed", "directional", "env"])
+def test_bdpt_parity_punctual_and_environment_lights(ylib, oracle, light_type):
+    # light subpaths from point, spot and directional lights, or from the environment when the scene
+    # has no lights; connections to origins without geometry
+    from yapt_amd import camera, scenes
+    scene, cam, env = scenes.punctual_lights(light_type, sphere_res=(12, 12))
+    frame = camera.make_frame(cam, 72, 48, max_depth=6, env_intensity_scale=env, light_path_vertices=6)
+    for wf in (1, 0):
+        _compare_render(ylib, oracle, scene, frame, ylib.YPT_INTEGRATOR_BDPT, passes=2, options={"bdpt_wavefront": wf})
+
+
 def test_bdpt_first_hit_features_match_nee(ylib, cornell):
     # the first camera vertex is the same surface point in both integrators (same texel jitter)
     from yapt_amd import camera

@@ -4,6 +4,7 @@ The shared library ``libyapt_pt.so`` is built in-tree by ``yapt_amd.build``. Loa
 when it is missing: there is no CPU fallback for the product path.
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -42,6 +43,64 @@ class Material(ctypes.Structure):
     ]
 
 
+YPT_LIGHT_POINT, YPT_LIGHT_SPOT, YPT_LIGHT_DIRECTIONAL = 1, 2, 3
+
+
+class PunctualLight(ctypes.Structure):
+    """ypt_punctual_light: PunctualLightParameters (LightManager.h:15-23)."""
+    _fields_ = [
+        ("type", ctypes.c_uint32), ("position", ctypes.c_float * 3), ("direction", ctypes.c_float * 3),
+        ("intensity", ctypes.c_float * 3), ("cos_inner", ctypes.c_float), ("cos_outer", ctypes.c_float),
+    ]
+
+
+class Light:
+    """Host-side light with the py_yapt Light surface (PyLight.cpp:17-35, LightProxy.cpp): type, color,
+    intensity, spot cone angles (outer clamped to [0, pi/2], inner to [0, outer]) and a look-at
+    transform. params() packs it as LightManager::fillParameters does (LightManager.cpp:97-107):
+    direction = transform * (0, 0, -1), intensity = color * max(0, intensity)."""
+    POINT, SPOT, DIRECTIONAL = YPT_LIGHT_POINT, YPT_LIGHT_SPOT, YPT_LIGHT_DIRECTIONAL
+
+    def __init__(self, light_type=YPT_LIGHT_POINT):
+        self.type = int(light_type)
+        self.color = np.ones(3, np.float32)
+        self.intensity = 1.0
+        self.spot_inner = 0.0
+        self.spot_outer = math.pi * 0.25
+        self.position = np.zeros(3, np.float32)
+        self.forward = np.array([0.0, 0.0, -1.0], np.float32)
+
+    def setType(self, t):
+        self.type = int(t)
+
+    def setColor(self, rgb):
+        self.color = np.asarray(rgb, np.float32)
+
+    def setIntensity(self, v):
+        self.intensity = float(v)
+
+    def setSpotConeAngles(self, inner, outer):
+        self.spot_outer = min(max(float(outer), 0.0), math.pi * 0.5)
+        self.spot_inner = min(max(float(inner), 0.0), self.spot_outer)
+
+    def lookAt(self, position, target):
+        """the light's -z axis towards target (Transform::lookAt)"""
+        self.position = np.asarray(position, np.float32)
+        d = np.asarray(target, np.float32) - self.position
+        n = float(np.linalg.norm(d))
+        self.forward = d / n if n > 0 else np.array([0.0, 0.0, -1.0], np.float32)
+
+    def params(self):
+        p = PunctualLight()
+        p.type = self.type
+        p.position[:] = [float(x) for x in self.position]
+        p.direction[:] = [float(x) for x in self.forward]
+        p.intensity[:] = [float(c) * max(0.0, self.intensity) for c in self.color]
+        p.cos_inner = math.cos(self.spot_inner)
+        p.cos_outer = math.cos(self.spot_outer)
+        return p
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [
         ("positions", ctypes.POINTER(ctypes.c_float)), ("normals", ctypes.POINTER(ctypes.c_float)),
@@ -50,6 +109,7 @@ class SceneDesc(ctypes.Structure):
         ("instance_triangle_count", ctypes.POINTER(ctypes.c_uint32)),
         ("instance_material", ctypes.POINTER(ctypes.c_uint32)), ("instance_count", ctypes.c_uint32),
         ("materials", ctypes.POINTER(Material)), ("material_count", ctypes.c_uint32),
+        ("punctual_lights", ctypes.POINTER(PunctualLight)), ("punctual_light_count", ctypes.c_uint32),
     ]
 
 
@@ -76,7 +136,8 @@ def _uptr(a):
 class SceneArrays:
     """Keeps the numpy arrays a SceneDesc points into alive."""
 
-    def __init__(self, positions, normals, indices, inst_first, inst_count, inst_material, materials, tangents=None):
+    def __init__(self, positions, normals, indices, inst_first, inst_count, inst_material, materials, tangents=None,
+                 lights=()):
         self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
         self.tangents = None if tangents is None else np.ascontiguousarray(tangents, dtype=np.float32).reshape(-1, 4)
@@ -85,11 +146,14 @@ class SceneArrays:
         self.inst_count = np.ascontiguousarray(inst_count, dtype=np.uint32)
         self.inst_material = np.ascontiguousarray(inst_material, dtype=np.uint32)
         self.materials = (Material * len(materials))(*materials)
+        self.lights = [l if isinstance(l, PunctualLight) else l.params() for l in lights]
+        self.light_array = (PunctualLight * max(1, len(self.lights)))(*self.lights)
         self.desc = SceneDesc(
             _fptr(self.positions), None if self.normals is None else _fptr(self.normals),
             None if self.tangents is None else _fptr(self.tangents), len(self.positions),
             _uptr(self.indices), len(self.indices), _uptr(self.inst_first), _uptr(self.inst_count),
-            _uptr(self.inst_material), len(self.inst_first), self.materials, len(materials))
+            _uptr(self.inst_material), len(self.inst_first), self.materials, len(materials),
+            self.light_array, len(self.lights))
 
     @property
     def triangle_count(self):

@@ -30,8 +30,11 @@ static constexpr float BDPT_VISIBILITY_EPS = 0.001f;
 // LIGHT_PATH_NODE_FLAG_* (BDPTCommon.hlsl:14-28)
 static constexpr uint32_t LN_SECONDARY_TERMINATED = 1u << 0;
 static constexpr uint32_t LN_FRONT_FACE = 1u << 1;
+static constexpr uint32_t LN_ENV = 1u << 2;
 static constexpr uint32_t LN_DELTA = 1u << 3;
 static constexpr uint32_t LN_REGULARIZED = 1u << 4;
+static constexpr uint32_t LN_PUNCTUAL = 1u << 6;
+static constexpr uint32_t LN_DIRECTIONAL = 1u << 7;
 
 // ---------------------------------------------------------------- half precision of the packed normal
 // f32tof16 / f16tof32 with round-to-nearest-even; the packed node keeps its normal at this precision.
@@ -324,6 +327,93 @@ YPT_HD float pdf_environment_position(const FrameConstants& fc) {
   return safe_div(1.f, PI * dot(ext, ext));
 }
 
+// getWorldCenterAndRadiusSqr (raytraceCommonResources.hlsl:99-104)
+YPT_HD vec3 world_center_radius_sqr(const FrameConstants& fc, float& radiusSqr) {
+  vec3 lo(fc.worldBoundsMin[0], fc.worldBoundsMin[1], fc.worldBoundsMin[2]);
+  vec3 hi(fc.worldBoundsMax[0], fc.worldBoundsMax[1], fc.worldBoundsMax[2]);
+  vec3 c = (hi + lo) * 0.5f;
+  vec3 ext = hi - c;
+  radiusSqr = dot(ext, ext);
+  return c;
+}
+
+// environmentSelectionPDF (lightSampling.hlsl:25-28): light subpaths start from the environment only
+// when the scene has no lights
+YPT_HD float environment_selection_pdf(const FrameConstants& fc) { return fc.lightCount == 0 ? 1.f : 0.f; }
+
+// pdfEnvironmentDirection (lightSampling.hlsl:64-70): directions uniform in spherical coordinates about +z
+YPT_HD float pdf_environment_direction(vec3 dir) {
+  float cosTheta = clampf(dir.z, -1.f, 1.f);
+  float sinTheta = sqrtf(fmax_(0.f, 1.f - cosTheta * cosTheta));
+  return safe_div(1.f, 2.f * PI * PI * sinTheta);
+}
+
+// getSkyBoxColor with no environment texture, times the scale (as the miss shader sees it)
+YPT_HD vec3 sky_color(const FrameConstants& fc, vec3 dir) {
+  if (fc.flags & FRAME_FLAG_WHITE_FURNACE) return vec3(1.f);
+  return vabs(dir) * fc.envIntensityScale;
+}
+
+// encodeOctahedral / decodeOctahedral (commonMath.hlsl:96-117)
+YPT_HD void encode_octahedral(vec3 n, float& px, float& py) {
+  n = n / (fabsf(n.x) + fabsf(n.y) + fabsf(n.z));
+  px = n.x; py = n.y;
+  if (n.z < 0.f) {
+    float sx = px >= 0.f ? 1.f : -1.f, sy = py >= 0.f ? 1.f : -1.f;
+    float ax = (1.f - fabsf(py)) * sx, ay = (1.f - fabsf(px)) * sy;
+    px = ax; py = ay;
+  }
+}
+YPT_HD vec3 decode_octahedral(float px, float py) {
+  vec3 n(px, py, 1.f - fabsf(px) - fabsf(py));
+  if (n.z < 0.f) {
+    float sx = n.x >= 0.f ? 1.f : -1.f, sy = n.y >= 0.f ? 1.f : -1.f;
+    float ax = (1.f - fabsf(n.y)) * sx, ay = (1.f - fabsf(n.x)) * sy;
+    n.x = ax; n.y = ay;
+  }
+  return normalize(n);
+}
+
+// sampleSphere, sampleUniformCone, pdfUniformCone (miscBrdf.hlsl:70-94)
+YPT_HD vec3 sample_sphere(float u0, float u1) {
+  float y = 1.f - 2.f * u0;
+  float r = sqrtf(1.f - sqr(y));
+  float s, c;
+  ysincos(2.f * PI * u1, &s, &c);
+  return vec3(r * c, y, r * s);
+}
+YPT_HD vec3 sample_uniform_cone(float u0, float u1, float cosMax) {
+  float y = 1.f - u0 * (1.f - cosMax);
+  float r = sqrtf(fmax_(0.f, 1.f - sqr(y)));
+  float s, c;
+  ysincos(2.f * PI * u1, &s, &c);
+  return vec3(r * c, y, r * s);
+}
+YPT_HD float pdf_uniform_cone(float cosMax) { return safe_div(1.f, 2.f * PI * (1.f - cosMax)); }
+YPT_HD float pdf_sphere() { return 1.f / (4.f * PI); }
+
+// pdfPunctualLightEmissionDirection (lightSampling.hlsl:294-307): 1 for directional lights (delta)
+YPT_HD float pdf_punctual_emission_direction(const LightGPU& l, vec3 dirFromLight) {
+  if (l.type == LIGHT_TYPE_SPOT) {
+    vec3 axis(l.directionCosOuter[0], l.directionCosOuter[1], l.directionCosOuter[2]);
+    return dot(dirFromLight, axis) >= l.directionCosOuter[3] ? pdf_uniform_cone(l.directionCosOuter[3]) : 0.f;
+  }
+  if (l.type == LIGHT_TYPE_POINT) return pdf_sphere();
+  return 1.f;
+}
+
+// direction in which an infinite light origin (directional light or environment) emits
+// (getInfiniteLightNodeDirection, BDPTCommon.hlsl:600-607); environment nodes keep it octahedral
+// encoded in their barycentrics
+YPT_HD bool infinite_node(uint32_t flags) { return (flags & (LN_DIRECTIONAL | LN_ENV)) != 0; }
+YPT_HD vec3 infinite_node_direction(const SceneView& S, const LightNode& n) {
+  if (n.flags & LN_DIRECTIONAL) {
+    const LightGPU& l = S.lights[n.tri];
+    return vec3(l.directionCosOuter[0], l.directionCosOuter[1], l.directionCosOuter[2]);
+  }
+  return decode_octahedral(n.b1, n.b2);
+}
+
 // ---------------------------------------------------------------- light subpath
 YPT_HD uint32_t camera_dim_offset(const FrameConstants& fc) { return fc.lightPathVertices * 5u + 12u; }      // :226-227
 YPT_HD uint32_t pure_lens_dim_offset(const FrameConstants& fc) { return fc.lightPathVertices * 5u + 9u; }    // :229
@@ -341,57 +431,132 @@ YPT_HD uint32_t trace_light_path(const SceneView& S, const FrameConstants& fc, c
   float r0[4], r1[4];
   rand.r4(r0);
   rand.r4(r1);
-  if (fc.lightCount == 0) return 0;
-
-  // sampleLightOrEnv + sampleLight (lightSampling.hlsl:122-189)
-  float pick;
-  uint32_t li = sample_light_index(S.lights, fc.lightCount, r1[3], pick);
-  const LightGPU& light = S.lights[li];
-  uint32_t primCount = light.triCount;
-  uint32_t prim = (uint32_t)fmin_((float)primCount * r0[2], (float)(primCount - 1u));
-  float bx = r0[0], by = r0[1];
-  if (bx + by > 1.f) { bx = 1.f - bx; by = 1.f - by; }
-  uint32_t tri = light.firstTri + prim;
-  TriGeometry g;
-  fetch_triangle(S, tri, bx, by, g);
-  vec3 gn = cross(g.p2 - g.p1, g.p3 - g.p1);
-  float gl = length(gn);
-  gn = gn / gl;
-  float area = gl * 0.5f;
-  float b0 = 1.f - bx - by;
-  vec3 lightPos = b0 * g.p1 + bx * g.p2 + by * g.p3;
-  vec3 tangent = normalize(g.tangent);
-  const MaterialGPU& lm = S.mats[light.material];
-  bool twoSided = surface_two_sided(lm.flags);
-  float focus = lm.emissionFocus;
-  vec3 local = twoSided ? sample_cosine_power_sphere(r1[0], r1[1], focus) : sample_cosine_power_hemisphere(r1[0], r1[1], focus);
-  mat3 tanToWS = basis_transform(gn, tangent);
-  Spectrum radiance = spectrum_unbounded(lm.emissive, L, S.spec);
-  float pdfPos = 1.f / ((float)primCount * area);
-  float pdfDir = twoSided ? pdf_cosine_power_sphere(local.y, focus) : pdf_cosine_power_hemisphere(local.y, focus);
-  float profile = emission_profile(local.y, focus);
-  vec3 lightDir = mul(local, tanToWS);
+  // sampleLightOrEnv (BDPTCommon.hlsl:680-720): the environment when the scene has no lights, else a
+  // light by the alias table, sampled by sampleLight (mesh) or samplePunctualLightEmission
+  float envPdf = environment_selection_pdf(fc);
+  Spectrum radiance;
+  vec3 lightPos, lightDir, lightNormal;
+  float pdfPos, pdfDir, pick, profile;
+  uint32_t originTri = NO_HIT, originFlags = 0;
+  float originB1 = -1.f, originB2 = -1.f;
+  if (r1[3] < envPdf) {
+    // sampleEnvironmentLighting (lightSampling.hlsl:78-120)
+    float st, ct, sp, cp;
+    ysincos(r0[0] * PI, &st, &ct);
+    ysincos(r0[1] * 2.f * PI, &sp, &cp);
+    lightDir = vec3(st * cp, st * sp, ct);
+    pdfDir = safe_div(1.f, 2.f * PI * PI * st);
+    float rsq;
+    vec3 c = world_center_radius_sqr(fc, rsq);
+    vec3 v1, v2;
+    construct_vector_base(lightDir, v1, v2);
+    float cx, cy;
+    sample_concentric_disk(r0[2], r0[3], cx, cy);
+    float radius = sqrtf(rsq);
+    lightPos = c - lightDir * radius + radius * (cx * v1 + cy * v2);
+    pdfPos = safe_div(1.f, PI * rsq);
+    radiance = rgb_unbounded_spectrum(S, sky_color(fc, -lightDir), L);
+    lightNormal = lightDir;  // the disk faces the emitted direction
+    profile = 1.f;
+    pick = envPdf;
+    originFlags = LN_ENV;
+    encode_octahedral(lightDir, originB1, originB2);
+  } else {
+    uint32_t li = sample_light_index(S.lights, fc.lightCount, (r1[3] - envPdf) / (1.f - envPdf), pick);
+    pick = (1.f - envPdf) * pick;
+    const LightGPU& light = S.lights[li];
+    if (light.type != LIGHT_TYPE_MESH) {
+      // samplePunctualLightEmission (lightSampling.hlsl:311-357) with random values (r1.x, r1.y, r0.x, r0.y)
+      radiance = rgb_unbounded_spectrum(S, vec3(light.intensity[0], light.intensity[1], light.intensity[2]), L);
+      originTri = li;
+      originFlags = LN_PUNCTUAL;
+      vec3 axis(light.directionCosOuter[0], light.directionCosOuter[1], light.directionCosOuter[2]);
+      if (light.type == LIGHT_TYPE_DIRECTIONAL) {
+        originFlags |= LN_DIRECTIONAL;
+        float rsq;
+        vec3 c = world_center_radius_sqr(fc, rsq);
+        float radius = sqrtf(rsq);
+        vec3 v1, v2;
+        construct_vector_base(axis, v1, v2);
+        float cx, cy;
+        sample_concentric_disk(r1[0], r1[1], cx, cy);
+        lightPos = c - axis * radius + radius * (cx * v1 + cy * v2);
+        lightDir = axis;
+        pdfPos = safe_div(1.f, PI * rsq);  // pdfDirectionalLightPosition
+        pdfDir = 1.f;
+        profile = 1.f;
+      } else {
+        lightPos = vec3(light.positionCosInner[0], light.positionCosInner[1], light.positionCosInner[2]);
+        pdfPos = 1.f;
+        if (light.type == LIGHT_TYPE_SPOT) {
+          vec3 v1, v2;
+          construct_vector_base(axis, v1, v2);
+          vec3 local = sample_uniform_cone(r1[0], r1[1], light.directionCosOuter[3]);
+          lightDir = normalize(local.x * v1 + local.y * axis + local.z * v2);
+          pdfDir = pdf_uniform_cone(light.directionCosOuter[3]);
+          profile = punctual_falloff(light, lightDir);
+        } else {
+          lightDir = sample_sphere(r1[0], r1[1]);
+          pdfDir = pdf_sphere();
+          profile = 1.f;
+        }
+      }
+      lightNormal = lightDir;  // the cosine at the light is 1
+    } else {
+      // sampleLight (lightSampling.hlsl:122-189)
+      uint32_t primCount = light.triCount;
+      uint32_t prim = (uint32_t)fmin_((float)primCount * r0[2], (float)(primCount - 1u));
+      float bx = r0[0], by = r0[1];
+      if (bx + by > 1.f) { bx = 1.f - bx; by = 1.f - by; }
+      uint32_t tri = light.firstTri + prim;
+      TriGeometry g;
+      fetch_triangle(S, tri, bx, by, g);
+      vec3 gn = cross(g.p2 - g.p1, g.p3 - g.p1);
+      float gl = length(gn);
+      gn = gn / gl;
+      float area = gl * 0.5f;
+      float b0 = 1.f - bx - by;
+      lightPos = b0 * g.p1 + bx * g.p2 + by * g.p3;
+      vec3 tangent = normalize(g.tangent);
+      const MaterialGPU& lm = S.mats[light.material];
+      bool twoSided = surface_two_sided(lm.flags);
+      float focus = lm.emissionFocus;
+      vec3 local = twoSided ? sample_cosine_power_sphere(r1[0], r1[1], focus) : sample_cosine_power_hemisphere(r1[0], r1[1], focus);
+      mat3 tanToWS = basis_transform(gn, tangent);
+      radiance = spectrum_unbounded(lm.emissive, L, S.spec);
+      pdfPos = 1.f / ((float)primCount * area);
+      pdfDir = twoSided ? pdf_cosine_power_sphere(local.y, focus) : pdf_cosine_power_hemisphere(local.y, focus);
+      profile = emission_profile(local.y, focus);
+      lightDir = mul(local, tanToWS);
+      lightNormal = gn;
+      originTri = tri;
+    }
+  }
   if (radiance.all_equal(0.f) || pdfPos == 0.f || pdfDir == 0.f || pick == 0.f) return 0;
   float maxRadiance = radiance.max_value();
+  const bool originPunctual = (originFlags & LN_PUNCTUAL) != 0;
+  const bool originInfinite = infinite_node(originFlags);
+  const bool originEnv = (originFlags & LN_ENV) != 0;
 
   LightNode nodes[2];
   nodes[0].radiance = radiance;
   nodes[0].pos = lightPos;
-  nodes[0].nrm = gn;
-  nodes[0].tri = tri;
-  nodes[0].b1 = -1.f;
-  nodes[0].b2 = -1.f;
-  nodes[0].pdfFwd = pdfPos * pick;
+  nodes[0].nrm = lightNormal;
+  nodes[0].tri = originTri;
+  nodes[0].b1 = originB1;
+  nodes[0].b2 = originB2;
+  // for the environment the origin is the direction: its density is the directional one
+  nodes[0].pdfFwd = originEnv ? pdfDir * pick : pdfPos * pick;
   nodes[0].pdfBwd = 0.f;
   nodes[0].iorPrev = 1.f;
   nodes[0].iorCur = 1.f;
-  nodes[0].flags = 0;
+  nodes[0].flags = originFlags;
   nodes[1] = nodes[0];
   nodes[1].pdfBwd = 0.f;
   float pdfForward = pdfDir;
   vec3 rayDir = lightDir;
-  vec3 rayOrigin = spawn_ray_origin(lightPos, gn, lightDir);
-  radiance = radiance * profile * fabsf(dot(gn, lightDir)) / (pick * pdfPos * pdfDir);
+  vec3 rayOrigin = spawn_ray_origin(lightPos, lightNormal, lightDir);
+  radiance = radiance * profile * fabsf(dot(lightNormal, lightDir)) / (pick * pdfPos * pdfDir);
 
   uint32_t stored = 0, nodeCount = 1;
   bool previousWasDelta = false;
@@ -411,7 +576,9 @@ YPT_HD uint32_t trace_light_path(const SceneView& S, const FrameConstants& fc, c
     if (rs.flags & RS_SECONDARY_TERMINATED) flags |= LN_SECONDARY_TERMINATED;
     if (rs.flags & RS_REGULARIZE) flags |= LN_REGULARIZED;
     if (is_delta_distribution(v.ctx.pre.ax, v.ctx.pre.ay, v.ctx.def)) flags |= LN_DELTA;
-    float curFwd = previousWasDelta ? -1.f : pdfForward * solid_angle_to_area(v.pos - rayOrigin, v.normalWS);
+    float curFwd;
+    if (i == 1u && originInfinite) curFwd = pdfPos * fabsf(dot(rayDir, v.normalWS));  // planar density of the disk
+    else curFwd = previousWasDelta ? -1.f : pdfForward * solid_angle_to_area(v.pos - rayOrigin, v.normalWS);
     Spectrum ab = rs.get_absorption();
     if (!ab.all_equal(0.f)) radiance = radiance * transmittance(h.t, ab);
     nodes[ci].radiance = radiance;
@@ -433,6 +600,8 @@ YPT_HD uint32_t trace_light_path(const SceneView& S, const FrameConstants& fc, c
     pdfForward = ev.pdfFwd;
     Spectrum sampleWeight = ev.weight * safe_div(1.f, pdfForward);
     nodes[pi].pdfBwd = isDelta ? -1.f : ev.pdfBwd * solid_angle_to_area(rayOrigin - v.pos, nodes[pi].nrm);
+    if (i == 1u && originPunctual) nodes[pi].pdfBwd = 0.f;                 // a camera path never hits a delta light
+    else if (i == 1u && originEnv && !isDelta) nodes[pi].pdfBwd = ev.pdfBwd;  // found by direction: stays per solid angle
     out[stored++] = pack_node(nodes[pi]);
     ++nodeCount;
     radiance = radiance * sampleWeight;
@@ -499,13 +668,47 @@ struct NodeSurfaceTable {
 
 // calculateConnectingLightNodeWeightsAndPDFs (:701-875) for a mesh-light origin (index 0) or a
 // surface vertex with its surface `ls`; false when the connection carries nothing.
-YPT_HD bool connect_light_node(const SceneView& S, const LightNode& n, uint32_t index, const LightNodeSurface* ls, vec3 nextPos,
-                               bool secondaryTerminated, const Lambdas& L, ScatterEval& ev, float& pdfFwdMIS) {
+YPT_HD bool connect_light_node(const SceneView& S, const FrameConstants& fc, const LightNode& n, uint32_t index,
+                               const LightNodeSurface* ls, vec3 nextPos, bool secondaryTerminated, const Lambdas& L,
+                               ScatterEval& ev, float& pdfFwdMIS) {
   ev.weight.set(0.f);
   ev.pdfFwd = 0.f;
   ev.pdfBwd = 0.f;
   ev.dispersed = false;
   bool frontFace = (n.flags & LN_FRONT_FACE) != 0;
+  if (index == 0 && (n.flags & LN_PUNCTUAL)) {
+    // punctual origin: the node radiance is an intensity (point, spot) or irradiance (directional)
+    const LightGPU& light = S.lights[n.tri];
+    float pick = (1.f - environment_selection_pdf(fc)) * light.selectionPdf;
+    if (light.type == LIGHT_TYPE_DIRECTIONAL) {
+      ev.pdfFwd = pick;
+      ev.weight.set(1.f / pick);
+      float rsq;
+      world_center_radius_sqr(fc, rsq);
+      pdfFwdMIS = safe_div(1.f, PI * rsq);  // planar density reaching the next vertex
+    } else {
+      vec3 fromLight = nextPos - n.pos;
+      float distSqr = fmax_(dot(fromLight, fromLight), 1e-10f);
+      vec3 fromLightDir = fromLight * (1.f / sqrtf(distSqr));
+      float falloff = punctual_falloff(light, fromLightDir);
+      if (falloff <= 0.f) { pdfFwdMIS = 0.f; return false; }
+      ev.pdfFwd = pick * distSqr;
+      ev.weight.set(falloff / ev.pdfFwd);
+      pdfFwdMIS = pdf_punctual_emission_direction(light, fromLightDir);
+    }
+    ev.pdfBwd = 0.f;
+    return true;
+  }
+  if (index == 0 && (n.flags & LN_ENV)) {
+    // environment origin: connecting along its direction is like sampling that direction
+    float pdfDir = environment_selection_pdf(fc) * pdf_environment_direction(infinite_node_direction(S, n));
+    if (pdfDir <= 0.f) { pdfFwdMIS = 0.f; return false; }
+    ev.pdfFwd = pdfDir;
+    ev.weight.set(1.f / pdfDir);
+    ev.pdfBwd = 0.f;
+    pdfFwdMIS = pdf_environment_position(fc);
+    return true;
+  }
   if (index == 0) {
     const MaterialGPU& m = S.mats[S.inst[S.triInst[n.tri]].material];
     bool twoSided = surface_two_sided(m.flags);
@@ -543,6 +746,7 @@ struct LightVertexCtx {  // LightVertexContext (:898-906)
   float prevFwdMIS;
   float riSum;
   uint32_t prevFlags;
+  uint32_t originFlags;  // flags of the subpath's light origin (vertex 0)
 };
 
 struct ConnectionPdfs {
@@ -550,25 +754,44 @@ struct ConnectionPdfs {
 };
 
 // calculateMISWeight (:918-1007)
-YPT_HD float mis_connection(const SceneView& S, uint32_t camIndex, const CameraVertexCtx& cv, uint32_t lightIndex,
-                            const LightVertexCtx& lv, const LightNode& n, const ConnectionPdfs& p) {
+YPT_HD float mis_connection(const SceneView& S, const FrameConstants& fc, uint32_t camIndex, const CameraVertexCtx& cv,
+                            uint32_t lightIndex, const LightVertexCtx& lv, const LightNode& n, const ConnectionPdfs& p) {
   float riCam = cv.pdfRatioSum;
   if (camIndex > 0) {
     float pb = p.bwdCam * solid_angle_to_area(cv.prevPos - cv.pos, cv.prevNrm);
     riCam = accumulate_ri(riCam, pb, cv.prevFwdMIS);
   }
-  riCam = accumulate_ri(riCam, p.fwdLight * solid_angle_to_area(cv.pos - n.pos, cv.normalWS), cv.curFwdMIS);
+  float toCamArea;
+  if (lightIndex == 0 && infinite_node(n.flags))  // from a disk: planar density times the cosine
+    toCamArea = p.fwdLight * fabsf(dot(cv.normalWS, infinite_node_direction(S, n)));
+  else
+    toCamArea = p.fwdLight * solid_angle_to_area(cv.pos - n.pos, cv.normalWS);
+  riCam = accumulate_ri(riCam, toCamArea, cv.curFwdMIS);
   float riLight = lv.riSum;
   if (lightIndex == 0) {
-    vec3 camToLight = n.pos - cv.pos;
-    float len = length(camToLight);
-    vec3 lightToCam = (-camToLight) / len;
-    float pick, posPdf, dirPdf;
-    pdf_sampling_light(S, n.tri, n.nrm, lightToCam, pick, posPdf, dirPdf);
-    float toLight = p.fwdCam * solid_angle_to_area(camToLight, n.nrm);
+    float pick, posPdf, dirPdf, toLight;
+    if (n.flags & LN_PUNCTUAL) {  // a camera path never hits a delta light
+      pick = 1.f;
+      posPdf = 1.f;
+      toLight = 0.f;
+    } else if (n.flags & LN_ENV) {  // found by the same direction: both densities per solid angle
+      pick = environment_selection_pdf(fc);
+      posPdf = pdf_environment_direction(infinite_node_direction(S, n));
+      toLight = p.fwdCam;
+    } else {
+      vec3 camToLight = n.pos - cv.pos;
+      float len = length(camToLight);
+      vec3 lightToCam = (-camToLight) / len;
+      pdf_sampling_light(S, n.tri, n.nrm, lightToCam, pick, posPdf, dirPdf);
+      toLight = p.fwdCam * solid_angle_to_area(camToLight, n.nrm);
+    }
     riLight = pdf_ratio(toLight, pick * posPdf);
   } else {
-    float pbl = p.bwdLight * solid_angle_to_area(lv.prevPos - n.pos, lv.prevNrm);
+    bool prevPunctual = lightIndex == 1 && (lv.originFlags & LN_PUNCTUAL) != 0;
+    bool prevEnv = lightIndex == 1 && (lv.originFlags & LN_ENV) != 0;
+    float pbl = p.bwdLight;
+    if (prevPunctual) pbl = 0.f;
+    else if (!prevEnv) pbl = pbl * solid_angle_to_area(lv.prevPos - n.pos, lv.prevNrm);
     riLight = accumulate_ri(riLight, pbl, lv.prevFwdMIS);
     riLight = accumulate_ri(riLight, p.fwdCam * solid_angle_to_area(n.pos - cv.pos, n.nrm), n.pdfFwd);
   }
@@ -588,13 +811,13 @@ YPT_HD float mis_camera_hits_light(const SceneView& S, uint32_t camIndex, uint32
   return 1.f / (1.f + riCam);
 }
 
-// calculateMISWeightOnCameraPathMissing (:1040-1058), environment selection probability 0
+// calculateMISWeightOnCameraPathMissing (:1040-1058)
 YPT_HD float mis_camera_misses(const FrameConstants& fc, uint32_t camIndex, const CameraVertexCtx& cv, vec3 rayDir) {
   if (camIndex == 0) return 1.f;
   float riCam = cv.pdfRatioSum;
   float pdfEnvToLast = pdf_environment_position(fc) * fabsf(dot(cv.prevNrm, rayDir));
   riCam = accumulate_ri(riCam, pdfEnvToLast, cv.prevFwdMIS);
-  float pdfEnvDirection = 0.f;
+  float pdfEnvDirection = environment_selection_pdf(fc) * pdf_environment_direction(-rayDir);
   riCam = accumulate_ri(riCam, pdfEnvDirection, cv.curFwdMIS);
   return 1.f / (1.f + riCam);
 }
@@ -611,7 +834,8 @@ struct PendingConnection {
 // calculateLightPathRadiance (:1060-1185) up to its visibility ray; rs is the camera subpath's
 // state, taken by value. Returns false when the connection carries nothing. Visibility only gates
 // the result, so testing it after the MIS weight changes no bit of the contribution.
-YPT_HD bool connection_prepare(const SceneView& S, uint32_t camIndex, const CameraVertexCtx& cv, const SurfaceContext& camCtx,
+YPT_HD bool connection_prepare(const SceneView& S, const FrameConstants& fc, uint32_t camIndex, const CameraVertexCtx& cv,
+                               const SurfaceContext& camCtx,
                                const LightNode& n, uint32_t lightIndex, const LightNodeSurface* ls, const LightVertexCtx& lv,
                                BdptState rs, const Lambdas& L, PendingConnection& out) {
   bool camTerminated = (rs.flags & RS_SECONDARY_TERMINATED) != 0;
@@ -625,21 +849,36 @@ YPT_HD bool connection_prepare(const SceneView& S, uint32_t camIndex, const Came
   p.fwdCam = ec.pdfFwd;
   p.bwdCam = ec.pdfBwd;
   ScatterEval el;
-  if (!connect_light_node(S, n, lightIndex, ls, cv.pos, (rs.flags & RS_SECONDARY_TERMINATED) != 0, L, el, p.fwdLight))
+  if (!connect_light_node(S, fc, n, lightIndex, ls, cv.pos, (rs.flags & RS_SECONDARY_TERMINATED) != 0, L, el, p.fwdLight))
     return false;
   Spectrum wLight = el.weight;
   p.bwdLight = el.pdfBwd;
-  vec3 toNode = n.pos - cv.pos;
-  float len = length(toNode);
-  out.dir = toNode / len;
+  // getLightNodeConnection (:616-630): infinite origins are reached along their direction; origins
+  // without geometry are visible when nothing is hit on the way
+  bool atInfinity = lightIndex == 0 && infinite_node(n.flags);
+  bool noGeometry = lightIndex == 0 && (n.flags & (LN_PUNCTUAL | LN_ENV)) != 0;
+  float len;
+  if (atInfinity) {
+    out.dir = -infinite_node_direction(S, n);
+    len = __builtin_inff();
+  } else {
+    vec3 toNode = n.pos - cv.pos;
+    len = length(toNode);
+    out.dir = toNode / len;
+  }
   out.origin = spawn_ray_origin(cv.pos, cv.normalWS, out.dir);
-  out.tmax = RAY_MIN_T + len + BDPT_VISIBILITY_EPS;
-  out.tri = n.tri;
-  float mis = mis_connection(S, camIndex, cv, lightIndex, lv, n, p);
+  if (noGeometry) {
+    out.tmax = atInfinity ? __builtin_inff() : len - RAY_MIN_T;
+    out.tri = NO_HIT;
+  } else {
+    out.tmax = RAY_MIN_T + len + BDPT_VISIBILITY_EPS;
+    out.tri = n.tri;
+  }
+  float mis = mis_connection(S, fc, camIndex, cv, lightIndex, lv, n, p);
   float geom = 1.f;
   if (lightIndex > 0) geom = 1.f / (len * len);
   Spectrum ab = rs.get_absorption();
-  if (!ab.all_equal(0.f)) wCam = wCam * transmittance(len, ab);
+  if (!atInfinity && !ab.all_equal(0.f)) wCam = wCam * transmittance(len, ab);
   Spectrum r = wLight * wCam * cv.throughput * n.radiance * geom * mis;
   if (camTerminated && lightTerminated) r.undo_secondary_rescale();
   out.contrib = r;
@@ -748,6 +987,7 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
     LightVertexCtx lv;
     lv.riSum = 0.f;
     lv.prevFlags = 0;
+    lv.originFlags = 0;
     lv.prevPos = vec3(0.f);
     lv.prevNrm = vec3(0.f);
     lv.prevFwdMIS = 0.f;
@@ -755,12 +995,16 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
     for (uint32_t j = 0; j < nodeCount; ++j) {
       LightNode n = unpack_node(nodes[j]);
       if ((n.flags & LN_DELTA) == 0) {
-        vec3 toNode = n.pos - v.pos;
-        float dist = length(toNode);
-        cv.wi = normalize(toNode / dist);
+        if (j == 0 && infinite_node(n.flags)) {
+          cv.wi = normalize(-infinite_node_direction(S, n));
+        } else {
+          vec3 toNode = n.pos - v.pos;
+          float dist = length(toNode);
+          cv.wi = normalize(toNode / dist);
+        }
         const LightNodeSurface* ls = j > 0 ? surfs.get(S, n, j, lv.prevPos, L) : nullptr;
         PendingConnection pc;
-        if (connection_prepare(S, k, cv, v.ctx, n, j, ls, lv, c.rs, L, pc)) sink(c, pc);
+        if (connection_prepare(S, fc, k, cv, v.ctx, n, j, ls, lv, c.rs, L, pc)) sink(c, pc);
       }
       lv.riSum = deferredLight;
       deferredLight = accumulate_ri(deferredLight, n.pdfBwd, n.pdfFwd);
@@ -768,6 +1012,7 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
       lv.prevPos = n.pos;
       lv.prevFwdMIS = n.pdfFwd;
       lv.prevFlags = n.flags;
+      if (j == 0) lv.originFlags = n.flags;
     }
   }
   c.throughput = cv.throughput;
@@ -800,7 +1045,9 @@ template <class Tracer>
 struct TraceNowSink {  // tests visibility at once and adds visible contributions in order
   Tracer* tr;
   YPT_HD void operator()(CameraPath& c, const PendingConnection& pc) {
-    if (tr->hits_target(pc.origin, pc.dir, RAY_MIN_T, pc.tmax, pc.tri)) c.total = c.total + pc.contrib;
+    bool visible = pc.tri == NO_HIT ? tr->unoccluded(pc.origin, pc.dir, RAY_MIN_T, pc.tmax)
+                                       : tr->hits_target(pc.origin, pc.dir, RAY_MIN_T, pc.tmax, pc.tri);
+    if (visible) c.total = c.total + pc.contrib;
   }
 };
 
@@ -888,14 +1135,17 @@ YPT_HD void splat_light_path(const SceneView& S, const FrameConstants& fc, const
     if (project_to_uv(fc, lensPosVS, n.pos, u, v) && (n.flags & LN_DELTA) == 0) {
       ScatterEval ev;
       float unused;
-      if (connect_light_node(S, n, i, surfs.get(S, n, i, prevPos, L), lensPosWS, false, L, ev, unused) &&
+      if (connect_light_node(S, fc, n, i, surfs.get(S, n, i, prevPos, L), lensPosWS, false, L, ev, unused) &&
           !ev.weight.all_equal(0.f)) {
         // checkVisibility (:70-101): from the vertex to the lens point, t in [0.01, distance]
         vec3 fromTo = lensPosWS - n.pos;
         float len = sqrtf(dot(fromTo, fromTo));
         bool visible = len < 0.0001f || tr.unoccluded(n.pos, fromTo / len, 0.01f, len);
         if (visible) {
-          float pbPrev = ev.pdfBwd * solid_angle_to_area(prevPos - n.pos, prevNrm);
+          bool prevEnv = i == 1 && (n0.flags & LN_ENV) != 0;
+          bool prevPunctual = i == 1 && (n0.flags & LN_PUNCTUAL) != 0;
+          float pbPrev = prevEnv ? ev.pdfBwd : ev.pdfBwd * solid_angle_to_area(prevPos - n.pos, prevNrm);
+          if (prevPunctual) pbPrev = 0.f;
           vec3 camToNode = n.pos - lensPosWS;
           float pdfCam = camera_dir_pdf(fc, camToNode);
           float ri = accumulate_ri(riSum, pbPrev, prevFwd);

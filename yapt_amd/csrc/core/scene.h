@@ -16,15 +16,29 @@ struct InstanceGPU {
   float lightSelectionPdf;  // g_instanceLightSelectionPDFs: 0 for instances that are not lights
 };
 
-// LightEntryGPU (materialsCommonResources.hlsl:51-71), mesh lights.
+// light types (lightTypes.h:5-8)
+static constexpr uint32_t LIGHT_TYPE_MESH = 0;
+static constexpr uint32_t LIGHT_TYPE_POINT = 1;
+static constexpr uint32_t LIGHT_TYPE_SPOT = 2;
+static constexpr uint32_t LIGHT_TYPE_DIRECTIONAL = 3;
+
+// LightEntryGPU (materialsCommonResources.hlsl:51-71): mesh lights first, then the punctual lights
+// (RaytraceCommonResources.cpp:452-545).
 struct LightGPU {
-  uint32_t instance;
+  uint32_t instance;     // mesh lights: the emitting instance and its triangle range
   uint32_t material;
   uint32_t firstTri;
   uint32_t triCount;
   float selectionPdf;    // alias table over all lights (RaytraceCommonResources.cpp:325-368)
   float aliasThreshold;
   uint32_t aliasIndex;
+  uint32_t type;         // LIGHT_TYPE_*
+  float positionCosInner[4];   // punctual lights
+  float directionCosOuter[4];  // emission direction (normalized), cosine of the outer spot angle
+  float intensity[4];          // RGB color * intensity
+  float punctualSelectionPdf;  // alias table over the punctual lights only
+  float punctualAliasThreshold;
+  uint32_t punctualAliasIndex;
   uint32_t pad;
 };
 
@@ -52,7 +66,7 @@ struct FrameConstants {
   // bidirectional integrator (BidirectionalPathTraceConstants, BDPTCommon.hlsl:30-38)
   uint32_t lightPathVertices;  // maxVerticesPerLightPath
   uint32_t lightPathsPerDim[2];
-  uint32_t pad;
+  uint32_t punctualLightCount;  // the last punctualLightCount entries of the light table
 };
 
 struct SceneView {

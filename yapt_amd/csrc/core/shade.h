@@ -44,7 +44,7 @@ struct Hit {
 struct ShadowRay {
   vec3 origin, dir;
   float tmax;
-  uint32_t targetTri;  // closest hit must be this triangle for the light to count
+  uint32_t targetTri;  // closest hit must be this triangle for the light to count; NO_HIT: nothing may be hit
   Spectrum contrib;    // added to totalLight when visible
   bool valid;
 };
@@ -230,20 +230,76 @@ YPT_HD uint32_t sample_light_index(const LightGPU* lights, uint32_t count, float
   return chosen;
 }
 
+// samplePunctualLightIndex / punctualLightSelectionPDF (lightSampling.hlsl:44-57): the punctual
+// lights' own alias table
+YPT_HD uint32_t sample_punctual_light_index(const LightGPU* lights, uint32_t count, uint32_t punctualCount, float u,
+                                            float& pickPdf) {
+  uint32_t first = count - punctualCount;
+  float scaled = u * (float)punctualCount;
+  uint32_t index = (uint32_t)scaled;
+  if (index > punctualCount - 1) index = punctualCount - 1;
+  const LightGPU& e = lights[first + index];
+  uint32_t chosen = first + ((scaled - (float)index) < e.punctualAliasThreshold ? index : e.punctualAliasIndex);
+  pickPdf = lights[chosen].punctualSelectionPdf;
+  return chosen;
+}
+
+YPT_HD float smoothstep_(float a, float b, float x) {  // HLSL smoothstep
+  float t = saturate((x - a) / (b - a));
+  return t * t * (3.f - 2.f * t);
+}
+
+// spotFalloff / punctualLightAngularFalloff (lightSampling.hlsl:238-256)
+YPT_HD float punctual_falloff(const LightGPU& l, vec3 dirFromLight) {
+  if (l.type != LIGHT_TYPE_SPOT) return 1.f;
+  vec3 axis(l.directionCosOuter[0], l.directionCosOuter[1], l.directionCosOuter[2]);
+  float cosTheta = dot(dirFromLight, axis);
+  float cosInner = l.positionCosInner[3], cosOuter = l.directionCosOuter[3];
+  if (cosInner <= cosOuter) return cosTheta >= cosOuter ? 1.f : 0.f;
+  return smoothstep_(cosOuter, cosInner, cosTheta);
+}
+
+// evaluatePunctualLightIncidence (lightSampling.hlsl:258-284): direction and distance to the light
+// (infinite for directional lights) and the incident RGB intensity at pos
+YPT_HD vec3 punctual_incidence(const LightGPU& l, vec3 pos, vec3& toLight, float& distance) {
+  vec3 I(l.intensity[0], l.intensity[1], l.intensity[2]);
+  if (l.type == LIGHT_TYPE_DIRECTIONAL) {
+    toLight = -vec3(l.directionCosOuter[0], l.directionCosOuter[1], l.directionCosOuter[2]);
+    distance = __builtin_inff();
+    return I;
+  }
+  vec3 d = vec3(l.positionCosInner[0], l.positionCosInner[1], l.positionCosInner[2]) - pos;
+  float distSqr = fmax_(dot(d, d), 1e-10f);
+  distance = sqrtf(distSqr);
+  toLight = d / distance;
+  return I * (punctual_falloff(l, -toLight) / distSqr);
+}
+
 struct ExplicitLightSample {
-  Spectrum emission;
+  Spectrum emission;  // radiance with emission profile (mesh lights), incident intensity (punctual)
   vec3 toLight;
-  float distance;
-  float pdf;
-  uint32_t tri;
+  float distance;     // infinite for directional lights
+  float pdf;          // solid-angle pdf with the pick (mesh lights), the pick alone (punctual)
+  uint32_t tri;       // NO_HIT for punctual lights
+  bool punctual;
 };
 
-// sampleExplicitLight for mesh lights (rayHitDefault.hlsl:33-82) with sampleLight (lightSampling.hlsl:134-197)
-YPT_HD void sample_explicit_light(const SceneView& S, const FrameConstants& fc, vec3 pos, const float r[4], const Lambdas& L,
-                                  ExplicitLightSample& out) {
+// sampleExplicitLight (rayHitDefault.hlsl:36-82) with sampleLight (lightSampling.hlsl:134-197): NEE picks
+// from all lights, BPT from the punctual lights only (BSDF sampling never reaches them)
+YPT_HD void sample_explicit_light(const SceneView& S, const FrameConstants& fc, bool nee, vec3 pos, const float r[4],
+                                  const Lambdas& L, ExplicitLightSample& out) {
   float pickPdf;
-  uint32_t li = sample_light_index(S.lights, fc.lightCount, r[3], pickPdf);
+  uint32_t li = nee ? sample_light_index(S.lights, fc.lightCount, r[3], pickPdf)
+                    : sample_punctual_light_index(S.lights, fc.lightCount, fc.punctualLightCount, r[3], pickPdf);
   const LightGPU& light = S.lights[li];
+  out.punctual = light.type != LIGHT_TYPE_MESH;
+  if (out.punctual) {
+    vec3 rgb = punctual_incidence(light, pos, out.toLight, out.distance);
+    out.emission = rgb_unbounded_spectrum(S, rgb, L);
+    out.pdf = pickPdf;
+    out.tri = NO_HIT;
+    return;
+  }
   uint32_t primCount = light.triCount;
   uint32_t prim = (uint32_t)fmin_((float)primCount * r[2], (float)(primCount - 1u));
   float bx = r[0], by = r[1];
@@ -393,12 +449,12 @@ YPT_HD void shade_hit(const SceneView& S, const FrameConstants& fc, const uint32
   make_surface_context(ctx, sd, current_ior(ps), previous_ior(ps), wo, frontFace, L, false);
   vec3 normalWS = normalize(sd.geometryNormal);
 
-  bool hasLights = nee ? fc.lightCount > 0 : false;
+  bool hasLights = nee ? fc.lightCount > 0 : fc.punctualLightCount > 0;  // hasExplicitlySampledLights
   if (hasLights && !is_delta_distribution(ctx.pre.ax, ctx.pre.ay, sd)) {
     float rl[4];
     rand.r4(rl);
     ExplicitLightSample ls;
-    sample_explicit_light(S, fc, hitPos, rl, L, ls);
+    sample_explicit_light(S, fc, nee, hitPos, rl, L, ls);
     if (!ls.emission.all_equal(0.f) && ls.pdf > 0.f) {
       vec3 toLightOS = normalize(ls.toLight);
       SurfaceEval ev = evaluate_surface(ctx, S.luts, wo, toLightOS, EVAL_NONE);
@@ -407,9 +463,11 @@ YPT_HD void shade_hit(const SceneView& S, const FrameConstants& fc, const uint32
       if (!w.all_equal(0.f)) {
         sr.origin = spawn_ray_origin(hitPos, normalWS, ls.toLight);
         sr.dir = ls.toLight;
-        sr.tmax = ls.distance + RAY_MIN_T;
+        // punctual lights: visible when nothing is hit on the way (checkUnoccluded); delta lights are
+        // reached only by explicit sampling, so there is nothing to weight against
+        sr.tmax = ls.punctual ? ls.distance - RAY_MIN_T : ls.distance + RAY_MIN_T;
         sr.targetTri = ls.tri;
-        float wMIS = weight_mis(ls.pdf, ev.pdf);
+        float wMIS = ls.punctual ? 1.f : weight_mis(ls.pdf, ev.pdf);
         w = (w / ls.pdf) * wMIS;
         sr.contrib = ps.throughput * w * ls.emission;
         sr.valid = true;

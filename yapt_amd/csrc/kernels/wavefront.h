@@ -224,13 +224,18 @@ __global__ void k_shadow(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth) {
     float4 a = Q.sOrgTmax[i], b = Q.sDirTri[i];
     vec3 o(a.x, a.y, a.z), d(b.x, b.y, b.z);
     uint32_t target = __float_as_uint(b.w);
-    // the light triangle itself first: the connection counts only if it is the closest hit
-    RayPre rp = ray_precompute(o, d);
-    vec3 p1, p2, p3;
-    tri_vertices(S.sv, target, p1, p2, p3);
-    float tt, b1, b2;
+    // the light triangle itself first: the connection counts only if it is the closest hit (a light
+    // without geometry, NO_HIT_ID, counts when nothing is hit up to tmax)
+    float tt = a.w, b1, b2;
     bool visible = false;
-    if (intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2)) {
+    bool reaches = true;
+    if (target != NO_HIT_ID) {
+      RayPre rp = ray_precompute(o, d);
+      vec3 p1, p2, p3;
+      tri_vertices(S.sv, target, p1, p2, p3);
+      reaches = intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2);
+    }
+    if (reaches) {
       TravStack st;
       st.lds = lds; st.ovf = Q.ovf; st.block = blockDim.x; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
       st.ovfStride = Q.ovfStride; st.gid = i; st.sp = 0;
@@ -346,12 +351,17 @@ __global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, u
       float4 a = Q.sOrgTmax[idx], b = Q.sDirTri[idx];
       vec3 o(a.x, a.y, a.z), d(b.x, b.y, b.z);
       uint32_t target = __float_as_uint(b.w);
-      // the light triangle itself first: the connection counts only if it is the closest hit
-      RayPre rp = ray_precompute(o, d);
-      vec3 p1, p2, p3;
-      tri_vertices(S.sv, target, p1, p2, p3);
-      float tt, b1, b2;
-      if (intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2)) {
+      // the light triangle itself first: the connection counts only if it is the closest hit (a light
+      // without geometry, NO_HIT_ID, counts when nothing is hit up to tmax)
+      float tt = a.w, b1, b2;
+      bool reaches = true;
+      if (target != NO_HIT_ID) {
+        RayPre rp = ray_precompute(o, d);
+        vec3 p1, p2, p3;
+        tri_vertices(S.sv, target, p1, p2, p3);
+        reaches = intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2);
+      }
+      if (reaches) {
         trav_init(r, o, d, RAY_MIN_T, tt, target);
         st.sp = 0;
         has = true;

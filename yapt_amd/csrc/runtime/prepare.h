@@ -28,6 +28,7 @@ struct PreparedScene {
   std::vector<InstanceGPU> inst;
   std::vector<MaterialGPU> mats;
   std::vector<LightGPU> lights;
+  uint32_t punctualCount = 0;  // the last entries of lights
   float boundsMin[3], boundsMax[3];
 };
 
@@ -231,6 +232,43 @@ inline void prepare_scene(const ypt_scene_desc& d, Rgb2Spec& r2s, PreparedScene&
     L.material = d.instance_material[i];
     L.firstTri = out.inst[i].firstTri;
     L.triCount = out.inst[i].triCount;
+    L.type = LIGHT_TYPE_MESH;
+    out.lights.push_back(L);
+  }
+  size_t meshLights = out.lights.size();
+  out.punctualCount = d.punctual_light_count;
+  if (d.punctual_light_count && !d.punctual_lights) throw std::invalid_argument("punctual_lights is null");
+  std::vector<float> punctualPower;
+  for (uint32_t k = 0; k < d.punctual_light_count; ++k) {  // uploadLightData (RaytraceCommonResources.cpp:518-541)
+    const ypt_punctual_light& p = d.punctual_lights[k];
+    if (p.type < YPT_LIGHT_POINT || p.type > YPT_LIGHT_DIRECTIONAL) throw std::invalid_argument("unknown punctual light type");
+    LightGPU L;
+    std::memset(&L, 0, sizeof(L));
+    L.instance = 0xFFFFFFFFu;
+    L.type = p.type;
+    float dl = std::sqrt(p.direction[0] * p.direction[0] + p.direction[1] * p.direction[1] + p.direction[2] * p.direction[2]);
+    float dir[3] = {0.f, 0.f, -1.f};
+    if (dl > 0.f)
+      for (int a = 0; a < 3; ++a) dir[a] = p.direction[a] / dl;
+    for (int a = 0; a < 3; ++a) {
+      L.positionCosInner[a] = p.position[a];
+      L.directionCosOuter[a] = dir[a];
+      L.intensity[a] = std::max(0.f, p.intensity[a]);
+    }
+    L.positionCosInner[3] = p.cos_inner;
+    L.directionCosOuter[3] = p.cos_outer;
+    // estimatePunctualLightPower (RaytraceCommonResources.cpp:407-423)
+    float lum = 0.2126f * L.intensity[0] + 0.7152f * L.intensity[1] + 0.0722f * L.intensity[2];
+    float pw = lum;
+    if (p.type == YPT_LIGHT_POINT) pw = 4.f * 3.14159265358979323846f * lum;
+    else if (p.type == YPT_LIGHT_SPOT) pw = 2.f * 3.14159265358979323846f * (1.f - 0.5f * (p.cos_inner + p.cos_outer)) * lum;
+    else {
+      float ex = (out.boundsMax[0] - out.boundsMin[0]) * 0.5f, ey = (out.boundsMax[1] - out.boundsMin[1]) * 0.5f,
+            ez = (out.boundsMax[2] - out.boundsMin[2]) * 0.5f;
+      pw = 3.14159265358979323846f * (ex * ex + ey * ey + ez * ez) * lum;
+    }
+    power.push_back(pw);
+    punctualPower.push_back(pw);
     out.lights.push_back(L);
   }
   if (!out.lights.empty()) {
@@ -242,7 +280,19 @@ inline void prepare_scene(const ypt_scene_desc& d, Rgb2Spec& r2s, PreparedScene&
       out.lights[l].selectionPdf = pdfs[l];
       out.lights[l].aliasThreshold = thr[l];
       out.lights[l].aliasIndex = ali[l];
-      out.inst[out.lights[l].instance].lightSelectionPdf = pdfs[l];
+      if (l < meshLights) out.inst[out.lights[l].instance].lightSelectionPdf = pdfs[l];
+    }
+  }
+  if (!punctualPower.empty()) {
+    std::vector<float> pdfs = punctualPower, thr;
+    std::vector<uint32_t> ali;
+    selection_pdfs(pdfs);
+    alias_table(pdfs, thr, ali);
+    for (size_t k = 0; k < pdfs.size(); ++k) {
+      LightGPU& L = out.lights[meshLights + k];
+      L.punctualSelectionPdf = pdfs[k];
+      L.punctualAliasThreshold = thr[k];
+      L.punctualAliasIndex = ali[k];
     }
   }
 }
@@ -271,6 +321,7 @@ inline void prepare_frame(const ypt_frame_desc& f, const PreparedScene& s, uint3
   fc.flags = f.flags;
   fc.envIntensityScale = f.env_intensity_scale;
   fc.lightCount = (uint32_t)s.lights.size();
+  fc.punctualLightCount = s.punctualCount;
   fc.regularizeAfterVertices = f.regularize_after;
   fc.integrator = integrator;
   fc.lensParams[0] = std::max(0.f, f.aperture_radius);
@@ -282,7 +333,6 @@ inline void prepare_frame(const ypt_frame_desc& f, const PreparedScene& s, uint3
   fc.lightPathVertices = lv;
   fc.lightPathsPerDim[0] = (f.width + 7u) / 8u;   // one light subpath per 8x8 texels (TEXELS_PER_LIGHTPATH)
   fc.lightPathsPerDim[1] = (f.height + 7u) / 8u;
-  fc.pad = 0;
 }
 
 }  // namespace ypt

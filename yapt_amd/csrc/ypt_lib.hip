@@ -372,6 +372,11 @@ uint32_t shade_grid(const ypt_integrator* h, uint32_t nslots) {
   return std::min(g, std::max<uint32_t>(1u, (uint32_t)h->cus * (uint32_t)h->shadeWaves));
 }
 
+// explicit light connections (shadow rays) exist with NEE, and for BPT when the scene has punctual lights
+bool traces_shadow_rays(const ypt_integrator* h) {
+  return h->type == YPT_INTEGRATOR_NEE || h->fc.punctualLightCount > 0;
+}
+
 void launch_shade(ypt_integrator* h, const DevScene& S, const uint32_t* point, uint32_t npix, const PathSoA& P, const Queues& Q,
                   const FilmBuffers& F, int q, uint32_t sgrid) {
   hipStream_t s = h->stream;
@@ -433,7 +438,7 @@ void launch_batch(ypt_integrator* h, const DevScene& S, const PathSoA& P, const 
   for (uint32_t b = 0; b < h->fc.maxRayDepth; ++b) {
     if (h->bvhWidth == 8) launch_extend<8>(h, S, P, Q, q, grid);
     else launch_extend<2>(h, S, P, Q, q, grid);
-    if (h->type == YPT_INTEGRATOR_NEE) {
+    if (traces_shadow_rays(h)) {
       launch_shade(h, S, point, npix, P, Q, F, q, sgrid);
       if (h->bvhWidth == 8) launch_shadow<8>(h, S, P, Q, grid);
       else launch_shadow<2>(h, S, P, Q, grid);
@@ -917,7 +922,9 @@ int ypt_set_frame(ypt_integrator* h, const ypt_frame_desc* f) {
     HIP_CHECK(hipSetDevice(h->device));
     FrameConstants fc;
     prepare_frame(*f, h->res->scene, (uint32_t)h->type, fc);
-    if (f->env_intensity_scale != 0.f && !(f->flags & YPT_FLAG_WHITE_FURNACE) && h->dRgb2spec.n == 0) {
+    // RGB -> spectrum at run time: environment colours and punctual light intensities
+    bool needTable = (f->env_intensity_scale != 0.f && !(f->flags & YPT_FLAG_WHITE_FURNACE)) || fc.punctualLightCount > 0;
+    if (needTable && h->dRgb2spec.n == 0) {
       std::vector<float> table;
       host_tables().r2s.full_table(table, std::max(1u, std::thread::hardware_concurrency()));
       h->dRgb2spec.upload(table.data(), table.size(), h->stream);
@@ -1020,7 +1027,7 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
         launch_shade(h, S, point, npix, P, Q, F, q, sgrid);
         hipEvent_t a2 = h->profile ? mark() : nullptr;
         if (h->profile) { spans.push_back({a0, a1, 0}); spans.push_back({a1, a2, 1}); }
-        if (h->type == YPT_INTEGRATOR_NEE) {
+        if (traces_shadow_rays(h)) {
           if (h->bvhWidth == 8) launch_shadow<8>(h, S, P, Q, grid);
           else launch_shadow<2>(h, S, P, Q, grid);
           if (h->profile) spans.push_back({a2, mark(), 2});

@@ -145,11 +145,11 @@ class MeshBuilder:
         p = np.stack([c, c + u, c + u + v, c + v])
         self.add(p, np.tile(n, (4, 1)), [[0, 1, 2], [0, 2, 3]], material)
 
-    def arrays(self):
+    def arrays(self, lights=()):
         nrm = np.concatenate(self.nrm)
         return _lib.SceneArrays(np.concatenate(self.pos), nrm, np.concatenate(self.idx),
                                 self.inst_first, self.inst_count, self.inst_mat, self.materials,
-                                tangents=arbitrary_tangents(nrm))
+                                tangents=arbitrary_tangents(nrm), lights=lights)
 
 
 def box_template():
@@ -280,6 +280,68 @@ def cornell_box():
     b.quad((-0.25, 1.98, -0.25), (0.5, 0, 0), (0, 0, 0.5), light)  # light, -y
     cam = Camera(eye=(0.0, 1.0, 3.9), target=(0.0, 1.0, 0.0), fovy=math.pi * 0.3)
     return b.arrays(), cam
+
+
+def punctual_lights(light_type="mixed", sphere_res=(24, 24)):
+    """The punctual-light test scene (Tests/IntegratorCompare/punctual_lights_scene.py:16-105) built
+    from generated meshes: a 40x40 floor and back wall, diffuse, copper and plastic spheres, lit by
+    punctual lights only (environment scale 0). light_type: point, spot, directional, mixed (all three
+    plus an emissive sphere), mesh_control (a small emissive sphere where the point light is), env (no
+    lights: the scene is lit by the environment at scale 0.5, returned as the third value).
+    The reference's plane.glb and Sphere.glb are absent; the floor is a 40x40 quad and the spheres
+    have radius = scale. Returns (scene, camera, env_intensity_scale)."""
+    b = MeshBuilder()
+    diffuse = preset_material("PLASTIC")
+    diffuse.albedo[:] = (0.7, 0.7, 0.7)
+    diffuse.roughness = 1.0
+    diffuse.specular_amount = 0.0
+    d = b.material(diffuse)
+    glossy = preset_material("METAL_COPPER")
+    glossy.roughness = 0.35
+    g = b.material(glossy)
+    plastic = preset_material("PLASTIC")
+    plastic.albedo[:] = (0.2, 0.4, 0.8)
+    plastic.roughness = 0.25
+    pl = b.material(plastic)
+    b.quad((-20, 0, 20), (40, 0, 0), (0, 0, -40), d)      # floor, +y
+    b.quad((-20, 0, -20), (40, 0, 0), (0, 40, 0), d)      # back wall at z = -20, +z
+    P, N, I = sphere_template(*sphere_res)
+
+    def sphere(centre, radius, mat):
+        b.add(P * np.float32(radius) + np.asarray(centre, np.float32), N, I, mat)
+
+    sphere((-6, 3, 0), 3, d)
+    sphere((2, 3, -4), 3, g)
+    sphere((7, 2, 3), 2, pl)
+
+    def light(t, position, target, color, intensity):
+        l = _lib.Light(t)
+        l.lookAt(position, target)
+        l.setColor(color)
+        l.setIntensity(intensity)
+        return l
+
+    lights = []
+    if light_type in ("point", "mixed"):
+        lights.append(light(_lib.Light.POINT, (-2, 12, 6), (0, 0, 0), (1.0, 0.9, 0.8), 150.0))
+    if light_type in ("spot", "mixed"):
+        spot = light(_lib.Light.SPOT, (10, 16, 10), (2, 0, -2), (0.8, 0.9, 1.0), 600.0)
+        spot.setSpotConeAngles(0.3, 0.5)
+        lights.append(spot)
+    if light_type in ("directional", "mixed"):
+        lights.append(light(_lib.Light.DIRECTIONAL, (-6, 12, 8), (0, 0, 0), (1.0, 1.0, 1.0), 1.0))
+    if light_type == "mesh_control":
+        m = blank_material()
+        m.emissive[:] = (150.0, 135.0, 120.0)
+        sphere((-2, 12, 6), 0.5, b.material(m))
+    if light_type == "mixed":
+        m = blank_material()
+        m.emissive[:] = (6.0, 3.0, 1.0)
+        sphere((-10, 6, -10), 1.5, b.material(m))
+    ang = math.radians(25.0)
+    eye = (0.0, 14.0, 26.0)
+    cam = Camera(eye=eye, target=(0.0, 14.0 - math.sin(ang), 26.0 - math.cos(ang)), fovy=math.pi * 0.3)
+    return b.arrays(lights), cam, (0.5 if light_type == "env" else 0.0)
 
 
 def blackbody_rgb(kelvin):
