@@ -198,6 +198,35 @@ int ypt_sample_dims(ypt_integrator* h, uint32_t sample_index, const uint32_t* se
 /* Sobol point of a sample index: 256 uint32 dimensions (RaytraceCommonResources.cpp:623-627). */
 int ypt_sobol_point(uint32_t sample_index, uint32_t* out256);
 
+/* The steps after the film (CombineSamplesSubStage + TonemapStage): the edge-avoiding a-trous
+ * denoiser (5 passes of denoise.comp over the film, its variance and the prefiltered feature
+ * moments of prefilterFeatures.comp; needs "first_hit_features" on while rendering), XYZ -> linear
+ * sRGB with the gamut clamp, exposure, the ACES fitted operator (tonemap.frag with TONEMAP_ACES),
+ * and the encoding of an RGBA8_SRGB target: the FinalColor readback. Defaults are the renderer
+ * variables (RendererVarsList.h:24-54). Auto exposure (the luminance histogram) is not built. */
+typedef struct ypt_post_params {
+  uint32_t denoise;              /* Denoise.Mode: 0 OFF, 1 ON */
+  uint32_t prefilter_features;   /* Denoise.FeaturePrefilter (1) */
+  float depth_sigma_scale;       /* 0.02 */
+  float normal_sigma;            /* 0.06 */
+  float material_sigma;          /* 0.05 */
+  float color_sigma;             /* 3 */
+  float feature_variance_scale;  /* 1 */
+  uint32_t tonemap;              /* Tonemap.Enable: the ACES operator (1); exposure applies regardless */
+  float exposure;                /* Tonemap.ManualExposure (1) */
+} ypt_post_params;
+void ypt_post_params_default(ypt_post_params* p);
+/* rgba8: W*H*4 bytes, display encoded; linear_rgba: W*H*4 floats, linear sRGB before exposure (the
+ * denoised film); either may be NULL */
+int ypt_final_color(ypt_integrator* h, const ypt_post_params* p, uint8_t* rgba8, float* linear_rgba);
+/* Feature moments accumulated with the film (mergeNewSamples.comp accumulateFeatures): 4 layers of
+ * W*H float4 (albedo, |albedo|^2 / normal, coverage / roughness, metalness, transparency, anisotropy /
+ * ior/2, |material|^2, depth, depth^2) */
+int ypt_get_feature_moments(ypt_integrator* h, float* out16);
+/* Per-pixel variance the film merge estimates from its two accumulation targets (log luminance),
+ * the denoiser's colour-weight input: W*H floats */
+int ypt_get_film_variance(ypt_integrator* h, float* out);
+
 const char* ypt_last_error(void);
 const char* ypt_version(void);
 

@@ -101,6 +101,23 @@ class Light:
         return p
 
 
+class PostParams(ctypes.Structure):
+    """ypt_post_params: the denoise and tonemap renderer variables (RendererVarsList.h:24-54)."""
+    _fields_ = [
+        ("denoise", ctypes.c_uint32), ("prefilter_features", ctypes.c_uint32),
+        ("depth_sigma_scale", ctypes.c_float), ("normal_sigma", ctypes.c_float), ("material_sigma", ctypes.c_float),
+        ("color_sigma", ctypes.c_float), ("feature_variance_scale", ctypes.c_float),
+        ("tonemap", ctypes.c_uint32), ("exposure", ctypes.c_float),
+    ]
+
+    @classmethod
+    def defaults(cls, **kw):
+        p = cls(0, 1, 0.02, 0.06, 0.05, 3.0, 1.0, 1, 1.0)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [
         ("positions", ctypes.POINTER(ctypes.c_float)), ("normals", ctypes.POINTER(ctypes.c_float)),
@@ -205,6 +222,9 @@ def load():
                                       ctypes.POINTER(ctypes.c_uint64)]
     lib.ypt_server_wait.argtypes = [P, ctypes.c_uint64, ctypes.c_double]
     lib.ypt_server_set_option.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
+    lib.ypt_final_color.argtypes = [P, ctypes.POINTER(PostParams), ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_float)]
+    lib.ypt_post_params_default.argtypes = [ctypes.POINTER(PostParams)]
+    lib.ypt_post_params_default.restype = None
     lib.ypt_server_take.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]
     lib.ypt_server_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     lib.ypt_clock.restype = ctypes.c_double
@@ -218,7 +238,8 @@ EXPORTED_SYMBOLS = [
     "ypt_set_option", "ypt_trace_closest", "ypt_sample_dims", "ypt_sobol_point", "ypt_last_error", "ypt_version",
     "ypt_scene_create", "ypt_scene_destroy", "ypt_scene_info", "ypt_set_resident_scene",
     "ypt_server_create", "ypt_server_destroy", "ypt_server_submit", "ypt_server_wait", "ypt_server_take",
-    "ypt_server_stats", "ypt_clock", "ypt_server_set_option",
+    "ypt_server_stats", "ypt_clock", "ypt_server_set_option", "ypt_final_color", "ypt_post_params_default",
+    "ypt_get_feature_moments", "ypt_get_film_variance",
 ]
 
 
@@ -281,6 +302,26 @@ class Integrator:
     def last_samples(self):
         out = np.zeros((self.height, self.width, 4), np.float32)
         check(self.lib.ypt_get_last_samples(self.h, _fptr(out)))
+        return out
+
+    def final_color(self, params=None, linear=False):
+        """FinalColor: H x W x 4 uint8 (display encoded); with linear=True also the H x W x 4 float32
+        linear sRGB image before exposure (the denoised film)."""
+        p = params if params is not None else PostParams.defaults()
+        rgba = np.zeros((self.height, self.width, 4), np.uint8)
+        lin = np.zeros((self.height, self.width, 4), np.float32) if linear else None
+        check(self.lib.ypt_final_color(self.h, ctypes.byref(p), rgba.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                       None if lin is None else _fptr(lin)))
+        return (rgba, lin) if linear else rgba
+
+    def feature_moments(self):
+        out = np.zeros((4, self.height, self.width, 4), np.float32)
+        check(self.lib.ypt_get_feature_moments(self.h, _fptr(out)))
+        return out
+
+    def film_variance(self):
+        out = np.zeros((self.height, self.width), np.float32)
+        check(self.lib.ypt_get_film_variance(self.h, _fptr(out)))
         return out
 
     def first_hit_features(self):

@@ -57,6 +57,21 @@ struct FirstHitFeatures {
   uint32_t hit;
 };
 
+// Per-sample feature record in device memory: normal xyz, albedo rgb, roughness, ior, metalness,
+// transparency, depth, hit (1 or 0), anisotropy, 3 unused; all zero for a miss. The first 12 floats
+// are what ypt_get_first_hit_features returns per pixel.
+static constexpr uint32_t FEATURE_STRIDE = 16;
+YPT_HD void store_features(float* f, const FirstHitFeatures& fh) {
+  if (!fh.hit) {
+    for (uint32_t k = 0; k < FEATURE_STRIDE; ++k) f[k] = 0.f;
+    return;
+  }
+  f[0] = fh.normal.x; f[1] = fh.normal.y; f[2] = fh.normal.z;
+  f[3] = fh.albedo.x; f[4] = fh.albedo.y; f[5] = fh.albedo.z;
+  f[6] = fh.roughness; f[7] = fh.ior; f[8] = fh.metalness; f[9] = fh.transparency; f[10] = fh.depth; f[11] = 1.f;
+  f[12] = fh.anisotropy; f[13] = 0.f; f[14] = 0.f; f[15] = 0.f;
+}
+
 // ---------------------------------------------------------------- volume stack (payload.hlsl)
 YPT_HD float current_ior(const PathState& p) { return p.nvol != 0 ? sel_get(p.volIor, p.nvol - 1) : 1.0f; }
 YPT_HD float previous_ior(const PathState& p) { return p.nvol > 1 ? sel_get(p.volIor, p.nvol - 2) : 1.0f; }

@@ -201,16 +201,7 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_camera(FrameConstants fc, Dev
   vec3 xyz = spectrum_to_xyz(L, spectral_lambdas(lambda), S.sv.spec);
   B.camXYZ[pix] = make_float4(xyz.x, xyz.y, xyz.z, lambda);
   tr.flush(B.work);
-  if (features) {
-    float* f = features + (size_t)pix * 12;
-    if (fh.hit) {
-      f[0] = fh.normal.x; f[1] = fh.normal.y; f[2] = fh.normal.z;
-      f[3] = fh.albedo.x; f[4] = fh.albedo.y; f[5] = fh.albedo.z;
-      f[6] = fh.roughness; f[7] = fh.ior; f[8] = fh.metalness; f[9] = fh.transparency; f[10] = fh.depth; f[11] = 1.f;
-    } else {
-      for (int k = 0; k < 12; ++k) f[k] = 0.f;
-    }
-  }
+  if (features) store_features(features + (size_t)pix * FEATURE_STRIDE, fh);
 }
 
 template <int WIDTH, bool COUNT>
@@ -383,8 +374,9 @@ __global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuff
   store_cam(C, i, c, lp, count, false);
   Q.ray[0][(q >> 16) * W + (q & 0xFFFFu)] = i;  // primary rays queued in screen order: coherent traversal
   if (features) {
-    float* f = features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * 12;
-    for (int k = 0; k < 12; ++k) f[k] = 0.f;
+    FirstHitFeatures none;
+    none.hit = 0u;
+    store_features(features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * FEATURE_STRIDE, none);
   }
 }
 
@@ -438,10 +430,7 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
       cont = c.alive;
       if (features && first && fh.hit) {
         uint32_t q = slot_texel(fc, B, slot);
-        float* f = features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * 12;
-        f[0] = fh.normal.x; f[1] = fh.normal.y; f[2] = fh.normal.z;
-        f[3] = fh.albedo.x; f[4] = fh.albedo.y; f[5] = fh.albedo.z;
-        f[6] = fh.roughness; f[7] = fh.ior; f[8] = fh.metalness; f[9] = fh.transparency; f[10] = fh.depth; f[11] = 1.f;
+        store_features(features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * FEATURE_STRIDE, fh);
       }
     }
     uint32_t ri = wave_append(&Q.counters[qsel ^ 1], cont);

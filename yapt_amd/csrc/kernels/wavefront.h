@@ -49,8 +49,29 @@ struct FilmBuffers {
   uint32_t* counts[2];
   float* variance;
   float4* lastSample;  // X, Y, Z, hero wavelength
-  float* features;     // 12 floats per pixel (first-hit features), optional
+  float* features;     // FEATURE_STRIDE floats per path slot (first-hit features), optional
+  float4* featAccum;   // [4][npix] running means of the feature moments, with features
 };
+
+// featureSampleToMoments (denoiseFeatures.hlsl:29-44): albedo, |albedo|^2 / normal, coverage /
+// roughness, metalness, transparency, anisotropy / ior * 0.5, |material|^2, depth, depth^2; zeros for
+// a miss or a non-finite record
+YPT_HD void feature_moments(const float* f, float4 m[4]) {
+  bool finite = true;
+  for (int k = 0; k < 13; ++k) finite = finite && !is_nan(f[k]) && fabsf(f[k]) <= 3.402823466e38f;
+  if (!(f[11] > 0.f) || !finite) {
+    for (int k = 0; k < 4; ++k) m[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  float ax = f[3], ay = f[4], az = f[5];
+  float r = f[6], me = f[8], tr = f[9], an = f[12];
+  float ior = f[7] * 0.5f;  // FEATURE_IOR_SCALE
+  float d = f[10];
+  m[0] = make_float4(ax, ay, az, ax * ax + ay * ay + az * az);
+  m[1] = make_float4(f[0], f[1], f[2], 1.f);
+  m[2] = make_float4(r, me, tr, an);
+  m[3] = make_float4(ior, (r * r + me * me + tr * tr + an * an) + ior * ior, d, d * d);
+}
 
 struct DevScene {
   SceneView sv;
@@ -130,8 +151,11 @@ __global__ void k_start(FrameConstants fc, const uint32_t* points, PathSoA P, Qu
   start_path(fc, points + (size_t)pass * YPT_SOBOL_POINT_DIMS, pix % W, pix / W, ps);
   store_path(P, i, ps, false);
   Q.ray[0][i] = i;
-  if (writeFeatures)
-    for (int k = 0; k < 12; ++k) F.features[(size_t)i * 12 + k] = 0.f;
+  if (writeFeatures) {
+    FirstHitFeatures none;
+    none.hit = 0u;
+    store_features(F.features + (size_t)i * FEATURE_STRIDE, none);
+  }
 }
 
 template <int WIDTH, bool COUNT>
@@ -195,12 +219,7 @@ __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene 
       after_segment(point, ps);
       cont = ps.rayState == RAY_ALIVE && ps.pathLength < fc.maxRayDepth;
       store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);
-      if (writeFeatures && first && fh.hit) {
-        float* f = F.features + (size_t)slot * 12;
-        f[0] = fh.normal.x; f[1] = fh.normal.y; f[2] = fh.normal.z;
-        f[3] = fh.albedo.x; f[4] = fh.albedo.y; f[5] = fh.albedo.z;
-        f[6] = fh.roughness; f[7] = fh.ior; f[8] = fh.metalness; f[9] = fh.transparency; f[10] = fh.depth; f[11] = 1.f;
-      }
+      if (writeFeatures && first && fh.hit) store_features(F.features + (size_t)slot * FEATURE_STRIDE, fh);
     }
     uint32_t si = wave_append(&Q.counters[2], sr.valid);
     if (sr.valid) {
@@ -480,6 +499,18 @@ __global__ void k_merge(DevScene S, PathSoA P, FilmBuffers F, MergeParams mp, co
   if (newSamples > 0) {
     xyz = clamp_firefly(mp, F, x, y, xyz, oldSamples);
     countOut = newSamples + oldSamples;
+    if (F.features && F.featAccum) {  // accumulateFeatures (mergeNewSamples.comp:93-113)
+      float4 m[4];
+      feature_moments(F.features + (size_t)(mp.slotBase + p) * FEATURE_STRIDE, m);
+      float srcBlend = (float)newSamples / (float)(oldSamples + newSamples);
+      size_t np = (size_t)mp.width * mp.height;
+      for (int k = 0; k < 4; ++k) {
+        float4 o = F.featAccum[k * np + p];
+        F.featAccum[k * np + p] = oldSamples == 0 ? m[k]
+                                                  : make_float4(o.x + srcBlend * (m[k].x - o.x), o.y + srcBlend * (m[k].y - o.y),
+                                                                o.z + srcBlend * (m[k].z - o.z), o.w + srcBlend * (m[k].w - o.w));
+      }
+    }
   }
   F.counts[mp.cur][p] = countOut;
   size_t npix = (size_t)mp.width * mp.height;

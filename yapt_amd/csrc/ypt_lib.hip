@@ -19,6 +19,7 @@
 #include "../../include/yapt_pt.h"
 #include "core/shade.h"
 #include "kernels/bdpt_kernels.h"
+#include "kernels/post.h"
 #include "kernels/wavefront.h"
 #include "runtime/bvh.h"
 #include "runtime/luts.h"
@@ -184,6 +185,12 @@ struct ypt_integrator {
   DevBuf<float4> accum, merged0, merged1, lastSample;
   DevBuf<uint32_t> counts0, counts1;
   DevBuf<float> variance, features;
+  DevBuf<float4> featAccum;  // feature moments accumulated like the film (4 layers per pixel)
+  // after the film: packed prefiltered features, a-trous ping-pong colour and variance, 8-bit output
+  DevBuf<uint4> postF0, postF1;
+  DevBuf<float4> postA, postB;
+  DevBuf<float> postVarA, postVarB;
+  DevBuf<uchar4> postOut;
   uint32_t filmSamples = 0;  // passes merged since the last clear
   uint32_t filmCur = 0;
   size_t slots = 0;          // path slots allocated
@@ -256,6 +263,7 @@ struct ypt_integrator {
     F.accum = accum.p; F.merged[0] = merged0.p; F.merged[1] = merged1.p;
     F.counts[0] = counts0.p; F.counts[1] = counts1.p; F.variance = variance.p; F.lastSample = lastSample.p;
     F.features = features.p;
+    F.featAccum = writeFeatures ? featAccum.p : nullptr;  // moments only where the features are written
     return F;
   }
 };
@@ -287,7 +295,7 @@ void ensure_slots(ypt_integrator* h, size_t slots) {
   h->hit.ensure(slots);
   h->rayQ0.ensure(slots); h->rayQ1.ensure(slots);
   h->sOrg.ensure(slots); h->sDir.ensure(slots); h->sContrib.ensure(slots); h->sSlot.ensure(slots);
-  h->features.ensure(slots * 12);
+  h->features.ensure(slots * FEATURE_STRIDE);
   h->slots = slots;
 }
 
@@ -568,7 +576,7 @@ void ensure_bdpt(ypt_integrator* h) {
     }
     HIP_CHECK(hipEventCreateWithFlags(&h->camReady, hipEventDisableTiming));
   }
-  if (h->writeFeatures) h->features.ensure((size_t)npix * 12);
+  if (h->writeFeatures) h->features.ensure((size_t)npix * FEATURE_STRIDE);
   if (h->bdptWavefront) {
     h->cOrg.ensure(npix); h->cDir.ensure(npix); h->cThr.ensure(npix); h->cTotal.ensure(npix);
     h->cPrev.ensure(npix); h->cPrevN.ensure(npix); h->cDeferred.ensure(npix);
@@ -956,6 +964,7 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
     if (!h->hasScene || !h->hasFrame) return fail(YPT_E_NOSCENE, "render needs a scene and a frame");
     HIP_CHECK(hipSetDevice(h->device));
     if (stream) h->stream = (hipStream_t)stream;
+    if (h->writeFeatures) h->featAccum.ensure((size_t)4 * h->width * h->height);
     if (h->type == YPT_INTEGRATOR_BDPT) {
       render_bdpt(h, sample_begin, sample_count);
       return YPT_OK;
@@ -1122,6 +1131,100 @@ int ypt_get_film(ypt_integrator* h, float* xyzw, uint32_t* counts) {
   });
 }
 
+void ypt_post_params_default(ypt_post_params* p) {
+  if (!p) return;
+  p->denoise = 0;
+  p->prefilter_features = 1;
+  p->depth_sigma_scale = 0.02f;
+  p->normal_sigma = 0.06f;
+  p->material_sigma = 0.05f;
+  p->color_sigma = 3.f;
+  p->feature_variance_scale = 1.f;
+  p->tonemap = 1;
+  p->exposure = 1.f;
+}
+
+int ypt_final_color(ypt_integrator* h, const ypt_post_params* prm, uint8_t* rgba8, float* linear_rgba) {
+  if (!h || !prm || (!rgba8 && !linear_rgba)) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (!h->hasFrame) return fail(YPT_E_NOSCENE, "no frame");
+    HIP_CHECK(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    uint32_t W = h->width, H = h->height;
+    size_t npix = (size_t)W * H;
+    h->postF0.ensure(npix); h->postF1.ensure(npix);
+    h->postA.ensure(npix); h->postB.ensure(npix);
+    h->postVarA.ensure(npix); h->postVarB.ensure(npix);
+    h->postOut.ensure(npix);
+    PostParams pp;
+    pp.uvToView = h->fc.uvToView;
+    pp.viewToWorld = h->fc.viewToWorld;
+    pp.width = W; pp.height = H;
+    pp.depthSigmaScale = prm->depth_sigma_scale;
+    pp.normalSigma = prm->normal_sigma;
+    pp.materialSigma = prm->material_sigma;
+    pp.colorSigma = prm->color_sigma;
+    pp.featureVarianceScale = prm->feature_variance_scale;
+    pp.prefilter = prm->prefilter_features ? 1u : 0u;
+    const float4* film = (h->filmCur ^ 1) ? h->merged1.p : h->merged0.p;  // last written
+    dim3 tiles((W + 7) / 8, (H + 7) / 8);
+    if (prm->denoise) {
+      if (h->featAccum.n >= 4 * npix && h->filmSamples > 0) {
+        hipLaunchKernelGGL(k_prefilter_features, tiles, dim3(64), 0, s, pp, (const float4*)h->featAccum.p, h->postF0.p, h->postF1.p);
+      } else {  // no accumulated features (first_hit_features off): every pixel reads as a miss and passes through
+        HIP_CHECK(hipMemsetAsync(h->postF0.p, 0, npix * sizeof(uint4), s));
+        HIP_CHECK(hipMemsetAsync(h->postF1.p, 0, npix * sizeof(uint4), s));
+      }
+      const float4* src = film;
+      const float* srcVar = h->variance.p;
+      float4* bufs[2] = {h->postA.p, h->postB.p};
+      float* vars[2] = {h->postVarA.p, h->postVarB.p};
+      const uint32_t passes = 5;  // DENOISE_PASSES_COUNT (CombineSamplesSubStage.cpp:11)
+      for (uint32_t i = 0; i < passes; ++i) {
+        hipLaunchKernelGGL(k_atrous, tiles, dim3(64), 0, s, pp, i, 1, i + 1 == passes ? 1 : 0, src, srcVar,
+                           (const uint4*)h->postF0.p, (const uint4*)h->postF1.p, bufs[i & 1], vars[i & 1]);
+        src = bufs[i & 1];
+        srcVar = vars[i & 1];
+      }
+      if (src != h->postA.p) HIP_CHECK(hipMemcpyAsync(h->postA.p, src, npix * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    } else {
+      hipLaunchKernelGGL(k_atrous, tiles, dim3(64), 0, s, pp, 0u, 0, 1, film, (const float*)h->variance.p,
+                         (const uint4*)h->postF0.p, (const uint4*)h->postF1.p, h->postA.p, h->postVarA.p);
+    }
+    hipLaunchKernelGGL(k_tonemap, dim3(blocks_for(npix, 256)), dim3(256), 0, s, (uint32_t)npix, (const float4*)h->postA.p,
+                       prm->exposure, prm->tonemap ? 1 : 0, h->postOut.p);
+    HIP_CHECK(hipGetLastError());
+    if (linear_rgba) HIP_CHECK(hipMemcpyAsync(linear_rgba, h->postA.p, npix * sizeof(float4), hipMemcpyDeviceToHost, s));
+    if (rgba8) HIP_CHECK(hipMemcpyAsync(rgba8, h->postOut.p, npix * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return YPT_OK;
+  });
+}
+
+int ypt_get_film_variance(ypt_integrator* h, float* out) {
+  if (!h || !out) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (!h->hasFrame) return fail(YPT_E_NOSCENE, "no frame");
+    HIP_CHECK(hipSetDevice(h->device));
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    HIP_CHECK(hipMemcpy(out, h->variance.p, (size_t)h->width * h->height * sizeof(float), hipMemcpyDeviceToHost));
+    return YPT_OK;
+  });
+}
+
+int ypt_get_feature_moments(ypt_integrator* h, float* out16) {
+  if (!h || !out16) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (!h->hasFrame) return fail(YPT_E_NOSCENE, "no frame");
+    size_t npix = (size_t)h->width * h->height;
+    if (h->featAccum.n < 4 * npix) return fail(YPT_E_INVALID, "no feature moments: render with first_hit_features on");
+    HIP_CHECK(hipSetDevice(h->device));
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    HIP_CHECK(hipMemcpy(out16, h->featAccum.p, 4 * npix * sizeof(float4), hipMemcpyDeviceToHost));
+    return YPT_OK;
+  });
+}
+
 int ypt_get_last_samples(ypt_integrator* h, float* xyzl) {
   if (!h || !xyzl) return fail(YPT_E_INVALID, "null argument");
   return guarded([&]() -> int {
@@ -1139,8 +1242,9 @@ int ypt_get_first_hit_features(ypt_integrator* h, float* out) {
     if (!h->hasFrame) return fail(YPT_E_NOSCENE, "no frame");
     HIP_CHECK(hipSetDevice(h->device));
     HIP_CHECK(hipStreamSynchronize(h->stream));
-    HIP_CHECK(hipMemcpy(out, h->features.p + (size_t)h->lastPassBase * 12, (size_t)h->width * h->height * 12 * sizeof(float),
-                        hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy2D(out, 12 * sizeof(float), h->features.p + (size_t)h->lastPassBase * FEATURE_STRIDE,
+                          FEATURE_STRIDE * sizeof(float), 12 * sizeof(float), (size_t)h->width * h->height,
+                          hipMemcpyDeviceToHost));
     return YPT_OK;
   });
 }
