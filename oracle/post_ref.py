@@ -244,19 +244,3 @@ def final_color(linear_rgba, exposure=1.0, tonemap=True):
     rgb = aces_fitted(c[..., :3]) if tonemap else c[..., :3]
     alpha = np.floor(np.clip(c[..., 3], 0, 1) * 255 + 0.5).astype(np.uint8)
     return np.concatenate([srgb_encode(rgb), alpha[..., None]], -1)
-
-
-def difference_stats(a, b, downsample=(15, 11)):
-    """IntegratorCompare's difference figures (report.py:59-85) of two 8-bit captures: the relative mean
-    absolute difference at full resolution and after a box downsample ("rel|d| 16x")"""
-    from PIL import Image
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    d = np.abs(a - b)
-    mid = (a.mean() + b.mean()) / 2
-    sa = np.asarray(Image.fromarray(a.astype(np.uint8)).resize(downsample, Image.BOX), np.float64)
-    sb = np.asarray(Image.fromarray(b.astype(np.uint8)).resize(downsample, Image.BOX), np.float64)
-    smid = (sa.mean() + sb.mean()) / 2
-    return {"rel_full": 100.0 * d.mean() / mid if mid else 0.0,
-            "rel_down": 100.0 * np.abs(sa - sb).mean() / smid if smid else 0.0,
-            "mean_gap": 100.0 * (b.mean() - a.mean()) / a.mean() if a.mean() else 0.0}

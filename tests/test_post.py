@@ -121,12 +121,17 @@ def test_pixels_without_geometry_pass_through():
     assert np.array_equal(out, film)
 
 
-def test_difference_stats_of_identical_and_shifted_captures():
+def test_difference_stats_of_identical_noisy_and_shifted_captures():
+    from yapt_amd.compare import difference_stats
     rng = np.random.default_rng(4)
-    a = rng.integers(0, 256, (176, 240, 3)).astype(np.uint8)
-    assert R.difference_stats(a, a)["rel_down"] == 0
-    b = np.clip(a.astype(int) + 10, 0, 255).astype(np.uint8)
-    assert R.difference_stats(a, b)["mean_gap"] > 0
+    base = rng.uniform(40, 200, (1, 1, 3))
+    a = np.clip(base + rng.normal(0, 20, (176, 240, 3)), 0, 255).astype(np.uint8)
+    b = np.clip(base + rng.normal(0, 20, (176, 240, 3)), 0, 255).astype(np.uint8)
+    assert difference_stats(a, a)["rel_down"] == 0
+    s = difference_stats(a, b)   # two noisy captures of one image: the block average removes the noise
+    assert s["rel_full"] > 10 * s["rel_down"]
+    c = np.clip(a.astype(int) + 10, 0, 255).astype(np.uint8)
+    assert difference_stats(a, c)["mean_gap"] > 0
 
 
 # ---------------------------------------------------------------- GPU
