@@ -9,6 +9,8 @@
    against a 4096-spp plain render, for plain 64 spp and for 4 spp + denoiser, with the wall time of
    each (render, and render + denoise, on the GPU).
 4. The denoiser's conv kernels: achieved TFLOP/s of one 1080p forward pass (HIP events).
+5. The reference's non-learned denoiser (feature prefilter + 5 a-trous passes) on the same 4-spp
+   film, as the comparison point: relMSE in linear sRGB for every variant.
 Prints one JSON line.
 """
 import argparse
@@ -20,6 +22,19 @@ import time
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+XYZ_TO_SRGB = np.array([[3.2406, -1.5372, -0.4986], [-0.9689, 1.8758, 0.0415], [0.0557, -0.2040, 1.0570]])
+
+
+def xyz_to_linear_srgb(xyz):
+    """XYZ film -> linear sRGB with the gamut clamp the FinalColor path applies"""
+    rgb = np.asarray(xyz, np.float64)[..., :3] @ XYZ_TO_SRGB.T
+    lum = rgb @ np.array([0.2126, 0.7152, 0.0722])
+    m = rgb.min(-1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.where(lum - m != 0, lum / np.where(lum - m != 0, lum - m, 1), 0.0)
+    return np.where((m < 0)[..., None], lum[..., None] + t[..., None] * (rgb - lum[..., None]), rgb)
 
 
 def conv_flops(net, H, W):
@@ -121,6 +136,12 @@ def main():
     torch.cuda.synchronize()
     t_denoise = time.perf_counter() - t1
     den = out.cpu().numpy()
+    # the reference's own (non-learned) denoiser on the same 4-spp film: prefilter + 5 a-trous passes
+    bil_params = _lib.PostParams.defaults(denoise=1, tonemap=0)
+    it.final_color(bil_params, linear=True)   # warm-up
+    t2 = time.perf_counter()
+    _, bil = it.final_color(bil_params, linear=True)
+    t_bil = time.perf_counter() - t2
 
     # conv kernels: achieved TFLOP/s over one forward pass at the evaluation size (device time)
     x = denoiser.pack_input(x_noisy[None], x_feat[None], net.multiple)
@@ -138,11 +159,18 @@ def main():
     r_plain = denoiser.rel_mse(plain, ref)
     r_noisy = denoiser.rel_mse(noisy, ref)
     r_den = denoiser.rel_mse(den, ref)
+    ref_s = xyz_to_linear_srgb(ref)
+    srgb = {"plain_64spp": denoiser.rel_mse(xyz_to_linear_srgb(plain), ref_s),
+            "plain_4spp": denoiser.rel_mse(xyz_to_linear_srgb(noisy), ref_s),
+            "learned_4spp": denoiser.rel_mse(xyz_to_linear_srgb(den), ref_s),
+            "bilateral_4spp": denoiser.rel_mse(bil[..., :3], ref_s)}
     t_den_total = t_render4 + t_denoise
     res = {
         "metric": "relMSE vs %d-spp plain render, 4 spp + denoiser vs plain 64 spp (%dx%d)" % (a.eval_ref_spp, W, H),
         "relmse": {"plain_64spp": r_plain, "plain_4spp": r_noisy, "denoised_4spp": r_den},
-        "wall_s": {"plain_64spp": t_plain, "render_4spp": t_render4, "denoise": t_denoise, "denoised_4spp_total": t_den_total},
+        "wall_s": {"plain_64spp": t_plain, "render_4spp": t_render4, "denoise": t_denoise, "denoised_4spp_total": t_den_total,
+                   "bilateral_denoise": t_bil},
+        "relmse_linear_srgb": srgb,
         "speedup_vs_plain_64spp": t_plain / t_den_total,
         "denoised_matches_64spp": r_den <= r_plain,
         "gain_over_4spp": r_noisy / r_den,
