@@ -1,0 +1,284 @@
+"""glTF 2.0 scenes (.gltf with external or data-URI buffers, .glb) as a resident-ready scene.
+
+The reference loads glTF through pygltflib in its Python scene loader (pythonScripts/yapt/scene_loader.py)
+and its C++ mesh loader; pygltflib is not part of this build, so the container, accessors and node
+hierarchy are parsed here. What follows the reference:
+  * materials (scene_loader.py:105-243): base colour factor -> albedo, transparency 0, metallic and
+    roughness factors (default 1), emissive factor x KHR_materials_emissive_strength, doubleSided ->
+    two sided, KHR_materials_transmission / specular / clearcoat (IOR 1.5) / sheen / volume
+    (absorption = -ln(attenuationColor) / attenuationDistance) / iridescence / ior, and BLEND coverage
+    folded into the transparency (1 - coverage * (1 - transmission)); primitives without a material get
+    the PLASTIC preset with specular amount 0;
+  * KHR_lights_punctual (scene_loader.py:362-390): colour, intensity / 683 (lm/W at 555 nm), spot cone
+    angles; the light looks down its node's -Z axis;
+  * node transforms (translation, rotation quaternion, scale, or a matrix) composed down the hierarchy
+    and baked into world space, normals by the inverse transpose, tangents by the matrix.
+Textures are parsed as references (returned in info["textures"]) but not applied: texture sampling
+is not part of this build. Triangle lists (mode 4) only.
+
+    scene, info = load("model.glb")
+"""
+import base64
+import json
+import math
+import os
+import struct
+
+import numpy as np
+
+from . import _lib, scenes
+
+GLTF_LUMENS_PER_WATT = 683.0
+_COMPONENT = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
+_NCOMP = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT2": 4, "MAT3": 9, "MAT4": 16}
+_LIGHT_TYPES = {"point": _lib.Light.POINT, "spot": _lib.Light.SPOT, "directional": _lib.Light.DIRECTIONAL}
+MASK_TWO_SIDED, MASK_ALPHA_BLEND = 1 << 0, 1 << 2
+
+
+def _read_container(path):
+    with open(path, "rb") as f:
+        data = f.read()
+    if data[:4] == b"glTF":
+        version, length = struct.unpack_from("<II", data, 4)
+        if version != 2:
+            raise ValueError("glTF version %d is not supported" % version)
+        off, doc, binchunk = 12, None, None
+        while off < length:
+            clen, ctype = struct.unpack_from("<II", data, off)
+            chunk = data[off + 8: off + 8 + clen]
+            if ctype == 0x4E4F534A:
+                doc = json.loads(chunk.decode("utf-8"))
+            elif ctype == 0x004E4942:
+                binchunk = chunk
+            off += 8 + clen
+        return doc, binchunk
+    return json.loads(data.decode("utf-8")), None
+
+
+def _buffers(doc, base, binchunk):
+    out = []
+    for i, b in enumerate(doc.get("buffers", [])):
+        uri = b.get("uri")
+        if uri is None:
+            if binchunk is None:
+                raise ValueError("buffer %d has no uri and the file has no BIN chunk" % i)
+            out.append(binchunk)
+        elif uri.startswith("data:"):
+            out.append(base64.b64decode(uri.split(",", 1)[1]))
+        else:
+            with open(os.path.join(base, uri), "rb") as f:
+                out.append(f.read())
+    return out
+
+
+def _accessor(doc, bufs, index):
+    a = doc["accessors"][index]
+    dtype = np.dtype(_COMPONENT[a["componentType"]])
+    n, count = _NCOMP[a["type"]], a["count"]
+    if "bufferView" not in a:
+        return np.zeros((count, n), np.float32)
+    bv = doc["bufferViews"][a["bufferView"]]
+    raw = bufs[bv["buffer"]]
+    start = bv.get("byteOffset", 0) + a.get("byteOffset", 0)
+    elem = dtype.itemsize * n
+    stride = bv.get("byteStride", 0) or elem
+    if stride == elem:
+        arr = np.frombuffer(raw, dtype, count * n, start).reshape(count, n)
+    else:
+        rows = np.frombuffer(raw, np.uint8, (count - 1) * stride + elem, start)
+        arr = np.stack([np.frombuffer(rows[i * stride: i * stride + elem].tobytes(), dtype) for i in range(count)])
+    arr = arr.astype(np.float64)
+    if a.get("normalized") and dtype != np.float32:
+        arr = np.maximum(arr / np.iinfo(dtype).max, -1.0)
+    return arr
+
+
+def _quat_matrix(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def node_matrix(node):
+    """local transform of a node: its matrix (column-major in the file) or T * R * S"""
+    if "matrix" in node:
+        return np.asarray(node["matrix"], np.float64).reshape(4, 4).T
+    m = np.eye(4)
+    m[:3, :3] = _quat_matrix(node.get("rotation", [0, 0, 0, 1])) * np.asarray(node.get("scale", [1, 1, 1]), np.float64)
+    m[:3, 3] = node.get("translation", [0, 0, 0])
+    return m
+
+
+def _material(doc, mat):
+    """scene_loader.py _load_material_gltf on factors (textures are reported, not applied)"""
+    m = scenes.blank_material()
+    ext = mat.get("extensions", {}) or {}
+    strength = ext.get("KHR_materials_emissive_strength", {}).get("emissiveStrength", 1.0)
+    if mat.get("emissiveFactor"):
+        m.emissive[:] = [c * strength for c in mat["emissiveFactor"]]
+    if mat.get("doubleSided"):
+        m.material_mask |= MASK_TWO_SIDED
+    coverage, blended, transmission = 1.0, False, 0.0
+    pbr = mat.get("pbrMetallicRoughness")
+    if pbr is not None:
+        color = pbr.get("baseColorFactor", [1, 1, 1, 1])
+        m.albedo[:] = color[:3]
+        m.transparency = 0.0
+        has_tex = "baseColorTexture" in pbr
+        mode = mat.get("alphaMode", "OPAQUE")
+        factor_alpha = color[3] if len(color) > 3 else 1.0
+        if mode == "BLEND" and not has_tex and factor_alpha < 1.0:   # texture-free branch of the BLEND rule
+            coverage, blended = factor_alpha, True
+        m.metalness = pbr.get("metallicFactor", 1.0)
+        m.roughness = pbr.get("roughnessFactor", 1.0)
+    if "KHR_materials_transmission" in ext:
+        transmission = ext["KHR_materials_transmission"].get("transmissionFactor", 0.0)
+        m.transparency = transmission
+        vol = ext.get("KHR_materials_volume")
+        thin = vol is None or vol.get("thicknessFactor", 0.0) == 0
+        m.material_mask = (m.material_mask | MASK_TWO_SIDED) if thin else (m.material_mask & ~MASK_TWO_SIDED)
+    if "KHR_materials_specular" in ext:
+        m.specular_amount = ext["KHR_materials_specular"].get("specularFactor", 1.0)
+    if "KHR_materials_clearcoat" in ext:
+        cc = ext["KHR_materials_clearcoat"]
+        m.clear_coat_amount = cc.get("clearcoatFactor", 0.0)
+        m.clear_coat_roughness = cc.get("clearcoatRoughnessFactor", 0.0)
+        m.clear_coat_ior = 1.5
+    if "KHR_materials_sheen" in ext:
+        sh = ext["KHR_materials_sheen"]
+        col = sh.get("sheenColorFactor", [0.0, 0.0, 0.0])
+        amount = max(col)
+        if amount > 0:
+            m.sheen_amount = amount
+            m.sheen_tint[:] = [c / amount for c in col]
+            r = sh.get("sheenRoughnessFactor", 0.0)
+            m.sheen_roughness = r * r
+    if "KHR_materials_volume" in ext:
+        vol = ext["KHR_materials_volume"]
+        if vol.get("thicknessFactor", None) == 0:
+            m.material_mask |= MASK_TWO_SIDED
+        dist = vol.get("attenuationDistance", -1)
+        col = vol.get("attenuationColor", [1, 1, 1])
+        if dist > 0 and any(c != 1 for c in col):
+            m.absorption[:] = [-math.log(c) / dist for c in col]
+    if "KHR_materials_iridescence" in ext:
+        ir = ext["KHR_materials_iridescence"]
+        m.clear_coat_ior = ir.get("iridescenceIor", 1.3)
+        m.clear_coat_amount = ir.get("iridescenceFactor", 0.0)
+        m.thin_film_thickness_nm = ir.get("iridescenceThicknessMaximum", 400)
+    if "KHR_materials_ior" in ext:
+        m.dielectric_ior = ext["KHR_materials_ior"].get("ior", 1.5)
+    if blended:
+        m.transparency = 1.0 - coverage * (1.0 - transmission)
+        if transmission == 0.0:
+            m.material_mask |= MASK_TWO_SIDED
+    textures = {}
+    for key, slot in (("normalTexture", "normal"), ("emissiveTexture", "emissive"), ("occlusionTexture", "orm")):
+        if key in mat:
+            textures[slot] = mat[key]["index"]
+    if pbr is not None:
+        for key, slot in (("baseColorTexture", "albedo"), ("metallicRoughnessTexture", "orm")):
+            if key in pbr:
+                textures[slot] = pbr[key]["index"]
+    return m, textures
+
+
+def _area_normals(P, I):
+    n = np.zeros_like(P)
+    fn = np.cross(P[I[:, 1]] - P[I[:, 0]], P[I[:, 2]] - P[I[:, 0]])
+    for k in range(3):
+        np.add.at(n, I[:, k], fn)
+    l = np.linalg.norm(n, axis=1, keepdims=True)
+    return np.where(l > 0, n / np.where(l > 0, l, 1), np.array([0.0, 1.0, 0.0]))
+
+
+def load(path, scene_index=None):
+    """Returns (SceneArrays, info) with info = {"lights": [...], "materials": [names], "textures": {material: {...}},
+    "camera": (eye, target, yfov) of the first camera node or None, "triangles": n}"""
+    doc, binchunk = _read_container(path)
+    bufs = _buffers(doc, os.path.dirname(os.path.abspath(path)), binchunk)
+    b = scenes.MeshBuilder()
+    mats, tex_info = [], {}
+    for k, mat in enumerate(doc.get("materials", [])):
+        m, t = _material(doc, mat)
+        mats.append(b.material(m))
+        if t:
+            tex_info[mat.get("name", "material_%d" % k)] = t
+    default = None
+    lights, camera = [], None
+    light_defs = doc.get("extensions", {}).get("KHR_lights_punctual", {}).get("lights", [])
+    tangents = []
+
+    def visit(ni, parent):
+        nonlocal default, camera
+        node = doc["nodes"][ni]
+        world = parent @ node_matrix(node)
+        if "mesh" in node:
+            nmat = np.linalg.inv(world[:3, :3]).T
+            for prim in doc["meshes"][node["mesh"]]["primitives"]:
+                if prim.get("mode", 4) != 4:
+                    continue
+                attr = prim["attributes"]
+                P = _accessor(doc, bufs, attr["POSITION"])[:, :3]
+                I = (_accessor(doc, bufs, prim["indices"]).astype(np.int64).reshape(-1, 3) if "indices" in prim
+                     else np.arange(len(P)).reshape(-1, 3))
+                Pw = P @ world[:3, :3].T + world[:3, 3]
+                if np.linalg.det(world[:3, :3]) < 0:        # mirrored: keep the front faces facing out
+                    I = I[:, [0, 2, 1]]
+                if "NORMAL" in attr:
+                    N = _accessor(doc, bufs, attr["NORMAL"])[:, :3] @ nmat.T
+                    N = N / np.maximum(np.linalg.norm(N, axis=1, keepdims=True), 1e-30)
+                else:
+                    N = _area_normals(Pw, I)
+                if "TANGENT" in attr:
+                    T4 = _accessor(doc, bufs, attr["TANGENT"])
+                    T = T4[:, :3] @ world[:3, :3].T
+                    T = T / np.maximum(np.linalg.norm(T, axis=1, keepdims=True), 1e-30)
+                    tangents.append(np.concatenate([T, T4[:, 3:4]], 1))
+                else:
+                    tangents.append(scenes.arbitrary_tangents(N.astype(np.float32)))
+                if "material" in prim:
+                    mi = mats[prim["material"]]
+                else:
+                    if default is None:
+                        dm = scenes.preset_material("PLASTIC")
+                        dm.specular_amount = 0.0
+                        default = b.material(dm)
+                    mi = default
+                b.add(Pw, N, I, mi)
+        ext = node.get("extensions", {}).get("KHR_lights_punctual")
+        if ext is not None and ext.get("light") is not None and ext["light"] < len(light_defs):
+            ld = light_defs[ext["light"]]
+            t = _LIGHT_TYPES.get(ld.get("type"))
+            if t is not None:
+                l = _lib.Light(t)
+                pos = world[:3, 3]
+                fwd = -world[:3, 2]
+                l.lookAt(pos, pos + fwd)
+                l.setColor(ld.get("color", [1.0, 1.0, 1.0]))
+                l.setIntensity(ld.get("intensity", 1.0) / GLTF_LUMENS_PER_WATT)
+                if t == _lib.Light.SPOT:
+                    spot = ld.get("spot", {})
+                    l.setSpotConeAngles(spot.get("innerConeAngle", 0.0), spot.get("outerConeAngle", math.pi / 4.0))
+                lights.append(l)
+        if "camera" in node and camera is None:
+            cam = doc["cameras"][node["camera"]]
+            if cam.get("type") == "perspective":
+                eye = world[:3, 3]
+                camera = (tuple(eye), tuple(eye - world[:3, 2]), cam["perspective"].get("yfov", math.pi * 0.3))
+        for c in node.get("children", []):
+            visit(c, world)
+
+    si = scene_index if scene_index is not None else doc.get("scene", 0)
+    roots = doc["scenes"][si]["nodes"] if doc.get("scenes") else list(range(len(doc.get("nodes", []))))
+    for r in roots:
+        visit(r, np.eye(4))
+    if not b.pos:
+        raise ValueError("no triangle geometry in %s" % path)
+    nrm = np.concatenate(b.nrm)
+    arrays = _lib.SceneArrays(np.concatenate(b.pos), nrm, np.concatenate(b.idx), b.inst_first, b.inst_count, b.inst_mat,
+                              b.materials, tangents=np.concatenate(tangents).astype(np.float32), lights=lights)
+    info = {"lights": lights, "materials": [m.get("name") for m in doc.get("materials", [])], "textures": tex_info,
+            "camera": camera, "triangles": arrays.triangle_count}
+    return arrays, info
