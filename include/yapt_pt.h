@@ -66,12 +66,22 @@ typedef struct ypt_material {
   float subsurface_radius[3];
   float subsurface_amount;
   float subsurface_radius_scale;
+  /* textures (MaterialParameters albedoTex / normalTex / ormTex / emissiveTex): an index into
+   * ypt_scene_desc.textures, or any value outside [0, texture_count) for none (TEX_UNBOUND_INDEX),
+   * and the UV scale of each (the texture transform scale) */
+  int32_t albedo_tex, normal_tex, orm_tex, emissive_tex;
+  float albedo_tex_scale[2], normal_tex_scale[2], orm_tex_scale[2], emissive_tex_scale[2];
 } ypt_material;
 
-/* Scene: world-space triangle soup (render-object transforms already applied, the
- * SceneData::objToWorld of PathIntegratorSubStage::sceneChanged), one record per instance
- * (render object x submesh) owning a contiguous triangle range. Mesh lights are the instances
- * whose material is emissive (LightManager.cpp:145). */
+/* An RGBA8 texture, sampled bilinearly with wrapping at mip level 0 (g_colorSampler, SampleLevel 0).
+ * srgb: the texels are sRGB encoded and decoded before filtering (R8G8B8A8_SRGB, used for colour);
+ * 0 for data (normal, occlusion/roughness/metalness: R8G8B8A8_UNORM). */
+typedef struct ypt_texture {
+  uint32_t width, height;
+  uint32_t srgb;
+  const uint8_t* rgba8;  /* width * height * 4, row-major from v = 0 */
+} ypt_texture;
+
 /* Punctual lights (lightTypes.h:6-8) */
 #define YPT_LIGHT_POINT 1
 #define YPT_LIGHT_SPOT 2
@@ -90,6 +100,10 @@ typedef struct ypt_punctual_light {
   float cos_inner, cos_outer;    /* spot */
 } ypt_punctual_light;
 
+/* Scene: world-space triangle soup (render-object transforms already applied, the
+ * SceneData::objToWorld of PathIntegratorSubStage::sceneChanged), one record per instance
+ * (render object x submesh) owning a contiguous triangle range. Mesh lights are the instances
+ * whose material is emissive (LightManager.cpp:145). */
 typedef struct ypt_scene_desc {
   const float* positions;       /* 3 * vertex_count */
   const float* normals;         /* 3 * vertex_count, or NULL for area-weighted vertex normals */
@@ -106,6 +120,9 @@ typedef struct ypt_scene_desc {
   uint32_t material_count;
   const ypt_punctual_light* punctual_lights;  /* may be NULL when punctual_light_count is 0 */
   uint32_t punctual_light_count;
+  const float* uvs;              /* 2 * vertex_count, or NULL: UV (0.5, 0.5) as for meshes without UVs */
+  const ypt_texture* textures;   /* may be NULL when texture_count is 0 */
+  uint32_t texture_count;
 } ypt_scene_desc;
 
 /* Per-frame constants: the fields of RaytraceConstantData (raytraceCommonResources.hlsl:8-31) the

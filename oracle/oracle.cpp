@@ -146,11 +146,13 @@ struct Geometry {
     float best = tmax;
     uint32_t bid = NO_HIT_ID;
     float bb1 = 0.f, bb2 = 0.f;
+    const AlphaView av{S->alphaTris.data(), S->textures.data(), S->texels.data()};
     auto test = [&](uint32_t t) {
       vec3 a, b, c;
       tri(t, a, b, c);
       float tt, b1, b2;
-      if (intersect_triangle(rp, a, b, c, tmin, best, tt, b1, b2) && hit_closer(tt, t, best, bid)) {
+      if (intersect_triangle(rp, a, b, c, tmin, best, tt, b1, b2) && hit_closer(tt, t, best, bid) &&
+          (S->triAlpha[t] == 0u || alpha_test_passes(av, S->triAlpha[t] - 1u, b1, b2))) {
         best = tt; bid = t; bb1 = b1; bb2 = b2;
       }
     };
@@ -271,6 +273,13 @@ void make_view(Ctx& c) {
   c.sv.mats = c.scene.mats.data();
   c.sv.lights = c.scene.lights.empty() ? nullptr : c.scene.lights.data();
   c.sv.rgb2spec = T.rgb2specFull.empty() ? nullptr : T.rgb2specFull.data();
+  c.sv.uv = c.scene.uv.empty() ? nullptr : c.scene.uv.data();
+  c.sv.textures = c.scene.textures.empty() ? nullptr : c.scene.textures.data();
+  c.sv.texels = c.scene.texels.empty() ? nullptr : c.scene.texels.data();
+  c.sv.srgbLut = c.scene.srgbLut.empty() ? nullptr : c.scene.srgbLut.data();
+  c.sv.triAlpha = c.scene.alphaTris.empty() ? nullptr : c.scene.triAlpha.data();
+  c.sv.alpha = AlphaView{c.scene.alphaTris.empty() ? nullptr : c.scene.alphaTris.data(), c.scene.textures.data(),
+                         c.scene.texels.data()};
   c.sv.ntris = (uint32_t)c.scene.triInst.size();
   c.sv.ninst = (uint32_t)c.scene.inst.size();
   c.sv.spec.cie_xyz4 = YPT_CIE_XYZ;
@@ -430,7 +439,8 @@ int setup(Ctx& c, const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integr
   }
   prepare_scene(*sd, tables().r2s, c.scene);
   prepare_frame(*fd, c.scene, (uint32_t)integrator, c.fc);
-  bool needTable = (fd->env_intensity_scale != 0.f && !(fd->flags & YPT_FLAG_WHITE_FURNACE)) || c.fc.punctualLightCount > 0;
+  bool needTable = (fd->env_intensity_scale != 0.f && !(fd->flags & YPT_FLAG_WHITE_FURNACE)) || c.fc.punctualLightCount > 0 ||
+                   !c.scene.textures.empty();
   if (needTable && tables().rgb2specFull.empty())
     tables().r2s.full_table(tables().rgb2specFull, std::max(1u, std::thread::hardware_concurrency()));
   c.nee = integrator == YPT_INTEGRATOR_NEE;

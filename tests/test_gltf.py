@@ -130,3 +130,99 @@ def test_loaded_scene_renders_like_the_checker(tmp_path, oracle):
     c = oracle.render(scene, frame, _lib.YPT_INTEGRATOR_NEE, 0, 4)["film"]
     assert g[..., 1].max() > 0
     assert np.allclose(g[..., :3], c[..., :3], rtol=1e-4, atol=1e-6)
+
+
+def _png(rgba):
+    import io
+    from PIL import Image
+    f = io.BytesIO()
+    Image.fromarray(rgba, "RGBA").save(f, format="PNG")
+    return f.getvalue()
+
+
+def _textured_doc(alpha_mode, albedo_alpha, in_buffer):
+    """one quad with TEXCOORD_0 and albedo, ORM (occlusion + metallic-roughness) and normal textures"""
+    rng = np.random.default_rng(3)
+    albedo = rng.integers(0, 256, (4, 8, 4)).astype(np.uint8)
+    albedo[..., 3] = albedo_alpha
+    orm = rng.integers(0, 256, (2, 2, 4)).astype(np.uint8)
+    normal = np.full((2, 2, 4), 128, np.uint8)
+    normal[..., 2] = 255
+    P = np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], np.float32)
+    UV = np.array([[0, 0], [2, 0], [2, 1], [0, 1]], np.float32)
+    I = np.array([0, 2, 1, 0, 3, 2], np.uint16)
+    blob = bytearray(P.tobytes() + UV.tobytes() + I.tobytes() + b"\0\0")
+    views = [{"buffer": 0, "byteOffset": 0, "byteLength": 48}, {"buffer": 0, "byteOffset": 48, "byteLength": 32},
+             {"buffer": 0, "byteOffset": 80, "byteLength": 12}]
+    images = []
+    for name, img in (("base", albedo), ("orm", orm), ("normalmap", normal)):
+        data = _png(img)
+        if in_buffer:
+            views.append({"buffer": 0, "byteOffset": len(blob), "byteLength": len(data)})
+            images.append({"name": name, "bufferView": len(views) - 1, "mimeType": "image/png"})
+            blob += data + b"\0" * ((4 - len(data) % 4) % 4)
+        else:
+            images.append({"name": name, "uri": "data:image/png;base64," + base64.b64encode(data).decode()})
+    doc = {
+        "asset": {"version": "2.0"}, "scene": 0, "scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0}],
+        "meshes": [{"primitives": [{"attributes": {"POSITION": 0, "TEXCOORD_0": 1}, "indices": 2, "material": 0}]}],
+        "materials": [{"name": "tex", "alphaMode": alpha_mode, "alphaCutoff": 0.3,
+                       "normalTexture": {"index": 2}, "occlusionTexture": {"index": 1},
+                       "pbrMetallicRoughness": {"baseColorFactor": [1, 1, 1, 0.5], "baseColorTexture": {"index": 0},
+                                                "metallicRoughnessTexture": {"index": 1}}}],
+        "textures": [{"source": 0}, {"source": 1}, {"source": 2}], "images": images,
+        "bufferViews": views,
+        "accessors": [{"bufferView": 0, "componentType": 5126, "count": 4, "type": "VEC3"},
+                      {"bufferView": 1, "componentType": 5126, "count": 4, "type": "VEC2"},
+                      {"bufferView": 2, "componentType": 5123, "count": 6, "type": "SCALAR"}],
+    }
+    return doc, bytes(blob), (albedo, orm, normal)
+
+
+def _save(tmp_path, doc, blob, glb):
+    if glb:
+        doc["buffers"] = [{"byteLength": len(blob)}]
+        js = json.dumps(doc).encode()
+        js += b" " * ((4 - len(js) % 4) % 4)
+        body = struct.pack("<II", len(js), 0x4E4F534A) + js + struct.pack("<II", len(blob), 0x004E4942) + blob
+        p = tmp_path / "tex.glb"
+        p.write_bytes(b"glTF" + struct.pack("<II", 2, 12 + len(body)) + body)
+    else:
+        doc["buffers"] = [{"byteLength": len(blob), "uri": "data:application/octet-stream;base64,"
+                           + base64.b64encode(blob).decode()}]
+        p = tmp_path / "tex.gltf"
+        p.write_text(json.dumps(doc))
+    return str(p)
+
+
+@pytest.mark.parametrize("glb", [False, True])
+def test_textures_are_decoded_and_bound(tmp_path, glb):
+    doc, blob, (albedo, orm, normal) = _textured_doc("OPAQUE", 255, in_buffer=glb)
+    scene, info = gltf.load(_save(tmp_path, doc, blob, glb))
+    m = scene.materials[0]
+    assert not info["images_failed"]
+    slots = {"albedo": m.albedo_tex, "orm": m.orm_tex, "normal": m.normal_tex}
+    assert sorted(slots.values()) == [0, 1, 2] and m.emissive_tex == -1
+    assert info["textures"]["tex"] == slots
+    assert list(m.albedo_tex_scale) == [1.0, 1.0] and list(m.orm_tex_scale) == [1.0, 1.0]
+    texels, desc = scene.texel_arrays, scene.texture_array
+    for slot, img, srgb in (("albedo", albedo, 1), ("orm", orm, 0), ("normal", normal, 0)):
+        # colour is sRGB; metallic-roughness, occlusion and normal maps are data
+        assert np.array_equal(texels[slots[slot]], img) and desc[slots[slot]].srgb == srgb, slot
+    assert np.allclose(scene.uvs, [[0, 0], [2, 0], [2, 1], [0, 1]])
+    assert abs(m.transparency) < 1e-7 and m.alpha_cutoff == 0.0
+
+
+def test_alpha_modes_follow_the_reference_loader(tmp_path):
+    def load(mode, alpha):
+        doc, blob, _ = _textured_doc(mode, alpha, in_buffer=False)
+        return gltf.load(_save(tmp_path, doc, blob, False))[0].materials[0]
+    m = load("MASK", 0)                                   # texels below the cutoff: alpha testing at the cutoff
+    assert abs(m.alpha_cutoff - 0.3) < 1e-7 and not (m.material_mask & gltf.MASK_ALPHA_BLEND)
+    m = load("MASK", 200)                                 # every texel passes: stays opaque
+    assert m.alpha_cutoff == 0.0
+    m = load("BLEND", 2)                                  # cutouts: alpha testing at 1/2
+    assert m.alpha_cutoff == 0.5 and abs(m.transparency) < 1e-7
+    m = load("BLEND", 128)                                # tinted coverage: per-texel alpha, factor 0.5 folded in
+    assert m.alpha_cutoff == 0.0 and m.material_mask & gltf.MASK_ALPHA_BLEND
+    assert abs(m.transparency - 0.5) < 1e-7 and m.material_mask & gltf.MASK_TWO_SIDED

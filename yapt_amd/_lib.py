@@ -40,7 +40,17 @@ class Material(ctypes.Structure):
         ("alpha_cutoff", ctypes.c_float), ("cauchy_coeffs", ctypes.c_float * 2),
         ("subsurface_radius", F3), ("subsurface_amount", ctypes.c_float),
         ("subsurface_radius_scale", ctypes.c_float),
+        ("albedo_tex", ctypes.c_int32), ("normal_tex", ctypes.c_int32), ("orm_tex", ctypes.c_int32),
+        ("emissive_tex", ctypes.c_int32),
+        ("albedo_tex_scale", ctypes.c_float * 2), ("normal_tex_scale", ctypes.c_float * 2),
+        ("orm_tex_scale", ctypes.c_float * 2), ("emissive_tex_scale", ctypes.c_float * 2),
     ]
+
+
+class Texture(ctypes.Structure):
+    """ypt_texture: RGBA8 texels, srgb = colour data decoded before filtering"""
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("srgb", ctypes.c_uint32),
+                ("rgba8", ctypes.POINTER(ctypes.c_uint8))]
 
 
 YPT_LIGHT_POINT, YPT_LIGHT_SPOT, YPT_LIGHT_DIRECTIONAL = 1, 2, 3
@@ -127,6 +137,7 @@ class SceneDesc(ctypes.Structure):
         ("instance_material", ctypes.POINTER(ctypes.c_uint32)), ("instance_count", ctypes.c_uint32),
         ("materials", ctypes.POINTER(Material)), ("material_count", ctypes.c_uint32),
         ("punctual_lights", ctypes.POINTER(PunctualLight)), ("punctual_light_count", ctypes.c_uint32),
+        ("uvs", ctypes.POINTER(ctypes.c_float)), ("textures", ctypes.POINTER(Texture)), ("texture_count", ctypes.c_uint32),
     ]
 
 
@@ -154,7 +165,8 @@ class SceneArrays:
     """Keeps the numpy arrays a SceneDesc points into alive."""
 
     def __init__(self, positions, normals, indices, inst_first, inst_count, inst_material, materials, tangents=None,
-                 lights=()):
+                 lights=(), uvs=None, textures=()):
+        """textures: (H x W x 4 uint8 array, srgb) pairs; uvs: V x 2 floats"""
         self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
         self.tangents = None if tangents is None else np.ascontiguousarray(tangents, dtype=np.float32).reshape(-1, 4)
@@ -165,12 +177,21 @@ class SceneArrays:
         self.materials = (Material * len(materials))(*materials)
         self.lights = [l if isinstance(l, PunctualLight) else l.params() for l in lights]
         self.light_array = (PunctualLight * max(1, len(self.lights)))(*self.lights)
+        self.uvs = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+        self.texel_arrays = [np.ascontiguousarray(t, dtype=np.uint8) for t, _ in textures]
+        texs = [Texture(a.shape[1], a.shape[0], 1 if srgb else 0, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+                for a, (_, srgb) in zip(self.texel_arrays, textures)]
+        for a in self.texel_arrays:
+            if a.ndim != 3 or a.shape[2] != 4:
+                raise ValueError("textures must be H x W x 4 uint8")
+        self.texture_array = (Texture * max(1, len(texs)))(*texs)
         self.desc = SceneDesc(
             _fptr(self.positions), None if self.normals is None else _fptr(self.normals),
             None if self.tangents is None else _fptr(self.tangents), len(self.positions),
             _uptr(self.indices), len(self.indices), _uptr(self.inst_first), _uptr(self.inst_count),
             _uptr(self.inst_material), len(self.inst_first), self.materials, len(materials),
-            self.light_array, len(self.lights))
+            self.light_array, len(self.lights), None if self.uvs is None else _fptr(self.uvs), self.texture_array,
+            len(self.texel_arrays))
 
     @property
     def triangle_count(self):

@@ -36,40 +36,6 @@ static constexpr uint32_t LN_REGULARIZED = 1u << 4;
 static constexpr uint32_t LN_PUNCTUAL = 1u << 6;
 static constexpr uint32_t LN_DIRECTIONAL = 1u << 7;
 
-// ---------------------------------------------------------------- half precision of the packed normal
-// f32tof16 / f16tof32 with round-to-nearest-even; the packed node keeps its normal at this precision.
-YPT_HD uint32_t f32_to_f16(float f) {
-  uint32_t x = as_uint(f);
-  uint32_t sign = (x >> 16) & 0x8000u;
-  uint32_t e = (x >> 23) & 0xFFu;
-  uint32_t m = x & 0x7FFFFFu;
-  if (e == 0xFFu) return sign | 0x7C00u | (m ? 0x200u : 0u);
-  int ex = (int)e - 127 + 15;
-  if (ex >= 31) return sign | 0x7C00u;
-  if (ex <= 0) {
-    if (ex < -10) return sign;
-    m |= 0x800000u;
-    uint32_t shift = (uint32_t)(14 - ex);
-    uint32_t hm = m >> shift;
-    uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
-    if (rem > half || (rem == half && (hm & 1u))) hm++;
-    return sign | hm;
-  }
-  uint32_t h = sign | ((uint32_t)ex << 10) | (m >> 13);
-  uint32_t rem = m & 0x1FFFu;
-  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
-  return h;
-}
-YPT_HD float f16_to_f32(uint32_t h) {
-  uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
-  if (e == 0) {
-    float v = (float)m * 5.9604644775390625e-8f;  // m * 2^-24, exact
-    return sign ? -v : v;
-  }
-  if (e == 31) return as_float(sign | 0x7F800000u | (m << 13));
-  return as_float(sign | ((e + 112u) << 23) | (m << 13));
-}
-
 // ---------------------------------------------------------------- light subpath nodes
 // LightPathNodePacked (BDPTCommon.hlsl:37-44, 300-335): 64 bytes here (no next-index link: the
 // vertices of a subpath are stored in order, light first).
@@ -206,7 +172,9 @@ YPT_HD void bdpt_surface(const SceneView& S, BdptState& rs, const Hit& h, vec3 r
   v.wo = normalize(-rayDir);
   v.frontFace = dot(rayDir, cross(g.p2 - g.p1, g.p3 - g.p1)) < 0.f;
   bool reg = (rs.flags & RS_REGULARIZE) != 0;
-  SurfaceDef sd = make_surface(m, g.geometryNormal, g.normal, g.tangent, v.wo, true, reg, L, S.spec);
+  SurfaceTexels tx;
+  sample_surface_texels(S, m, h.tri, h.b1, h.b2, tx);
+  SurfaceDef sd = make_surface(m, g.geometryNormal, g.normal, g.tangent, v.wo, true, reg, L, S.spec, &tx);
   // handleRayStartInsideMedium (hitShadersCommon.hlsl:213-222)
   if (dot(-v.wo, sd.geometryNormal) > 0.f && rs.n == 0 && !surface_two_sided(sd.flags))
     rs.entered(sd.dielectricIOR, sd.absorption, matId);
@@ -523,7 +491,8 @@ YPT_HD uint32_t trace_light_path(const SceneView& S, const FrameConstants& fc, c
       float focus = lm.emissionFocus;
       vec3 local = twoSided ? sample_cosine_power_sphere(r1[0], r1[1], focus) : sample_cosine_power_hemisphere(r1[0], r1[1], focus);
       mat3 tanToWS = basis_transform(gn, tangent);
-      radiance = spectrum_unbounded(lm.emissive, L, S.spec);
+      radiance = surface_emission(S, lm, tri, bx, by, L);
+      if (alpha_cut(S, tri, bx, by)) radiance.set(0.f);
       pdfPos = 1.f / ((float)primCount * area);
       pdfDir = twoSided ? pdf_cosine_power_sphere(local.y, focus) : pdf_cosine_power_hemisphere(local.y, focus);
       profile = emission_profile(local.y, focus);
@@ -644,8 +613,10 @@ YPT_HD void light_node_surface(const SceneView& S, const LightNode& n, vec3 prev
   TriGeometry g;
   fetch_triangle(S, n.tri, n.b1, n.b2, g);
   // no two-sided re-orientation here, as in the reference
+  SurfaceTexels tx;
+  sample_surface_texels(S, m, n.tri, n.b1, n.b2, tx);
   SurfaceDef sd = make_surface(m, g.geometryNormal, g.normal, g.tangent, out.wo, false, (n.flags & LN_REGULARIZED) != 0, L,
-                               S.spec);
+                               S.spec, &tx);
   BdptState rs = light_node_state(n, false);
   make_surface_context(out.ctx, sd, rs.current_ior(), rs.previous_ior(), out.wo, frontFace, L, true);
 }
@@ -962,7 +933,11 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
   BdptVertex v;
   const MaterialGPU* m = nullptr;
   bdpt_surface(S, c.rs, h, c.rayOrigin, c.rayDir, L, v, &m);
-  if (k == 0 && fh) first_hit_features(fc, *m, v.ctx.def, v.pos, *fh);
+  if (k == 0 && fh) {
+    SurfaceTexels tx;
+    sample_surface_texels(S, *m, h.tri, h.b1, h.b2, tx);
+    first_hit_features(fc, *m, v.ctx.def, v.pos, *fh, &tx);
+  }
   cv.pos = v.pos;
   cv.frontFace = v.frontFace;
   cv.wo = v.wo;

@@ -51,6 +51,40 @@ YPT_HD void sel_set(T (&a)[N], uint32_t i, const T& v) {
   for (int k = 0; k < N; ++k) a[k] = pick((uint32_t)k == i, v, a[k]);
 }
 
+// f32tof16 / f16tof32 with round-to-nearest-even (packed light-vertex normals, packed occlusion and
+// denoiser features are kept at this precision)
+YPT_HD uint32_t f32_to_f16(float f) {
+  uint32_t x = as_uint(f);
+  uint32_t sign = (x >> 16) & 0x8000u;
+  uint32_t e = (x >> 23) & 0xFFu;
+  uint32_t m = x & 0x7FFFFFu;
+  if (e == 0xFFu) return sign | 0x7C00u | (m ? 0x200u : 0u);
+  int ex = (int)e - 127 + 15;
+  if (ex >= 31) return sign | 0x7C00u;
+  if (ex <= 0) {
+    if (ex < -10) return sign;
+    m |= 0x800000u;
+    uint32_t shift = (uint32_t)(14 - ex);
+    uint32_t hm = m >> shift;
+    uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (hm & 1u))) hm++;
+    return sign | hm;
+  }
+  uint32_t h = sign | ((uint32_t)ex << 10) | (m >> 13);
+  uint32_t rem = m & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+  return h;
+}
+YPT_HD float f16_to_f32(uint32_t h) {
+  uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+  if (e == 0) {
+    float v = (float)m * 5.9604644775390625e-8f;  // m * 2^-24, exact
+    return sign ? -v : v;
+  }
+  if (e == 31) return as_float(sign | 0x7F800000u | (m << 13));
+  return as_float(sign | ((e + 112u) << 23) | (m << 13));
+}
+
 YPT_HD float fmin_(float a, float b) { return a < b ? a : b; }   // HLSL min on non-NaN operands
 YPT_HD float fmax_(float a, float b) { return a > b ? a : b; }
 YPT_HD float clampf(float x, float lo, float hi) { return fmin_(fmax_(x, lo), hi); }

@@ -27,8 +27,21 @@ struct MaterialGPU {
   float cauchyA, cauchyB;
   uint32_t flags;
   uint32_t emissiveNonZero;
-  // linear RGB kept for the first-hit feature buffers
+  // linear RGB: the first-hit feature buffers, and the factors textures multiply
   float albedoRGB[3], emissiveRGB[3], sheenRGB[3], pad0[3];
+  // textures (albedo, normal, occlusion-roughness-metalness, emissive): index or -1, UV scale
+  int32_t tex[4];
+  float texScale[8];
+};
+
+static constexpr int TEX_ALBEDO = 0, TEX_NORMAL = 1, TEX_ORM = 2, TEX_EMISSIVE = 3;
+
+// Texture values at a surface point, sampled by the caller (scene.h sample_surface_texels), and the
+// RGB -> spectrum table that textured colours go through at run time.
+struct SurfaceTexels {
+  uint32_t mask;   // bit k: texture k is bound and was sampled
+  float v[4][4];   // RGBA per texture, linear
+  const float* rgb2spec;
 };
 
 struct SurfaceDef {
@@ -119,7 +132,7 @@ YPT_HD bool is_delta_distribution(float ax, float ay, const SurfaceDef& s) {
 // hitShadersCommon.hlsl:51-91, materialsCommon.hlsl:313-352, 413-418, 440-448, 487-501.
 YPT_HD SurfaceDef make_surface(const MaterialGPU& m, vec3 geometryNormal, vec3 shadingNormal, vec3 tangent,
                                vec3 wo, bool applyTwoSided, bool regularizePath, const Lambdas& L,
-                               const SpectralTables& T) {
+                               const SpectralTables& T, const SurfaceTexels* tx = nullptr) {
   SurfaceDef s;
   s.transparency = m.transparency;
   s.metalness = m.metalness;
@@ -145,6 +158,40 @@ YPT_HD SurfaceDef make_surface(const MaterialGPU& m, vec3 geometryNormal, vec3 s
   s.baseNormal = shadingNormal;
   s.geometryNormal = geometryNormal;
   s.tangent = tangent;
+  // modifySurfaceMaterialParametersWithTextures (materialsCommon.hlsl:354-397), on the RGB values
+  SpectralColor albedoCol = m.albedo, emissiveCol = m.emissive;
+  if (tx && tx->mask) {
+    if (tx->mask & (1u << TEX_ALBEDO)) {
+      const float* a = tx->v[TEX_ALBEDO];
+      vec3 rgb(m.albedoRGB[0] * a[0], m.albedoRGB[1] * a[1], m.albedoRGB[2] * a[2]);
+      rgb_coeffs_table(tx->rgb2spec, rgb, albedoCol.c);
+      albedoCol.scale = 1.f;
+      if (s.flags & MATMASK_ALPHA_BLEND) s.transparency = 1.f - a[3] * (1.f - s.transparency);
+    }
+    if (tx->mask & (1u << TEX_NORMAL)) {
+      const float* t = tx->v[TEX_NORMAL];
+      vec3 n = normalize(vec3(t[0] * 2.f - 1.f, t[1] * 2.f - 1.f, t[2] * 2.f - 1.f));
+      mat3 tbase = basis_transform(s.baseNormal, s.tangent);
+      s.baseNormal = mul(vec3(n.x, n.z, n.y), tbase);
+    }
+    if (tx->mask & (1u << TEX_ORM)) {
+      const float* o = tx->v[TEX_ORM];
+      float occ = f16_to_f32(f32_to_f16(o[0]));  // packOcclusion / unpackOcclusion: half precision
+      s.diffOcclusion = occ;
+      s.specOcclusion = occ;
+      s.roughness *= o[1];
+      s.metalness *= o[2];
+    }
+    if (tx->mask & (1u << TEX_EMISSIVE)) {
+      const float* e = tx->v[TEX_EMISSIVE];
+      vec3 rgb(m.emissiveRGB[0] * e[0], m.emissiveRGB[1] * e[1], m.emissiveRGB[2] * e[2]);
+      float mx = fmax_(fmax_(rgb.x, rgb.y), rgb.z);
+      float scale = 2.f * mx;
+      vec3 nv = scale == 0.f ? vec3(0.f) : rgb / scale;
+      rgb_coeffs_table(tx->rgb2spec, nv, emissiveCol.c);
+      emissiveCol.scale = scale;
+    }
+  }
   if (applyTwoSided && s.transparency == 0.f && surface_two_sided(s.flags)) {
     s.baseNormal = orient_to_same_side(wo, s.baseNormal);
     s.coatingNormal = orient_to_same_side(wo, s.coatingNormal);
@@ -163,9 +210,9 @@ YPT_HD SurfaceDef make_surface(const MaterialGPU& m, vec3 geometryNormal, vec3 s
     s.clearCoatRoughness = regularize(s.clearCoatRoughness, true);
   }
   s.specular = spectrum_bounded(m.specular, L);
-  s.albedo = spectrum_bounded(m.albedo, L);
+  s.albedo = spectrum_bounded(albedoCol, L);
   s.absorption = spectrum_unbounded_no_illum(m.absorption, L);
-  s.emissive = spectrum_unbounded(m.emissive, L, T);
+  s.emissive = spectrum_unbounded(emissiveCol, L, T);
   s.sheenColor = spectrum_bounded(m.sheenColor, L);
   return s;
 }

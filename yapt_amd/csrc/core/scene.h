@@ -69,6 +69,32 @@ struct FrameConstants {
   uint32_t punctualLightCount;  // the last punctualLightCount entries of the light table
 };
 
+// one RGBA8 texture in the texel buffer (ypt_texture)
+struct TextureGPU {
+  uint32_t offset;  // first texel
+  uint32_t width, height;
+  uint32_t srgb;    // decode sRGB before filtering
+};
+
+// Alpha testing (hitShadersCommon.hlsl:162-183): a candidate hit on a triangle whose material has
+// an alpha cutoff, an albedo texture and UVs is taken only where the albedo texture's alpha at the
+// hit is at least the cutoff, for closest hits and shadow queries alike. Each such triangle has a
+// record with its vertex UVs, so that traversal samples the alpha without the mesh buffers; the UV
+// interpolation is the one surface_uv does and the alpha the one sample_texture filters.
+struct alignas(16) AlphaTri {
+  float uv[6];     // vertex UVs
+  float scale[2];  // albedo texture UV scale
+  uint32_t tex;    // albedo texture
+  float cutoff;
+  uint32_t pad[2];
+};
+
+struct AlphaView {
+  const AlphaTri* tris;  // null: the scene has no alpha-tested triangles
+  const TextureGPU* textures;
+  const uint32_t* texels;
+};
+
 struct SceneView {
   const float* pos;          // 3 floats per vertex
   const float* nrm;          // 3 floats per vertex
@@ -79,6 +105,12 @@ struct SceneView {
   const MaterialGPU* mats;
   const LightGPU* lights;
   const float* rgb2spec;     // optional 64^3 x 4 RGB->coefficient table (environment colours)
+  const float* uv;           // 2 floats per vertex, or null: UV (0.5, 0.5)
+  const TextureGPU* textures;
+  const uint32_t* texels;    // RGBA8 per texel
+  const float* srgbLut;      // 256 entries: sRGB-encoded byte -> linear
+  const uint32_t* triAlpha;  // per triangle: 0 opaque, else 1 + its alpha-test record; null: none
+  AlphaView alpha;
   uint32_t ntris;
   uint32_t ninst;
   SpectralTables spec;
@@ -90,6 +122,99 @@ YPT_HD vec3 load3(const float* p, uint32_t i) { return vec3(p[3 * i], p[3 * i + 
 YPT_HD void tri_vertices(const SceneView& S, uint32_t tri, vec3& p1, vec3& p2, vec3& p3) {
   uint32_t i0 = S.idx[3 * tri], i1 = S.idx[3 * tri + 1], i2 = S.idx[3 * tri + 2];
   p1 = load3(S.pos, i0); p2 = load3(S.pos, i1); p3 = load3(S.pos, i2);
+}
+
+// Texture2D.SampleLevel(g_colorSampler, uv, 0): bilinear between texel centres, wrap addressing,
+// sRGB textures decoded per texel before the filter as an _SRGB format is
+YPT_HD void texel_rgba(const SceneView& S, const TextureGPU& t, uint32_t x, uint32_t y, float out[4]) {
+  uint32_t v = S.texels[t.offset + y * t.width + x];
+  for (int k = 0; k < 3; ++k) {
+    uint32_t b = (v >> (8 * k)) & 0xFFu;
+    out[k] = t.srgb ? S.srgbLut[b] : (float)b / 255.f;
+  }
+  out[3] = (float)(v >> 24) / 255.f;
+}
+
+YPT_HD void sample_texture(const SceneView& S, uint32_t index, float u, float v, float out[4]) {
+  const TextureGPU& t = S.textures[index];
+  float uf = u - floorf(u), vf = v - floorf(v);  // wrap first: keeps the texel index in range
+  float x = uf * (float)t.width - 0.5f, y = vf * (float)t.height - 0.5f;
+  float fx = floorf(x), fy = floorf(y);
+  float ax = x - fx, ay = y - fy;
+  uint32_t x0 = fx < 0.f ? t.width - 1u : (uint32_t)fx, y0 = fy < 0.f ? t.height - 1u : (uint32_t)fy;
+  if (x0 >= t.width) x0 = t.width - 1u;
+  if (y0 >= t.height) y0 = t.height - 1u;
+  uint32_t x1 = x0 + 1u == t.width ? 0u : x0 + 1u, y1 = y0 + 1u == t.height ? 0u : y0 + 1u;
+  float a[4], b[4], c[4], d[4];
+  texel_rgba(S, t, x0, y0, a);
+  texel_rgba(S, t, x1, y0, b);
+  texel_rgba(S, t, x0, y1, c);
+  texel_rgba(S, t, x1, y1, d);
+  for (int k = 0; k < 4; ++k) {
+    float top = a[k] + ax * (b[k] - a[k]);
+    float bot = c[k] + ax * (d[k] - c[k]);
+    out[k] = top + ay * (bot - top);
+  }
+}
+
+// fetchMeshUV (materialsCommon.hlsl): interpolated UV, (0.5, 0.5) for meshes without UVs
+YPT_HD void surface_uv(const SceneView& S, uint32_t tri, float b1, float b2, float& u, float& v) {
+  if (!S.uv) { u = 0.5f; v = 0.5f; return; }
+  uint32_t i0 = S.idx[3 * tri], i1 = S.idx[3 * tri + 1], i2 = S.idx[3 * tri + 2];
+  float b0 = 1.f - b1 - b2;
+  u = b0 * S.uv[2 * i0] + b1 * S.uv[2 * i1] + b2 * S.uv[2 * i2];
+  v = b0 * S.uv[2 * i0 + 1] + b1 * S.uv[2 * i1 + 1] + b2 * S.uv[2 * i2 + 1];
+}
+
+YPT_HD float texel_alpha(const uint32_t* texels, const TextureGPU& t, uint32_t x, uint32_t y) {
+  return (float)(texels[t.offset + y * t.width + x] >> 24) / 255.f;
+}
+
+// channel 3 of sample_texture, on the same texel coordinates and weights
+YPT_HD float sample_alpha(const TextureGPU& t, const uint32_t* texels, float u, float v) {
+  float uf = u - floorf(u), vf = v - floorf(v);
+  float x = uf * (float)t.width - 0.5f, y = vf * (float)t.height - 0.5f;
+  float fx = floorf(x), fy = floorf(y);
+  float ax = x - fx, ay = y - fy;
+  uint32_t x0 = fx < 0.f ? t.width - 1u : (uint32_t)fx, y0 = fy < 0.f ? t.height - 1u : (uint32_t)fy;
+  if (x0 >= t.width) x0 = t.width - 1u;
+  if (y0 >= t.height) y0 = t.height - 1u;
+  uint32_t x1 = x0 + 1u == t.width ? 0u : x0 + 1u, y1 = y0 + 1u == t.height ? 0u : y0 + 1u;
+  float a = texel_alpha(texels, t, x0, y0), b = texel_alpha(texels, t, x1, y0);
+  float c = texel_alpha(texels, t, x0, y1), d = texel_alpha(texels, t, x1, y1);
+  float top = a + ax * (b - a);
+  float bot = c + ax * (d - c);
+  return top + ay * (bot - top);
+}
+
+// record k (the triangle's alpha word minus one) passes at barycentrics (b1, b2)
+YPT_HD bool alpha_test_passes(const AlphaView& A, uint32_t k, float b1, float b2) {
+  const AlphaTri& r = A.tris[k];
+  float b0 = 1.f - b1 - b2;
+  float u = b0 * r.uv[0] + b1 * r.uv[2] + b2 * r.uv[4];
+  float v = b0 * r.uv[1] + b1 * r.uv[3] + b2 * r.uv[5];
+  return sample_alpha(A.textures[r.tex], A.texels, u * r.scale[0], v * r.scale[1]) >= r.cutoff;
+}
+
+// a sampled point of an alpha-tested emitter that the alpha test cuts away emits nothing
+YPT_HD bool alpha_cut(const SceneView& S, uint32_t tri, float b1, float b2) {
+  if (!S.triAlpha) return false;
+  uint32_t a = S.triAlpha[tri];
+  return a != 0u && !alpha_test_passes(S.alpha, a - 1u, b1, b2);
+}
+
+// the bound textures of material m at a surface point (tri, b1, b2); mask 0 when it has none
+YPT_HD void sample_surface_texels(const SceneView& S, const MaterialGPU& m, uint32_t tri, float b1, float b2, SurfaceTexels& tx) {
+  tx.mask = 0;
+  tx.rgb2spec = S.rgb2spec;
+  if (m.tex[0] < 0 && m.tex[1] < 0 && m.tex[2] < 0 && m.tex[3] < 0) return;
+  float u, v;
+  surface_uv(S, tri, b1, b2, u, v);
+  for (int k = 0; k < 4; ++k) {
+    if (m.tex[k] < 0) continue;
+    sample_texture(S, (uint32_t)m.tex[k], u * m.texScale[2 * k], v * m.texScale[2 * k + 1], tx.v[k]);
+    tx.mask |= 1u << k;
+  }
 }
 
 }  // namespace ypt

@@ -199,36 +199,14 @@ YPT_HD void shade_miss(const SceneView& S, const FrameConstants& fc, PathState& 
 }
 
 // RGB -> coefficients via the 64^3 table (only needed for non-grey colours on the device).
-YPT_HD void rgb_coeffs(const SceneView& S, vec3 v, float c[3]) {
-  if (v.x == v.y && v.x == v.z) { c[0] = 0.f; c[1] = 0.f; c[2] = sigmoid_inv(v.x); return; }
-  const int R = 64;
-  int i0[3], i1[3]; float f[3];
-  for (int a = 0; a < 3; ++a) lut1_index(lut1_coord(v[a], R), R, i0[a], i1[a], f[a]);
-  for (int k = 0; k < 3; ++k) {
-    float cz[2];
-    for (int zz = 0; zz < 2; ++zz) {
-      int z = zz ? i1[2] : i0[2];
-      float cy[2];
-      for (int yy = 0; yy < 2; ++yy) {
-        int y = yy ? i1[1] : i0[1];
-        float a = S.rgb2spec[((z * R + y) * R + i0[0]) * 4 + k];
-        float b = S.rgb2spec[((z * R + y) * R + i1[0]) * 4 + k];
-        cy[yy] = a + f[0] * (b - a);
-      }
-      cz[zz] = cy[0] + f[1] * (cy[1] - cy[0]);
-    }
-    c[k] = cz[0] + f[2] * (cz[1] - cz[0]);
-  }
-}
+YPT_HD void rgb_coeffs(const SceneView& S, vec3 v, float c[3]) { rgb_coeffs_table(S.rgb2spec, v, c); }
 
 YPT_HD Spectrum rgb_unbounded_spectrum(const SceneView& S, vec3 values, const Lambdas& L) {  // setFromRGBUnbounded
   float m = fmax_(fmax_(values.x, values.y), values.z);
   float scale = 2.f * m;
   vec3 nv = scale == 0.f ? vec3(0.f) : values / scale;
   SpectralColor col;
-  if (nv.x == nv.y && nv.x == nv.z) { col.c[0] = 0.f; col.c[1] = 0.f; col.c[2] = sigmoid_inv(nv.x); }
-  else if (S.rgb2spec) rgb_coeffs(S, nv, col.c);
-  else { col.c[0] = 0.f; col.c[1] = 0.f; col.c[2] = as_float(0x7fc00000u); }  // table missing: NaN marks the sample
+  rgb_coeffs(S, nv, col.c);
   col.scale = scale;
   return spectrum_unbounded(col, L, S.spec);
 }
@@ -243,6 +221,16 @@ YPT_HD uint32_t sample_light_index(const LightGPU* lights, uint32_t count, float
   uint32_t chosen = (scaled - (float)index) < e.aliasThreshold ? index : e.aliasIndex;
   pickPdf = lights[chosen].selectionPdf;
   return chosen;
+}
+
+// emission of material m at a point of triangle tri (fetchSurfaceMaterialParameters +
+// modifySurfaceEmissionWithTexture, materialsCommon.hlsl:420-428)
+YPT_HD Spectrum surface_emission(const SceneView& S, const MaterialGPU& m, uint32_t tri, float b1, float b2, const Lambdas& L) {
+  if (m.tex[TEX_EMISSIVE] < 0) return spectrum_unbounded(m.emissive, L, S.spec);
+  float u, v, e[4];
+  surface_uv(S, tri, b1, b2, u, v);
+  sample_texture(S, (uint32_t)m.tex[TEX_EMISSIVE], u * m.texScale[2 * TEX_EMISSIVE], v * m.texScale[2 * TEX_EMISSIVE + 1], e);
+  return rgb_unbounded_spectrum(S, vec3(m.emissiveRGB[0] * e[0], m.emissiveRGB[1] * e[1], m.emissiveRGB[2] * e[2]), L);
 }
 
 // samplePunctualLightIndex / punctualLightSelectionPDF (lightSampling.hlsl:44-57): the punctual
@@ -330,7 +318,7 @@ YPT_HD void sample_explicit_light(const SceneView& S, const FrameConstants& fc, 
   vec3 lp = b0 * p1 + bx * p2 + by * p3;
   const MaterialGPU& m = S.mats[light.material];
   bool twoSided = surface_two_sided(m.flags);
-  Spectrum radiance = spectrum_unbounded(m.emissive, L, S.spec);
+  Spectrum radiance = surface_emission(S, m, tri, bx, by, L);
   float pdfPos = 1.f / ((float)primCount * area);
   vec3 toLight = lp - pos;
   float len = length(toLight);
@@ -359,7 +347,7 @@ YPT_HD float light_connection_pdf(const SceneView& S, vec3 fromToLight, uint32_t
 
 // writeMaterialParamsForFirstBounce (hitShadersCommon.hlsl:94-136), unpacked
 YPT_HD void first_hit_features(const FrameConstants& fc, const MaterialGPU& m, const SurfaceDef& sd, vec3 hitPos,
-                               FirstHitFeatures& f) {
+                               FirstHitFeatures& f, const SurfaceTexels* tx = nullptr) {
   float cw = sd.clearCoatAmount, sw = sd.sheenAmount, bw = sd.specularAmount;
   float wsum = cw + sw + bw;
   if (wsum == 0.f) wsum = 1.f;
@@ -369,6 +357,10 @@ YPT_HD void first_hit_features(const FrameConstants& fc, const MaterialGPU& m, c
   f.normal = normalize(n);
   vec3 em(m.emissiveRGB[0], m.emissiveRGB[1], m.emissiveRGB[2]);
   vec3 al(m.albedoRGB[0], m.albedoRGB[1], m.albedoRGB[2]);
+  if (tx && (tx->mask & (1u << TEX_ALBEDO)))
+    al = vec3(al.x * tx->v[TEX_ALBEDO][0], al.y * tx->v[TEX_ALBEDO][1], al.z * tx->v[TEX_ALBEDO][2]);
+  if (tx && (tx->mask & (1u << TEX_EMISSIVE)))
+    em = vec3(em.x * tx->v[TEX_EMISSIVE][0], em.y * tx->v[TEX_EMISSIVE][1], em.z * tx->v[TEX_EMISSIVE][2]);
   vec3 sh(m.sheenRGB[0], m.sheenRGB[1], m.sheenRGB[2]);
   f.albedo = em / (vec3(1.f) + em) + al + sh * sd.sheenAmount;
   f.roughness = sd.roughness * bw + sd.clearCoatRoughness * cw + sd.sheenRoughness * sw;
@@ -412,10 +404,12 @@ YPT_HD void shade_hit(const SceneView& S, const FrameConstants& fc, const uint32
 
   if (ps.pathLength >= fc.regularizeAfterVertices) ps.flags |= RS_REGULARIZE;
   bool regularizePath = (ps.flags & RS_REGULARIZE) != 0;
-  SurfaceDef sd = make_surface(m, geometryNormal, normal, tangent, -rayDir, true, regularizePath, L, S.spec);
+  SurfaceTexels tx;
+  sample_surface_texels(S, m, h.tri, h.b1, h.b2, tx);
+  SurfaceDef sd = make_surface(m, geometryNormal, normal, tangent, -rayDir, true, regularizePath, L, S.spec, &tx);
   vec3 hitPos = ps.origin + ps.dir * h.t;
 
-  if (ps.pathLength == 0 && fh) first_hit_features(fc, m, sd, hitPos, *fh);
+  if (ps.pathLength == 0 && fh) first_hit_features(fc, m, sd, hitPos, *fh, &tx);
 
   // handleRayStartInsideMedium (hitShadersCommon.hlsl:213-222)
   if (dot(rayDir, geometryNormal) > 0.f && ps.nvol == 0 && !surface_two_sided(sd.flags))

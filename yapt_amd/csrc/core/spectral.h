@@ -155,6 +155,30 @@ struct SpectralColor {
   float scale;
 };
 
+// RGB -> sigmoid-polynomial coefficients via the 64^3 table (trilinear; grey colours exactly)
+YPT_HD void rgb_coeffs_table(const float* table, vec3 v, float c[3]) {
+  if (v.x == v.y && v.x == v.z) { c[0] = 0.f; c[1] = 0.f; c[2] = sigmoid_inv(v.x); return; }
+  if (!table) { c[0] = 0.f; c[1] = 0.f; c[2] = as_float(0x7fc00000u); return; }  // table missing: NaN marks the sample
+  const int R = 64;
+  int i0[3], i1[3]; float f[3];
+  for (int a = 0; a < 3; ++a) lut1_index(lut1_coord(v[a], R), R, i0[a], i1[a], f[a]);
+  for (int k = 0; k < 3; ++k) {
+    float cz[2];
+    for (int zz = 0; zz < 2; ++zz) {
+      int z = zz ? i1[2] : i0[2];
+      float cy[2];
+      for (int yy = 0; yy < 2; ++yy) {
+        int y = yy ? i1[1] : i0[1];
+        float a = table[((z * R + y) * R + i0[0]) * 4 + k];
+        float b = table[((z * R + y) * R + i1[0]) * 4 + k];
+        cy[yy] = a + f[0] * (b - a);
+      }
+      cz[zz] = cy[0] + f[1] * (cy[1] - cy[0]);
+    }
+    c[k] = cz[0] + f[2] * (cz[1] - cz[0]);
+  }
+}
+
 YPT_HD Spectrum spectrum_bounded(const SpectralColor& col, const Lambdas& L) { return spectrum_from_coeffs(col.c, L); }
 YPT_HD Spectrum spectrum_unbounded_no_illum(const SpectralColor& col, const Lambdas& L) {
   Spectrum r = spectrum_from_coeffs(col.c, L);

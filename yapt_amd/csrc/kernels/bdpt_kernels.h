@@ -21,7 +21,7 @@ template <int WIDTH, bool COUNT>
 struct GpuTracer {
   const Bvh2Node* nodes2;
   const Bvh8Node* nodes8;
-  const TriRecord* tris;
+  TriSet tris;
   SceneView sv;
   TravStack st;
   TraversalCounters cnt;
@@ -58,7 +58,7 @@ struct GpuTracer {
     vec3 p1, p2, p3;
     tri_vertices(sv, tri, p1, p2, p3);
     float tt, b1, b2;
-    if (!intersect_triangle(rp, p1, p2, p3, tmin, tmax, tt, b1, b2)) return false;
+    if (!intersect_triangle(rp, p1, p2, p3, tmin, tmax, tt, b1, b2) || alpha_cut(sv, tri, b1, b2)) return false;
     float t;
     uint32_t id;
     return !trace(1, o, d, tmin, tt, tri, t, id, b1, b2);

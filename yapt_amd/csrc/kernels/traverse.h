@@ -1,12 +1,20 @@
 // BVH traversal on gfx950: one ray per lane, short traversal stack in LDS with a global-memory
 // overflow, both children's boxes fetched in one 64-byte load (BVH2) or eight in 256 bytes (BVH8),
 // leaves intersected as soon as their box is hit. Closest hit = minimum (t, triangle id); the
-// shadow query stops at the first triangle that is closer than the light triangle.
+// shadow query stops at the first triangle that is closer than the light triangle. A triangle
+// whose record carries an alpha word counts only where its albedo alpha passes the cutoff; ALPHA =
+// false leaves that test out for scenes without alpha-tested triangles.
 #pragma once
 #include "../core/intersect.h"
 #include "../runtime/bvh.h"
 
 namespace ypt {
+
+// the triangle records of a BVH and what alpha testing of them reads
+struct TriSet {
+  const TriRecord* recs;
+  AlphaView alpha;
+};
 
 struct TraversalCounters {
   uint32_t nodes;
@@ -39,40 +47,40 @@ __device__ __forceinline__ TriRecord load_tri(const TriRecord* tris, uint32_t k)
   float4 a = p[0], b = p[1], c = p[2];
   TriRecord r;
   r.v0[0] = a.x; r.v0[1] = a.y; r.v0[2] = a.z; r.id = __float_as_uint(a.w);
-  r.v1[0] = b.x; r.v1[1] = b.y; r.v1[2] = b.z;
+  r.v1[0] = b.x; r.v1[1] = b.y; r.v1[2] = b.z; r.alpha = __float_as_uint(b.w);
   r.v2[0] = c.x; r.v2[1] = c.y; r.v2[2] = c.z;
   return r;
 }
 
-template <bool COUNT>
-__device__ __forceinline__ void leaf_closest(const TriRecord* tris, uint32_t ref, const RayPre& rp, float tmin, float& best,
+template <bool COUNT, bool ALPHA = true>
+__device__ __forceinline__ void leaf_closest(const TriSet& tris, uint32_t ref, const RayPre& rp, float tmin, float& best,
                                              uint32_t& bestId, float& bb1, float& bb2, TraversalCounters& cnt) {
   uint32_t first = ref & 0x07FFFFFFu, count = ((ref >> 27) & 15u) + 1u;
   for (uint32_t k = first; k < first + count; ++k) {
-    TriRecord r = load_tri(tris, k);
+    TriRecord r = load_tri(tris.recs, k);
     if (COUNT) cnt.tris++;
     float t, b1, b2;
     if (intersect_triangle(rp, vec3(r.v0[0], r.v0[1], r.v0[2]), vec3(r.v1[0], r.v1[1], r.v1[2]), vec3(r.v2[0], r.v2[1], r.v2[2]),
                            tmin, best, t, b1, b2) &&
-        hit_closer(t, r.id, best, bestId)) {
+        hit_closer(t, r.id, best, bestId) && (!ALPHA || r.alpha == 0u || alpha_test_passes(tris.alpha, r.alpha - 1u, b1, b2))) {
       best = t; bestId = r.id; bb1 = b1; bb2 = b2;
     }
   }
 }
 
 // true when a triangle other than `target` is hit closer than (targetT, target)
-template <bool COUNT>
-__device__ __forceinline__ bool leaf_occluded(const TriRecord* tris, uint32_t ref, const RayPre& rp, float tmin, float targetT,
+template <bool COUNT, bool ALPHA = true>
+__device__ __forceinline__ bool leaf_occluded(const TriSet& tris, uint32_t ref, const RayPre& rp, float tmin, float targetT,
                                               uint32_t target, TraversalCounters& cnt) {
   uint32_t first = ref & 0x07FFFFFFu, count = ((ref >> 27) & 15u) + 1u;
   for (uint32_t k = first; k < first + count; ++k) {
-    TriRecord r = load_tri(tris, k);
+    TriRecord r = load_tri(tris.recs, k);
     if (COUNT) cnt.tris++;
     if (r.id == target) continue;
     float t, b1, b2;
     if (intersect_triangle(rp, vec3(r.v0[0], r.v0[1], r.v0[2]), vec3(r.v1[0], r.v1[1], r.v1[2]), vec3(r.v2[0], r.v2[1], r.v2[2]),
                            tmin, targetT, t, b1, b2) &&
-        hit_closer(t, r.id, targetT, target))
+        hit_closer(t, r.id, targetT, target) && (!ALPHA || r.alpha == 0u || alpha_test_passes(tris.alpha, r.alpha - 1u, b1, b2)))
       return true;
   }
   return false;
@@ -107,8 +115,8 @@ __device__ __forceinline__ void trav_init(RayTrav& r, vec3 o, vec3 d, float tmin
 
 // BVH2: both child boxes in one 64-byte node; hit leaves are intersected at once, the nearer inner
 // child is visited next and the farther one pushed. Returns true when the ray is finished.
-template <int MODE, bool COUNT>
-__device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriRecord* tris, RayTrav& r, TravStack& st,
+template <int MODE, bool COUNT, bool ALPHA = true>
+__device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriSet& tris, RayTrav& r, TravStack& st,
                                           TraversalCounters& cnt) {
   const float4* np = reinterpret_cast<const float4*>(nodes + r.cur);
   float4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
@@ -118,13 +126,13 @@ __device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriRecord
   bool h0 = c0 != BVH_EMPTY && slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), r.o, r.inv, r.tmin, r.best, tn0);
   bool h1 = c1 != BVH_EMPTY && slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), r.o, r.inv, r.tmin, r.best, tn1);
   if (h0 && (c0 & BVH_LEAF_BIT)) {
-    if (MODE == 0) leaf_closest<COUNT>(tris, c0, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
-    else if (leaf_occluded<COUNT>(tris, c0, r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
+    if (MODE == 0) leaf_closest<COUNT, ALPHA>(tris, c0, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
+    else if (leaf_occluded<COUNT, ALPHA>(tris, c0, r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
     h0 = false;
   }
   if (h1 && (c1 & BVH_LEAF_BIT)) {
-    if (MODE == 0) leaf_closest<COUNT>(tris, c1, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
-    else if (leaf_occluded<COUNT>(tris, c1, r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
+    if (MODE == 0) leaf_closest<COUNT, ALPHA>(tris, c1, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
+    else if (leaf_occluded<COUNT, ALPHA>(tris, c1, r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
     h1 = false;
   }
   if (h0 && h1) {
@@ -143,8 +151,8 @@ __device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriRecord
 }
 
 // BVH8: test all eight boxes of a node, intersect hit leaves, push hit inner children far to near.
-template <int MODE, bool COUNT>
-__device__ __forceinline__ bool step_bvh8(const Bvh8Node* nodes, const TriRecord* tris, RayTrav& r, TravStack& st,
+template <int MODE, bool COUNT, bool ALPHA = true>
+__device__ __forceinline__ bool step_bvh8(const Bvh8Node* nodes, const TriSet& tris, RayTrav& r, TravStack& st,
                                           TraversalCounters& cnt) {
   const float4* np = reinterpret_cast<const float4*>(nodes + r.cur);
   if (COUNT) cnt.nodes++;
@@ -168,8 +176,8 @@ __device__ __forceinline__ bool step_bvh8(const Bvh8Node* nodes, const TriRecord
     float tn;
     if (!slab_test(vec3(lx[s], ly[s], lz[s]), vec3(hx[s], hy[s], hz[s]), r.o, r.inv, r.tmin, r.best, tn)) continue;
     if (ch[s] & BVH_LEAF_BIT) {
-      if (MODE == 0) leaf_closest<COUNT>(tris, ch[s], r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
-      else if (leaf_occluded<COUNT>(tris, ch[s], r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
+      if (MODE == 0) leaf_closest<COUNT, ALPHA>(tris, ch[s], r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
+      else if (leaf_occluded<COUNT, ALPHA>(tris, ch[s], r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
       continue;
     }
     int p = nh++;
@@ -186,36 +194,36 @@ __device__ __forceinline__ bool step_bvh8(const Bvh8Node* nodes, const TriRecord
   return false;
 }
 
-template <int WIDTH, int MODE, bool COUNT>
-__device__ __forceinline__ bool trav_step(const Bvh2Node* nodes2, const Bvh8Node* nodes8, const TriRecord* tris, RayTrav& r,
+template <int WIDTH, int MODE, bool COUNT, bool ALPHA = true>
+__device__ __forceinline__ bool trav_step(const Bvh2Node* nodes2, const Bvh8Node* nodes8, const TriSet& tris, RayTrav& r,
                                           TravStack& st, TraversalCounters& cnt) {
-  if (WIDTH == 2) return step_bvh2<MODE, COUNT>(nodes2, tris, r, st, cnt);
-  return step_bvh8<MODE, COUNT>(nodes8, tris, r, st, cnt);
+  if (WIDTH == 2) return step_bvh2<MODE, COUNT, ALPHA>(nodes2, tris, r, st, cnt);
+  return step_bvh8<MODE, COUNT, ALPHA>(nodes8, tris, r, st, cnt);
 }
 
 // Whole traversal of one ray. MODE 0 returns whether anything was hit (closest in bestT/bestId);
 // MODE 1 returns whether a triangle other than `target` lies closer than (tmax, target).
-template <int MODE, bool COUNT>
-__device__ bool traverse_bvh2(const Bvh2Node* nodes, const TriRecord* tris, vec3 o, vec3 d, float tmin, float tmax,
+template <int MODE, bool COUNT, bool ALPHA = true>
+__device__ bool traverse_bvh2(const Bvh2Node* nodes, TriSet tris, vec3 o, vec3 d, float tmin, float tmax,
                               TravStack& st, float& bestT, uint32_t& bestId, float& b1, float& b2, uint32_t target,
                               TraversalCounters& cnt) {
   RayTrav r;
   trav_init(r, o, d, tmin, tmax, MODE == 0 ? NO_HIT_ID : target);
   st.sp = 0;
-  while (!step_bvh2<MODE, COUNT>(nodes, tris, r, st, cnt)) {}
+  while (!step_bvh2<MODE, COUNT, ALPHA>(nodes, tris, r, st, cnt)) {}
   if (MODE == 1) return r.occluded != 0;
   bestT = r.best; bestId = r.bid; b1 = r.b1; b2 = r.b2;
   return r.bid != NO_HIT_ID;
 }
 
-template <int MODE, bool COUNT>
-__device__ bool traverse_bvh8(const Bvh8Node* nodes, const TriRecord* tris, vec3 o, vec3 d, float tmin, float tmax,
+template <int MODE, bool COUNT, bool ALPHA = true>
+__device__ bool traverse_bvh8(const Bvh8Node* nodes, TriSet tris, vec3 o, vec3 d, float tmin, float tmax,
                               TravStack& st, float& bestT, uint32_t& bestId, float& b1, float& b2, uint32_t target,
                               TraversalCounters& cnt) {
   RayTrav r;
   trav_init(r, o, d, tmin, tmax, MODE == 0 ? NO_HIT_ID : target);
   st.sp = 0;
-  while (!step_bvh8<MODE, COUNT>(nodes, tris, r, st, cnt)) {}
+  while (!step_bvh8<MODE, COUNT, ALPHA>(nodes, tris, r, st, cnt)) {}
   if (MODE == 1) return r.occluded != 0;
   bestT = r.best; bestId = r.bid; b1 = r.b1; b2 = r.b2;
   return r.bid != NO_HIT_ID;

@@ -77,7 +77,7 @@ struct DevScene {
   SceneView sv;
   const Bvh2Node* nodes2;
   const Bvh8Node* nodes8;
-  const TriRecord* tris;
+  TriSet tris;
 };
 
 __device__ __forceinline__ void load_path(const PathSoA& P, uint32_t s, PathState& ps) {
@@ -244,7 +244,8 @@ __global__ void k_shadow(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth) {
     vec3 o(a.x, a.y, a.z), d(b.x, b.y, b.z);
     uint32_t target = __float_as_uint(b.w);
     // the light triangle itself first: the connection counts only if it is the closest hit (a light
-    // without geometry, NO_HIT_ID, counts when nothing is hit up to tmax)
+    // without geometry, NO_HIT_ID, counts when nothing is hit up to tmax); an alpha-tested light
+    // triangle must pass its alpha test where the ray reaches it
     float tt = a.w, b1, b2;
     bool visible = false;
     bool reaches = true;
@@ -252,7 +253,7 @@ __global__ void k_shadow(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth) {
       RayPre rp = ray_precompute(o, d);
       vec3 p1, p2, p3;
       tri_vertices(S.sv, target, p1, p2, p3);
-      reaches = intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2);
+      reaches = intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2) && !alpha_cut(S.sv, target, b1, b2);
     }
     if (reaches) {
       TravStack st;
@@ -305,7 +306,7 @@ struct WaveFetch {
   }
 };
 
-template <int WIDTH, bool COUNT>
+template <int WIDTH, bool COUNT, bool ALPHA = true>
 __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t ldsDepth, uint32_t refill) {
   extern __shared__ uint32_t lds[];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -334,7 +335,7 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
       if (wf.drained) break;
       continue;
     }
-    if (has && trav_step<WIDTH, 0, COUNT>(S.nodes2, S.nodes8, S.tris, r, st, cnt)) {
+    if (has && trav_step<WIDTH, 0, COUNT, ALPHA>(S.nodes2, S.nodes8, S.tris, r, st, cnt)) {
       Q.hit[slot] = make_float4(r.best, r.b1, r.b2, __uint_as_float(r.bid));
       has = false;
     }
@@ -349,7 +350,7 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
 
 // ATOMIC: several shadow rays of one path slot in one launch (bidirectional connections), so the
 // contributions are added atomically
-template <int WIDTH, bool COUNT, bool ATOMIC = false>
+template <int WIDTH, bool COUNT, bool ATOMIC = false, bool ALPHA = true>
 __global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, uint32_t refill) {
   extern __shared__ uint32_t lds[];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -378,7 +379,7 @@ __global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, u
         RayPre rp = ray_precompute(o, d);
         vec3 p1, p2, p3;
         tri_vertices(S.sv, target, p1, p2, p3);
-        reaches = intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2);
+        reaches = intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2) && !(ALPHA && alpha_cut(S.sv, target, b1, b2));
       }
       if (reaches) {
         trav_init(r, o, d, RAY_MIN_T, tt, target);
@@ -391,7 +392,7 @@ __global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, u
       if (wf.drained) break;
       continue;
     }
-    if (has && trav_step<WIDTH, 1, COUNT>(S.nodes2, S.nodes8, S.tris, r, st, cnt)) {
+    if (has && trav_step<WIDTH, 1, COUNT, ALPHA>(S.nodes2, S.nodes8, S.tris, r, st, cnt)) {
       if (!r.occluded) {
         uint32_t slot = Q.sSlot[cur];
         float4 c = Q.sContrib[cur];

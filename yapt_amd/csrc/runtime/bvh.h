@@ -43,7 +43,7 @@ struct alignas(16) TriRecord {
   float v0[3];
   uint32_t id;
   float v1[3];
-  uint32_t pad0;
+  uint32_t alpha;  // 0: opaque; else 1 + the triangle's alpha-test record
   float v2[3];
   uint32_t pad1;
 };

@@ -29,6 +29,12 @@ struct PreparedScene {
   std::vector<MaterialGPU> mats;
   std::vector<LightGPU> lights;
   uint32_t punctualCount = 0;  // the last entries of lights
+  std::vector<float> uv;                // 2 per vertex, empty: no UVs
+  std::vector<TextureGPU> textures;
+  std::vector<uint32_t> texels;         // RGBA8
+  std::vector<float> srgbLut;           // 256 entries, with textures
+  std::vector<AlphaTri> alphaTris;      // alpha-tested triangles
+  std::vector<uint32_t> triAlpha;       // per triangle: 0 = opaque, else 1 + its alphaTris record
   float boundsMin[3], boundsMax[3];
 };
 
@@ -57,7 +63,7 @@ inline SpectralColor unbounded_color(Rgb2Spec& r2s, const float v[3]) {
   return s;
 }
 
-inline MaterialGPU convert_material(Rgb2Spec& r2s, const ypt_material& in) {
+inline MaterialGPU convert_material(Rgb2Spec& r2s, const ypt_material& in, uint32_t textureCount) {
   ypt_material p = in;
   // validateMaterial
   p.roughness = std::max(0.001f, p.roughness);
@@ -92,6 +98,15 @@ inline MaterialGPU convert_material(Rgb2Spec& r2s, const ypt_material& in) {
   m.alphaCutoff = p.alpha_cutoff;
   m.sheenRoughness = p.sheen_roughness;
   m.subsurfaceAmount = sat(p.subsurface_amount);
+  // texture bindings: indices outside [0, texture count) are unbound (TEX_UNBOUND_INDEX); the UV scale
+  // is kept at half precision (unpackTextureTransformScale)
+  const int32_t* texIdx[4] = {&p.albedo_tex, &p.normal_tex, &p.orm_tex, &p.emissive_tex};
+  const float* texScale[4] = {p.albedo_tex_scale, p.normal_tex_scale, p.orm_tex_scale, p.emissive_tex_scale};
+  for (int k = 0; k < 4; ++k) {
+    m.tex[k] = (*texIdx[k] >= 0 && (uint32_t)*texIdx[k] < textureCount) ? *texIdx[k] : -1;
+    m.texScale[2 * k] = f16_to_f32(f32_to_f16(texScale[k][0]));
+    m.texScale[2 * k + 1] = f16_to_f32(f32_to_f16(texScale[k][1]));
+  }
   float ssr[3];
   for (int k = 0; k < 3; ++k) ssr[k] = p.subsurface_radius[k] * p.subsurface_radius_scale;
   m.specular = bounded_color(r2s, p.specular);
@@ -180,9 +195,33 @@ inline void prepare_scene(const ypt_scene_desc& d, Rgb2Spec& r2s, PreparedScene&
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; ++t)
       th.emplace_back([&, t]() {
-        for (uint32_t m = t; m < d.material_count; m += nt) out.mats[m] = convert_material(r2s, d.materials[m]);
+        for (uint32_t m = t; m < d.material_count; m += nt) out.mats[m] = convert_material(r2s, d.materials[m], d.texture_count);
       });
     for (auto& x : th) x.join();
+  }
+
+  out.uv.clear();
+  if (d.uvs) out.uv.assign(d.uvs, d.uvs + 2 * (size_t)d.vertex_count);
+  out.textures.clear();
+  out.texels.clear();
+  out.srgbLut.clear();
+  if (d.texture_count && !d.textures) throw std::invalid_argument("textures is null");
+  for (uint32_t t = 0; t < d.texture_count; ++t) {
+    const ypt_texture& tx = d.textures[t];
+    if (tx.width == 0 || tx.height == 0 || !tx.rgba8) throw std::invalid_argument("texture without texels");
+    if (out.texels.size() + (size_t)tx.width * tx.height > 0xFFFFFFFFull) throw std::invalid_argument("textures exceed 4G texels");
+    out.textures.push_back(TextureGPU{(uint32_t)out.texels.size(), tx.width, tx.height, tx.srgb ? 1u : 0u});
+    size_t n = (size_t)tx.width * tx.height;
+    size_t base = out.texels.size();
+    out.texels.resize(base + n);
+    std::memcpy(out.texels.data() + base, tx.rgba8, n * 4);
+  }
+  if (d.texture_count) {  // IEC 61966-2-1 decoding of each byte value
+    out.srgbLut.resize(256);
+    for (int b = 0; b < 256; ++b) {
+      double c = b / 255.0;
+      out.srgbLut[b] = (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+    }
   }
 
   out.triInst.assign(d.triangle_count, 0xFFFFFFFFu);
@@ -199,6 +238,27 @@ inline void prepare_scene(const ypt_scene_desc& d, Rgb2Spec& r2s, PreparedScene&
   }
   for (uint32_t t = 0; t < d.triangle_count; ++t)
     if (out.triInst[t] == 0xFFFFFFFFu) throw std::invalid_argument("triangle without instance");
+
+  // alpha testing needs a cutoff, an albedo texture and UVs (hitShadersCommon.hlsl:166-175)
+  out.alphaTris.clear();
+  out.triAlpha.assign(d.triangle_count, 0u);
+  if (!out.uv.empty()) {
+    for (uint32_t t = 0; t < d.triangle_count; ++t) {
+      const MaterialGPU& m = out.mats[out.inst[out.triInst[t]].material];
+      if (!(m.alphaCutoff > 0.f) || m.tex[TEX_ALBEDO] < 0) continue;
+      AlphaTri a{};
+      for (int k = 0; k < 3; ++k) {
+        a.uv[2 * k] = out.uv[2 * out.idx[3 * t + k]];
+        a.uv[2 * k + 1] = out.uv[2 * out.idx[3 * t + k] + 1];
+      }
+      a.scale[0] = m.texScale[2 * TEX_ALBEDO];
+      a.scale[1] = m.texScale[2 * TEX_ALBEDO + 1];
+      a.tex = (uint32_t)m.tex[TEX_ALBEDO];
+      a.cutoff = m.alphaCutoff;
+      out.alphaTris.push_back(a);
+      out.triAlpha[t] = (uint32_t)out.alphaTris.size();
+    }
+  }
 
   for (int a = 0; a < 3; ++a) { out.boundsMin[a] = 3.4e38f; out.boundsMax[a] = -3.4e38f; }
   for (size_t v = 0; v < d.vertex_count; ++v)

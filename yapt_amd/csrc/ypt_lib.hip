@@ -116,11 +116,17 @@ struct ResidentScene {
   DevBuf<Bvh2Node> dNodes2;
   DevBuf<Bvh8Node> dNodes8;
   DevBuf<TriRecord> dTris;
+  DevBuf<float> dUv, dSrgbLut;
+  DevBuf<TextureGPU> dTex;
+  DevBuf<uint32_t> dTexels;
+  DevBuf<AlphaTri> dAlpha;
+  DevBuf<uint32_t> dTriAlpha;
   uint32_t stackBound2 = BVH_MAX_DEPTH, stackBound8 = 8 * BVH_MAX_DEPTH;
   size_t device_bytes() const {
     return dPos.n * 4 + dNrm.n * 4 + dTan.n * 4 + dIdx.n * 4 + dTriInst.n * 4 + dInst.n * sizeof(InstanceGPU) +
            dMats.n * sizeof(MaterialGPU) + dLights.n * sizeof(LightGPU) + dNodes2.n * sizeof(Bvh2Node) +
-           dNodes8.n * sizeof(Bvh8Node) + dTris.n * sizeof(TriRecord);
+           dNodes8.n * sizeof(Bvh8Node) + dTris.n * sizeof(TriRecord) + dUv.n * 4 + dTex.n * sizeof(TextureGPU) + dTexels.n * 4 +
+           dSrgbLut.n * 4 + dAlpha.n * sizeof(AlphaTri) + dTriAlpha.n * 4;
   }
 };
 
@@ -234,12 +240,20 @@ struct ypt_integrator {
     s.sv.tan = r.scene.tan.empty() ? nullptr : r.dTan.p;
     s.sv.inst = r.dInst.p; s.sv.mats = r.dMats.p; s.sv.lights = r.dLights.p;
     s.sv.rgb2spec = dRgb2spec.n ? dRgb2spec.p : nullptr;
+    s.sv.uv = r.dUv.n ? r.dUv.p : nullptr;
+    s.sv.textures = r.dTex.n ? r.dTex.p : nullptr;
+    s.sv.texels = r.dTexels.n ? r.dTexels.p : nullptr;
+    s.sv.srgbLut = r.dSrgbLut.n ? r.dSrgbLut.p : nullptr;
+    s.sv.triAlpha = r.dAlpha.n ? r.dTriAlpha.p : nullptr;
+    s.sv.alpha = AlphaView{r.dAlpha.n ? r.dAlpha.p : nullptr, r.dTex.p, r.dTexels.p};
     s.sv.ntris = (uint32_t)r.scene.triInst.size();
     s.sv.ninst = (uint32_t)r.scene.inst.size();
     s.sv.spec.cie_xyz4 = dCie.p;
     s.sv.spec.d65 = dD65.p;
     s.sv.luts.base = dMsl.p;
-    s.nodes2 = r.dNodes2.p; s.nodes8 = r.dNodes8.p; s.tris = r.dTris.p;
+    s.nodes2 = r.dNodes2.p; s.nodes8 = r.dNodes8.p;
+    s.tris.recs = r.dTris.p;
+    s.tris.alpha = s.sv.alpha;
     return s;
   }
   PathSoA path_soa() const {
@@ -404,10 +418,13 @@ void launch_extend(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
   size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
   uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
   if (h->persistent) {
+    bool alpha = S.tris.alpha.tris != nullptr;
     if (h->countTraversal)
       hipLaunchKernelGGL((k_extend_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r);
+    else if (alpha)
+      hipLaunchKernelGGL((k_extend_p<WIDTH, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r);
     else
-      hipLaunchKernelGGL((k_extend_p<WIDTH, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r);
+      hipLaunchKernelGGL((k_extend_p<WIDTH, false, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r);
   } else if (h->countTraversal) {
     hipLaunchKernelGGL((k_extend<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d);
   } else {
@@ -419,10 +436,13 @@ void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
   size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
   uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
   if (h->persistent) {
+    bool alpha = S.tris.alpha.tris != nullptr;
     if (h->countTraversal)
       hipLaunchKernelGGL((k_shadow_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+    else if (alpha)
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
     else
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, false, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
   } else if (h->countTraversal) {
     hipLaunchKernelGGL((k_shadow<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d);
   } else {
@@ -511,6 +531,7 @@ std::shared_ptr<ResidentScene> make_resident(const ypt_scene_desc& d, int device
   r->device = device;
   prepare_scene(d, host_tables().r2s, r->scene);
   BvhBuilder().build(r->scene.pos.data(), r->scene.idx.data(), (uint32_t)r->scene.triInst.size(), r->bvh);
+  for (TriRecord& t : r->bvh.tris) t.alpha = r->scene.triAlpha[t.id];
   r->stackBound2 = r->bvh.stats.maxDepth + 1;
   r->stackBound8 = bvh8_stack_bound(r->bvh.nodes8);
   const PreparedScene& ps = r->scene;
@@ -526,6 +547,16 @@ std::shared_ptr<ResidentScene> make_resident(const ypt_scene_desc& d, int device
   r->dNodes2.upload(r->bvh.nodes2.data(), r->bvh.nodes2.size(), s);
   r->dNodes8.upload(r->bvh.nodes8.data(), r->bvh.nodes8.size(), s);
   r->dTris.upload(r->bvh.tris.data(), r->bvh.tris.size(), s);
+  if (!ps.uv.empty()) r->dUv.upload(ps.uv.data(), ps.uv.size(), s);
+  if (!ps.textures.empty()) {
+    r->dTex.upload(ps.textures.data(), ps.textures.size(), s);
+    r->dTexels.upload(ps.texels.data(), ps.texels.size(), s);
+    r->dSrgbLut.upload(ps.srgbLut.data(), ps.srgbLut.size(), s);
+  }
+  if (!ps.alphaTris.empty()) {
+    r->dAlpha.upload(ps.alphaTris.data(), ps.alphaTris.size(), s);
+    r->dTriAlpha.upload(ps.triAlpha.data(), ps.triAlpha.size(), s);
+  }
   HIP_CHECK(hipStreamSynchronize(s));
   return r;
 }
@@ -613,16 +644,23 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
   size_t lds = (size_t)h->ldsDepth * Bk * 4;
   uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
   hipLaunchKernelGGL(k_bdpt_cstart, dim3(blocks_for(npix, 256)), dim3(256), 0, s, fc, point, B, C, Q, feats);
+  bool alpha = S.tris.alpha.tris != nullptr;
   int q = 0;
   for (uint32_t k = 0; k < fc.maxRayDepth; ++k) {
-    hipLaunchKernelGGL((k_extend_p<WIDTH, COUNT>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r);
+    if (COUNT || alpha)
+      hipLaunchKernelGGL((k_extend_p<WIDTH, COUNT>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r);
+    else
+      hipLaunchKernelGGL((k_extend_p<WIDTH, false, false>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r);
     if (h->bdptWaves == 1)
       hipLaunchKernelGGL(k_bdpt_cshade<1>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     else if (h->bdptWaves == 2)
       hipLaunchKernelGGL(k_bdpt_cshade<2>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     else
       hipLaunchKernelGGL(k_bdpt_cshade<4>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
-    hipLaunchKernelGGL((k_shadow_p<WIDTH, COUNT, true>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r);
+    if (COUNT || alpha)
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, COUNT, true>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r);
+    else
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, true, false>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r);
     q ^= 1;
   }
   hipLaunchKernelGGL(k_bdpt_cfinish, dim3(blocks_for(npix, 256)), dim3(256), 0, s, fc, S, B, C);
@@ -931,7 +969,8 @@ int ypt_set_frame(ypt_integrator* h, const ypt_frame_desc* f) {
     FrameConstants fc;
     prepare_frame(*f, h->res->scene, (uint32_t)h->type, fc);
     // RGB -> spectrum at run time: environment colours and punctual light intensities
-    bool needTable = (f->env_intensity_scale != 0.f && !(f->flags & YPT_FLAG_WHITE_FURNACE)) || fc.punctualLightCount > 0;
+    bool needTable = (f->env_intensity_scale != 0.f && !(f->flags & YPT_FLAG_WHITE_FURNACE)) || fc.punctualLightCount > 0 ||
+                     !h->res->scene.textures.empty();
     if (needTable && h->dRgb2spec.n == 0) {
       std::vector<float> table;
       host_tables().r2s.full_table(table, std::max(1u, std::thread::hardware_concurrency()));

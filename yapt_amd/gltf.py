@@ -13,12 +13,18 @@ hierarchy are parsed here. What follows the reference:
     angles; the light looks down its node's -Z axis;
   * node transforms (translation, rotation quaternion, scale, or a matrix) composed down the hierarchy
     and baked into world space, normals by the inverse transpose, tangents by the matrix.
-Textures are parsed as references (returned in info["textures"]) but not applied: texture sampling
-is not part of this build. Triangle lists (mode 4) only.
+  * textures (scene_loader.py:39-59, 88-99, 286-309): images decoded to RGBA8, sRGB unless an image
+    holds data (a normal, occlusion or metallic-roughness map, or an image named "normal"), a uv scale
+    of (1, 1) and TEXCOORD_0; occlusion and metallic-roughness share the one ORM slot, the
+    metallic-roughness texture winning; MASK and BLEND alpha become alpha testing or per-texel
+    coverage by the smallest texel alpha (scene_loader.py:150-173).
+The reference loads external image files only; images in a bufferView or a data URI are decoded
+here as well. Triangle lists (mode 4) only.
 
     scene, info = load("model.glb")
 """
 import base64
+import io
 import json
 import math
 import os
@@ -110,8 +116,78 @@ def node_matrix(node):
     return m
 
 
-def _material(doc, mat):
-    """scene_loader.py _load_material_gltf on factors (textures are reported, not applied)"""
+def _linear_images(doc):
+    """scene_loader.py:286-300: images used as normal, occlusion or metallic-roughness maps hold data"""
+    out, texs = set(), doc.get("textures", [])
+    for mat in doc.get("materials", []):
+        defs = [mat.get("normalTexture"), mat.get("occlusionTexture"),
+                (mat.get("pbrMetallicRoughness") or {}).get("metallicRoughnessTexture")]
+        for d in defs:
+            if d is not None and d.get("index") is not None and d["index"] < len(texs):
+                src = texs[d["index"]].get("source")
+                if src is not None:
+                    out.add(src)
+    return out
+
+
+class _Images:
+    """glTF images decoded on first use into RGBA8 texture slots (scene_loader.py:39-59, 302-309)"""
+
+    def __init__(self, doc, bufs, base):
+        self.doc, self.bufs, self.base = doc, bufs, base
+        self.linear = _linear_images(doc)
+        self.slot, self.textures, self.min_alpha, self.failed = {}, [], [], []
+
+    def _bytes(self, img):
+        uri = img.get("uri")
+        if uri is not None:
+            if uri.startswith("data:"):
+                return base64.b64decode(uri.split(",", 1)[1])
+            with open(os.path.join(self.base, uri), "rb") as f:
+                return f.read()
+        bv = self.doc["bufferViews"][img["bufferView"]]
+        off = bv.get("byteOffset", 0)
+        return bytes(self.bufs[bv["buffer"]][off: off + bv["byteLength"]])
+
+    def texture(self, tex_index):
+        """texture slot of a glTF texture, -1 if its image cannot be loaded (the reference then keeps the
+        material untextured, scene_loader.py:88-99)"""
+        texs, imgs = self.doc.get("textures", []), self.doc.get("images", [])
+        if tex_index is None or tex_index >= len(texs):
+            return -1
+        src = texs[tex_index].get("source")
+        if src is None or src >= len(imgs):
+            return -1
+        if src not in self.slot:
+            img = imgs[src]
+            try:
+                from PIL import Image
+                rgba = np.asarray(Image.open(io.BytesIO(self._bytes(img))).convert("RGBA"), np.uint8)
+            except Exception as e:                    # an unreadable image leaves the slot unbound
+                self.failed.append((img.get("name") or img.get("uri") or src, str(e)))
+                self.slot[src] = -1
+                return -1
+            name = img.get("name") or ""
+            srgb = not (src in self.linear or "normal" in name.casefold())
+            self.slot[src] = len(self.textures)
+            self.textures.append((np.ascontiguousarray(rgba), srgb))
+            self.min_alpha.append(float(rgba[..., 3].min()) / 255.0)
+        return self.slot[src]
+
+    def texture_min_alpha(self, slot):
+        """smallest texel alpha, 0 when the image is unknown so that alpha testing stays on
+        (scene_loader.py:280-284)"""
+        return self.min_alpha[slot] if slot >= 0 else 0.0
+
+
+def _bind(m, slot_name, slot):
+    if slot >= 0:
+        setattr(m, slot_name + "_tex", slot)
+        getattr(m, slot_name + "_tex_scale")[:] = (1.0, 1.0)
+
+
+def _material(doc, mat, images):
+    """scene_loader.py _load_material_gltf (105-243)"""
     m = scenes.blank_material()
     ext = mat.get("extensions", {}) or {}
     strength = ext.get("KHR_materials_emissive_strength", {}).get("emissiveStrength", 1.0)
@@ -126,10 +202,22 @@ def _material(doc, mat):
         m.albedo[:] = color[:3]
         m.transparency = 0.0
         has_tex = "baseColorTexture" in pbr
+        albedo = images.texture(pbr["baseColorTexture"].get("index")) if has_tex else -1
+        _bind(m, "albedo", albedo)
         mode = mat.get("alphaMode", "OPAQUE")
         factor_alpha = color[3] if len(color) > 3 else 1.0
-        if mode == "BLEND" and not has_tex and factor_alpha < 1.0:   # texture-free branch of the BLEND rule
-            coverage, blended = factor_alpha, True
+        if mode == "MASK" and has_tex:
+            cutoff = mat.get("alphaCutoff", 0.5)
+            if images.texture_min_alpha(albedo) < cutoff:
+                m.alpha_cutoff = cutoff
+        elif mode == "BLEND":
+            tex_min_alpha = images.texture_min_alpha(albedo) if has_tex else 1.0
+            if tex_min_alpha < 0.02:                  # cutouts: alpha testing at 1/2
+                m.alpha_cutoff = 0.5
+            elif tex_min_alpha < 1.0 or factor_alpha < 1.0:
+                coverage, blended = factor_alpha, True
+                if tex_min_alpha < 1.0:
+                    m.material_mask |= MASK_ALPHA_BLEND
         m.metalness = pbr.get("metallicFactor", 1.0)
         m.roughness = pbr.get("roughnessFactor", 1.0)
     if "KHR_materials_transmission" in ext:
@@ -173,15 +261,17 @@ def _material(doc, mat):
         m.transparency = 1.0 - coverage * (1.0 - transmission)
         if transmission == 0.0:
             m.material_mask |= MASK_TWO_SIDED
-    textures = {}
-    for key, slot in (("normalTexture", "normal"), ("emissiveTexture", "emissive"), ("occlusionTexture", "orm")):
+    slots = {}
+    for key, slot in (("emissiveTexture", "emissive"), ("normalTexture", "normal"), ("occlusionTexture", "orm")):
         if key in mat:
-            textures[slot] = mat[key]["index"]
-    if pbr is not None:
-        for key, slot in (("baseColorTexture", "albedo"), ("metallicRoughnessTexture", "orm")):
-            if key in pbr:
-                textures[slot] = pbr[key]["index"]
-    return m, textures
+            slots[slot] = images.texture(mat[key].get("index"))
+    if pbr is not None and "metallicRoughnessTexture" in pbr:   # one ORM slot: metallic-roughness wins
+        slots["orm"] = images.texture(pbr["metallicRoughnessTexture"].get("index"))
+    for slot, t in slots.items():
+        _bind(m, slot, t)
+    if pbr is not None and "baseColorTexture" in pbr:
+        slots["albedo"] = m.albedo_tex
+    return m, slots
 
 
 def _area_normals(P, I):
@@ -194,14 +284,16 @@ def _area_normals(P, I):
 
 
 def load(path, scene_index=None):
-    """Returns (SceneArrays, info) with info = {"lights": [...], "materials": [names], "textures": {material: {...}},
+    """Returns (SceneArrays, info) with info = {"lights": [...], "materials": [names], "textures": {material: {slot:
+    texture index, -1 where the image could not be loaded}}, "images_failed": [(image, error)],
     "camera": (eye, target, yfov) of the first camera node or None, "triangles": n}"""
     doc, binchunk = _read_container(path)
     bufs = _buffers(doc, os.path.dirname(os.path.abspath(path)), binchunk)
     b = scenes.MeshBuilder()
-    mats, tex_info = [], {}
+    images = _Images(doc, bufs, os.path.dirname(os.path.abspath(path)))
+    mats, tex_info, uvs = [], {}, []
     for k, mat in enumerate(doc.get("materials", [])):
-        m, t = _material(doc, mat)
+        m, t = _material(doc, mat, images)
         mats.append(b.material(m))
         if t:
             tex_info[mat.get("name", "material_%d" % k)] = t
@@ -238,6 +330,8 @@ def load(path, scene_index=None):
                     tangents.append(np.concatenate([T, T4[:, 3:4]], 1))
                 else:
                     tangents.append(scenes.arbitrary_tangents(N.astype(np.float32)))
+                uvs.append(_accessor(doc, bufs, attr["TEXCOORD_0"])[:, :2] if "TEXCOORD_0" in attr
+                           else np.zeros((len(P), 2)))
                 if "material" in prim:
                     mi = mats[prim["material"]]
                 else:
@@ -278,7 +372,9 @@ def load(path, scene_index=None):
         raise ValueError("no triangle geometry in %s" % path)
     nrm = np.concatenate(b.nrm)
     arrays = _lib.SceneArrays(np.concatenate(b.pos), nrm, np.concatenate(b.idx), b.inst_first, b.inst_count, b.inst_mat,
-                              b.materials, tangents=np.concatenate(tangents).astype(np.float32), lights=lights)
+                              b.materials, tangents=np.concatenate(tangents).astype(np.float32), lights=lights,
+                              uvs=np.concatenate(uvs).astype(np.float32) if images.textures else None,
+                              textures=images.textures)
     info = {"lights": lights, "materials": [m.get("name") for m in doc.get("materials", [])], "textures": tex_info,
-            "camera": camera, "triangles": arrays.triangle_count}
+            "images_failed": images.failed, "camera": camera, "triangles": arrays.triangle_count}
     return arrays, info

@@ -48,6 +48,9 @@ def blank_material():
     m.subsurface_radius[:] = (1.0, 0.2, 0.1)
     m.subsurface_amount = 0.0
     m.subsurface_radius_scale = 1.0
+    m.albedo_tex = m.normal_tex = m.orm_tex = m.emissive_tex = -1   # TEX_UNBOUND_INDEX
+    for sc in (m.albedo_tex_scale, m.normal_tex_scale, m.orm_tex_scale, m.emissive_tex_scale):
+        sc[:] = (1.0, 1.0)
     return m
 
 
@@ -145,11 +148,11 @@ class MeshBuilder:
         p = np.stack([c, c + u, c + u + v, c + v])
         self.add(p, np.tile(n, (4, 1)), [[0, 1, 2], [0, 2, 3]], material)
 
-    def arrays(self, lights=()):
+    def arrays(self, lights=(), uvs=None, textures=()):
         nrm = np.concatenate(self.nrm)
         return _lib.SceneArrays(np.concatenate(self.pos), nrm, np.concatenate(self.idx),
                                 self.inst_first, self.inst_count, self.inst_mat, self.materials,
-                                tangents=arbitrary_tangents(nrm), lights=lights)
+                                tangents=arbitrary_tangents(nrm), lights=lights, uvs=uvs, textures=textures)
 
 
 def box_template():
