@@ -37,6 +37,11 @@ extern "C" {
 #define YPT_MATERIAL_TWO_SIDED (1u << 0)
 #define YPT_MATERIAL_DISPERSION (1u << 1)
 
+/* material_mask bits (MaterialMask_*); ALPHA_BLEND turns the albedo texel alpha into coverage */
+#define YPT_MATERIAL_TWO_SIDED (1u << 0)
+#define YPT_MATERIAL_DISPERSION (1u << 1)
+#define YPT_MATERIAL_ALPHA_BLEND (1u << 2)
+
 /* Material parameters as the reference's MaterialParameters (Source/Renderer/inc/Renderer/Shared/
  * MaterialInternal.h:41-125); the library applies the validation of MaterialInternal.cpp:27-43 and
  * the packing clamps of BindlessMaterialManager.cpp:70-100. Colours are linear RGB. */
@@ -51,7 +56,7 @@ typedef struct ypt_material {
   float roughness;
   float anisotropy;
   float anisotropy_rotation;
-  uint32_t material_mask;
+  uint32_t material_mask;  /* YPT_MATERIAL_TWO_SIDED | YPT_MATERIAL_DISPERSION | YPT_MATERIAL_ALPHA_BLEND */
   float specular_amount;
   float clear_coat_amount;
   float clear_coat_ior;
@@ -61,7 +66,7 @@ typedef struct ypt_material {
   float sheen_amount;
   float thin_film_thickness_nm;
   float emission_focus;
-  float alpha_cutoff;
+  float alpha_cutoff;      /* > 0: hits where the albedo texture's alpha is below it are ignored */
   float cauchy_coeffs[2];
   float subsurface_radius[3];
   float subsurface_amount;

@@ -19,6 +19,7 @@ YPT_FLAG_DISABLE_JITTER = 1 << 1
 YPT_FLAG_WHITE_FURNACE = 1 << 8
 YPT_MATERIAL_TWO_SIDED = 1 << 0
 YPT_MATERIAL_DISPERSION = 1 << 1
+YPT_MATERIAL_ALPHA_BLEND = 1 << 2
 
 F3 = ctypes.c_float * 3
 F16 = ctypes.c_float * 16

@@ -38,7 +38,7 @@ GLTF_LUMENS_PER_WATT = 683.0
 _COMPONENT = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
 _NCOMP = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT2": 4, "MAT3": 9, "MAT4": 16}
 _LIGHT_TYPES = {"point": _lib.Light.POINT, "spot": _lib.Light.SPOT, "directional": _lib.Light.DIRECTIONAL}
-MASK_TWO_SIDED, MASK_ALPHA_BLEND = 1 << 0, 1 << 2
+MASK_TWO_SIDED, MASK_ALPHA_BLEND = _lib.YPT_MATERIAL_TWO_SIDED, _lib.YPT_MATERIAL_ALPHA_BLEND
 
 
 def _read_container(path):
