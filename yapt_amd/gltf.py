@@ -12,7 +12,8 @@ hierarchy are parsed here. What follows the reference:
   * KHR_lights_punctual (scene_loader.py:362-390): colour, intensity / 683 (lm/W at 555 nm), spot cone
     angles; the light looks down its node's -Z axis;
   * node transforms (translation, rotation quaternion, scale, or a matrix) composed down the hierarchy
-    and baked into world space, normals by the inverse transpose, tangents by the matrix.
+    and baked into world space, normals by the inverse transpose, tangents by the matrix (their
+    handedness flipped, and the winding reversed, under a mirroring transform).
   * textures (scene_loader.py:39-59, 88-99, 286-309): images decoded to RGBA8, sRGB unless an image
     holds data (a normal, occlusion or metallic-roughness map, or an image named "normal"), a uv scale
     of (1, 1) and TEXCOORD_0; occlusion and metallic-roughness share the one ORM slot, the
@@ -327,7 +328,8 @@ def load(path, scene_index=None):
                     T4 = _accessor(doc, bufs, attr["TANGENT"])
                     T = T4[:, :3] @ world[:3, :3].T
                     T = T / np.maximum(np.linalg.norm(T, axis=1, keepdims=True), 1e-30)
-                    tangents.append(np.concatenate([T, T4[:, 3:4]], 1))
+                    flip = -1.0 if np.linalg.det(world[:3, :3]) < 0 else 1.0   # mirrored: bitangent flips
+                    tangents.append(np.concatenate([T, T4[:, 3:4] * flip], 1))
                 else:
                     tangents.append(scenes.arbitrary_tangents(N.astype(np.float32)))
                 uvs.append(_accessor(doc, bufs, attr["TEXCOORD_0"])[:, :2] if "TEXCOORD_0" in attr
@@ -378,3 +380,27 @@ def load(path, scene_index=None):
     info = {"lights": lights, "materials": [m.get("name") for m in doc.get("materials", [])], "textures": tex_info,
             "images_failed": images.failed, "camera": camera, "triangles": arrays.triangle_count}
     return arrays, info
+
+
+def mesh_primitives(path):
+    """Object-space triangle primitives of every mesh of a glTF file, in file order: a list per mesh of
+    dicts with positions, normals, tangents (x, y, z, handedness), uvs (TEXCOORD_0) or None where the
+    file has no such attribute, indices and the glTF material index (or None). Node transforms are
+    not applied (the role of MeshLoader.create_and_upload_from_gltf, mesh_utils.py:87-97)."""
+    doc, binchunk = _read_container(path)
+    bufs = _buffers(doc, os.path.dirname(os.path.abspath(path)), binchunk)
+    out = []
+    for mesh in doc.get("meshes", []):
+        prims = []
+        for prim in mesh.get("primitives", []):
+            if prim.get("mode", 4) != 4:
+                continue
+            attr = prim["attributes"]
+            P = _accessor(doc, bufs, attr["POSITION"])[:, :3].astype(np.float32)
+            I = (_accessor(doc, bufs, prim["indices"]).astype(np.uint32).reshape(-1) if "indices" in prim
+                 else np.arange(len(P), dtype=np.uint32))
+            get = lambda k, n: _accessor(doc, bufs, attr[k])[:, :n].astype(np.float32) if k in attr else None
+            prims.append({"positions": P, "normals": get("NORMAL", 3), "tangents": get("TANGENT", 4),
+                          "uvs": get("TEXCOORD_0", 2), "indices": I, "material": prim.get("material")})
+        out.append(prims)
+    return out
