@@ -20,7 +20,8 @@ hierarchy are parsed here. What follows the reference:
     metallic-roughness texture winning; MASK and BLEND alpha become alpha testing or per-texel
     coverage by the smallest texel alpha (scene_loader.py:150-173).
 The reference loads external image files only; images in a bufferView or a data URI are decoded
-here as well. Triangle lists (mode 4) only.
+here as well, and KTX images (KHR_texture_basisu sources included) go through ktx.py, keeping the
+encoding that their format states. Triangle lists (mode 4) only.
 
     scene, info = load("model.glb")
 """
@@ -33,7 +34,7 @@ import struct
 
 import numpy as np
 
-from . import _lib, scenes
+from . import _lib, ktx, scenes
 
 GLTF_LUMENS_PER_WATT = 683.0
 _COMPONENT = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
@@ -156,20 +157,27 @@ class _Images:
         texs, imgs = self.doc.get("textures", []), self.doc.get("images", [])
         if tex_index is None or tex_index >= len(texs):
             return -1
-        src = texs[tex_index].get("source")
+        tex = texs[tex_index]
+        src = tex.get("source")
+        if src is None:                               # KHR_texture_basisu names its KTX2 image here
+            src = (tex.get("extensions") or {}).get("KHR_texture_basisu", {}).get("source")
         if src is None or src >= len(imgs):
             return -1
         if src not in self.slot:
             img = imgs[src]
+            name = img.get("name") or ""
+            srgb = not (src in self.linear or "normal" in name.casefold())
             try:
-                from PIL import Image
-                rgba = np.asarray(Image.open(io.BytesIO(self._bytes(img))).convert("RGBA"), np.uint8)
+                raw = self._bytes(img)
+                if ktx.is_ktx(raw):                   # a KTX file states its own encoding
+                    rgba, srgb = ktx.decode(raw)
+                else:
+                    from PIL import Image
+                    rgba = np.asarray(Image.open(io.BytesIO(raw)).convert("RGBA"), np.uint8)
             except Exception as e:                    # an unreadable image leaves the slot unbound
                 self.failed.append((img.get("name") or img.get("uri") or src, str(e)))
                 self.slot[src] = -1
                 return -1
-            name = img.get("name") or ""
-            srgb = not (src in self.linear or "normal" in name.casefold())
             self.slot[src] = len(self.textures)
             self.textures.append((np.ascontiguousarray(rgba), srgb))
             self.min_alpha.append(float(rgba[..., 3].min()) / 255.0)

@@ -1056,9 +1056,16 @@ class Resources:
         return out
 
     def load_texture_2D(self, tex_path, fmt="R8G8B8A8_SRGB", verbose=False):
-        from PIL import Image
-        rgba = np.asarray(Image.open(tex_path).convert("RGBA"), np.uint8)
-        tex = Texture(os.path.basename(tex_path), rgba, srgb=str(fmt).upper().endswith("SRGB"))
+        """an image file as a texture in `fmt` (R8G8B8A8_SRGB or R8G8B8A8_UNORM); a KTX file keeps the
+        format it was written in"""
+        from . import ktx
+        if str(tex_path).lower().endswith((".ktx", ".ktx2")):
+            rgba, srgb = ktx.read(tex_path)
+        else:
+            from PIL import Image
+            rgba = np.asarray(Image.open(tex_path).convert("RGBA"), np.uint8)
+            srgb = str(fmt).upper().endswith("SRGB")
+        tex = Texture(os.path.basename(str(tex_path)), rgba, srgb=srgb)
         self._min_alpha[str(tex_path)] = tex.min_alpha()
         return tex
 
