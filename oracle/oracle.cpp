@@ -140,14 +140,17 @@ struct Geometry {
     }
   }
 
-  // minimum (t, id) over triangles hit in (tmin, tmax]
-  bool closest(vec3 o, vec3 d, float tmin, float tmax, Hit& h, uint64_t* work) const {
+  // minimum (t, id) over triangles hit in (tmin, tmax]; with `range` set, over the triangles
+  // [lo, lo + range) only (a subsurface probe's instance)
+  bool closest(vec3 o, vec3 d, float tmin, float tmax, Hit& h, uint64_t* work, uint32_t lo = 0u,
+               uint32_t range = 0xFFFFFFFFu) const {
     RayPre rp = ray_precompute(o, d);
     float best = tmax;
     uint32_t bid = NO_HIT_ID;
     float bb1 = 0.f, bb2 = 0.f;
     const AlphaView av{S->alphaTris.data(), S->textures.data(), S->texels.data()};
     auto test = [&](uint32_t t) {
+      if (t - lo >= range) return;
       vec3 a, b, c;
       tri(t, a, b, c);
       float tt, b1, b2;
@@ -287,6 +290,17 @@ void make_view(Ctx& c) {
   c.sv.luts.base = T.msl.data();
 }
 
+// subsurface probes: the closest hit among one instance's triangles
+struct CpuProbe {
+  static constexpr bool enabled = true;
+  const Geometry* geo;
+  uint64_t* work;
+  bool closest_in_range(vec3 o, vec3 d, float tmin, float tmax, uint32_t first, uint32_t count, Hit& h) {
+    if (work) work[2]++;
+    return geo->closest(o, d, tmin, tmax, h, work, first, count);
+  }
+};
+
 // rayGenPrimaryRays with rayHitDefault / rayMissEnvironment / inline shadow query
 vec3 trace_pixel(const Ctx& c, const uint32_t* point, uint32_t px, uint32_t py, float& lambdaOut, uint64_t* work,
                  FirstHitFeatures* fh) {
@@ -297,15 +311,18 @@ vec3 trace_pixel(const Ctx& c, const uint32_t* point, uint32_t px, uint32_t py, 
     Hit h;
     c.geo.closest(ps.origin, ps.dir, RAY_MIN_T, __builtin_inff(), h, work);
     if (work) work[2]++;
-    ShadowRay sr;
+    ShadowRay sr, sr2;
     sr.valid = false;
+    sr2.valid = false;
+    CpuProbe probe{&c.geo, work};
     if (h.tri == NO_HIT_ID) shade_miss(c.sv, c.fc, ps);
-    else shade_hit(c.sv, c.fc, point, c.nee, ps, h, sr, i == 0 ? fh : nullptr);
-    if (sr.valid) {
+    else shade_hit(c.sv, c.fc, point, c.nee, ps, h, sr, i == 0 ? fh : nullptr, &probe, &sr2);
+    for (const ShadowRay* s : {&sr, &sr2}) {  // the vertex's light connection, then the subsurface exit's
+      if (!s->valid) continue;
       Hit sh;
-      c.geo.closest(sr.origin, sr.dir, RAY_MIN_T, sr.tmax, sh, work);
+      c.geo.closest(s->origin, s->dir, RAY_MIN_T, s->tmax, sh, work);
       if (work) work[3]++;
-      if (sh.tri == sr.targetTri) ps.totalLight = ps.totalLight + sr.contrib;
+      if (sh.tri == s->targetTri) ps.totalLight = ps.totalLight + s->contrib;
     }
     after_segment(point, ps);
   }
@@ -615,6 +632,8 @@ int oracle_render(const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integr
 // 16 ss_dir(cos, r)  17 ss_avg(r)  18 ssms_dir(eta, cos, r)  19 sheen_dir(cos, r)  20 tr_dir(eta, cos, r)
 // 21 emission_profile(cos, n)  22 f32_to_f16(x) (uint bits out)  23 f16_to_f32(h bits)
 // 24 accumulate_ri(sum, nom, denom, connectable)  25 pdf_ratio(nom, denom)
+// 26 burley_profile_area(r, d)  27 burley_sample_radius(u, d)  28 burley_d(radius, albedo)
+// 29 pdf_subsurface_exit(albedo (grey), radius (grey), to.xyz on the plane y = 0 from the origin, normal +y)
 extern "C" int oracle_eval(int fn, const float* in, uint32_t n, float* out) {
   MSLuts L;
   L.base = tables().msl.data();
@@ -649,6 +668,11 @@ extern "C" int oracle_eval(int fn, const float* in, uint32_t n, float* out) {
       case 23: o[0] = f16_to_f32(as_uint(a[0])); break;
       case 24: o[0] = accumulate_ri(a[0], a[1], a[2], a[3] != 0.f); break;
       case 25: o[0] = pdf_ratio(a[0], a[1]); break;
+      case 26: o[0] = burley_profile_area(a[0], a[1]); break;
+      case 27: o[0] = burley_sample_radius(a[0], a[1]); break;
+      case 28: o[0] = burley_d(a[0], a[1]); break;
+      case 29: o[0] = pdf_subsurface_exit(Spectrum(a[0]), Spectrum(a[1]), vec3(0.f), vec3(0.f, 1.f, 0.f), vec3(a[2], a[3], a[4]),
+                                          vec3(a[5], a[6], a[7])); break;
       default: g_err = "unknown function"; return YPT_E_INVALID;
     }
   }

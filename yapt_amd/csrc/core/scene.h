@@ -117,6 +117,13 @@ struct SceneView {
   MSLuts luts;
 };
 
+static constexpr float RAY_MIN_T = 5e-4f;  // DEFAULT_RAY_MIN_T (raytraceCommon.hlsl:22-23)
+
+struct Hit {
+  float t, b1, b2;  // b1, b2: weights of the second and third vertex (DXR barycentrics)
+  uint32_t tri;     // NO_HIT on a miss
+};
+
 YPT_HD vec3 load3(const float* p, uint32_t i) { return vec3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 
 YPT_HD void tri_vertices(const SceneView& S, uint32_t tri, vec3& p1, vec3& p2, vec3& p3) {

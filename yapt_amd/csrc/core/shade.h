@@ -12,10 +12,10 @@
 #pragma once
 #include "scene.h"
 #include "sampler.h"
+#include "subsurface.h"
 
 namespace ypt {
 
-static constexpr float RAY_MIN_T = 5e-4f;  // DEFAULT_RAY_MIN_T
 static constexpr uint32_t RAY_ALIVE = 0, RAY_TERMINATED = 1, RAY_CANCELLED = 2;
 static constexpr uint32_t RS_SECONDARY_TERMINATED = 1u << 0;
 static constexpr uint32_t RS_REGULARIZE = 1u << 1;
@@ -34,11 +34,6 @@ struct PathState {
   float volIor[MAX_VOLUMES];
   Spectrum volAbs[MAX_VOLUMES];
   uint32_t volMat[MAX_VOLUMES];
-};
-
-struct Hit {
-  float t, b1, b2;  // b1, b2: weights of the second and third vertex (DXR barycentrics)
-  uint32_t tri;     // NO_HIT on a miss
 };
 
 struct ShadowRay {
@@ -376,8 +371,22 @@ YPT_HD void first_hit_features(const FrameConstants& fc, const MaterialGPU& m, c
 // ---------------------------------------------------------------- closest hit
 // rayHitDefault (rayHitDefault.hlsl:464-585) with evaluateSurfaceAndGenerateNextSampleDirection (:316-435)
 // and connectToExplicitLight (:204-257). `nee` selects BPT_WITH_NEE.
+// Subsurface policy of shade_hit: NoSubsurface ends a path that takes the subsurface layer (the
+// form for scenes without subsurface materials); a policy with enabled = true brings the probe
+// tracer (closest_in_range) and continues the path from the exit point.
+struct NoSubsurface {
+  static constexpr bool enabled = false;
+  YPT_HD bool closest_in_range(vec3, vec3, float, float, uint32_t, uint32_t, Hit&) { return false; }
+};
+
+YPT_HD void continue_from_subsurface_exit(const SceneView& S, const FrameConstants& fc, bool nee, bool hasLights,
+                                          PathState& ps, const MaterialGPU& m, const SubsurfaceExit& ex,
+                                          RandomSampler& rand, const Lambdas& L, ShadowRay& sr);
+
+// sr: the shadow ray of the vertex's light connection; sr2: that of the subsurface exit point
+template <class SSS = NoSubsurface>
 YPT_HD void shade_hit(const SceneView& S, const FrameConstants& fc, const uint32_t* point, bool nee, PathState& ps,
-                      const Hit& h, ShadowRay& sr, FirstHitFeatures* fh) {
+                      const Hit& h, ShadowRay& sr, FirstHitFeatures* fh, SSS* sss = nullptr, ShadowRay* sr2 = nullptr) {
   sr.valid = false;
   Lambdas L = spectral_lambdas(ps.lambda);
   uint32_t instId = S.triInst[h.tri];
@@ -485,6 +494,34 @@ YPT_HD void shade_hit(const SceneView& S, const FrameConstants& fc, const uint32
   }
 
   int layer = select_layer(rb[0], ctx.prob);
+  if constexpr (SSS::enabled) {
+    if (layer == LAYER_SUBSURFACE) {  // evaluateSurfaceAndGenerateNextSampleDirection (rayHitDefault.hlsl:339-355)
+      SurfacePoint entry;
+      entry.pos = hitPos;
+      entry.n = orient_to_same_side(-rayDir, normalWS);
+      entry.inst = instId;
+      entry.tri = h.tri;
+      entry.b1 = h.b1;
+      entry.b2 = h.b2;
+      entry.frontFace = frontFace;
+      Spectrum radius = spectrum_unbounded_no_illum(m.subsurfaceRadius, L);
+      SubsurfaceExit ex;
+      if (!sample_subsurface_exit(S, sd.albedo, radius, entry, rb[1], rb[2], rb[3], *sss, ex)) {
+        ps.pdfThisRay = 0.f;
+        ps.flags &= ~RS_DELTA;
+        ps.throughput.set(0.f);
+        ps.dim = rand.dim;
+        ps.rayState = RAY_CANCELLED;
+        return;
+      }
+      // evaluateSubsurfaceEntry: the entry weight over the density of the subsurface edge
+      Spectrum entryWeight = ex.weight * (subsurface_entry_factor(ctx, S.luts) * ex.pdfPosition);
+      float pdfForward = ctx.prob[LAYER_SUBSURFACE] * ex.pdfPosition;
+      ps.throughput = ps.throughput * entryWeight * safe_div(1.f, pdfForward);
+      continue_from_subsurface_exit(S, fc, nee, hasLights, ps, m, ex, rand, L, *sr2);
+      return;
+    }
+  }
   bool isDiffuse = true, isDelta = false;
   vec3 wi(0.f);
   if (layer != LAYER_SUBSURFACE) wi = sample_layer_direction(ctx, S.luts, layer, rb[1], rb[2], rb[3], isDiffuse, isDelta);
@@ -524,6 +561,85 @@ YPT_HD void shade_hit(const SceneView& S, const FrameConstants& fc, const uint32
   ps.throughput = ps.throughput * ev.weight;
   ps.dim = rand.dim;
   if (is_zero(nextDir) || ps.pdfThisRay == 0.0f) {
+    ps.rayState = RAY_CANCELLED;
+  } else {
+    ps.dir = nextDir;
+    ps.origin = nextOrigin;
+  }
+}
+
+// continueFromSubsurfaceExit (rayHitDefault.hlsl:259-314): the surface at the exit point, set up for
+// the outside (the outward normal stands in for the incoming direction), a light connection through
+// the exit lobe only, and the next direction from the exit lobe. The entry weight is already in
+// the throughput.
+YPT_HD void continue_from_subsurface_exit(const SceneView& S, const FrameConstants& fc, bool nee, bool hasLights,
+                                          PathState& ps, const MaterialGPU& m, const SubsurfaceExit& ex,
+                                          RandomSampler& rand, const Lambdas& L, ShadowRay& sr) {
+  sr.valid = false;
+  uint32_t tri = ex.p.tri;
+  float b1 = ex.p.b1, b2 = ex.p.b2, b0 = 1.f - b1 - b2;
+  uint32_t i0 = S.idx[3 * tri], i1 = S.idx[3 * tri + 1], i2 = S.idx[3 * tri + 2];
+  vec3 p1 = load3(S.pos, i0), p2 = load3(S.pos, i1), p3 = load3(S.pos, i2);
+  vec3 geometryNormal = normalize(cross(p2 - p1, p3 - p1));
+  vec3 normal = normalize(b0 * load3(S.nrm, i0) + b1 * load3(S.nrm, i1) + b2 * load3(S.nrm, i2));
+  vec3 tangent(1.f, 0.f, 0.f);
+  if (S.tan) {
+    const float* t0 = S.tan + 4 * i0;
+    const float* t1 = S.tan + 4 * i1;
+    const float* t2 = S.tan + 4 * i2;
+    vec3 a(t0[0] * t0[3], t0[1] * t0[3], t0[2] * t0[3]);
+    vec3 b(t1[0] * t1[3], t1[1] * t1[3], t1[2] * t1[3]);
+    vec3 c(t2[0] * t2[3], t2[1] * t2[3], t2[2] * t2[3]);
+    tangent = normalize(b0 * a + b1 * b + b2 * c);
+  }
+  vec3 wo = normalize(ex.p.n);
+  SurfaceTexels tx;
+  sample_surface_texels(S, m, tri, b1, b2, tx);
+  SurfaceDef sd = make_surface(m, geometryNormal, normal, tangent, wo, true, (ps.flags & RS_REGULARIZE) != 0, L, S.spec, &tx);
+  SurfaceContext ctx;
+  make_surface_context(ctx, sd, current_ior(ps), previous_ior(ps), wo, ex.p.frontFace, L, false);
+  if (hasLights) {
+    float rl[4];
+    rand.r4(rl);
+    ExplicitLightSample ls;
+    sample_explicit_light(S, fc, nee, ex.p.pos, rl, L, ls);
+    if (!ls.emission.all_equal(0.f) && ls.pdf > 0.f) {
+      Spectrum w;
+      float bsdfPdf;
+      evaluate_subsurface_exit_lobe(ctx, wo, normalize(ls.toLight), EVAL_NONE, w, bsdfPdf);
+      if (!w.all_equal(0.f)) {
+        sr.origin = spawn_ray_origin(ex.p.pos, ex.p.n, ls.toLight);
+        sr.dir = ls.toLight;
+        sr.tmax = ls.punctual ? ls.distance - RAY_MIN_T : ls.distance + RAY_MIN_T;
+        sr.targetTri = ls.tri;
+        float wMIS = ls.punctual ? 1.f : weight_mis(ls.pdf, bsdfPdf);
+        w = (w / ls.pdf) * wMIS;
+        sr.contrib = ps.throughput * w * ls.emission;
+        sr.valid = true;
+      }
+    }
+  }
+  float re[4];
+  rand.r4(re);
+  vec3 wi = sample_subsurface_exit_lobe(ctx, wo, re[2], re[3]);
+  Spectrum weight(0.f);
+  float pdf = 0.f;
+  vec3 nextDir(0.f), nextOrigin = ex.p.pos;
+  if (!is_zero(wi)) {
+    Spectrum w;
+    evaluate_subsurface_exit_lobe(ctx, wo, wi, EVAL_NONE, w, pdf);
+    if (pdf > 0.f) {
+      weight = w / pdf;
+      nextDir = normalize(wi);
+      nextOrigin = spawn_ray_origin(ex.p.pos, ex.p.n, nextDir);
+    }
+  }
+  ps.flags |= RS_REGULARIZE;
+  ps.flags &= ~RS_DELTA;
+  ps.pdfThisRay = pdf;
+  ps.throughput = ps.throughput * weight;
+  ps.dim = rand.dim;
+  if (is_zero(nextDir) || pdf == 0.0f) {
     ps.rayState = RAY_CANCELLED;
   } else {
     ps.dir = nextDir;

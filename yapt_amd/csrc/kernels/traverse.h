@@ -52,13 +52,15 @@ __device__ __forceinline__ TriRecord load_tri(const TriRecord* tris, uint32_t k)
   return r;
 }
 
-template <bool COUNT, bool ALPHA = true>
+template <bool COUNT, bool ALPHA = true, bool FILTER = false>
 __device__ __forceinline__ void leaf_closest(const TriSet& tris, uint32_t ref, const RayPre& rp, float tmin, float& best,
-                                             uint32_t& bestId, float& bb1, float& bb2, TraversalCounters& cnt) {
+                                             uint32_t& bestId, float& bb1, float& bb2, TraversalCounters& cnt,
+                                             uint32_t lo = 0u, uint32_t range = 0u) {
   uint32_t first = ref & 0x07FFFFFFu, count = ((ref >> 27) & 15u) + 1u;
   for (uint32_t k = first; k < first + count; ++k) {
     TriRecord r = load_tri(tris.recs, k);
     if (COUNT) cnt.tris++;
+    if (FILTER && r.id - lo >= range) continue;  // only triangles [lo, lo + range)
     float t, b1, b2;
     if (intersect_triangle(rp, vec3(r.v0[0], r.v0[1], r.v0[2]), vec3(r.v1[0], r.v1[1], r.v1[2]), vec3(r.v2[0], r.v2[1], r.v2[2]),
                            tmin, best, t, b1, b2) &&
@@ -98,6 +100,7 @@ struct RayTrav {
   uint32_t bid;       // closest triangle so far (MODE 0) or the target triangle (MODE 1)
   uint32_t cur;       // node to visit next
   uint32_t occluded;  // MODE 1 result
+  uint32_t lo, range; // FILTER: the closest hit among triangles [lo, lo + range) only
 };
 
 __device__ __forceinline__ void trav_init(RayTrav& r, vec3 o, vec3 d, float tmin, float tmax, uint32_t target) {
@@ -115,7 +118,7 @@ __device__ __forceinline__ void trav_init(RayTrav& r, vec3 o, vec3 d, float tmin
 
 // BVH2: both child boxes in one 64-byte node; hit leaves are intersected at once, the nearer inner
 // child is visited next and the farther one pushed. Returns true when the ray is finished.
-template <int MODE, bool COUNT, bool ALPHA = true>
+template <int MODE, bool COUNT, bool ALPHA = true, bool FILTER = false>
 __device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriSet& tris, RayTrav& r, TravStack& st,
                                           TraversalCounters& cnt) {
   const float4* np = reinterpret_cast<const float4*>(nodes + r.cur);
@@ -126,12 +129,12 @@ __device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriSet& t
   bool h0 = c0 != BVH_EMPTY && slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), r.o, r.inv, r.tmin, r.best, tn0);
   bool h1 = c1 != BVH_EMPTY && slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), r.o, r.inv, r.tmin, r.best, tn1);
   if (h0 && (c0 & BVH_LEAF_BIT)) {
-    if (MODE == 0) leaf_closest<COUNT, ALPHA>(tris, c0, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
+    if (MODE == 0) leaf_closest<COUNT, ALPHA, FILTER>(tris, c0, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt, r.lo, r.range);
     else if (leaf_occluded<COUNT, ALPHA>(tris, c0, r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
     h0 = false;
   }
   if (h1 && (c1 & BVH_LEAF_BIT)) {
-    if (MODE == 0) leaf_closest<COUNT, ALPHA>(tris, c1, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
+    if (MODE == 0) leaf_closest<COUNT, ALPHA, FILTER>(tris, c1, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt, r.lo, r.range);
     else if (leaf_occluded<COUNT, ALPHA>(tris, c1, r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
     h1 = false;
   }
@@ -151,7 +154,7 @@ __device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriSet& t
 }
 
 // BVH8: test all eight boxes of a node, intersect hit leaves, push hit inner children far to near.
-template <int MODE, bool COUNT, bool ALPHA = true>
+template <int MODE, bool COUNT, bool ALPHA = true, bool FILTER = false>
 __device__ __forceinline__ bool step_bvh8(const Bvh8Node* nodes, const TriSet& tris, RayTrav& r, TravStack& st,
                                           TraversalCounters& cnt) {
   const float4* np = reinterpret_cast<const float4*>(nodes + r.cur);
@@ -176,7 +179,7 @@ __device__ __forceinline__ bool step_bvh8(const Bvh8Node* nodes, const TriSet& t
     float tn;
     if (!slab_test(vec3(lx[s], ly[s], lz[s]), vec3(hx[s], hy[s], hz[s]), r.o, r.inv, r.tmin, r.best, tn)) continue;
     if (ch[s] & BVH_LEAF_BIT) {
-      if (MODE == 0) leaf_closest<COUNT, ALPHA>(tris, ch[s], r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt);
+      if (MODE == 0) leaf_closest<COUNT, ALPHA, FILTER>(tris, ch[s], r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt, r.lo, r.range);
       else if (leaf_occluded<COUNT, ALPHA>(tris, ch[s], r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
       continue;
     }
@@ -194,11 +197,27 @@ __device__ __forceinline__ bool step_bvh8(const Bvh8Node* nodes, const TriSet& t
   return false;
 }
 
-template <int WIDTH, int MODE, bool COUNT, bool ALPHA = true>
+template <int WIDTH, int MODE, bool COUNT, bool ALPHA = true, bool FILTER = false>
 __device__ __forceinline__ bool trav_step(const Bvh2Node* nodes2, const Bvh8Node* nodes8, const TriSet& tris, RayTrav& r,
                                           TravStack& st, TraversalCounters& cnt) {
-  if (WIDTH == 2) return step_bvh2<MODE, COUNT, ALPHA>(nodes2, tris, r, st, cnt);
-  return step_bvh8<MODE, COUNT, ALPHA>(nodes8, tris, r, st, cnt);
+  if (WIDTH == 2) return step_bvh2<MODE, COUNT, ALPHA, FILTER>(nodes2, tris, r, st, cnt);
+  return step_bvh8<MODE, COUNT, ALPHA, FILTER>(nodes8, tris, r, st, cnt);
+}
+
+// closest hit among the triangles [first, first + count) (an instance's), for subsurface probes
+template <int WIDTH>
+__device__ bool trace_closest_in_range(const Bvh2Node* nodes2, const Bvh8Node* nodes8, const TriSet& tris, TravStack& st, vec3 o,
+                                 vec3 d, float tmin, float tmax, uint32_t first, uint32_t count, float& t, uint32_t& tri,
+                                 float& b1, float& b2) {
+  RayTrav r;
+  trav_init(r, o, d, tmin, tmax, NO_HIT_ID);
+  r.lo = first;
+  r.range = count;
+  st.sp = 0;
+  TraversalCounters cnt = {0u, 0u};
+  while (!trav_step<WIDTH, 0, false, true, true>(nodes2, nodes8, tris, r, st, cnt)) {}
+  t = r.best; tri = r.bid; b1 = r.b1; b2 = r.b2;
+  return r.bid != NO_HIT_ID;
 }
 
 // Whole traversal of one ray. MODE 0 returns whether anything was hit (closest in bestT/bestId);

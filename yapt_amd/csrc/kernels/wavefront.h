@@ -41,6 +41,7 @@ struct Queues {
   uint32_t* ovf;       // traversal stack overflow
   uint32_t ovfStride;
   unsigned long long* workCounters;  // [0] nodes, [1] tris (extension), [2] nodes, [3] tris (shadow)
+  uint32_t* sssStack;  // subsurface probe traversal stacks of the shading launch, [depth][threads]
 };
 
 struct FilmBuffers {
@@ -190,7 +191,26 @@ __global__ void k_extend(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t lds
 // short queues of deep bounces cost little. The loop bound is uniform across each workgroup, so
 // every wave reaches the queue appends with all of its lanes.
 // 64-thread workgroups; MINW: waves per SIMD the register allocation must allow
-template <bool NEE, int MINW>
+// subsurface probe rays of a shading thread: BVH2 traversal restricted to one instance's triangles
+struct GpuProbe {
+  static constexpr bool enabled = true;
+  const Bvh2Node* nodes2;
+  const Bvh8Node* nodes8;
+  TriSet tris;
+  TravStack st;
+  __device__ bool closest_in_range(vec3 o, vec3 d, float tmin, float tmax, uint32_t first, uint32_t count, Hit& h) {
+    float t, b1, b2;
+    uint32_t tri;
+    bool hit = trace_closest_in_range<2>(nodes2, nodes8, tris, st, o, d, tmin, tmax, first, count, t, tri, b1, b2);
+    h.t = t; h.tri = hit ? tri : NO_HIT; h.b1 = b1; h.b2 = b2;
+    return hit;
+  }
+};
+
+// SSS: the scene has subsurface materials. Probe rays then run inside the shading launch, each
+// thread with its own stack column in Q.sssStack, and a path may hand two shadow rays (vertex and
+// subsurface exit) to the shadow queue, which is sized for that and applied atomically.
+template <bool NEE, int MINW, bool SSS = false>
 __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene S, const uint32_t* points, uint32_t npix,
                                                     PathSoA P, Queues Q, FilmBuffers F, int qsel, int writeFeatures) {
   if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;  // fetch counter of the next extension launch
@@ -198,8 +218,9 @@ __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene 
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
-    ShadowRay sr;
+    ShadowRay sr, sr2;
     sr.valid = false;
+    sr2.valid = false;
     bool cont = false;
     uint32_t slot = 0;
     if (i < n) {
@@ -214,8 +235,17 @@ __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene 
       FirstHitFeatures fh;
       fh.hit = 0u;
       bool first = ps.pathLength == 0;
-      if (h.tri == NO_HIT_ID) shade_miss(S.sv, fc, ps);
-      else shade_hit(S.sv, fc, point, NEE, ps, h, sr, &fh);  // features written below only when requested
+      if (h.tri == NO_HIT_ID) {
+        shade_miss(S.sv, fc, ps);
+      } else if constexpr (SSS) {
+        GpuProbe pr;
+        pr.nodes2 = S.nodes2; pr.nodes8 = S.nodes8; pr.tris = S.tris;
+        pr.st.lds = nullptr; pr.st.ovf = Q.sssStack; pr.st.block = 64; pr.st.lane = threadIdx.x; pr.st.ldsDepth = 0;
+        pr.st.ovfStride = stride; pr.st.gid = blockIdx.x * blockDim.x + threadIdx.x; pr.st.sp = 0;
+        shade_hit(S.sv, fc, point, NEE, ps, h, sr, &fh, &pr, &sr2);
+      } else {
+        shade_hit(S.sv, fc, point, NEE, ps, h, sr, &fh);  // features written below only when requested
+      }
       after_segment(point, ps);
       cont = ps.rayState == RAY_ALIVE && ps.pathLength < fc.maxRayDepth;
       store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);
@@ -227,6 +257,15 @@ __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene 
       Q.sDirTri[si] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, __uint_as_float(sr.targetTri));
       Q.sContrib[si] = make_float4(sr.contrib.s[0], sr.contrib.s[1], sr.contrib.s[2], sr.contrib.s[3]);
       Q.sSlot[si] = slot;
+    }
+    if constexpr (SSS) {
+      uint32_t si2 = wave_append(&Q.counters[2], sr2.valid);
+      if (sr2.valid) {
+        Q.sOrgTmax[si2] = make_float4(sr2.origin.x, sr2.origin.y, sr2.origin.z, sr2.tmax);
+        Q.sDirTri[si2] = make_float4(sr2.dir.x, sr2.dir.y, sr2.dir.z, __uint_as_float(sr2.targetTri));
+        Q.sContrib[si2] = make_float4(sr2.contrib.s[0], sr2.contrib.s[1], sr2.contrib.s[2], sr2.contrib.s[3]);
+        Q.sSlot[si2] = slot;
+      }
     }
     uint32_t ri = wave_append(&Q.counters[qsel ^ 1], cont);
     if (cont) Q.ray[qsel ^ 1][ri] = slot;

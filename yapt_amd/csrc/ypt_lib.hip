@@ -122,6 +122,7 @@ struct ResidentScene {
   DevBuf<AlphaTri> dAlpha;
   DevBuf<uint32_t> dTriAlpha;
   uint32_t stackBound2 = BVH_MAX_DEPTH, stackBound8 = 8 * BVH_MAX_DEPTH;
+  bool subsurface = false;  // some material has subsurfaceAmount > 0
   size_t device_bytes() const {
     return dPos.n * 4 + dNrm.n * 4 + dTan.n * 4 + dIdx.n * 4 + dTriInst.n * 4 + dInst.n * sizeof(InstanceGPU) +
            dMats.n * sizeof(MaterialGPU) + dLights.n * sizeof(LightGPU) + dNodes2.n * sizeof(Bvh2Node) +
@@ -151,6 +152,7 @@ struct ypt_integrator {
   DevBuf<float4> pOrg, pDir, pThr, pLight, pVolIor, pVolAbs, hit, sOrg, sDir, sContrib;
   DevBuf<uint4> pSt0, pSt1, pVolMat;
   DevBuf<uint32_t> rayQ0, rayQ1, counters, sSlot, ovf, sobolPoints;
+  DevBuf<uint32_t> sssStack;  // subsurface probe stacks of the shading launch
   DevBuf<unsigned long long> work;
   uint32_t ovfStride = 0;
 
@@ -269,6 +271,7 @@ struct ypt_integrator {
     Q.ray[0] = rayQ0.p; Q.ray[1] = rayQ1.p; Q.counters = counters.p; Q.hit = hit.p;
     Q.sOrgTmax = sOrg.p; Q.sDirTri = sDir.p; Q.sContrib = sContrib.p; Q.sSlot = sSlot.p;
     Q.ovf = ovf.p; Q.ovfStride = ovfStride; Q.workCounters = work.p;
+    Q.sssStack = sssStack.p;
     return Q;
   }
   FilmBuffers film() const {
@@ -316,7 +319,14 @@ void ensure_slots(ypt_integrator* h, size_t slots) {
 // traversal-stack overflow: one column per thread that can traverse. A thread's id is below the
 // queue length (one workgroup per entry, rounded up to a workgroup of at most 1024) or below the
 // persistent launch size.
+uint32_t sss_shade_grid(const ypt_integrator* h, uint32_t sgrid);
+
 void ensure_overflow(ypt_integrator* h) {
+  if (h->res && h->res->subsurface) {
+    // two shadow rays per path (vertex and subsurface exit), and the probe stacks of the shading launch
+    h->sOrg.ensure(2 * h->slots); h->sDir.ensure(2 * h->slots); h->sContrib.ensure(2 * h->slots); h->sSlot.ensure(2 * h->slots);
+    h->sssStack.ensure((size_t)h->res->stackBound2 * sss_shade_grid(h, 0xFFFFFFFFu) * 64);
+  }
   size_t stride = h->slots + 1024;
   if (h->persistent) stride = std::min(stride, (size_t)h->cus * h->persistentWaves * 64);
   uint32_t bound = h->bvhWidth == 8 ? h->res->stackBound8 : h->res->stackBound2;
@@ -394,6 +404,11 @@ uint32_t shade_grid(const ypt_integrator* h, uint32_t nslots) {
   return std::min(g, std::max<uint32_t>(1u, (uint32_t)h->cus * (uint32_t)h->shadeWaves));
 }
 
+// shading with subsurface probes: at most 32 workgroups of 64 per CU, which bounds the probe stacks
+uint32_t sss_shade_grid(const ypt_integrator* h, uint32_t sgrid) {
+  return std::min<uint32_t>(sgrid, std::max<uint32_t>(1u, (uint32_t)h->cus * 32u));
+}
+
 // explicit light connections (shadow rays) exist with NEE, and for BPT when the scene has punctual lights
 bool traces_shadow_rays(const ypt_integrator* h) {
   return h->type == YPT_INTEGRATOR_NEE || h->fc.punctualLightCount > 0;
@@ -405,7 +420,11 @@ void launch_shade(ypt_integrator* h, const DevScene& S, const uint32_t* point, u
   bool nee = h->type == YPT_INTEGRATOR_NEE;
   int w = h->shadeRegWaves;
 #define YPT_SHADE(N, M) hipLaunchKernelGGL((k_shade<N, M>), dim3(sgrid), dim3(64), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures)
-  if (nee) {
+  if (h->res->subsurface) {
+    uint32_t g = sss_shade_grid(h, sgrid);
+    if (nee) hipLaunchKernelGGL((k_shade<true, 2, true>), dim3(g), dim3(64), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
+    else hipLaunchKernelGGL((k_shade<false, 2, true>), dim3(g), dim3(64), 0, s, h->fc, S, point, npix, P, Q, F, q, h->writeFeatures);
+  } else if (nee) {
     if (w == 2) YPT_SHADE(true, 2); else if (w == 8) YPT_SHADE(true, 8); else YPT_SHADE(true, 4);
   } else {
     if (w == 2) YPT_SHADE(false, 2); else if (w == 8) YPT_SHADE(false, 8); else YPT_SHADE(false, 4);
@@ -435,7 +454,12 @@ template <int WIDTH>
 void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, uint32_t grid) {
   size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
   uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
-  if (h->persistent) {
+  if (h->res->subsurface) {  // two rays may add to one path: the atomic form, persistent at any grid
+    if (h->countTraversal)
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, true, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+    else
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+  } else if (h->persistent) {
     bool alpha = S.tris.alpha.tris != nullptr;
     if (h->countTraversal)
       hipLaunchKernelGGL((k_shadow_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
@@ -532,6 +556,7 @@ std::shared_ptr<ResidentScene> make_resident(const ypt_scene_desc& d, int device
   prepare_scene(d, host_tables().r2s, r->scene);
   BvhBuilder().build(r->scene.pos.data(), r->scene.idx.data(), (uint32_t)r->scene.triInst.size(), r->bvh);
   for (TriRecord& t : r->bvh.tris) t.alpha = r->scene.triAlpha[t.id];
+  for (const MaterialGPU& m : r->scene.mats) r->subsurface = r->subsurface || m.subsurfaceAmount > 0.f;
   r->stackBound2 = r->bvh.stats.maxDepth + 1;
   r->stackBound8 = bvh8_stack_bound(r->bvh.nodes8);
   const PreparedScene& ps = r->scene;
