@@ -163,7 +163,8 @@ template <int WIDTH, bool COUNT>
 __global__ void k_extend(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t ldsDepth) {
   extern __shared__ uint32_t lds[];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) { Q.counters[qsel ^ 1] = 0; Q.counters[2] = 0; }
+  // counters[5]: fetch counter of a persistent shadow launch that may follow (subsurface scenes)
+  if (i == 0) { Q.counters[qsel ^ 1] = 0; Q.counters[2] = 0; Q.counters[5] = 0; }
   uint32_t n = Q.counters[qsel];
   TraversalCounters cnt = {0u, 0u};
   if (i < n) {
