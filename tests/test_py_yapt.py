@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from yapt_amd import _lib, camera, scenes
+from yapt_amd.gltf import load as gltf_load
 from yapt_amd import py_yapt as Y
 from yapt_amd.py_yapt import Light, Material, vec3, vec4
 
@@ -289,3 +290,54 @@ def test_rendering_through_the_surface_matches_the_checker(tmp_path, oracle):
         assert rb.getState() == Y.ReadbackState.Ready and rb.getData().shape == (64, 96, 4)
         assert rb.getData()[..., :3].max() > 0
     app.shutdown()
+
+
+def test_scene_loader_builds_the_scene_gltf_load_bakes(tmp_path):
+    """the node tree, materials, textures and lights that SceneLoader creates commit to the arrays that
+    gltf.load bakes directly (matrix nodes included)"""
+    import test_gltf as G
+    path = str(G._write_gltf(tmp_path))
+    want, _ = gltf_load(path)
+    app = Y.Application(32, 32)
+    loader = Y.SceneLoader(app.get_resources(), app.get_scene(), app.get_renderer())
+    roots = loader.load_scene_gltf(path)
+    assert len(roots) == 2 and loader.get_material("m0") is not None
+    got = app.get_scene().arrays()
+    assert got.triangle_count == want.triangle_count
+    assert np.allclose(np.sort(got.positions, axis=0), np.sort(want.positions, axis=0), atol=1e-5)
+    assert np.allclose(np.sort(got.normals, axis=0), np.sort(want.normals, axis=0), atol=1e-5)
+    gl = sorted(got.lights, key=lambda l: l.type)
+    wl = sorted(want.lights, key=lambda l: l.type)
+    for a, b in zip(gl, wl):
+        assert a.type == b.type
+        assert np.allclose(list(a.position), list(b.position), atol=1e-5)
+        assert np.allclose(list(a.direction), list(b.direction), atol=1e-5)
+        assert np.allclose(list(a.intensity), list(b.intensity))
+    assert bytes(loader.get_material("m0").record(lambda t: -1)) == bytes(want.materials[0])
+    doc, blob, _ = G._textured_doc("MASK", 0, in_buffer=True)
+    tpath = G._save(tmp_path, doc, blob, True)
+    want, _ = gltf_load(tpath)
+    app = Y.Application(32, 32)
+    Y.SceneLoader(app.get_resources(), app.get_scene(), app.get_renderer()).load_scene_gltf(tpath)
+    got = app.get_scene().arrays()
+    assert np.allclose(got.uvs, want.uvs) and len(got.texel_arrays) == len(want.texel_arrays) == 3
+    m = got.materials[0]
+    assert abs(m.alpha_cutoff - 0.3) < 1e-7
+    for slot in ("albedo", "orm", "normal"):
+        k = getattr(m, slot + "_tex")
+        assert np.array_equal(got.texel_arrays[k], want.texel_arrays[getattr(want.materials[0], slot + "_tex")])
+
+
+def test_matrix_nodes_decompose_into_trs():
+    q = Y.axis_angle_quat((1, 2, 3), 40)
+    t = Y.Transform()
+    t.setOrientation(q)
+    t.setScale(vec3([2, 3, -4]))
+    t.setTranslation(vec3([1, -2, 5]))
+    m = t.getMatrixWS()
+    tt, qq, ss = Y._decompose(m)
+    u = Y.Transform()
+    u.setTranslation(tt)
+    u.setOrientation(qq)
+    u.setScale(ss)
+    assert np.allclose(u.getMatrixWS(), m, atol=1e-9)

@@ -3,11 +3,14 @@
     python -m yapt_amd.profsum stats  <kernel-trace dir> "<command>" [rows]  > summary.md
     python -m yapt_amd.profsum traffic <fetch dir> <write dir> <calib fetch dir> <calib write dir> \
            <kernel substring> <key> <calib MiB> [traffic.json [summary.md "<command>"...]]
+    python -m yapt_amd.profsum l2 <TCC_HIT/TCC_MISS dir> <kernel substring> ["<command>"...] > summary.md
 
 ``stats`` renders the ``*_kernel_stats.csv`` of a ``rocprofv3 --kernel-trace --stats`` run as a
 markdown table. ``traffic`` turns the FETCH_SIZE / WRITE_SIZE counters (KiB per dispatch) of one
 kernel into HBM bytes per launch, each scaled by what the same counter reads for a plain device copy
 of known size (``yapt_amd.calib_copy``), and stores the result under ``key`` in traffic.json.
+``l2`` gives the L2 (TCC) hit rate of a kernel from a ``--pmc TCC_HIT_sum TCC_MISS_sum`` run: the
+share of its memory requests that never reach the fabric (MALL or HBM).
 """
 import csv
 import glob
@@ -89,6 +92,24 @@ def traffic_markdown(t, key, commands):
     return "\n".join(out) + "\n"
 
 
+def l2_hit_rate(dirname, kernel_substr):
+    hits = counter_values(dirname, "TCC_HIT_sum", kernel_substr)
+    miss = counter_values(dirname, "TCC_MISS_sum", kernel_substr)
+    h, m = sum(hits), sum(miss)
+    return {"hit_rate": h / (h + m) if h + m else 0.0, "hits_per_launch": _mean(hits),
+            "misses_per_launch": _mean(miss), "dispatches": [len(hits), len(miss)], "kernel": kernel_substr}
+
+
+def l2_markdown(r, commands):
+    out = ["# L2 hit rate: `%s`" % r["kernel"], "", "Commands:", "", "```"] + list(commands) + ["```", "",
+           "| quantity | value |", "|---|---|",
+           "| dispatches averaged (HIT, MISS) | %d, %d |" % tuple(r["dispatches"]),
+           "| L2 hits per launch | %.4g |" % r["hits_per_launch"],
+           "| L2 misses per launch | %.4g |" % r["misses_per_launch"],
+           "| L2 hit rate | %.3f |" % r["hit_rate"]]
+    return "\n".join(out) + "\n"
+
+
 def main(argv):
     if argv[0] == "stats":
         sys.stdout.write(stats_markdown(argv[1], argv[2], int(argv[3]) if len(argv) > 3 else 12))
@@ -109,6 +130,9 @@ def main(argv):
         if len(argv) > 9:
             with open(argv[9], "w") as f:
                 f.write(traffic_markdown(t, key, argv[10:]))
+    elif argv[0] == "l2":
+        r = l2_hit_rate(argv[1], argv[2])
+        sys.stdout.write(l2_markdown(r, argv[3:]))
     else:
         raise SystemExit(__doc__)
 

@@ -2,7 +2,8 @@
 (Source/Python/srcBindings: PyRenderer.cpp, PyScene.cpp, PyMaterial.cpp, PyMesh.cpp,
 PyRenderObject.cpp, PySceneObject.cpp, PyTransform.cpp, PyCamera.cpp, PyLight.cpp,
 PyRendererVar.cpp, PyReadback.cpp) and the pythonScripts/yapt wrappers around them
-(scene_wrapper.py, resources.py), so that a script written against them builds the same scene here.
+(scene_wrapper.py, resources.py, scene_loader.py), so that a script written against them builds the
+same scene here.
 
     app = Application(width=960, height=704)
     renderer, scene, resources = app.get_renderer(), app.get_scene(), app.get_resources()
@@ -1030,29 +1031,8 @@ class Resources:
         out = []
         base = os.path.splitext(os.path.basename(mesh_path))[0]
         for mi, prims in enumerate(gltf.mesh_primitives(mesh_path)):
-            if not prims:
-                continue
-            P, N, T, UV, I, subs = [], [], [], [], [], []
-            nv = ni = 0
-            has_n = all(p["normals"] is not None for p in prims)
-            has_t = all(p["tangents"] is not None for p in prims)
-            has_uv = any(p["uvs"] is not None for p in prims)
-            for p in prims:
-                P.append(p["positions"])
-                if has_n:
-                    N.append(p["normals"])
-                if has_t:
-                    T.append(p["tangents"])
-                if has_uv:
-                    UV.append(p["uvs"] if p["uvs"] is not None else np.zeros((len(p["positions"]), 2), np.float32))
-                I.append(p["indices"])
-                subs.append((ni, len(p["indices"]), nv))
-                nv += len(p["positions"])
-                ni += len(p["indices"])
-            mesh = Mesh("%s_%d" % (base, mi), np.concatenate(P), np.concatenate(I),
-                        np.concatenate(N) if has_n else None, np.concatenate(UV) if has_uv else None,
-                        np.concatenate(T) if has_t else None, subs)
-            out.append(([mesh], {k: (0, k) for k in range(len(prims))}))
+            if prims:
+                out.append(([_mesh_from_primitives("%s_%d" % (base, mi), prims)], {k: (0, k) for k in range(len(prims))}))
         return out
 
     def load_texture_2D(self, tex_path, fmt="R8G8B8A8_SRGB", verbose=False):
@@ -1105,3 +1085,148 @@ class Application:
 
     def shutdown(self):
         self._renderer.shutdown()
+
+
+def _decompose(m):
+    """translation, rotation (x, y, z, w) and scale of an affine matrix without shear"""
+    m = np.asarray(m, np.float64)
+    t = m[:3, 3].copy()
+    a = m[:3, :3]
+    s = np.linalg.norm(a, axis=0)
+    if np.linalg.det(a) < 0:
+        s[0] = -s[0]
+    r = a / np.where(s == 0, 1.0, s)
+    return t, matrix_to_quat(r), s
+
+
+class SceneLoader:
+    """pythonScripts/yapt/scene_loader.py over this build's glTF parser: a node tree of scene objects
+    with their local transforms (a `matrix` node is decomposed into translation, rotation and
+    scale, where the reference raises), one render object per mesh node with a material per
+    primitive, KHR_lights_punctual lights as children of their nodes, and materials and textures
+    mapped as in yapt_amd/gltf.py."""
+
+    def __init__(self, resources, scene, renderer):
+        self._resources, self._scene, self._renderer = resources, scene, renderer
+        self._materials, self._doc, self._default = [], None, None
+
+    def load_scene_gltf(self, scene_path, verbose=False):
+        from . import gltf
+        doc, binchunk = gltf._read_container(scene_path)
+        base = os.path.dirname(os.path.abspath(scene_path))
+        bufs = gltf._buffers(doc, base, binchunk)
+        self._doc = doc
+        images = gltf._Images(doc, bufs, base)
+        self._materials = []
+        for k, mdef in enumerate(doc.get("materials", [])):
+            rec, slots = gltf._material(doc, mdef, images)
+            mat = self._renderer.createMaterial(mdef.get("name", "material_%d" % k))
+            mat._p = _lib.Material.from_buffer_copy(rec)
+            self._materials.append((mat, slots))
+        textures = [self._renderer.createTexture("image_%d" % i, rgba, srgb) for i, (rgba, srgb) in enumerate(images.textures)]
+        for mat, slots in self._materials:
+            for slot, index in slots.items():
+                if index >= 0:
+                    mat._bind(slot, textures[index], getattr(mat._p, slot + "_tex_scale"))
+        self._meshes = []
+        name = os.path.splitext(os.path.basename(scene_path))[0]
+        for mi, prims in enumerate(gltf.mesh_primitives(scene_path)):
+            self._meshes.append(_mesh_from_primitives("%s_%d" % (name, mi), prims) if prims else None)
+        self._prims = [[p for p in m.get("primitives", []) if p.get("mode", 4) == 4] for m in doc.get("meshes", [])]
+        si = doc.get("scene", 0)
+        roots = doc["scenes"][si]["nodes"] if doc.get("scenes") else list(range(len(doc.get("nodes", []))))
+        return [self._node_tree(r) for r in roots]
+
+    def get_gltf(self):
+        return self._doc
+
+    def get_material(self, gltf_material_name):
+        for mat, _ in self._materials:
+            if mat.getName() == gltf_material_name:
+                return mat
+        return None
+
+    def _default_material(self):
+        if self._default is None:
+            self._default = self._renderer.createMaterial("default_mat")
+            self._default.setFromMaterialPreset(MaterialPreset.PLASTIC)
+            self._default.setSpecularAmount(0.0)
+        return self._default
+
+    def _node_tree(self, index):
+        node = self._doc["nodes"][index]
+        obj = self._node_object(node, index)
+        self._node_light(node, obj)
+        for c in node.get("children", []):
+            child = self._node_tree(c)
+            child.getTransform().setParent(obj.getTransform())
+        return obj
+
+    def _node_object(self, node, index):
+        name = node.get("name", "node_%d" % index)
+        obj = self._scene.create_scene_object(name)
+        tr = obj.getTransform()
+        if "matrix" in node:
+            from . import gltf
+            t, q, s = _decompose(gltf.node_matrix(node))
+            tr.setTranslation(t)
+            tr.setOrientation(q)
+            tr.setScale(s)
+        else:
+            if "translation" in node:
+                tr.setTranslation(vec3(node["translation"]))
+            if "scale" in node:
+                tr.setScale(vec3(node["scale"]))
+            if "rotation" in node:
+                tr.setOrientation(vec4(node["rotation"]))
+        mesh = self._meshes[node["mesh"]] if "mesh" in node and node["mesh"] < len(self._meshes) else None
+        if mesh is not None:
+            ro = self._scene.create_render_object("%s_0" % name)
+            ro.setMesh(mesh)
+            ro.getTransform().setParent(tr)
+            for k, prim in enumerate(self._prims[node["mesh"]]):
+                m = prim.get("material")
+                ro.setMaterial(self._materials[m][0] if m is not None else self._default_material(), k)
+        return obj
+
+    def _node_light(self, node, obj):
+        from . import gltf
+        ref = (node.get("extensions") or {}).get("KHR_lights_punctual")
+        defs = (self._doc.get("extensions") or {}).get("KHR_lights_punctual", {}).get("lights", [])
+        if ref is None or ref.get("light") is None or ref["light"] >= len(defs):
+            return None
+        d = defs[ref["light"]]
+        t = {"point": LightType.POINT, "spot": LightType.SPOT, "directional": LightType.DIRECTIONAL}.get(d.get("type"))
+        if t is None:
+            return None
+        light = self._scene.create_light(d.get("name") or "%s_light" % node.get("name", "node"), t)
+        light.setColor(vec3(d.get("color", [1.0, 1.0, 1.0])))
+        light.setIntensity(d.get("intensity", 1.0) / gltf.GLTF_LUMENS_PER_WATT)
+        if t == LightType.SPOT:
+            spot = d.get("spot", {})
+            light.setSpotConeAngles(spot.get("innerConeAngle", 0.0), spot.get("outerConeAngle", math.pi / 4.0))
+        light.getTransform().setParent(obj.getTransform())
+        return light
+
+
+def _mesh_from_primitives(name, prims):
+    """one Mesh with a submesh per primitive"""
+    P, N, T, UV, I, subs = [], [], [], [], [], []
+    nv = ni = 0
+    has_n = all(p["normals"] is not None for p in prims)
+    has_t = all(p["tangents"] is not None for p in prims)
+    has_uv = any(p["uvs"] is not None for p in prims)
+    for p in prims:
+        P.append(p["positions"])
+        if has_n:
+            N.append(p["normals"])
+        if has_t:
+            T.append(p["tangents"])
+        if has_uv:
+            UV.append(p["uvs"] if p["uvs"] is not None else np.zeros((len(p["positions"]), 2), np.float32))
+        I.append(p["indices"])
+        subs.append((ni, len(p["indices"]), nv))
+        nv += len(p["positions"])
+        ni += len(p["indices"])
+    return Mesh(name, np.concatenate(P), np.concatenate(I), np.concatenate(N) if has_n else None,
+                np.concatenate(UV) if has_uv else None, np.concatenate(T) if has_t else None, subs)
