@@ -338,6 +338,10 @@ struct CpuTracer {
     if (work) work[2]++;
     return geo->closest(o, d, tmin, tmax, h, work);
   }
+  bool closest_in_range(vec3 o, vec3 d, float tmin, float tmax, uint32_t first, uint32_t count, Hit& h) {
+    if (work) work[2]++;
+    return geo->closest(o, d, tmin, tmax, h, work, first, count);
+  }
   bool hits_target(vec3 o, vec3 d, float tmin, float tmax, uint32_t tri) {
     if (work) work[3]++;
     Hit h;

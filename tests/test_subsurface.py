@@ -2,8 +2,8 @@
 normalisation and the closed-form CDF of its radius sampler, the searchlight fit of d, the exit-point
 density's axis and lane mixture on closed-form configurations, energy under the white furnace (a
 subsurface plane returns what a Lambertian plane of the same albedo returns, less the profile's tail
-beyond r_max), BPT against NEE on a lit subsurface scene, and on the GPU the subsurface kernels
-against the checker for both unidirectional integrators."""
+beyond r_max), BPT and BDPT against NEE on a lit subsurface scene, and on the GPU the subsurface
+kernels against the checker for all three integrators (both forms of the BDPT camera stage)."""
 import math
 
 import numpy as np
@@ -11,7 +11,7 @@ import pytest
 
 from yapt_amd import _lib, camera, scenes
 
-NEE, BPT = _lib.YPT_INTEGRATOR_NEE, _lib.YPT_INTEGRATOR_BPT
+NEE, BPT, BDPT = _lib.YPT_INTEGRATOR_NEE, _lib.YPT_INTEGRATOR_BPT, _lib.YPT_INTEGRATOR_BDPT
 RMAX_FACTOR = 3 * math.log(1000)
 
 
@@ -128,12 +128,29 @@ def test_bpt_and_nee_agree_on_a_subsurface_scene(oracle):
     assert abs(a.mean() / b.mean() - 1) < 0.03, (a.mean(), b.mean())
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("integrator", [BPT, NEE])
-def test_subsurface_render_matches_the_checker(ylib, oracle, integrator):
+def test_bdpt_agrees_with_nee_on_a_subsurface_scene(oracle):
+    """subsurface exits in both subpaths, with their area-density edges kept out of the MIS sums where
+    no strategy can connect them: BDPT converges to the same image as NEE"""
     scene, cam = _lit_scene()
-    frame = camera.make_frame(cam, 64, 48, max_depth=6)
+    fr = camera.make_frame(cam, 40, 30, max_depth=6, firefly_clamp_scale=0.0, light_path_vertices=6)
+    a = oracle.render(scene, fr, NEE, 0, 384)["film"][..., 1]
+    b = oracle.render(scene, fr, BDPT, 0, 384)["film"][..., 1]
+    assert np.isfinite(b).all()
+    assert abs(b.mean() / a.mean() - 1) < 0.015, (a.mean(), b.mean())
+    # per 10 x 10 block with enough light, where one estimator's bias would show against the other's noise
+    ba, bb = a.reshape(3, 10, 4, 10).mean((1, 3)), b.reshape(3, 10, 4, 10).mean((1, 3))
+    lit = ba > 0.25 * ba.max()
+    assert lit.sum() >= 6 and np.all(np.abs(bb[lit] / ba[lit] - 1) < 0.05), bb / ba
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integrator,wavefront", [(BPT, 1), (NEE, 1), (BDPT, 1), (BDPT, 0)])
+def test_subsurface_render_matches_the_checker(ylib, oracle, integrator, wavefront):
+    scene, cam = _lit_scene()
+    frame = camera.make_frame(cam, 64, 48, max_depth=6, light_path_vertices=6)
     it = ylib.Integrator(integrator, 0)
+    if integrator == BDPT:
+        it.set_option("bdpt_wavefront", wavefront)
     it.set_scene(scene)
     it.set_frame(frame)
     it.render(0, 3)

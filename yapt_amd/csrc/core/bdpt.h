@@ -12,11 +12,14 @@
 //   bool closest(vec3 o, vec3 d, float tmin, float tmax, Hit& h)         closest (t, id) in [tmin, tmax]
 //   bool hits_target(vec3 o, vec3 d, float tmin, float tmax, uint32_t tri) the closest hit is `tri`
 //   bool unoccluded(vec3 o, vec3 d, float tmin, float tmax)              nothing is hit in [tmin, tmax]
+//   bool closest_in_range(vec3 o, vec3 d, float tmin, float tmax, uint32_t first, uint32_t count, Hit& h)
+//                                                                        subsurface probes (one instance)
 // so that the GPU kernels and the CPU checker run the same vertex arithmetic over their own BVHs.
 //
-// Environment light subpaths (environmentSelectionPDF = 1 when a scene has no lights) and punctual
-// lights are not part of this build: light subpaths start on mesh lights only, and camera subpaths
-// that escape are weighted as the only strategy that finds the environment.
+// Subsurface scattering: a subpath that takes the subsurface event continues from the exit point as
+// its next vertex, with no ray in between; the edge's densities are area densities, and exit
+// vertices cannot be reached by a connection, so they stay out of the MIS sums where the reference
+// keeps them out (accumulate_ri's `connectable`).
 #pragma once
 #include "shade.h"
 
@@ -33,6 +36,7 @@ static constexpr uint32_t LN_FRONT_FACE = 1u << 1;
 static constexpr uint32_t LN_ENV = 1u << 2;
 static constexpr uint32_t LN_DELTA = 1u << 3;
 static constexpr uint32_t LN_REGULARIZED = 1u << 4;
+static constexpr uint32_t LN_SUBSURFACE_EXIT = 1u << 5;  // reached through the subsurface: no ray before it
 static constexpr uint32_t LN_PUNCTUAL = 1u << 6;
 static constexpr uint32_t LN_DIRECTIONAL = 1u << 7;
 
@@ -183,6 +187,33 @@ YPT_HD void bdpt_surface(const SceneView& S, BdptState& rs, const Hit& h, vec3 r
   if (matOut) *matOut = &m;
 }
 
+// makePathHitFromSubsurfaceExit + calculateCommonSurfaceParams (BDPTCommon.hlsl:507-521, 637-655):
+// the surface at a subsurface exit point, set up with the outward normal as the incoming direction
+YPT_HD void bdpt_surface_exit(const SceneView& S, BdptState& rs, const SurfacePoint& p, const Lambdas& L, BdptVertex& v,
+                              const MaterialGPU** matOut) {
+  TriGeometry g;
+  fetch_triangle(S, p.tri, p.b1, p.b2, g);
+  uint32_t matId = S.inst[S.triInst[p.tri]].material;
+  const MaterialGPU& m = S.mats[matId];
+  v.tri = p.tri;
+  v.matId = matId;
+  v.b1 = p.b1;
+  v.b2 = p.b2;
+  v.t = 0.f;
+  v.pos = p.pos;
+  v.wo = normalize(p.n);
+  v.frontFace = p.frontFace;
+  bool reg = (rs.flags & RS_REGULARIZE) != 0;
+  SurfaceTexels tx;
+  sample_surface_texels(S, m, p.tri, p.b1, p.b2, tx);
+  SurfaceDef sd = make_surface(m, g.geometryNormal, g.normal, g.tangent, v.wo, true, reg, L, S.spec, &tx);
+  if (dot(-v.wo, sd.geometryNormal) > 0.f && rs.n == 0 && !surface_two_sided(sd.flags))
+    rs.entered(sd.dielectricIOR, sd.absorption, matId);
+  make_surface_context(v.ctx, sd, rs.current_ior(), rs.previous_ior(), v.wo, v.frontFace, L, true);
+  v.normalWS = normalize(sd.geometryNormal);
+  if (matOut) *matOut = &m;
+}
+
 YPT_HD Spectrum transmittance(float t, const Spectrum& a) {
   Spectrum tr;
   for (int i = 0; i < 4; ++i) tr.s[i] = yexp(-a.s[i] * t);
@@ -231,14 +262,72 @@ YPT_HD ScatterEval eval_and_pdfs(const SceneView& S, const SurfaceContext& ctx, 
   return r;
 }
 
-// sampleScatteringEvent (scatteringSample.hlsl:30-63) without subsurface: a zero direction is
-// SCATTERING_EVENT_NONE.
-YPT_HD vec3 sample_scattering(const SceneView& S, const SurfaceContext& ctx, const float r[4], bool& isDiffuse, bool& isDelta) {
-  int layer = select_layer(r[0], ctx.prob);
-  isDiffuse = true;
-  isDelta = false;
-  if (layer == LAYER_SUBSURFACE) return vec3(0.f);
-  return sample_layer_direction(ctx, S.luts, layer, r[1], r[2], r[3], isDiffuse, isDelta);
+// sampleScatteringEvent (scatteringSample.hlsl:30-63): a local direction, a subsurface entry (with
+// its sampled exit point), the exit lobe at a subsurface exit, or nothing
+static constexpr int EV_NONE = 0, EV_LOCAL = 1, EV_SSS_ENTRY = 2, EV_SSS_EXIT_LOBE = 3;
+struct ScatterEvent {
+  SubsurfaceExit exit;  // EV_SSS_ENTRY
+  vec3 wi;              // EV_LOCAL, EV_SSS_EXIT_LOBE
+  int type;
+  bool isDiffuse, isDelta;
+};
+
+template <class Tracer>
+YPT_HD void sample_scattering_event(const SceneView& S, const BdptVertex& v, bool atExit, const float r[4], Tracer& tr,
+                                    ScatterEvent& ev) {
+  ev.type = EV_NONE;
+  ev.wi = vec3(0.f);
+  ev.isDiffuse = true;
+  ev.isDelta = false;
+  if (atExit) {
+    ev.wi = sample_subsurface_exit_lobe(v.ctx, v.wo, r[2], r[3]);
+    if (!is_zero(ev.wi)) ev.type = EV_SSS_EXIT_LOBE;
+    return;
+  }
+  int layer = select_layer(r[0], v.ctx.prob);
+  if (layer == LAYER_SUBSURFACE) {
+    SurfacePoint e;
+    e.pos = v.pos;
+    e.n = orient_to_same_side(v.wo, v.normalWS);  // makeSurfacePoint
+    e.inst = S.triInst[v.tri];
+    e.tri = v.tri;
+    e.b1 = v.b1;
+    e.b2 = v.b2;
+    e.frontFace = v.frontFace;
+    Spectrum radius = spectrum_unbounded_no_illum(S.mats[v.matId].subsurfaceRadius, v.ctx.lambdas);
+    if (sample_subsurface_exit(S, v.ctx.def.albedo, radius, e, r[1], r[2], r[3], tr, ev.exit)) ev.type = EV_SSS_ENTRY;
+    return;
+  }
+  ev.wi = sample_layer_direction(v.ctx, S.luts, layer, r[1], r[2], r[3], ev.isDiffuse, ev.isDelta);
+  if (!is_zero(ev.wi)) ev.type = EV_LOCAL;
+}
+
+// evaluateSubsurfaceEntry (scatteringSample.hlsl:66-76): the forward density is one of area (the
+// subsurface edge), the backward one that of the exit lobe leaving along wo
+YPT_HD ScatterEval eval_subsurface_entry(const SceneView& S, const BdptVertex& v, const SubsurfaceExit& ex) {
+  ScatterEval r;
+  r.weight = ex.weight * (subsurface_entry_factor(v.ctx, S.luts) * ex.pdfPosition);
+  r.pdfFwd = v.ctx.prob[LAYER_SUBSURFACE] * ex.pdfPosition;
+  r.dispersed = false;
+  Spectrum unused;
+  evaluate_subsurface_exit_lobe(v.ctx, v.wo, v.wo, EVAL_PDF_ONLY, unused, r.pdfBwd);
+  return r;
+}
+
+// evaluateSubsurfaceExitAndPDFs (BDPTCommon.hlsl:463-472): the exit lobe forward, and backward the
+// area density of the entry point seen from the exit, with the reversed surface's subsurface
+// probability
+YPT_HD ScatterEval eval_subsurface_exit(const SceneView& S, const SurfaceContext& ctx, const BdptState& rs, uint32_t matId,
+                                        vec3 wo, vec3 wi, bool frontFace, bool fromLight, vec3 exitPos, vec3 exitN,
+                                        vec3 entryPos, vec3 entryN) {
+  ScatterEval r;
+  r.dispersed = false;
+  evaluate_subsurface_exit_lobe(ctx, wo, wi, fromLight ? EVAL_LIGHT_PATH : EVAL_NONE, r.weight, r.pdfFwd);
+  SurfaceContext rev;
+  make_surface_context(rev, ctx.def, rs.current_ior(), rs.previous_ior(), wi, frontFace, ctx.lambdas, true);
+  Spectrum radius = spectrum_unbounded_no_illum(S.mats[matId].subsurfaceRadius, ctx.lambdas);
+  r.pdfBwd = rev.prob[LAYER_SUBSURFACE] * pdf_subsurface_exit(ctx.def.albedo, radius, exitPos, exitN, entryPos, entryN);
+  return r;
 }
 
 // ---------------------------------------------------------------- MIS bookkeeping
@@ -531,55 +620,82 @@ YPT_HD uint32_t trace_light_path(const SceneView& S, const FrameConstants& fc, c
   bool previousWasDelta = false;
   BdptState rs;
   rs.init();
+  // set when the previous vertex took the subsurface event: the next vertex is its exit point
+  bool pendingExit = false;
+  SubsurfaceExit exit;
   for (uint32_t i = 1; i < fc.lightPathVertices; ++i) {
     float rs0[4];
     rand.r4(rs0);
     Hit h;
-    if (!tr.closest(rayOrigin, rayDir, RAY_MIN_T, __builtin_inff(), h)) break;
+    bool isExit = pendingExit;
+    pendingExit = false;
+    if (!isExit && !tr.closest(rayOrigin, rayDir, RAY_MIN_T, __builtin_inff(), h)) break;
     uint32_t ci = i & 1u, pi = (i - 1u) & 1u;
     if (i - 1u >= fc.regularizeAfterVertices) rs.flags |= RS_REGULARIZE;
     uint32_t flags = 0;
     BdptVertex v;
-    bdpt_surface(S, rs, h, rayOrigin, rayDir, L, v, nullptr);
+    if (isExit) bdpt_surface_exit(S, rs, exit.p, L, v, nullptr);
+    else bdpt_surface(S, rs, h, rayOrigin, rayDir, L, v, nullptr);
     if (v.frontFace) flags |= LN_FRONT_FACE;
     if (rs.flags & RS_SECONDARY_TERMINATED) flags |= LN_SECONDARY_TERMINATED;
     if (rs.flags & RS_REGULARIZE) flags |= LN_REGULARIZED;
-    if (is_delta_distribution(v.ctx.pre.ax, v.ctx.pre.ay, v.ctx.def)) flags |= LN_DELTA;
+    if (isExit) flags |= LN_SUBSURFACE_EXIT;
+    // a subsurface exit only has the exit lobe
+    else if (is_delta_distribution(v.ctx.pre.ax, v.ctx.pre.ay, v.ctx.def)) flags |= LN_DELTA;
+    vec3 nrm = isExit ? orient_to_same_side(v.wo, v.normalWS) : v.normalWS;  // exit nodes keep the outward normal
     float curFwd;
-    if (i == 1u && originInfinite) curFwd = pdfPos * fabsf(dot(rayDir, v.normalWS));  // planar density of the disk
-    else curFwd = previousWasDelta ? -1.f : pdfForward * solid_angle_to_area(v.pos - rayOrigin, v.normalWS);
-    Spectrum ab = rs.get_absorption();
-    if (!ab.all_equal(0.f)) radiance = radiance * transmittance(h.t, ab);
+    if (isExit) {
+      curFwd = pdfForward;  // the subsurface edge density is one of area already
+    } else {
+      if (i == 1u && originInfinite) curFwd = pdfPos * fabsf(dot(rayDir, v.normalWS));  // planar density of the disk
+      else curFwd = previousWasDelta ? -1.f : pdfForward * solid_angle_to_area(v.pos - rayOrigin, v.normalWS);
+      Spectrum ab = rs.get_absorption();
+      if (!ab.all_equal(0.f)) radiance = radiance * transmittance(h.t, ab);
+    }
     nodes[ci].radiance = radiance;
     nodes[ci].pos = v.pos;
-    nodes[ci].nrm = v.normalWS;
-    nodes[ci].tri = h.tri;
-    nodes[ci].b1 = h.b1;
-    nodes[ci].b2 = h.b2;
+    nodes[ci].nrm = nrm;
+    nodes[ci].tri = v.tri;
+    nodes[ci].b1 = v.b1;
+    nodes[ci].b2 = v.b2;
     nodes[ci].pdfFwd = curFwd;
     nodes[ci].iorPrev = rs.previous_ior();
     nodes[ci].iorCur = rs.current_ior();
     nodes[ci].flags = flags;
 
-    bool isDiffuse, isDelta;
-    vec3 wi = sample_scattering(S, v.ctx, rs0, isDiffuse, isDelta);
-    if (is_zero(wi)) break;
-    if (isDiffuse) rs.flags |= RS_REGULARIZE;
-    ScatterEval ev = eval_and_pdfs(S, v.ctx, rs, v.matId, v.wo, wi, v.frontFace, true, isDelta);
+    ScatterEvent se;
+    sample_scattering_event(S, v, isExit, rs0, tr, se);
+    if (se.type == EV_NONE) break;
+    if (se.isDiffuse) rs.flags |= RS_REGULARIZE;
+    ScatterEval ev;
+    bool bwdIsArea = false;
+    if (se.type == EV_SSS_ENTRY) {
+      ev = eval_subsurface_entry(S, v, se.exit);
+    } else if (se.type == EV_SSS_EXIT_LOBE) {
+      ev = eval_subsurface_exit(S, v.ctx, rs, v.matId, v.wo, se.wi, v.frontFace, true, v.pos, nrm, nodes[pi].pos, nodes[pi].nrm);
+      bwdIsArea = true;
+    } else {
+      ev = eval_and_pdfs(S, v.ctx, rs, v.matId, v.wo, se.wi, v.frontFace, true, se.isDelta);
+    }
     pdfForward = ev.pdfFwd;
     Spectrum sampleWeight = ev.weight * safe_div(1.f, pdfForward);
-    nodes[pi].pdfBwd = isDelta ? -1.f : ev.pdfBwd * solid_angle_to_area(rayOrigin - v.pos, nodes[pi].nrm);
-    if (i == 1u && originPunctual) nodes[pi].pdfBwd = 0.f;                 // a camera path never hits a delta light
-    else if (i == 1u && originEnv && !isDelta) nodes[pi].pdfBwd = ev.pdfBwd;  // found by direction: stays per solid angle
+    nodes[pi].pdfBwd = se.isDelta ? -1.f : (bwdIsArea ? ev.pdfBwd : ev.pdfBwd * solid_angle_to_area(rayOrigin - v.pos, nodes[pi].nrm));
+    if (i == 1u && originPunctual) nodes[pi].pdfBwd = 0.f;                    // a camera path never hits a delta light
+    else if (i == 1u && originEnv && !se.isDelta) nodes[pi].pdfBwd = ev.pdfBwd;  // found by direction: stays per solid angle
     out[stored++] = pack_node(nodes[pi]);
     ++nodeCount;
     radiance = radiance * sampleWeight;
     if (ev.dispersed) radiance.terminate_secondary();
     if (radiance.russian_roulette(rand.r1(), maxRadiance)) break;
     if (radiance.all_equal(0.f)) break;
-    rayDir = normalize(wi);
-    rayOrigin = spawn_ray_origin(v.pos, v.normalWS, rayDir);
-    previousWasDelta = isDelta;
+    if (se.type == EV_SSS_ENTRY) {
+      pendingExit = true;
+      exit = se.exit;
+    } else {
+      rayDir = normalize(se.wi);
+      rayOrigin = spawn_ray_origin(v.pos, nrm, rayDir);
+    }
+    previousWasDelta = se.isDelta;
   }
   out[stored++] = pack_node(nodes[(nodeCount - 1u) & 1u]);
   return stored;
@@ -609,13 +725,16 @@ YPT_HD BdptState light_node_state(const LightNode& n, bool secondaryTerminated) 
 YPT_HD void light_node_surface(const SceneView& S, const LightNode& n, vec3 prevPos, const Lambdas& L, LightNodeSurface& out) {
   const MaterialGPU& m = S.mats[S.inst[S.triInst[n.tri]].material];
   bool frontFace = (n.flags & LN_FRONT_FACE) != 0;
-  out.wo = normalize(prevPos - n.pos);
+  // a subsurface exit node stores the outward normal, which stands in for wo and sets the surface up
+  // for the outside (the two-sided orientation applies there); other nodes face the vertex before
+  // them, with no two-sided re-orientation, as in the reference
+  bool exitNode = (n.flags & LN_SUBSURFACE_EXIT) != 0;
+  out.wo = exitNode ? normalize(n.nrm) : normalize(prevPos - n.pos);
   TriGeometry g;
   fetch_triangle(S, n.tri, n.b1, n.b2, g);
-  // no two-sided re-orientation here, as in the reference
   SurfaceTexels tx;
   sample_surface_texels(S, m, n.tri, n.b1, n.b2, tx);
-  SurfaceDef sd = make_surface(m, g.geometryNormal, g.normal, g.tangent, out.wo, false, (n.flags & LN_REGULARIZED) != 0, L,
+  SurfaceDef sd = make_surface(m, g.geometryNormal, g.normal, g.tangent, out.wo, exitNode, (n.flags & LN_REGULARIZED) != 0, L,
                                S.spec, &tx);
   BdptState rs = light_node_state(n, false);
   make_surface_context(out.ctx, sd, rs.current_ior(), rs.previous_ior(), out.wo, frontFace, L, true);
@@ -641,7 +760,7 @@ struct NodeSurfaceTable {
 // surface vertex with its surface `ls`; false when the connection carries nothing.
 YPT_HD bool connect_light_node(const SceneView& S, const FrameConstants& fc, const LightNode& n, uint32_t index,
                                const LightNodeSurface* ls, vec3 nextPos, bool secondaryTerminated, const Lambdas& L,
-                               ScatterEval& ev, float& pdfFwdMIS) {
+                               ScatterEval& ev, float& pdfFwdMIS, vec3 prevPos = vec3(0.f), vec3 prevNrm = vec3(0.f)) {
   ev.weight.set(0.f);
   ev.pdfFwd = 0.f;
   ev.pdfBwd = 0.f;
@@ -694,6 +813,14 @@ YPT_HD bool connect_light_node(const SceneView& S, const FrameConstants& fc, con
     return true;
   }
   vec3 wi = normalize(nextPos - n.pos);
+  if (n.flags & LN_SUBSURFACE_EXIT) {  // the exit lobe, and the entry point (prevPos) as the backward density
+    BdptState rs = light_node_state(n, false);
+    rs.flags = 0u;
+    ev = eval_subsurface_exit(S, ls->ctx, rs, S.inst[S.triInst[n.tri]].material, ls->wo, wi, frontFace, true, n.pos, n.nrm,
+                              prevPos, prevNrm);
+    pdfFwdMIS = ev.pdfFwd;
+    return !(ev.pdfBwd == 0.f || ev.weight.all_equal(0.f));
+  }
   BdptState rs = light_node_state(n, secondaryTerminated);
   ev = eval_and_pdfs(S, ls->ctx, rs, VOLUME_NONE, ls->wo, wi, frontFace, true, false);
   if (ev.dispersed) ev.weight.terminate_secondary();
@@ -710,6 +837,7 @@ struct CameraVertexCtx {  // CameraVertexContext (:877-896)
   float prevFwdMIS, curFwdMIS;
   uint32_t matId;
   bool frontFace;
+  bool isExit, prevIsExit;  // reached through the subsurface from the previous vertex (this one, the one before)
 };
 
 struct LightVertexCtx {  // LightVertexContext (:898-906)
@@ -728,16 +856,16 @@ struct ConnectionPdfs {
 YPT_HD float mis_connection(const SceneView& S, const FrameConstants& fc, uint32_t camIndex, const CameraVertexCtx& cv,
                             uint32_t lightIndex, const LightVertexCtx& lv, const LightNode& n, const ConnectionPdfs& p) {
   float riCam = cv.pdfRatioSum;
-  if (camIndex > 0) {
-    float pb = p.bwdCam * solid_angle_to_area(cv.prevPos - cv.pos, cv.prevNrm);
-    riCam = accumulate_ri(riCam, pb, cv.prevFwdMIS);
+  if (camIndex > 0) {  // from a subsurface exit the backward density is one of area already
+    float pb = cv.isExit ? p.bwdCam : p.bwdCam * solid_angle_to_area(cv.prevPos - cv.pos, cv.prevNrm);
+    riCam = accumulate_ri(riCam, pb, cv.prevFwdMIS, !cv.prevIsExit);
   }
   float toCamArea;
   if (lightIndex == 0 && infinite_node(n.flags))  // from a disk: planar density times the cosine
     toCamArea = p.fwdLight * fabsf(dot(cv.normalWS, infinite_node_direction(S, n)));
   else
     toCamArea = p.fwdLight * solid_angle_to_area(cv.pos - n.pos, cv.normalWS);
-  riCam = accumulate_ri(riCam, toCamArea, cv.curFwdMIS);
+  riCam = accumulate_ri(riCam, toCamArea, cv.curFwdMIS, !cv.isExit);
   float riLight = lv.riSum;
   if (lightIndex == 0) {
     float pick, posPdf, dirPdf, toLight;
@@ -760,11 +888,13 @@ YPT_HD float mis_connection(const SceneView& S, const FrameConstants& fc, uint32
   } else {
     bool prevPunctual = lightIndex == 1 && (lv.originFlags & LN_PUNCTUAL) != 0;
     bool prevEnv = lightIndex == 1 && (lv.originFlags & LN_ENV) != 0;
+    bool nodeExit = (n.flags & LN_SUBSURFACE_EXIT) != 0;
+    bool prevExit = (lv.prevFlags & LN_SUBSURFACE_EXIT) != 0;
     float pbl = p.bwdLight;
     if (prevPunctual) pbl = 0.f;
-    else if (!prevEnv) pbl = pbl * solid_angle_to_area(lv.prevPos - n.pos, lv.prevNrm);
-    riLight = accumulate_ri(riLight, pbl, lv.prevFwdMIS);
-    riLight = accumulate_ri(riLight, p.fwdCam * solid_angle_to_area(n.pos - cv.pos, n.nrm), n.pdfFwd);
+    else if (!prevEnv && !nodeExit) pbl = pbl * solid_angle_to_area(lv.prevPos - n.pos, lv.prevNrm);
+    riLight = accumulate_ri(riLight, pbl, lv.prevFwdMIS, !prevExit);
+    riLight = accumulate_ri(riLight, p.fwdCam * solid_angle_to_area(n.pos - cv.pos, n.nrm), n.pdfFwd, !nodeExit);
   }
   return 1.f / (1.f + riCam + riLight);
 }
@@ -777,7 +907,7 @@ YPT_HD float mis_camera_hits_light(const SceneView& S, uint32_t camIndex, uint32
   float pick, posPdf, dirPdf;
   pdf_sampling_light(S, tri, cv.normalWS, towardsPrevDir, pick, posPdf, dirPdf);
   float riCam = cv.pdfRatioSum;
-  riCam = accumulate_ri(riCam, dirPdf * solid_angle_to_area(towardsPrev, cv.prevNrm), cv.prevFwdMIS);
+  riCam = accumulate_ri(riCam, dirPdf * solid_angle_to_area(towardsPrev, cv.prevNrm), cv.prevFwdMIS, !cv.prevIsExit);
   riCam = accumulate_ri(riCam, pick * posPdf, cv.curFwdMIS);
   return 1.f / (1.f + riCam);
 }
@@ -787,7 +917,7 @@ YPT_HD float mis_camera_misses(const FrameConstants& fc, uint32_t camIndex, cons
   if (camIndex == 0) return 1.f;
   float riCam = cv.pdfRatioSum;
   float pdfEnvToLast = pdf_environment_position(fc) * fabsf(dot(cv.prevNrm, rayDir));
-  riCam = accumulate_ri(riCam, pdfEnvToLast, cv.prevFwdMIS);
+  riCam = accumulate_ri(riCam, pdfEnvToLast, cv.prevFwdMIS, !cv.prevIsExit);
   float pdfEnvDirection = environment_selection_pdf(fc) * pdf_environment_direction(-rayDir);
   riCam = accumulate_ri(riCam, pdfEnvDirection, cv.curFwdMIS);
   return 1.f / (1.f + riCam);
@@ -813,14 +943,17 @@ YPT_HD bool connection_prepare(const SceneView& S, const FrameConstants& fc, uin
   bool lightTerminated = (n.flags & LN_SECONDARY_TERMINATED) != 0;
   if (lightTerminated) rs.flags |= RS_SECONDARY_TERMINATED;
   ConnectionPdfs p;
-  ScatterEval ec = eval_and_pdfs(S, camCtx, rs, cv.matId, cv.wo, cv.wi, cv.frontFace, false, false);
+  ScatterEval ec = cv.isExit ? eval_subsurface_exit(S, camCtx, rs, cv.matId, cv.wo, cv.wi, cv.frontFace, false, cv.pos,
+                                                    cv.normalWS, cv.prevPos, cv.prevNrm)
+                             : eval_and_pdfs(S, camCtx, rs, cv.matId, cv.wo, cv.wi, cv.frontFace, false, false);
   if (ec.dispersed) ec.weight.terminate_secondary();
   if (ec.pdfFwd == 0.f || ec.weight.all_equal(0.f)) return false;
   Spectrum wCam = ec.weight;
   p.fwdCam = ec.pdfFwd;
   p.bwdCam = ec.pdfBwd;
   ScatterEval el;
-  if (!connect_light_node(S, fc, n, lightIndex, ls, cv.pos, (rs.flags & RS_SECONDARY_TERMINATED) != 0, L, el, p.fwdLight))
+  if (!connect_light_node(S, fc, n, lightIndex, ls, cv.pos, (rs.flags & RS_SECONDARY_TERMINATED) != 0, L, el, p.fwdLight,
+                          lv.prevPos, lv.prevNrm))
     return false;
   Spectrum wLight = el.weight;
   p.bwdLight = el.pdfBwd;
@@ -871,6 +1004,9 @@ struct CameraPath {
   uint32_t dim, seed;
   uint32_t k;      // camera vertex index of the ray in flight
   bool alive;
+  bool pendingExit;    // the next vertex is the exit point of the subsurface event just taken (no ray)
+  bool prevIsExit;     // the previous vertex was a subsurface exit
+  SurfacePoint exitP;
 };
 
 // extraDim: the workgroup's first thread drew the fallback wavelength from its own sampler when
@@ -900,12 +1036,16 @@ YPT_HD void camera_begin(const FrameConstants& fc, const uint32_t* point, uint32
   c.seed = rand.seed;
   c.k = 0;
   c.alive = fc.maxRayDepth > 0;
+  c.pendingExit = false;
+  c.prevIsExit = false;
 }
 
-template <class Sink, class Surfaces>
+// A vertex is the traced ray's hit or miss, or (c.pendingExit) the exit point of the subsurface
+// event taken at the previous vertex; `probe` traces the subsurface probe rays.
+template <class Sink, class Surfaces, class Probe>
 YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const uint32_t* point, CameraPath& c, bool hit,
                           const Hit& h, const LightNodePacked* nodes, uint32_t nodeCount, Surfaces& surfs, Sink& sink,
-                          FirstHitFeatures* fh) {
+                          FirstHitFeatures* fh, Probe& probe) {
   RandomSampler rand;
   rand.point = point;
   rand.dim = c.dim;
@@ -921,7 +1061,11 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
   cv.prevNrm = c.prevNrm;
   cv.pdfRatioSum = c.pdfRatioSum;
   cv.prevFwdMIS = c.prevFwdMIS;
-  if (!hit) {
+  const bool isExit = c.pendingExit;
+  c.pendingExit = false;
+  cv.isExit = isExit;
+  cv.prevIsExit = c.prevIsExit;
+  if (!hit && !isExit) {
     cv.curFwdMIS = c.pdfForwardCamera;
     float w = mis_camera_misses(fc, k, cv, c.rayDir);
     vec3 sky = (fc.flags & FRAME_FLAG_WHITE_FURNACE) ? vec3(1.f) : vabs(c.rayDir) * fc.envIntensityScale;
@@ -932,7 +1076,8 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
   if (k >= fc.regularizeAfterVertices) c.rs.flags |= RS_REGULARIZE;
   BdptVertex v;
   const MaterialGPU* m = nullptr;
-  bdpt_surface(S, c.rs, h, c.rayOrigin, c.rayDir, L, v, &m);
+  if (isExit) bdpt_surface_exit(S, c.rs, c.exitP, L, v, &m);
+  else bdpt_surface(S, c.rs, h, c.rayOrigin, c.rayDir, L, v, &m);
   if (k == 0 && fh) {
     SurfaceTexels tx;
     sample_surface_texels(S, *m, h.tri, h.b1, h.b2, tx);
@@ -943,12 +1088,17 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
   cv.wo = v.wo;
   cv.matId = v.matId;
   cv.normalWS = v.normalWS;
-  bool delta = is_delta_distribution(v.ctx.pre.ax, v.ctx.pre.ay, v.ctx.def);
-  cv.curFwdMIS = c.pdfForwardCamera < 0.f ? -1.f : c.pdfForwardCamera * solid_angle_to_area(v.pos - c.rayOrigin, v.normalWS);
-  Spectrum ab = c.rs.get_absorption();
-  if (!ab.all_equal(0.f)) cv.throughput = cv.throughput * transmittance(h.t, ab);
+  // a subsurface exit only has the exit lobe, and its edge density is one of area already
+  bool delta = !isExit && is_delta_distribution(v.ctx.pre.ax, v.ctx.pre.ay, v.ctx.def);
+  if (isExit) {
+    cv.curFwdMIS = c.pdfForwardCamera;
+  } else {
+    cv.curFwdMIS = c.pdfForwardCamera < 0.f ? -1.f : c.pdfForwardCamera * solid_angle_to_area(v.pos - c.rayOrigin, v.normalWS);
+    Spectrum ab = c.rs.get_absorption();
+    if (!ab.all_equal(0.f)) cv.throughput = cv.throughput * transmittance(h.t, ab);
+  }
   const SurfaceDef& sd = v.ctx.def;
-  if (!sd.emissive.all_equal(0.f)) {
+  if (!isExit && !sd.emissive.all_equal(0.f)) {  // emission at an exit was counted at the entry
     float w = mis_camera_hits_light(S, k, h.tri, cv);
     if (surface_two_sided(sd.flags) || dot(sd.geometryNormal, v.wo) > 0.f) {
       float profile = emission_profile(dot(v.normalWS, -c.rayDir), sd.emissionFocus);
@@ -982,7 +1132,7 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
         if (connection_prepare(S, fc, k, cv, v.ctx, n, j, ls, lv, c.rs, L, pc)) sink(c, pc);
       }
       lv.riSum = deferredLight;
-      deferredLight = accumulate_ri(deferredLight, n.pdfBwd, n.pdfFwd);
+      deferredLight = accumulate_ri(deferredLight, n.pdfBwd, n.pdfFwd, (n.flags & LN_SUBSURFACE_EXIT) == 0);
       lv.prevNrm = n.nrm;
       lv.prevPos = n.pos;
       lv.prevFwdMIS = n.pdfFwd;
@@ -991,26 +1141,41 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
     }
   }
   c.throughput = cv.throughput;
-  bool isDiffuse, isDelta;
-  vec3 wi = sample_scattering(S, v.ctx, rs0, isDiffuse, isDelta);
-  if (is_zero(wi)) { c.dim = rand.dim; return; }
-  if (isDiffuse) c.rs.flags |= RS_REGULARIZE;
-  ScatterEval ev = eval_and_pdfs(S, v.ctx, c.rs, v.matId, v.wo, wi, v.frontFace, false, isDelta);
+  ScatterEvent se;
+  sample_scattering_event(S, v, isExit, rs0, probe, se);
+  if (se.type == EV_NONE) { c.dim = rand.dim; return; }
+  if (se.isDiffuse) c.rs.flags |= RS_REGULARIZE;
+  ScatterEval ev;
+  bool bwdIsArea = false;
+  if (se.type == EV_SSS_ENTRY) {
+    ev = eval_subsurface_entry(S, v, se.exit);
+  } else if (se.type == EV_SSS_EXIT_LOBE) {
+    ev = eval_subsurface_exit(S, v.ctx, c.rs, v.matId, v.wo, se.wi, v.frontFace, false, v.pos, v.normalWS, c.prevPos, c.prevNrm);
+    bwdIsArea = true;
+  } else {
+    ev = eval_and_pdfs(S, v.ctx, c.rs, v.matId, v.wo, se.wi, v.frontFace, false, se.isDelta);
+  }
   Spectrum sampleWeight = ev.weight * safe_div(1.f, ev.pdfFwd);
   if (ev.dispersed) c.throughput.terminate_secondary();
   if (c.throughput.russian_roulette(rand.r1())) { c.dim = rand.dim; return; }
   if (k > 0) {
-    float pb = isDelta ? -1.f : ev.pdfBwd * solid_angle_to_area(c.rayOrigin - v.pos, c.prevNrm);
-    c.deferredCam = accumulate_ri(c.deferredCam, pb, c.prevFwdMIS);
+    float pb = se.isDelta ? -1.f : (bwdIsArea ? ev.pdfBwd : ev.pdfBwd * solid_angle_to_area(c.rayOrigin - v.pos, c.prevNrm));
+    c.deferredCam = accumulate_ri(c.deferredCam, pb, c.prevFwdMIS, !c.prevIsExit);
     c.pdfRatioSum = c.deferredCam;
   }
   c.prevNrm = v.normalWS;
   c.prevPos = v.pos;
   c.prevFwdMIS = cv.curFwdMIS;
-  c.pdfForwardCamera = isDelta ? -1.f : ev.pdfFwd;
+  c.prevIsExit = isExit;
+  c.pdfForwardCamera = se.isDelta ? -1.f : ev.pdfFwd;
   c.throughput = c.throughput * sampleWeight;
-  c.rayDir = normalize(wi);
-  c.rayOrigin = spawn_ray_origin(v.pos, v.normalWS, c.rayDir);
+  if (se.type == EV_SSS_ENTRY) {
+    c.pendingExit = true;
+    c.exitP = se.exit.p;
+  } else {
+    c.rayDir = normalize(se.wi);
+    c.rayOrigin = spawn_ray_origin(v.pos, v.normalWS, c.rayDir);
+  }
   c.dim = rand.dim;
   c.k = k + 1;
   c.alive = c.k < fc.maxRayDepth;
@@ -1037,8 +1202,8 @@ YPT_HD Spectrum trace_camera_path(const SceneView& S, const FrameConstants& fc, 
   sink.tr = &tr;
   while (c.alive) {
     Hit h;
-    bool hit = tr.closest(c.rayOrigin, c.rayDir, RAY_MIN_T, __builtin_inff(), h);
-    camera_vertex(S, fc, point, c, hit, h, nodes, nodeCount, surfs, sink, c.k == 0 ? fh : nullptr);
+    bool hit = !c.pendingExit && tr.closest(c.rayOrigin, c.rayDir, RAY_MIN_T, __builtin_inff(), h);
+    camera_vertex(S, fc, point, c, hit, h, nodes, nodeCount, surfs, sink, c.k == 0 ? fh : nullptr, tr);
   }
   return c.total;
 }
@@ -1104,13 +1269,14 @@ YPT_HD void splat_light_path(const SceneView& S, const FrameConstants& fc, const
   float deferred = accumulate_ri(0.f, n0.pdfBwd, n0.pdfFwd);
   vec3 prevPos = n0.pos, prevNrm = n0.nrm;
   float prevFwd = n0.pdfFwd;
+  uint32_t prevFlags = n0.flags;
   for (uint32_t i = 1; i < count; ++i) {
     LightNode n = unpack_node(nodes[i]);
     float u, v;
     if (project_to_uv(fc, lensPosVS, n.pos, u, v) && (n.flags & LN_DELTA) == 0) {
       ScatterEval ev;
       float unused;
-      if (connect_light_node(S, fc, n, i, surfs.get(S, n, i, prevPos, L), lensPosWS, false, L, ev, unused) &&
+      if (connect_light_node(S, fc, n, i, surfs.get(S, n, i, prevPos, L), lensPosWS, false, L, ev, unused, prevPos, prevNrm) &&
           !ev.weight.all_equal(0.f)) {
         // checkVisibility (:70-101): from the vertex to the lens point, t in [0.01, distance]
         vec3 fromTo = lensPosWS - n.pos;
@@ -1119,12 +1285,13 @@ YPT_HD void splat_light_path(const SceneView& S, const FrameConstants& fc, const
         if (visible) {
           bool prevEnv = i == 1 && (n0.flags & LN_ENV) != 0;
           bool prevPunctual = i == 1 && (n0.flags & LN_PUNCTUAL) != 0;
-          float pbPrev = prevEnv ? ev.pdfBwd : ev.pdfBwd * solid_angle_to_area(prevPos - n.pos, prevNrm);
+          bool nodeExit = (n.flags & LN_SUBSURFACE_EXIT) != 0;  // its backward density is one of area already
+          float pbPrev = (prevEnv || nodeExit) ? ev.pdfBwd : ev.pdfBwd * solid_angle_to_area(prevPos - n.pos, prevNrm);
           if (prevPunctual) pbPrev = 0.f;
           vec3 camToNode = n.pos - lensPosWS;
           float pdfCam = camera_dir_pdf(fc, camToNode);
-          float ri = accumulate_ri(riSum, pbPrev, prevFwd);
-          ri = accumulate_ri(ri, pdfCam * solid_angle_to_area(camToNode, n.nrm), n.pdfFwd);
+          float ri = accumulate_ri(riSum, pbPrev, prevFwd, (prevFlags & LN_SUBSURFACE_EXIT) == 0);
+          ri = accumulate_ri(ri, pdfCam * solid_angle_to_area(camToNode, n.nrm), n.pdfFwd, !nodeExit);
           float mis = 1.f / (1.f + ri);
           float geom = 1.f / fmax_(0.0001f, dot(camToNode, camToNode));
           float importance = safe_div(pdfCam, (float)headerCount);
@@ -1138,10 +1305,11 @@ YPT_HD void splat_light_path(const SceneView& S, const FrameConstants& fc, const
       }
     }
     riSum = deferred;
-    deferred = accumulate_ri(deferred, n.pdfBwd, n.pdfFwd);
+    deferred = accumulate_ri(deferred, n.pdfBwd, n.pdfFwd, (n.flags & LN_SUBSURFACE_EXIT) == 0);
     prevPos = n.pos;
     prevNrm = n.nrm;
     prevFwd = n.pdfFwd;
+    prevFlags = n.flags;
   }
 }
 

@@ -17,7 +17,8 @@
 namespace ypt {
 
 // COUNT: nodes visited, triangles tested and rays traced, for the algorithmic-byte figures
-template <int WIDTH, bool COUNT>
+// SSS: the scene has subsurface materials (probe rays); without, the probe query is compiled out
+template <int WIDTH, bool COUNT, bool SSS = false>
 struct GpuTracer {
   const Bvh2Node* nodes2;
   const Bvh8Node* nodes8;
@@ -68,6 +69,19 @@ struct GpuTracer {
     uint32_t id;
     return !trace(1, o, d, tmin, tmax, NO_HIT_ID, t, id, b1, b2);
   }
+  // subsurface probes: the closest hit among one instance's triangles
+  __device__ bool closest_in_range(vec3 o, vec3 d, float tmin, float tmax, uint32_t first, uint32_t count, Hit& h) {
+    if constexpr (!SSS) {
+      return false;
+    } else {
+      if (COUNT) rays++;
+      float t, b1, b2;
+      uint32_t tri;
+      bool hit = trace_closest_in_range<WIDTH>(nodes2, nodes8, tris, st, o, d, tmin, tmax, first, count, t, tri, b1, b2);
+      h.t = t; h.tri = hit ? tri : NO_HIT; h.b1 = b1; h.b2 = b2;
+      return hit;
+    }
+  }
 };
 
 struct BdptBuffers {
@@ -89,10 +103,10 @@ struct BdptBuffers {
   unsigned long long* work;  // counting launches: [0..2] camera subpaths, [3..5] light subpaths and splats
 };
 
-template <int WIDTH, bool COUNT>
-__device__ __forceinline__ GpuTracer<WIDTH, COUNT> make_tracer(const DevScene& S, uint32_t* lds, const BdptBuffers& B,
-                                                               uint32_t gid, uint32_t ldsDepth) {
-  GpuTracer<WIDTH, COUNT> tr;
+template <int WIDTH, bool COUNT, bool SSS = false>
+__device__ __forceinline__ GpuTracer<WIDTH, COUNT, SSS> make_tracer(const DevScene& S, uint32_t* lds, const BdptBuffers& B,
+                                                                    uint32_t gid, uint32_t ldsDepth) {
+  GpuTracer<WIDTH, COUNT, SSS> tr;
   tr.cnt.nodes = 0;
   tr.cnt.tris = 0;
   tr.rays = 0;
@@ -111,12 +125,12 @@ __device__ __forceinline__ GpuTracer<WIDTH, COUNT> make_tracer(const DevScene& S
   return tr;
 }
 
-template <int WIDTH, bool COUNT>
+template <int WIDTH, bool COUNT, bool SSS = false>
 __global__ __launch_bounds__(64) void k_bdpt_light(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, uint32_t ldsDepth) {
   extern __shared__ uint32_t lds[];
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= B.npaths) return;
-  GpuTracer<WIDTH, COUNT> tr = make_tracer<WIDTH, COUNT>(S, lds, B, p, ldsDepth);
+  GpuTracer<WIDTH, COUNT, SSS> tr = make_tracer<WIDTH, COUNT, SSS>(S, lds, B, p, ldsDepth);
   float lambda;
   uint32_t c = trace_light_path(S.sv, fc, point, p, tr, B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES, lambda);
   B.counts[p] = c;
@@ -165,7 +179,7 @@ __global__ void k_bdpt_node_surfaces(DevScene S, BdptBuffers B) {
 
 // 64 threads per workgroup = one 8x8 dispatch tile
 // MINW: waves per SIMD the register allocation must allow (1: up to 512 VGPRs, 2: 256)
-template <int WIDTH, bool COUNT, int MINW>
+template <int WIDTH, bool COUNT, int MINW, bool SSS = false>
 __global__ __launch_bounds__(64, MINW) void k_bdpt_camera(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, float* features,
                               uint32_t ldsDepth) {
   extern __shared__ uint32_t lds[];
@@ -191,7 +205,7 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_camera(FrameConstants fc, Dev
   uint32_t q = B.perm[dy * W + dx];
   uint32_t px = q & 0xFFFFu, py = q >> 16;
   uint32_t pix = py * W + px;
-  GpuTracer<WIDTH, COUNT> tr = make_tracer<WIDTH, COUNT>(S, lds, B, g * blockDim.x + t, ldsDepth);
+  GpuTracer<WIDTH, COUNT, SSS> tr = make_tracer<WIDTH, COUNT, SSS>(S, lds, B, g * blockDim.x + t, ldsDepth);
   FirstHitFeatures fh;
   fh.hit = 0u;
   NodeSurfaceTable surfs;
@@ -261,7 +275,8 @@ struct CamSoA {
   float4* thr;         // throughput
   float4* total;       // radiance so far
   uint4* st0;          // dim, seed, state flags, vertex index
-  uint4* st1;          // alive, volumes, light subpath (0xFFFFFFFF: none), its vertex count
+  uint4* st1;          // alive | previous vertex was a subsurface exit << 1, volumes, light subpath
+                       // (0xFFFFFFFF: none), its vertex count
   float4* prev;        // previous vertex position, its forward pdf
   float4* prevN;       // previous vertex normal, pdf ratio sum
   float* deferred;     // deferred ratio sum of the camera subpath
@@ -285,7 +300,9 @@ __device__ __forceinline__ void load_cam(const CamSoA& C, uint32_t s, CameraPath
   c.pdfRatioSum = pn.w;
   c.deferredCam = C.deferred[s];
   c.dim = u0.x; c.seed = u0.y; c.rs.flags = u0.z; c.k = u0.w;
-  c.alive = u1.x != 0u;
+  c.alive = (u1.x & 1u) != 0u;
+  c.prevIsExit = (u1.x & 2u) != 0u;
+  c.pendingExit = false;  // an exit vertex is shaded in the launch that took the subsurface event
   c.rs.n = u1.y;
   lp = u1.z;
   count = u1.w;
@@ -313,7 +330,7 @@ __device__ __forceinline__ void store_cam(const CamSoA& C, uint32_t s, const Cam
   C.prevN[s] = make_float4(c.prevNrm.x, c.prevNrm.y, c.prevNrm.z, c.pdfRatioSum);
   C.deferred[s] = c.deferredCam;
   C.st0[s] = make_uint4(c.dim, c.seed, c.rs.flags, c.k);
-  C.st1[s] = make_uint4(c.alive ? 1u : 0u, c.rs.n, lp, count);
+  C.st1[s] = make_uint4((c.alive ? 1u : 0u) | (c.prevIsExit ? 2u : 0u), c.rs.n, lp, count);
   if (volumes) {
     C.volIor[s] = make_float4(c.rs.ior[0], c.rs.ior[1], c.rs.ior[2], c.rs.ior[3]);
     C.volMat[s] = make_uint4(c.rs.mat[0], c.rs.mat[1], c.rs.mat[2], c.rs.mat[3]);
@@ -395,7 +412,10 @@ struct QueueSink {  // connection rays into the shadow queue, one atomic per wav
 // grid-stride over the ray queue with a loop bound uniform per workgroup (see k_shade);
 // first: the first vertex, whose queue holds every slot; the slots are then shaded in slot (tile)
 // order rather than the screen order of the primary-ray queue
-template <int MINW>
+// SSS: the scene has subsurface materials; probes then run here (stacks in Q.sssStack) and the exit
+// vertex of a subsurface event is shaded in the same launch, so a path can queue the connections of
+// two vertices
+template <int MINW, bool SSS = false>
 __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B,
                                                           CamSoA C, Queues Q, int qsel, int first, float* features) {
   if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;
@@ -425,7 +445,18 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
       FirstHitFeatures fh;
       fh.hit = 0u;
       bool first = c.k == 0;
-      camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh);  // written below only when requested
+      if constexpr (SSS) {
+        GpuProbe pr;
+        pr.nodes2 = S.nodes2; pr.nodes8 = S.nodes8; pr.tris = S.tris;
+        pr.st.lds = nullptr; pr.st.ovf = Q.sssStack; pr.st.block = 64; pr.st.lane = threadIdx.x; pr.st.ldsDepth = 0;
+        pr.st.ovfStride = stride; pr.st.gid = blockIdx.x * blockDim.x + threadIdx.x; pr.st.sp = 0;
+        camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, pr);
+        // the exit point of a subsurface event is the next vertex, with no ray to trace: take it now
+        if (c.pendingExit && c.alive) camera_vertex(S.sv, fc, point, c, false, h, nodes, count, surfs, sink, nullptr, pr);
+      } else {
+        NoSubsurface none;
+        camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none);  // features written below only when requested
+      }
       store_cam(C, slot, c, lp, count, nvolIn > 0 || c.rs.n > 0);
       cont = c.alive;
       if (features && first && fh.hit) {

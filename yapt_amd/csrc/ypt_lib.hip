@@ -637,7 +637,8 @@ void ensure_bdpt(ypt_integrator* h) {
     h->cOrg.ensure(npix); h->cDir.ensure(npix); h->cThr.ensure(npix); h->cTotal.ensure(npix);
     h->cPrev.ensure(npix); h->cPrevN.ensure(npix); h->cDeferred.ensure(npix);
     h->cSt0.ensure(npix); h->cSt1.ensure(npix); h->cVolIor.ensure(npix); h->cVolMat.ensure(npix); h->cVolAbs.ensure((size_t)npix * 4);
-    size_t nsh = (size_t)npix * h->fc.lightPathVertices;
+    // a subsurface entry and its exit are shaded in one launch: twice the connections per path
+    size_t nsh = (size_t)npix * h->fc.lightPathVertices * (h->res->subsurface ? 2u : 1u);
     h->cSOrg.ensure(nsh); h->cSDir.ensure(nsh); h->cSContrib.ensure(nsh); h->cSSlot.ensure(nsh);
     ensure_slots(h, npix);
     ensure_overflow(h);
@@ -676,7 +677,9 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
       hipLaunchKernelGGL((k_extend_p<WIDTH, COUNT>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r);
     else
       hipLaunchKernelGGL((k_extend_p<WIDTH, false, false>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r);
-    if (h->bdptWaves == 1)
+    if (h->res->subsurface)  // probes in the shading launch, within the stack buffer's 32 workgroups per CU
+      hipLaunchKernelGGL((k_bdpt_cshade<1, true>), dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
+    else if (h->bdptWaves == 1)
       hipLaunchKernelGGL(k_bdpt_cshade<1>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     else if (h->bdptWaves == 2)
       hipLaunchKernelGGL(k_bdpt_cshade<2>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
@@ -707,7 +710,10 @@ void bdpt_light_stage(ypt_integrator* h, const DevScene& S, const uint32_t* poin
   uint32_t d = (uint32_t)h->ldsDepth;
   size_t lds = (size_t)h->ldsDepth * 64 * 4;
   mark_span(h, ls, ev);
-  hipLaunchKernelGGL((k_bdpt_light<WIDTH, COUNT>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, ls, fc, S, point, B, d);
+  if (h->res->subsurface)
+    hipLaunchKernelGGL((k_bdpt_light<WIDTH, COUNT, true>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, ls, fc, S, point, B, d);
+  else
+    hipLaunchKernelGGL((k_bdpt_light<WIDTH, COUNT>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, ls, fc, S, point, B, d);
   hipLaunchKernelGGL(k_bdpt_node_surfaces, dim3(blocks_for((size_t)B.npaths * BDPT_MAX_LIGHT_VERTICES, 256)), dim3(256), 0, ls, S, B);
   mark_span(h, ls, ev);
   hipLaunchKernelGGL(k_bdpt_headers, dim3(1), dim3(1024), 0, ls, B);
@@ -723,6 +729,9 @@ void bdpt_camera_stage(ypt_integrator* h, const DevScene& S, const uint32_t* poi
   size_t lds = (size_t)h->ldsDepth * 64 * 4;
   mark_span(h, s, ev);
   if (h->bdptWavefront) bdpt_camera_wavefront<WIDTH, COUNT>(h, S, point, B);
+  else if (h->res->subsurface)
+    hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 1, true>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
+                       h->writeFeatures ? h->features.p : (float*)nullptr, d);
   else if (h->bdptWaves == 2)
     hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 2>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
                        h->writeFeatures ? h->features.p : (float*)nullptr, d);
