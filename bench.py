@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--passes-in-flight", type=int, default=0, help="sample passes per wavefront batch; 0 = tuned")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the checker sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--spectral-extensions", action="store_true",
+                   help="BK7 Sellmeier glass and Planck-spectrum emitters instead of the reference's Cauchy and RGB x D65")
     p.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
                    help="launch option over the tuned choice (repeatable)")
     return p.parse_args()
@@ -81,7 +83,8 @@ def main():
     a.depth = a.depth or (8 if bdpt else 12)
     light_vertices = 9 if bdpt else 16
     t0 = time.time()
-    scene, cam = scenes.random_shapes(n_shapes=a.shapes, seed=2, emissive_triangles=8 if bdpt else 0)
+    scene, cam = scenes.random_shapes(n_shapes=a.shapes, seed=2, emissive_triangles=8 if bdpt else 0,
+                                      spectral_extensions=a.spectral_extensions)
     frame = camera.make_frame(cam, a.width, a.height, max_depth=a.depth, light_path_vertices=light_vertices)
     t_scene = time.time() - t0
 

@@ -76,6 +76,13 @@ typedef struct ypt_material {
    * and the UV scale of each (the texture transform scale) */
   int32_t albedo_tex, normal_tex, orm_tex, emissive_tex;
   float albedo_tex_scale[2], normal_tex_scale[2], orm_tex_scale[2], emissive_tex_scale[2];
+  /* Extensions with no counterpart in the reference (0 = off, the reference's behaviour):
+   * blackbody_kelvin > 0 makes the emission a Planck spectrum of that temperature, scaled to 1 at
+   * 560 nm times the largest emissive component; sellmeier_b[0] > 0 makes the dispersive IOR
+   * n^2(lambda) = 1 + sum B_i lambda^2 / (lambda^2 - C_i) (lambda and sqrt(C_i) in micrometres)
+   * instead of Cauchy's A + B / lambda^2. */
+  float blackbody_kelvin;
+  float sellmeier_b[3], sellmeier_c[3];
 } ypt_material;
 
 /* An RGBA8 texture, sampled bilinearly with wrapping at mip level 0 (g_colorSampler, SampleLevel 0).

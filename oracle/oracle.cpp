@@ -638,6 +638,7 @@ int oracle_render(const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integr
 // 24 accumulate_ri(sum, nom, denom, connectable)  25 pdf_ratio(nom, denom)
 // 26 burley_profile_area(r, d)  27 burley_sample_radius(u, d)  28 burley_d(radius, albedo)
 // 29 pdf_subsurface_exit(albedo (grey), radius (grey), to.xyz on the plane y = 0 from the origin, normal +y)
+// 30 sellmeier_ior(lambda nm, B1, B2, B3, C1, C2, C3)  31 planck_relative(lambda nm, kelvin)
 extern "C" int oracle_eval(int fn, const float* in, uint32_t n, float* out) {
   MSLuts L;
   L.base = tables().msl.data();
@@ -677,6 +678,8 @@ extern "C" int oracle_eval(int fn, const float* in, uint32_t n, float* out) {
       case 28: o[0] = burley_d(a[0], a[1]); break;
       case 29: o[0] = pdf_subsurface_exit(Spectrum(a[0]), Spectrum(a[1]), vec3(0.f), vec3(0.f, 1.f, 0.f), vec3(a[2], a[3], a[4]),
                                           vec3(a[5], a[6], a[7])); break;
+      case 30: o[0] = sellmeier_ior(a + 1, a + 4, a[0]); break;
+      case 31: o[0] = planck_relative(a[0], a[1]); break;
       default: g_err = "unknown function"; return YPT_E_INVALID;
     }
   }

@@ -106,7 +106,7 @@ FN = {"d_ggx": 0, "pdf_wm_ggx": 1, "sample_wm_ggx": 2, "fresnel_dielectric": 3, 
       "hash_pair": 13, "owen_scramble": 14, "uint_to_float01": 15, "ss_dir": 16, "ss_avg": 17, "ssms_dir": 18,
       "sheen_dir": 19, "tr_dir": 20, "emission_profile": 21, "f32_to_f16": 22, "f16_to_f32": 23, "accumulate_ri": 24,
       "pdf_ratio": 25, "burley_profile_area": 26, "burley_sample_radius": 27, "burley_d": 28,
-      "pdf_subsurface_exit_plane": 29}
+      "pdf_subsurface_exit_plane": 29, "sellmeier_ior": 30, "planck_relative": 31}
 
 
 def evaluate(name, args):

@@ -45,6 +45,7 @@ class Material(ctypes.Structure):
         ("emissive_tex", ctypes.c_int32),
         ("albedo_tex_scale", ctypes.c_float * 2), ("normal_tex_scale", ctypes.c_float * 2),
         ("orm_tex_scale", ctypes.c_float * 2), ("emissive_tex_scale", ctypes.c_float * 2),
+        ("blackbody_kelvin", ctypes.c_float), ("sellmeier_b", F3), ("sellmeier_c", F3),
     ]
 
 

@@ -28,7 +28,9 @@ struct MaterialGPU {
   uint32_t flags;
   uint32_t emissiveNonZero;
   // linear RGB: the first-hit feature buffers, and the factors textures multiply
-  float albedoRGB[3], emissiveRGB[3], sheenRGB[3], pad0[3];
+  float albedoRGB[3], emissiveRGB[3], sheenRGB[3];
+  float blackbodyK;                     // > 0: Planck emission spectrum (an extension)
+  float sellmeierB[3], sellmeierC[3];   // sellmeierB[0] > 0: Sellmeier dispersion (an extension)
   // textures (albedo, normal, occlusion-roughness-metalness, emissive): index or -1, UV scale
   int32_t tex[4];
   float texScale[8];
@@ -48,7 +50,7 @@ struct SurfaceDef {
   Spectrum specular, albedo, absorption, emissive, sheenColor;
   vec3 coatingNormal, baseNormal, geometryNormal, tangent;
   float transparency, metalness, dielectricIOR, roughness;
-  float cauchyA, cauchyB, specularAmount, clearCoatAmount;
+  float dispersedIOR, specularAmount, clearCoatAmount;  // dispersedIOR: at the hero wavelength
   float clearCoatIOR, clearCoatRoughness, anisotropy, anisotropyRotation;
   float thinFilmThicknessNM, sheenRoughness, sheenAmount, emissionFocus;
   float subsurfaceAmount, diffOcclusion, specOcclusion;
@@ -79,6 +81,34 @@ YPT_HD vec3 orient_to_same_side(vec3 wo, vec3 n) { return dot(wo, n) < 0.f ? -n 
 YPT_HD float refractive_index_for_wavelength(float A, float B, float lambdaNM) {  // Cauchy, :503-511
   float um = lambdaNM * 0.001f;
   return A + (B / (um * um));
+}
+
+// Sellmeier's equation, n^2 = 1 + sum B_i l^2 / (l^2 - C_i) with l in micrometres: the dispersion the
+// benchmark glass names (BK7), an extension next to the reference's Cauchy form
+YPT_HD float sellmeier_ior(const float B[3], const float C[3], float lambdaNM) {
+  float l2 = lambdaNM * 0.001f * (lambdaNM * 0.001f);
+  float n2 = 1.f + B[0] * l2 / (l2 - C[0]) + B[1] * l2 / (l2 - C[1]) + B[2] * l2 / (l2 - C[2]);
+  return sqrtf(fmax_(n2, 1e-6f));
+}
+
+// the IOR a dispersive material has at the hero wavelength
+YPT_HD float material_dispersed_ior(const MaterialGPU& m, float heroNM) {
+  return m.sellmeierB[0] > 0.f ? sellmeier_ior(m.sellmeierB, m.sellmeierC, heroNM)
+                               : refractive_index_for_wavelength(m.cauchyA, m.cauchyB, heroNM);
+}
+
+// Planck's law relative to its value at 560 nm: (560 / l)^5 (e^(c2 / (560 nm T)) - 1) / (e^(c2 / (l T)) - 1)
+YPT_HD float planck_relative(float lambdaNM, float kelvin) {
+  const float c2 = 1.4387769e7f;  // h c / k in nm K
+  float a = 560.f / lambdaNM;
+  float num = yexp(c2 / (560.f * kelvin)) - 1.f;
+  float den = yexp(c2 / (lambdaNM * kelvin)) - 1.f;
+  return a * a * a * a * a * (num / den);
+}
+YPT_HD Spectrum blackbody_spectrum(const Lambdas& L, float kelvin, float strength) {
+  Spectrum r;
+  for (int i = 0; i < 4; ++i) r.s[i] = strength * planck_relative(L.l[i], kelvin);
+  return r;
 }
 
 // thin film (materialsCommon.hlsl:515-568)
@@ -142,8 +172,7 @@ YPT_HD SurfaceDef make_surface(const MaterialGPU& m, vec3 geometryNormal, vec3 s
   s.clearCoatAmount = m.clearCoatAmount;
   s.clearCoatIOR = m.clearCoatIOR;
   s.clearCoatRoughness = m.clearCoatRoughness;
-  s.cauchyA = m.cauchyA;
-  s.cauchyB = m.cauchyB;
+  s.dispersedIOR = has_dispersion(m.flags) ? material_dispersed_ior(m, L.l[0]) : 0.f;
   s.anisotropy = m.anisotropy;
   s.anisotropyRotation = m.anisotropyRotation;
   s.thinFilmThicknessNM = m.thinFilmThicknessNM;
@@ -212,7 +241,15 @@ YPT_HD SurfaceDef make_surface(const MaterialGPU& m, vec3 geometryNormal, vec3 s
   s.specular = spectrum_bounded(m.specular, L);
   s.albedo = spectrum_bounded(albedoCol, L);
   s.absorption = spectrum_unbounded_no_illum(m.absorption, L);
-  s.emissive = spectrum_unbounded(emissiveCol, L, T);
+  if (m.blackbodyK > 0.f) {  // Planck spectrum scaled by the largest (textured) emissive component
+    float e0 = m.emissiveRGB[0], e1 = m.emissiveRGB[1], e2 = m.emissiveRGB[2];
+    if (tx && (tx->mask & (1u << TEX_EMISSIVE))) {
+      e0 *= tx->v[TEX_EMISSIVE][0]; e1 *= tx->v[TEX_EMISSIVE][1]; e2 *= tx->v[TEX_EMISSIVE][2];
+    }
+    s.emissive = blackbody_spectrum(L, m.blackbodyK, fmax_(fmax_(e0, e1), e2));
+  } else {
+    s.emissive = spectrum_unbounded(emissiveCol, L, T);
+  }
   s.sheenColor = spectrum_bounded(m.sheenColor, L);
   return s;
 }
@@ -404,9 +441,7 @@ YPT_HD Spectrum conductor_eval(const MSLuts& Lu, bool rec, const Spectrum& c0, c
   return (w + ms) * fabsf(wi.y);
 }
 
-YPT_HD float dispersed_ior(const SurfaceDef& s, const Lambdas& L) {
-  return refractive_index_for_wavelength(s.cauchyA, s.cauchyB, L.l[0]);
-}
+YPT_HD float dispersed_ior(const SurfaceDef& s, const Lambdas&) { return s.dispersedIOR; }
 
 // getSampleDirectionOSForLayer (materialSample.hlsl:148-262)
 YPT_HD vec3 sample_layer_direction(const SurfaceContext& c, const MSLuts& Lu, int layer, float randSel, float s0, float s1,

@@ -221,11 +221,16 @@ YPT_HD uint32_t sample_light_index(const LightGPU* lights, uint32_t count, float
 // emission of material m at a point of triangle tri (fetchSurfaceMaterialParameters +
 // modifySurfaceEmissionWithTexture, materialsCommon.hlsl:420-428)
 YPT_HD Spectrum surface_emission(const SceneView& S, const MaterialGPU& m, uint32_t tri, float b1, float b2, const Lambdas& L) {
-  if (m.tex[TEX_EMISSIVE] < 0) return spectrum_unbounded(m.emissive, L, S.spec);
+  if (m.tex[TEX_EMISSIVE] < 0) {
+    if (m.blackbodyK > 0.f) return blackbody_spectrum(L, m.blackbodyK, fmax_(fmax_(m.emissiveRGB[0], m.emissiveRGB[1]), m.emissiveRGB[2]));
+    return spectrum_unbounded(m.emissive, L, S.spec);
+  }
   float u, v, e[4];
   surface_uv(S, tri, b1, b2, u, v);
   sample_texture(S, (uint32_t)m.tex[TEX_EMISSIVE], u * m.texScale[2 * TEX_EMISSIVE], v * m.texScale[2 * TEX_EMISSIVE + 1], e);
-  return rgb_unbounded_spectrum(S, vec3(m.emissiveRGB[0] * e[0], m.emissiveRGB[1] * e[1], m.emissiveRGB[2] * e[2]), L);
+  vec3 rgb(m.emissiveRGB[0] * e[0], m.emissiveRGB[1] * e[1], m.emissiveRGB[2] * e[2]);
+  if (m.blackbodyK > 0.f) return blackbody_spectrum(L, m.blackbodyK, fmax_(fmax_(rgb.x, rgb.y), rgb.z));
+  return rgb_unbounded_spectrum(S, rgb, L);
 }
 
 // samplePunctualLightIndex / punctualLightSelectionPDF (lightSampling.hlsl:44-57): the punctual

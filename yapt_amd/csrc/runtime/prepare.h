@@ -93,6 +93,12 @@ inline MaterialGPU convert_material(Rgb2Spec& r2s, const ypt_material& in, uint3
   m.thinFilmThicknessNM = p.thin_film_thickness_nm;
   m.cauchyA = p.cauchy_coeffs[0];
   m.cauchyB = p.cauchy_coeffs[1];
+  // extensions: a blackbody below 500 K emits nothing visible in 380-700 nm at float precision
+  m.blackbodyK = p.blackbody_kelvin > 0.f ? std::max(p.blackbody_kelvin, 500.f) : 0.f;
+  for (int k = 0; k < 3; ++k) {
+    m.sellmeierB[k] = p.sellmeier_b[0] > 0.f ? p.sellmeier_b[k] : 0.f;
+    m.sellmeierC[k] = p.sellmeier_b[0] > 0.f ? p.sellmeier_c[k] : 0.f;
+  }
   m.sheenAmount = p.sheen_amount;
   m.emissionFocus = p.emission_focus;
   m.alphaCutoff = p.alpha_cutoff;

@@ -492,6 +492,18 @@ class Material:
     def setSubsurfaceRadiusScale(self, v): self._set("subsurface_radius_scale", v)
     def getSubsurfaceRadiusScale(self): return self._p.subsurface_radius_scale
 
+    # extensions with no counterpart in the reference (0 turns them off)
+    def setBlackbodyTemperature(self, kelvin): self._set("blackbody_kelvin", kelvin)
+    def getBlackbodyTemperature(self): return self._p.blackbody_kelvin
+
+    def setSellmeierCoefficients(self, b, c):
+        self._p.sellmeier_b[:] = [float(x) for x in vec3(b)]
+        self._p.sellmeier_c[:] = [float(x) for x in vec3(c)]
+        self._touch()
+
+    def getSellmeierCoefficients(self):
+        return vec3(list(self._p.sellmeier_b)), vec3(list(self._p.sellmeier_c))
+
     def setCauchysCoefficients(self, v):
         self._p.cauchy_coeffs[:] = [float(c) for c in vec2(v)]
         self._touch()

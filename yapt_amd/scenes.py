@@ -51,7 +51,13 @@ def blank_material():
     m.albedo_tex = m.normal_tex = m.orm_tex = m.emissive_tex = -1   # TEX_UNBOUND_INDEX
     for sc in (m.albedo_tex_scale, m.normal_tex_scale, m.orm_tex_scale, m.emissive_tex_scale):
         sc[:] = (1.0, 1.0)
+    m.blackbody_kelvin = 0.0                        # extensions: off
+    m.sellmeier_b[:] = (0.0, 0.0, 0.0)
+    m.sellmeier_c[:] = (0.0, 0.0, 0.0)
     return m
+
+
+BK7_SELLMEIER = ((1.03961212, 0.231792344, 1.01046945), (0.00600069867, 0.0200179144, 103.560653))
 
 
 def _pow22(c):
@@ -360,9 +366,12 @@ def blackbody_rgb(kelvin):
 
 
 def random_shapes(n_shapes=2000, seed=2, cube=20.0, n_lights=4, light_emission=80.0, sphere_res=(20, 20),
-                  torus_res=(32, 12), emissive_triangles=0, triangle_emission=200.0):
+                  torus_res=(32, 12), emissive_triangles=0, triangle_emission=200.0, spectral_extensions=False):
     """~1M-triangle random-shapes scene (n_shapes=2000, seed 2). emissive_triangles > 0 adds that many
-    seeded two-sided emissive triangles (edge ~1.5) coloured as blackbodies of 2700-9000 K."""
+    seeded two-sided emissive triangles (edge ~1.5) coloured as blackbodies of 2700-9000 K.
+    spectral_extensions: the glass disperses by BK7's Sellmeier coefficients instead of Cauchy's, the
+    emissive triangles emit Planck spectra of their temperatures and the area lights one of 6500 K
+    (the benchmark text's materials; the reference's own are the default)."""
     rng = np.random.default_rng(seed)
     b = MeshBuilder()
     templates = [sphere_template(*sphere_res), box_template(), torus_template(*torus_res)]
@@ -377,6 +386,8 @@ def random_shapes(n_shapes=2000, seed=2, cube=20.0, n_lights=4, light_emission=8
             m = preset_material("GLASS")
             m.material_mask = _lib.YPT_MATERIAL_DISPERSION
             m.roughness = float(rng.uniform(0.0, 0.3))
+            if spectral_extensions:
+                m.sellmeier_b[:], m.sellmeier_c[:] = BK7_SELLMEIER
         else:
             metal = rng.random() < 0.3
             m = preset_material("METAL_SILVER" if metal else "PLASTIC")
@@ -401,6 +412,8 @@ def random_shapes(n_shapes=2000, seed=2, cube=20.0, n_lights=4, light_emission=8
     lm.specular_amount = 0.0
     lm.roughness = 1.0
     lm.emissive[:] = (light_emission,) * 3
+    if spectral_extensions:
+        lm.blackbody_kelvin = 6500.0
     light = b.material(lm)
     side = cube * 0.4
     for i in range(n_lights):
@@ -415,7 +428,10 @@ def random_shapes(n_shapes=2000, seed=2, cube=20.0, n_lights=4, light_emission=8
             em.specular_amount = 0.0
             em.roughness = 1.0
             em.material_mask = _lib.YPT_MATERIAL_TWO_SIDED
-            em.emissive[:] = tuple(float(v) for v in blackbody_rgb(er.uniform(2700.0, 9000.0)) * triangle_emission)
+            kelvin = er.uniform(2700.0, 9000.0)
+            em.emissive[:] = tuple(float(v) for v in blackbody_rgb(kelvin) * triangle_emission)
+            if spectral_extensions:
+                em.blackbody_kelvin = float(kelvin)
             mid = b.material(em)
             c = er.uniform(-cube / 2, cube / 2, size=3)
             e1, e2 = er.normal(size=3), er.normal(size=3)
