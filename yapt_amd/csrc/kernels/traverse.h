@@ -52,21 +52,43 @@ __device__ __forceinline__ TriRecord load_tri(const TriRecord* tris, uint32_t k)
   return r;
 }
 
+// one triangle record against the closest hit so far
+template <bool ALPHA, bool FILTER>
+__device__ __forceinline__ void tri_closest(const TriSet& tris, uint32_t k, const RayPre& rp, float tmin, float& best,
+                                            uint32_t& bestId, float& bb1, float& bb2, uint32_t lo, uint32_t range) {
+  TriRecord r = load_tri(tris.recs, k);
+  if (FILTER && r.id - lo >= range) return;  // only triangles [lo, lo + range)
+  float t, b1, b2;
+  if (intersect_triangle(rp, vec3(r.v0[0], r.v0[1], r.v0[2]), vec3(r.v1[0], r.v1[1], r.v1[2]), vec3(r.v2[0], r.v2[1], r.v2[2]),
+                         tmin, best, t, b1, b2) &&
+      hit_closer(t, r.id, best, bestId) && (!ALPHA || r.alpha == 0u || alpha_test_passes(tris.alpha, r.alpha - 1u, b1, b2))) {
+    best = t; bestId = r.id; bb1 = b1; bb2 = b2;
+  }
+}
+
+// true when triangle record k is not `target` and is hit closer than (targetT, target)
+template <bool ALPHA>
+__device__ __forceinline__ bool tri_occludes(const TriSet& tris, uint32_t k, const RayPre& rp, float tmin, float targetT,
+                                             uint32_t target) {
+  TriRecord r = load_tri(tris.recs, k);
+  if (r.id == target) return false;
+  float t, b1, b2;
+  return intersect_triangle(rp, vec3(r.v0[0], r.v0[1], r.v0[2]), vec3(r.v1[0], r.v1[1], r.v1[2]), vec3(r.v2[0], r.v2[1], r.v2[2]),
+                            tmin, targetT, t, b1, b2) &&
+         hit_closer(t, r.id, targetT, target) && (!ALPHA || r.alpha == 0u || alpha_test_passes(tris.alpha, r.alpha - 1u, b1, b2));
+}
+
+__device__ __forceinline__ uint32_t leaf_first(uint32_t ref) { return ref & 0x07FFFFFFu; }
+__device__ __forceinline__ uint32_t leaf_count(uint32_t ref) { return ((ref >> 27) & 15u) + 1u; }
+
 template <bool COUNT, bool ALPHA = true, bool FILTER = false>
 __device__ __forceinline__ void leaf_closest(const TriSet& tris, uint32_t ref, const RayPre& rp, float tmin, float& best,
                                              uint32_t& bestId, float& bb1, float& bb2, TraversalCounters& cnt,
                                              uint32_t lo = 0u, uint32_t range = 0u) {
-  uint32_t first = ref & 0x07FFFFFFu, count = ((ref >> 27) & 15u) + 1u;
+  uint32_t first = leaf_first(ref), count = leaf_count(ref);
   for (uint32_t k = first; k < first + count; ++k) {
-    TriRecord r = load_tri(tris.recs, k);
     if (COUNT) cnt.tris++;
-    if (FILTER && r.id - lo >= range) continue;  // only triangles [lo, lo + range)
-    float t, b1, b2;
-    if (intersect_triangle(rp, vec3(r.v0[0], r.v0[1], r.v0[2]), vec3(r.v1[0], r.v1[1], r.v1[2]), vec3(r.v2[0], r.v2[1], r.v2[2]),
-                           tmin, best, t, b1, b2) &&
-        hit_closer(t, r.id, best, bestId) && (!ALPHA || r.alpha == 0u || alpha_test_passes(tris.alpha, r.alpha - 1u, b1, b2))) {
-      best = t; bestId = r.id; bb1 = b1; bb2 = b2;
-    }
+    tri_closest<ALPHA, FILTER>(tris, k, rp, tmin, best, bestId, bb1, bb2, lo, range);
   }
 }
 
@@ -74,16 +96,10 @@ __device__ __forceinline__ void leaf_closest(const TriSet& tris, uint32_t ref, c
 template <bool COUNT, bool ALPHA = true>
 __device__ __forceinline__ bool leaf_occluded(const TriSet& tris, uint32_t ref, const RayPre& rp, float tmin, float targetT,
                                               uint32_t target, TraversalCounters& cnt) {
-  uint32_t first = ref & 0x07FFFFFFu, count = ((ref >> 27) & 15u) + 1u;
+  uint32_t first = leaf_first(ref), count = leaf_count(ref);
   for (uint32_t k = first; k < first + count; ++k) {
-    TriRecord r = load_tri(tris.recs, k);
     if (COUNT) cnt.tris++;
-    if (r.id == target) continue;
-    float t, b1, b2;
-    if (intersect_triangle(rp, vec3(r.v0[0], r.v0[1], r.v0[2]), vec3(r.v1[0], r.v1[1], r.v1[2]), vec3(r.v2[0], r.v2[1], r.v2[2]),
-                           tmin, targetT, t, b1, b2) &&
-        hit_closer(t, r.id, targetT, target) && (!ALPHA || r.alpha == 0u || alpha_test_passes(tris.alpha, r.alpha - 1u, b1, b2)))
-      return true;
+    if (tri_occludes<ALPHA>(tris, k, rp, tmin, targetT, target)) return true;
   }
   return false;
 }
@@ -128,15 +144,15 @@ __device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriSet& t
   uint32_t c0 = __float_as_uint(qc.x), c1 = __float_as_uint(qc.y);
   bool h0 = c0 != BVH_EMPTY && slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), r.o, r.inv, r.tmin, r.best, tn0);
   bool h1 = c1 != BVH_EMPTY && slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), r.o, r.inv, r.tmin, r.best, tn1);
-  if (h0 && (c0 & BVH_LEAF_BIT)) {
-    if (MODE == 0) leaf_closest<COUNT, ALPHA, FILTER>(tris, c0, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt, r.lo, r.range);
-    else if (leaf_occluded<COUNT, ALPHA>(tris, c0, r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
-    h0 = false;
-  }
-  if (h1 && (c1 & BVH_LEAF_BIT)) {
-    if (MODE == 0) leaf_closest<COUNT, ALPHA, FILTER>(tris, c1, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, cnt, r.lo, r.range);
-    else if (leaf_occluded<COUNT, ALPHA>(tris, c1, r.rp, r.tmin, r.best, r.bid, cnt)) { r.occluded = 1; return true; }
-    h1 = false;
+  // the triangles of both hit leaves in one loop, so that lanes hitting either leaf share iterations
+  uint32_t f0 = 0, n0 = 0, f1 = 0, n1 = 0;
+  if (h0 && (c0 & BVH_LEAF_BIT)) { f0 = leaf_first(c0); n0 = leaf_count(c0); h0 = false; }
+  if (h1 && (c1 & BVH_LEAF_BIT)) { f1 = leaf_first(c1); n1 = leaf_count(c1); h1 = false; }
+  for (uint32_t k = 0; k < n0 + n1; ++k) {
+    uint32_t rec = k < n0 ? f0 + k : f1 + (k - n0);
+    if (COUNT) cnt.tris++;
+    if (MODE == 0) tri_closest<ALPHA, FILTER>(tris, rec, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, r.lo, r.range);
+    else if (tri_occludes<ALPHA>(tris, rec, r.rp, r.tmin, r.best, r.bid)) { r.occluded = 1; return true; }
   }
   if (h0 && h1) {
     if (tn1 < tn0) { uint32_t t = c0; c0 = c1; c1 = t; }

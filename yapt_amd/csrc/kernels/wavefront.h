@@ -328,18 +328,34 @@ struct WaveFetch {
   uint32_t* counter;
   uint32_t n;
   uint32_t refill;
+  uint32_t statics;   // queue entries handed out without the counter: one per lane of the grid
+  uint32_t waveBase;  // this wave's first entry among them
+  bool started;
   bool drained;
-  // returns the queue index for this lane (>= n when none), or UINT32_MAX when the wave did not fetch
+  __device__ __forceinline__ void init(uint32_t* c, uint32_t count, uint32_t refillLanes) {
+    counter = c; n = count; refill = refillLanes;
+    statics = gridDim.x * blockDim.x;
+    waveBase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    started = false; drained = false;
+  }
+  // returns the queue index for this lane (>= n when none), or UINT32_MAX when the wave did not fetch.
+  // The first call gives every lane the entry of its own global index, so a queue no longer than the
+  // grid costs no atomic; entries beyond the grid are taken with one atomic per refill.
   __device__ __forceinline__ uint32_t next(bool idle) {
     if (drained) return 0xFFFFFFFFu;
+    uint32_t lane = __lane_id();
+    if (!started) {
+      started = true;
+      if (statics >= n) drained = true;
+      return waveBase + lane;
+    }
     unsigned long long need = __ballot(idle ? 1 : 0);
     uint32_t k = (uint32_t)__popcll(need);
     if (k < refill) return 0xFFFFFFFFu;
-    uint32_t lane = __lane_id();
     int leader = __ffsll((long long)need) - 1;
     uint32_t base = 0;
     if ((int)lane == leader) base = atomicAdd(counter, k);
-    base = (uint32_t)__shfl((int)base, leader);
+    base = statics + (uint32_t)__shfl((int)base, leader);
     if (base + k >= n) drained = true;
     if (!idle) return 0xFFFFFFFFu;
     return base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
@@ -352,7 +368,7 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid == 0) { Q.counters[qsel ^ 1] = 0; Q.counters[2] = 0; Q.counters[5] = 0; }
   WaveFetch wf;
-  wf.counter = &Q.counters[4]; wf.n = Q.counters[qsel]; wf.refill = refill; wf.drained = false;
+  wf.init(&Q.counters[4], Q.counters[qsel], refill);
   TravStack st;
   st.lds = lds; st.ovf = Q.ovf; st.block = blockDim.x; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
   st.ovfStride = Q.ovfStride; st.gid = gid; st.sp = 0;
@@ -391,11 +407,11 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
 // ATOMIC: several shadow rays of one path slot in one launch (bidirectional connections), so the
 // contributions are added atomically
 template <int WIDTH, bool COUNT, bool ATOMIC = false, bool ALPHA = true>
-__global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, uint32_t refill) {
+__global__ __launch_bounds__(256, (ALPHA || COUNT) ? 1 : 8) void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, uint32_t refill) {
   extern __shared__ uint32_t lds[];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   WaveFetch wf;
-  wf.counter = &Q.counters[5]; wf.n = Q.counters[2]; wf.refill = refill; wf.drained = false;
+  wf.init(&Q.counters[5], Q.counters[2], refill);
   TravStack st;
   st.lds = lds; st.ovf = Q.ovf; st.block = blockDim.x; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
   st.ovfStride = Q.ovfStride; st.gid = gid; st.sp = 0;
