@@ -114,10 +114,12 @@ def test_render_parity_shapes_nee(ylib, oracle, shapes_small):
     {"persistent": 1, "persistent_waves": 1, "refill_threshold": 1},
     {"persistent": 1, "persistent_waves": 2, "refill_threshold": 64, "block_size": 192},
     {"persistent": 1, "persistent_waves": 32, "refill_threshold": 16, "passes_in_flight": 3},
+    {"persistent": 1, "leaf_share": 1},
+    {"persistent": 1, "leaf_share": 64, "refill_threshold": 8},
 ])
 def test_launch_shapes_give_the_same_film(ylib, oracle, shapes_small, opts):
-    # persistent or one-workgroup-per-entry launches, any refill threshold, block size or number of
-    # passes in flight: every path touches only its own slot, so the films agree bit for bit
+    # persistent or one-workgroup-per-entry launches, any refill threshold, leaf share, block size or
+    # number of passes in flight: every path touches only its own slot, so the films agree bit for bit
     from yapt_amd import camera
     scene, cam = shapes_small
     frame = camera.make_frame(cam, 83, 61, max_depth=12)

@@ -117,6 +117,7 @@ struct RayTrav {
   uint32_t cur;       // node to visit next
   uint32_t occluded;  // MODE 1 result
   uint32_t lo, range; // FILTER: the closest hit among triangles [lo, lo + range) only
+  uint32_t pl0, pl1;  // deferred traversal: hit leaves waiting for the wave's next leaf phase (0: none)
 };
 
 __device__ __forceinline__ void trav_init(RayTrav& r, vec3 o, vec3 d, float tmin, float tmax, uint32_t target) {
@@ -130,6 +131,8 @@ __device__ __forceinline__ void trav_init(RayTrav& r, vec3 o, vec3 d, float tmin
   r.bid = target;
   r.cur = 0;
   r.occluded = 0;
+  r.pl0 = 0;
+  r.pl1 = 0;
 }
 
 // BVH2: both child boxes in one 64-byte node; hit leaves are intersected at once, the nearer inner
@@ -167,6 +170,56 @@ __device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriSet& t
     r.cur = st.pop();
   }
   return false;
+}
+
+// BVH2 with deferred leaves, for the persistent launches, where a wave's lanes run different rays.
+// Each step is a node phase or a leaf phase for the whole wave. In a node phase the lanes without
+// waiting leaves visit one node; leaves whose boxes are hit are kept in pl0/pl1 and the lane waits.
+// A leaf phase comes once at least 1/leafShare of the active lanes wait; they then intersect their
+// leaves together, so the triangle loop runs with many lanes instead of the few that hit a leaf in
+// one node step. The closest hit is the minimum of (t, id) over the same triangles, so the result
+// does not depend on this order. Returns true when the lane's ray is finished.
+template <int MODE, bool COUNT, bool ALPHA = true>
+__device__ __forceinline__ bool step_bvh2_deferred(const Bvh2Node* nodes, const TriSet& tris, RayTrav& r, TravStack& st,
+                                                   TraversalCounters& cnt, uint32_t leafShare) {
+  bool wait = (r.pl0 | r.pl1) != 0u;
+  uint32_t nWait = (uint32_t)__popcll(__ballot(wait ? 1 : 0)), nActive = (uint32_t)__popcll(__ballot(1));
+  if (nWait * leafShare >= nActive) {
+    if (!wait) return false;
+    uint32_t f0 = leaf_first(r.pl0), n0 = r.pl0 ? leaf_count(r.pl0) : 0u;
+    uint32_t f1 = leaf_first(r.pl1), n1 = r.pl1 ? leaf_count(r.pl1) : 0u;
+    r.pl0 = 0u;
+    r.pl1 = 0u;
+    for (uint32_t k = 0; k < n0 + n1; ++k) {
+      uint32_t rec = k < n0 ? f0 + k : f1 + (k - n0);
+      if (COUNT) cnt.tris++;
+      if (MODE == 0) tri_closest<ALPHA, false>(tris, rec, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, 0u, 0u);
+      else if (tri_occludes<ALPHA>(tris, rec, r.rp, r.tmin, r.best, r.bid)) { r.occluded = 1; return true; }
+    }
+    return r.cur == BVH_EMPTY;
+  }
+  if (wait) return false;
+  const float4* np = reinterpret_cast<const float4*>(nodes + r.cur);
+  float4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
+  if (COUNT) cnt.nodes++;
+  float tn0, tn1;
+  uint32_t c0 = __float_as_uint(qc.x), c1 = __float_as_uint(qc.y);
+  bool h0 = c0 != BVH_EMPTY && slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), r.o, r.inv, r.tmin, r.best, tn0);
+  bool h1 = c1 != BVH_EMPTY && slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), r.o, r.inv, r.tmin, r.best, tn1);
+  if (h0 && (c0 & BVH_LEAF_BIT)) { r.pl0 = c0; h0 = false; }
+  if (h1 && (c1 & BVH_LEAF_BIT)) { r.pl1 = c1; h1 = false; }
+  if (h0 && h1) {
+    if (tn1 < tn0) { uint32_t t = c0; c0 = c1; c1 = t; }
+    st.push(c1);
+    r.cur = c0;
+  } else if (h0) {
+    r.cur = c0;
+  } else if (h1) {
+    r.cur = c1;
+  } else {
+    r.cur = st.sp == 0 ? BVH_EMPTY : st.pop();
+  }
+  return r.cur == BVH_EMPTY && (r.pl0 | r.pl1) == 0u;
 }
 
 // BVH8: test all eight boxes of a node, intersect hit leaves, push hit inner children far to near.

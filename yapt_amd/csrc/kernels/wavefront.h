@@ -363,7 +363,7 @@ struct WaveFetch {
 };
 
 template <int WIDTH, bool COUNT, bool ALPHA = true>
-__global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t ldsDepth, uint32_t refill) {
+__global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t ldsDepth, uint32_t refill, uint32_t leafShare) {
   extern __shared__ uint32_t lds[];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid == 0) { Q.counters[qsel ^ 1] = 0; Q.counters[2] = 0; Q.counters[5] = 0; }
@@ -391,7 +391,12 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
       if (wf.drained) break;
       continue;
     }
-    if (has && trav_step<WIDTH, 0, COUNT, ALPHA>(S.nodes2, S.nodes8, S.tris, r, st, cnt)) {
+    bool done = false;
+    if (has) {
+      if constexpr (WIDTH == 2) done = step_bvh2_deferred<0, COUNT, ALPHA>(S.nodes2, S.tris, r, st, cnt, leafShare);
+      else done = trav_step<WIDTH, 0, COUNT, ALPHA>(S.nodes2, S.nodes8, S.tris, r, st, cnt);
+    }
+    if (done) {
       Q.hit[slot] = make_float4(r.best, r.b1, r.b2, __uint_as_float(r.bid));
       has = false;
     }
@@ -407,7 +412,7 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
 // ATOMIC: several shadow rays of one path slot in one launch (bidirectional connections), so the
 // contributions are added atomically
 template <int WIDTH, bool COUNT, bool ATOMIC = false, bool ALPHA = true>
-__global__ __launch_bounds__(256, (ALPHA || COUNT) ? 1 : 8) void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, uint32_t refill) {
+__global__ __launch_bounds__(256, (ALPHA || COUNT) ? 1 : 8) void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, uint32_t refill, uint32_t leafShare) {
   extern __shared__ uint32_t lds[];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   WaveFetch wf;
@@ -448,7 +453,12 @@ __global__ __launch_bounds__(256, (ALPHA || COUNT) ? 1 : 8) void k_shadow_p(DevS
       if (wf.drained) break;
       continue;
     }
-    if (has && trav_step<WIDTH, 1, COUNT, ALPHA>(S.nodes2, S.nodes8, S.tris, r, st, cnt)) {
+    bool done = false;
+    if (has) {
+      if constexpr (WIDTH == 2) done = step_bvh2_deferred<1, COUNT, ALPHA>(S.nodes2, S.tris, r, st, cnt, leafShare);
+      else done = trav_step<WIDTH, 1, COUNT, ALPHA>(S.nodes2, S.nodes8, S.tris, r, st, cnt);
+    }
+    if (done) {
       if (!r.occluded) {
         uint32_t slot = Q.sSlot[cur];
         float4 c = Q.sContrib[cur];

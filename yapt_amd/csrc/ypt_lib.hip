@@ -214,6 +214,7 @@ struct ypt_integrator {
   int persistent = 1;      // traversal/shading kernels loop over their queue (see work_grid)
   int persistentWaves = 32;
   int refill = 16;         // idle lanes of a wave before it fetches new rays
+  int leafShare = 2;       // BVH2 persistent traversal: leaf phase once 1/leafShare of the active lanes wait
   int shadeWaves = 64;     // waves per CU of the shading launch (0: one workgroup per queue entry)
   int shadeRegWaves = 4;   // register budget of the shading kernel: 2, 4 or 8 waves per SIMD
   int bdptWaves = 4;       // register budget of the camera-subpath kernel: waves per SIMD (1, 2, 3, 4, 6, 8)
@@ -439,11 +440,11 @@ void launch_extend(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
   if (h->persistent) {
     bool alpha = S.tris.alpha.tris != nullptr;
     if (h->countTraversal)
-      hipLaunchKernelGGL((k_extend_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r);
+      hipLaunchKernelGGL((k_extend_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r, (uint32_t)h->leafShare);
     else if (alpha)
-      hipLaunchKernelGGL((k_extend_p<WIDTH, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r);
+      hipLaunchKernelGGL((k_extend_p<WIDTH, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r, (uint32_t)h->leafShare);
     else
-      hipLaunchKernelGGL((k_extend_p<WIDTH, false, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r);
+      hipLaunchKernelGGL((k_extend_p<WIDTH, false, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r, (uint32_t)h->leafShare);
   } else if (h->countTraversal) {
     hipLaunchKernelGGL((k_extend<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d);
   } else {
@@ -456,17 +457,17 @@ void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
   uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
   if (h->res->subsurface) {  // two rays may add to one path: the atomic form, persistent at any grid
     if (h->countTraversal)
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, true, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, true, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
     else
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
   } else if (h->persistent) {
     bool alpha = S.tris.alpha.tris != nullptr;
     if (h->countTraversal)
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
     else if (alpha)
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
     else
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, false, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r);
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, false, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
   } else if (h->countTraversal) {
     hipLaunchKernelGGL((k_shadow<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d);
   } else {
@@ -674,9 +675,9 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
   int q = 0;
   for (uint32_t k = 0; k < fc.maxRayDepth; ++k) {
     if (COUNT || alpha)
-      hipLaunchKernelGGL((k_extend_p<WIDTH, COUNT>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r);
+      hipLaunchKernelGGL((k_extend_p<WIDTH, COUNT>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r, (uint32_t)h->leafShare);
     else
-      hipLaunchKernelGGL((k_extend_p<WIDTH, false, false>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r);
+      hipLaunchKernelGGL((k_extend_p<WIDTH, false, false>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r, (uint32_t)h->leafShare);
     if (h->res->subsurface)  // probes in the shading launch, within the stack buffer's 32 workgroups per CU
       hipLaunchKernelGGL((k_bdpt_cshade<1, true>), dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     else if (h->bdptWaves == 1)
@@ -686,9 +687,9 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
     else
       hipLaunchKernelGGL(k_bdpt_cshade<4>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     if (COUNT || alpha)
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, COUNT, true>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r);
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, COUNT, true>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
     else
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, true, false>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r);
+      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, true, false>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
     q ^= 1;
   }
   hipLaunchKernelGGL(k_bdpt_cfinish, dim3(blocks_for(npix, 256)), dim3(256), 0, s, fc, S, B, C);
@@ -977,6 +978,9 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
   } else if (n == "shade_waves") {
     if (value < 0 || value > 1024) return fail(YPT_E_INVALID, "shade_waves must be in [0, 1024] (0 = one workgroup per entry)");
     h->shadeWaves = (int)value;
+  } else if (n == "leaf_share") {
+    if (value < 1 || value > 64) return fail(YPT_E_INVALID, "leaf_share must be in [1, 64]");
+    h->leafShare = (int)value;
   } else if (n == "refill_threshold") {
     if (value < 1 || value > 64) return fail(YPT_E_INVALID, "refill_threshold must be in [1, 64] (lanes of a wave)");
     h->refill = (int)value;

@@ -15,7 +15,7 @@ TUNED = os.path.join(HERE, "tuned.json")
 ROOT = os.path.dirname(HERE)
 
 DEFAULT = {"bvh_width": 2, "block_size": 64, "lds_stack_depth": 16, "passes_in_flight": 0, "persistent": 1,
-           "persistent_waves": 32, "refill_threshold": 16, "shade_waves": 64, "shade_reg_waves": 4}
+           "persistent_waves": 32, "refill_threshold": 16, "leaf_share": 2, "shade_waves": 64, "shade_reg_waves": 4}
 
 
 def _load(path=None):
@@ -53,6 +53,7 @@ SPACE = {
     "persistent": (0, 1),
     "persistent_waves": (8, 16, 24, 32),
     "refill_threshold": (4, 8, 16, 32, 64),
+    "leaf_share": (1, 2, 3, 4),
     "shade_waves": (0, 16, 32, 64, 128),
     "shade_reg_waves": (2, 4, 8),
 }
@@ -64,6 +65,7 @@ SPACE_BDPT = {
     "block_size": (64, 128, 256),
     "persistent_waves": (16, 24, 32),
     "refill_threshold": (8, 16, 32),
+    "leaf_share": (1, 2, 4),
     "lds_stack_depth": (12, 16, 24),
 }
 
