@@ -1,6 +1,7 @@
 // BVH traversal on gfx950: one ray per lane, short traversal stack in LDS with a global-memory
-// overflow, both children's boxes fetched in one 64-byte load (BVH2) or eight in 256 bytes (BVH8),
-// leaves intersected as soon as their box is hit. Closest hit = minimum (t, triangle id); the
+// overflow, both children's boxes fetched in one 64-byte load (BVH2) or eight in 256 bytes (BVH8).
+// Leaves are intersected as soon as their box is hit, or, in the persistent launches, together with
+// the leaves of the wave's other lanes (step_bvh2_deferred). Closest hit = minimum (t, triangle id); the
 // shadow query stops at the first triangle that is closer than the light triangle. A triangle
 // whose record carries an alpha word counts only where its albedo alpha passes the cutoff; ALPHA =
 // false leaves that test out for scenes without alpha-tested triangles.
