@@ -116,6 +116,7 @@ def test_render_parity_shapes_nee(ylib, oracle, shapes_small):
     {"persistent": 1, "persistent_waves": 32, "refill_threshold": 16, "passes_in_flight": 3},
     {"persistent": 1, "leaf_share": 1},
     {"persistent": 1, "leaf_share": 64, "refill_threshold": 8},
+    {"persistent": 1, "block_size": 512},
 ])
 def test_launch_shapes_give_the_same_film(ylib, oracle, shapes_small, opts):
     # persistent or one-workgroup-per-entry launches, any refill threshold, leaf share, block size or

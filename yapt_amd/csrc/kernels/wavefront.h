@@ -412,7 +412,7 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
 // ATOMIC: several shadow rays of one path slot in one launch (bidirectional connections), so the
 // contributions are added atomically
 template <int WIDTH, bool COUNT, bool ATOMIC = false, bool ALPHA = true>
-__global__ __launch_bounds__(256, (ALPHA || COUNT) ? 1 : 8) void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, uint32_t refill, uint32_t leafShare) {
+__global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, uint32_t refill, uint32_t leafShare) {
   extern __shared__ uint32_t lds[];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   WaveFetch wf;
