@@ -7,6 +7,8 @@
 #include <condition_variable>
 #include <cstdio>
 #include <deque>
+#include <map>
+#include <tuple>
 #include <thread>
 #include <unordered_map>
 #include <cstring>
@@ -224,6 +226,7 @@ struct ypt_integrator {
   double stats[16] = {0};
   hipEvent_t ev[8] = {};
   bool profile = false;
+  std::map<std::tuple<const void*, uint32_t, size_t>, int> occupancy;  // workgroups per CU of a kernel launch shape
   std::vector<hipEvent_t> evPool;  // profile mode: one event per kernel boundary, read after the render
   size_t evUsed = 0;
   hipEvent_t next_event() {
@@ -433,6 +436,30 @@ void launch_shade(ypt_integrator* h, const DevScene& S, const uint32_t* point, u
 #undef YPT_SHADE
 }
 
+// A persistent launch gets no more workgroups than fit on the device at once (the kernel's
+// occupancy at this block size and LDS): a workgroup beyond that would start only after others had
+// drained the queue, and its first rays would then run alone at the end of the launch.
+template <class K>
+uint32_t resident_grid(ypt_integrator* h, K kernel, uint32_t block, size_t lds, uint32_t grid) {
+  const void* f = reinterpret_cast<const void*>(kernel);
+  auto key = std::make_tuple(f, block, lds);
+  auto it = h->occupancy.find(key);
+  int nb = 0;
+  if (it != h->occupancy.end()) {
+    nb = it->second;
+  } else {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, (int)block, lds) != hipSuccess || nb < 1) nb = 0;
+    h->occupancy[key] = nb;
+  }
+  if (nb == 0) return grid;
+  return std::max<uint32_t>(1u, std::min<uint32_t>(grid, (uint32_t)nb * (uint32_t)h->cus));
+}
+
+// persistent traversal launch, its grid clamped to what is resident
+#define YPT_PERSISTENT(KERNEL, GRID, BLOCK, LDS, STREAM, ...)                                              \
+  hipLaunchKernelGGL(KERNEL, dim3(resident_grid(h, KERNEL, (BLOCK), (LDS), (GRID))), dim3(BLOCK), (LDS), \
+                     (STREAM), __VA_ARGS__)
+
 template <int WIDTH>
 void launch_extend(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, int q, uint32_t grid) {
   size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
@@ -440,11 +467,11 @@ void launch_extend(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
   if (h->persistent) {
     bool alpha = S.tris.alpha.tris != nullptr;
     if (h->countTraversal)
-      hipLaunchKernelGGL((k_extend_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_extend_p<WIDTH, true>), grid, h->blockSize, lds, h->stream, S, P, Q, q, d, r, (uint32_t)h->leafShare);
     else if (alpha)
-      hipLaunchKernelGGL((k_extend_p<WIDTH, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_extend_p<WIDTH, false, true>), grid, h->blockSize, lds, h->stream, S, P, Q, q, d, r, (uint32_t)h->leafShare);
     else
-      hipLaunchKernelGGL((k_extend_p<WIDTH, false, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_extend_p<WIDTH, false, false>), grid, h->blockSize, lds, h->stream, S, P, Q, q, d, r, (uint32_t)h->leafShare);
   } else if (h->countTraversal) {
     hipLaunchKernelGGL((k_extend<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, q, d);
   } else {
@@ -457,17 +484,17 @@ void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
   uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
   if (h->res->subsurface) {  // two rays may add to one path: the atomic form, persistent at any grid
     if (h->countTraversal)
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, true, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_shadow_p<WIDTH, true, true>), grid, h->blockSize, lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
     else
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_shadow_p<WIDTH, false, true>), grid, h->blockSize, lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
   } else if (h->persistent) {
     bool alpha = S.tris.alpha.tris != nullptr;
     if (h->countTraversal)
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_shadow_p<WIDTH, true>), grid, h->blockSize, lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
     else if (alpha)
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, false, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_shadow_p<WIDTH, false, false, true>), grid, h->blockSize, lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
     else
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, false, false>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_shadow_p<WIDTH, false, false, false>), grid, h->blockSize, lds, h->stream, S, P, Q, d, r, (uint32_t)h->leafShare);
   } else if (h->countTraversal) {
     hipLaunchKernelGGL((k_shadow<WIDTH, true>), dim3(grid), dim3(h->blockSize), lds, h->stream, S, P, Q, d);
   } else {
@@ -675,9 +702,9 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
   int q = 0;
   for (uint32_t k = 0; k < fc.maxRayDepth; ++k) {
     if (COUNT || alpha)
-      hipLaunchKernelGGL((k_extend_p<WIDTH, COUNT>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_extend_p<WIDTH, COUNT>), pgrid, Bk, lds, s, S, P, Q, q, d, r, (uint32_t)h->leafShare);
     else
-      hipLaunchKernelGGL((k_extend_p<WIDTH, false, false>), dim3(pgrid), dim3(Bk), lds, s, S, P, Q, q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_extend_p<WIDTH, false, false>), pgrid, Bk, lds, s, S, P, Q, q, d, r, (uint32_t)h->leafShare);
     if (h->res->subsurface)  // probes in the shading launch, within the stack buffer's 32 workgroups per CU
       hipLaunchKernelGGL((k_bdpt_cshade<1, true>), dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     else if (h->bdptWaves == 1)
@@ -687,9 +714,9 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
     else
       hipLaunchKernelGGL(k_bdpt_cshade<4>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     if (COUNT || alpha)
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, COUNT, true>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_shadow_p<WIDTH, COUNT, true>), sgrid, Bk, lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
     else
-      hipLaunchKernelGGL((k_shadow_p<WIDTH, false, true, false>), dim3(sgrid), dim3(Bk), lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_shadow_p<WIDTH, false, true, false>), sgrid, Bk, lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
     q ^= 1;
   }
   hipLaunchKernelGGL(k_bdpt_cfinish, dim3(blocks_for(npix, 256)), dim3(256), 0, s, fc, S, B, C);
