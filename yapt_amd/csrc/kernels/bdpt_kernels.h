@@ -138,6 +138,7 @@ __global__ __launch_bounds__(64) void k_bdpt_light(FrameConstants fc, DevScene S
   tr.flush(B.work + 3);
 }
 
+#ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit
 // one workgroup of 1024 threads: subpaths with vertices, compacted in index order
 __global__ void k_bdpt_headers(BdptBuffers B) {
   __shared__ uint32_t sums[1024];
@@ -163,7 +164,9 @@ __global__ void k_bdpt_headers(BdptBuffers B) {
   for (uint32_t i = begin; i < end; ++i)
     if (B.counts[i] > 0u) B.headers[o++] = i;
 }
+#endif
 
+#ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit
 // surface of every stored light vertex after the first, one thread per (subpath, vertex)
 __global__ void k_bdpt_node_surfaces(DevScene S, BdptBuffers B) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -176,6 +179,7 @@ __global__ void k_bdpt_node_surfaces(DevScene S, BdptBuffers B) {
   light_node_surface(S.sv, n, prev.pos, spectral_lambdas(B.lambdas[p]), out);
   B.surf[(size_t)p * BDPT_MAX_LIGHT_VERTICES + j] = out;
 }
+#endif
 
 // 64 threads per workgroup = one 8x8 dispatch tile
 // MINW: waves per SIMD the register allocation must allow (1: up to 512 VGPRs, 2: 256)
@@ -245,6 +249,7 @@ __global__ __launch_bounds__(64) void k_bdpt_splat(FrameConstants fc, DevScene S
   }
 }
 
+#ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit
 // sorted records: the first record of each texel sums its run in record order
 __global__ void k_bdpt_accum(const uint32_t* keys, const uint32_t* vals, const float4* recXYZ, uint32_t n, uint32_t npix,
                              float4* camXYZ) {
@@ -263,6 +268,7 @@ __global__ void k_bdpt_accum(const uint32_t* keys, const uint32_t* vals, const f
     camXYZ[key] = make_float4(c.x + sx, c.y + sy, c.z + sz, c.w);
   }
 }
+#endif
 
 // ---------------------------------------------------------------- camera subpaths as a wavefront
 // The camera subpaths of a pass as queues in HBM: extension rays go through the persistent
@@ -362,6 +368,7 @@ __device__ __forceinline__ uint32_t slot_texel(const FrameConstants& fc, const B
   return B.perm[dy * W + dx];
 }
 
+#ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit
 // one slot per dispatch position; the 8x8 dispatch tile picks one light subpath
 __global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuffers B, CamSoA C, Queues Q, float* features) {
   uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
@@ -396,6 +403,7 @@ __global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuff
     store_features(features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * FEATURE_STRIDE, none);
   }
 }
+#endif
 
 struct QueueSink {  // connection rays into the shadow queue, one atomic per wave and connection index
   Queues Q;
@@ -469,6 +477,7 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
   }
 }
 
+#ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit
 // XYZ of every camera subpath into its texel
 __global__ void k_bdpt_cfinish(FrameConstants fc, DevScene S, BdptBuffers B, CamSoA C) {
   uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
@@ -480,5 +489,6 @@ __global__ void k_bdpt_cfinish(FrameConstants fc, DevScene S, BdptBuffers B, Cam
   uint32_t q = slot_texel(fc, B, i);
   B.camXYZ[(q >> 16) * W + (q & 0xFFFFu)] = make_float4(xyz.x, xyz.y, xyz.z, lambda);
 }
+#endif
 
 }  // namespace ypt

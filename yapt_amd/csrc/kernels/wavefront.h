@@ -139,6 +139,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
+#ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit
 // Slots [p * npix, (p + 1) * npix) hold the paths of pass p of the batch (several sample indices are
 // in flight at once so that every bounce launch carries enough rays to hide traversal latency).
 __global__ void k_start(FrameConstants fc, const uint32_t* points, PathSoA P, Queues Q, FilmBuffers F, uint32_t npix,
@@ -158,6 +159,7 @@ __global__ void k_start(FrameConstants fc, const uint32_t* points, PathSoA P, Qu
     store_features(F.features + (size_t)i * FEATURE_STRIDE, none);
   }
 }
+#endif
 
 template <int WIDTH, bool COUNT>
 __global__ void k_extend(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t ldsDepth) {
@@ -493,15 +495,19 @@ struct MergeParams {
   uint32_t pass;              // pass index within the batch (with a device-side base, see k_merge)
 };
 
+#ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit
 // device-side film bookkeeping of a captured batch: dyn[0] passes merged so far, dyn[1] current buffer
 __global__ void k_set_dyn(uint32_t* dyn, uint32_t samples, uint32_t cur) {
   dyn[0] = samples;
   dyn[1] = cur;
 }
+#endif
+#ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit
 __global__ void k_advance_dyn(uint32_t* dyn, uint32_t g) {
   dyn[0] += g;
   dyn[1] ^= (g & 1u);
 }
+#endif
 
 // clampFireflySample (mergeNewSamples.comp:122-170), reading the previous pass's merged film
 __device__ vec3 clamp_firefly(const MergeParams& mp, const FilmBuffers& F, int x, int y, vec3 c, uint32_t accumulated) {
@@ -532,6 +538,7 @@ __device__ vec3 clamp_firefly(const MergeParams& mp, const FilmBuffers& F, int x
   return c * s;
 }
 
+#ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit
 // path XYZ + mergeNewSamples (mergeNewSamples.comp:173-265) with per-pixel sample counts
 // sampleXYZ: XYZ and hero wavelength per texel (bidirectional integrator); null: XYZ of the path slots.
 // dyn (captured batches): sample count and current buffer come from the device, advanced by pass.
@@ -607,6 +614,7 @@ __global__ void k_merge(DevScene S, PathSoA P, FilmBuffers F, MergeParams mp, co
   F.merged[mp.cur][p] = make_float4(sum.x, sum.y, sum.z, 1.f);
   F.variance[p] = mp.sampleCount == 0 ? 0.f : variance;
 }
+#endif
 
 // ---------------------------------------------------------------- query kernels used by tests
 template <int WIDTH>
@@ -631,10 +639,12 @@ __global__ void k_trace_closest(DevScene S, const float* rays, uint32_t n, float
   outB[2 * i + 1] = tri == NO_HIT_ID ? 0.f : b2;
 }
 
+#ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit
 __global__ void k_sample_dims(const uint32_t* point, const uint32_t* seeds, const uint32_t* dims, uint32_t n, float* out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   out[i] = sample_dim(point, dims[i], seeds[i]);
 }
+#endif
 
 }  // namespace ypt

@@ -21,6 +21,7 @@
 #include "../../include/yapt_pt.h"
 #include "core/shade.h"
 #include "kernels/bdpt_kernels.h"
+#include "kernels/instances.h"
 #include "kernels/post.h"
 #include "kernels/wavefront.h"
 #include "runtime/bvh.h"
@@ -32,6 +33,11 @@
 #include "data/sobol_matrices.inc"
 
 using namespace ypt;
+
+// the template kernels are compiled in instances.hip
+namespace ypt {
+YPT_INST_ALL(extern template)
+}
 
 namespace {
 
