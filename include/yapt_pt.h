@@ -33,11 +33,8 @@ extern "C" {
 #define YPT_FLAG_DISABLE_JITTER (1u << 1)  /* CONSTANTS_FLAG_DISABLE_TEXEL_JITTER */
 #define YPT_FLAG_WHITE_FURNACE (1u << 8)   /* WHITE_FURNACE_TEST as a runtime switch */
 
-/* material mask bits (MaterialInternal.h / materialsCommon.hlsl:8-10) */
-#define YPT_MATERIAL_TWO_SIDED (1u << 0)
-#define YPT_MATERIAL_DISPERSION (1u << 1)
-
-/* material_mask bits (MaterialMask_*); ALPHA_BLEND turns the albedo texel alpha into coverage */
+/* material_mask bits (MaterialMask_*, MaterialInternal.h / materialsCommon.hlsl:8-10); ALPHA_BLEND
+ * turns the albedo texel alpha into coverage */
 #define YPT_MATERIAL_TWO_SIDED (1u << 0)
 #define YPT_MATERIAL_DISPERSION (1u << 1)
 #define YPT_MATERIAL_ALPHA_BLEND (1u << 2)
@@ -178,7 +175,10 @@ int ypt_set_frame(ypt_integrator* h, const ypt_frame_desc* frame);
 int ypt_clear_film(ypt_integrator* h);
 /* PathIntegratorSubStage::update + executeRaytrace + mergeNewSamples, for Sobol sample indices
  * [sample_begin, sample_begin + sample_count): one pass per index, each merged into the film.
- * stream: a hipStream_t or NULL for the library's stream. Asynchronous; see ypt_synchronize. */
+ * stream: a hipStream_t or NULL for the library's stream. The kernels of this call run on that stream,
+ * after the work already queued on the handle, and every later call on the handle sees their result;
+ * the handle keeps no reference to the caller's stream after the call. Asynchronous; see
+ * ypt_synchronize. */
 int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, void* stream);
 int ypt_synchronize(ypt_integrator* h);
 /* getOutput(COLOR): merged film, W*H float4 (X, Y, Z, 1) into host memory; counts: W*H uint32 or NULL */

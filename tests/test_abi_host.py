@@ -87,3 +87,20 @@ def test_camera_matrices_round_trip():
     assert np.allclose(ray(0.5, 0.5), fwd, atol=1e-5)
     assert abs(math.acos(np.dot(ray(0.5, 0.0), fwd)) - math.pi * 0.15) < 1e-4
     assert ray(0.5, 0.0)[1] > fwd[1]   # v = 0 is the top of the image (D3D texture space)
+
+
+def test_denoiser_resampling_rejects_odd_and_empty_shapes():
+    """Pooling and upsampling entry points validate the full-resolution shape on the host: an odd H or
+    W would index past the half-resolution tensor. The fake pointers are never dereferenced, since the
+    checks return before any launch."""
+    from yapt_amd import denoiser
+    lib = denoiser.load()
+    p = ctypes.c_void_p(256)
+    for H, W in ((7, 8), (8, 7), (0, 8)):
+        assert lib.ydn_pool2(p, 1, H, W, 16, p, None) == -1
+        assert lib.ydn_pool2_grad(p, 1, H, W, 16, p, None) == -1
+        assert lib.ydn_up2_concat(p, 16, p, 16, 1, H, W, p, None) == -1
+        assert lib.ydn_up2_concat_grad(p, 16, 16, 1, H, W, p, p, 0, None) == -1
+    assert b"even H and W" in lib.ydn_last_error() or b"empty shape" in lib.ydn_last_error()
+    assert lib.ydn_up2_concat(None, 16, p, 16, 1, 8, 8, p, None) == -1
+    assert b"null tensor" in lib.ydn_last_error()

@@ -276,3 +276,39 @@ def test_bdpt_light_stage_overlap_matches_one_stream(ylib, shapes_small):
         films.append(it.film())
         it.close()
     assert np.array_equal(films[0].view(np.uint32), films[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("integrator", [1, 2])
+def test_render_on_a_caller_stream(ylib, shapes_small, integrator):
+    # passes rendered on a caller's stream, then on the library's stream, give the film of a handle
+    # that used its own stream throughout; destroying the handle leaves the caller's stream usable
+    import ctypes
+    import torch
+    from yapt_amd import camera
+    scene, cam = shapes_small
+    frame = camera.make_frame(cam, 64, 48, max_depth=8, light_path_vertices=6)
+
+    def handle():
+        it = ylib.Integrator(integrator, 0)
+        if integrator == 2:
+            it.set_option("bdpt_wavefront", 0)  # one fixed order of every sum: bit-comparable films
+        it.set_scene(scene)
+        it.set_frame(frame)
+        return it
+
+    a = handle()
+    a.render(0, 6)
+    want = a.film()
+    a.close()
+    caller = torch.cuda.Stream()
+    b = handle()
+    b.render(0, 2, stream=ctypes.c_void_p(caller.cuda_stream))
+    b.render(2, 2)
+    b.render(4, 2, stream=ctypes.c_void_p(caller.cuda_stream))
+    got = b.film()
+    b.close()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    with torch.cuda.stream(caller):
+        x = torch.arange(1024, device="cuda", dtype=torch.float32) * 2.0
+    caller.synchronize()
+    assert float(x[-1]) == 2046.0

@@ -1,6 +1,6 @@
 """Command-line autotuner for the benchmark workloads.
 
-    python -m yapt_amd.autotune [--out PATH] [--width 1920 --height 1080 --depth 12 --shapes 2000]
+    python -m yapt_amd.autotune [--out PATH] [--history PATH] [--width 1920 --height 1080 --depth 12 --shapes 2000]
 
 Coordinate-descent search over the launch options (yapt_amd.tuning.SPACE) for the random-shapes
 workload; stores the fastest configuration (yapt_amd/tuned.json by default).
@@ -13,6 +13,7 @@ from . import _lib, camera, scenes, tuning
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--out", default=None)
+    p.add_argument("--history", default=None, help="JSON file for the timing of every configuration tried")
     p.add_argument("--width", type=int, default=1920)
     p.add_argument("--height", type=int, default=1080)
     p.add_argument("--depth", type=int, default=12)
@@ -31,7 +32,7 @@ def main():
     it.set_scene(scene)
     it.set_frame(frame)
     key = "%s-shapes%d-%dx%d-d%d" % (a.integrator, a.shapes, a.width, a.height, a.depth)
-    best = tuning.autotune(it, key, passes=a.passes, rounds=a.rounds, out=a.out,
+    best = tuning.autotune(it, key, passes=a.passes, rounds=a.rounds, out=a.out, history=a.history,
                            space=tuning.SPACE_BDPT if bdpt else None)
     print("best", key, best)
     it.close()

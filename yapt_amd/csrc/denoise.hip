@@ -303,6 +303,14 @@ __global__ void k_f32_to_bf16(const float* a, uint16_t* b, long n) {
   if (i < n) b[i] = f2bf(a[i]);
 }
 
+// the full-resolution shape (N, H, W) of a pooling or upsampling step: positive, H and W even (the
+// half-resolution tensor has H/2 x W/2 texels, so an odd size would reach past it)
+int check_half_shape(int N, int H, int W, int C) {
+  if (N <= 0 || H <= 0 || W <= 0 || C <= 0) return fail(YDN_E_INVALID, "empty shape");
+  if ((H | W) & 1) return fail(YDN_E_INVALID, "pooling and upsampling need even H and W");
+  return YDN_OK;
+}
+
 unsigned blocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 int check_launch() {
@@ -364,19 +372,25 @@ int ydn_relu_grad(const uint16_t* out, uint16_t* g, long n, void* stream) {
 }
 
 int ydn_pool2(const uint16_t* in, int N, int H, int W, int C, uint16_t* out, void* stream) {
-  if ((H | W) & 1) return fail(YDN_E_INVALID, "pooling needs even H and W");
+  if (!in || !out) return fail(YDN_E_INVALID, "null tensor");
+  if (int rc = check_half_shape(N, H, W, C)) return rc;
   long n = (long)N * (H / 2) * (W / 2) * C;
   hipLaunchKernelGGL(k_pool2, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, in, N, H, W, C, out);
   return check_launch();
 }
 
 int ydn_pool2_grad(const uint16_t* gout, int N, int H, int W, int C, uint16_t* gin, void* stream) {
+  if (!gout || !gin) return fail(YDN_E_INVALID, "null tensor");
+  if (int rc = check_half_shape(N, H, W, C)) return rc;
   long n = (long)N * H * W * C;
   hipLaunchKernelGGL(k_pool2_grad, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, gout, N, H, W, C, gin);
   return check_launch();
 }
 
 int ydn_up2_concat(const uint16_t* in, int C1, const uint16_t* skip, int C2, int N, int H, int W, uint16_t* out, void* stream) {
+  if (!in || !skip || !out) return fail(YDN_E_INVALID, "null tensor");
+  if (C2 <= 0) return fail(YDN_E_INVALID, "empty shape");
+  if (int rc = check_half_shape(N, H, W, C1)) return rc;
   long n = (long)N * H * W * (C1 + C2);
   hipLaunchKernelGGL(k_up2_concat, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, in, C1, skip, C2, N, H, W, out);
   return check_launch();
@@ -384,6 +398,9 @@ int ydn_up2_concat(const uint16_t* in, int C1, const uint16_t* skip, int C2, int
 
 int ydn_up2_concat_grad(const uint16_t* g, int C1, int C2, int N, int H, int W, uint16_t* gin, uint16_t* gskip,
                         int accumulate_skip, void* stream) {
+  if (!g || !gin || !gskip) return fail(YDN_E_INVALID, "null tensor");
+  if (C2 <= 0) return fail(YDN_E_INVALID, "empty shape");
+  if (int rc = check_half_shape(N, H, W, C1)) return rc;
   long n = (long)N * H * W * (C1 > C2 ? C1 : C2);
   hipLaunchKernelGGL(k_up2_concat_grad, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, g, C1, C2, N, H, W, gin,
                      gskip, accumulate_skip);

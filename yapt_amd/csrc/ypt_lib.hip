@@ -142,8 +142,11 @@ struct ResidentScene {
 struct ypt_integrator {
   int type = YPT_INTEGRATOR_NEE;
   int device = 0;
+  // the stream the handle's work runs on: the library's own stream (ownStream), or the caller's
+  // stream for the duration of a ypt_render that names one
   hipStream_t stream = nullptr;
-  bool ownStream = false;
+  hipStream_t ownStream = nullptr;
+  hipEvent_t streamJoin = nullptr;  // orders the caller's stream and the library's stream
 
   // scene (resident, possibly shared with other handles)
   std::shared_ptr<ResidentScene> res;
@@ -560,9 +563,10 @@ hipGraphExec_t batch_graph(ypt_integrator* h, const DevScene& S, const PathSoA& 
   uint64_t k = 1469598103934665603ull;
   k = fnv(k, &h->fc, sizeof(h->fc));
   k = fnv(k, &S, sizeof(S)); k = fnv(k, &P, sizeof(P)); k = fnv(k, &Q, sizeof(Q)); k = fnv(k, &F, sizeof(F));
-  int opts[15] = {h->type, h->bvhWidth, h->blockSize, h->ldsDepth, h->persistent, h->persistentWaves, h->refill,
+  // every launch option that changes a kernel choice, a launch shape or a kernel argument
+  int opts[16] = {h->type, h->bvhWidth, h->blockSize, h->ldsDepth, h->persistent, h->persistentWaves, h->refill,
                   h->shadeWaves, h->writeFeatures, h->cus, (int)g, (int)h->width, (int)h->height, (int)h->frameDesc.kill_nan_samples,
-                  h->shadeRegWaves};
+                  h->shadeRegWaves, h->leafShare};
   k = fnv(k, opts, sizeof(opts));
   float fp[2] = {h->frameDesc.firefly_clamp_scale, h->frameDesc.firefly_clamp_relax_power};
   k = fnv(k, fp, sizeof(fp));
@@ -911,6 +915,26 @@ void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count
   h->stats[13] = 1;
 }
 
+// A render on a caller's stream: that stream first waits for the work already queued on the
+// library's stream, the render's kernels run on it, and the library's stream then waits for them,
+// so every later call on the handle (which synchronizes the library's stream) sees the result. The
+// handle keeps no reference to the caller's stream after the call.
+struct CallerStream {
+  ypt_integrator* h;
+  bool active;
+  CallerStream(ypt_integrator* hh, hipStream_t caller) : h(hh), active(caller != nullptr && caller != hh->ownStream) {
+    if (!active) return;
+    HIP_CHECK(hipEventRecord(h->streamJoin, h->ownStream));
+    HIP_CHECK(hipStreamWaitEvent(caller, h->streamJoin, 0));
+    h->stream = caller;
+  }
+  ~CallerStream() {
+    if (!active) return;
+    if (hipEventRecord(h->streamJoin, h->stream) == hipSuccess) (void)hipStreamWaitEvent(h->ownStream, h->streamJoin, 0);
+    h->stream = h->ownStream;
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -938,8 +962,9 @@ int ypt_create(int integrator_type, int device, ypt_integrator** out) {
     h->device = device;
     HIP_CHECK(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, device));
     if (h->cus < 1) h->cus = 1;
-    HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    h->ownStream = true;
+    HIP_CHECK(hipStreamCreateWithFlags(&h->ownStream, hipStreamNonBlocking));
+    h->stream = h->ownStream;
+    HIP_CHECK(hipEventCreateWithFlags(&h->streamJoin, hipEventDisableTiming));
     for (auto& e : h->ev) HIP_CHECK(hipEventCreate(&e));
     upload_constant_tables(h.get());
     *out = h.release();
@@ -957,7 +982,8 @@ int ypt_destroy(ypt_integrator* h) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (g.graph) (void)hipGraphDestroy(g.graph);
   }
-  if (h->ownStream && h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->streamJoin) (void)hipEventDestroy(h->streamJoin);
+  if (h->ownStream) (void)hipStreamDestroy(h->ownStream);  // never a caller's stream
   if (h->lightStream) {
     (void)hipStreamSynchronize(h->lightStream);
     for (int i = 0; i < 2; ++i) {
@@ -1073,7 +1099,7 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
   return guarded([&]() -> int {
     if (!h->hasScene || !h->hasFrame) return fail(YPT_E_NOSCENE, "render needs a scene and a frame");
     HIP_CHECK(hipSetDevice(h->device));
-    if (stream) h->stream = (hipStream_t)stream;
+    CallerStream scope(h, (hipStream_t)stream);
     if (h->writeFeatures) h->featAccum.ensure((size_t)4 * h->width * h->height);
     if (h->type == YPT_INTEGRATOR_BDPT) {
       render_bdpt(h, sample_begin, sample_count);
@@ -1485,6 +1511,7 @@ struct ServerJob {
   uint32_t spp = 0, chunk = 1, passesInFlight = 1;
   uint32_t issued = 0;
   uint64_t mem = 0;
+  std::vector<std::pair<std::string, int64_t>> options;  // launch options of its type, copied at admission
   ypt_integrator* h = nullptr;
   hipEvent_t ev = nullptr;
   bool inFlight = false;
@@ -1551,7 +1578,7 @@ struct ypt_server {
     ypt_integrator* h = j.h;
     if (!j.ev) HIP_CHECK(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
     if (h->res != j.scene) attach_scene(h, j.scene);
-    for (const auto& o : options[j.type])
+    for (const auto& o : j.options)
       if (ypt_set_option(h, o.first.c_str(), o.second) != YPT_OK) throw std::runtime_error(ypt_last_error());
     if (ypt_set_frame(h, &j.frame) != YPT_OK) throw std::runtime_error(std::string("server: ") + ypt_last_error());
     if (ypt_set_option(h, "passes_in_flight", j.passesInFlight) != YPT_OK) throw std::runtime_error(ypt_last_error());
@@ -1583,6 +1610,7 @@ struct ypt_server {
           if (!active.empty() && memUsed + j->mem > budget) break;  // wait for memory to free up
           queued.pop_front();
           memUsed += j->mem;
+          j->options = options[j->type];
           j->state = 1;
           j->tAdmit = now_s();
           active.push_back(j);

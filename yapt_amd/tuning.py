@@ -81,11 +81,12 @@ def _time(integrator, cfg, passes):
     return (time.perf_counter() - t) / passes * 1e3
 
 
-def autotune(integrator, key, passes=16, space=None, rounds=2, start=None, verbose=True, out=None):
+def autotune(integrator, key, passes=16, space=None, rounds=2, start=None, verbose=True, out=None, history=None):
     """Coordinate descent over the launch options: starting from `start` (the stored or default
     configuration), each option in turn is set to every candidate of `space` with the others fixed,
     timing `passes` full passes (scene and frame already set), and the fastest value is kept;
-    `rounds` sweeps over all options. Stores the best configuration under `key` and applies it."""
+    `rounds` sweeps over all options. Stores the best configuration under `key` and applies it; the
+    timing of every configuration tried goes to the JSON file `history` when one is given."""
     space = space or SPACE
     cur = {k: v for k, v in dict(start or best_config(key)).items() if k in space or k in DEFAULT}
     for k in space:
@@ -118,9 +119,12 @@ def autotune(integrator, key, passes=16, space=None, rounds=2, start=None, verbo
         if not changed:
             break
     store = _load(out)
-    store[key] = {"config": cur, "ms_per_pass": best_ms, "all": results}
+    store[key] = {"config": cur, "ms_per_pass": best_ms}
     with open(out or TUNED, "w") as f:
         json.dump(store, f, indent=1, sort_keys=True)
+    if history:
+        with open(history, "w") as f:
+            json.dump({key: results}, f, indent=1, sort_keys=True)
     for k, v in cur.items():
         integrator.set_option(k, v)
     return {"config": cur, "ms_per_pass": best_ms}
