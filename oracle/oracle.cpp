@@ -639,6 +639,10 @@ int oracle_render(const ypt_scene_desc* sd, const ypt_frame_desc* fd, int integr
 // 26 burley_profile_area(r, d)  27 burley_sample_radius(u, d)  28 burley_d(radius, albedo)
 // 29 pdf_subsurface_exit(albedo (grey), radius (grey), to.xyz on the plane y = 0 from the origin, normal +y)
 // 30 sellmeier_ior(lambda nm, B1, B2, B3, C1, C2, C3)  31 planck_relative(lambda nm, kelvin)
+// 32 avg_fresnel(eta)  33 glambda_sheen(r, cos)  34 d_sheen(cos of the microfacet normal, r)
+// 35 refractive_index_for_wavelength(A, B, lambda nm) (Cauchy)
+// 36 apply_thin_film(A1, A2, nFilm, etaRTop, phaseShift, cosIncident, thickness nm, hero lambda nm) -> 4 lanes
+// 37 ssms_avg(eta, r)  38 average_albedo_translucent(eta, r)
 extern "C" int oracle_eval(int fn, const float* in, uint32_t n, float* out) {
   MSLuts L;
   L.base = tables().msl.data();
@@ -680,6 +684,18 @@ extern "C" int oracle_eval(int fn, const float* in, uint32_t n, float* out) {
                                           vec3(a[5], a[6], a[7])); break;
       case 30: o[0] = sellmeier_ior(a + 1, a + 4, a[0]); break;
       case 31: o[0] = planck_relative(a[0], a[1]); break;
+      case 32: o[0] = avg_fresnel(a[0]); break;
+      case 33: o[0] = glambda_sheen(a[0], a[1]); break;
+      case 34: { float c = a[0]; o[0] = d_sheen(vec3(std::sqrt(std::max(0.f, 1.f - c * c)), c, 0.f), a[1]); break; }
+      case 35: o[0] = refractive_index_for_wavelength(a[0], a[1], a[2]); break;
+      case 36: {
+        Lambdas lam = spectral_lambdas(a[7]);
+        Spectrum r = apply_thin_film(lam, Spectrum(a[0]), Spectrum(a[1]), a[2], a[3], a[4], a[5], a[6]);
+        for (int k = 0; k < 4; ++k) o[k] = r.s[k];
+        break;
+      }
+      case 37: o[0] = L.ssms_avg(a[0], a[1]); break;
+      case 38: o[0] = average_albedo_translucent(L, a[0], a[1]); break;
       default: g_err = "unknown function"; return YPT_E_INVALID;
     }
   }
@@ -692,4 +708,34 @@ extern "C" int oracle_rgb_reproduce(const float* coeffs, float* rgb_out) {
   tables().r2s.reproduce(c, r);
   for (int k = 0; k < 3; ++k) rgb_out[k] = (float)r[k];
   return YPT_OK;
+}
+
+// evaluate_surface for one untextured material in a frame whose geometry and shading normals are +y
+// and whose tangent is +x, seen from outside (current and previous IOR 1): the layered BSDF's
+// weight per hero lane (projected by |wi.y| unless flags ask otherwise) and its summed pdf.
+// out: weight[4], pdf, transmission flags
+extern "C" int oracle_eval_surface(const ypt_material* m, const float* wo, const float* wi, float hero, uint32_t flags,
+                                   int reciprocal, float* out6) {
+  try {
+    Tables& T = tables();
+    MaterialGPU mg = convert_material(T.r2s, *m, 0);
+    SpectralTables st;
+    st.cie_xyz4 = YPT_CIE_XYZ;
+    st.d65 = YPT_CIE_D65;
+    Lambdas lam = spectral_lambdas(hero);
+    vec3 n(0.f, 1.f, 0.f), t(1.f, 0.f, 0.f), o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    SurfaceDef sd = make_surface(mg, n, n, t, o, true, false, lam, st);
+    SurfaceContext c;
+    make_surface_context(c, sd, 1.f, 1.f, o, true, lam, reciprocal != 0);
+    MSLuts L;
+    L.base = T.msl.data();
+    SurfaceEval e = evaluate_surface(c, L, o, i, flags);
+    for (int k = 0; k < 4; ++k) out6[k] = e.weight.s[k];
+    out6[4] = e.pdf;
+    out6[5] = (float)e.transmission;
+    return YPT_OK;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return YPT_E_INVALID;
+  }
 }

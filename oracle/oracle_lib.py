@@ -106,7 +106,8 @@ FN = {"d_ggx": 0, "pdf_wm_ggx": 1, "sample_wm_ggx": 2, "fresnel_dielectric": 3, 
       "hash_pair": 13, "owen_scramble": 14, "uint_to_float01": 15, "ss_dir": 16, "ss_avg": 17, "ssms_dir": 18,
       "sheen_dir": 19, "tr_dir": 20, "emission_profile": 21, "f32_to_f16": 22, "f16_to_f32": 23, "accumulate_ri": 24,
       "pdf_ratio": 25, "burley_profile_area": 26, "burley_sample_radius": 27, "burley_d": 28,
-      "pdf_subsurface_exit_plane": 29, "sellmeier_ior": 30, "planck_relative": 31}
+      "pdf_subsurface_exit_plane": 29, "sellmeier_ior": 30, "planck_relative": 31, "avg_fresnel": 32,
+      "glambda_sheen": 33, "d_sheen": 34, "cauchy_ior": 35, "apply_thin_film": 36, "ssms_avg": 37, "tr_avg": 38}
 
 
 def evaluate(name, args):
@@ -131,3 +132,20 @@ def rgb_reproduce(coeffs):
     out = np.zeros(3, np.float32)
     _check(lib.oracle_rgb_reproduce(_f(c), _f(out)))
     return out
+
+
+EVAL_NONE, EVAL_PDF_ONLY, EVAL_LIGHT_PATH, EVAL_TREAT_AS_DELTA, EVAL_DONT_PROJECT = 1, 2, 4, 8, 16
+
+
+def eval_surface(material, wo, wi, hero, flags=EVAL_NONE, reciprocal=False):
+    """The layered BSDF of one untextured material (a yapt_amd._lib.Material) at a point whose normal
+    is +y and tangent +x, seen from outside: (weight per hero lane (4), pdf, transmission flags)."""
+    lib = load()
+    lib.oracle_eval_surface.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                        ctypes.c_float, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    o = np.ascontiguousarray(wo, np.float32)
+    i = np.ascontiguousarray(wi, np.float32)
+    out = np.zeros(6, np.float32)
+    _check(lib.oracle_eval_surface(ctypes.addressof(material), _f(o), _f(i), float(hero), int(flags), int(bool(reciprocal)),
+                                   _f(out)))
+    return out[:4].copy(), float(out[4]), int(out[5])
