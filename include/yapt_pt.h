@@ -210,9 +210,10 @@ int ypt_get_stats(ypt_integrator* h, double* out16);
  *  Bidirectional integrator: "bdpt_wavefront" 0/1 (camera subpaths as queued kernels rather than
  *  one kernel per texel), "bdpt_waves" 1 | 2 | 3 | 4 | 6 | 8 (register budget of the camera kernels;
  *  the queued shading kernel takes 1, 2 or 4).
- *  None of them changes a result bit of the unidirectional integrators. With "bdpt_wavefront" on,
- *  the visible connections of a texel are added atomically, in an order that can vary from run to
- *  run (agreement to about 1e-6 relative). */
+ *  None of them changes a result bit: the visible connections of a camera vertex are added in the
+ *  order they were made whichever "bdpt_wavefront" form runs. (Scenes with subsurface materials add
+ *  the two shadow rays of a subsurface event atomically, which can change the last bits of a sum
+ *  from run to run.) */
 int ypt_set_option(ypt_integrator* h, const char* name, int64_t value);
 
 /* Closest-hit queries against the scene's BVH on the GPU (the TraceRay / RayQuery the integrators

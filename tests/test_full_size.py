@@ -1,8 +1,9 @@
 """The benchmark workloads at full size on the GPU, checked against the checker where it is cheap: the
 1M-triangle random-shapes scene (seed 2) with closest hits of 50,000 rays bit for bit through both
 BVH widths, NEE at 1920x1080 and depth 12 with the per-sample radiance of every 97th pixel of the
-last pass, and BDPT at 1280x720 with 8 emissive triangles and light subpaths of 8 vertices, checked
-on every 7th camera workgroup. Wavelengths must match bit for bit and radiance within 1e-4."""
+last pass, and BDPT at 1280x720 with 8 emissive triangles and light subpaths of 9 vertices (camera and
+light subpaths of depth 8, as benchmarked), checked on every 7th camera workgroup. Wavelengths must
+match bit for bit and radiance within 1e-4 (in practice every bit matches)."""
 import numpy as np
 import pytest
 
@@ -42,11 +43,14 @@ def test_full_size_closest_hits_are_bit_exact(ylib, oracle, shapes_1m, width):
     assert np.array_equal(t.view(np.uint32), rt.view(np.uint32))
 
 
-def _compare_last(g_last, ref_last, rtol=1e-4):
+def _compare_last(g_last, ref_last):
     traced = ref_last[..., 3] != 0
     assert traced.sum() > 1000
-    assert np.array_equal(g_last[traced][:, 3].view(np.uint32), ref_last[traced][:, 3].view(np.uint32))
-    assert np.allclose(g_last[traced][:, :3], ref_last[traced][:, :3], rtol=rtol, atol=1e-6)
+    g, r = g_last[traced], ref_last[traced]
+    assert np.array_equal(g[:, 3].view(np.uint32), r[:, 3].view(np.uint32))
+    d = np.abs(g[:, :3].astype(np.float64) - r[:, :3])
+    assert np.all((d <= 1e-4 * np.abs(r[:, :3])) | (d == 0)), d.max()
+    assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
 
 
 def test_full_size_nee_samples_match_the_checker(ylib, oracle, shapes_1m):
@@ -64,7 +68,7 @@ def test_full_size_nee_samples_match_the_checker(ylib, oracle, shapes_1m):
 
 def test_full_size_bdpt_samples_match_the_checker(ylib, oracle, shapes_1m_bdpt):
     scene, cam = shapes_1m_bdpt
-    frame = camera.make_frame(cam, 1280, 720, max_depth=8, light_path_vertices=8)
+    frame = camera.make_frame(cam, 1280, 720, max_depth=8, light_path_vertices=9)
     it = ylib.Integrator(_lib.YPT_INTEGRATOR_BDPT, 0)
     it.set_scene(scene)
     it.set_frame(frame)

@@ -86,10 +86,12 @@ def _compare_render(ylib, oracle, scene, frame, integrator, passes, begin=0, wid
     c_last, c_film = ref["last"], ref["film"]
     assert np.array_equal(g_last[..., 3].view(np.uint32), c_last[..., 3].view(np.uint32)), "hero wavelengths"
     d = np.abs(g_last[..., :3].astype(np.float64) - c_last[..., :3])
-    rel = d / np.maximum(np.abs(c_last[..., :3]), 1e-6)
-    frac_bad = float(((rel > 1e-4) & (d > 1e-7)).mean())
-    assert frac_bad < 1e-3, "per-sample radiance: %.4g of values beyond 1e-4 relative (max rel %.3g)" % (frac_bad, rel.max())
-    assert np.allclose(g_film[..., :3], c_film[..., :3], rtol=1e-4, atol=1e-6)
+    # every per-sample value within 1e-4 relative (the bar), with no exempt fraction; in practice the
+    # shared FP32 core, the same hits and one fixed order of every sum give the same bits
+    rel = d / np.maximum(np.abs(c_last[..., :3]), 1e-30)
+    assert np.all((rel <= 1e-4) | (d <= 1e-12)), "per-sample radiance: max rel %.3g" % rel.max()
+    assert np.array_equal(g_last.view(np.uint32), c_last.view(np.uint32)), "per-sample XYZ bits (max |d| %.3g)" % d.max()
+    assert np.array_equal(g_film.view(np.uint32), c_film.view(np.uint32)), "film bits"
     return g_film
 
 
@@ -235,8 +237,7 @@ def test_graph_batches_match_eager(ylib, shapes_small, integrator):
 
 
 def test_bdpt_wavefront_matches_megakernel(ylib, shapes_small):
-    # the same camera subpaths traced as one kernel per texel or as queues; only the order in which
-    # visible connections are added differs
+    # the same camera subpaths traced as one kernel per texel or as queues
     from yapt_amd import camera
     scene, cam = shapes_small
     frame = camera.make_frame(cam, 72, 48, max_depth=8, light_path_vertices=8)
@@ -250,10 +251,9 @@ def test_bdpt_wavefront_matches_megakernel(ylib, shapes_small):
         out.append((it.last_samples(), it.film()))
         it.close()
     (la, fa), (lb, fb) = out
-    assert np.array_equal(la[..., 3], lb[..., 3])
-    d = np.abs(la[..., :3].astype(np.float64) - lb[..., :3])
-    assert np.all(d <= 1e-5 * np.abs(la[..., :3]) + 1e-7), d.max()
-    assert np.allclose(fa, fb, rtol=1e-5, atol=1e-7)
+    # connection records added in queue order: the wavefront adds in the megakernel's order
+    assert np.array_equal(la.view(np.uint32), lb.view(np.uint32))
+    assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
 
 
 @pytest.mark.gpu

@@ -199,7 +199,7 @@ def test_camera_frame_matches_a_look_at_camera():
     g = camera.make_frame(camera.Camera(eye=(0, 3, 6), target=(0, 0, 0), fovy=0.8), 96, 64)
     assert np.allclose(list(f.view_to_world), list(g.view_to_world), atol=1e-6)
     assert np.allclose(list(f.uv_to_view), list(g.uv_to_view), atol=1e-6)
-    assert (f.width, f.height, f.max_depth) == (96, 64, 8)
+    assert (f.width, f.height, f.max_depth) == (96, 64, 32)   # maxRayDepth of RaytraceCommonResources.cpp:80
 
 
 # ---------------------------------------------------------------- resources and scripts

@@ -274,7 +274,11 @@ __global__ void k_bdpt_accum(const uint32_t* keys, const uint32_t* vals, const f
 // The camera subpaths of a pass as queues in HBM: extension rays go through the persistent
 // closest-hit kernel, each vertex is shaded by k_bdpt_cshade (surface, emission, every connection's
 // contribution and visibility ray, next direction), and the connection rays go through the
-// persistent occlusion kernel, which adds the visible ones to the path's total atomically.
+// persistent occlusion kernel. Each connection a vertex queues has a record of its own (zeroed when
+// queued; the occlusion kernel adds the contribution when the ray is unoccluded, one ray per record),
+// and the next shading launch of the path, or k_bdpt_cfinish, adds the records to the path's total in
+// the order they were queued: the order in which the megakernel and the oracle add them, so every
+// form of the integrator gives the same bits.
 struct CamSoA {
   float4* orgLambda;   // ray origin, hero wavelength
   float4* dirPdf;      // ray direction, pdfForwardCamera
@@ -289,7 +293,20 @@ struct CamSoA {
   float4* volIor;
   uint4* volMat;
   float4* volAbs;      // 4 per slot
+  float4* conn;        // connection records: connCap per slot
+  uint32_t* pending;   // records of the slot's last shaded vertices not yet added to its total
+  uint32_t connCap;
 };
+
+// adds the slot's pending connection records to its total, in the order they were queued
+__device__ __forceinline__ void fold_connections(const CamSoA& C, uint32_t s, Spectrum& total) {
+  uint32_t n = C.pending[s];
+  const float4* r = C.conn + (size_t)s * C.connCap;
+  for (uint32_t j = 0; j < n; ++j) {
+    float4 v = r[j];
+    total = total + Spectrum(v.x, v.y, v.z, v.w);
+  }
+}
 
 __device__ __forceinline__ void load_cam(const CamSoA& C, uint32_t s, CameraPath& c, uint32_t& lp, uint32_t& count) {
   float4 a = C.orgLambda[s], b = C.dirPdf[s], t = C.thr[s], l = C.total[s], pv = C.prev[s], pn = C.prevN[s];
@@ -396,6 +413,7 @@ __global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuff
   CameraPath c;
   camera_begin(fc, point, q & 0xFFFFu, q >> 16, lambda, hc == 0 && t == 0, c);
   store_cam(C, i, c, lp, count, false);
+  C.pending[i] = 0u;
   Q.ray[0][(q >> 16) * W + (q & 0xFFFFu)] = i;  // primary rays queued in screen order: coherent traversal
   if (features) {
     FirstHitFeatures none;
@@ -405,15 +423,21 @@ __global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuff
 }
 #endif
 
-struct QueueSink {  // connection rays into the shadow queue, one atomic per wave and connection index
+// connection rays into the shadow queue (one atomic per wave and connection index), each with a
+// zeroed record of the slot that the occlusion kernel fills when the ray is unoccluded
+struct QueueSink {
   Queues Q;
-  uint32_t slot;
+  float4* records;  // this slot's records
+  uint32_t first;   // index of records[0] among all records
+  uint32_t count;
   __device__ __forceinline__ void operator()(CameraPath&, const PendingConnection& pc) {
     uint32_t si = wave_append(&Q.counters[2], true);
     Q.sOrgTmax[si] = make_float4(pc.origin.x, pc.origin.y, pc.origin.z, pc.tmax);
     Q.sDirTri[si] = make_float4(pc.dir.x, pc.dir.y, pc.dir.z, __uint_as_float(pc.tri));
     Q.sContrib[si] = make_float4(pc.contrib.s[0], pc.contrib.s[1], pc.contrib.s[2], pc.contrib.s[3]);
-    Q.sSlot[si] = slot;
+    records[count] = make_float4(0.f, 0.f, 0.f, 0.f);
+    Q.sSlot[si] = first + count;
+    count++;
   }
 };
 
@@ -439,6 +463,7 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
       CameraPath c;
       uint32_t lp, count;
       load_cam(C, slot, c, lp, count);
+      fold_connections(C, slot, c.total);  // the previous vertex's visible connections, in queue order
       uint32_t nvolIn = c.rs.n;
       float4 hv = Q.hit[slot];
       Hit h;
@@ -447,7 +472,9 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
       const LightNodePacked* nodes = lp != 0xFFFFFFFFu ? B.nodes + (size_t)lp * BDPT_MAX_LIGHT_VERTICES : nullptr;
       QueueSink sink;
       sink.Q = Q;
-      sink.slot = slot;
+      sink.first = slot * C.connCap;
+      sink.records = C.conn + sink.first;
+      sink.count = 0;
       NodeSurfaceTable surfs;
       surfs.p = nodes ? B.surf + (size_t)lp * BDPT_MAX_LIGHT_VERTICES : nullptr;
       FirstHitFeatures fh;
@@ -466,6 +493,7 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
         camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none);  // features written below only when requested
       }
       store_cam(C, slot, c, lp, count, nvolIn > 0 || c.rs.n > 0);
+      C.pending[slot] = sink.count;
       cont = c.alive;
       if (features && first && fh.hit) {
         uint32_t q = slot_texel(fc, B, slot);
@@ -484,8 +512,10 @@ __global__ void k_bdpt_cfinish(FrameConstants fc, DevScene S, BdptBuffers B, Cam
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W * H) return;
   float4 l = C.total[i];
+  Spectrum total(l.x, l.y, l.z, l.w);
+  fold_connections(C, i, total);  // the last shaded vertex's visible connections
   float lambda = C.orgLambda[i].w;
-  vec3 xyz = spectrum_to_xyz(Spectrum(l.x, l.y, l.z, l.w), spectral_lambdas(lambda), S.sv.spec);
+  vec3 xyz = spectrum_to_xyz(total, spectral_lambdas(lambda), S.sv.spec);
   uint32_t q = slot_texel(fc, B, i);
   B.camXYZ[(q >> 16) * W + (q & 0xFFFFu)] = make_float4(xyz.x, xyz.y, xyz.z, lambda);
 }

@@ -411,8 +411,10 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
   }
 }
 
-// ATOMIC: several shadow rays of one path slot in one launch (bidirectional connections), so the
-// contributions are added atomically
+// ATOMIC: two shadow rays of one path slot in one launch (the vertex and the exit point of a
+// subsurface event), so the contributions are added atomically. Otherwise each queue entry's slot
+// owns its P.light entry in this launch (one ray per path, or one connection record per ray of the
+// bidirectional camera wavefront), which takes the plain read-add-write.
 template <int WIDTH, bool COUNT, bool ATOMIC = false, bool ALPHA = true>
 __global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, uint32_t refill, uint32_t leafShare) {
   extern __shared__ uint32_t lds[];

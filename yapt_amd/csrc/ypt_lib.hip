@@ -185,7 +185,8 @@ struct ypt_integrator {
   DevBuf<float4> cOrg, cDir, cThr, cTotal, cPrev, cPrevN, cVolIor, cVolAbs, cSOrg, cSDir, cSContrib;
   DevBuf<uint4> cSt0, cSt1, cVolMat;
   DevBuf<float> cDeferred;
-  DevBuf<uint32_t> cSSlot;
+  DevBuf<uint32_t> cSSlot, cPending;
+  DevBuf<float4> cConn;  // connection records, connCap per texel
   int bdptWavefront = 1;
   size_t bSortBytes = 0;
 
@@ -678,6 +679,7 @@ void ensure_bdpt(ypt_integrator* h) {
     // a subsurface entry and its exit are shaded in one launch: twice the connections per path
     size_t nsh = (size_t)npix * h->fc.lightPathVertices * (h->res->subsurface ? 2u : 1u);
     h->cSOrg.ensure(nsh); h->cSDir.ensure(nsh); h->cSContrib.ensure(nsh); h->cSSlot.ensure(nsh);
+    h->cConn.ensure(nsh); h->cPending.ensure(npix);
     ensure_slots(h, npix);
     ensure_overflow(h);
   }
@@ -694,9 +696,13 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
   C.orgLambda = h->cOrg.p; C.dirPdf = h->cDir.p; C.thr = h->cThr.p; C.total = h->cTotal.p;
   C.st0 = h->cSt0.p; C.st1 = h->cSt1.p; C.prev = h->cPrev.p; C.prevN = h->cPrevN.p; C.deferred = h->cDeferred.p;
   C.volIor = h->cVolIor.p; C.volMat = h->cVolMat.p; C.volAbs = h->cVolAbs.p;
+  C.conn = h->cConn.p; C.pending = h->cPending.p;
+  C.connCap = fc.lightPathVertices * (h->res->subsurface ? 2u : 1u);
   PathSoA P;
   std::memset(&P, 0, sizeof(P));
-  P.orgLambda = C.orgLambda; P.dirPdf = C.dirPdf; P.light = C.total;
+  // the occlusion kernel adds each unoccluded connection to its own record (one ray per record, so
+  // the plain read-add-write form); the queue entry's slot is the record index
+  P.orgLambda = C.orgLambda; P.dirPdf = C.dirPdf; P.light = C.conn;
   Queues Q = h->queues();
   Q.sOrgTmax = h->cSOrg.p; Q.sDirTri = h->cSDir.p; Q.sContrib = h->cSContrib.p; Q.sSlot = h->cSSlot.p;
   Q.workCounters = h->work.p + 8;
@@ -724,9 +730,9 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
     else
       hipLaunchKernelGGL(k_bdpt_cshade<4>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     if (COUNT || alpha)
-      YPT_PERSISTENT((k_shadow_p<WIDTH, COUNT, true>), sgrid, Bk, lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_shadow_p<WIDTH, COUNT, false, true>), sgrid, Bk, lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
     else
-      YPT_PERSISTENT((k_shadow_p<WIDTH, false, true, false>), sgrid, Bk, lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
+      YPT_PERSISTENT((k_shadow_p<WIDTH, false, false, false>), sgrid, Bk, lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
     q ^= 1;
   }
   hipLaunchKernelGGL(k_bdpt_cfinish, dim3(blocks_for(npix, 256)), dim3(256), 0, s, fc, S, B, C);
@@ -1530,7 +1536,7 @@ void plan_job(ServerJob& j, int cus) {
     bytes += npaths * BDPT_MAX_LIGHT_VERTICES * (sizeof(LightNodePacked) + sizeof(LightNodeSurface)) + npaths * 15 * (4 * 4 + 16) +
              npix * (16 + 4) + npaths * 64 * 48 * 4;
     // camera subpaths as a wavefront: state per texel, and connection rays for every light vertex
-    bytes += npix * (16 * 9 + 4 + 16 * 4) + npix * BDPT_MAX_LIGHT_VERTICES * (16 * 3 + 4);
+    bytes += npix * (16 * 9 + 4 + 16 * 4 + 4) + npix * BDPT_MAX_LIGHT_VERTICES * (16 * 3 + 4 + 16);
     j.passesInFlight = 1;
     j.chunk = std::min<uint32_t>(j.spp, 2);
   } else {

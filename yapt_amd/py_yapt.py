@@ -694,8 +694,9 @@ def _renderer_variables(r):
         ("Tonemap.ExposureCompensation", F, 0.0, -10000.0, 10000.0, None),
         ("Tonemap.EyeAdaptationSpeed", F, 1.0, 0.0, 1000.0, None),
         ("Tonemap.ManualExposure", F, 1.0, 0.0, 9999999999999999.0, None),
-        # the bounce limit of the path loop and BDPT's light subpath length, exposed here as variables
-        ("Integrator.MaxDepth", I, 8, 1, 64, None),
+        # the bounce limit of the path loop (the reference fixes maxRayDepth = 32,
+        # RaytraceCommonResources.cpp:80) and BDPT's light subpath length, exposed here as variables
+        ("Integrator.MaxDepth", I, 32, 1, 64, None),
         ("Integrator.LightPathVertices", I, 16, 1, 16, None),
     ]
     return {name: RendererVariable(r, name, kind, d, lo, hi, opts) for name, kind, d, lo, hi, opts in defs}

@@ -291,13 +291,49 @@ def cornell_box():
     return b.arrays(), cam
 
 
+def focused_emission(focus=8.0, two_sided=False):
+    """The focused-emitter box of the reference's integrator comparison (Tests/IntegratorCompare/
+    focused_emission_scene.py:51-77) from generated quads: a closed room of white diffuse PLASTIC
+    (albedo 0.7, roughness 1, no specular) whose floor (y = 0), ceiling (y = 40) and walls (x, z = +-20)
+    are 80x80 quads turned inwards, and one 12x12 emitter (emission 8, cos^n focus `focus`) 2 units
+    below the ceiling facing the floor; a two-sided emitter hangs in the middle of the room instead.
+    The reference's plane.glb is a 2x2 quad (load_material_showcase_scene.py:583) scaled by 40 for the
+    room and 6 for the emitter. Camera at (0, 22, 18) pitched 25 degrees down, the default 54-degree
+    field of view; environment scale 0, manual exposure 1."""
+    b = MeshBuilder()
+    room = preset_material("PLASTIC")
+    room.albedo[:] = (0.7, 0.7, 0.7)
+    room.roughness = 1.0
+    room.specular_amount = 0.0
+    rm = b.material(room)
+    L, h, S = 40.0, 20.0, 80.0   # room height, half width, quad size (2 x scale 40)
+    b.quad((-L, 0, L), (S, 0, 0), (0, 0, -S), rm)        # floor, +y
+    b.quad((-L, L, -L), (S, 0, 0), (0, 0, S), rm)        # ceiling, -y
+    b.quad((L, h - L, h), (-S, 0, 0), (0, S, 0), rm)     # wall z = +20, -z
+    b.quad((h, h - L, -L), (0, 0, S), (0, S, 0), rm)     # wall x = +20, -x
+    b.quad((-L, h - L, -h), (S, 0, 0), (0, S, 0), rm)    # wall z = -20, +z
+    b.quad((-h, h - L, L), (0, 0, -S), (0, S, 0), rm)    # wall x = -20, +x
+    light = preset_material("BLANK")
+    light.emissive[:] = (8.0, 8.0, 8.0)
+    light.emission_focus = float(focus)
+    light.material_mask = 1 if two_sided else 0          # MaterialMask_TwoSided
+    lm = b.material(light)
+    y = h if two_sided else L - 2.0
+    b.quad((-6.0, y, -6.0), (12.0, 0, 0), (0, 0, 12.0), lm)  # scale 6, facing -y
+    pitch = math.radians(25.0)
+    eye = (0.0, 22.0, h - 2.0)
+    cam = Camera(eye=eye, target=(0.0, 22.0 - math.sin(pitch), h - 2.0 - math.cos(pitch)), fovy=math.pi * 0.3)
+    return b.arrays(), cam
+
+
 def punctual_lights(light_type="mixed", sphere_res=(24, 24)):
     """The punctual-light test scene (Tests/IntegratorCompare/punctual_lights_scene.py:16-105) built
-    from generated meshes: a 40x40 floor and back wall, diffuse, copper and plastic spheres, lit by
+    from generated meshes: an 80x80 floor and back wall, diffuse, copper and plastic spheres, lit by
     punctual lights only (environment scale 0). light_type: point, spot, directional, mixed (all three
     plus an emissive sphere), mesh_control (a small emissive sphere where the point light is), env (no
     lights: the scene is lit by the environment at scale 0.5, returned as the third value).
-    The reference's plane.glb and Sphere.glb are absent; the floor is a 40x40 quad and the spheres
+    The reference's plane.glb (a 2x2 quad, load_material_showcase_scene.py:583) and Sphere.glb (unit
+    radius) are absent: the planes here are quads of twice their render objects' scale, and the spheres
     have radius = scale. Returns (scene, camera, env_intensity_scale)."""
     b = MeshBuilder()
     diffuse = preset_material("PLASTIC")
@@ -312,8 +348,8 @@ def punctual_lights(light_type="mixed", sphere_res=(24, 24)):
     plastic.albedo[:] = (0.2, 0.4, 0.8)
     plastic.roughness = 0.25
     pl = b.material(plastic)
-    b.quad((-20, 0, 20), (40, 0, 0), (0, 0, -40), d)      # floor, +y
-    b.quad((-20, 0, -20), (40, 0, 0), (0, 40, 0), d)      # back wall at z = -20, +z
+    b.quad((-40, 0, 40), (80, 0, 0), (0, 0, -80), d)      # floor (scale 40), +y
+    b.quad((-40, -20, -20), (80, 0, 0), (0, 80, 0), d)    # back wall at z = -20 (scale 40, centre y = 20), +z
     P, N, I = sphere_template(*sphere_res)
 
     def sphere(centre, radius, mat):
