@@ -103,6 +103,15 @@ def test_render_parity_cornell(ylib, oracle, cornell, integrator):
     _compare_render(ylib, oracle, scene, frame, integrator, passes=4)
 
 
+def test_render_parity_cornell_configured_size(ylib, oracle, cornell):
+    # the Cornell configuration as stated: NEE, 256x256, 16 spp, depth 8
+    from yapt_amd import camera
+    scene, cam = cornell
+    frame = camera.make_frame(cam, 256, 256, max_depth=8)
+    film = _compare_render(ylib, oracle, scene, frame, ylib.YPT_INTEGRATOR_NEE, passes=16)
+    assert film[..., 1].mean() > 0.02
+
+
 def test_render_parity_shapes_nee(ylib, oracle, shapes_small):
     from yapt_amd import camera
     scene, cam = shapes_small
