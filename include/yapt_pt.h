@@ -221,6 +221,18 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value);
  * index (0xFFFFFFFF on a miss) and barycentrics b1, b2 (may be NULL). */
 int ypt_trace_closest(ypt_integrator* h, const float* rays, uint32_t n, float* t, uint32_t* tri, float* b12);
 
+/* ypt_trace_closest on device buffers, allocating nothing. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). rays: n x 8 floats in device memory; t, tri, b12 (none may be
+ * NULL): device outputs as in ypt_trace_closest. Asynchronous on `stream` (NULL: the handle's stream).
+ * The traversal-stack overflow lives in a 16 MiB device array of the library, one per GPU, held until
+ * the process ends; rays run in launches of as many rays as it holds (65536 for a stack bound of 64).
+ * A BVH whose stack would not fit 64 rays is refused (YPT_E_MEMORY). The array needs no clearing, and
+ * every call on one GPU shares it: calls that overlap on one GPU (on different streams, or from
+ * different threads) race. */
+int ypt_trace_closest_noalloc(ypt_integrator* h, const float* rays, uint32_t n, float* t, uint32_t* tri, float* b12,
+                              void* stream);
+
 /* Scrambled sampler values on the GPU: for each (seed, dimension) pair the value of
  * randomSampler.hlsl getRandomSampleFloat for Sobol sample `sample_index`. */
 int ypt_sample_dims(ypt_integrator* h, uint32_t sample_index, const uint32_t* seeds, const uint32_t* dims,

@@ -262,7 +262,7 @@ EXPORTED_SYMBOLS = [
     "ypt_scene_create", "ypt_scene_destroy", "ypt_scene_info", "ypt_set_resident_scene",
     "ypt_server_create", "ypt_server_destroy", "ypt_server_submit", "ypt_server_wait", "ypt_server_take",
     "ypt_server_stats", "ypt_clock", "ypt_server_set_option", "ypt_final_color", "ypt_post_params_default",
-    "ypt_get_feature_moments", "ypt_get_film_variance",
+    "ypt_get_feature_moments", "ypt_get_film_variance", "ypt_trace_closest_noalloc",
 ]
 
 

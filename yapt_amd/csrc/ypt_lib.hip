@@ -108,6 +108,35 @@ HostTables& host_tables() {
 
 uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
+// Device scratch of ypt_trace_closest_noalloc: the traversal-stack overflow of its launches, in a
+// file-scope array rather than a per-call allocation. Each GPU holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress returns on that device
+// (never the array's name, which is a host-side stand-in). Every call on one GPU shares it.
+constexpr size_t NOALLOC_SCRATCH_WORDS = (size_t)4 << 20;  // 16 MiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratchMutex;
+void* g_scratchAddr[NOALLOC_MAX_DEVICES] = {};
+std::string g_scratchError;
+
+}  // namespace
+
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCRATCH_WORDS];
+
+namespace {
+
+// the scratch array's address on `device` (current), looked up once per device at handle creation
+void init_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratchMutex);
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES || g_scratchAddr[device]) return;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_scratchError = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return;
+  }
+  g_scratchAddr[device] = p;
+}
+
 }  // namespace
 
 // A scene resident in device memory: host copy, BVH and device buffers. Integrator handles and the
@@ -972,6 +1001,7 @@ int ypt_create(int integrator_type, int device, ypt_integrator** out) {
     h->stream = h->ownStream;
     HIP_CHECK(hipEventCreateWithFlags(&h->streamJoin, hipEventDisableTiming));
     for (auto& e : h->ev) HIP_CHECK(hipEventCreate(&e));
+    init_scratch(device);
     upload_constant_tables(h.get());
     *out = h.release();
     return YPT_OK;
@@ -1416,6 +1446,42 @@ int ypt_trace_closest(ypt_integrator* h, const float* rays, uint32_t n, float* t
     HIP_CHECK(hipMemcpy(t, dt.p, n * sizeof(float), hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(tri, dtri.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (b12) HIP_CHECK(hipMemcpy(b12, db.p, 2 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    return YPT_OK;
+  });
+}
+
+int ypt_trace_closest_noalloc(ypt_integrator* h, const float* rays, uint32_t n, float* t, uint32_t* tri, float* b12,
+                              void* stream) {
+  if (!h || !rays || !t || !tri || !b12) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (!h->hasScene) return fail(YPT_E_NOSCENE, "no scene");
+    if (n == 0) return YPT_OK;
+    if (h->device < 0 || h->device >= NOALLOC_MAX_DEVICES || !g_scratchAddr[h->device])
+      return fail(YPT_E_DEVICE, "no scratch address for this device: " + g_scratchError);
+    uint32_t bound = h->bvhWidth == 8 ? h->res->stackBound8 : h->res->stackBound2;
+    size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
+    // rays per launch: one overflow column of `depth` words each, within the scratch array
+    size_t chunk = NOALLOC_SCRATCH_WORDS / depth;
+    if (chunk < 64)
+      return fail(YPT_E_MEMORY, "the BVH's traversal stack (" + std::to_string(depth) + " words past the LDS) exceeds the " +
+                                    std::to_string(NOALLOC_SCRATCH_WORDS * 4 >> 20) + " MiB scratch of ypt_trace_closest_noalloc");
+    chunk = std::min<size_t>(chunk & ~(size_t)63, (size_t)1 << 24);
+    HIP_CHECK(hipSetDevice(h->device));
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    uint32_t* ovf = (uint32_t*)g_scratchAddr[h->device];
+    DevScene S = h->dev_scene();
+    size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
+    for (size_t first = 0; first < n; first += chunk) {
+      uint32_t m = (uint32_t)std::min<size_t>(chunk, n - first);
+      const float* r = rays + first * 8;
+      if (h->bvhWidth == 8)
+        hipLaunchKernelGGL(k_trace_closest<8>, dim3(blocks_for(m, h->blockSize)), dim3(h->blockSize), lds, s, S, r, m, t + first,
+                           tri + first, b12 + 2 * first, ovf, (uint32_t)h->ldsDepth);
+      else
+        hipLaunchKernelGGL(k_trace_closest<2>, dim3(blocks_for(m, h->blockSize)), dim3(h->blockSize), lds, s, S, r, m, t + first,
+                           tri + first, b12 + 2 * first, ovf, (uint32_t)h->ldsDepth);
+      HIP_CHECK(hipGetLastError());
+    }
     return YPT_OK;
   });
 }
