@@ -8,8 +8,8 @@
 // mergeNewSamples.comp:173-265. Surface and light arithmetic (layered BSDF, spectra, sampler,
 // camera) comes from the shared FP32 core under yapt_amd/csrc/core, compiled here by g++ with
 // -ffp-contract=off; what this file owns is independent of the GPU path:
-//   * closest hit by brute force over all triangles, or through its own median-split BVH
-//     (both give the minimum (t, triangle id) of the shared watertight test);
+//   * closest hit by brute force over all triangles, or through its own binned-SAH BVH traversed
+//     near child first (both give the minimum (t, triangle id) of the shared watertight test);
 //   * the path loop and the per-pixel film merge (clamp against the previous pass's film);
 //   * Sobol points straight from the generator matrices.
 // Parity unpinned against the reference itself (no per-sample vectors exist); pinned by the tests
@@ -120,15 +120,69 @@ struct Geometry {
       nd.left = nd.right = 0;
       nd.first = it.first;
       nd.count = it.count;
-      if (it.count > 4) {  // median split on the widest centroid axis
+      if (it.count > 4) {  // binned SAH split (16 bins per axis over the centroid bounds); median if none pays
         float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
         for (uint32_t i = it.first; i < it.first + it.count; ++i)
           for (int k = 0; k < 3; ++k) { clo[k] = std::min(clo[k], cent[3 * order[i] + k]); chi[k] = std::max(chi[k], cent[3 * order[i] + k]); }
-        int ax = 0;
-        for (int k = 1; k < 3; ++k) if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
-        uint32_t mid = it.first + it.count / 2;
-        std::nth_element(order.begin() + it.first, order.begin() + mid, order.begin() + it.first + it.count,
-                         [&](uint32_t a, uint32_t b) { return cent[3 * a + ax] < cent[3 * b + ax] || (cent[3 * a + ax] == cent[3 * b + ax] && a < b); });
+        const int NB = 16;
+        int bestAx = -1;
+        float bestCut = 0.f, bestCost = FLT_MAX;
+        for (int ax = 0; ax < 3; ++ax) {
+          float ext = chi[ax] - clo[ax];
+          if (!(ext > 0.f)) continue;
+          float blo[NB][3], bhi[NB][3];
+          uint32_t bcnt[NB] = {0};
+          for (int b = 0; b < NB; ++b)
+            for (int k = 0; k < 3; ++k) { blo[b][k] = FLT_MAX; bhi[b][k] = -FLT_MAX; }
+          for (uint32_t i = it.first; i < it.first + it.count; ++i) {
+            int b = std::min(NB - 1, (int)((cent[3 * order[i] + ax] - clo[ax]) / ext * NB));
+            vec3 v[3];
+            tri(order[i], v[0], v[1], v[2]);
+            bcnt[b]++;
+            for (int j = 0; j < 3; ++j)
+              for (int k = 0; k < 3; ++k) { blo[b][k] = std::min(blo[b][k], v[j][k]); bhi[b][k] = std::max(bhi[b][k], v[j][k]); }
+          }
+          auto area = [](const float lo[3], const float hi[3]) {
+            float e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+            return e0 < 0.f ? 0.f : 2.f * (e0 * e1 + e1 * e2 + e2 * e0);
+          };
+          float rlo[NB][3], rhi[NB][3];
+          uint32_t rc[NB];
+          float alo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, ahi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+          uint32_t acc = 0;
+          for (int b = NB - 1; b >= 1; --b) {
+            for (int k = 0; k < 3; ++k) { alo[k] = std::min(alo[k], blo[b][k]); ahi[k] = std::max(ahi[k], bhi[b][k]); }
+            acc += bcnt[b];
+            for (int k = 0; k < 3; ++k) { rlo[b][k] = alo[k]; rhi[b][k] = ahi[k]; }
+            rc[b] = acc;
+          }
+          float llo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, lhi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+          uint32_t lc = 0;
+          for (int b = 0; b < NB - 1; ++b) {
+            for (int k = 0; k < 3; ++k) { llo[k] = std::min(llo[k], blo[b][k]); lhi[k] = std::max(lhi[k], bhi[b][k]); }
+            lc += bcnt[b];
+            if (lc == 0 || rc[b + 1] == 0) continue;
+            float cost = area(llo, lhi) * lc + area(rlo[b + 1], rhi[b + 1]) * rc[b + 1];
+            if (cost < bestCost) { bestCost = cost; bestAx = ax; bestCut = clo[ax] + ext * (float)(b + 1) / NB; }
+          }
+        }
+        uint32_t mid;
+        if (bestAx >= 0) {
+          auto left = [&](uint32_t t) {
+            return std::min(NB - 1, (int)((cent[3 * t + bestAx] - clo[bestAx]) / (chi[bestAx] - clo[bestAx]) * NB)) <
+                   std::min(NB, (int)((bestCut - clo[bestAx]) / (chi[bestAx] - clo[bestAx]) * NB + 0.5f));
+          };
+          mid = (uint32_t)(std::stable_partition(order.begin() + it.first, order.begin() + it.first + it.count, left) - order.begin());
+        } else {
+          mid = it.first;
+        }
+        if (mid == it.first || mid == it.first + it.count) {  // degenerate: median on the widest axis
+          int ax = 0;
+          for (int k = 1; k < 3; ++k) if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+          mid = it.first + it.count / 2;
+          std::nth_element(order.begin() + it.first, order.begin() + mid, order.begin() + it.first + it.count,
+                           [&](uint32_t a, uint32_t b) { return cent[3 * a + ax] < cent[3 * b + ax] || (cent[3 * a + ax] == cent[3 * b + ax] && a < b); });
+        }
         uint32_t l = (uint32_t)nodes.size();
         nodes.push_back(CpuBvhNode());
         nodes.push_back(CpuBvhNode());
@@ -165,11 +219,16 @@ struct Geometry {
       if (work) work[1] += n;
     } else {
       vec3 inv(1.f / d.x, 1.f / d.y, 1.f / d.z);
+      // near child first; a popped node is skipped when its entry lies beyond the closest hit so far
       uint32_t stack[128];
+      float entry[128];
       int sp = 0;
-      stack[sp++] = 0;
+      stack[sp] = 0;
+      entry[sp++] = tmin;
       while (sp) {
-        const CpuBvhNode& nd = nodes[stack[--sp]];
+        --sp;
+        if (entry[sp] > best * SLAB_FAR_SCALE) continue;  // as conservative as the slab test itself
+        const CpuBvhNode& nd = nodes[stack[sp]];
         if (work) work[0]++;
         float tn;
         if (!slab_test(vec3(nd.lo[0], nd.lo[1], nd.lo[2]), vec3(nd.hi[0], nd.hi[1], nd.hi[2]), o, inv, tmin, best, tn)) continue;
@@ -178,8 +237,20 @@ struct Geometry {
           if (work) work[1] += nd.count;
         } else {
           if (sp + 2 > 128) throw std::runtime_error("oracle BVH stack overflow");
-          stack[sp++] = nd.right;
-          stack[sp++] = nd.left;
+          const CpuBvhNode& a = nodes[nd.left];
+          const CpuBvhNode& b = nodes[nd.right];
+          float ta = FLT_MAX, tb = FLT_MAX;
+          bool ha = slab_test(vec3(a.lo[0], a.lo[1], a.lo[2]), vec3(a.hi[0], a.hi[1], a.hi[2]), o, inv, tmin, best, ta);
+          bool hb = slab_test(vec3(b.lo[0], b.lo[1], b.lo[2]), vec3(b.hi[0], b.hi[1], b.hi[2]), o, inv, tmin, best, tb);
+          if (ha && hb) {
+            bool aFirst = ta <= tb;
+            stack[sp] = aFirst ? nd.right : nd.left; entry[sp++] = aFirst ? tb : ta;
+            stack[sp] = aFirst ? nd.left : nd.right; entry[sp++] = aFirst ? ta : tb;
+          } else if (ha) {
+            stack[sp] = nd.left; entry[sp++] = ta;
+          } else if (hb) {
+            stack[sp] = nd.right; entry[sp++] = tb;
+          }
         }
       }
     }

@@ -207,6 +207,17 @@ class YptError(RuntimeError):
 _LIB = None
 
 
+def share_hip_runtime():
+    """PyTorch's wheel carries its own HIP runtime under the soname the library links
+    (libamdhip64.so.7), and a process serves both from whichever copy was loaded first; if the system's
+    copy comes first, torch later finds no GPU. Importing torch (without initialising its device) before
+    the library is opened makes one runtime serve both, so device tensors and streams pass between them."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load():
     """Loads libyapt_pt.so (built by yapt_amd.build). Raises when it is missing."""
     global _LIB
@@ -214,6 +225,7 @@ def load():
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise YptError("libyapt_pt.so is not built; run yapt_amd.build.build() (no CPU fallback exists)")
+    share_hip_runtime()
     lib = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     lib.ypt_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]

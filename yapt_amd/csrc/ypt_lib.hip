@@ -378,13 +378,18 @@ void ensure_overflow(ypt_integrator* h) {
   h->ovfStride = (uint32_t)stride;
 }
 
-// passes in flight: enough slots (~32M paths, ~10 GB of path state) that the deep, nearly empty
-// bounce launches of one batch are shared by many passes
+// passes in flight: enough slots (~64M paths, ~22 GB of path state at 1080p) that the deep, nearly
+// empty bounce launches of one batch are shared by many passes. The passes of one render are split
+// into batches of equal size (40 passes at a limit of 32 run as 20 + 20, not 32 + 8), so no batch is
+// left with a short, latency-bound tail.
+constexpr uint32_t MAX_PASSES_IN_FLIGHT = 32;
 uint32_t passes_in_flight(const ypt_integrator* h, uint32_t count) {
   size_t npix = (size_t)h->width * h->height;
-  uint32_t g = h->passesInFlight > 0 ? (uint32_t)h->passesInFlight : (uint32_t)std::max<size_t>(1, (32u << 20) / npix);
-  g = std::min<uint32_t>(g, 16u);
-  return std::max(1u, std::min(g, count));
+  uint32_t g = h->passesInFlight > 0 ? (uint32_t)h->passesInFlight : (uint32_t)std::max<size_t>(1, (64u << 20) / npix);
+  g = std::max(1u, std::min<uint32_t>(g, MAX_PASSES_IN_FLIGHT));
+  if (count <= g) return std::max(1u, count);
+  uint32_t batches = (count + g - 1) / g;
+  return (count + batches - 1) / batches;
 }
 
 void clear_film(ypt_integrator* h) {
@@ -1051,7 +1056,8 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
   } else if (n == "first_hit_features") {
     h->writeFeatures = value != 0;
   } else if (n == "passes_in_flight") {
-    if (value < 0 || value > 16) return fail(YPT_E_INVALID, "passes_in_flight must be in [0, 16] (0 = automatic)");
+    if (value < 0 || value > (int64_t)MAX_PASSES_IN_FLIGHT)
+      return fail(YPT_E_INVALID, "passes_in_flight must be in [0, " + std::to_string(MAX_PASSES_IN_FLIGHT) + "] (0 = automatic)");
     h->passesInFlight = (int)value;
   } else if (n == "profile") {
     h->profile = value != 0;
@@ -1171,7 +1177,7 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
     hipEvent_t eAll0 = nullptr;
     if (h->profile) eAll0 = mark();
     if (h->useGraph && !h->profile && !h->countTraversal) {
-      h->batchPoints.ensure((size_t)16 * YPT_SOBOL_POINT_DIMS);
+      h->batchPoints.ensure((size_t)MAX_PASSES_IN_FLIGHT * YPT_SOBOL_POINT_DIMS);
       h->mergeDyn.ensure(2);
       hipLaunchKernelGGL(k_set_dyn, dim3(1), dim3(1), 0, s, h->mergeDyn.p, h->filmSamples, h->filmCur);
       for (uint32_t k = 0; k < sample_count; k += G) {

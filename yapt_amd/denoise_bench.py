@@ -53,48 +53,23 @@ def conv_flops(net, H, W):
     return total
 
 
-def main():
-    p = argparse.ArgumentParser()
-    p.add_argument("--scenes", type=int, default=48)
-    p.add_argument("--res", type=int, default=256)
-    p.add_argument("--spp-ref", type=int, default=1024)
-    p.add_argument("--iters", type=int, default=3000)
-    p.add_argument("--batch", type=int, default=8)
-    p.add_argument("--crop", type=int, default=128)
-    p.add_argument("--lr", type=float, default=1e-3)
-    p.add_argument("--eval-width", type=int, default=1920)
-    p.add_argument("--eval-height", type=int, default=1080)
-    p.add_argument("--eval-ref-spp", type=int, default=4096)
-    p.add_argument("--out", default=os.path.join(HERE, "data", "denoiser_weights.npz"))
-    p.add_argument("--load", default=None, help="skip training and load these weights")
-    a = p.parse_args()
+def load_weights(path=None):
+    """the trained U-Net (yapt_amd/data/denoiser_weights.npz by default)"""
+    from . import denoiser
+    net = denoiser.UNet(seed=0)
+    st = np.load(path or os.path.join(HERE, "data", "denoiser_weights.npz"))
+    net.load_state({"layers": [{"w": st["w%d" % k], "b": st["b%d" % k]} for k in range(len(net.layers()))]})
+    return net
+
+
+def judge(net, W=1920, H=1080, ref_spp=4096, log=lambda m: None):
+    """Held-out judgement on the 1M-triangle random-shapes scene (seed 2) at W x H: relMSE against a
+    ref_spp plain render for plain 64 spp and for 4 spp + the denoiser, the wall time of each (render,
+    and render + denoise, on the GPU), the reference's bilateral denoiser on the same 4-spp film, and
+    the conv kernels' achieved TFLOP/s over one forward pass."""
     import torch
     from . import _lib, camera, denoiser, scenes, tuning
-
-    log = lambda m: print(m, file=sys.stderr, flush=True)
-    net = denoiser.UNet(seed=0)
-    t_data = t_train = 0.0
-    hist = []
-    if a.load:
-        st = np.load(a.load)
-        layers = [{"w": st["w%d" % k], "b": st["b%d" % k]} for k in range(len(net.layers()))]
-        net.load_state({"layers": layers})
-    else:
-        t0 = time.time()
-        data = denoiser.make_dataset(a.scenes, res=a.res, spp_in=4, spp_ref=a.spp_ref, seed=100, log=log)
-        t_data = time.time() - t0
-        t0 = time.time()
-        hist = denoiser.train(net, data, iters=a.iters, batch=a.batch, crop=a.crop, lr=a.lr, seed=1, log=log)
-        torch.cuda.synchronize()
-        t_train = time.time() - t0
-        st = net.state()
-        np.savez(a.out, **{"w%d" % k: l["w"] for k, l in enumerate(st["layers"])},
-                 **{"b%d" % k: l["b"] for k, l in enumerate(st["layers"])}, channels=np.array(net.ch))
-        del data
-
-    # held-out judgement
     scene, cam = scenes.random_shapes(n_shapes=2000, seed=2)
-    W, H = a.eval_width, a.eval_height
     frame = camera.make_frame(cam, W, H, max_depth=12)
     it = _lib.Integrator(_lib.YPT_INTEGRATOR_NEE, 0)
     for k, v in tuning.best_config("nee-shapes2000-%dx%d-d12" % (W, H)).items():
@@ -104,9 +79,9 @@ def main():
     it.set_frame(frame)
     it.render(0, 4)   # warm-up
     it.synchronize()
-    log("reference %d spp" % a.eval_ref_spp)
+    log("reference %d spp" % ref_spp)
     it.clear_film()
-    for s0 in range(0, a.eval_ref_spp, 256):
+    for s0 in range(0, ref_spp, 256):
         it.render(10000 + s0, 256)
         it.synchronize()
     ref = it.film()
@@ -166,7 +141,7 @@ def main():
             "bilateral_4spp": denoiser.rel_mse(bil[..., :3], ref_s)}
     t_den_total = t_render4 + t_denoise
     res = {
-        "metric": "relMSE vs %d-spp plain render, 4 spp + denoiser vs plain 64 spp (%dx%d)" % (a.eval_ref_spp, W, H),
+        "metric": "relMSE vs %d-spp plain render, 4 spp + denoiser vs plain 64 spp (%dx%d)" % (ref_spp, W, H),
         "relmse": {"plain_64spp": r_plain, "plain_4spp": r_noisy, "denoised_4spp": r_den},
         "wall_s": {"plain_64spp": t_plain, "render_4spp": t_render4, "denoise": t_denoise, "denoised_4spp_total": t_den_total,
                    "bilateral_denoise": t_bil},
@@ -176,13 +151,54 @@ def main():
         "gain_over_4spp": r_noisy / r_den,
         "conv_roofline": {"bound": "mfma", "achieved": flops / (fwd_ms * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
                           "frac": flops / (fwd_ms * 1e-3) / 1e12 / 2500.0, "forward_ms": fwd_ms, "flops": flops},
-        "training": {"scenes": a.scenes, "res": a.res, "spp_ref": a.spp_ref, "iters": a.iters, "batch": a.batch,
-                     "crop": a.crop, "data_s": t_data, "train_s": t_train,
-                     "loss_first_last": [hist[0][1], hist[-1][1]] if hist else None},
         "channels": list(net.ch),
     }
-    print(json.dumps(res))
     it.close()
+    return res
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--scenes", type=int, default=48)
+    p.add_argument("--res", type=int, default=256)
+    p.add_argument("--spp-ref", type=int, default=1024)
+    p.add_argument("--iters", type=int, default=3000)
+    p.add_argument("--batch", type=int, default=8)
+    p.add_argument("--crop", type=int, default=128)
+    p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--eval-width", type=int, default=1920)
+    p.add_argument("--eval-height", type=int, default=1080)
+    p.add_argument("--eval-ref-spp", type=int, default=4096)
+    p.add_argument("--out", default=os.path.join(HERE, "data", "denoiser_weights.npz"))
+    p.add_argument("--load", default=None, help="skip training and load these weights")
+    a = p.parse_args()
+    import torch
+    from . import denoiser
+
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    net = denoiser.UNet(seed=0)
+    t_data = t_train = 0.0
+    hist = []
+    if a.load:
+        net = load_weights(a.load)
+    else:
+        t0 = time.time()
+        data = denoiser.make_dataset(a.scenes, res=a.res, spp_in=4, spp_ref=a.spp_ref, seed=100, log=log)
+        t_data = time.time() - t0
+        t0 = time.time()
+        hist = denoiser.train(net, data, iters=a.iters, batch=a.batch, crop=a.crop, lr=a.lr, seed=1, log=log)
+        torch.cuda.synchronize()
+        t_train = time.time() - t0
+        st = net.state()
+        np.savez(a.out, **{"w%d" % k: l["w"] for k, l in enumerate(st["layers"])},
+                 **{"b%d" % k: l["b"] for k, l in enumerate(st["layers"])}, channels=np.array(net.ch))
+        del data
+
+    res = judge(net, a.eval_width, a.eval_height, a.eval_ref_spp, log)
+    res["training"] = {"scenes": a.scenes, "res": a.res, "spp_ref": a.spp_ref, "iters": a.iters, "batch": a.batch,
+                       "crop": a.crop, "data_s": t_data, "train_s": t_train,
+                       "loss_first_last": [hist[0][1], hist[-1][1]] if hist else None}
+    print(json.dumps(res))
 
 
 if __name__ == "__main__":

@@ -36,6 +36,8 @@ def load():
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise DenoiseError("libyapt_denoise.so is not built; run yapt_amd.build.build() (no CPU fallback exists)")
+    from . import _lib
+    _lib.share_hip_runtime()   # one HIP runtime for torch and the library
     lib = ctypes.CDLL(LIB_PATH)
     P, I, L, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_float
     lib.ydn_conv3x3.argtypes = [P, I, I, I, I, P, P, I, I, P, P]

@@ -4,6 +4,7 @@
     python -m yapt_amd.profsum traffic <fetch dir> <write dir> <calib fetch dir> <calib write dir> \
            <kernel substring> <key> <calib MiB> [traffic.json [summary.md "<command>"...]]
     python -m yapt_amd.profsum l2 <TCC_HIT/TCC_MISS dir> <kernel substring> ["<command>"...] > summary.md
+    python -m yapt_amd.profsum sq <pmc dir> <kernel-trace dir> "<substr1,substr2,...>" ["<command>"...] > summary.md
 
 ``stats`` renders the ``*_kernel_stats.csv`` of a ``rocprofv3 --kernel-trace --stats`` run as a
 markdown table. ``traffic`` turns the FETCH_SIZE / WRITE_SIZE counters (KiB per dispatch) of one
@@ -11,6 +12,10 @@ kernel into HBM bytes per launch, each scaled by what the same counter reads for
 of known size (``yapt_amd.calib_copy``), and stores the result under ``key`` in traffic.json.
 ``l2`` gives the L2 (TCC) hit rate of a kernel from a ``--pmc TCC_HIT_sum TCC_MISS_sum`` run: the
 share of its memory requests that never reach the fabric (MALL or HBM).
+``sq`` tabulates, per kernel, the mean per dispatch of every counter of a ``--pmc`` run (rocprofv3's
+derived VALUBusy, VALUUtilization, SALUBusy, OccupancyPercent next to raw SQ instruction counts) and the
+register and LDS allocation that the kernel trace records.
+``traffic`` takes ``--launch=<json>``: the launch configuration the counters were collected at.
 """
 import csv
 import glob
@@ -110,7 +115,55 @@ def l2_markdown(r, commands):
     return "\n".join(out) + "\n"
 
 
+def counters_by_kernel(dirname, kernel_substr):
+    """mean value per dispatch of every counter recorded for kernels whose name contains kernel_substr"""
+    acc = {}
+    with open(_one(dirname, "counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            if kernel_substr in r["Kernel_Name"]:
+                acc.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    return {k: _mean(v) for k, v in acc.items()}, max((len(v) for v in acc.values()), default=0)
+
+
+def resources_by_kernel(dirname, kernel_substr):
+    """register and LDS allocation of the first dispatch of a kernel in a kernel-trace CSV"""
+    with open(_one(dirname, "kernel_trace.csv")) as f:
+        for r in csv.DictReader(f):
+            if kernel_substr in r["Kernel_Name"]:
+                return {k: r[k] for k in r if any(t in k for t in ("VGPR", "SGPR", "LDS", "Scratch", "Workgroup_Size_X"))}
+    return {}
+
+
+def sq_markdown(pmc_dir, trace_dir, substrs, commands):
+    rows = []
+    names = []
+    for k in substrs:
+        vals, n = counters_by_kernel(pmc_dir, k)
+        res = resources_by_kernel(trace_dir, k) if trace_dir != "-" else {}
+        rows.append((k, n, vals, res))
+        for c in vals:
+            if c not in names:
+                names.append(c)
+    out = ["# Issue and occupancy counters per kernel", "", "Commands:", "", "```"] + list(commands) + ["```", "",
+           "Means per dispatch. VALUBusy, SALUBusy: % of GPU time the vector / scalar ALUs issue; VALUUtilization:"
+           " % of active lanes in issued vector instructions; OccupancyPercent: resident waves as % of the maximum.",
+           ""]
+    out.append("| kernel | dispatches | " + " | ".join(names) + " | SQ_INSTS_SALU / SQ_INSTS_VALU | allocation |")
+    out.append("|---" * (len(names) + 4) + "|")
+    for k, n, vals, res in rows:
+        cells = ["%.4g" % vals[c] if c in vals else "-" for c in names]
+        ratio = "%.2f" % (vals["SQ_INSTS_SALU"] / vals["SQ_INSTS_VALU"]) if vals.get("SQ_INSTS_VALU") else "-"
+        alloc = ", ".join("%s %s" % (a, b) for a, b in res.items()) or "-"
+        out.append("| `%s` | %d | %s | %s | %s |" % (k, n, " | ".join(cells), ratio, alloc))
+    return "\n".join(out) + "\n"
+
+
 def main(argv):
+    launch = None
+    for x in list(argv):
+        if x.startswith("--launch="):
+            launch = json.loads(x.split("=", 1)[1])
+            argv.remove(x)
     if argv[0] == "stats":
         sys.stdout.write(stats_markdown(argv[1], argv[2], int(argv[3]) if len(argv) > 3 else 12))
     elif argv[0] == "traffic":
@@ -118,6 +171,8 @@ def main(argv):
         path = argv[8] if len(argv) > 8 else os.path.join(os.path.dirname(os.path.dirname(
             os.path.abspath(__file__))), "profiles", "traffic.json")
         t = traffic(fetch_dir, write_dir, cfd, cwd, kern, float(mib))
+        if launch is not None:
+            t["launch"] = launch
         try:
             with open(path) as f:
                 store = json.load(f)
@@ -133,6 +188,8 @@ def main(argv):
     elif argv[0] == "l2":
         r = l2_hit_rate(argv[1], argv[2])
         sys.stdout.write(l2_markdown(r, argv[3:]))
+    elif argv[0] == "sq":
+        sys.stdout.write(sq_markdown(argv[1], argv[2], argv[3].split(","), argv[4:]))
     else:
         raise SystemExit(__doc__)
 

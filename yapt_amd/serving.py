@@ -70,6 +70,53 @@ def run(server, scenes, cams, requests, depth=8):
     return recs
 
 
+def trace(jobs=120, rate=40.0, shapes=SCENE_SHAPES, budget_gb=96.0, max_active=8, seed=4, log=lambda m: None):
+    """Builds the resident scenes, replays a seeded Poisson stream of `jobs` requests at `rate` per second
+    through a render server with the tuned launch options, and returns the latency and throughput
+    figures (seconds from submit to film complete)."""
+    from . import _lib, scenes as S, tuning
+    res_scenes, cams, tris = [], [], []
+    t_build = time.perf_counter()
+    for k, n in enumerate(shapes):
+        sc, cam = S.random_shapes(n_shapes=n, seed=2 + k, emissive_triangles=8)
+        res_scenes.append(_lib.ResidentScene(sc))
+        cams.append(cam)
+        tris.append(sc.triangle_count)
+        log("scene %d: %d triangles resident" % (k, sc.triangle_count))
+    t_build = time.perf_counter() - t_build
+    server = _lib.Server(0, int(budget_gb * (1 << 30)), max_active)
+    skip = ("passes_in_flight",)   # the server sizes batches per job
+    for integ, key in ((0, "nee-shapes2000-1920x1080-d12"), (1, "nee-shapes2000-1920x1080-d12"),
+                       (2, "bdpt-shapes2000-1280x720-d8")):
+        for k, v in tuning.best_config(key).items():
+            if k not in skip:
+                server.set_option(k, v, integ)
+    reqs = make_requests(jobs, rate, seed, len(shapes))
+    t = time.perf_counter()
+    recs = run(server, res_scenes, cams, reqs)
+    wall = time.perf_counter() - t
+    st = server.stats()
+    server.close()
+    for r in res_scenes:
+        r.close()
+    lat = [r["latency"] for r in recs]
+    first = min(r["submit"] for r in recs)
+    last = max(r["finish"] for r in recs)
+    return {
+        "metric": "render-job latency (Poisson arrivals, continuous batching)",
+        "rate_req_s": rate, "jobs": len(recs), "scene_triangles": tris, "scene_build_s": t_build,
+        "latency_s": percentiles(lat), "mean_latency_s": float(np.mean(lat)),
+        "queue_wait_s": percentiles([r["queue"] for r in recs]),
+        "throughput_jobs_s": len(recs) / (last - first),
+        "throughput_path_samples_s": sum(r["samples"] for r in recs) / (last - first),
+        "offered_path_samples_s": sum(r["samples"] for r in recs) / max(1e-9, reqs[-1]["t"]),
+        "wall_s": wall, "peak_active": st["peak_active"], "max_active": max_active,
+        "budget_gb": budget_gb, "all_finite": all(r["ok"] for r in recs),
+        "by_integrator": {name: percentiles([r["latency"] for r in recs if r["integrator"] == k])
+                          for k, name in ((0, "bpt"), (1, "nee"), (2, "bdpt"))},
+    }
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--rate", type=float, default=40.0)
@@ -79,45 +126,9 @@ def main():
     p.add_argument("--shapes", default=",".join(str(s) for s in SCENE_SHAPES))
     p.add_argument("--seed", type=int, default=4)
     a = p.parse_args()
-    from . import _lib, scenes as S
-    shapes = [int(x) for x in a.shapes.split(",")]
-    res_scenes, cams, tris = [], [], []
-    for k, n in enumerate(shapes):
-        sc, cam = S.random_shapes(n_shapes=n, seed=2 + k, emissive_triangles=8)
-        res_scenes.append(_lib.ResidentScene(sc))
-        cams.append(cam)
-        tris.append(sc.triangle_count)
-        print("scene %d: %d triangles resident" % (k, sc.triangle_count), file=sys.stderr, flush=True)
-    server = _lib.Server(0, int(a.budget_gb * (1 << 30)), a.max_active)
-    from . import tuning
-    skip = ("passes_in_flight",)   # the server sizes batches per job
-    for integ, key in ((0, "nee-shapes2000-1920x1080-d12"), (1, "nee-shapes2000-1920x1080-d12"),
-                       (2, "bdpt-shapes2000-1280x720-d8")):
-        for k, v in tuning.best_config(key).items():
-            if k not in skip:
-                server.set_option(k, v, integ)
-    reqs = make_requests(a.jobs, a.rate, a.seed, len(shapes))
-    t = time.perf_counter()
-    recs = run(server, res_scenes, cams, reqs)
-    wall = time.perf_counter() - t
-    st = server.stats()
-    lat = [r["latency"] for r in recs]
-    first = min(r["submit"] for r in recs)
-    last = max(r["finish"] for r in recs)
-    out = {
-        "metric": "render-job latency (Poisson arrivals, continuous batching)",
-        "rate_req_s": a.rate, "jobs": len(recs), "scene_triangles": tris,
-        "latency_s": percentiles(lat), "mean_latency_s": float(np.mean(lat)),
-        "queue_wait_s": percentiles([r["queue"] for r in recs]),
-        "throughput_jobs_s": len(recs) / (last - first),
-        "throughput_path_samples_s": sum(r["samples"] for r in recs) / (last - first),
-        "wall_s": wall, "peak_active": st["peak_active"], "max_active": a.max_active,
-        "budget_gb": a.budget_gb, "all_finite": all(r["ok"] for r in recs),
-        "by_integrator": {name: percentiles([r["latency"] for r in recs if r["integrator"] == k])
-                          for k, name in ((0, "bpt"), (1, "nee"), (2, "bdpt"))},
-    }
+    out = trace(a.jobs, a.rate, [int(x) for x in a.shapes.split(",")], a.budget_gb, a.max_active, a.seed,
+                log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out))
-    server.close()
 
 
 if __name__ == "__main__":

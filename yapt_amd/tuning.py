@@ -35,21 +35,21 @@ def best_config(key):
 
 
 def measured_traffic(kernel_key):
-    """HBM bytes per launch of a kernel from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs (profiles/)."""
+    """The record of a kernel's rocprofv3 FETCH_SIZE/WRITE_SIZE runs (profiles/traffic.json): HBM bytes per
+    launch ("bytes_per_launch") and the launch shape it was profiled at ("launch"), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            v = json.load(f).get(kernel_key)
+            return json.load(f).get(kernel_key)
     except (OSError, ValueError):
         return None
-    return None if v is None else v.get("bytes_per_launch")
 
 
 SPACE = {
     "bvh_width": (2, 8),
     "block_size": (64, 128, 256),
     "lds_stack_depth": (8, 12, 16, 24),
-    "passes_in_flight": (4, 8, 16),
+    "passes_in_flight": (8, 16, 24, 32),
     "persistent": (0, 1),
     "persistent_waves": (8, 16, 24, 32),
     "refill_threshold": (4, 8, 16, 32, 64),
