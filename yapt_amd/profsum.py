@@ -4,7 +4,7 @@
     python -m yapt_amd.profsum traffic <fetch dir> <write dir> <calib fetch dir> <calib write dir> \
            <kernel substring> <key> <calib MiB> [traffic.json [summary.md "<command>"...]]
     python -m yapt_amd.profsum l2 <TCC_HIT/TCC_MISS dir> <kernel substring> ["<command>"...] > summary.md
-    python -m yapt_amd.profsum sq <pmc dir> <kernel-trace dir> "<substr1,substr2,...>" ["<command>"...] > summary.md
+    python -m yapt_amd.profsum sq <pmc dir> <kernel-trace dir> "<substr1;substr2;...>" ["<command>"...] > summary.md
 
 ``stats`` renders the ``*_kernel_stats.csv`` of a ``rocprofv3 --kernel-trace --stats`` run as a
 markdown table. ``traffic`` turns the FETCH_SIZE / WRITE_SIZE counters (KiB per dispatch) of one
@@ -189,7 +189,7 @@ def main(argv):
         r = l2_hit_rate(argv[1], argv[2])
         sys.stdout.write(l2_markdown(r, argv[3:]))
     elif argv[0] == "sq":
-        sys.stdout.write(sq_markdown(argv[1], argv[2], argv[3].split(","), argv[4:]))
+        sys.stdout.write(sq_markdown(argv[1], argv[2], argv[3].split(";"), argv[4:]))
     else:
         raise SystemExit(__doc__)
 
