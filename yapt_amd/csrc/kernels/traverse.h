@@ -36,10 +36,19 @@ struct TravStack {
     else ovf[(size_t)(sp - ldsDepth) * ovfStride + gid] = v;
     ++sp;
   }
+  // A stack with an LDS part (ldsDepth > 0, uniform) reads its LDS entry on every pop, an LDS read
+  // rather than a flat access choosing between LDS and global memory; the global overflow is read only
+  // for entries beyond the LDS part. A stack without one (lds null, as the subsurface probes use) never
+  // touches LDS.
   __device__ uint32_t pop() {
     --sp;
-    if (sp < ldsDepth) return lds[sp * block + lane];
-    return ovf[(size_t)(sp - ldsDepth) * ovfStride + gid];
+    uint32_t v = 0u;
+    if (ldsDepth != 0u) {
+      v = lds[(sp < ldsDepth ? sp : ldsDepth - 1u) * block + lane];
+      asm volatile("" : "+v"(v));  // a value barrier (no instruction): the two reads are not merged into one flat load
+    }
+    if (sp >= ldsDepth) v = ovf[(size_t)(sp - ldsDepth) * ovfStride + gid];
+    return v;
   }
 };
 
@@ -200,26 +209,27 @@ __device__ __forceinline__ bool step_bvh2_deferred(const Bvh2Node* nodes, const 
     return r.cur == BVH_EMPTY;
   }
   if (wait) return false;
+  // the node as four 16-byte loads, both boxes tested unconditionally (an empty child's box is
+  // ignored), and the child choice by selects rather than branches
   const float4* np = reinterpret_cast<const float4*>(nodes + r.cur);
   float4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
   if (COUNT) cnt.nodes++;
   float tn0, tn1;
   uint32_t c0 = __float_as_uint(qc.x), c1 = __float_as_uint(qc.y);
-  bool h0 = c0 != BVH_EMPTY && slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), r.o, r.inv, r.tmin, r.best, tn0);
-  bool h1 = c1 != BVH_EMPTY && slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), r.o, r.inv, r.tmin, r.best, tn1);
-  if (h0 && (c0 & BVH_LEAF_BIT)) { r.pl0 = c0; h0 = false; }
-  if (h1 && (c1 & BVH_LEAF_BIT)) { r.pl1 = c1; h1 = false; }
-  if (h0 && h1) {
-    if (tn1 < tn0) { uint32_t t = c0; c0 = c1; c1 = t; }
-    st.push(c1);
-    r.cur = c0;
-  } else if (h0) {
-    r.cur = c0;
-  } else if (h1) {
-    r.cur = c1;
-  } else {
-    r.cur = st.sp == 0 ? BVH_EMPTY : st.pop();
-  }
+  bool s0 = slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), r.o, r.inv, r.tmin, r.best, tn0);
+  bool s1 = slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), r.o, r.inv, r.tmin, r.best, tn1);
+  bool h0 = s0 && c0 != BVH_EMPTY, h1 = s1 && c1 != BVH_EMPTY;
+  bool l0 = h0 && (c0 & BVH_LEAF_BIT) != 0u, l1 = h1 && (c1 & BVH_LEAF_BIT) != 0u;
+  r.pl0 = l0 ? c0 : 0u;
+  r.pl1 = l1 ? c1 : 0u;
+  h0 = h0 && !l0;
+  h1 = h1 && !l1;
+  bool both = h0 && h1;
+  bool first1 = h1 && (!h0 || tn1 < tn0);   // child 1 is visited next
+  uint32_t next = first1 ? c1 : c0;
+  if (both) st.push(first1 ? c0 : c1);
+  if (!(h0 || h1)) next = st.sp == 0 ? BVH_EMPTY : st.pop();
+  r.cur = next;
   return r.cur == BVH_EMPTY && (r.pl0 | r.pl1) == 0u;
 }
 
