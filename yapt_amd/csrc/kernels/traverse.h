@@ -150,35 +150,36 @@ __device__ __forceinline__ void trav_init(RayTrav& r, vec3 o, vec3 d, float tmin
 template <int MODE, bool COUNT, bool ALPHA = true, bool FILTER = false>
 __device__ __forceinline__ bool step_bvh2(const Bvh2Node* nodes, const TriSet& tris, RayTrav& r, TravStack& st,
                                           TraversalCounters& cnt) {
+  // the node as four 16-byte loads and both boxes tested unconditionally (an empty child's box is
+  // ignored), as in step_bvh2_deferred
   const float4* np = reinterpret_cast<const float4*>(nodes + r.cur);
   float4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
   if (COUNT) cnt.nodes++;
   float tn0, tn1;
   uint32_t c0 = __float_as_uint(qc.x), c1 = __float_as_uint(qc.y);
-  bool h0 = c0 != BVH_EMPTY && slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), r.o, r.inv, r.tmin, r.best, tn0);
-  bool h1 = c1 != BVH_EMPTY && slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), r.o, r.inv, r.tmin, r.best, tn1);
+  bool s0 = slab_test(vec3(qx.x, qy.x, qz.x), vec3(qx.z, qy.z, qz.z), r.o, r.inv, r.tmin, r.best, tn0);
+  bool s1 = slab_test(vec3(qx.y, qy.y, qz.y), vec3(qx.w, qy.w, qz.w), r.o, r.inv, r.tmin, r.best, tn1);
+  bool h0 = s0 && c0 != BVH_EMPTY, h1 = s1 && c1 != BVH_EMPTY;
+  bool l0 = h0 && (c0 & BVH_LEAF_BIT) != 0u, l1 = h1 && (c1 & BVH_LEAF_BIT) != 0u;
   // the triangles of both hit leaves in one loop, so that lanes hitting either leaf share iterations
-  uint32_t f0 = 0, n0 = 0, f1 = 0, n1 = 0;
-  if (h0 && (c0 & BVH_LEAF_BIT)) { f0 = leaf_first(c0); n0 = leaf_count(c0); h0 = false; }
-  if (h1 && (c1 & BVH_LEAF_BIT)) { f1 = leaf_first(c1); n1 = leaf_count(c1); h1 = false; }
+  uint32_t f0 = leaf_first(c0), n0 = l0 ? leaf_count(c0) : 0u;
+  uint32_t f1 = leaf_first(c1), n1 = l1 ? leaf_count(c1) : 0u;
+  h0 = h0 && !l0;
+  h1 = h1 && !l1;
   for (uint32_t k = 0; k < n0 + n1; ++k) {
     uint32_t rec = k < n0 ? f0 + k : f1 + (k - n0);
     if (COUNT) cnt.tris++;
     if (MODE == 0) tri_closest<ALPHA, FILTER>(tris, rec, r.rp, r.tmin, r.best, r.bid, r.b1, r.b2, r.lo, r.range);
     else if (tri_occludes<ALPHA>(tris, rec, r.rp, r.tmin, r.best, r.bid)) { r.occluded = 1; return true; }
   }
-  if (h0 && h1) {
-    if (tn1 < tn0) { uint32_t t = c0; c0 = c1; c1 = t; }
-    st.push(c1);
-    r.cur = c0;
-  } else if (h0) {
-    r.cur = c0;
-  } else if (h1) {
-    r.cur = c1;
-  } else {
-    if (st.sp == 0) return true;
-    r.cur = st.pop();
+  bool first1 = h1 && (!h0 || tn1 < tn0);   // child 1 is visited next
+  if (h0 && h1) st.push(first1 ? c0 : c1);
+  if (h0 || h1) {
+    r.cur = first1 ? c1 : c0;
+    return false;
   }
+  if (st.sp == 0) return true;
+  r.cur = st.pop();
   return false;
 }
 
