@@ -23,6 +23,8 @@ extern "C" {
 #define YPT_E_DEVICE (-2)    /* HIP runtime error */
 #define YPT_E_MEMORY (-3)    /* device allocation failed */
 #define YPT_E_NOSCENE (-4)   /* render before set_scene / set_frame */
+#define YPT_E_CANCELLED (-5) /* render server: the job was cancelled */
+#define YPT_E_TIMEOUT (-6)   /* render server: the job passed its deadline */
 
 /* integrator types: RendererVarsList.h:24 option order of "RenderPipeline" */
 #define YPT_INTEGRATOR_BPT 0
@@ -299,10 +301,18 @@ int ypt_server_submit(ypt_server* server, ypt_scene* scene, int integrator_type,
 /* ypt_set_option for the handles that run jobs of one integrator type (< 0: every type), from the
  * next admitted job on */
 int ypt_server_set_option(ypt_server* server, int integrator_type, const char* name, int64_t value);
-/* YPT_OK when the job has finished, 1 when timeout_s (< 0: no limit) passed first */
+/* YPT_OK when the job has finished, 1 when timeout_s (< 0: no limit) passed first; YPT_E_CANCELLED or
+ * YPT_E_TIMEOUT for a job that ended without its film */
 int ypt_server_wait(ypt_server* server, uint64_t job_id, double timeout_s);
+/* Cancels a queued or running job; a running one stops before its next chunk of passes, and the jobs
+ * around it are untouched. YPT_E_INVALID for a job that has already ended. */
+int ypt_server_cancel(ypt_server* server, uint64_t job_id);
+/* Gives the job a deadline `seconds` after its submission: past it, the job is dropped from the queue or
+ * stopped before its next chunk of passes, and waits on it return YPT_E_TIMEOUT. */
+int ypt_server_set_deadline(ypt_server* server, uint64_t job_id, double seconds);
 /* film of a finished job (W*H float4 X, Y, Z, 1; may be NULL) and times3 = submit, admit, finish in
- * seconds of ypt_clock (may be NULL); the job is forgotten afterwards */
+ * seconds of ypt_clock (may be NULL); the job is forgotten afterwards (also one that was cancelled or
+ * timed out, for which the call returns YPT_E_CANCELLED or YPT_E_TIMEOUT) */
 int ypt_server_take(ypt_server* server, uint64_t job_id, float* xyzw, double* times3);
 /* [0] queued, [1] active, [2] completed, [3] device bytes of active jobs, [4] peak active, [5] budget */
 int ypt_server_stats(ypt_server* server, double* out6);

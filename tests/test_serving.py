@@ -1,5 +1,6 @@
 """Render server: the request generator on the CPU; on the GPU, results that do not depend on the
-mix of jobs around them, admission by memory budget and active-job limit."""
+mix of jobs around them, admission by memory budget and active-job limit, rejection of a job over the
+budget, cancellation and deadlines."""
 import numpy as np
 import pytest
 
@@ -112,5 +113,147 @@ def test_server_launch_options_change_no_result_bit(ylib, shapes_small):
     (bpt_a, nee_a, bdpt_a), (bpt_b, nee_b, bdpt_b) = films
     assert np.array_equal(bpt_a.view(np.uint32), bpt_b.view(np.uint32))
     assert np.array_equal(nee_a.view(np.uint32), nee_b.view(np.uint32))
-    # bidirectional: the megakernel and the queued camera stage add visible connections in another order
-    assert np.allclose(bdpt_a, bdpt_b, rtol=1e-5, atol=1e-7)
+    # bidirectional: the megakernel and the queued camera stage fold connections in the same order
+    assert np.array_equal(bdpt_a.view(np.uint32), bdpt_b.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_job_over_the_memory_budget_is_rejected(ylib, cornell):
+    from yapt_amd import camera
+    sc = ylib.ResidentScene(cornell[0])
+    srv = ylib.Server(0, 1 << 20, 2)
+    try:
+        with pytest.raises(ylib.YptError) as e:
+            srv.submit(sc, ylib.YPT_INTEGRATOR_NEE, camera.make_frame(cornell[1], 256, 256, max_depth=4), 4)
+        assert e.value.code == ylib.YPT_E_MEMORY
+        assert srv.stats()["queued"] == 0
+    finally:
+        srv.close()
+        sc.close()
+
+
+def _alone(ylib, scene, integ, frame, spp):
+    it = ylib.Integrator(integ, 0)
+    try:
+        it.set_scene(scene)
+        it.set_frame(frame)
+        it.render(0, spp)
+        return it.film()
+    finally:
+        it.close()
+
+
+@pytest.mark.gpu
+def test_cancelled_jobs_end_and_leave_their_neighbours_unchanged(ylib, cornell, shapes_small):
+    from yapt_amd import camera
+    sc_a = ylib.ResidentScene(cornell[0])
+    sc_b = ylib.ResidentScene(shapes_small[0])
+    fr_long = camera.make_frame(cornell[1], 512, 512, max_depth=8)
+    fr_a = camera.make_frame(cornell[1], 64, 48, max_depth=8)
+    fr_b = camera.make_frame(shapes_small[1], 72, 40, max_depth=6)
+    srv = ylib.Server(0, 8 << 30, 3)
+    try:
+        long_job = srv.submit(sc_a, ylib.YPT_INTEGRATOR_NEE, fr_long, 4096)   # seconds of work, in chunks
+        ja = srv.submit(sc_a, ylib.YPT_INTEGRATOR_NEE, fr_a, 8)
+        jb = srv.submit(sc_b, ylib.YPT_INTEGRATOR_BDPT, fr_b, 3)
+        assert not srv.wait(long_job, 0.2)
+        srv.cancel(long_job)
+        with pytest.raises(ylib.YptError) as e:
+            srv.wait(long_job)
+        assert e.value.code == ylib.YPT_E_CANCELLED
+        assert srv.forget(long_job) == ylib.YPT_E_CANCELLED
+        # a job cancelled while it waits in the queue (one active slot, held by a long job)
+        srv2 = ylib.Server(0, 8 << 30, 1)
+        try:
+            blocker = srv2.submit(sc_a, ylib.YPT_INTEGRATOR_NEE, fr_long, 4096)
+            queued = srv2.submit(sc_a, ylib.YPT_INTEGRATOR_NEE, fr_a, 8)
+            srv2.cancel(queued)
+            with pytest.raises(ylib.YptError) as e:
+                srv2.wait(queued)
+            assert e.value.code == ylib.YPT_E_CANCELLED
+            srv2.cancel(blocker)
+            with pytest.raises(ylib.YptError):
+                srv2.wait(blocker)
+            after = srv2.submit(sc_a, ylib.YPT_INTEGRATOR_NEE, fr_a, 8)   # the pooled handle renders cleanly
+            srv2.wait(after)
+            film_after, _ = srv2.take(after)
+        finally:
+            srv2.close()
+        for j, sc, integ, fr, spp in ((ja, sc_a, ylib.YPT_INTEGRATOR_NEE, fr_a, 8),
+                                      (jb, sc_b, ylib.YPT_INTEGRATOR_BDPT, fr_b, 3)):
+            srv.wait(j)
+            film, _ = srv.take(j)
+            assert np.array_equal(film.view(np.uint32), _alone(ylib, sc, integ, fr, spp).view(np.uint32))
+        assert np.array_equal(film_after.view(np.uint32), _alone(ylib, sc_a, ylib.YPT_INTEGRATOR_NEE, fr_a, 8).view(np.uint32))
+        # a cancelled job can be cancelled no more
+        with pytest.raises(ylib.YptError):
+            srv.cancel(long_job)
+    finally:
+        srv.close()
+        sc_a.close()
+        sc_b.close()
+
+
+@pytest.mark.gpu
+def test_deadline_stops_a_job_and_spares_one_that_meets_it(ylib, cornell):
+    from yapt_amd import camera
+    sc = ylib.ResidentScene(cornell[0])
+    fr_long = camera.make_frame(cornell[1], 512, 512, max_depth=8)
+    fr = camera.make_frame(cornell[1], 64, 48, max_depth=8)
+    srv = ylib.Server(0, 8 << 30, 2)
+    try:
+        slow = srv.submit(sc, ylib.YPT_INTEGRATOR_NEE, fr_long, 4096)
+        srv.set_deadline(slow, 0.3)
+        quick = srv.submit(sc, ylib.YPT_INTEGRATOR_NEE, fr, 8)
+        srv.set_deadline(quick, 60.0)
+        with pytest.raises(ylib.YptError) as e:
+            srv.wait(slow)
+        assert e.value.code == ylib.YPT_E_TIMEOUT
+        t_end = srv.clock()
+        assert srv.forget(slow) == ylib.YPT_E_TIMEOUT
+        srv.wait(quick)
+        film, times = srv.take(quick)
+        assert t_end - times[0] < 30.0
+        assert np.array_equal(film.view(np.uint32), _alone(ylib, sc, ylib.YPT_INTEGRATOR_NEE, fr, 8).view(np.uint32))
+        with pytest.raises(ylib.YptError):
+            srv.set_deadline(slow, 1.0)     # forgotten
+    finally:
+        srv.close()
+        sc.close()
+
+
+@pytest.mark.gpu
+def test_poisson_trace_over_resident_scenes_matches_jobs_rendered_alone(ylib):
+    """A short seeded Poisson trace (12 requests at 40 per second) over resident scenes of ~10k, ~100k, ~1M and
+    ~5M triangles: a sampled job of each integrator gives the same film bits as that job rendered alone."""
+    from yapt_amd import camera, scenes as S
+    shapes = serving.SCENE_SHAPES
+    built, cams = [], []
+    try:
+        for k, n in enumerate(shapes):
+            sc, cam = S.random_shapes(n_shapes=n, seed=2 + k, emissive_triangles=8)
+            built.append(ylib.ResidentScene(sc))
+            cams.append(cam)
+        reqs = serving.make_requests(12, 40.0, seed=7, n_scenes=len(shapes))
+        sample = {}
+        for i, r in enumerate(reqs):       # the cheapest request of each integrator on the two largest scenes
+            if r["scene"] >= 2:
+                cost = r["res"][0] * r["res"][1] * r["spp"]
+                if r["integrator"] not in sample or cost < sample[r["integrator"]][1]:
+                    sample[r["integrator"]] = (i, cost)
+        assert sample
+        srv = ylib.Server(0, 96 << 30, 8)
+        try:
+            recs = serving.run(srv, built, cams, reqs, depth=8, keep_films={i for i, _ in sample.values()})
+            peak = srv.stats()["peak_active"]
+        finally:
+            srv.close()
+        assert all(r["ok"] for r in recs) and peak >= 2
+        for i, _ in sample.values():
+            r = reqs[i]
+            fr = camera.make_frame(cams[r["scene"]], r["res"][0], r["res"][1], max_depth=8)
+            alone = _alone(ylib, built[r["scene"]], r["integrator"], fr, r["spp"])
+            assert np.array_equal(recs[i]["film"].view(np.uint32), alone.view(np.uint32)), r
+    finally:
+        for b in built:
+            b.close()

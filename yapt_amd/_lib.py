@@ -200,8 +200,15 @@ class SceneArrays:
         return len(self.indices)
 
 
+YPT_E_INVALID, YPT_E_DEVICE, YPT_E_MEMORY, YPT_E_NOSCENE, YPT_E_CANCELLED, YPT_E_TIMEOUT = -1, -2, -3, -4, -5, -6
+
+
 class YptError(RuntimeError):
-    pass
+    """an error status of the library; `code` is the YPT_E_* value"""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 _LIB = None
@@ -262,6 +269,8 @@ def load():
     lib.ypt_post_params_default.restype = None
     lib.ypt_server_take.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]
     lib.ypt_server_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+    lib.ypt_server_cancel.argtypes = [P, ctypes.c_uint64]
+    lib.ypt_server_set_deadline.argtypes = [P, ctypes.c_uint64, ctypes.c_double]
     lib.ypt_clock.restype = ctypes.c_double
     _LIB = lib
     return lib
@@ -274,13 +283,14 @@ EXPORTED_SYMBOLS = [
     "ypt_scene_create", "ypt_scene_destroy", "ypt_scene_info", "ypt_set_resident_scene",
     "ypt_server_create", "ypt_server_destroy", "ypt_server_submit", "ypt_server_wait", "ypt_server_take",
     "ypt_server_stats", "ypt_clock", "ypt_server_set_option", "ypt_final_color", "ypt_post_params_default",
-    "ypt_get_feature_moments", "ypt_get_film_variance", "ypt_trace_closest_noalloc",
+    "ypt_get_feature_moments", "ypt_get_film_variance", "ypt_trace_closest_noalloc", "ypt_server_cancel",
+    "ypt_server_set_deadline",
 ]
 
 
 def check(rc):
     if rc != 0:
-        raise YptError("yapt_pt error %d: %s" % (rc, load().ypt_last_error().decode()))
+        raise YptError("yapt_pt error %d: %s" % (rc, load().ypt_last_error().decode()), rc)
 
 
 class Integrator:
@@ -437,11 +447,19 @@ class Server:
         check(self.lib.ypt_server_set_option(self.h, int(integrator), name.encode(), int(value)))
 
     def wait(self, job, timeout=-1.0):
+        """True when the job has finished, False when `timeout` seconds passed first; raises YptError for a
+        job that failed, was cancelled (code YPT_E_CANCELLED) or passed its deadline (YPT_E_TIMEOUT)."""
         rc = self.lib.ypt_server_wait(self.h, int(job), float(timeout))
         if rc == 1:
             return False
         check(rc)
         return True
+
+    def cancel(self, job):
+        check(self.lib.ypt_server_cancel(self.h, int(job)))
+
+    def set_deadline(self, job, seconds):
+        check(self.lib.ypt_server_set_deadline(self.h, int(job), float(seconds)))
 
     def take(self, job):
         """Film (H x W x 4) and (submit, admit, finish) times of a finished job."""
@@ -450,6 +468,11 @@ class Server:
         times = np.zeros(3, np.float64)
         check(self.lib.ypt_server_take(self.h, int(job), _fptr(film), times.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return film, times
+
+    def forget(self, job):
+        """drops a job that ended without a film (cancelled, timed out or failed); returns its status code"""
+        self._frames.pop(job, None)
+        return self.lib.ypt_server_take(self.h, int(job), None, None)
 
     def stats(self):
         out = np.zeros(6, np.float64)

@@ -1581,6 +1581,8 @@ double now_s() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+enum JobState { JOB_QUEUED = 0, JOB_ACTIVE = 1, JOB_DONE = 2, JOB_FAILED = 3, JOB_CANCELLED = 4, JOB_TIMED_OUT = 5 };
+
 struct ServerJob {
   uint64_t id = 0;
   std::shared_ptr<ResidentScene> scene;
@@ -1593,7 +1595,9 @@ struct ServerJob {
   ypt_integrator* h = nullptr;
   hipEvent_t ev = nullptr;
   bool inFlight = false;
-  std::atomic<int> state{0};  // 0 queued, 1 active, 2 done, 3 failed
+  std::atomic<int> state{0};  // JOB_QUEUED, JOB_ACTIVE, JOB_DONE, JOB_FAILED, JOB_CANCELLED, JOB_TIMED_OUT
+  std::atomic<bool> cancel{false};
+  std::atomic<double> deadline{1e300};  // ypt_clock time after which the job is given up
   std::string err;
   double tSubmit = 0, tAdmit = 0, tFinish = 0;
   std::vector<float> film;
@@ -1680,16 +1684,30 @@ struct ypt_server {
     (void)hipSetDevice(device);
     while (true) {
       std::vector<std::shared_ptr<ServerJob>> fresh, act;
+      bool dropped = false;
       {
         std::lock_guard<std::mutex> lk(m);
         if (stop) break;
+        double t = now_s();
+        for (size_t i = 0; i < queued.size();) {  // cancelled or expired before admission
+          auto& q = queued[i];
+          if (q->cancel || t > q->deadline) {
+            q->state = q->cancel ? JOB_CANCELLED : JOB_TIMED_OUT;
+            q->tFinish = t;
+            queued.erase(queued.begin() + i);
+            dropped = true;
+          } else {
+            ++i;
+          }
+        }
+        if (dropped) cvDone.notify_all();
         while (!queued.empty() && active.size() < maxActive) {
           auto j = queued.front();
           if (!active.empty() && memUsed + j->mem > budget) break;  // wait for memory to free up
           queued.pop_front();
           memUsed += j->mem;
           j->options = options[j->type];
-          j->state = 1;
+          j->state = JOB_ACTIVE;
           j->tAdmit = now_s();
           active.push_back(j);
           fresh.push_back(j);
@@ -1703,19 +1721,28 @@ struct ypt_server {
           start_job(*j);
         } catch (const std::exception& e) {
           j->err = e.what();
-          j->state = 3;
+          j->state = JOB_FAILED;
           ended.push_back(j);
         }
       }
-      bool progress = !fresh.empty();
+      bool progress = !fresh.empty() || dropped;
       for (auto& j : act) {
-        if (j->state != 1) continue;
+        if (j->state != JOB_ACTIVE) continue;
         try {
           if (j->inFlight) {
             hipError_t q = hipEventQuery(j->ev);
             if (q == hipErrorNotReady) continue;
             HIP_CHECK(q);
             j->inFlight = false;
+          }
+          // a cancelled or expired job stops between chunks: nothing of it is in flight any more, so its
+          // handle goes back to the pool untouched by the jobs around it
+          if (j->cancel || now_s() > j->deadline) {
+            j->state = j->cancel ? JOB_CANCELLED : JOB_TIMED_OUT;
+            j->tFinish = now_s();
+            ended.push_back(j);
+            progress = true;
+            continue;
           }
           if (j->issued < j->spp) {
             uint32_t n = std::min(j->chunk, j->spp - j->issued);
@@ -1726,13 +1753,13 @@ struct ypt_server {
           } else {
             finish_job(*j);
             j->tFinish = now_s();
-            j->state = 2;
+            j->state = JOB_DONE;
             ended.push_back(j);
           }
           progress = true;
         } catch (const std::exception& e) {
           j->err = e.what();
-          j->state = 3;
+          j->state = JOB_FAILED;
           ended.push_back(j);
         }
       }
@@ -1741,7 +1768,7 @@ struct ypt_server {
         for (auto& j : ended) {
           release(*j);
           memUsed -= j->mem;
-          if (j->state == 2) completed++;
+          if (j->state == JOB_DONE) completed++;
           for (size_t i = 0; i < active.size(); ++i)
             if (active[i] == j) { active.erase(active.begin() + i); break; }
         }
@@ -1843,10 +1870,12 @@ int ypt_server_wait(ypt_server* s, uint64_t job_id, double timeout_s) {
   auto it = s->jobs.find(job_id);
   if (it == s->jobs.end()) return fail(YPT_E_INVALID, "unknown job");
   auto j = it->second;
-  auto done = [&]() { return j->state >= 2; };
+  auto done = [&]() { return j->state >= JOB_DONE; };
   if (timeout_s < 0) s->cvDone.wait(lk, done);
   else if (!s->cvDone.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return 1;
-  if (j->state == 3) return fail(YPT_E_DEVICE, "job failed: " + j->err);
+  if (j->state == JOB_FAILED) return fail(YPT_E_DEVICE, "job failed: " + j->err);
+  if (j->state == JOB_CANCELLED) return fail(YPT_E_CANCELLED, "job cancelled");
+  if (j->state == JOB_TIMED_OUT) return fail(YPT_E_TIMEOUT, "job passed its deadline");
   return YPT_OK;
 }
 
@@ -1858,15 +1887,39 @@ int ypt_server_take(ypt_server* s, uint64_t job_id, float* xyzw, double* times3)
   auto it = s->jobs.find(job_id);
   if (it == s->jobs.end()) return fail(YPT_E_INVALID, "unknown job");
   ServerJob& j = *it->second;
-  if (j.state < 2) return fail(YPT_E_INVALID, "job has not finished");
-  if (j.state == 3) {
+  if (j.state < JOB_DONE) return fail(YPT_E_INVALID, "job has not finished");
+  if (j.state != JOB_DONE) {
+    int st = j.state;
     std::string e = j.err;
     s->jobs.erase(it);
+    if (st == JOB_CANCELLED) return fail(YPT_E_CANCELLED, "job cancelled");
+    if (st == JOB_TIMED_OUT) return fail(YPT_E_TIMEOUT, "job passed its deadline");
     return fail(YPT_E_DEVICE, "job failed: " + e);
   }
   if (xyzw) std::memcpy(xyzw, j.film.data(), j.film.size() * sizeof(float));
   if (times3) { times3[0] = j.tSubmit; times3[1] = j.tAdmit; times3[2] = j.tFinish; }
   s->jobs.erase(it);
+  return YPT_OK;
+}
+
+// cancels a queued or running job: it stops before its next chunk of passes
+int ypt_server_cancel(ypt_server* s, uint64_t job_id) {
+  if (!s) return fail(YPT_E_INVALID, "null server");
+  std::lock_guard<std::mutex> lk(s->m);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return fail(YPT_E_INVALID, "unknown job");
+  if (it->second->state >= JOB_DONE) return fail(YPT_E_INVALID, "job has already ended");
+  it->second->cancel = true;
+  return YPT_OK;
+}
+
+// the job is given up (YPT_E_TIMEOUT) if it has not finished `seconds` after its submission
+int ypt_server_set_deadline(ypt_server* s, uint64_t job_id, double seconds) {
+  if (!s || !(seconds >= 0.0)) return fail(YPT_E_INVALID, "null server or negative deadline");
+  std::lock_guard<std::mutex> lk(s->m);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return fail(YPT_E_INVALID, "unknown job");
+  it->second->deadline = it->second->tSubmit + seconds;
   return YPT_OK;
 }
 

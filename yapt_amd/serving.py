@@ -41,9 +41,9 @@ def percentiles(latencies, qs=(50, 90, 99)):
     return {"p%d" % q: float(np.percentile(a, q)) if len(a) else None for q in qs}
 
 
-def run(server, scenes, cams, requests, depth=8):
+def run(server, scenes, cams, requests, depth=8, keep_films=()):
     """Submits each request at its arrival time (relative to now) from a feeder thread; returns one
-    record per request with its latency."""
+    record per request with its latency (and its film, for the request indices in `keep_films`)."""
     from . import camera
     t0 = time.perf_counter()
     ids = [None] * len(requests)
@@ -66,7 +66,7 @@ def run(server, scenes, cams, requests, depth=8):
         film, times = server.take(ids[i])
         recs.append(dict(r, latency=times[2] - times[0], queue=times[1] - times[0],
                          samples=r["res"][0] * r["res"][1] * r["spp"], finish=times[2], submit=times[0],
-                         ok=bool(np.isfinite(film).all())))
+                         ok=bool(np.isfinite(film).all()), film=film if i in keep_films else None))
     return recs
 
 
