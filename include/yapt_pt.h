@@ -34,6 +34,7 @@ extern "C" {
 /* frame flags */
 #define YPT_FLAG_DISABLE_JITTER (1u << 1)  /* CONSTANTS_FLAG_DISABLE_TEXEL_JITTER */
 #define YPT_FLAG_WHITE_FURNACE (1u << 8)   /* WHITE_FURNACE_TEST as a runtime switch */
+#define YPT_FLAG_BDPT_NODE_RANGES (1u << 9) /* use ypt_frame_desc.bdpt_node_ranges (else 0..128 for both) */
 
 /* material_mask bits (MaterialMask_*, MaterialInternal.h / materialsCommon.hlsl:8-10); ALPHA_BLEND
  * turns the albedo texel alpha into coverage */
@@ -93,6 +94,22 @@ typedef struct ypt_texture {
   const uint8_t* rgba8;  /* width * height * 4, row-major from v = 0 */
 } ypt_texture;
 
+/* Environment texture: the renderer variable World.Skycube (CRenderer.h:18), classified as
+ * RaytraceCommonResources.cpp:285-313 does (ENVIRONMENT_TYPE_*, RaytraceCommonResources.h:17-19) and
+ * looked up by getSkyBoxColor (materialsCommonResources.hlsl:181-223) on every escaping ray, times the
+ * frame's env_intensity_scale. Linear RGBA float texels (the reference loads R16G16B16A16_SFLOAT
+ * images), rows top to bottom, filtered bilinearly at mip level 0: a long-lat image with wrapping, at
+ * (atan2(z, x) / 2pi + 1/2, acos(y) / pi); a cube map as six faces of width x height in the order
+ * +X, -X, +Y, -Y, +Z, -Z with the D3D/Vulkan face orientation, clamped at face edges. */
+#define YPT_ENVIRONMENT_NONE 0     /* |direction| as the colour, the reference's texture-less sky */
+#define YPT_ENVIRONMENT_CUBE 1
+#define YPT_ENVIRONMENT_LONGLAT 2
+typedef struct ypt_environment {
+  uint32_t type;          /* YPT_ENVIRONMENT_* */
+  uint32_t width, height; /* of the long-lat image, or of one (square) cube face */
+  const float* rgba;      /* width * height * 4 floats, six times that for a cube map */
+} ypt_environment;
+
 /* Punctual lights (lightTypes.h:6-8) */
 #define YPT_LIGHT_POINT 1
 #define YPT_LIGHT_SPOT 2
@@ -134,6 +151,9 @@ typedef struct ypt_scene_desc {
   const float* uvs;              /* 2 * vertex_count, or NULL: UV (0.5, 0.5) as for meshes without UVs */
   const ypt_texture* textures;   /* may be NULL when texture_count is 0 */
   uint32_t texture_count;
+  const ypt_environment* environment;  /* NULL: YPT_ENVIRONMENT_NONE. The reference holds it as a renderer
+                                          variable; here it travels with the scene, so a resident scene that
+                                          several handles or server jobs share carries its sky. */
 } ypt_scene_desc;
 
 /* Per-frame constants: the fields of RaytraceConstantData (raytraceCommonResources.hlsl:8-31) the
@@ -159,6 +179,11 @@ typedef struct ypt_frame_desc {
   uint32_t light_path_vertices;  /* BDPT: vertices per light subpath incl. the light, 2..16
                                     (maxVerticesPerLightPath, BidirectionalPathIntegratorSubStage.cpp:482);
                                     0 = 16 */
+  uint32_t bdpt_node_ranges[4];  /* with YPT_FLAG_BDPT_NODE_RANGES: the BDPT strategies that contribute, light
+                                    vertices in [0]..[1] and camera vertices in [2]..[3] (the debug variables
+                                    _DEBUG.BDPT.EffectiveLightPathNodes / EffectiveCameraPathNodes,
+                                    RendererVarsList.h:46-47, applied by shouldPathContribute,
+                                    BDPTCommon.hlsl:1217-1223) */
 } ypt_frame_desc;
 
 typedef struct ypt_integrator ypt_integrator;

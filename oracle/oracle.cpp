@@ -354,6 +354,8 @@ void make_view(Ctx& c) {
   c.sv.triAlpha = c.scene.alphaTris.empty() ? nullptr : c.scene.triAlpha.data();
   c.sv.alpha = AlphaView{c.scene.alphaTris.empty() ? nullptr : c.scene.alphaTris.data(), c.scene.textures.data(),
                          c.scene.texels.data()};
+  c.sv.env = EnvView{c.scene.envTexels.empty() ? nullptr : c.scene.envTexels.data(),
+                     c.scene.envTexels.empty() ? ENV_NONE : c.scene.envType, c.scene.envWidth, c.scene.envHeight};
   c.sv.ntris = (uint32_t)c.scene.triInst.size();
   c.sv.ninst = (uint32_t)c.scene.inst.size();
   c.sv.spec.cie_xyz4 = YPT_CIE_XYZ;
@@ -767,6 +769,9 @@ extern "C" int oracle_eval(int fn, const float* in, uint32_t n, float* out) {
       }
       case 37: o[0] = L.ssms_avg(a[0], a[1]); break;
       case 38: o[0] = average_albedo_translucent(L, a[0], a[1]); break;
+      case 39: o[0] = yatan2(a[0], a[1]); break;
+      case 40: o[0] = yacos(a[0]); break;
+      case 41: o[0] = yasin(a[0]); break;
       default: g_err = "unknown function"; return YPT_E_INVALID;
     }
   }
@@ -783,6 +788,29 @@ extern "C" int oracle_rgb_reproduce(const float* coeffs, float* rgb_out) {
 
 // evaluate_surface for one untextured material in a frame whose geometry and shading normals are +y
 // and whose tangent is +x, seen from outside (current and previous IOR 1): the layered BSDF's
+// getSkyBoxColor of the shared core over n directions (3 floats each): RGB per direction, for an
+// environment as the scene description takes it (NULL: the texture-less |direction| colour)
+extern "C" int oracle_env_color(const ypt_environment* env, const float* dirs, uint32_t n, float* rgb) {
+  try {
+    ypt_scene_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.environment = env;
+    PreparedScene ps;
+    prepare_environment(d, ps);
+    SceneView sv;
+    std::memset(&sv, 0, sizeof(sv));
+    sv.env = EnvView{ps.envTexels.empty() ? nullptr : ps.envTexels.data(), ps.envType, ps.envWidth, ps.envHeight};
+    for (uint32_t i = 0; i < n; ++i) {
+      vec3 c = env_color(sv, vec3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+      rgb[3 * i] = c.x; rgb[3 * i + 1] = c.y; rgb[3 * i + 2] = c.z;
+    }
+    return YPT_OK;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return YPT_E_INVALID;
+  }
+}
+
 // weight per hero lane (projected by |wi.y| unless flags ask otherwise) and its summed pdf.
 // out: weight[4], pdf, transmission flags
 extern "C" int oracle_eval_surface(const ypt_material* m, const float* wo, const float* wi, float hero, uint32_t flags,

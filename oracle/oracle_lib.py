@@ -107,7 +107,8 @@ FN = {"d_ggx": 0, "pdf_wm_ggx": 1, "sample_wm_ggx": 2, "fresnel_dielectric": 3, 
       "sheen_dir": 19, "tr_dir": 20, "emission_profile": 21, "f32_to_f16": 22, "f16_to_f32": 23, "accumulate_ri": 24,
       "pdf_ratio": 25, "burley_profile_area": 26, "burley_sample_radius": 27, "burley_d": 28,
       "pdf_subsurface_exit_plane": 29, "sellmeier_ior": 30, "planck_relative": 31, "avg_fresnel": 32,
-      "glambda_sheen": 33, "d_sheen": 34, "cauchy_ior": 35, "apply_thin_film": 36, "ssms_avg": 37, "tr_avg": 38}
+      "glambda_sheen": 33, "d_sheen": 34, "cauchy_ior": 35, "apply_thin_film": 36, "ssms_avg": 37, "tr_avg": 38,
+      "atan2": 39, "acos": 40, "asin": 41}
 
 
 def evaluate(name, args):
@@ -135,6 +136,23 @@ def rgb_reproduce(coeffs):
 
 
 EVAL_NONE, EVAL_PDF_ONLY, EVAL_LIGHT_PATH, EVAL_TREAT_AS_DELTA, EVAL_DONT_PROJECT = 1, 2, 4, 8, 16
+
+
+def env_color(environment, dirs):
+    """getSkyBoxColor of the shared core (RGB per direction) for an environment given as (type, texels) the
+    way yapt_amd._lib.SceneArrays takes it, or None for the texture-less sky"""
+    from yapt_amd import _lib as ylib
+    lib = load()
+    lib.oracle_env_color.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32,
+                                     ctypes.POINTER(ctypes.c_float)]
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros((len(d), 3), np.float32)
+    env = None
+    if environment is not None:
+        texels = ylib.environment_texels(int(environment[0]), environment[1])
+        env = ylib.Environment(int(environment[0]), texels.shape[-2], texels.shape[-3], _f(texels))
+    _check(lib.oracle_env_color(None if env is None else ctypes.addressof(env), _f(d), len(d), _f(out)))
+    return out
 
 
 def eval_surface(material, wo, wi, hero, flags=EVAL_NONE, reciprocal=False):

@@ -321,3 +321,28 @@ def test_render_on_a_caller_stream(ylib, shapes_small, integrator):
         x = torch.arange(1024, device="cuda", dtype=torch.float32) * 2.0
     caller.synchronize()
     assert float(x[-1]) == 2046.0
+
+
+@pytest.mark.parametrize("kind,integrator,lights", [("longlat", 0, "env"), ("longlat", 1, "env"), ("cube", 1, "env"),
+                                                    ("longlat", 2, "env"), ("cube", 2, "env"), ("cube", 1, "mixed"),
+                                                    ("longlat", 2, "mixed")])
+def test_render_parity_environment_texture(ylib, oracle, kind, integrator, lights):
+    # escaping rays read the sky texture (long-lat with its wrap, a cube map by face); with no lights BDPT
+    # starts light subpaths at the textured sky
+    from yapt_amd import camera, scenes
+    env = scenes.longlat_environment(64, 32) if kind == "longlat" else scenes.cube_environment(24)
+    scene, cam, _ = scenes.punctual_lights(lights, sphere_res=(12, 12), environment=env)
+    frame = camera.make_frame(cam, 72, 44, max_depth=6, env_intensity_scale=0.5, light_path_vertices=6)
+    for opts in ([{"bdpt_wavefront": 1}, {"bdpt_wavefront": 0}] if integrator == 2 else [{}]):
+        film = _compare_render(ylib, oracle, scene, frame, integrator, passes=2, options=opts)
+        assert film[..., 1].max() > 0
+
+
+def test_bdpt_parity_strategy_ranges(ylib, oracle, cornell):
+    # the debug strategy filter (light vertices 1..3, camera vertices 1..4) in both camera-stage forms
+    from yapt_amd import camera
+    scene, cam = cornell
+    frame = camera.make_frame(cam, 64, 48, max_depth=8, bdpt_light_nodes=(1, 3), bdpt_camera_nodes=(1, 4))
+    for opts in ({"bdpt_wavefront": 1}, {"bdpt_wavefront": 0}):
+        film = _compare_render(ylib, oracle, scene, frame, ylib.YPT_INTEGRATOR_BDPT, passes=2, options=opts)
+        assert film[..., 1].max() > 0

@@ -91,3 +91,36 @@ def test_bdpt_light_path_length_setting(oracle, cornell):
     a, b, c = (oracle.render(scene, f, BDPT, 0, 256)["film"][..., :3].mean() for f in frames)
     assert abs(a - b) / a < 0.01
     assert 0.9 * a < c < a
+
+
+def _strategies(oracle, cornell, light, cam, passes=2):
+    scene, c = cornell
+    frame = camera.make_frame(c, 40, 40, max_depth=6, bdpt_light_nodes=light, bdpt_camera_nodes=cam)
+    return oracle.render(scene, frame, _lib.YPT_INTEGRATOR_BDPT, 0, passes, use_bvh=True)["last"]
+
+
+def test_bdpt_strategy_ranges_partition_the_estimate(oracle, cornell):
+    # the debug variables _DEBUG.BDPT.EffectiveLightPathNodes / EffectiveCameraPathNodes keep the strategies
+    # whose vertex counts lie in their ranges (shouldPathContribute, BDPTCommon.hlsl:1217-1223); disjoint
+    # ranges split each sample's estimate into parts that add up to it
+    full = _strategies(oracle, cornell, (0, 128), (0, 128))
+    assert np.array_equal(full, _strategies(oracle, cornell, (0, 10 ** 6), (0, 10 ** 6)))
+    parts = [_strategies(oracle, cornell, l, (0, 128)) for l in ((0, 0), (1, 1), (2, 3), (4, 128))]
+    assert np.array_equal(parts[0][..., 3], full[..., 3])
+    total = sum(p[..., :3].astype(np.float64) for p in parts)
+    assert np.allclose(total, full[..., :3], rtol=2e-5, atol=1e-7)
+    assert all(p[..., 1].sum() > 0 for p in parts)
+    by_camera = [_strategies(oracle, cornell, (0, 128), c) for c in ((0, 0), (1, 2), (3, 128))]
+    assert np.allclose(sum(p[..., :3].astype(np.float64) for p in by_camera), full[..., :3], rtol=2e-5, atol=1e-7)
+    assert not np.any(_strategies(oracle, cornell, (200, 300), (0, 128))[..., :3])
+
+
+def test_bdpt_strategy_ranges_count_vertices_from_one(oracle, cornell):
+    # one camera vertex and no light vertex: only emitters seen directly, at their emission (the first hit's
+    # emission with MIS weight 1, as no other strategy reaches the camera's own first vertex from the light)
+    direct = _strategies(oracle, cornell, (0, 0), (1, 1))
+    lit = direct[..., 1] > 0
+    assert 0 < lit.mean() < 0.2
+    # light tracing: a light subpath's own light vertex is never splatted, its first bounce (2 vertices) is
+    assert not np.any(_strategies(oracle, cornell, (1, 1), (0, 0))[..., :3])
+    assert _strategies(oracle, cornell, (2, 2), (0, 0))[..., 1].sum() > 0

@@ -17,6 +17,7 @@ YPT_INTEGRATOR_NEE = 1
 YPT_INTEGRATOR_BDPT = 2
 YPT_FLAG_DISABLE_JITTER = 1 << 1
 YPT_FLAG_WHITE_FURNACE = 1 << 8
+YPT_FLAG_BDPT_NODE_RANGES = 1 << 9
 YPT_MATERIAL_TWO_SIDED = 1 << 0
 YPT_MATERIAL_DISPERSION = 1 << 1
 YPT_MATERIAL_ALPHA_BLEND = 1 << 2
@@ -53,6 +54,33 @@ class Texture(ctypes.Structure):
     """ypt_texture: RGBA8 texels, srgb = colour data decoded before filtering"""
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("srgb", ctypes.c_uint32),
                 ("rgba8", ctypes.POINTER(ctypes.c_uint8))]
+
+
+YPT_ENVIRONMENT_NONE, YPT_ENVIRONMENT_CUBE, YPT_ENVIRONMENT_LONGLAT = 0, 1, 2
+
+
+class Environment(ctypes.Structure):
+    """ypt_environment: the World.Skycube texture, linear RGBA float texels (six square faces +X, -X, +Y, -Y,
+    +Z, -Z for a cube map, one image for long-lat)"""
+    _fields_ = [("type", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("rgba", ctypes.POINTER(ctypes.c_float))]
+
+
+def environment_texels(kind, texels):
+    """validated float32 texels of an environment: H x W x 4 (or x 3, alpha 1 added) for long-lat, 6 x N x N x 4
+    (or x 3) for a cube map"""
+    a = np.asarray(texels, dtype=np.float32)
+    if a.ndim not in (3, 4) or a.shape[-1] not in (3, 4):
+        raise ValueError("environment texels are H x W x 4 (long-lat) or 6 x N x N x 4 (cube)")
+    if a.shape[-1] == 3:
+        a = np.concatenate([a, np.ones(a.shape[:-1] + (1,), np.float32)], axis=-1)
+    if kind == YPT_ENVIRONMENT_CUBE and (a.ndim != 4 or a.shape[0] != 6 or a.shape[1] != a.shape[2]):
+        raise ValueError("a cube map is 6 square faces: 6 x N x N x 4")
+    if kind == YPT_ENVIRONMENT_LONGLAT and a.ndim != 3:
+        raise ValueError("a long-lat environment is one H x W x 4 image")
+    if kind not in (YPT_ENVIRONMENT_CUBE, YPT_ENVIRONMENT_LONGLAT):
+        raise ValueError("unknown environment type %r" % (kind,))
+    return np.ascontiguousarray(a)
 
 
 YPT_LIGHT_POINT, YPT_LIGHT_SPOT, YPT_LIGHT_DIRECTIONAL = 1, 2, 3
@@ -140,6 +168,7 @@ class SceneDesc(ctypes.Structure):
         ("materials", ctypes.POINTER(Material)), ("material_count", ctypes.c_uint32),
         ("punctual_lights", ctypes.POINTER(PunctualLight)), ("punctual_light_count", ctypes.c_uint32),
         ("uvs", ctypes.POINTER(ctypes.c_float)), ("textures", ctypes.POINTER(Texture)), ("texture_count", ctypes.c_uint32),
+        ("environment", ctypes.POINTER(Environment)),
     ]
 
 
@@ -152,6 +181,7 @@ class FrameDesc(ctypes.Structure):
         ("max_depth", ctypes.c_uint32), ("regularize_after", ctypes.c_uint32), ("flags", ctypes.c_uint32),
         ("firefly_clamp_scale", ctypes.c_float), ("firefly_clamp_relax_power", ctypes.c_float),
         ("kill_nan_samples", ctypes.c_uint32), ("light_path_vertices", ctypes.c_uint32),
+        ("bdpt_node_ranges", ctypes.c_uint32 * 4),
     ]
 
 
@@ -167,8 +197,9 @@ class SceneArrays:
     """Keeps the numpy arrays a SceneDesc points into alive."""
 
     def __init__(self, positions, normals, indices, inst_first, inst_count, inst_material, materials, tangents=None,
-                 lights=(), uvs=None, textures=()):
-        """textures: (H x W x 4 uint8 array, srgb) pairs; uvs: V x 2 floats"""
+                 lights=(), uvs=None, textures=(), environment=None):
+        """textures: (H x W x 4 uint8 array, srgb) pairs; uvs: V x 2 floats; environment: None or
+        (YPT_ENVIRONMENT_CUBE or YPT_ENVIRONMENT_LONGLAT, float texels as environment_texels takes them)"""
         self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
         self.tangents = None if tangents is None else np.ascontiguousarray(tangents, dtype=np.float32).reshape(-1, 4)
@@ -194,6 +225,17 @@ class SceneArrays:
             _uptr(self.inst_material), len(self.inst_first), self.materials, len(materials),
             self.light_array, len(self.lights), None if self.uvs is None else _fptr(self.uvs), self.texture_array,
             len(self.texel_arrays))
+        self.set_environment(environment)
+
+    def set_environment(self, environment):
+        """the World.Skycube texture of the scene (None: the texture-less |direction| sky)"""
+        self.env_texels, self.env = None, None
+        if environment is not None:
+            kind, texels = environment
+            self.env_texels = environment_texels(int(kind), texels)
+            h, w = self.env_texels.shape[-3], self.env_texels.shape[-2]
+            self.env = Environment(int(kind), w, h, _fptr(self.env_texels))
+        self.desc.environment = ctypes.pointer(self.env) if self.env is not None else None
 
     @property
     def triangle_count(self):

@@ -65,8 +65,11 @@ class Camera:
 
 
 def make_frame(camera, width, height, max_depth=8, env_intensity_scale=0.0, regularize_after=8, flags=0,
-               firefly_clamp_scale=12.0, firefly_clamp_relax_power=0.5, kill_nan_samples=1, light_path_vertices=16):
-    """ypt_frame_desc for a camera; defaults are the reference's renderer variables (RendererVarsList.h:21-42)."""
+               firefly_clamp_scale=12.0, firefly_clamp_relax_power=0.5, kill_nan_samples=1, light_path_vertices=16,
+               bdpt_light_nodes=(0, 128), bdpt_camera_nodes=(0, 128)):
+    """ypt_frame_desc for a camera; defaults are the reference's renderer variables (RendererVarsList.h:21-47).
+    bdpt_light_nodes / bdpt_camera_nodes: the BDPT strategies that contribute (inclusive vertex-count ranges of
+    the debug variables _DEBUG.BDPT.EffectiveLightPathNodes / EffectiveCameraPathNodes)."""
     m = camera.matrices(width, height)
     f = _lib.FrameDesc()
     f.width, f.height = width, height
@@ -83,4 +86,8 @@ def make_frame(camera, width, height, max_depth=8, env_intensity_scale=0.0, regu
     f.firefly_clamp_relax_power = firefly_clamp_relax_power
     f.kill_nan_samples = kill_nan_samples
     f.light_path_vertices = light_path_vertices
+    ranges = [int(v) for v in tuple(bdpt_light_nodes) + tuple(bdpt_camera_nodes)]
+    f.bdpt_node_ranges[:] = ranges
+    if ranges != [0, 128, 0, 128]:
+        f.flags |= _lib.YPT_FLAG_BDPT_NODE_RANGES
     return f

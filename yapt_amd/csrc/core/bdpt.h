@@ -405,10 +405,17 @@ YPT_HD float pdf_environment_direction(vec3 dir) {
   return safe_div(1.f, 2.f * PI * PI * sinTheta);
 }
 
-// getSkyBoxColor with no environment texture, times the scale (as the miss shader sees it)
-YPT_HD vec3 sky_color(const FrameConstants& fc, vec3 dir) {
+// shouldPathContribute (BDPTCommon.hlsl:1217-1223): the renderer's debug filter of strategies by their
+// camera and light vertex counts (_DEBUG.BDPT.EffectiveCameraPathNodes / EffectiveLightPathNodes)
+YPT_HD bool should_contribute(const FrameConstants& fc, uint32_t cameraNodes, uint32_t lightNodes) {
+  return cameraNodes >= fc.nodeRanges[2] && cameraNodes <= fc.nodeRanges[3] && lightNodes >= fc.nodeRanges[0] &&
+         lightNodes <= fc.nodeRanges[1];
+}
+
+// getSkyBoxColor times the scale (as the miss shader sees it); the white furnace's unit radiance
+YPT_HD vec3 sky_color(const SceneView& S, const FrameConstants& fc, vec3 dir) {
   if (fc.flags & FRAME_FLAG_WHITE_FURNACE) return vec3(1.f);
-  return vabs(dir) * fc.envIntensityScale;
+  return env_color(S, dir) * fc.envIntensityScale;
 }
 
 // encodeOctahedral / decodeOctahedral (commonMath.hlsl:96-117)
@@ -512,7 +519,7 @@ YPT_HD uint32_t trace_light_path(const SceneView& S, const FrameConstants& fc, c
     float radius = sqrtf(rsq);
     lightPos = c - lightDir * radius + radius * (cx * v1 + cy * v2);
     pdfPos = safe_div(1.f, PI * rsq);
-    radiance = rgb_unbounded_spectrum(S, sky_color(fc, -lightDir), L);
+    radiance = rgb_unbounded_spectrum(S, sky_color(S, fc, -lightDir), L);
     lightNormal = lightDir;  // the disk faces the emitted direction
     profile = 1.f;
     pick = envPdf;
@@ -1068,8 +1075,8 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
   if (!hit && !isExit) {
     cv.curFwdMIS = c.pdfForwardCamera;
     float w = mis_camera_misses(fc, k, cv, c.rayDir);
-    vec3 sky = (fc.flags & FRAME_FLAG_WHITE_FURNACE) ? vec3(1.f) : vabs(c.rayDir) * fc.envIntensityScale;
-    c.total = c.total + cv.throughput * rgb_unbounded_spectrum(S, sky, L) * w;
+    if (should_contribute(fc, k + 1u, 0u))
+      c.total = c.total + cv.throughput * rgb_unbounded_spectrum(S, sky_color(S, fc, c.rayDir), L) * w;
     c.dim = rand.dim;
     return;
   }
@@ -1100,7 +1107,7 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
   const SurfaceDef& sd = v.ctx.def;
   if (!isExit && !sd.emissive.all_equal(0.f)) {  // emission at an exit was counted at the entry
     float w = mis_camera_hits_light(S, k, h.tri, cv);
-    if (surface_two_sided(sd.flags) || dot(sd.geometryNormal, v.wo) > 0.f) {
+    if (should_contribute(fc, k + 1u, 0u) && (surface_two_sided(sd.flags) || dot(sd.geometryNormal, v.wo) > 0.f)) {
       float profile = emission_profile(dot(v.normalWS, -c.rayDir), sd.emissionFocus);
       c.total = c.total + cv.throughput * sd.emissive * w * profile;
     }
@@ -1119,7 +1126,7 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
     float deferredLight = 0.f;
     for (uint32_t j = 0; j < nodeCount; ++j) {
       LightNode n = unpack_node(nodes[j]);
-      if ((n.flags & LN_DELTA) == 0) {
+      if ((n.flags & LN_DELTA) == 0 && should_contribute(fc, k + 1u, j + 1u)) {
         if (j == 0 && infinite_node(n.flags)) {
           cv.wi = normalize(-infinite_node_direction(S, n));
         } else {
@@ -1273,7 +1280,7 @@ YPT_HD void splat_light_path(const SceneView& S, const FrameConstants& fc, const
   for (uint32_t i = 1; i < count; ++i) {
     LightNode n = unpack_node(nodes[i]);
     float u, v;
-    if (project_to_uv(fc, lensPosVS, n.pos, u, v) && (n.flags & LN_DELTA) == 0) {
+    if (project_to_uv(fc, lensPosVS, n.pos, u, v) && (n.flags & LN_DELTA) == 0 && should_contribute(fc, 0u, i + 1u)) {
       ScatterEval ev;
       float unused;
       if (connect_light_node(S, fc, n, i, surfs.get(S, n, i, prevPos, L), lensPosWS, false, L, ev, unused, prevPos, prevNrm) &&

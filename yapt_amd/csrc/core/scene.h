@@ -6,6 +6,7 @@
 // owning a contiguous triangle range, one material and optionally one mesh light.
 #pragma once
 #include "material.h"
+#include "ymath.h"
 
 namespace ypt {
 
@@ -67,6 +68,9 @@ struct FrameConstants {
   uint32_t lightPathVertices;  // maxVerticesPerLightPath
   uint32_t lightPathsPerDim[2];
   uint32_t punctualLightCount;  // the last punctualLightCount entries of the light table
+  // strategies that contribute: camera vertices in [nodeRanges[2], nodeRanges[3]] and light vertices in
+  // [nodeRanges[0], nodeRanges[1]] (lightAndCameraPathConstraints, BDPTCommon.hlsl:233-234)
+  uint32_t nodeRanges[4];
 };
 
 // one RGBA8 texture in the texel buffer (ypt_texture)
@@ -95,6 +99,17 @@ struct AlphaView {
   const uint32_t* texels;
 };
 
+// Environment texture, the World.Skycube renderer variable (RVARNAME_SKYBOX, CRenderer.h:18) as
+// RaytraceCommonResources.cpp:285-313 classifies it: none, a cube map or a long-lat image
+// (ENVIRONMENT_TYPE_*, RaytraceCommonResources.h:17-19).
+static constexpr uint32_t ENV_NONE = 0, ENV_CUBE = 1, ENV_LONGLAT = 2;
+
+struct EnvView {
+  const float* texels;  // linear RGBA, rows top to bottom; a cube map holds its faces +X, -X, +Y, -Y, +Z, -Z
+  uint32_t type;        // ENV_*
+  uint32_t width, height;  // of the long-lat image or of one cube face
+};
+
 struct SceneView {
   const float* pos;          // 3 floats per vertex
   const float* nrm;          // 3 floats per vertex
@@ -115,6 +130,7 @@ struct SceneView {
   uint32_t ninst;
   SpectralTables spec;
   MSLuts luts;
+  EnvView env;
 };
 
 static constexpr float RAY_MIN_T = 5e-4f;  // DEFAULT_RAY_MIN_T (raytraceCommon.hlsl:22-23)
@@ -162,6 +178,99 @@ YPT_HD void sample_texture(const SceneView& S, uint32_t index, float u, float v,
     float bot = c[k] + ax * (d[k] - c[k]);
     out[k] = top + ay * (bot - top);
   }
+}
+
+// Bilinear filter of one environment image (a long-lat image or a cube face) at (u, v), between texel
+// centres: wrap addressing for the long-lat image (the LINEAR_REPEAT sampler, also across the poles, as
+// the sampler does), clamp to the face edge for a cube face. Hardware cube sampling filters across face
+// edges; here a lookup within half a texel of an edge repeats the edge texels instead.
+YPT_HD vec3 env_bilinear(const EnvView& e, uint32_t face, float u, float v, bool wrap) {
+  const uint32_t W = e.width, H = e.height;
+  if (wrap) {
+    u = u - floorf(u);
+    v = v - floorf(v);
+  }
+  if (!(u == u)) u = 0.f;  // a NaN or infinite coordinate (a degenerate direction) reads a corner texel
+  if (!(v == v)) v = 0.f;
+  float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+  float fx = floorf(x), fy = floorf(y);
+  float ax = x - fx, ay = y - fy;
+  uint32_t x0, x1, y0, y1;
+  if (wrap) {
+    x0 = fx < 0.f ? W - 1u : (uint32_t)fx;
+    y0 = fy < 0.f ? H - 1u : (uint32_t)fy;
+    if (!(x0 < W)) x0 = W - 1u;
+    if (!(y0 < H)) y0 = H - 1u;
+    x1 = x0 + 1u == W ? 0u : x0 + 1u;
+    y1 = y0 + 1u == H ? 0u : y0 + 1u;
+  } else {
+    x0 = fx < 0.f ? 0u : (fx >= (float)(W - 1u) ? W - 1u : (uint32_t)fx);
+    y0 = fy < 0.f ? 0u : (fy >= (float)(H - 1u) ? H - 1u : (uint32_t)fy);
+    x1 = fx + 1.f >= (float)(W - 1u) ? W - 1u : (fx + 1.f < 0.f ? 0u : (uint32_t)(fx + 1.f));
+    y1 = fy + 1.f >= (float)(H - 1u) ? H - 1u : (fy + 1.f < 0.f ? 0u : (uint32_t)(fy + 1.f));
+  }
+  const float* base = e.texels + (size_t)face * W * H * 4u;
+  const float* a = base + 4u * ((size_t)y0 * W + x0);
+  const float* b = base + 4u * ((size_t)y0 * W + x1);
+  const float* c = base + 4u * ((size_t)y1 * W + x0);
+  const float* d = base + 4u * ((size_t)y1 * W + x1);
+  float out[3];
+  for (int k = 0; k < 3; ++k) {
+    float top = a[k] + ax * (b[k] - a[k]);
+    float bot = c[k] + ax * (d[k] - c[k]);
+    out[k] = top + ay * (bot - top);
+  }
+  return vec3(out[0], out[1], out[2]);
+}
+
+// Cube-map face and face coordinates of a direction: the major axis picks the face (z before y before x
+// on ties, as the hardware's cube-id rule), then (s, t) = ((sc / |ma| + 1) / 2, (tc / |ma| + 1) / 2) with
+// the face table shared by D3D and Vulkan (+X: -z, -y; -X: z, -y; +Y: x, z; -Y: x, -z; +Z: x, -y;
+// -Z: -x, -y).
+YPT_HD void env_cube_coords(vec3 d, uint32_t& face, float& s, float& t) {
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  float ma, sc, tc;
+  if (az >= ax && az >= ay) {
+    bool pos = !(d.z < 0.f);
+    face = pos ? 4u : 5u;
+    ma = az;
+    sc = pos ? d.x : -d.x;
+    tc = -d.y;
+  } else if (ay >= ax) {
+    bool pos = !(d.y < 0.f);
+    face = pos ? 2u : 3u;
+    ma = ay;
+    sc = d.x;
+    tc = pos ? d.z : -d.z;
+  } else {
+    bool pos = !(d.x < 0.f);
+    face = pos ? 0u : 1u;
+    ma = ax;
+    sc = pos ? -d.z : d.z;
+    tc = -d.y;
+  }
+  s = 0.5f * (sc / ma + 1.f);
+  t = 0.5f * (tc / ma + 1.f);
+}
+
+// getSkyBoxColor (materialsCommonResources.hlsl:181-223), RGB: the long-lat image at
+// (atan2(z, x) / 2pi + 1/2, acos(y) / pi) of the normalised direction (cartesianToSpherical,
+// commonMath.hlsl:144-151), the cube map along the direction, or |dir| without a texture
+YPT_HD vec3 env_color(const SceneView& S, vec3 dir) {
+  const EnvView& e = S.env;
+  if (e.type == ENV_LONGLAT) {
+    vec3 n = normalize(dir);
+    float theta = yatan2(n.z, n.x);
+    float phi = yacos(clampf(n.y, -1.f, 1.f));
+    return env_bilinear(e, 0u, theta * INVPI * 0.5f + 0.5f, phi * INVPI, true);
+  }
+  if (e.type == ENV_CUBE) {
+    uint32_t face;
+    float s, t;
+    env_cube_coords(dir, face, s, t);
+    return env_bilinear(e, face, s, t, false);
+  }
+  return vabs(dir);
 }
 
 // fetchMeshUV (materialsCommon.hlsl): interpolated UV, (0.5, 0.5) for meshes without UVs

@@ -188,7 +188,7 @@ YPT_HD void shade_miss(const SceneView& S, const FrameConstants& fc, PathState& 
   Lambdas L = spectral_lambdas(ps.lambda);
   vec3 color;
   if (fc.flags & FRAME_FLAG_WHITE_FURNACE) color = vec3(1.f);
-  else color = vabs(ps.dir) * fc.envIntensityScale;  // getSkyBoxColor with no environment texture
+  else color = env_color(S, ps.dir) * fc.envIntensityScale;  // getSkyBoxColor (rayMissEnvironment.hlsl:23)
   ps.totalLight = ps.totalLight + ps.throughput * rgb_unbounded_spectrum(S, color, L);
   ps.rayState = RAY_TERMINATED;
 }

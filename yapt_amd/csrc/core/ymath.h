@@ -5,7 +5,9 @@
 // +,-,*,/ and exact operations (floor, bit manipulation), so they return the same bits on both.
 // Method and coefficients: Cephes single-precision library (S. L. Moshier), sinf.c / cosf.c
 // (three-part Cody-Waite reduction by pi/4, degree-7/8 minimax polynomials), expf.c (reduction by
-// ln2, degree-6 polynomial), logf.c (mantissa in [sqrt(1/2), sqrt(2)), degree-9 polynomial).
+// ln2, degree-6 polynomial), logf.c (mantissa in [sqrt(1/2), sqrt(2)), degree-9 polynomial),
+// atanf.c (reduction by tan(3pi/8) and tan(pi/8), odd degree-9 polynomial), atan2f.c (quadrant
+// offset by pi), asinf.c (degree-11 odd polynomial, half-angle identity above 1/2) and acosf.c.
 // Accuracy is a few ulp over the argument ranges the renderer uses (|x| < 8192 for sin/cos).
 #pragma once
 #include "common.h"
@@ -98,6 +100,66 @@ YPT_HD float ypow(float x, float y) {
   if (x == 0.f) return y > 0.f ? 0.f : as_float(0x7f800000u);
   if (x < 0.f) return as_float(0x7fc00000u);
   return yexp(y * ylog(x));
+}
+
+// Cephes atanf: |x| above tan(3pi/8) maps to -1/|x| with offset pi/2, above tan(pi/8) to
+// (|x| - 1) / (|x| + 1) with offset pi/4.
+YPT_HD float yatan(float xin) {
+  float x = fabsf(xin), y;
+  if (x > 2.414213562373095f) {
+    y = PI_OVER_2;
+    x = -(1.f / x);
+  } else if (x > 0.4142135623730950f) {
+    y = PI_OVER_4;
+    x = (x - 1.f) / (x + 1.f);
+  } else {
+    y = 0.f;
+  }
+  float z = x * x;
+  y = y + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z * x + x);
+  return xin < 0.f ? -y : y;
+}
+
+// Cephes atan2f: the angle of (x, y) in [-pi, pi]
+YPT_HD float yatan2(float y, float x) {
+  if (x == 0.f) return y < 0.f ? -PI_OVER_2 : (y == 0.f ? 0.f : PI_OVER_2);
+  if (y == 0.f) return x < 0.f ? PI : 0.f;
+  float w = x < 0.f ? (y < 0.f ? -PI : PI) : 0.f;
+  return w + yatan(y / x);
+}
+
+// Cephes asinf for |x| <= 1 (0 outside, as Cephes's domain error returns)
+YPT_HD float yasin(float xin) {
+  float a = fabsf(xin);
+  if (a > 1.f) return 0.f;
+  float z;
+  if (a < 1e-4f) {
+    z = a;
+  } else {
+    bool flag = a > 0.5f;
+    float x;
+    if (flag) {
+      z = 0.5f * (1.f - a);
+      x = sqrtf(z);
+    } else {
+      x = a;
+      z = x * x;
+    }
+    z = (((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z + 1.6666752422e-1f) * z * x) + x;
+    if (flag) {
+      z = z + z;
+      z = PI_OVER_2 - z;
+    }
+  }
+  return xin < 0.f ? -z : z;
+}
+
+// Cephes acosf for |x| <= 1 (0 outside)
+YPT_HD float yacos(float x) {
+  if (x < -1.f || x > 1.f) return 0.f;
+  if (x < -0.5f) return PI - 2.f * yasin(sqrtf(0.5f * (1.f + x)));
+  if (x > 0.5f) return 2.f * yasin(sqrtf(0.5f * (1.f - x)));
+  return PI_OVER_2 - yasin(x);
 }
 
 }  // namespace ypt

@@ -35,6 +35,8 @@ struct PreparedScene {
   std::vector<float> srgbLut;           // 256 entries, with textures
   std::vector<AlphaTri> alphaTris;      // alpha-tested triangles
   std::vector<uint32_t> triAlpha;       // per triangle: 0 = opaque, else 1 + its alphaTris record
+  uint32_t envType = ENV_NONE, envWidth = 0, envHeight = 0;
+  std::vector<float> envTexels;         // RGBA, one image or six cube faces
   float boundsMin[3], boundsMax[3];
 };
 
@@ -61,6 +63,27 @@ inline SpectralColor unbounded_color(Rgb2Spec& r2s, const float v[3]) {
   rgb_to_coeffs(r2s, n, s.c);
   s.scale = scale;
   return s;
+}
+
+// the World.Skycube texture of a scene description, validated (RaytraceCommonResources.cpp:285-313)
+inline void prepare_environment(const ypt_scene_desc& d, PreparedScene& out) {
+  out.envType = ENV_NONE;
+  out.envWidth = out.envHeight = 0;
+  out.envTexels.clear();
+  if (d.environment && d.environment->type != YPT_ENVIRONMENT_NONE) {
+    const ypt_environment& e = *d.environment;
+    if (e.type != YPT_ENVIRONMENT_CUBE && e.type != YPT_ENVIRONMENT_LONGLAT) throw std::invalid_argument("unknown environment type");
+    if (e.width == 0 || e.height == 0 || !e.rgba) throw std::invalid_argument("environment without texels");
+    if (e.type == YPT_ENVIRONMENT_CUBE && e.width != e.height) throw std::invalid_argument("cube map faces must be square");
+    if (e.width > 65536u || e.height > 65536u) throw std::invalid_argument("environment larger than 65536 texels a side");
+    size_t n = (size_t)e.width * e.height * 4u * (e.type == YPT_ENVIRONMENT_CUBE ? 6u : 1u);
+    out.envTexels.assign(e.rgba, e.rgba + n);
+    for (float v : out.envTexels)
+      if (!(v >= 0.f) || !(v < INFINITY)) throw std::invalid_argument("environment texels must be finite and non-negative");
+    out.envType = e.type == YPT_ENVIRONMENT_CUBE ? ENV_CUBE : ENV_LONGLAT;
+    out.envWidth = e.width;
+    out.envHeight = e.height;
+  }
 }
 
 inline MaterialGPU convert_material(Rgb2Spec& r2s, const ypt_material& in, uint32_t textureCount) {
@@ -222,6 +245,7 @@ inline void prepare_scene(const ypt_scene_desc& d, Rgb2Spec& r2s, PreparedScene&
     out.texels.resize(base + n);
     std::memcpy(out.texels.data() + base, tx.rgba8, n * 4);
   }
+  prepare_environment(d, out);
   if (d.texture_count) {  // IEC 61966-2-1 decoding of each byte value
     out.srgbLut.resize(256);
     for (int b = 0; b < 256; ++b) {
@@ -399,6 +423,9 @@ inline void prepare_frame(const ypt_frame_desc& f, const PreparedScene& s, uint3
   fc.lightPathVertices = lv;
   fc.lightPathsPerDim[0] = (f.width + 7u) / 8u;   // one light subpath per 8x8 texels (TEXELS_PER_LIGHTPATH)
   fc.lightPathsPerDim[1] = (f.height + 7u) / 8u;
+  // _DEBUG.BDPT.EffectiveLightPathNodes / EffectiveCameraPathNodes, default (0, 128) each (RendererVarsList.h:46-47)
+  const bool ranges = (f.flags & YPT_FLAG_BDPT_NODE_RANGES) != 0;
+  for (int k = 0; k < 4; ++k) fc.nodeRanges[k] = ranges ? f.bdpt_node_ranges[k] : (k & 1 ? 128u : 0u);
 }
 
 }  // namespace ypt

@@ -153,7 +153,7 @@ struct ResidentScene {
   DevBuf<Bvh2Node> dNodes2;
   DevBuf<Bvh8Node> dNodes8;
   DevBuf<TriRecord> dTris;
-  DevBuf<float> dUv, dSrgbLut;
+  DevBuf<float> dUv, dSrgbLut, dEnv;
   DevBuf<TextureGPU> dTex;
   DevBuf<uint32_t> dTexels;
   DevBuf<AlphaTri> dAlpha;
@@ -164,7 +164,7 @@ struct ResidentScene {
     return dPos.n * 4 + dNrm.n * 4 + dTan.n * 4 + dIdx.n * 4 + dTriInst.n * 4 + dInst.n * sizeof(InstanceGPU) +
            dMats.n * sizeof(MaterialGPU) + dLights.n * sizeof(LightGPU) + dNodes2.n * sizeof(Bvh2Node) +
            dNodes8.n * sizeof(Bvh8Node) + dTris.n * sizeof(TriRecord) + dUv.n * 4 + dTex.n * sizeof(TextureGPU) + dTexels.n * 4 +
-           dSrgbLut.n * 4 + dAlpha.n * sizeof(AlphaTri) + dTriAlpha.n * 4;
+           dSrgbLut.n * 4 + dAlpha.n * sizeof(AlphaTri) + dTriAlpha.n * 4 + dEnv.n * 4;
   }
 };
 
@@ -291,6 +291,7 @@ struct ypt_integrator {
     s.sv.srgbLut = r.dSrgbLut.n ? r.dSrgbLut.p : nullptr;
     s.sv.triAlpha = r.dAlpha.n ? r.dTriAlpha.p : nullptr;
     s.sv.alpha = AlphaView{r.dAlpha.n ? r.dAlpha.p : nullptr, r.dTex.p, r.dTexels.p};
+    s.sv.env = EnvView{r.scene.envType != ENV_NONE ? r.dEnv.p : nullptr, r.scene.envType, r.scene.envWidth, r.scene.envHeight};
     s.sv.ntris = (uint32_t)r.scene.triInst.size();
     s.sv.ninst = (uint32_t)r.scene.inst.size();
     s.sv.spec.cie_xyz4 = dCie.p;
@@ -651,6 +652,7 @@ std::shared_ptr<ResidentScene> make_resident(const ypt_scene_desc& d, int device
     r->dTexels.upload(ps.texels.data(), ps.texels.size(), s);
     r->dSrgbLut.upload(ps.srgbLut.data(), ps.srgbLut.size(), s);
   }
+  if (!ps.envTexels.empty()) r->dEnv.upload(ps.envTexels.data(), ps.envTexels.size(), s);
   if (!ps.alphaTris.empty()) {
     r->dAlpha.upload(ps.alphaTris.data(), ps.alphaTris.size(), s);
     r->dTriAlpha.upload(ps.triAlpha.data(), ps.triAlpha.size(), s);

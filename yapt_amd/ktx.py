@@ -1,5 +1,6 @@
 """KTX 2.0 and KTX 1.1 textures with uncompressed 8-bit texels, read into the RGBA8 arrays the
-library uploads.
+library uploads; KTX 2.0 float textures and cube maps, read into the linear float arrays of an
+environment (decode_float).
 
 The reference converts its images with the Khronos `ktx create` tool to R8G8B8A8_SRGB or
 R8G8B8A8_UNORM KTX2 files and loads them with pyktx (pythonScripts/yapt/texture_loader_ktx.py:95-140,
@@ -29,6 +30,11 @@ VK_FORMATS = {
     44: (4, True, False), 50: (4, True, True),             # B8G8R8A8_UNORM, B8G8R8A8_SRGB
 }
 SUPERCOMPRESSION = {1: "BasisLZ", 2: "Zstandard", 3: "ZLIB"}
+# float vkFormats (for HDR environments) -> (numpy type, channels)
+VK_FLOAT_FORMATS = {
+    76: (np.float16, 1), 83: (np.float16, 2), 90: (np.float16, 3), 97: (np.float16, 4),   # R16.._SFLOAT
+    100: (np.float32, 1), 103: (np.float32, 2), 106: (np.float32, 3), 109: (np.float32, 4),  # R32.._SFLOAT
+}
 
 # KTX 1: glFormat -> channels, and the sRGB internal formats
 GL_RED, GL_RG, GL_RGB, GL_RGBA, GL_UNSIGNED_BYTE = 0x1903, 0x8227, 0x1907, 0x1908, 0x1401
@@ -69,6 +75,66 @@ def _read_ktx2(data):
     if bgr:
         texels = texels[..., [2, 1, 0] + ([3] if channels == 4 else [])]
     return _expand(texels, channels), srgb
+
+
+def _srgb_to_linear(c):
+    return np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
+
+
+def decode_float(data):
+    """(texels, cube) from the bytes of a KTX2 file, for an environment: linear float32 H x W x 4, or
+    6 x N x N x 4 when the file holds the six faces of a cube map (faceCount 6, in the order +X, -X, +Y,
+    -Y, +Z, -Z). _SFLOAT formats keep their values; 8-bit formats are decoded (sRGB or unorm) to
+    linear. Mip level 0 only."""
+    data = bytes(data)
+    if data[:12] != KTX2_IDENTIFIER:
+        raise ValueError("not a KTX2 file")
+    (vk_format, type_size, width, height, depth, layers, faces, levels,
+     scheme) = struct.unpack_from("<9I", data, 12)
+    if scheme:
+        raise ValueError("KTX2 supercompression %s is not supported" % SUPERCOMPRESSION.get(scheme, scheme))
+    if depth > 1 or layers > 1 or faces not in (1, 6):
+        raise ValueError("only 2D and cube-map KTX2 textures are supported (depth %d, layers %d, faces %d)" % (depth, layers, faces))
+    if width == 0 or height == 0:
+        raise ValueError("KTX2 texture without texels")
+    offset, length, _ = struct.unpack_from("<3Q", data, 80)
+    if vk_format in VK_FLOAT_FORMATS:
+        dt, channels = VK_FLOAT_FORMATS[vk_format]
+        size = np.dtype(dt).itemsize
+        n = faces * height * width * channels
+        if length < n * size or offset + n * size > len(data):
+            raise ValueError("KTX2 level 0 is shorter than %d face(s) of %d x %d texels" % (faces, width, height))
+        t = np.frombuffer(data, dt, n, offset).astype(np.float32).reshape(faces, height, width, channels)
+        out = np.zeros((faces, height, width, 4), np.float32)
+        out[..., 3] = 1.0
+        if channels == 1:
+            out[..., :3] = t
+        else:
+            out[..., :channels] = t
+    elif vk_format in VK_FORMATS:
+        channels, bgr, srgb = VK_FORMATS[vk_format]
+        n = faces * height * width * channels
+        if length < n or offset + n > len(data):
+            raise ValueError("KTX2 level 0 is shorter than %d face(s) of %d x %d texels" % (faces, width, height))
+        t = np.frombuffer(data, np.uint8, n, offset).reshape(faces, height, width, channels)
+        if bgr:
+            t = t[..., [2, 1, 0] + ([3] if channels == 4 else [])]
+        out = np.stack([_expand(f, channels) for f in t]).astype(np.float64) / 255.0
+        if srgb:
+            out[..., :3] = _srgb_to_linear(out[..., :3])
+        out = out.astype(np.float32)
+    else:
+        raise ValueError("KTX2 vkFormat %d is not supported (uncompressed 8-bit, 16- and 32-bit float formats)" % vk_format)
+    if faces == 6:
+        if width != height:
+            raise ValueError("cube-map faces must be square")
+        return out, True
+    return out[0], False
+
+
+def read_float(path):
+    with open(path, "rb") as f:
+        return decode_float(f.read())
 
 
 def _read_ktx1(data):

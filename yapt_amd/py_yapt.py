@@ -541,20 +541,55 @@ class Material:
         return m
 
 
-class Texture:
-    """an RGBA8 image; srgb means the texels are sRGB-encoded colour (an _SRGB format)"""
+class ResourceDimension(enum.IntEnum):
+    """the texture dimensions of PyRendererBasicTypes.cpp that textures here take"""
+    TEXTURE_2D = 3
+    TEXTURE_CUBEMAP = 5
 
-    def __init__(self, name, rgba8, srgb=True):
-        a = np.ascontiguousarray(rgba8, dtype=np.uint8)
-        if a.ndim != 3 or a.shape[2] != 4:
-            raise ValueError("textures are H x W x 4 uint8")
-        self._name, self.texels, self.srgb = name, a, bool(srgb)
+
+class Texture:
+    """An RGBA8 image (srgb: the texels are sRGB-encoded colour, an _SRGB format), or a linear float image
+    (an _SFLOAT format: H x W x 4, or six square faces +X, -X, +Y, -Y, +Z, -Z for a cube map), the kind an
+    HDR environment is. Materials bind RGBA8 textures; the World.Skycube variable takes either."""
+
+    def __init__(self, name, texels, srgb=True, cube=False):
+        a = np.asarray(texels)
+        self._name = name
+        self.is_float = a.dtype.kind == "f"
+        self.dimension = ResourceDimension.TEXTURE_CUBEMAP if cube else ResourceDimension.TEXTURE_2D
+        if self.is_float:
+            kind = _lib.YPT_ENVIRONMENT_CUBE if cube else _lib.YPT_ENVIRONMENT_LONGLAT
+            self.texels, self.srgb = _lib.environment_texels(kind, a), False
+        else:
+            if cube:
+                raise ValueError("cube maps take float texels (6 x N x N x 4)")
+            a = np.ascontiguousarray(a, dtype=np.uint8)
+            if a.ndim != 3 or a.shape[2] != 4:
+                raise ValueError("textures are H x W x 4 uint8")
+            self.texels, self.srgb = a, bool(srgb)
 
     def getName(self):
         return self._name
 
+    def getDimension(self):
+        return self.dimension
+
     def min_alpha(self):
+        if self.is_float:
+            return None
         return float(self.texels[..., 3].min()) / 255.0
+
+    def environment(self):
+        """(YPT_ENVIRONMENT_*, linear float texels) for this texture as the sky: a 2D texture is a long-lat
+        image (RaytraceCommonResources.cpp:290-303); RGBA8 texels are decoded (sRGB or unorm) first"""
+        if self.dimension == ResourceDimension.TEXTURE_CUBEMAP:
+            return _lib.YPT_ENVIRONMENT_CUBE, self.texels
+        if self.is_float:
+            return _lib.YPT_ENVIRONMENT_LONGLAT, self.texels
+        c = self.texels.astype(np.float64) / 255.0
+        if self.srgb:
+            c[..., :3] = np.where(c[..., :3] <= 0.04045, c[..., :3] / 12.92, ((c[..., :3] + 0.055) / 1.055) ** 2.4)
+        return _lib.YPT_ENVIRONMENT_LONGLAT, c.astype(np.float32)
 
 
 class Mesh:
@@ -626,6 +661,18 @@ class RendererVariable:
         self._value = v
         self._renderer._touch_frame()
 
+    def setTexture(self, texture):
+        """a TEXTURE variable (World.Skycube): the texture, or None to clear it"""
+        if self._kind != RendererVariableType.TEXTURE:
+            raise TypeError("%s is not a texture variable" % self._name)
+        if texture is not None and not isinstance(texture, Texture):
+            raise TypeError("%s takes a Texture" % self._name)
+        self._texture = texture
+        self._renderer._touch()     # the sky travels with the committed scene
+
+    def getTexture(self):
+        return getattr(self, "_texture", None)
+
     def setFromFloatArray(self, values):
         if self._kind != RendererVariableType.FLOAT:
             raise TypeError("%s is not a float variable" % self._name)
@@ -666,9 +713,10 @@ class RendererVariable:
 
 
 def _renderer_variables(r):
-    F, I, O = RendererVariableType.FLOAT, RendererVariableType.INT, RendererVariableType.OPTIONS
+    F, I, O, T = RendererVariableType.FLOAT, RendererVariableType.INT, RendererVariableType.OPTIONS, RendererVariableType.TEXTURE
     on_off = ("OFF", "ON")
     defs = [   # RendererVarsList.h:21-54
+        ("World.Skycube", T, 0, None, None, None),
         ("World.EnvIntensityScale", F, 1.0, 0.0, 100000.0, None),
         ("Generic.RenderResolution", I, (1920, 1080), (2, 2), (99999999, 99999999), None),
         ("RenderPipeline", O, 1, None, None, ("BPT", "BPT_WITH_NEE", "BDPT", "DBG_Normals", "DBG_Tangents", "DBG_UV")),
@@ -696,6 +744,8 @@ def _renderer_variables(r):
         ("Tonemap.ManualExposure", F, 1.0, 0.0, 9999999999999999.0, None),
         # the bounce limit of the path loop (the reference fixes maxRayDepth = 32,
         # RaytraceCommonResources.cpp:80) and BDPT's light subpath length, exposed here as variables
+        ("_DEBUG.BDPT.EffectiveLightPathNodes", I, (0, 128), (0, 0), (99999999, 99999999), None),
+        ("_DEBUG.BDPT.EffectiveCameraPathNodes", I, (0, 128), (0, 0), (99999999, 99999999), None),
         ("Integrator.MaxDepth", I, 32, 1, 64, None),
         ("Integrator.LightPathVertices", I, 16, 1, 16, None),
     ]
@@ -818,6 +868,8 @@ class Scene:
         nverts = ntris = 0
 
         def texture_slot(t):
+            if t.is_float:
+                raise ValueError("material textures are RGBA8; %s holds float texels" % t.getName())
             if id(t) not in tex_index:
                 tex_index[id(t)] = len(textures)
                 textures.append((t.texels, t.srgb))
@@ -881,9 +933,11 @@ class Scene:
         if not pos:
             raise ValueError("the scene has no render object with a mesh")
         lights = [o._world_light() for o in self._objects if isinstance(o, Light)]
+        sky = self._renderer.getRendererVariable("World.Skycube").getTexture()
         return _lib.SceneArrays(np.concatenate(pos), np.concatenate(nrm), np.concatenate(idx), inst_first, inst_count,
                                 inst_mat, materials, tangents=np.concatenate(tan).astype(np.float32), lights=lights,
-                                uvs=np.concatenate(uv) if any_uv else None, textures=textures)
+                                uvs=np.concatenate(uv) if any_uv else None, textures=textures,
+                                environment=None if sky is None else sky.environment())
 
 
 def _area_normals(P, I):
@@ -937,8 +991,8 @@ class Renderer:
     def createMesh(self, name, positions, indices, normals=None, uvs=None, tangents=None, submeshes=None):
         return Mesh(name, positions, indices, normals, uvs, tangents, submeshes)
 
-    def createTexture(self, name, rgba8, srgb=True):
-        return Texture(name, rgba8, srgb)
+    def createTexture(self, name, texels, srgb=True, cube=False):
+        return Texture(name, texels, srgb, cube)
 
     def getRendererVariable(self, name):
         return self._vars.get(name)
@@ -964,7 +1018,9 @@ class Renderer:
             firefly_clamp_scale=v("Firefly.FireflyClampScale").getAsFloatArray()[0],
             firefly_clamp_relax_power=v("Firefly.FireflyClampRelaxPower").getAsFloatArray()[0],
             kill_nan_samples=v("Firefly.KillNaNSamples").getSelectedOption(),
-            light_path_vertices=v("Integrator.LightPathVertices").getAsIntArray()[0])
+            light_path_vertices=v("Integrator.LightPathVertices").getAsIntArray()[0],
+            bdpt_light_nodes=[min(x, 0xFFFFFFFF) for x in v("_DEBUG.BDPT.EffectiveLightPathNodes").getAsIntArray()],
+            bdpt_camera_nodes=[min(x, 0xFFFFFFFF) for x in v("_DEBUG.BDPT.EffectiveCameraPathNodes").getAsIntArray()])
 
     def post_params(self):
         v = self._var
@@ -1050,9 +1106,13 @@ class Resources:
 
     def load_texture_2D(self, tex_path, fmt="R8G8B8A8_SRGB", verbose=False):
         """an image file as a texture in `fmt` (R8G8B8A8_SRGB or R8G8B8A8_UNORM); a KTX file keeps the
-        format it was written in"""
+        format it was written in. An _SFLOAT format, or an .hdr / .exr / .npy file, gives a linear float
+        texture, the kind World.Skycube takes for a long-lat environment."""
         from . import ktx
-        if str(tex_path).lower().endswith((".ktx", ".ktx2")):
+        low = str(tex_path).lower()
+        if "SFLOAT" in str(fmt).upper() or low.endswith((".hdr", ".exr", ".npy")):
+            return self._load_float(tex_path)
+        if low.endswith((".ktx", ".ktx2")):
             rgba, srgb = ktx.read(tex_path)
         else:
             from PIL import Image
@@ -1061,6 +1121,36 @@ class Resources:
         tex = Texture(os.path.basename(str(tex_path)), rgba, srgb=srgb)
         self._min_alpha[str(tex_path)] = tex.min_alpha()
         return tex
+
+    def _load_float(self, tex_path):
+        from . import hdr_images, ktx
+        name = os.path.basename(str(tex_path))
+        if str(tex_path).lower().endswith((".ktx", ".ktx2")):
+            texels, cube = ktx.read_float(tex_path)
+            return Texture(name, texels, cube=cube)
+        if str(tex_path).lower().endswith((".png", ".jpg", ".jpeg")):
+            from PIL import Image
+            c = np.asarray(Image.open(tex_path).convert("RGBA"), np.float64) / 255.0
+            c[..., :3] = np.where(c[..., :3] <= 0.04045, c[..., :3] / 12.92, ((c[..., :3] + 0.055) / 1.055) ** 2.4)
+            return Texture(name, c.astype(np.float32))
+        return Texture(name, hdr_images.read(tex_path))
+
+    def load_texture_cube(self, tex_path, fmt="R16G16B16A16_SFLOAT", faces=None, verbose=False):
+        """a cube map for World.Skycube: a KTX2 file with six faces, or six face images given as `faces` in
+        the order +X, -X, +Y, -Y, +Z, -Z (texture_loader_ktx.py:150-175); tex_path names the texture"""
+        name = os.path.basename(str(tex_path))
+        if faces is None:
+            from . import ktx
+            texels, cube = ktx.read_float(tex_path)
+            if not cube:
+                raise ValueError("%s holds no cube map (faceCount 1)" % tex_path)
+            return Texture(name, texels, cube=True)
+        if len(faces) != 6:
+            raise ValueError("a cube map takes 6 face images, got %d" % len(faces))
+        imgs = [self._load_float(f).texels for f in faces]
+        if any(i.shape != imgs[0].shape or i.shape[0] != i.shape[1] for i in imgs):
+            raise ValueError("cube-map faces must be square and of one size")
+        return Texture(name, np.stack(imgs), cube=True)
 
     def get_texture_min_alpha(self, tex_path):
         return self._min_alpha.get(str(tex_path))

@@ -154,11 +154,12 @@ class MeshBuilder:
         p = np.stack([c, c + u, c + u + v, c + v])
         self.add(p, np.tile(n, (4, 1)), [[0, 1, 2], [0, 2, 3]], material)
 
-    def arrays(self, lights=(), uvs=None, textures=()):
+    def arrays(self, lights=(), uvs=None, textures=(), environment=None):
         nrm = np.concatenate(self.nrm)
         return _lib.SceneArrays(np.concatenate(self.pos), nrm, np.concatenate(self.idx),
                                 self.inst_first, self.inst_count, self.inst_mat, self.materials,
-                                tangents=arbitrary_tangents(nrm), lights=lights, uvs=uvs, textures=textures)
+                                tangents=arbitrary_tangents(nrm), lights=lights, uvs=uvs, textures=textures,
+                                environment=environment)
 
 
 def box_template():
@@ -326,12 +327,67 @@ def focused_emission(focus=8.0, two_sided=False):
     return b.arrays(), cam
 
 
-def punctual_lights(light_type="mixed", sphere_res=(24, 24)):
+def longlat_directions(width, height):
+    """unit directions of the texel centres of a long-lat environment image (H x W x 3): the inverse of
+    getSkyBoxColor's mapping u = atan2(z, x) / 2pi + 1/2, v = acos(y) / pi (row 0 looks up)"""
+    u = (np.arange(width) + 0.5) / width
+    v = (np.arange(height) + 0.5) / height
+    theta = (u - 0.5) * 2.0 * math.pi
+    phi = v * math.pi
+    st, ct = np.sin(phi)[:, None], np.cos(phi)[:, None]
+    return np.stack(np.broadcast_arrays(st * np.cos(theta)[None, :], ct, st * np.sin(theta)[None, :]), axis=-1)
+
+
+def cube_face_directions(size):
+    """unit directions of the texel centres of the six faces +X, -X, +Y, -Y, +Z, -Z (6 x N x N x 3), with the
+    D3D / Vulkan face orientation: for face coordinates sc = 2s - 1, tc = 2t - 1 (s along a row, t down the
+    rows) the direction is +X (1, -tc, -sc), -X (-1, -tc, sc), +Y (sc, 1, tc), -Y (sc, -1, -tc),
+    +Z (sc, -tc, 1), -Z (-sc, -tc, -1)"""
+    c = (np.arange(size) + 0.5) / size * 2.0 - 1.0
+    sc, tc = np.meshgrid(c, c)       # sc varies along a row, tc down the rows
+    one = np.ones_like(sc)
+    faces = [(one, -tc, -sc), (-one, -tc, sc), (sc, one, tc), (sc, -one, -tc), (sc, -tc, one), (-sc, -tc, -one)]
+    d = np.stack([np.stack(f, axis=-1) for f in faces])
+    return d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+SUN_DIRECTION = np.array([0.4, 0.6, -0.5]) / np.linalg.norm([0.4, 0.6, -0.5])
+
+
+def sky_radiance(d, sun=40.0):
+    """A procedural HDR sky over unit directions d (..., 3), linear RGB: a blue zenith fading to a pale
+    horizon above the horizon, a darker warm ground below it, and a small bright sun of peak `sun`
+    towards SUN_DIRECTION. Stands in for the reference's HDR environment images, which are absent."""
+    d = np.asarray(d, np.float64)
+    y = d[..., 1:2]
+    up = np.clip(y, 0.0, 1.0)
+    sky = np.array([0.55, 0.7, 0.95]) * (1.0 - up) + np.array([0.15, 0.3, 0.8]) * up
+    ground = np.array([0.25, 0.2, 0.15]) * (0.6 + 0.4 * np.clip(-y, 0.0, 1.0))
+    c = np.clip(d @ SUN_DIRECTION, -1.0, 1.0)[..., None]
+    glow = sun * np.exp((c - 1.0) * 600.0) * np.array([1.0, 0.9, 0.75])
+    return np.where(y > 0.0, sky, ground) + glow
+
+
+def longlat_environment(width=64, height=32, fn=sky_radiance):
+    """(YPT_ENVIRONMENT_LONGLAT, H x W x 4 float32) with fn evaluated at the texel centres"""
+    rgb = fn(longlat_directions(width, height))
+    return _lib.YPT_ENVIRONMENT_LONGLAT, np.concatenate([rgb, np.ones(rgb.shape[:-1] + (1,))], -1).astype(np.float32)
+
+
+def cube_environment(size=16, fn=sky_radiance):
+    """(YPT_ENVIRONMENT_CUBE, 6 x N x N x 4 float32) with fn evaluated at the texel centres of each face"""
+    rgb = fn(cube_face_directions(size))
+    return _lib.YPT_ENVIRONMENT_CUBE, np.concatenate([rgb, np.ones(rgb.shape[:-1] + (1,))], -1).astype(np.float32)
+
+
+def punctual_lights(light_type="mixed", sphere_res=(24, 24), environment=None):
     """The punctual-light test scene (Tests/IntegratorCompare/punctual_lights_scene.py:16-105) built
     from generated meshes: an 80x80 floor and back wall, diffuse, copper and plastic spheres, lit by
     punctual lights only (environment scale 0). light_type: point, spot, directional, mixed (all three
     plus an emissive sphere), mesh_control (a small emissive sphere where the point light is), env (no
-    lights: the scene is lit by the environment at scale 0.5, returned as the third value).
+    lights: the scene is lit by the environment at scale 0.5, returned as the third value). environment:
+    the scene's World.Skycube texture ((type, texels), e.g. longlat_environment()), None for the
+    texture-less sky.
     The reference's plane.glb (a 2x2 quad, load_material_showcase_scene.py:583) and Sphere.glb (unit
     radius) are absent: the planes here are quads of twice their render objects' scale, and the spheres
     have radius = scale. Returns (scene, camera, env_intensity_scale)."""
@@ -386,7 +442,7 @@ def punctual_lights(light_type="mixed", sphere_res=(24, 24)):
     ang = math.radians(25.0)
     eye = (0.0, 14.0, 26.0)
     cam = Camera(eye=eye, target=(0.0, 14.0 - math.sin(ang), 26.0 - math.cos(ang)), fovy=math.pi * 0.3)
-    return b.arrays(lights), cam, (0.5 if light_type == "env" else 0.0)
+    return b.arrays(lights, environment=environment), cam, (0.5 if light_type == "env" else 0.0)
 
 
 def blackbody_rgb(kelvin):
