@@ -20,6 +20,11 @@ extern "C" {
  * Cin % 16 == 0, Cout % 32 == 0; bias may be NULL. */
 int ydn_conv3x3(const uint16_t* in, int N, int H, int W, int Cin, const uint16_t* w, const float* bias, int Cout, int relu,
                 uint16_t* out, void* stream);
+/* the conv of the concatenation [nearest 2x upsample of coarse[N][H/2][W/2][C1], skip[N][H][W][C2]] with
+ * w[Cout][3][3][C1 + C2]: ydn_up2_concat followed by ydn_conv3x3, without the concatenated tensor.
+ * H and W even, C1 and C2 multiples of 16. */
+int ydn_conv3x3_up2cat(const uint16_t* coarse, int C1, const uint16_t* skip, int C2, int N, int H, int W, const uint16_t* w,
+                       const float* bias, int Cout, int relu, uint16_t* out, void* stream);
 /* dw[Cout][9][Cin] += sum over pixels of dout x input patches; db[Cout] += sum of dout (db may be NULL) */
 int ydn_conv3x3_wgrad(const uint16_t* in, const uint16_t* dout, int N, int H, int W, int Cin, int Cout, float* dw, float* db,
                       void* stream);

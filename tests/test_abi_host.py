@@ -104,3 +104,18 @@ def test_denoiser_resampling_rejects_odd_and_empty_shapes():
     assert b"even H and W" in lib.ydn_last_error() or b"empty shape" in lib.ydn_last_error()
     assert lib.ydn_up2_concat(None, 16, p, 16, 1, 8, 8, p, None) == -1
     assert b"null tensor" in lib.ydn_last_error()
+    for H, W, c1, c2, cout in ((7, 8, 32, 32, 32), (8, 8, 24, 32, 32), (8, 8, 32, 32, 48), (0, 8, 32, 32, 32)):
+        assert lib.ydn_conv3x3_up2cat(p, c1, p, c2, 1, H, W, p, None, cout, 1, p, None) == -1
+    assert lib.ydn_conv3x3(p, 1, 8, 8, 24, p, None, 32, 1, p, None) == -1
+    assert b"multiple of 16" in lib.ydn_last_error()
+
+
+def test_denoiser_library_exports_every_declared_function():
+    import re
+    from yapt_amd import denoiser
+    lib = denoiser.load()
+    with open(os.path.join(ROOT, "include", "yapt_denoise.h")) as f:
+        names = re.findall(r"\b(ydn_\w+)\(", f.read())
+    assert len(names) >= 16
+    for n in names:
+        assert hasattr(lib, n), n

@@ -84,7 +84,8 @@ def _f(t):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(2, 12, 20, 16, 32), (1, 9, 7, 32, 64), (2, 16, 16, 48, 128), (1, 5, 33, 96, 96)])
+@pytest.mark.parametrize("shape", [(2, 12, 20, 16, 32), (1, 9, 7, 32, 64), (2, 16, 16, 48, 128), (1, 5, 33, 96, 96),
+                                   (1, 19, 70, 48, 32), (1, 17, 65, 224, 96)])
 def test_conv_forward_matches_torch_fp32_and_checker(shape):
     import torch
     from yapt_amd import denoiser
@@ -111,6 +112,31 @@ def test_conv_forward_matches_torch_fp32_and_checker(shape):
         scale = np.abs(ref_c).max()
         assert np.abs(ref_t - ref_c).max() < 1e-4 * scale                      # the checker is the torch op
         assert np.abs(got - ref_c).max() < 8e-3 * scale, np.abs(got - ref_c).max() / scale   # bf16 output rounding
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(2, 12, 20, 32, 16, 32), (1, 10, 70, 64, 32, 64), (1, 6, 34, 128, 96, 96)])
+def test_fused_upsample_concat_conv_is_the_two_step_conv(shape):
+    # ydn_conv3x3_up2cat stages the upsampled coarse channels and the skip channels itself: the same values
+    # reach the same matrix-core sums as ydn_up2_concat followed by ydn_conv3x3
+    import torch
+    from yapt_amd import denoiser
+    lib = denoiser.load()
+    N, H, W, C1, C2, Cout = shape
+    rng = np.random.default_rng(9)
+    coarse = _bits(R.bf16(rng.normal(size=(N, H // 2, W // 2, C1))))
+    skip = _bits(R.bf16(rng.normal(size=(N, H, W, C2))))
+    w = _bits(R.bf16(rng.normal(0, 0.2, size=(Cout, 3, 3, C1 + C2))))
+    b = torch.from_numpy(rng.normal(0, 0.1, size=Cout).astype(np.float32)).cuda()
+    cat = denoiser.bf16_empty((N, H, W, C1 + C2))
+    two, one = denoiser.bf16_empty((N, H, W, Cout)), denoiser.bf16_empty((N, H, W, Cout))
+    denoiser.check(lib.ydn_up2_concat(denoiser._p(coarse), C1, denoiser._p(skip), C2, N, H, W, denoiser._p(cat), None))
+    denoiser.check(lib.ydn_conv3x3(denoiser._p(cat), N, H, W, C1 + C2, denoiser._p(w), denoiser._p(b), Cout, 1, denoiser._p(two), None))
+    denoiser.check(lib.ydn_conv3x3_up2cat(denoiser._p(coarse), C1, denoiser._p(skip), C2, N, H, W, denoiser._p(w), denoiser._p(b),
+                                          Cout, 1, denoiser._p(one), None))
+    torch.cuda.synchronize()
+    assert torch.equal(one, two)
+    assert _f(one).max() > 0
 
 
 @pytest.mark.gpu

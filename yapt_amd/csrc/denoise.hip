@@ -6,10 +6,10 @@
 // (v_mfma_f32_32x32x16_bf16: a 32x16 by 16x32 product per wavefront and instruction).
 //
 // conv3x3 forward is an implicit GEMM: output [pixels x Cout] = input patches [pixels x 9*Cin] times
-// weights [9*Cin x Cout]. A wavefront owns a 32-output-channel x 32-pixel tile and walks K in steps
-// of 16 (one tap, 16 input channels). Its operands come straight from global memory in 16-byte
-// lane loads (each lane holds 8 consecutive K values of one row or column), which the L1/L2 caches
-// share between the wavefronts of a workgroup.
+// weights [9*Cin x Cout], tiled through LDS (k_conv3x3: a workgroup stages an input patch with its halo
+// and the tile's weights, chunk by chunk of input channels, and its wavefronts multiply from LDS). The
+// decoder's nearest-neighbour upsample and channel concatenation happen in that staging
+// (ydn_conv3x3_up2cat), so the concatenated tensor is never written.
 // The data gradient is the same kernel on the gradient with flipped, transposed weights; the weight
 // gradient is a GEMM over pixels ([Cout x P] times [P x 9*Cin]) split into pixel chunks whose FP32
 // partial tiles are added atomically.
@@ -46,56 +46,207 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
 __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
 
 // ---------------------------------------------------------------- conv3x3 forward (implicit GEMM)
-// grid: (ceil(P / 32), Cout / 32 / WPB), block: 64 * WPB threads; wave w of a block takes output
-// channels [co0 + 32 w, +32). Requirements: Cin % 16 == 0, Cout % 32 == 0.
-template <int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_conv3x3(const uint16_t* __restrict__ in, int N, int H, int W, int Cin,
-                                                      const uint16_t* __restrict__ w, const float* __restrict__ bias,
-                                                      int Cout, int relu, uint16_t* __restrict__ out) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int co0 = (blockIdx.y * WPB + wave) * 32;
-  if (co0 >= Cout) return;
-  const long P = (long)N * H * W;
-  const long p0 = (long)blockIdx.x * 32;
-  const int col = lane & 31, half = lane >> 5;
-  // B operand: this lane's pixel
-  long p = p0 + col;
-  bool pv = p < P;
-  int n = 0, y = 0, x = 0;
-  if (pv) {
-    n = (int)(p / ((long)H * W));
-    int r = (int)(p - (long)n * H * W);
-    y = r / W;
-    x = r - y * W;
-  }
-  // A operand: this lane's output channel row
-  const uint16_t* wrow = w + (size_t)(co0 + col) * 9 * Cin + half * 8;
-  f32x16 acc;
-  for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-  for (int tap = 0; tap < 9; ++tap) {
-    int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-    bool v = pv && yy >= 0 && yy < H && xx >= 0 && xx < W;
-    const uint16_t* src = in + (((size_t)n * H + (v ? yy : 0)) * W + (v ? xx : 0)) * Cin + half * 8;
-    const uint16_t* wt = wrow + tap * Cin;
-    for (int c = 0; c < Cin; c += 16) {
-      bf16x8 a = load8(wt + c);
-      bf16x8 b = v ? load8(src + c) : zero8();
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+// The input of a conv: one NHWC tensor, or the channel concatenation of a half-resolution tensor
+// upsampled 2x by nearest neighbour (channels [0, Ca)) and a full-resolution one (channels [Ca, Ca + Cb)),
+// which fuses the decoder's upsample-and-concatenate into the conv's staging.
+struct ConvIn {
+  const uint16_t* a;
+  int Ca;
+  const uint16_t* b;  // null: the input is `a` alone
+  int Cb;
+  int upA;            // a is at half resolution
+};
+
+__device__ __forceinline__ bf16x8 lds8(const uint8_t* p) {
+  uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+// A workgroup of four wavefronts computes tiles of TH = 4 RPW output rows x 32 columns x 32 COT output
+// channels of one image, as an implicit GEMM on the matrix cores: C[co][pixel] += W[co][(tap, ci)] x
+// patch[(tap, ci)][pixel]. Input channels go in chunks of CIC: the chunk's (TH + 2) x 34 patch with its
+// halo (zeros outside the image, the conv's zero padding) and the tile's weights for the chunk are staged
+// in LDS; then each wavefront runs 9 taps x CIC / 16 k-steps of v_mfma_f32_32x32x16_bf16 on its RPW x COT
+// accumulators (RPW output rows, COT channel tiles), with operands read from LDS: a lane holds 8
+// consecutive k of one weight row (A) or one pixel (B). Staged pixels and weight rows are padded by 16
+// bytes, so the 16-byte reads of 16 consecutive lanes fall in distinct LDS banks.
+// The grid is persistent: a workgroup walks the (tile, chunk) steps of its jobs (job = tile + tiles
+// (n + N co-block), strided by the grid), and the 16-byte global loads of the next step are issued into
+// registers before the current step's matrix work, so their latency hides behind it. When the input has
+// one chunk, the weights stay in LDS from one job to the next of the same channel block.
+// Requirements: Cin % 16 == 0 (CIC divides Cin), Ca % 8 == 0, Cout % 32 == 0. block 256.
+template <int CIC, int COT, int RPW, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_conv3x3(ConvIn src, int N, int H, int W, const uint16_t* __restrict__ w,
+                                                    const float* __restrict__ bias, int Cout, int relu,
+                                                    uint16_t* __restrict__ out) {
+  constexpr int TH = 4 * RPW, TW = 32, PW = TW + 2, PH = TH + 2;
+  constexpr int PIX = CIC * 2 + 16;       // LDS bytes per staged pixel
+  constexpr int WROW = 9 * CIC * 2 + 16;  // LDS bytes per staged output channel
+  constexpr int VPP = CIC / 8;            // 16-byte vectors per pixel and chunk
+  constexpr int NVI = PH * PW * VPP;      // input vectors per step
+  constexpr int NVW = 32 * COT * 9 * VPP; // weight vectors per step
+  constexpr int NIN = (NVI + 255) / 256, NW = (NVW + 255) / 256;
+  __shared__ __attribute__((aligned(16))) uint8_t sIn[PH * PW * PIX];
+  __shared__ __attribute__((aligned(16))) uint8_t sW[32 * COT * WROW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, col = lane & 31, half = lane >> 5;
+  const int tilesX = (W + TW - 1) / TW, tiles = tilesX * ((H + TH - 1) / TH);
+  const int Cin = src.Ca + src.Cb, nChunks = Cin / CIC;
+  const long jobs = (long)tiles * N * ((Cout + 32 * COT - 1) / (32 * COT));
+  const int Ha = src.upA ? H >> 1 : H, Wa = src.upA ? W >> 1 : W;
+
+  long job = blockIdx.x;
+  if (job >= jobs) return;
+  int chunk = 0;
+  int tx0, ty0, n, co0;
+  auto decode = [&](long j, int& x0, int& y0, int& img, int& c0) {
+    const int tile = (int)(j % tiles);
+    const long rest = j / tiles;
+    img = (int)(rest % N);
+    c0 = (int)(rest / N) * 32 * COT;
+    x0 = (tile % tilesX) * TW;
+    y0 = (tile / tilesX) * TH;
+  };
+  uint4 pin[NIN], pw[NW];
+  auto load_in = [&](int x0, int y0, int img, int ci0) {
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) {
+      const int i = tid + 256 * k;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (i < NVI) {
+        const int pix = i / VPP, part = i - pix * VPP;
+        const int py = pix / PW, px = pix - py * PW;
+        const int gy = y0 + py - 1, gx = x0 + px - 1;
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+          const int c = ci0 + part * 8;
+          const uint16_t* q;
+          if (c < src.Ca) {
+            const int ay = src.upA ? gy >> 1 : gy, ax = src.upA ? gx >> 1 : gx;
+            q = src.a + (((size_t)img * Ha + ay) * Wa + ax) * src.Ca + c;
+          } else {
+            q = src.b + (((size_t)img * H + gy) * W + gx) * src.Cb + (c - src.Ca);
+          }
+          v = *reinterpret_cast<const uint4*>(q);
+        }
+      }
+      pin[k] = v;
     }
-  }
-  if (!pv) return;
-  // acc[r]: output channel co0 + 8 (r / 4) + 4 half + r % 4 of pixel p
-  uint16_t* dst = out + (size_t)p * Cout + co0 + 4 * half;
-  for (int g = 0; g < 4; ++g) {
-    uint16_t o[4];
-    for (int j = 0; j < 4; ++j) {
-      int co = co0 + 8 * g + 4 * half + j;
-      float v = acc[4 * g + j] + (bias ? bias[co] : 0.f);
-      if (relu && v < 0.f) v = 0.f;
-      o[j] = f2bf(v);
+  };
+  auto load_w = [&](int c0, int ci0) {
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const int i = tid + 256 * k;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (i < NVW) {
+        const int co = i / (9 * VPP), rem = i - co * (9 * VPP);
+        const int tap = rem / VPP, part = rem - tap * VPP;
+        if (c0 + co < Cout) v = *reinterpret_cast<const uint4*>(w + ((size_t)(c0 + co) * 9 + tap) * Cin + ci0 + part * 8);
+      }
+      pw[k] = v;
     }
-    uint2 pk = make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
-    *reinterpret_cast<uint2*>(dst + 8 * g) = pk;
+  };
+
+  decode(job, tx0, ty0, n, co0);
+  load_in(tx0, ty0, n, 0);
+  load_w(co0, 0);
+  bool stageW = true;
+  int wCo = co0;  // channel block of the weights last loaded
+  f32x16 acc[RPW][COT];
+  for (int r = 0; r < RPW; ++r)
+    for (int t = 0; t < COT; ++t)
+      for (int i = 0; i < 16; ++i) acc[r][t][i] = 0.f;
+  while (true) {
+    __syncthreads();  // every wavefront is done reading the previous step
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) {
+      const int i = tid + 256 * k;
+      if (i < NVI) {
+        const int pix = i / VPP, part = i - pix * VPP;
+        *reinterpret_cast<uint4*>(sIn + pix * PIX + part * 16) = pin[k];
+      }
+    }
+    if (stageW) {
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        const int i = tid + 256 * k;
+        if (i < NVW) {
+          const int co = i / (9 * VPP), rem = i - co * (9 * VPP);
+          const int tap = rem / VPP, part = rem - tap * VPP;
+          *reinterpret_cast<uint4*>(sW + co * WROW + tap * (CIC * 2) + part * 16) = pw[k];
+        }
+      }
+    }
+    __syncthreads();
+    // the next step's loads, in flight during this step's matrix work
+    long nJob = job;
+    int nChunk = chunk + 1;
+    if (nChunk == nChunks) {
+      nChunk = 0;
+      nJob = job + gridDim.x;
+    }
+    const bool more = nJob < jobs;
+    int nx0 = 0, ny0 = 0, nn = 0, nco = 0;
+    bool nStageW = false;
+    if (more) {
+      decode(nJob, nx0, ny0, nn, nco);
+      load_in(nx0, ny0, nn, nChunk * CIC);
+      nStageW = !(nChunks == 1 && nco == wCo);
+      if (nStageW) {
+        load_w(nco, nChunk * CIC);
+        wCo = nco;
+      }
+    }
+#pragma unroll 1
+    for (int tap = 0; tap < 9; ++tap) {
+      const int ky = tap / 3, kx = tap % 3;
+#pragma unroll
+      for (int kk = 0; kk < CIC / 16; ++kk) {
+        bf16x8 a[COT];
+#pragma unroll
+        for (int t = 0; t < COT; ++t) a[t] = lds8(sW + (t * 32 + col) * WROW + tap * (CIC * 2) + kk * 32 + half * 16);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+          const int row = wave * RPW + r;
+          bf16x8 b = lds8(sIn + ((row + ky) * PW + col + kx) * PIX + kk * 32 + half * 16);
+#pragma unroll
+          for (int t = 0; t < COT; ++t) acc[r][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[t], b, acc[r][t], 0, 0, 0);
+        }
+      }
+    }
+    if (chunk == nChunks - 1) {
+      // acc[r][t][i]: output channel co0 + 32 t + 8 (i / 4) + 4 half + i % 4 of pixel (row r of this wave, column col)
+      const int x = tx0 + col;
+      for (int r = 0; r < RPW; ++r) {
+        const int y = ty0 + wave * RPW + r;
+        if (y < H && x < W) {
+          uint16_t* dst = out + (((size_t)n * H + y) * W + x) * Cout;
+          for (int t = 0; t < COT; ++t) {
+            const int cb = co0 + 32 * t;
+            if (cb >= Cout) break;
+            for (int g = 0; g < 4; ++g) {
+              const int co = cb + 8 * g + 4 * half;
+              uint16_t o[4];
+              for (int j = 0; j < 4; ++j) {
+                float v = acc[r][t][4 * g + j] + (bias ? bias[co + j] : 0.f);
+                if (relu && v < 0.f) v = 0.f;
+                o[j] = f2bf(v);
+              }
+              *reinterpret_cast<uint2*>(dst + co) =
+                  make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+            }
+          }
+        }
+        for (int t = 0; t < COT; ++t)
+          for (int i = 0; i < 16; ++i) acc[r][t][i] = 0.f;
+      }
+    }
+    if (!more) break;
+    job = nJob;
+    chunk = nChunk;
+    tx0 = nx0;
+    ty0 = ny0;
+    n = nn;
+    co0 = nco;
+    stageW = nStageW;
   }
 }
 
@@ -319,6 +470,49 @@ int check_launch() {
   return YDN_OK;
 }
 
+// persistent grid: `occ` workgroups per CU (as many as their LDS and registers allow), at most one per job
+unsigned conv_grid(long jobs, int occ) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+  }
+  long g = (long)occ * cus;
+  return (unsigned)(jobs < g ? jobs : g);
+}
+
+template <int COT, int RPW, int OCC>
+void launch_conv_t(const ConvIn& src, int N, int H, int W, const uint16_t* w, const float* bias, int Cout, int relu,
+                   uint16_t* out, hipStream_t s) {
+  const long jobs = (long)((W + 31) / 32) * ((H + 4 * RPW - 1) / (4 * RPW)) * N * ((Cout + 32 * COT - 1) / (32 * COT));
+  dim3 grid(conv_grid(jobs, OCC));
+  if ((src.Ca + src.Cb) % 32 == 0)
+    hipLaunchKernelGGL((k_conv3x3<32, COT, RPW, OCC>), grid, dim3(256), 0, s, src, N, H, W, w, bias, Cout, relu, out);
+  else
+    hipLaunchKernelGGL((k_conv3x3<16, COT, RPW, OCC>), grid, dim3(256), 0, s, src, N, H, W, w, bias, Cout, relu, out);
+}
+
+// One 32-channel tile and two output rows per wavefront (8 x 32 pixels per workgroup), three workgroups
+// per CU: of the shapes measured on the 1080p U-Net forward pass (rows 2 to 8 per wavefront, one or two
+// channel tiles, one to three workgroups per CU), the one with the most workgroups in flight was
+// fastest, as each step's global loads wait on the memory system rather than on the matrix cores.
+int launch_conv(const ConvIn& src, int N, int H, int W, const uint16_t* w, const float* bias, int Cout, int relu,
+                uint16_t* out, hipStream_t s) {
+  launch_conv_t<1, 2, 3>(src, N, H, W, w, bias, Cout, relu, out, s);
+  return check_launch();
+}
+
+int check_conv(int N, int H, int W, int Cin, int Cout) {
+  if (N <= 0 || H <= 0 || W <= 0 || Cin <= 0 || Cout <= 0) return fail(YDN_E_INVALID, "empty shape");
+  if (N > 65535) return fail(YDN_E_INVALID, "more than 65535 images");
+  if (Cin % 16 != 0) return fail(YDN_E_INVALID, "Cin must be a multiple of 16");
+  if (Cout % 32 != 0) return fail(YDN_E_INVALID, "Cout must be a multiple of 32");
+  if ((long)((W + 31) / 32) * ((H + 7) / 8) > 0x7FFFFFFFL) return fail(YDN_E_INVALID, "image too large");
+  return YDN_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -328,18 +522,19 @@ const char* ydn_last_error(void) { return g_err.c_str(); }
 int ydn_conv3x3(const uint16_t* in, int N, int H, int W, int Cin, const uint16_t* w, const float* bias, int Cout, int relu,
                 uint16_t* out, void* stream) {
   if (!in || !w || !out) return fail(YDN_E_INVALID, "null tensor");
-  if (N <= 0 || H <= 0 || W <= 0 || Cin <= 0 || Cout <= 0) return fail(YDN_E_INVALID, "empty shape");
-  if (Cin % 16 != 0) return fail(YDN_E_INVALID, "Cin must be a multiple of 16");
-  if (Cout % 32 != 0) return fail(YDN_E_INVALID, "Cout must be a multiple of 32");
-  long P = (long)N * H * W;
-  hipStream_t s = (hipStream_t)stream;
-  if (Cout % 128 == 0)
-    hipLaunchKernelGGL(k_conv3x3<4>, dim3(blocks(P, 32), Cout / 128), dim3(256), 0, s, in, N, H, W, Cin, w, bias, Cout, relu, out);
-  else if (Cout % 64 == 0)
-    hipLaunchKernelGGL(k_conv3x3<2>, dim3(blocks(P, 32), Cout / 64), dim3(128), 0, s, in, N, H, W, Cin, w, bias, Cout, relu, out);
-  else
-    hipLaunchKernelGGL(k_conv3x3<1>, dim3(blocks(P, 32), Cout / 32), dim3(64), 0, s, in, N, H, W, Cin, w, bias, Cout, relu, out);
-  return check_launch();
+  if (int rc = check_conv(N, H, W, Cin, Cout)) return rc;
+  ConvIn src{in, Cin, nullptr, 0, 0};
+  return launch_conv(src, N, H, W, w, bias, Cout, relu, out, (hipStream_t)stream);
+}
+
+int ydn_conv3x3_up2cat(const uint16_t* coarse, int C1, const uint16_t* skip, int C2, int N, int H, int W, const uint16_t* w,
+                       const float* bias, int Cout, int relu, uint16_t* out, void* stream) {
+  if (!coarse || !skip || !w || !out) return fail(YDN_E_INVALID, "null tensor");
+  if (int rc = check_half_shape(N, H, W, C1)) return rc;
+  if (C2 <= 0 || C1 % 16 != 0 || C2 % 16 != 0) return fail(YDN_E_INVALID, "C1 and C2 must be positive multiples of 16");
+  if (int rc = check_conv(N, H, W, C1 + C2, Cout)) return rc;
+  ConvIn src{coarse, C1, skip, C2, 1};
+  return launch_conv(src, N, H, W, w, bias, Cout, relu, out, (hipStream_t)stream);
 }
 
 int ydn_conv3x3_wgrad(const uint16_t* in, const uint16_t* dout, int N, int H, int W, int Cin, int Cout, float* dw, float* db,

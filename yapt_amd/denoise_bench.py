@@ -157,8 +157,33 @@ def judge(net, W=1920, H=1080, ref_spp=4096, log=lambda m: None):
     return res
 
 
+def forward_roofline(net, W=1920, H=1080, reps=10, seed=0):
+    """the conv kernels' achieved TFLOP/s over one U-Net forward pass at W x H (padded to the network's
+    multiple), timed with HIP events on the library's stream after a warm-up pass, on a seeded input"""
+    import torch
+    from . import denoiser
+    m = net.multiple
+    Hp, Wp = (H + m - 1) // m * m, (W + m - 1) // m * m
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    x = denoiser.to_bf16_bits(torch.randn((1, Hp, Wp, denoiser.IN_CHANNELS), device="cuda", generator=gen) * 0.5)
+    net.forward(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        net.forward(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flops = conv_flops(net, Hp, Wp)
+    return {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": flops / (ms * 1e-3) / 1e12 / 2500.0, "forward_ms": ms, "flops": flops, "shape": [Hp, Wp]}
+
+
 def main():
     p = argparse.ArgumentParser()
+    p.add_argument("--forward-only", action="store_true",
+                   help="time the U-Net forward pass at the evaluation size and print its conv roofline")
     p.add_argument("--scenes", type=int, default=48)
     p.add_argument("--res", type=int, default=256)
     p.add_argument("--spp-ref", type=int, default=1024)
@@ -176,6 +201,10 @@ def main():
     from . import denoiser
 
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    if a.forward_only:
+        net = load_weights(a.load)
+        print(json.dumps(forward_roofline(net, a.eval_width, a.eval_height)))
+        return
     net = denoiser.UNet(seed=0)
     t_data = t_train = 0.0
     hist = []

@@ -42,6 +42,7 @@ def load():
     P, I, L, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_float
     lib.ydn_conv3x3.argtypes = [P, I, I, I, I, P, P, I, I, P, P]
     lib.ydn_conv3x3_wgrad.argtypes = [P, P, I, I, I, I, I, P, P, P]
+    lib.ydn_conv3x3_up2cat.argtypes = [P, I, P, I, I, I, I, P, P, I, I, P, P]
     lib.ydn_flip_weights.argtypes = [P, I, I, P, P]
     lib.ydn_relu_grad.argtypes = [P, P, L, P]
     lib.ydn_pool2.argtypes = [P, I, I, I, I, P, P]
@@ -116,6 +117,16 @@ class Conv:
         check(load().ydn_conv3x3(_p(x), N, H, W, C, _p(self.w16), _p(self.b), self.cout, int(self.relu), _p(y), _stream()))
         if keep:
             self.x, self.y = x, y
+        return y
+
+    def forward_up2cat(self, coarse, skip):
+        """the conv of [nearest 2x upsample of coarse, skip] without writing the concatenation (inference)"""
+        N, H, W, c2 = skip.shape
+        c1 = coarse.shape[3]
+        assert c1 + c2 == self.cin, (c1, c2, self.cin)
+        y = bf16_empty((N, H, W, self.cout))
+        check(load().ydn_conv3x3_up2cat(_p(coarse), c1, _p(skip), c2, N, H, W, _p(self.w16), _p(self.b), self.cout,
+                                        int(self.relu), _p(y), _stream()))
         return y
 
     def backward(self, g, need_input_grad=True):
@@ -195,11 +206,14 @@ class UNet:
             self.skips.append(h)
         for l in range(L - 2, -1, -1):
             skip = self.skips[l]
-            n_, hh, ww, c2 = skip.shape
-            c1 = h.shape[3]
-            cat = bf16_empty((n_, hh, ww, c1 + c2))
-            check(lib.ydn_up2_concat(_p(h), c1, _p(skip), c2, n_, hh, ww, _p(cat), s))
-            h = self.dec[l][0].forward(cat, keep)
+            if keep:    # training keeps the concatenation: the weight gradient reads it
+                n_, hh, ww, c2 = skip.shape
+                c1 = h.shape[3]
+                cat = bf16_empty((n_, hh, ww, c1 + c2))
+                check(lib.ydn_up2_concat(_p(h), c1, _p(skip), c2, n_, hh, ww, _p(cat), s))
+                h = self.dec[l][0].forward(cat, keep)
+            else:
+                h = self.dec[l][0].forward_up2cat(h, skip)
             h = self.dec[l][1].forward(h, keep)
         y = self.out.forward(h, keep)
         if not keep:
