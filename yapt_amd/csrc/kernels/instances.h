@@ -31,7 +31,14 @@ static constexpr int YPT_INST_GROUPS = 10;
   EXT __global__ void k_shadow_p<W, true, false, true>(YPT_TRAV_ARGS_SHADOW_P);                        \
   EXT __global__ void k_shadow_p<W, false, false, true>(YPT_TRAV_ARGS_SHADOW_P);                       \
   EXT __global__ void k_shadow_p<W, false, false, false>(YPT_TRAV_ARGS_SHADOW_P);                      \
-  EXT __global__ void k_trace_closest<W>(DevScene, const float*, uint32_t, float*, uint32_t*, float*, uint32_t*, uint32_t);
+  EXT __global__ void k_extend_w<W, true, true>(DevScene, PathSoA, Queues, int, uint32_t);                \
+  EXT __global__ void k_extend_w<W, false, true>(DevScene, PathSoA, Queues, int, uint32_t);               \
+  EXT __global__ void k_extend_w<W, false, false>(DevScene, PathSoA, Queues, int, uint32_t);              \
+  EXT __global__ void k_shadow_w<W, true, true>(DevScene, PathSoA, Queues, uint32_t);                     \
+  EXT __global__ void k_shadow_w<W, false, true>(DevScene, PathSoA, Queues, uint32_t);                    \
+  EXT __global__ void k_shadow_w<W, false, false>(DevScene, PathSoA, Queues, uint32_t);                   \
+  EXT __global__ void k_trace_closest<W>(DevScene, const float*, uint32_t, float*, uint32_t*, float*, uint32_t*, uint32_t);  \
+  EXT __global__ void k_trace_closest_w<W>(DevScene, const float*, uint32_t, float*, uint32_t*, float*, uint32_t*, uint32_t);
 
 #define YPT_SHADE_ARGS FrameConstants, DevScene, const uint32_t*, uint32_t, PathSoA, Queues, FilmBuffers, int, int
 #define YPT_INST_SHADE(EXT, NEE)                                      \

@@ -15,6 +15,7 @@
 #pragma once
 #include "../core/shade.h"
 #include "traverse.h"
+#include "traverse_wave.h"
 
 namespace ypt {
 
@@ -42,6 +43,7 @@ struct Queues {
   uint32_t ovfStride;
   unsigned long long* workCounters;  // [0] nodes, [1] tris (extension), [2] nodes, [3] tris (shadow)
   uint32_t* sssStack;  // subsurface probe traversal stacks of the shading launch, [depth][threads]
+  uint32_t waveBelow;  // ray and shadow queues shorter than this are traced wave-per-ray (k_extend_w, k_shadow_w)
 };
 
 struct FilmBuffers {
@@ -168,6 +170,7 @@ __global__ void k_extend(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t lds
   // counters[5]: fetch counter of a persistent shadow launch that may follow (subsurface scenes)
   if (i == 0) { Q.counters[qsel ^ 1] = 0; Q.counters[2] = 0; Q.counters[5] = 0; }
   uint32_t n = Q.counters[qsel];
+  if (n < Q.waveBelow) return;  // a short queue: k_extend_w traces it
   TraversalCounters cnt = {0u, 0u};
   if (i < n) {
     uint32_t slot = Q.ray[qsel][i];
@@ -280,6 +283,7 @@ __global__ void k_shadow(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth) {
   extern __shared__ uint32_t lds[];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t n = Q.counters[2];
+  if (n < Q.waveBelow) return;  // a short queue: k_shadow_w traces it
   TraversalCounters cnt = {0u, 0u};
   if (i < n) {
     float4 a = Q.sOrgTmax[i], b = Q.sDirTri[i];
@@ -369,6 +373,7 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
   extern __shared__ uint32_t lds[];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid == 0) { Q.counters[qsel ^ 1] = 0; Q.counters[2] = 0; Q.counters[5] = 0; }
+  if (Q.counters[qsel] < Q.waveBelow) return;  // a short queue: k_extend_w traces it
   WaveFetch wf;
   wf.init(&Q.counters[4], Q.counters[qsel], refill);
   TravStack st;
@@ -420,6 +425,7 @@ __global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, u
   extern __shared__ uint32_t lds[];
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   WaveFetch wf;
+  if (Q.counters[2] < Q.waveBelow) return;  // a short queue: k_shadow_w traces it
   wf.init(&Q.counters[5], Q.counters[2], refill);
   TravStack st;
   st.lds = lds; st.ovf = Q.ovf; st.block = blockDim.x; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
@@ -481,6 +487,99 @@ __global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, u
     unsigned long long nn = wave_sum(cnt.nodes), nt = wave_sum(cnt.tris), nr = wave_sum(nrays);
     if (__lane_id() == 0 && (nr | nt)) {
       atomicAdd(&Q.workCounters[2], nn); atomicAdd(&Q.workCounters[3], nt); atomicAdd(&Q.workCounters[5], nr);
+    }
+  }
+}
+
+// Wave-per-ray forms of the closest-hit and occlusion launches for short queues (fewer entries
+// than Q.waveBelow; the thread-per-ray launches of the same queue return at once and these take
+// it). One 64-lane workgroup per ray at a time; a wave takes its first ray by its workgroup index
+// and the next ones from the queue's fetch counter (counters[4] extension, counters[5] shadow).
+// LDS: the lanes' traversal stacks [ldsDepth][64], then the wave's 64-entry node list.
+template <int WIDTH, bool COUNT, bool ALPHA = true>
+__global__ __launch_bounds__(64) void k_extend_w(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t ldsDepth) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t n = Q.counters[qsel];
+  if (n >= Q.waveBelow) return;
+  TravStack st;
+  st.lds = lds; st.ovf = Q.ovf; st.block = 64; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
+  st.ovfStride = Q.ovfStride; st.gid = blockIdx.x * 64 + threadIdx.x; st.sp = 0;
+  uint32_t* front = lds + ldsDepth * 64;
+  const void* nodes = WIDTH == 2 ? (const void*)S.nodes2 : (const void*)S.nodes8;
+  TraversalCounters cnt = {0u, 0u};
+  unsigned long long nrays = 0;
+  uint32_t idx = blockIdx.x;
+  while (idx < n) {
+    uint32_t slot = Q.ray[qsel][idx];
+    float4 a = P.orgLambda[slot], b = P.dirPdf[slot];
+    RayTrav r;
+    trav_init(r, vec3(a.x, a.y, a.z), vec3(b.x, b.y, b.z), RAY_MIN_T, __builtin_inff(), NO_HIT_ID);
+    wave_trace<WIDTH, 0, COUNT, ALPHA>(nodes, S.tris, r, st, front, cnt);
+    uint32_t next = 0;
+    if (threadIdx.x == 0) {
+      Q.hit[slot] = make_float4(r.best, r.b1, r.b2, __uint_as_float(r.bid));
+      next = atomicAdd(&Q.counters[4], 1u);
+      if (COUNT) nrays++;
+    }
+    idx = gridDim.x + (uint32_t)__shfl((int)next, 0);
+  }
+  if (COUNT) {
+    unsigned long long nn = wave_sum(cnt.nodes), nt = wave_sum(cnt.tris);
+    if (threadIdx.x == 0 && (nrays | nt)) {
+      atomicAdd(&Q.workCounters[0], nn); atomicAdd(&Q.workCounters[1], nt); atomicAdd(&Q.workCounters[4], nrays);
+    }
+  }
+}
+
+template <int WIDTH, bool COUNT, bool ALPHA = true>
+__global__ __launch_bounds__(64) void k_shadow_w(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t n = Q.counters[2];
+  if (n >= Q.waveBelow) return;
+  TravStack st;
+  st.lds = lds; st.ovf = Q.ovf; st.block = 64; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
+  st.ovfStride = Q.ovfStride; st.gid = blockIdx.x * 64 + threadIdx.x; st.sp = 0;
+  uint32_t* front = lds + ldsDepth * 64;
+  const void* nodes = WIDTH == 2 ? (const void*)S.nodes2 : (const void*)S.nodes8;
+  TraversalCounters cnt = {0u, 0u};
+  unsigned long long nrays = 0;
+  uint32_t idx = blockIdx.x;
+  while (idx < n) {
+    float4 a = Q.sOrgTmax[idx], b = Q.sDirTri[idx];
+    vec3 o(a.x, a.y, a.z), d(b.x, b.y, b.z);
+    uint32_t target = __float_as_uint(b.w);
+    // the light triangle itself first, as in k_shadow_p (the same on every lane)
+    float tt = a.w, b1, b2;
+    bool reaches = true;
+    if (target != NO_HIT_ID) {
+      RayPre rp = ray_precompute(o, d);
+      vec3 p1, p2, p3;
+      tri_vertices(S.sv, target, p1, p2, p3);
+      reaches = intersect_triangle(rp, p1, p2, p3, RAY_MIN_T, a.w, tt, b1, b2) && !(ALPHA && alpha_cut(S.sv, target, b1, b2));
+    }
+    bool visible = false;
+    if (reaches) {
+      RayTrav r;
+      trav_init(r, o, d, RAY_MIN_T, tt, target);
+      visible = !wave_trace<WIDTH, 1, COUNT, ALPHA>(nodes, S.tris, r, st, front, cnt);
+    }
+    uint32_t next = 0;
+    if (threadIdx.x == 0) {
+      if (visible) {
+        uint32_t slot = Q.sSlot[idx];
+        float4 c = Q.sContrib[idx];
+        float4 l = P.light[slot];
+        P.light[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w + c.w);
+      }
+      next = atomicAdd(&Q.counters[5], 1u);
+      if (COUNT) nrays++;
+    }
+    idx = gridDim.x + (uint32_t)__shfl((int)next, 0);
+  }
+  if (COUNT) {
+    unsigned long long nn = wave_sum(cnt.nodes), nt = wave_sum(cnt.tris);
+    if (threadIdx.x == 0 && (nrays | nt)) {
+      atomicAdd(&Q.workCounters[2], nn); atomicAdd(&Q.workCounters[3], nt); atomicAdd(&Q.workCounters[5], nrays);
     }
   }
 }
@@ -639,6 +738,33 @@ __global__ void k_trace_closest(DevScene S, const float* rays, uint32_t n, float
   outTri[i] = tri;
   outB[2 * i] = tri == NO_HIT_ID ? 0.f : b1;
   outB[2 * i + 1] = tri == NO_HIT_ID ? 0.f : b2;
+}
+
+// the wave-per-ray form of k_trace_closest: workgroups of 64 lanes take rays i = block, block +
+// grid, ...; overflow stacks [depth][grid * 64]
+template <int WIDTH>
+__global__ __launch_bounds__(64) void k_trace_closest_w(DevScene S, const float* rays, uint32_t n, float* outT, uint32_t* outTri,
+                                                        float* outB, uint32_t* ovf, uint32_t ldsDepth) {
+  extern __shared__ uint32_t lds[];
+  TravStack st;
+  st.lds = lds; st.ovf = ovf; st.block = 64; st.lane = threadIdx.x; st.ldsDepth = ldsDepth;
+  st.ovfStride = gridDim.x * 64; st.gid = blockIdx.x * 64 + threadIdx.x; st.sp = 0;
+  uint32_t* front = lds + ldsDepth * 64;
+  const void* nodes = WIDTH == 2 ? (const void*)S.nodes2 : (const void*)S.nodes8;
+  TraversalCounters cnt = {0u, 0u};
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const float* r = rays + (size_t)i * 8;
+    RayTrav rt;
+    trav_init(rt, vec3(r[0], r[1], r[2]), vec3(r[4], r[5], r[6]), r[3], r[7], NO_HIT_ID);
+    wave_trace<WIDTH, 0, false, true>(nodes, S.tris, rt, st, front, cnt);
+    if (threadIdx.x == 0) {
+      bool hit = rt.bid != NO_HIT_ID;
+      outT[i] = hit ? rt.best : 0.f;
+      outTri[i] = rt.bid;
+      outB[2 * i] = hit ? rt.b1 : 0.f;
+      outB[2 * i + 1] = hit ? rt.b2 : 0.f;
+    }
+  }
 }
 
 #ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit

@@ -259,6 +259,7 @@ struct ypt_integrator {
   int shadeWaves = 64;     // waves per CU of the shading launch (0: one workgroup per queue entry)
   int shadeRegWaves = 4;   // register budget of the shading kernel: 2, 4 or 8 waves per SIMD
   int bdptWaves = 4;       // register budget of the camera-subpath kernel: waves per SIMD (1, 2, 3, 4, 6, 8)
+  int waveBelow = 0;       // unidirectional bounces: ray and shadow queues shorter than this are traced wave-per-ray
   int cus = 256;           // compute units of the device
 
   // stats of the last render
@@ -505,8 +506,35 @@ uint32_t resident_grid(ypt_integrator* h, K kernel, uint32_t block, size_t lds, 
   hipLaunchKernelGGL(KERNEL, dim3(resident_grid(h, KERNEL, (BLOCK), (LDS), (GRID))), dim3(BLOCK), (LDS), \
                      (STREAM), __VA_ARGS__)
 
+// wave-per-ray launches (k_extend_w, k_shadow_w): one 64-lane workgroup per ray at a time, as many
+// as the overflow stacks have columns for, at most 32 per CU; LDS holds the lanes' stacks and the
+// wave's node list
+uint32_t wave_grid(const ypt_integrator* h) {
+  return std::max<uint32_t>(1u, std::min<uint32_t>(h->ovfStride / 64u, (uint32_t)h->cus * 32u));
+}
+size_t wave_lds(const ypt_integrator* h) { return ((size_t)h->ldsDepth * 64 + 64) * 4; }
+
 template <int WIDTH>
-void launch_extend(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, int q, uint32_t grid) {
+void launch_extend_wave(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, int q) {
+  if (Q.waveBelow == 0) return;
+  uint32_t d = (uint32_t)h->ldsDepth, g = wave_grid(h);
+  size_t lds = wave_lds(h);
+  if (h->countTraversal) YPT_PERSISTENT((k_extend_w<WIDTH, true, true>), g, 64, lds, h->stream, S, P, Q, q, d);
+  else if (S.tris.alpha.tris != nullptr) YPT_PERSISTENT((k_extend_w<WIDTH, false, true>), g, 64, lds, h->stream, S, P, Q, q, d);
+  else YPT_PERSISTENT((k_extend_w<WIDTH, false, false>), g, 64, lds, h->stream, S, P, Q, q, d);
+}
+template <int WIDTH>
+void launch_shadow_wave(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q) {
+  if (Q.waveBelow == 0) return;
+  uint32_t d = (uint32_t)h->ldsDepth, g = wave_grid(h);
+  size_t lds = wave_lds(h);
+  if (h->countTraversal) YPT_PERSISTENT((k_shadow_w<WIDTH, true, true>), g, 64, lds, h->stream, S, P, Q, d);
+  else if (S.tris.alpha.tris != nullptr) YPT_PERSISTENT((k_shadow_w<WIDTH, false, true>), g, 64, lds, h->stream, S, P, Q, d);
+  else YPT_PERSISTENT((k_shadow_w<WIDTH, false, false>), g, 64, lds, h->stream, S, P, Q, d);
+}
+
+template <int WIDTH>
+void launch_extend_thread(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, int q, uint32_t grid) {
   size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
   uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
   if (h->persistent) {
@@ -524,7 +552,7 @@ void launch_extend(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
   }
 }
 template <int WIDTH>
-void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, uint32_t grid) {
+void launch_shadow_thread(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, uint32_t grid) {
   size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
   uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill;
   if (h->res->subsurface) {  // two rays may add to one path: the atomic form, persistent at any grid
@@ -547,6 +575,19 @@ void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const
   }
 }
 
+
+// a bounce's closest-hit (occlusion) launch: the thread-per-ray kernel, then, when short queues go
+// wave-per-ray, the wave kernel; the one whose queue-length range does not hold the queue returns
+template <int WIDTH>
+void launch_extend(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, int q, uint32_t grid) {
+  launch_extend_thread<WIDTH>(h, S, P, Q, q, grid);
+  launch_extend_wave<WIDTH>(h, S, P, Q, q);
+}
+template <int WIDTH>
+void launch_shadow(ypt_integrator* h, const DevScene& S, const PathSoA& P, const Queues& Q, uint32_t grid) {
+  launch_shadow_thread<WIDTH>(h, S, P, Q, grid);
+  launch_shadow_wave<WIDTH>(h, S, P, Q);
+}
 
 
 // All launches of one batch of g passes for the unidirectional integrators, reading the Sobol
@@ -1087,6 +1128,9 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
   } else if (n == "refill_threshold") {
     if (value < 1 || value > 64) return fail(YPT_E_INVALID, "refill_threshold must be in [1, 64] (lanes of a wave)");
     h->refill = (int)value;
+  } else if (n == "wave_ray_below") {
+    if (value < 0 || value > (1 << 30)) return fail(YPT_E_INVALID, "wave_ray_below must be in [0, 2^30] (0: thread-per-ray only)");
+    h->waveBelow = (int)value;
   } else {
     return fail(YPT_E_INVALID, "unknown option " + n);
   }
@@ -1165,6 +1209,9 @@ int ypt_render(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count, 
     DevScene S = h->dev_scene();
     PathSoA P = h->path_soa();
     Queues Q = h->queues();
+    // short bounce queues wave-per-ray (not with subsurface scenes, whose shadow launch adds two rays
+    // per path atomically)
+    Q.waveBelow = h->res->subsurface ? 0u : (uint32_t)h->waveBelow;
     FilmBuffers F = h->film();
     uint32_t B = (uint32_t)h->blockSize;
     // profile mode: events on this stream around every kernel, read once the render has finished
@@ -1442,13 +1489,25 @@ int ypt_trace_closest(ypt_integrator* h, const float* rays, uint32_t n, float* t
     dt.alloc(n); db.alloc(2 * (size_t)n); dtri.alloc(n);
     uint32_t bound = h->bvhWidth == 8 ? h->res->stackBound8 : h->res->stackBound2;
     size_t depth = bound > (uint32_t)h->ldsDepth ? bound - (uint32_t)h->ldsDepth : 1;
-    dovf.alloc(depth * n);
     DevScene S = h->dev_scene();
     size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
-    if (h->bvhWidth == 8)
-      hipLaunchKernelGGL(k_trace_closest<8>, dim3(blocks_for(n, h->blockSize)), dim3(h->blockSize), lds, s, S, dr.p, n, dt.p, dtri.p, db.p, dovf.p, (uint32_t)h->ldsDepth);
-    else
-      hipLaunchKernelGGL(k_trace_closest<2>, dim3(blocks_for(n, h->blockSize)), dim3(h->blockSize), lds, s, S, dr.p, n, dt.p, dtri.p, db.p, dovf.p, (uint32_t)h->ldsDepth);
+    uint32_t ld = (uint32_t)h->ldsDepth;
+    if (h->waveBelow > 0 && n < (uint32_t)h->waveBelow) {  // wave-per-ray: one 64-lane workgroup per ray
+      uint32_t g = std::min<uint32_t>(n, (uint32_t)h->cus * 16u);
+      dovf.alloc(depth * g * 64);
+      size_t wl = wave_lds(h);
+      if (h->bvhWidth == 8)
+        hipLaunchKernelGGL(k_trace_closest_w<8>, dim3(g), dim3(64), wl, s, S, dr.p, n, dt.p, dtri.p, db.p, dovf.p, ld);
+      else
+        hipLaunchKernelGGL(k_trace_closest_w<2>, dim3(g), dim3(64), wl, s, S, dr.p, n, dt.p, dtri.p, db.p, dovf.p, ld);
+    } else {
+      dovf.alloc(depth * n);
+      uint32_t g = blocks_for(n, h->blockSize);
+      if (h->bvhWidth == 8)
+        hipLaunchKernelGGL(k_trace_closest<8>, dim3(g), dim3(h->blockSize), lds, s, S, dr.p, n, dt.p, dtri.p, db.p, dovf.p, ld);
+      else
+        hipLaunchKernelGGL(k_trace_closest<2>, dim3(g), dim3(h->blockSize), lds, s, S, dr.p, n, dt.p, dtri.p, db.p, dovf.p, ld);
+    }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipMemcpy(t, dt.p, n * sizeof(float), hipMemcpyDeviceToHost));

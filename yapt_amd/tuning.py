@@ -2,7 +2,8 @@
 
 The autotuner searches BVH width (2 vs 8), threads per block, LDS traversal-stack depth, passes in
 flight (queue depth), persistent vs one-workgroup-per-ray launches with their waves per CU and refill
-threshold, and the shading launch size for a workload, timing whole passes, and records the fastest
+threshold, the queue length below which rays are traced wave-per-ray instead of thread-per-ray, and
+the shading launch size for a workload, timing whole passes, and records the fastest
 configuration in ``tuned.json`` next to this file. ``best_config`` returns the stored choice (or
 defaults).
 """
@@ -15,7 +16,8 @@ TUNED = os.path.join(HERE, "tuned.json")
 ROOT = os.path.dirname(HERE)
 
 DEFAULT = {"bvh_width": 2, "block_size": 64, "lds_stack_depth": 16, "passes_in_flight": 0, "persistent": 1,
-           "persistent_waves": 32, "refill_threshold": 16, "leaf_share": 2, "shade_waves": 64, "shade_reg_waves": 4}
+           "persistent_waves": 32, "refill_threshold": 16, "leaf_share": 2, "shade_waves": 64, "shade_reg_waves": 4,
+           "wave_ray_below": 0}
 
 
 def _load(path=None):
@@ -56,6 +58,7 @@ SPACE = {
     "leaf_share": (1, 2, 3, 4),
     "shade_waves": (0, 16, 32, 64, 128),
     "shade_reg_waves": (2, 4, 8),
+    "wave_ray_below": (0, 2048, 8192, 32768),
 }
 
 SPACE_BDPT = {
