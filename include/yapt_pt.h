@@ -315,9 +315,14 @@ int ypt_set_resident_scene(ypt_integrator* h, ypt_scene* scene);
 /* ---- Render server: continuous batching of progressive render jobs (no counterpart in the
  * reference, whose renderer renders one view; the jobs run the same integrators).
  * Jobs are admitted in arrival order while at most max_active_jobs run and their estimated device
- * memory stays within hbm_budget_bytes; each active job keeps its next chunk of passes in flight on
- * its own stream, so jobs overlap on the device. A job's film depends only on its scene, frame and
- * sample count (Sobol indices 0 .. spp-1), alone or in any mix. */
+ * memory stays within hbm_budget_bytes. BPT and NEE jobs (scenes without subsurface materials) run
+ * their chunks of passes in fused steps: one launch sequence whose queues carry the paths of up to 16
+ * jobs and their scenes together. BDPT jobs keep their next chunk in flight on their own stream,
+ * beside the fused steps. A job's film depends only on its scene, frame and sample count (Sobol
+ * indices 0 .. spp-1), alone or in any mix, fused or not.
+ * Server settings through ypt_server_set_option (integrator type ignored): "fuse_jobs" (1 default,
+ * 0: every job on its own stream) and "fuse_max_slots" (path slots of one step, 16M default; a larger
+ * single chunk runs alone). */
 typedef struct ypt_server ypt_server;
 int ypt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_active_jobs, ypt_server** out);
 int ypt_server_destroy(ypt_server* server);
@@ -341,6 +346,9 @@ int ypt_server_set_deadline(ypt_server* server, uint64_t job_id, double seconds)
 int ypt_server_take(ypt_server* server, uint64_t job_id, float* xyzw, double* times3);
 /* [0] queued, [1] active, [2] completed, [3] device bytes of active jobs, [4] peak active, [5] budget */
 int ypt_server_stats(ypt_server* server, double* out6);
+/* fused steps issued so far and the job chunks they carried (their ratio is the mean number of jobs
+ * per step) */
+int ypt_server_fused_stats(ypt_server* server, uint64_t* steps, uint64_t* job_chunks);
 /* the steady clock of the job times, in seconds */
 double ypt_clock(void);
 

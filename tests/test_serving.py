@@ -29,12 +29,14 @@ def test_percentiles():
 
 
 @pytest.mark.gpu
-def test_job_result_is_the_same_alone_and_in_the_mix(ylib, cornell, shapes_small):
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_job_result_is_the_same_alone_and_in_the_mix(ylib, cornell, shapes_small, fuse):
     from yapt_amd import camera
     sc_a = ylib.ResidentScene(cornell[0])
     sc_b = ylib.ResidentScene(shapes_small[0])
     fr_a = camera.make_frame(cornell[1], 64, 48, max_depth=8)
     srv = ylib.Server(0, 8 << 30, 4)
+    srv.set_option("fuse_jobs", fuse)
     try:
         ja = srv.submit(sc_a, ylib.YPT_INTEGRATOR_NEE, fr_a, 8)
         srv.wait(ja)
@@ -66,6 +68,62 @@ def test_job_result_is_the_same_alone_and_in_the_mix(ylib, cornell, shapes_small
         srv.close()
         sc_a.close()
         sc_b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("max_slots", [16 << 20, 20000])
+def test_fused_steps_give_each_job_its_own_film(ylib, cornell, shapes_small, max_slots):
+    # BPT and NEE jobs over four scenes (an alpha-tested one, punctual lights with BPT shadow rays)
+    # traced together in fused steps; every film equals a plain handle's render bit for bit. A small
+    # slot limit splits the steps (a chunk larger than the limit runs alone).
+    from yapt_amd import camera, scenes
+    from test_textures import _alpha_render_scene
+    alpha_scene, alpha_cam = _alpha_render_scene()
+    punct, pcam, env = scenes.punctual_lights("mixed", sphere_res=(12, 12))
+    scs = [(cornell[0], cornell[1], 0.0), (shapes_small[0], shapes_small[1], 0.0), (alpha_scene, alpha_cam, 0.0),
+           (punct, pcam, env)]
+    res = [ylib.ResidentScene(s) for s, _, _ in scs]
+    spec = [(0, 64, 48, 1, 4, 8), (1, 96, 64, 0, 3, 12), (2, 48, 40, 1, 5, 5), (3, 72, 44, 0, 2, 6),
+            (3, 40, 40, 1, 6, 6), (1, 128, 72, 1, 2, 12), (0, 32, 32, 0, 9, 8), (2, 80, 48, 0, 2, 5),
+            (1, 56, 56, 1, 17, 10), (0, 100, 60, 1, 1, 3)]
+    srv = ylib.Server(0, 16 << 30, 12)
+    srv.set_option("fuse_max_slots", max_slots)
+    try:
+        frames = [camera.make_frame(scs[k][1], w, h, max_depth=d, env_intensity_scale=scs[k][2]) for k, w, h, _, _, d in spec]
+        ids = [srv.submit(res[k], integ, frames[i], spp) for i, (k, w, h, integ, spp, d) in enumerate(spec)]
+        films = []
+        for j in ids:
+            srv.wait(j)
+            films.append(srv.take(j)[0])
+        fs = srv.fused_stats()
+        assert fs["steps"] > 0
+        if max_slots > 1 << 20:
+            assert fs["job_chunks"] > fs["steps"]      # steps carried several jobs
+        for i, (k, w, h, integ, spp, d) in enumerate(spec):
+            it = ylib.Integrator(integ, 0)
+            it.set_scene(res[k])
+            it.set_frame(frames[i])
+            it.render(0, spp)
+            direct = it.film()
+            it.close()
+            assert films[i][..., 1].max() > 0, i
+            assert np.array_equal(films[i].view(np.uint32), direct.view(np.uint32)), (i, spec[i])
+    finally:
+        srv.close()
+        for r in res:
+            r.close()
+
+
+@pytest.mark.gpu
+def test_fuse_settings_are_validated(ylib):
+    srv = ylib.Server(0, 1 << 30, 2)
+    try:
+        with pytest.raises(Exception):
+            srv.set_option("fuse_max_slots", 0)
+        srv.set_option("fuse_jobs", 0)
+        assert srv.fused_stats() == {"steps": 0, "job_chunks": 0}
+    finally:
+        srv.close()
 
 
 @pytest.mark.gpu

@@ -313,6 +313,7 @@ def load():
     lib.ypt_server_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     lib.ypt_server_cancel.argtypes = [P, ctypes.c_uint64]
     lib.ypt_server_set_deadline.argtypes = [P, ctypes.c_uint64, ctypes.c_double]
+    lib.ypt_server_fused_stats.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
     lib.ypt_clock.restype = ctypes.c_double
     _LIB = lib
     return lib
@@ -326,7 +327,7 @@ EXPORTED_SYMBOLS = [
     "ypt_server_create", "ypt_server_destroy", "ypt_server_submit", "ypt_server_wait", "ypt_server_take",
     "ypt_server_stats", "ypt_clock", "ypt_server_set_option", "ypt_final_color", "ypt_post_params_default",
     "ypt_get_feature_moments", "ypt_get_film_variance", "ypt_trace_closest_noalloc", "ypt_server_cancel",
-    "ypt_server_set_deadline",
+    "ypt_server_set_deadline", "ypt_server_fused_stats",
 ]
 
 
@@ -521,6 +522,12 @@ class Server:
         check(self.lib.ypt_server_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return {"queued": int(out[0]), "active": int(out[1]), "completed": int(out[2]), "reserved_bytes": out[3],
                 "peak_active": int(out[4]), "budget_bytes": out[5]}
+
+    def fused_stats(self):
+        """fused steps issued so far and the job chunks they carried"""
+        steps, chunks = ctypes.c_uint64(), ctypes.c_uint64()
+        check(self.lib.ypt_server_fused_stats(self.h, ctypes.byref(steps), ctypes.byref(chunks)))
+        return {"steps": steps.value, "job_chunks": chunks.value}
 
     def clock(self):
         return self.lib.ypt_clock()

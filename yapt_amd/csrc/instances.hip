@@ -30,6 +30,8 @@ YPT_INST_BDPT_CAMERA(template, 8, false)
 YPT_INST_BDPT_CAMERA(template, 8, true)
 #elif YPT_INST_GROUP == 9
 YPT_INST_BDPT_CSHADE(template)
+#elif YPT_INST_GROUP == 10
+YPT_INST_FUSED(template)
 #else
 #error "no such instance group"
 #endif

@@ -5,12 +5,13 @@
 // non-template kernels.
 #pragma once
 #include "bdpt_kernels.h"
+#include "multijob.h"
 #include "wavefront.h"
 
 namespace ypt {
 
 // instance groups compiled separately (instances.hip)
-static constexpr int YPT_INST_GROUPS = 10;
+static constexpr int YPT_INST_GROUPS = 11;
 
 #define YPT_TRAV_ARGS_EXT DevScene, PathSoA, Queues, int, uint32_t
 #define YPT_TRAV_ARGS_SHADOW DevScene, PathSoA, Queues, uint32_t
@@ -75,10 +76,35 @@ static constexpr int YPT_INST_GROUPS = 10;
   EXT __global__ void k_bdpt_cshade<4, false>(YPT_CSHADE_ARGS);        \
   EXT __global__ void k_bdpt_cshade<1, true>(YPT_CSHADE_ARGS);
 
+// kernels of the server's fused steps (several jobs per launch)
+#define YPT_FUSED_TRAV_ARGS FusedArgs, PathSoA, Queues, int, uint32_t, uint32_t, uint32_t
+#define YPT_FUSED_SHADOW_ARGS FusedArgs, PathSoA, Queues, uint32_t, uint32_t, uint32_t
+#define YPT_INST_FUSED(EXT)                                             \
+  EXT __global__ void k_extend_pm<2, true>(YPT_FUSED_TRAV_ARGS);        \
+  EXT __global__ void k_extend_pm<2, false>(YPT_FUSED_TRAV_ARGS);       \
+  EXT __global__ void k_extend_pm<8, true>(YPT_FUSED_TRAV_ARGS);        \
+  EXT __global__ void k_extend_pm<8, false>(YPT_FUSED_TRAV_ARGS);       \
+  EXT __global__ void k_shadow_pm<2, true>(YPT_FUSED_SHADOW_ARGS);      \
+  EXT __global__ void k_shadow_pm<2, false>(YPT_FUSED_SHADOW_ARGS);     \
+  EXT __global__ void k_shadow_pm<8, true>(YPT_FUSED_SHADOW_ARGS);      \
+  EXT __global__ void k_shadow_pm<8, false>(YPT_FUSED_SHADOW_ARGS);     \
+  EXT __global__ void k_shade_m<2>(FusedArgs, PathSoA, Queues, int);    \
+  EXT __global__ void k_shade_m<4>(FusedArgs, PathSoA, Queues, int);    \
+  EXT __global__ void k_shade_m<8>(FusedArgs, PathSoA, Queues, int);    \
+  EXT __global__ void k_extend_wm<2, true>(FusedArgs, PathSoA, Queues, int, uint32_t);   \
+  EXT __global__ void k_extend_wm<2, false>(FusedArgs, PathSoA, Queues, int, uint32_t);  \
+  EXT __global__ void k_extend_wm<8, true>(FusedArgs, PathSoA, Queues, int, uint32_t);   \
+  EXT __global__ void k_extend_wm<8, false>(FusedArgs, PathSoA, Queues, int, uint32_t);  \
+  EXT __global__ void k_shadow_wm<2, true>(FusedArgs, PathSoA, Queues, uint32_t);        \
+  EXT __global__ void k_shadow_wm<2, false>(FusedArgs, PathSoA, Queues, uint32_t);       \
+  EXT __global__ void k_shadow_wm<8, true>(FusedArgs, PathSoA, Queues, uint32_t);        \
+  EXT __global__ void k_shadow_wm<8, false>(FusedArgs, PathSoA, Queues, uint32_t);
+
 #define YPT_INST_ALL(EXT)                                                                          \
   YPT_INST_TRAV(EXT, 2) YPT_INST_TRAV(EXT, 8) YPT_INST_SHADE(EXT, true) YPT_INST_SHADE(EXT, false) \
   YPT_INST_BDPT_LIGHT(EXT, 2) YPT_INST_BDPT_LIGHT(EXT, 8)                                          \
   YPT_INST_BDPT_CAMERA(EXT, 2, false) YPT_INST_BDPT_CAMERA(EXT, 2, true)                           \
-  YPT_INST_BDPT_CAMERA(EXT, 8, false) YPT_INST_BDPT_CAMERA(EXT, 8, true) YPT_INST_BDPT_CSHADE(EXT)
+  YPT_INST_BDPT_CAMERA(EXT, 8, false) YPT_INST_BDPT_CAMERA(EXT, 8, true) YPT_INST_BDPT_CSHADE(EXT)     \
+  YPT_INST_FUSED(EXT)
 
 }  // namespace ypt

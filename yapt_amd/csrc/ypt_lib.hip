@@ -1656,6 +1656,7 @@ struct ServerJob {
   ypt_integrator* h = nullptr;
   hipEvent_t ev = nullptr;
   bool inFlight = false;
+  bool fused = false;  // its chunks run in the server's fused steps (not on its own stream)
   std::atomic<int> state{0};  // JOB_QUEUED, JOB_ACTIVE, JOB_DONE, JOB_FAILED, JOB_CANCELLED, JOB_TIMED_OUT
   std::atomic<bool> cancel{false};
   std::atomic<double> deadline{1e300};  // ypt_clock time after which the job is given up
@@ -1687,6 +1688,65 @@ void plan_job(ServerJob& j, int cus) {
   j.mem = bytes;
 }
 
+// One fused step's bounce loop on pool handle h (its stream, path-state pool, queues and launch
+// options): camera rays of every slot, then per bounce the closest-hit, shading and (when a job
+// traces them) occlusion launches over the queues that all jobs share.
+void launch_fused(ypt_integrator* h, const FusedArgs& A, uint32_t depth, bool alpha, bool shadow) {
+  hipStream_t s = h->stream;
+  PathSoA P = h->path_soa();
+  Queues Q = h->queues();
+  Q.waveBelow = (uint32_t)h->waveBelow;
+  uint32_t wg = wave_grid(h);
+  size_t wl = wave_lds(h);
+  uint32_t total = A.slotBase[A.njobs];
+  uint32_t grid = work_grid(h, total), sgrid = shade_grid(h, total);
+  size_t lds = (size_t)h->ldsDepth * h->blockSize * 4;
+  uint32_t d = (uint32_t)h->ldsDepth, r = (uint32_t)h->refill, ls = (uint32_t)h->leafShare;
+  uint32_t B = (uint32_t)h->blockSize;
+  hipLaunchKernelGGL(k_start_m, dim3(blocks_for(total, 256)), dim3(256), 0, s, A, P, Q);
+  int q = 0;
+  for (uint32_t b = 0; b < depth; ++b) {
+    if (h->bvhWidth == 8) {
+      if (alpha) YPT_PERSISTENT((k_extend_pm<8, true>), grid, B, lds, s, A, P, Q, q, d, r, ls);
+      else YPT_PERSISTENT((k_extend_pm<8, false>), grid, B, lds, s, A, P, Q, q, d, r, ls);
+    } else {
+      if (alpha) YPT_PERSISTENT((k_extend_pm<2, true>), grid, B, lds, s, A, P, Q, q, d, r, ls);
+      else YPT_PERSISTENT((k_extend_pm<2, false>), grid, B, lds, s, A, P, Q, q, d, r, ls);
+    }
+    if (Q.waveBelow) {
+      if (h->bvhWidth == 8) {
+        if (alpha) YPT_PERSISTENT((k_extend_wm<8, true>), wg, 64, wl, s, A, P, Q, q, d);
+        else YPT_PERSISTENT((k_extend_wm<8, false>), wg, 64, wl, s, A, P, Q, q, d);
+      } else {
+        if (alpha) YPT_PERSISTENT((k_extend_wm<2, true>), wg, 64, wl, s, A, P, Q, q, d);
+        else YPT_PERSISTENT((k_extend_wm<2, false>), wg, 64, wl, s, A, P, Q, q, d);
+      }
+    }
+    if (h->shadeRegWaves == 2) hipLaunchKernelGGL((k_shade_m<2>), dim3(sgrid), dim3(64), 0, s, A, P, Q, q);
+    else if (h->shadeRegWaves == 8) hipLaunchKernelGGL((k_shade_m<8>), dim3(sgrid), dim3(64), 0, s, A, P, Q, q);
+    else hipLaunchKernelGGL((k_shade_m<4>), dim3(sgrid), dim3(64), 0, s, A, P, Q, q);
+    if (shadow) {
+      if (h->bvhWidth == 8) {
+        if (alpha) YPT_PERSISTENT((k_shadow_pm<8, true>), grid, B, lds, s, A, P, Q, d, r, ls);
+        else YPT_PERSISTENT((k_shadow_pm<8, false>), grid, B, lds, s, A, P, Q, d, r, ls);
+      } else {
+        if (alpha) YPT_PERSISTENT((k_shadow_pm<2, true>), grid, B, lds, s, A, P, Q, d, r, ls);
+        else YPT_PERSISTENT((k_shadow_pm<2, false>), grid, B, lds, s, A, P, Q, d, r, ls);
+      }
+      if (Q.waveBelow) {
+        if (h->bvhWidth == 8) {
+          if (alpha) YPT_PERSISTENT((k_shadow_wm<8, true>), wg, 64, wl, s, A, P, Q, d);
+          else YPT_PERSISTENT((k_shadow_wm<8, false>), wg, 64, wl, s, A, P, Q, d);
+        } else {
+          if (alpha) YPT_PERSISTENT((k_shadow_wm<2, true>), wg, 64, wl, s, A, P, Q, d);
+          else YPT_PERSISTENT((k_shadow_wm<2, false>), wg, 64, wl, s, A, P, Q, d);
+        }
+      }
+    }
+    q ^= 1;
+  }
+}
+
 }  // namespace
 
 struct ypt_server {
@@ -1704,6 +1764,26 @@ struct ypt_server {
   uint64_t nextId = 1, memUsed = 0, completed = 0, peakActive = 0;
   bool stop = false;
   std::thread worker;
+
+  // Fused steps (kernels/multijob.h): the unidirectional jobs whose scenes have no subsurface
+  // materials run their chunks together, one launch sequence per step on a pool handle's stream,
+  // at most FUSED_MAX_JOBS jobs and fuseMaxSlots path slots per step (a larger single chunk runs
+  // alone). Two pools: a step can start while the other is still on the device, so a job that
+  // arrives during a long step does not wait for it. BDPT jobs keep their own streams beside them.
+  bool fuse = true;
+  uint64_t fuseMaxSlots = 16ull << 20;
+  struct Fused {
+    ypt_integrator* x = nullptr;  // pool: path state, queues, stream and launch options of a step
+    DevBuf<FusedJob> jobs;
+    DevBuf<uint32_t> points;
+    std::vector<FusedJob> hJobs;  // host copies of the job table and Sobol points, kept until the step ends
+    std::vector<uint32_t> hPoints;
+    std::vector<std::pair<std::string, int64_t>> applied;  // launch options set on the pool handle
+    hipEvent_t done = nullptr;
+    bool inFlight = false;
+    std::vector<std::shared_ptr<ServerJob>> members;
+  } fx[2];
+  std::atomic<uint64_t> fusedSteps{0}, fusedJobChunks{0};  // read by ypt_server_fused_stats from other threads
 
   ypt_integrator* take_handle(int type) {
     if (!pool[type].empty()) {
@@ -1726,6 +1806,114 @@ struct ypt_server {
     if (ypt_set_frame(h, &j.frame) != YPT_OK) throw std::runtime_error(std::string("server: ") + ypt_last_error());
     if (ypt_set_option(h, "passes_in_flight", j.passesInFlight) != YPT_OK) throw std::runtime_error(ypt_last_error());
     clear_film(h);
+    // a fused job's merges run on the pool's stream: its film is cleared before the first step
+    if (j.fused) HIP_CHECK(hipStreamSynchronize(h->stream));
+  }
+
+  // Issues one fused step over the jobs of `ready` (admission order) that fit the step, each for
+  // its next chunk of passes; film merges per job in pass order after the bounce loop.
+  void issue_fused(Fused& fx, const std::vector<std::shared_ptr<ServerJob>>& ready, uint64_t maxSlots,
+                   const std::vector<std::pair<std::string, int64_t>>& opts) {
+    if (!fx.x) {
+      if (ypt_create(YPT_INTEGRATOR_NEE, device, &fx.x) != YPT_OK) throw std::runtime_error(std::string("server: ") + ypt_last_error());
+      HIP_CHECK(hipEventCreateWithFlags(&fx.done, hipEventDisableTiming));
+    }
+    ypt_integrator* X = fx.x;
+    if (opts != fx.applied) {  // applied once per change: some options drop buffers sized by the old value
+      for (const auto& o : opts)
+        if (ypt_set_option(X, o.first.c_str(), o.second) != YPT_OK) throw std::runtime_error(ypt_last_error());
+      fx.applied = opts;
+    }
+    std::vector<std::shared_ptr<ServerJob>> mem;
+    std::vector<uint32_t> passes;
+    uint64_t total = 0;
+    for (const auto& j : ready) {
+      if (mem.size() == FUSED_MAX_JOBS) break;
+      uint32_t g = std::min(j->chunk, j->spp - j->issued);
+      uint64_t n = (uint64_t)j->frame.width * j->frame.height * g;
+      if (!mem.empty() && total + n > maxSlots) continue;
+      mem.push_back(j);
+      passes.push_back(g);
+      total += n;
+    }
+    FusedArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.njobs = (uint32_t)mem.size();
+    fx.hJobs.assign(mem.size(), FusedJob());
+    fx.hPoints.clear();
+    std::vector<size_t> ptOff(mem.size());
+    uint32_t depth = 0, bound = 0;
+    bool alpha = false, shadow = false;
+    uint64_t slot = 0;
+    for (size_t k = 0; k < mem.size(); ++k) {
+      ServerJob& j = *mem[k];
+      ypt_integrator* h = j.h;
+      uint32_t npix = h->width * h->height;
+      A.slotBase[k] = (uint32_t)slot;
+      slot += (uint64_t)npix * passes[k];
+      ptOff[k] = fx.hPoints.size();
+      for (uint32_t p = 0; p < passes[k]; ++p)
+        for (uint32_t d = 0; d < YPT_SOBOL_POINT_DIMS; ++d) fx.hPoints.push_back(sobol_uint(YPT_SOBOL_MATRICES, j.issued + p, d));
+      FusedJob& J = fx.hJobs[k];
+      std::memset(&J, 0, sizeof(J));
+      J.S = h->dev_scene();
+      J.fc = h->fc;
+      J.npix = npix;
+      J.nee = h->type == YPT_INTEGRATOR_NEE ? 1u : 0u;
+      depth = std::max(depth, h->fc.maxRayDepth);
+      bound = std::max(bound, X->bvhWidth == 8 ? h->res->stackBound8 : h->res->stackBound2);
+      alpha = alpha || J.S.tris.alpha.tris != nullptr;
+      shadow = shadow || traces_shadow_rays(h);
+    }
+    A.slotBase[mem.size()] = (uint32_t)slot;
+    // the pool's buffers (grow-only): path state and queues for the step's slots, stack overflow for
+    // the deepest BVH among its scenes
+    X->counters.ensure(16);
+    X->work.ensure(16);
+    ensure_slots(X, slot);
+    size_t stride = X->slots + 1024;
+    if (X->persistent) stride = std::min(stride, (size_t)X->cus * X->persistentWaves * 64);
+    size_t ovfDepth = bound > (uint32_t)X->ldsDepth ? bound - (uint32_t)X->ldsDepth : 1;
+    X->ovf.ensure(ovfDepth * stride);
+    X->ovfStride = (uint32_t)stride;
+    fx.points.ensure(fx.hPoints.size());
+    fx.jobs.ensure(mem.size());
+    for (size_t k = 0; k < mem.size(); ++k) fx.hJobs[k].points = fx.points.p + ptOff[k];
+    hipStream_t s = X->stream;
+    HIP_CHECK(hipMemcpyAsync(fx.points.p, fx.hPoints.data(), fx.hPoints.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(fx.jobs.p, fx.hJobs.data(), fx.hJobs.size() * sizeof(FusedJob), hipMemcpyHostToDevice, s));
+    A.jobs = fx.jobs.p;
+    launch_fused(X, A, depth, alpha, shadow);
+    PathSoA P = X->path_soa();
+    for (size_t k = 0; k < mem.size(); ++k) {
+      ServerJob& j = *mem[k];
+      ypt_integrator* h = j.h;
+      FilmBuffers F = h->film();
+      uint32_t npix = h->width * h->height;
+      for (uint32_t p = 0; p < passes[k]; ++p) {
+        MergeParams mp;
+        std::memset(&mp, 0, sizeof(mp));
+        mp.width = h->width; mp.height = h->height; mp.slotBase = A.slotBase[k] + p * npix;
+        mp.sampleCount = h->filmSamples; mp.accumTargets = 2;
+        mp.fireflyClampScale = h->frameDesc.firefly_clamp_scale;
+        mp.fireflyClampRelaxPower = h->frameDesc.firefly_clamp_relax_power;
+        mp.killNaN = h->frameDesc.kill_nan_samples;
+        mp.cur = h->filmCur;
+        mp.pass = p;
+        hipLaunchKernelGGL(k_merge, dim3(blocks_for(npix, 256)), dim3(256), 0, s, fx.hJobs[k].S, P, F, mp, (const float4*)nullptr,
+                           (const uint32_t*)nullptr);
+        h->filmSamples++;
+        h->filmCur ^= 1;
+      }
+      j.issued += passes[k];
+      j.inFlight = true;
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(fx.done, s));
+    fx.inFlight = true;
+    fx.members = mem;
+    fusedSteps++;
+    fusedJobChunks += mem.size();
   }
 
   void finish_job(ServerJob& j) {
@@ -1745,7 +1933,9 @@ struct ypt_server {
     (void)hipSetDevice(device);
     while (true) {
       std::vector<std::shared_ptr<ServerJob>> fresh, act;
-      bool dropped = false;
+      bool dropped = false, useFuse = true;
+      uint64_t maxSlots = 0;
+      std::vector<std::pair<std::string, int64_t>> fusedOpts;
       {
         std::lock_guard<std::mutex> lk(m);
         if (stop) break;
@@ -1775,10 +1965,14 @@ struct ypt_server {
         }
         peakActive = std::max<uint64_t>(peakActive, active.size());
         act = active;
+        useFuse = fuse;
+        maxSlots = fuseMaxSlots;
+        fusedOpts = options[YPT_INTEGRATOR_NEE];
       }
       std::vector<std::shared_ptr<ServerJob>> ended;
       for (auto& j : fresh) {
         try {
+          j->fused = useFuse && j->type != YPT_INTEGRATOR_BDPT && !j->scene->subsurface;
           start_job(*j);
         } catch (const std::exception& e) {
           j->err = e.what();
@@ -1787,10 +1981,30 @@ struct ypt_server {
         }
       }
       bool progress = !fresh.empty() || dropped;
+      // a fused step that has ended hands its jobs back
+      for (Fused& fx : this->fx) {
+        if (!fx.inFlight) continue;
+        hipError_t q = hipEventQuery(fx.done);
+        if (q != hipErrorNotReady) {
+          fx.inFlight = false;
+          for (auto& m : fx.members) {
+            m->inFlight = false;
+            if (q != hipSuccess && m->state == JOB_ACTIVE) {
+              m->err = std::string("fused step: ") + hipGetErrorString(q);
+              m->state = JOB_FAILED;
+              ended.push_back(m);
+            }
+          }
+          fx.members.clear();
+          progress = true;
+        }
+      }
+      std::vector<std::shared_ptr<ServerJob>> ready;  // fused jobs waiting for the next step
       for (auto& j : act) {
         if (j->state != JOB_ACTIVE) continue;
         try {
           if (j->inFlight) {
+            if (j->fused) continue;  // released when its step ends
             hipError_t q = hipEventQuery(j->ev);
             if (q == hipErrorNotReady) continue;
             HIP_CHECK(q);
@@ -1803,6 +2017,10 @@ struct ypt_server {
             j->tFinish = now_s();
             ended.push_back(j);
             progress = true;
+            continue;
+          }
+          if (j->issued < j->spp && j->fused) {
+            ready.push_back(j);
             continue;
           }
           if (j->issued < j->spp) {
@@ -1823,6 +2041,25 @@ struct ypt_server {
           j->state = JOB_FAILED;
           ended.push_back(j);
         }
+      }
+      Fused* freePool = !fx[0].inFlight ? &fx[0] : !fx[1].inFlight ? &fx[1] : nullptr;
+      if (!ready.empty() && freePool) {
+        Fused& fx = *freePool;
+        try {
+          issue_fused(fx, ready, maxSlots, fusedOpts);
+        } catch (const std::exception& e) {
+          // the step's stream is drained, and every job it would have carried fails
+          if (fx.x) (void)hipStreamSynchronize(fx.x->stream);
+          fx.inFlight = false;
+          fx.members.clear();
+          for (auto& j : ready) {
+            j->inFlight = false;
+            j->err = e.what();
+            j->state = JOB_FAILED;
+            ended.push_back(j);
+          }
+        }
+        progress = true;
       }
       if (!ended.empty()) {
         std::lock_guard<std::mutex> lk(m);
@@ -1867,6 +2104,13 @@ int ypt_server_destroy(ypt_server* s) {
     s->stop = true;
   }
   if (s->worker.joinable()) s->worker.join();
+  for (auto& fx : s->fx) {
+    if (!fx.x) continue;
+    (void)hipStreamSynchronize(fx.x->stream);
+    if (fx.done) (void)hipEventDestroy(fx.done);
+    ypt_destroy(fx.x);
+    fx.x = nullptr;
+  }
   for (auto& kv : s->jobs) {
     ServerJob& j = *kv.second;
     if (j.h) {
@@ -1907,6 +2151,15 @@ int ypt_server_submit(ypt_server* s, ypt_scene* sc, int integrator_type, const y
 int ypt_server_set_option(ypt_server* s, int integrator_type, const char* name, int64_t value) {
   if (!s || !name) return fail(YPT_E_INVALID, "null argument");
   if (integrator_type > YPT_INTEGRATOR_BDPT) return fail(YPT_E_INVALID, "unknown integrator type");
+  std::string n(name);
+  if (n == "fuse_jobs" || n == "fuse_max_slots") {  // server settings, for steps issued from now on
+    if (n == "fuse_max_slots" && (value < 1 || value > (int64_t)1 << 31))
+      return fail(YPT_E_INVALID, "fuse_max_slots must be in [1, 2^31]");
+    std::lock_guard<std::mutex> lk(s->m);
+    if (n == "fuse_jobs") s->fuse = value != 0;
+    else s->fuseMaxSlots = (uint64_t)value;
+    return YPT_OK;
+  }
   {
     // validated now on a handle that holds no device state, so a bad value fails here and not in a job
     std::unique_ptr<ypt_integrator> probe(new ypt_integrator());
@@ -1995,6 +2248,13 @@ int ypt_server_stats(ypt_server* s, double* out6) {
   out6[3] = (double)s->memUsed;
   out6[4] = (double)s->peakActive;
   out6[5] = (double)s->budget;
+  return YPT_OK;
+}
+
+int ypt_server_fused_stats(ypt_server* s, uint64_t* steps, uint64_t* job_chunks) {
+  if (!s || !steps || !job_chunks) return fail(YPT_E_INVALID, "null argument");
+  *steps = s->fusedSteps.load();
+  *job_chunks = s->fusedJobChunks.load();
   return YPT_OK;
 }
 

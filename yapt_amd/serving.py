@@ -2,7 +2,7 @@
 render server at their arrival times; reports p50/p99 latency (submit to film complete) and
 throughput.
 
-    python -m yapt_amd.serving [--rate 40] [--jobs 120] [--budget-gb 96] [--max-active 8]
+    python -m yapt_amd.serving [--rate 40] [--jobs 120] [--budget-gb 96] [--max-active 8] [--no-fuse]
 
 Default mix: 4 resident random-shapes scenes (~10k, ~100k, ~1M, ~5M triangles), resolution uniform in
 {256x256, 512x512, 1280x720, 1920x1080}, spp uniform in {1, 4, 16, 64}, integrator BPT : NEE : BDPT
@@ -70,7 +70,7 @@ def run(server, scenes, cams, requests, depth=8, keep_films=()):
     return recs
 
 
-def trace(jobs=120, rate=40.0, shapes=SCENE_SHAPES, budget_gb=96.0, max_active=8, seed=4, log=lambda m: None):
+def trace(jobs=120, rate=40.0, shapes=SCENE_SHAPES, budget_gb=96.0, max_active=8, seed=4, log=lambda m: None, fuse=True):
     """Builds the resident scenes, replays a seeded Poisson stream of `jobs` requests at `rate` per second
     through a render server with the tuned launch options, and returns the latency and throughput
     figures (seconds from submit to film complete)."""
@@ -85,6 +85,7 @@ def trace(jobs=120, rate=40.0, shapes=SCENE_SHAPES, budget_gb=96.0, max_active=8
         log("scene %d: %d triangles resident" % (k, sc.triangle_count))
     t_build = time.perf_counter() - t_build
     server = _lib.Server(0, int(budget_gb * (1 << 30)), max_active)
+    server.set_option("fuse_jobs", 1 if fuse else 0)
     skip = ("passes_in_flight",)   # the server sizes batches per job
     for integ, key in ((0, "nee-shapes2000-1920x1080-d12"), (1, "nee-shapes2000-1920x1080-d12"),
                        (2, "bdpt-shapes2000-1280x720-d8")):
@@ -96,6 +97,7 @@ def trace(jobs=120, rate=40.0, shapes=SCENE_SHAPES, budget_gb=96.0, max_active=8
     recs = run(server, res_scenes, cams, reqs)
     wall = time.perf_counter() - t
     st = server.stats()
+    fs = server.fused_stats()
     server.close()
     for r in res_scenes:
         r.close()
@@ -111,6 +113,7 @@ def trace(jobs=120, rate=40.0, shapes=SCENE_SHAPES, budget_gb=96.0, max_active=8
         "throughput_path_samples_s": sum(r["samples"] for r in recs) / (last - first),
         "offered_path_samples_s": sum(r["samples"] for r in recs) / max(1e-9, reqs[-1]["t"]),
         "wall_s": wall, "peak_active": st["peak_active"], "max_active": max_active,
+        "fused_steps": fs["steps"], "jobs_per_fused_step": fs["job_chunks"] / fs["steps"] if fs["steps"] else None,
         "budget_gb": budget_gb, "all_finite": all(r["ok"] for r in recs),
         "by_integrator": {name: percentiles([r["latency"] for r in recs if r["integrator"] == k])
                           for k, name in ((0, "bpt"), (1, "nee"), (2, "bdpt"))},
@@ -125,9 +128,10 @@ def main():
     p.add_argument("--max-active", type=int, default=8)
     p.add_argument("--shapes", default=",".join(str(s) for s in SCENE_SHAPES))
     p.add_argument("--seed", type=int, default=4)
+    p.add_argument("--no-fuse", action="store_true", help="every job on its own stream (no fused steps)")
     a = p.parse_args()
     out = trace(a.jobs, a.rate, [int(x) for x in a.shapes.split(",")], a.budget_gb, a.max_active, a.seed,
-                log=lambda m: print(m, file=sys.stderr, flush=True))
+                log=lambda m: print(m, file=sys.stderr, flush=True), fuse=not a.no_fuse)
     print(json.dumps(out))
 
 
