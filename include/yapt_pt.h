@@ -227,7 +227,7 @@ int ypt_get_first_hit_features(ypt_integrator* h, float* out);
 int ypt_get_stats(ypt_integrator* h, double* out16);
 /* Launch and diagnostics knobs (no counterpart in the reference, whose traversal is the driver's):
  *  "bvh_width" 2 | 8, "block_size" 64..1024 (multiple of 64), "lds_stack_depth" 1..64,
- *  "passes_in_flight" 0..32 (0 = automatic), "persistent" 0/1 (traversal kernels loop over their
+ *  "passes_in_flight" 0..128 (0 = automatic), "persistent" 0/1 (traversal kernels loop over their
  *  queue), "persistent_waves" 1..64 (waves per CU of those launches), "refill_threshold" 1..64
  *  (idle lanes before a wave fetches new rays), "leaf_share" 1..64 (BVH2 persistent traversal: a
  *  wave intersects its waiting leaves once 1/leaf_share of its active lanes wait), "shade_waves" 0..1024 (0 = one workgroup per queue

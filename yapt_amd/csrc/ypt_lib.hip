@@ -384,7 +384,7 @@ void ensure_overflow(ypt_integrator* h) {
 // empty bounce launches of one batch are shared by many passes. The passes of one render are split
 // into batches of equal size (40 passes at a limit of 32 run as 20 + 20, not 32 + 8), so no batch is
 // left with a short, latency-bound tail.
-constexpr uint32_t MAX_PASSES_IN_FLIGHT = 32;
+constexpr uint32_t MAX_PASSES_IN_FLIGHT = 128;
 uint32_t passes_in_flight(const ypt_integrator* h, uint32_t count) {
   size_t npix = (size_t)h->width * h->height;
   uint32_t g = h->passesInFlight > 0 ? (uint32_t)h->passesInFlight : (uint32_t)std::max<size_t>(1, (64u << 20) / npix);

@@ -1,9 +1,9 @@
 """Train the denoiser on the tracer's own renders and judge it at 1920x1080.
 
-    python -m yapt_amd.denoise_bench [--scenes 48] [--iters 3000] [--out yapt_amd/data/denoiser_weights.npz]
+    python -m yapt_amd.denoise_bench [--scenes 2000] [--iters 6000] [--out yapt_amd/data/denoiser_weights.npz]
 
-1. Dataset: seeded random-shapes scenes at 256x256, NEE, 4 spp inputs (film + first-hit features) and
-   1024 spp targets.
+1. Dataset: 2,000 seeded random-shapes scenes at 128x128, NEE, 4 spp inputs (film + first-hit
+   features) and 1024 spp targets.
 2. Training: Adam, random 128x128 crops, batch 8, cosine learning-rate decay.
 3. Judgement on a held-out scene (the 1M-triangle random-shapes scene, seed 2, 1920x1080): relMSE
    against a 4096-spp plain render, for plain 64 spp and for 4 spp + denoiser, with the wall time of
@@ -184,10 +184,10 @@ def main():
     p = argparse.ArgumentParser()
     p.add_argument("--forward-only", action="store_true",
                    help="time the U-Net forward pass at the evaluation size and print its conv roofline")
-    p.add_argument("--scenes", type=int, default=48)
-    p.add_argument("--res", type=int, default=256)
+    p.add_argument("--scenes", type=int, default=2000)
+    p.add_argument("--res", type=int, default=128)
     p.add_argument("--spp-ref", type=int, default=1024)
-    p.add_argument("--iters", type=int, default=3000)
+    p.add_argument("--iters", type=int, default=6000)
     p.add_argument("--batch", type=int, default=8)
     p.add_argument("--crop", type=int, default=128)
     p.add_argument("--lr", type=float, default=1e-3)

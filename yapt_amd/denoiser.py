@@ -325,27 +325,86 @@ def render_pair(it, frame, spp_in, spp_ref, start_ref):
     return noisy, feats, ref
 
 
-def make_dataset(n_scenes, res=256, spp_in=4, spp_ref=1024, seed=100, log=None):
-    """Renders of seeded random-shapes scenes (between 100 and 600 shapes, random camera) with the NEE
-    integrator: noisy inputs, features and references, as float32 arrays."""
-    from . import _lib, camera, scenes
-    rng = np.random.default_rng(seed)
+def _training_params(rng, n_scenes):
+    """the draws of each training scene, in scene order: shapes (100 to 600), quad lights (1 to 4),
+    emissive triangles (0 to 5), camera eye and target jitter and field of view"""
+    out = []
+    for _ in range(n_scenes):
+        n_shapes = int(rng.integers(100, 600))
+        n_lights = int(rng.integers(1, 5))
+        n_emissive = int(rng.integers(0, 6))
+        out.append((n_shapes, n_lights, n_emissive, rng.normal(0, 3.0, 3), rng.normal(0, 2.0, 3), float(rng.uniform(0.6, 1.2))))
+    return out
+
+
+def _training_scene(params, seed, k):
+    from . import camera, scenes
+    n_shapes, n_lights, n_emissive, deye, dtgt, fovy = params
+    scene, cam = scenes.random_shapes(n_shapes=n_shapes, seed=int(seed * 1000 + k), cube=12.0, n_lights=n_lights,
+                                      emissive_triangles=n_emissive)
+    eye = np.array(cam.eye, np.float64) + deye
+    tgt = np.array(cam.target, np.float64) + dtgt
+    return scene, camera.Camera(eye=tuple(eye), target=tuple(tgt), fovy=fovy)
+
+
+def make_dataset(n_scenes, res=256, spp_in=4, spp_ref=1024, seed=100, log=None, workers=3, prefetch=6):
+    """Renders of seeded random-shapes scenes with the NEE integrator: noisy inputs, features and
+    references, as float32 arrays. Worker threads generate the next scenes and build their BVHs
+    (resident scenes) while the GPU renders the current one; every scene's draws are made up front
+    from one seeded generator, so the dataset does not depend on the timing."""
+    import threading
+    from . import _lib, camera
+    params = _training_params(np.random.default_rng(seed), n_scenes)
+    ready = {}
+    cv = threading.Condition()
+    state = {"next": 0, "consumed": 0, "error": None}
+
+    def producer():
+        while True:
+            with cv:
+                while state["next"] < n_scenes and state["next"] >= state["consumed"] + prefetch and state["error"] is None:
+                    cv.wait()
+                if state["next"] >= n_scenes or state["error"] is not None:
+                    return
+                k = state["next"]
+                state["next"] += 1
+            try:
+                scene, cam = _training_scene(params[k], seed, k)
+                item = (_lib.ResidentScene(scene), cam, params[k][0])
+            except Exception as e:  # reported by the consumer
+                with cv:
+                    state["error"] = e
+                    cv.notify_all()
+                return
+            with cv:
+                ready[k] = item
+                cv.notify_all()
+
+    threads = [threading.Thread(target=producer, daemon=True) for _ in range(max(1, workers))]
+    for t in threads:
+        t.start()
     it = _lib.Integrator(_lib.YPT_INTEGRATOR_NEE, 0)
     it.set_option("first_hit_features", 1)
     data = []
     for k in range(n_scenes):
-        n_shapes = int(rng.integers(100, 600))
-        scene, cam = scenes.random_shapes(n_shapes=n_shapes, seed=int(seed * 1000 + k), cube=12.0, n_lights=int(rng.integers(1, 5)),
-                                          emissive_triangles=int(rng.integers(0, 6)))
-        eye = np.array(cam.eye, np.float64) + rng.normal(0, 3.0, 3)
-        tgt = np.array(cam.target, np.float64) + rng.normal(0, 2.0, 3)
-        cam = camera.Camera(eye=tuple(eye), target=tuple(tgt), fovy=float(rng.uniform(0.6, 1.2)))
-        it.set_scene(scene)
+        with cv:
+            while k not in ready and state["error"] is None:
+                cv.wait()
+            if k not in ready:
+                raise state["error"]
+            rs, cam, n_shapes = ready.pop(k)
+        it.set_scene(rs)
         fr = camera.make_frame(cam, res, res, max_depth=8)
         noisy, feats, ref = render_pair(it, fr, spp_in, spp_ref, 1000)
         data.append((noisy, feats, ref))
-        if log:
+        rs.close()
+        with cv:
+            state["consumed"] = k + 1
+            cv.notify_all()
+        if log and (k % 50 == 0 or k == n_scenes - 1):
             log("scene %d/%d: %d shapes" % (k + 1, n_scenes, n_shapes))
+    for t in threads:
+        t.join()
     it.close()
     return data
 
