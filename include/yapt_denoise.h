@@ -20,6 +20,10 @@ extern "C" {
  * Cin % 16 == 0, Cout % 32 == 0; bias may be NULL. */
 int ydn_conv3x3(const uint16_t* in, int N, int H, int W, int Cin, const uint16_t* w, const float* bias, int Cout, int relu,
                 uint16_t* out, void* stream);
+/* ydn_conv3x3 that also writes pooled[N][H/2][W/2][Cout] = ydn_pool2(out), from the tile in registers
+ * (the same values as the two calls; H and W even) */
+int ydn_conv3x3_pool2(const uint16_t* in, int N, int H, int W, int Cin, const uint16_t* w, const float* bias, int Cout,
+                      int relu, uint16_t* out, uint16_t* pooled, void* stream);
 /* the conv of the concatenation [nearest 2x upsample of coarse[N][H/2][W/2][C1], skip[N][H][W][C2]] with
  * w[Cout][3][3][C1 + C2]: ydn_up2_concat followed by ydn_conv3x3, without the concatenated tensor.
  * H and W even, C1 and C2 multiples of 16. */

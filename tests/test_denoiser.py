@@ -84,6 +84,35 @@ def _f(t):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(2, 12, 20, 16, 32), (1, 10, 66, 32, 64), (2, 16, 16, 48, 128), (1, 18, 70, 96, 32)])
+def test_conv_with_pooled_output_equals_conv_then_pool(shape):
+    # the pooled output written from the conv's tile is ydn_pool2 of its output, bit for bit, and the
+    # output itself is ydn_conv3x3's
+    import torch
+    from yapt_amd import denoiser
+    lib = denoiser.load()
+    N, H, W, Cin, Cout = shape
+    rng = np.random.default_rng(9)
+    xg = _bits(rng.normal(size=(N, H, W, Cin)))
+    wg = _bits(rng.normal(0, 0.2, size=(Cout, 3, 3, Cin)))
+    bg = torch.from_numpy(rng.normal(0, 0.1, size=Cout).astype(np.float32)).cuda()
+    P = denoiser._p
+    for relu in (0, 1):
+        out = denoiser.bf16_empty((N, H, W, Cout))
+        denoiser.check(lib.ydn_conv3x3(P(xg), N, H, W, Cin, P(wg), P(bg), Cout, relu, P(out), None))
+        pool = denoiser.bf16_empty((N, H // 2, W // 2, Cout))
+        denoiser.check(lib.ydn_pool2(P(out), N, H, W, Cout, P(pool), None))
+        out2 = denoiser.bf16_empty((N, H, W, Cout))
+        pool2 = denoiser.bf16_empty((N, H // 2, W // 2, Cout))
+        denoiser.check(lib.ydn_conv3x3_pool2(P(xg), N, H, W, Cin, P(wg), P(bg), Cout, relu, P(out2), P(pool2), None))
+        torch.cuda.synchronize()
+        assert torch.equal(out, out2)
+        assert torch.equal(pool, pool2)
+    bad = denoiser.bf16_empty((1, 4, 4, 32))
+    assert lib.ydn_conv3x3_pool2(P(xg), 1, 5, 4, Cin, P(wg), None, 32, 0, P(bad), P(bad), None) != 0  # odd H
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("shape", [(2, 12, 20, 16, 32), (1, 9, 7, 32, 64), (2, 16, 16, 48, 128), (1, 5, 33, 96, 96),
                                    (1, 19, 70, 48, 32), (1, 17, 65, 224, 96)])
 def test_conv_forward_matches_torch_fp32_and_checker(shape):

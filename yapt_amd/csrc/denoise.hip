@@ -78,7 +78,8 @@ __device__ __forceinline__ bf16x8 lds8(const uint8_t* p) {
 template <int CIC, int COT, int RPW, int OCC>
 __global__ __launch_bounds__(256, OCC) void k_conv3x3(ConvIn src, int N, int H, int W, const uint16_t* __restrict__ w,
                                                     const float* __restrict__ bias, int Cout, int relu,
-                                                    uint16_t* __restrict__ out) {
+                                                    uint16_t* __restrict__ out, uint16_t* __restrict__ pooled) {
+  static_assert(RPW % 2 == 0, "the pooled epilogue pairs the output rows of a wavefront");
   constexpr int TH = 4 * RPW, TW = 32, PW = TW + 2, PH = TH + 2;
   constexpr int PIX = CIC * 2 + 16;       // LDS bytes per staged pixel
   constexpr int WROW = 9 * CIC * 2 + 16;  // LDS bytes per staged output channel
@@ -213,31 +214,52 @@ __global__ __launch_bounds__(256, OCC) void k_conv3x3(ConvIn src, int N, int H, 
       }
     }
     if (chunk == nChunks - 1) {
-      // acc[r][t][i]: output channel co0 + 32 t + 8 (i / 4) + 4 half + i % 4 of pixel (row r of this wave, column col)
+      // acc[r][t][i]: output channel co0 + 32 t + 8 (i / 4) + 4 half + i % 4 of pixel (row r of this wave, column col).
+      // Rows go in pairs so that the pooled output (2x2 average of the stored values in k_pool2's sum order:
+      // rows r, r + 1, columns col (even) and col + 1, the next lane) is made from registers.
       const int x = tx0 + col;
-      for (int r = 0; r < RPW; ++r) {
+      const int Ho = H >> 1, Wo = W >> 1;
+      for (int r = 0; r < RPW; r += 2) {
         const int y = ty0 + wave * RPW + r;
-        if (y < H && x < W) {
-          uint16_t* dst = out + (((size_t)n * H + y) * W + x) * Cout;
-          for (int t = 0; t < COT; ++t) {
-            const int cb = co0 + 32 * t;
-            if (cb >= Cout) break;
-            for (int g = 0; g < 4; ++g) {
-              const int co = cb + 8 * g + 4 * half;
-              uint16_t o[4];
+        for (int t = 0; t < COT; ++t) {
+          const int cb = co0 + 32 * t;
+          if (cb >= Cout) break;
+          for (int g = 0; g < 4; ++g) {
+            const int co = cb + 8 * g + 4 * half;
+            uint16_t o0[4], o1[4];
+            for (int j = 0; j < 4; ++j) {
+              const float bj = bias ? bias[co + j] : 0.f;
+              float v0 = acc[r][t][4 * g + j] + bj, v1 = acc[r + 1][t][4 * g + j] + bj;
+              if (relu && v0 < 0.f) v0 = 0.f;
+              if (relu && v1 < 0.f) v1 = 0.f;
+              o0[j] = f2bf(v0);
+              o1[j] = f2bf(v1);
+            }
+            if (x < W) {
+              if (y < H)
+                *reinterpret_cast<uint2*>(out + (((size_t)n * H + y) * W + x) * Cout + co) =
+                    make_uint2((uint32_t)o0[0] | ((uint32_t)o0[1] << 16), (uint32_t)o0[2] | ((uint32_t)o0[3] << 16));
+              if (y + 1 < H)
+                *reinterpret_cast<uint2*>(out + (((size_t)n * H + y + 1) * W + x) * Cout + co) =
+                    make_uint2((uint32_t)o1[0] | ((uint32_t)o1[1] << 16), (uint32_t)o1[2] | ((uint32_t)o1[3] << 16));
+            }
+            if (pooled) {
+              uint16_t q[4];
               for (int j = 0; j < 4; ++j) {
-                float v = acc[r][t][4 * g + j] + (bias ? bias[co + j] : 0.f);
-                if (relu && v < 0.f) v = 0.f;
-                o[j] = f2bf(v);
+                const float x00 = bf2f(o0[j]), x10 = bf2f(o1[j]);
+                const float x01 = __shfl_xor(x00, 1), x11 = __shfl_xor(x10, 1);
+                q[j] = f2bf(0.25f * (((x00 + x01) + x10) + x11));
               }
-              *reinterpret_cast<uint2*>(dst + co) =
-                  make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+              if ((col & 1) == 0 && y < H && x < W)
+                *reinterpret_cast<uint2*>(pooled + (((size_t)n * Ho + (y >> 1)) * Wo + (x >> 1)) * Cout + co) =
+                    make_uint2((uint32_t)q[0] | ((uint32_t)q[1] << 16), (uint32_t)q[2] | ((uint32_t)q[3] << 16));
             }
           }
         }
+      }
+      for (int r = 0; r < RPW; ++r)
         for (int t = 0; t < COT; ++t)
           for (int i = 0; i < 16; ++i) acc[r][t][i] = 0.f;
-      }
     }
     if (!more) break;
     job = nJob;
@@ -485,13 +507,13 @@ unsigned conv_grid(long jobs, int occ) {
 
 template <int COT, int RPW, int OCC>
 void launch_conv_t(const ConvIn& src, int N, int H, int W, const uint16_t* w, const float* bias, int Cout, int relu,
-                   uint16_t* out, hipStream_t s) {
+                   uint16_t* out, uint16_t* pooled, hipStream_t s) {
   const long jobs = (long)((W + 31) / 32) * ((H + 4 * RPW - 1) / (4 * RPW)) * N * ((Cout + 32 * COT - 1) / (32 * COT));
   dim3 grid(conv_grid(jobs, OCC));
   if ((src.Ca + src.Cb) % 32 == 0)
-    hipLaunchKernelGGL((k_conv3x3<32, COT, RPW, OCC>), grid, dim3(256), 0, s, src, N, H, W, w, bias, Cout, relu, out);
+    hipLaunchKernelGGL((k_conv3x3<32, COT, RPW, OCC>), grid, dim3(256), 0, s, src, N, H, W, w, bias, Cout, relu, out, pooled);
   else
-    hipLaunchKernelGGL((k_conv3x3<16, COT, RPW, OCC>), grid, dim3(256), 0, s, src, N, H, W, w, bias, Cout, relu, out);
+    hipLaunchKernelGGL((k_conv3x3<16, COT, RPW, OCC>), grid, dim3(256), 0, s, src, N, H, W, w, bias, Cout, relu, out, pooled);
 }
 
 // One 32-channel tile and two output rows per wavefront (8 x 32 pixels per workgroup), three workgroups
@@ -499,8 +521,8 @@ void launch_conv_t(const ConvIn& src, int N, int H, int W, const uint16_t* w, co
 // channel tiles, one to three workgroups per CU), the one with the most workgroups in flight was
 // fastest, as each step's global loads wait on the memory system rather than on the matrix cores.
 int launch_conv(const ConvIn& src, int N, int H, int W, const uint16_t* w, const float* bias, int Cout, int relu,
-                uint16_t* out, hipStream_t s) {
-  launch_conv_t<1, 2, 3>(src, N, H, W, w, bias, Cout, relu, out, s);
+                uint16_t* out, hipStream_t s, uint16_t* pooled = nullptr) {
+  launch_conv_t<1, 2, 3>(src, N, H, W, w, bias, Cout, relu, out, pooled, s);
   return check_launch();
 }
 
@@ -525,6 +547,15 @@ int ydn_conv3x3(const uint16_t* in, int N, int H, int W, int Cin, const uint16_t
   if (int rc = check_conv(N, H, W, Cin, Cout)) return rc;
   ConvIn src{in, Cin, nullptr, 0, 0};
   return launch_conv(src, N, H, W, w, bias, Cout, relu, out, (hipStream_t)stream);
+}
+
+int ydn_conv3x3_pool2(const uint16_t* in, int N, int H, int W, int Cin, const uint16_t* w, const float* bias, int Cout, int relu,
+                      uint16_t* out, uint16_t* pooled, void* stream) {
+  if (!in || !w || !out || !pooled) return fail(YDN_E_INVALID, "null tensor");
+  if (int rc = check_half_shape(N, H, W, Cout)) return rc;
+  if (int rc = check_conv(N, H, W, Cin, Cout)) return rc;
+  ConvIn src{in, Cin, nullptr, 0, 0};
+  return launch_conv(src, N, H, W, w, bias, Cout, relu, out, (hipStream_t)stream, pooled);
 }
 
 int ydn_conv3x3_up2cat(const uint16_t* coarse, int C1, const uint16_t* skip, int C2, int N, int H, int W, const uint16_t* w,

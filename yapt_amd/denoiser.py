@@ -7,7 +7,7 @@ the stream; no torch operator computes any part of the network.
 
 Network: input = [log(1 + colour), albedo, normal, 7 zero channels] (16 channels). Encoder level l
 (l = 0..4, resolution / 2^l): two 3x3 conv + ReLU with widths 32, 64, 64, 96, 128, 2x2 average
-pooling between levels. Decoder level l (3..0): nearest 2x upsample, concatenation with the encoder
+pooling between levels (written by the level's last conv from its output tile). Decoder level l (3..0): nearest 2x upsample, concatenation with the encoder
 output of level l, two 3x3 conv + ReLU. Output: 3x3 conv to 32 channels (the first 3 used), added
 to log(1 + colour); loss L1 in that log space against a high-spp render.
 """
@@ -43,6 +43,7 @@ def load():
     lib.ydn_conv3x3.argtypes = [P, I, I, I, I, P, P, I, I, P, P]
     lib.ydn_conv3x3_wgrad.argtypes = [P, P, I, I, I, I, I, P, P, P]
     lib.ydn_conv3x3_up2cat.argtypes = [P, I, P, I, I, I, I, P, P, I, I, P, P]
+    lib.ydn_conv3x3_pool2.argtypes = [P, I, I, I, I, P, P, I, I, P, P, P]
     lib.ydn_flip_weights.argtypes = [P, I, I, P, P]
     lib.ydn_relu_grad.argtypes = [P, P, L, P]
     lib.ydn_pool2.argtypes = [P, I, I, I, I, P, P]
@@ -118,6 +119,18 @@ class Conv:
         if keep:
             self.x, self.y = x, y
         return y
+
+    def forward_pool(self, x, keep):
+        """forward() and the 2x2 average pooling of its output, both written by the conv kernel"""
+        N, H, W, C = x.shape
+        assert C == self.cin, (C, self.cin)
+        y = bf16_empty((N, H, W, self.cout))
+        p = bf16_empty((N, H // 2, W // 2, self.cout))
+        check(load().ydn_conv3x3_pool2(_p(x), N, H, W, C, _p(self.w16), _p(self.b), self.cout, int(self.relu), _p(y), _p(p),
+                                       _stream()))
+        if keep:
+            self.x, self.y = x, y
+        return y, p
 
     def forward_up2cat(self, coarse, skip):
         """the conv of [nearest 2x upsample of coarse, skip] without writing the concatenation (inference)"""
@@ -195,14 +208,15 @@ class UNet:
         assert H % self.multiple == 0 and W % self.multiple == 0, "H and W must be multiples of %d" % self.multiple
         self.skips = []
         h = x
+        pooled = None
         for l in range(L):
             if l > 0:
-                n_, hh, ww, c = h.shape
-                p = bf16_empty((n_, hh // 2, ww // 2, c))
-                check(lib.ydn_pool2(_p(h), n_, hh, ww, c, _p(p), s))
-                h = p
+                h = pooled   # the 2x2 average of the previous level's output, written by its last conv
             h = self.enc[l][0].forward(h, keep)
-            h = self.enc[l][1].forward(h, keep)
+            if l < L - 1:
+                h, pooled = self.enc[l][1].forward_pool(h, keep)
+            else:
+                h = self.enc[l][1].forward(h, keep)
             self.skips.append(h)
         for l in range(L - 2, -1, -1):
             skip = self.skips[l]
