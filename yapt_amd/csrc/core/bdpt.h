@@ -1252,71 +1252,107 @@ struct SplatRecord {
   vec3 xyz;
 };
 
-// calculatePureLightPaths (BDPTCalculatePureLightPaths.comp:107-206) for the light subpath in header
-// slot `header`: one record per vertex 1 .. count-1 into rec[0 .. count-1).
-template <class Tracer, class Surfaces>
-YPT_HD void splat_light_path(const SceneView& S, const FrameConstants& fc, const uint32_t* point, uint32_t header,
-                             uint32_t headerCount, float lambda, const LightNodePacked* nodes, uint32_t count, Surfaces& surfs,
-                             Tracer& tr, SplatRecord* rec) {
-  uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
-  uint32_t npix = W * H;
-  for (uint32_t i = 1; i < count; ++i) rec[i - 1].texel = npix;
-  if (count <= 1) return;
-  Lambdas L = spectral_lambdas(lambda);
+// calculatePureLightPaths (BDPTCalculatePureLightPaths.comp:107-206), in pieces: the lens point of a
+// light subpath (splat_lens), the MIS state that vertex i inherits from vertices 0 .. i-1
+// (SplatPrefix, advanced one vertex at a time), and the splat of one vertex (splat_vertex).
+// splat_light_path runs them in vertex order; the per-vertex launch of the GPU computes each vertex's
+// prefix with the same updates, so both give the same bits.
+struct SplatLens {
+  vec3 posVS, posWS;
+};
+
+YPT_HD void splat_lens(const FrameConstants& fc, const uint32_t* point, uint32_t header, SplatLens& out) {
   RandomSampler lr;
   lr.init(point, pure_lens_dim_offset(fc), header);
   float u2[2];
   lr.r2(u2);
   float dx, dy;
   sample_concentric_disk(u2[0], u2[1], dx, dy);
-  vec3 lensPosVS(dx * fc.lensParams[0], dy * fc.lensParams[0], 0.f);
-  vec3 lensPosWS = xform_point(fc.viewToWorld, lensPosVS);
-  LightNode n0 = unpack_node(nodes[0]);
-  float riSum = 0.f;
-  float deferred = accumulate_ri(0.f, n0.pdfBwd, n0.pdfFwd);
-  vec3 prevPos = n0.pos, prevNrm = n0.nrm;
-  float prevFwd = n0.pdfFwd;
-  uint32_t prevFlags = n0.flags;
-  for (uint32_t i = 1; i < count; ++i) {
-    LightNode n = unpack_node(nodes[i]);
-    float u, v;
-    if (project_to_uv(fc, lensPosVS, n.pos, u, v) && (n.flags & LN_DELTA) == 0 && should_contribute(fc, 0u, i + 1u)) {
-      ScatterEval ev;
-      float unused;
-      if (connect_light_node(S, fc, n, i, surfs.get(S, n, i, prevPos, L), lensPosWS, false, L, ev, unused, prevPos, prevNrm) &&
-          !ev.weight.all_equal(0.f)) {
-        // checkVisibility (:70-101): from the vertex to the lens point, t in [0.01, distance]
-        vec3 fromTo = lensPosWS - n.pos;
-        float len = sqrtf(dot(fromTo, fromTo));
-        bool visible = len < 0.0001f || tr.unoccluded(n.pos, fromTo / len, 0.01f, len);
-        if (visible) {
-          bool prevEnv = i == 1 && (n0.flags & LN_ENV) != 0;
-          bool prevPunctual = i == 1 && (n0.flags & LN_PUNCTUAL) != 0;
-          bool nodeExit = (n.flags & LN_SUBSURFACE_EXIT) != 0;  // its backward density is one of area already
-          float pbPrev = (prevEnv || nodeExit) ? ev.pdfBwd : ev.pdfBwd * solid_angle_to_area(prevPos - n.pos, prevNrm);
-          if (prevPunctual) pbPrev = 0.f;
-          vec3 camToNode = n.pos - lensPosWS;
-          float pdfCam = camera_dir_pdf(fc, camToNode);
-          float ri = accumulate_ri(riSum, pbPrev, prevFwd, (prevFlags & LN_SUBSURFACE_EXIT) == 0);
-          ri = accumulate_ri(ri, pdfCam * solid_angle_to_area(camToNode, n.nrm), n.pdfFwd, !nodeExit);
-          float mis = 1.f / (1.f + ri);
-          float geom = 1.f / fmax_(0.0001f, dot(camToNode, camToNode));
-          float importance = safe_div(pdfCam, (float)headerCount);
-          Spectrum c = n.radiance * ev.weight * mis * geom * importance;
-          float su = fmin_(fmax_(u, 0.f), 1.f), sv = fmin_(fmax_(v, 0.f), 1.f);
-          uint32_t tx = (uint32_t)floorf(fmin_(su * (float)W, (float)(W - 1u)));
-          uint32_t ty = (uint32_t)floorf(fmin_(sv * (float)H, (float)(H - 1u)));
-          rec[i - 1].texel = ty * W + tx;
-          rec[i - 1].xyz = spectrum_to_xyz(c, L, S.spec);
-        }
-      }
-    }
+  out.posVS = vec3(dx * fc.lensParams[0], dy * fc.lensParams[0], 0.f);
+  out.posWS = xform_point(fc.viewToWorld, out.posVS);
+}
+
+struct SplatPrefix {
+  float riSum, deferred;
+  vec3 prevPos, prevNrm;
+  float prevFwd;
+  uint32_t prevFlags;
+  YPT_HD void begin(const LightNode& n0) {
+    riSum = 0.f;
+    deferred = accumulate_ri(0.f, n0.pdfBwd, n0.pdfFwd);
+    prevPos = n0.pos;
+    prevNrm = n0.nrm;
+    prevFwd = n0.pdfFwd;
+    prevFlags = n0.flags;
+  }
+  // past vertex n (index >= 1): the state its successor inherits
+  YPT_HD void advance(const LightNode& n) {
     riSum = deferred;
     deferred = accumulate_ri(deferred, n.pdfBwd, n.pdfFwd, (n.flags & LN_SUBSURFACE_EXIT) == 0);
     prevPos = n.pos;
     prevNrm = n.nrm;
     prevFwd = n.pdfFwd;
     prevFlags = n.flags;
+  }
+};
+
+// the splat of light vertex i >= 1 (rec.texel = npix when it adds nothing)
+template <class Tracer, class Surfaces>
+YPT_HD void splat_vertex(const SceneView& S, const FrameConstants& fc, uint32_t headerCount, const Lambdas& L, const SplatLens& lens,
+                         const LightNode& n0, const LightNode& n, uint32_t i, const SplatPrefix& pf, Surfaces& surfs, Tracer& tr,
+                         SplatRecord& rec) {
+  uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
+  rec.texel = W * H;
+  float u, v;
+  if (!(project_to_uv(fc, lens.posVS, n.pos, u, v) && (n.flags & LN_DELTA) == 0 && should_contribute(fc, 0u, i + 1u))) return;
+  ScatterEval ev;
+  float unused;
+  if (!connect_light_node(S, fc, n, i, surfs.get(S, n, i, pf.prevPos, L), lens.posWS, false, L, ev, unused, pf.prevPos, pf.prevNrm) ||
+      ev.weight.all_equal(0.f))
+    return;
+  // checkVisibility (:70-101): from the vertex to the lens point, t in [0.01, distance]
+  vec3 fromTo = lens.posWS - n.pos;
+  float len = sqrtf(dot(fromTo, fromTo));
+  bool visible = len < 0.0001f || tr.unoccluded(n.pos, fromTo / len, 0.01f, len);
+  if (!visible) return;
+  bool prevEnv = i == 1 && (n0.flags & LN_ENV) != 0;
+  bool prevPunctual = i == 1 && (n0.flags & LN_PUNCTUAL) != 0;
+  bool nodeExit = (n.flags & LN_SUBSURFACE_EXIT) != 0;  // its backward density is one of area already
+  float pbPrev = (prevEnv || nodeExit) ? ev.pdfBwd : ev.pdfBwd * solid_angle_to_area(pf.prevPos - n.pos, pf.prevNrm);
+  if (prevPunctual) pbPrev = 0.f;
+  vec3 camToNode = n.pos - lens.posWS;
+  float pdfCam = camera_dir_pdf(fc, camToNode);
+  float ri = accumulate_ri(pf.riSum, pbPrev, pf.prevFwd, (pf.prevFlags & LN_SUBSURFACE_EXIT) == 0);
+  ri = accumulate_ri(ri, pdfCam * solid_angle_to_area(camToNode, n.nrm), n.pdfFwd, !nodeExit);
+  float mis = 1.f / (1.f + ri);
+  float geom = 1.f / fmax_(0.0001f, dot(camToNode, camToNode));
+  float importance = safe_div(pdfCam, (float)headerCount);
+  Spectrum c = n.radiance * ev.weight * mis * geom * importance;
+  float su = fmin_(fmax_(u, 0.f), 1.f), sv = fmin_(fmax_(v, 0.f), 1.f);
+  uint32_t tx = (uint32_t)floorf(fmin_(su * (float)W, (float)(W - 1u)));
+  uint32_t ty = (uint32_t)floorf(fmin_(sv * (float)H, (float)(H - 1u)));
+  rec.texel = ty * W + tx;
+  rec.xyz = spectrum_to_xyz(c, L, S.spec);
+}
+
+// the light subpath in header slot `header`: one record per vertex 1 .. count-1 into rec[0 .. count-1)
+template <class Tracer, class Surfaces>
+YPT_HD void splat_light_path(const SceneView& S, const FrameConstants& fc, const uint32_t* point, uint32_t header,
+                             uint32_t headerCount, float lambda, const LightNodePacked* nodes, uint32_t count, Surfaces& surfs,
+                             Tracer& tr, SplatRecord* rec) {
+  uint32_t npix = (uint32_t)fc.targetTexDimensions[0] * (uint32_t)fc.targetTexDimensions[1];
+  for (uint32_t i = 1; i < count; ++i) rec[i - 1].texel = npix;
+  if (count <= 1) return;
+  Lambdas L = spectral_lambdas(lambda);
+  SplatLens lens;
+  splat_lens(fc, point, header, lens);
+  LightNode n0 = unpack_node(nodes[0]);
+  SplatPrefix pf;
+  pf.begin(n0);
+  for (uint32_t i = 1; i < count; ++i) {
+    LightNode n = unpack_node(nodes[i]);
+    splat_vertex(S, fc, headerCount, L, lens, n0, n, i, pf, surfs, tr, rec[i - 1]);
+    pf.advance(n);
   }
 }
 

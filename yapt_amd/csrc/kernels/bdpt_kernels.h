@@ -222,31 +222,45 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_camera(FrameConstants fc, Dev
   if (features) store_features(features + (size_t)pix * FEATURE_STRIDE, fh);
 }
 
+// Light tracing (t = 1 splats) with one thread per light vertex: thread t takes vertex j + 1 of the
+// light subpath in header slot h (t = h * recPerPath + j) and writes record t. Its MIS prefix is
+// advanced over vertices 1 .. j with the updates splat_light_path makes, so the records are the ones
+// the per-subpath loop makes; a light subpath's vertices no longer wait on each other's visibility
+// rays.
 template <int WIDTH, bool COUNT>
 __global__ __launch_bounds__(64) void k_bdpt_splat(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B, uint32_t ldsDepth) {
   extern __shared__ uint32_t lds[];
-  uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= B.npaths) return;
-  uint32_t npix = (uint32_t)fc.targetTexDimensions[0] * (uint32_t)fc.targetTexDimensions[1];
-  size_t base = (size_t)h * B.recPerPath;
-  uint32_t hc = *B.headerCount;
-  SplatRecord rec[BDPT_MAX_LIGHT_VERTICES - 1];
-  uint32_t n = 0;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B.npaths * B.recPerPath) return;
+  const uint32_t h = t / B.recPerPath, j = t - h * B.recPerPath;
+  const uint32_t npix = (uint32_t)fc.targetTexDimensions[0] * (uint32_t)fc.targetTexDimensions[1];
+  const uint32_t hc = *B.headerCount;
+  SplatRecord rec;
+  rec.texel = npix;
   if (h < hc) {
-    uint32_t p = B.headers[h];
-    n = B.counts[p];
-    GpuTracer<WIDTH, COUNT> tr = make_tracer<WIDTH, COUNT>(S, lds, B, h, ldsDepth);
-    NodeSurfaceTable surfs;
-    surfs.p = B.surf + (size_t)p * BDPT_MAX_LIGHT_VERTICES;
-    splat_light_path(S.sv, fc, point, h, hc, B.lambdas[p], B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES, n, surfs, tr, rec);
-    tr.flush(B.work + 3);
+    const uint32_t p = B.headers[h];
+    const uint32_t i = j + 1u;
+    if (i < B.counts[p]) {
+      const LightNodePacked* nodes = B.nodes + (size_t)p * BDPT_MAX_LIGHT_VERTICES;
+      Lambdas L = spectral_lambdas(B.lambdas[p]);
+      SplatLens lens;
+      splat_lens(fc, point, h, lens);
+      LightNode n0 = unpack_node(nodes[0]);
+      SplatPrefix pf;
+      pf.begin(n0);
+      for (uint32_t k = 1; k < i; ++k) pf.advance(unpack_node(nodes[k]));
+      LightNode n = unpack_node(nodes[i]);
+      GpuTracer<WIDTH, COUNT> tr = make_tracer<WIDTH, COUNT>(S, lds, B, t, ldsDepth);
+      NodeSurfaceTable surfs;
+      surfs.p = B.surf + (size_t)p * BDPT_MAX_LIGHT_VERTICES;
+      splat_vertex(S.sv, fc, hc, L, lens, n0, n, i, pf, surfs, tr, rec);
+      tr.flush(B.work + 3);
+    }
   }
-  for (uint32_t j = 0; j < B.recPerPath; ++j) {
-    bool valid = j + 1u < n && rec[j].texel < npix;
-    B.recKeys[base + j] = valid ? rec[j].texel : npix;
-    B.recVals[base + j] = (uint32_t)(base + j);
-    if (valid) B.recXYZ[base + j] = make_float4(rec[j].xyz.x, rec[j].xyz.y, rec[j].xyz.z, 0.f);
-  }
+  const bool valid = rec.texel < npix;
+  B.recKeys[t] = valid ? rec.texel : npix;
+  B.recVals[t] = t;
+  if (valid) B.recXYZ[t] = make_float4(rec.xyz.x, rec.xyz.y, rec.xyz.z, 0.f);
 }
 
 #ifndef YPT_INSTANCE_TU  // defined once, in the library's host translation unit

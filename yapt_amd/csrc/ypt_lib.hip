@@ -872,7 +872,8 @@ void bdpt_camera_stage(ypt_integrator* h, const DevScene& S, const uint32_t* poi
     hipLaunchKernelGGL((k_bdpt_camera<WIDTH, COUNT, 1>), dim3(groups), dim3(64), lds, s, fc, S, point, B,
                        h->writeFeatures ? h->features.p : (float*)nullptr, d);
   mark_span(h, s, ev);
-  hipLaunchKernelGGL((k_bdpt_splat<WIDTH, COUNT>), dim3(blocks_for(B.npaths, 64)), dim3(64), lds, s, fc, S, point, B, d);
+  hipLaunchKernelGGL((k_bdpt_splat<WIDTH, COUNT>), dim3(blocks_for((uint64_t)B.npaths * B.recPerPath, 64)), dim3(64), lds, s, fc, S,
+                     point, B, d);
   mark_span(h, s, ev);
 }
 
