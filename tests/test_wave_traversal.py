@@ -45,6 +45,16 @@ def test_wave_render_parity_shapes(ylib, oracle, shapes_small, integrator, width
                     options={"wave_ray_below": ALL_WAVE})
 
 
+@pytest.mark.parametrize("width", [2, 8])
+def test_wave_bdpt_parity(ylib, oracle, shapes_small, width):
+    # the bidirectional camera wavefront's extension and connection queues wave-per-ray
+    from yapt_amd import camera
+    scene, cam = shapes_small
+    frame = camera.make_frame(cam, 48, 40, max_depth=8, light_path_vertices=6)
+    _compare_render(ylib, oracle, scene, frame, ylib.YPT_INTEGRATOR_BDPT, passes=2, begin=1, width=width,
+                    options={"wave_ray_below": ALL_WAVE, "bdpt_wavefront": 1})
+
+
 def test_wave_render_parity_punctual_lights(ylib, oracle):
     # occlusion rays without a target triangle (punctual lights) and with one (the area light)
     from yapt_amd import camera, scenes

@@ -783,6 +783,8 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
   Queues Q = h->queues();
   Q.sOrgTmax = h->cSOrg.p; Q.sDirTri = h->cSDir.p; Q.sContrib = h->cSContrib.p; Q.sSlot = h->cSSlot.p;
   Q.workCounters = h->work.p + 8;
+  // short queues wave-per-ray (not with subsurface scenes, whose probes share the stack buffers)
+  Q.waveBelow = h->res->subsurface ? 0u : (uint32_t)h->waveBelow;
   float* feats = h->writeFeatures ? h->features.p : (float*)nullptr;
   uint32_t Bk = (uint32_t)h->blockSize;
   uint32_t pgrid = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(npix, Bk), (uint32_t)((size_t)h->cus * h->persistentWaves * 64 / Bk)));
@@ -798,6 +800,7 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
       YPT_PERSISTENT((k_extend_p<WIDTH, COUNT>), pgrid, Bk, lds, s, S, P, Q, q, d, r, (uint32_t)h->leafShare);
     else
       YPT_PERSISTENT((k_extend_p<WIDTH, false, false>), pgrid, Bk, lds, s, S, P, Q, q, d, r, (uint32_t)h->leafShare);
+    launch_extend_wave<WIDTH>(h, S, P, Q, q);
     if (h->res->subsurface)  // probes in the shading launch, within the stack buffer's 32 workgroups per CU
       hipLaunchKernelGGL((k_bdpt_cshade<1, true>), dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     else if (h->bdptWaves == 1)
@@ -810,6 +813,7 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
       YPT_PERSISTENT((k_shadow_p<WIDTH, COUNT, false, true>), sgrid, Bk, lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
     else
       YPT_PERSISTENT((k_shadow_p<WIDTH, false, false, false>), sgrid, Bk, lds, s, S, P, Q, d, r, (uint32_t)h->leafShare);
+    launch_shadow_wave<WIDTH>(h, S, P, Q);
     q ^= 1;
   }
   hipLaunchKernelGGL(k_bdpt_cfinish, dim3(blocks_for(npix, 256)), dim3(256), 0, s, fc, S, B, C);
