@@ -439,6 +439,18 @@ class Integrator:
         return out
 
 
+def native():
+    """the pybind11 binding of the C ABI (yapt_amd/_yapt_native, built by yapt_amd.build), imported after
+    the library is loaded so that both use one copy of it and of the HIP runtime. Raises when it is
+    missing."""
+    load()
+    try:
+        from . import _yapt_native
+    except ImportError as e:
+        raise YptError("the pybind11 module _yapt_native is not built; run yapt_amd.build.build() (%s)" % e)
+    return _yapt_native
+
+
 def sobol_point(index):
     out = np.zeros(256, np.uint32)
     check(load().ypt_sobol_point(int(index), _uptr(out)))

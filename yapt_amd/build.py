@@ -1,5 +1,6 @@
-"""In-tree build of the gfx950 libraries: the path tracer (libyapt_pt.so) and the denoiser kernels
-(libyapt_denoise.so).
+"""In-tree build of the gfx950 libraries: the path tracer (libyapt_pt.so), the denoiser kernels
+(libyapt_denoise.so) and the pybind11 binding of the path tracer's C ABI (_yapt_native, g++, host
+code only).
 
 Flags: -ffp-contract=off and IEEE-correct fp32 division/sqrt so device arithmetic matches the host
 build bit for bit; packed-fp32 instructions disabled; -O3 for gfx950 only.
@@ -23,6 +24,22 @@ INSTANCES = os.path.join(CSRC, "instances.hip")
 OUT = os.path.join(HERE, "libyapt_pt.so")
 DENOISE_SRC = os.path.join(CSRC, "denoise.hip")
 DENOISE_OUT = os.path.join(HERE, "libyapt_denoise.so")
+NATIVE_SRC = os.path.join(CSRC, "py_native.cpp")
+
+
+def native_module_path():
+    """the pybind11 module yapt_amd/_yapt_native<EXT_SUFFIX>"""
+    import sysconfig
+    return os.path.join(HERE, "_yapt_native" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def _native_cmd(out):
+    """g++ line of the pybind11 binding (host code only), linked against the in-tree libyapt_pt.so"""
+    import sysconfig
+    import pybind11
+    return ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+            "-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"], NATIVE_SRC,
+            "-L" + HERE, "-lyapt_pt", "-Wl,-rpath,$ORIGIN", "-o", out]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -92,6 +109,9 @@ def build(verbose=False):
         _run([HIPCC, "--offload-arch=%s" % ARCH, "-shared", "-fPIC", "-o", OUT + ".tmp"] + [o for _, o, _ in units], verbose)
     os.replace(DENOISE_OUT + ".tmp", DENOISE_OUT)
     os.replace(OUT + ".tmp", OUT)
+    native = native_module_path()
+    _run(_native_cmd(native + ".tmp"), verbose)
+    os.replace(native + ".tmp", native)
     return OUT
 
 

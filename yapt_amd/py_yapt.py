@@ -1055,7 +1055,7 @@ class Renderer:
         state = (kind, self._scene_gen, self._frame_gen, features)
         it = self._integrators.get(kind)
         if it is None:
-            it = self._integrators[kind] = _lib.Integrator(kind, self._device)
+            it = self._integrators[kind] = _lib.native().Integrator(kind, self._device)
             self._committed = None
         if self._committed != state:
             if self._committed is None or self._committed[1] != self._scene_gen or self._committed[0] != kind:
