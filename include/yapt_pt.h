@@ -308,6 +308,15 @@ typedef struct ypt_scene ypt_scene;
 int ypt_scene_create(const ypt_scene_desc* scene, int device, ypt_scene** out);
 /* the device memory is released when no handle or job uses the scene any more */
 int ypt_scene_destroy(ypt_scene* scene);
+/* Moved geometry without a BVH rebuild (the reference rebuilds or updates its TLAS when render-object
+ * transforms change, AccelerationStructureHelper.cpp): `scene` has the same triangles, instances and
+ * alpha-tested triangles as the handle's scene, with new vertex data (positions, normals, tangents,
+ * UVs) and possibly new materials, lights or environment. The scene is re-prepared, its BVH refitted
+ * (same tree, new boxes) and its buffers rewritten in place; the frame constants are re-derived and
+ * the film cleared. The handle must be the scene's only user (YPT_E_INVALID otherwise). */
+int ypt_refit_scene(ypt_integrator* h, const ypt_scene_desc* scene);
+/* BVH builds (1) and refits of the handle's scene */
+int ypt_scene_bvh_counts(ypt_integrator* h, uint32_t* builds, uint32_t* refits);
 int ypt_scene_info(const ypt_scene* scene, uint64_t* device_bytes, uint32_t* triangles);
 /* like ypt_set_scene, without a copy or a BVH build */
 int ypt_set_resident_scene(ypt_integrator* h, ypt_scene* scene);

@@ -5,7 +5,8 @@
 // constants, no textures or environment map). It checks:
 //   * the binned-SAH builder (runtime/bvh.h): in both node layouts every triangle record is reached
 //     by exactly one leaf, the records are a permutation of the triangles, and every box holds the
-//     triangles and boxes below it;
+//     triangles and boxes below it; a refit to the built positions gives the built tree bit for bit,
+//     and a refit after moving an instance holds the same invariants;
 //   * the checker's multithreaded renders (NEE, BPT and BDPT on 4 worker threads, with its BVH), whose
 //     scene preparation runs the library's own preparation code (runtime/prepare.h, luts.h,
 //     rgb2spec.h): every film value finite.
@@ -122,6 +123,32 @@ int main(int argc, char** argv) {
   expect(perm, "triangle records are a permutation of the triangles");
   check_tree<2>(bvh.nodes2, bvh.tris, "BVH2 references");
   check_tree<8>(bvh.nodes8, bvh.tris, "BVH8 references");
+
+  // refit: to unchanged positions it gives the built tree bit for bit; after moving the first
+  // instance's vertices the boxes hold the moved triangles again
+  {
+    ypt::BvhResult same = bvh;
+    ypt::bvh_refit(pos.data(), idx.data(), same);
+    bool identical = same.nodes2.size() == bvh.nodes2.size() && same.nodes8.size() == bvh.nodes8.size() &&
+                     std::memcmp(same.nodes2.data(), bvh.nodes2.data(), bvh.nodes2.size() * sizeof(ypt::Bvh2Node)) == 0 &&
+                     std::memcmp(same.nodes8.data(), bvh.nodes8.data(), bvh.nodes8.size() * sizeof(ypt::Bvh8Node)) == 0 &&
+                     std::memcmp(same.tris.data(), bvh.tris.data(), bvh.tris.size() * sizeof(ypt::TriRecord)) == 0;
+    expect(identical, "refit to the built positions gives the built tree");
+    std::vector<float> moved = pos;
+    std::vector<uint8_t> touched(nv, 0);
+    for (uint32_t t = first[0]; t < first[0] + count[0]; ++t)
+      for (int k = 0; k < 3; ++k) touched[idx[3 * t + k]] = 1;
+    for (uint32_t v = 0; v < nv; ++v)
+      if (touched[v]) { moved[3 * v] += 0.75f; moved[3 * v + 1] -= 0.5f; moved[3 * v + 2] += 2.25f; }
+    ypt::BvhResult re = bvh;
+    ypt::bvh_refit(moved.data(), idx.data(), re);
+    bool records = true;
+    for (const auto& r : re.tris)
+      for (int a = 0; a < 3; ++a) records = records && r.v0[a] == moved[3 * idx[3 * r.id] + a] && r.v2[a] == moved[3 * idx[3 * r.id + 2] + a];
+    expect(records, "refitted triangle records hold the moved vertices");
+    check_tree<2>(re.nodes2, re.tris, "refitted BVH2 references");
+    check_tree<8>(re.nodes8, re.tris, "refitted BVH8 references");
+  }
 
   ypt_scene_desc sd;
   std::memset(&sd, 0, sizeof(sd));

@@ -314,6 +314,8 @@ def load():
     lib.ypt_server_cancel.argtypes = [P, ctypes.c_uint64]
     lib.ypt_server_set_deadline.argtypes = [P, ctypes.c_uint64, ctypes.c_double]
     lib.ypt_server_fused_stats.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+    lib.ypt_refit_scene.argtypes = [P, ctypes.POINTER(SceneDesc)]
+    lib.ypt_scene_bvh_counts.argtypes = [P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
     lib.ypt_clock.restype = ctypes.c_double
     _LIB = lib
     return lib
@@ -327,7 +329,7 @@ EXPORTED_SYMBOLS = [
     "ypt_server_create", "ypt_server_destroy", "ypt_server_submit", "ypt_server_wait", "ypt_server_take",
     "ypt_server_stats", "ypt_clock", "ypt_server_set_option", "ypt_final_color", "ypt_post_params_default",
     "ypt_get_feature_moments", "ypt_get_film_variance", "ypt_trace_closest_noalloc", "ypt_server_cancel",
-    "ypt_server_set_deadline", "ypt_server_fused_stats",
+    "ypt_server_set_deadline", "ypt_server_fused_stats", "ypt_refit_scene", "ypt_scene_bvh_counts",
 ]
 
 
@@ -371,6 +373,17 @@ class Integrator:
     def set_frame(self, frame):
         check(self.lib.ypt_set_frame(self.h, ctypes.byref(frame)))
         self.width, self.height = frame.width, frame.height
+
+    def refit_scene(self, scene):
+        """moved vertex data for the scene's triangles: refits the BVH instead of rebuilding it"""
+        check(self.lib.ypt_refit_scene(self.h, ctypes.byref(scene.desc)))
+        self._scene = scene
+
+    def bvh_counts(self):
+        """(builds, refits) of the handle's scene"""
+        b, r = ctypes.c_uint32(), ctypes.c_uint32()
+        check(self.lib.ypt_scene_bvh_counts(self.h, ctypes.byref(b), ctypes.byref(r)))
+        return b.value, r.value
 
     def clear_film(self):
         check(self.lib.ypt_clear_film(self.h))

@@ -150,6 +150,19 @@ struct Integrator {
     }
     scene = sc;
   }
+  void refit_scene(const py::object& sc) {
+    SceneDescView v(sc);
+    {
+      py::gil_scoped_release nogil;
+      check(ypt_refit_scene(handle(), &v.d));
+    }
+    scene = sc;
+  }
+  py::tuple bvh_counts() {
+    uint32_t b = 0, r = 0;
+    check(ypt_scene_bvh_counts(handle(), &b, &r));
+    return py::make_tuple(b, r);
+  }
   void set_frame(const py::object& frame) {
     const ypt_frame_desc* f = static_cast<const ypt_frame_desc*>(buffer_ptr(frame, sizeof(ypt_frame_desc), 1, "frame"));
     check(ypt_set_frame(handle(), f));
@@ -267,6 +280,8 @@ PYBIND11_MODULE(_yapt_native, m) {
       .def("set_option", &Integrator::set_option)
       .def("set_scene", &Integrator::set_scene)
       .def("set_frame", &Integrator::set_frame)
+      .def("refit_scene", &Integrator::refit_scene, "moved vertex data for the same triangles: BVH refit, no rebuild")
+      .def("bvh_counts", &Integrator::bvh_counts)
       .def("clear_film", &Integrator::clear_film)
       .def("render", &Integrator::render, py::arg("sample_begin"), py::arg("sample_count"))
       .def("synchronize", &Integrator::synchronize)

@@ -306,4 +306,72 @@ class BvhBuilder {
   }
 };
 
+// Refit: new vertex positions for the same triangles (a transform change of render objects) keep
+// the tree and recompute its boxes. Triangle records take their vertices from `pos` by triangle id;
+// nodes are numbered breadth-first, so every child comes after its parent and one pass in reverse
+// order sees children first. A box is the union of its triangles' vertices or of its child node's
+// boxes, as the build makes it, so refitting to unchanged positions gives the built tree bit for bit.
+// The tree's quality decays as objects move away from where it was built; the result stays exact.
+inline void bvh_refit(const float* pos, const uint32_t* idx, BvhResult& b) {
+  for (TriRecord& r : b.tris) {
+    const float* p[3] = {pos + 3 * idx[3 * r.id], pos + 3 * idx[3 * r.id + 1], pos + 3 * idx[3 * r.id + 2]};
+    for (int a = 0; a < 3; ++a) { r.v0[a] = p[0][a]; r.v1[a] = p[1][a]; r.v2[a] = p[2][a]; }
+  }
+  auto leaf_box = [&](uint32_t ref) {
+    Aabb box;
+    uint32_t first = ref & 0x07FFFFFFu, count = ((ref >> 27) & 15u) + 1u;
+    for (uint32_t k = first; k < first + count; ++k) {
+      const TriRecord& r = b.tris[k];
+      box.grow(r.v0); box.grow(r.v1); box.grow(r.v2);
+    }
+    return box;
+  };
+  auto set = [](float* lo_x, float* hi_x, float* lo_y, float* hi_y, float* lo_z, float* hi_z, int c, const Aabb& box) {
+    lo_x[c] = box.lo[0]; hi_x[c] = box.hi[0];
+    lo_y[c] = box.lo[1]; hi_y[c] = box.hi[1];
+    lo_z[c] = box.lo[2]; hi_z[c] = box.hi[2];
+  };
+  for (size_t k = b.nodes2.size(); k-- > 0;) {
+    Bvh2Node& g = b.nodes2[k];
+    for (int c = 0; c < 2; ++c) {
+      uint32_t ref = g.child[c];
+      if (ref == BVH_EMPTY) continue;
+      Aabb box;
+      if (ref & BVH_LEAF_BIT) {
+        box = leaf_box(ref);
+      } else {
+        const Bvh2Node& ch = b.nodes2[ref];
+        for (int d = 0; d < 2; ++d)
+          if (ch.child[d] != BVH_EMPTY) {
+            float lo[3] = {ch.lo_x[d], ch.lo_y[d], ch.lo_z[d]}, hi[3] = {ch.hi_x[d], ch.hi_y[d], ch.hi_z[d]};
+            box.grow(lo); box.grow(hi);
+          }
+      }
+      set(g.lo_x, g.hi_x, g.lo_y, g.hi_y, g.lo_z, g.hi_z, c, box);
+    }
+  }
+  for (size_t k = b.nodes8.size(); k-- > 0;) {
+    Bvh8Node& g = b.nodes8[k];
+    for (int c = 0; c < 8; ++c) {
+      uint32_t ref = g.child[c];
+      if (ref == BVH_EMPTY) continue;
+      Aabb box;
+      if (ref & BVH_LEAF_BIT) {
+        box = leaf_box(ref);
+      } else {
+        const Bvh8Node& ch = b.nodes8[ref];
+        for (int d = 0; d < 8; ++d)
+          if (ch.child[d] != BVH_EMPTY) {
+            float lo[3] = {ch.lo_x[d], ch.lo_y[d], ch.lo_z[d]}, hi[3] = {ch.hi_x[d], ch.hi_y[d], ch.hi_z[d]};
+            box.grow(lo); box.grow(hi);
+          }
+      }
+      set(g.lo_x, g.hi_x, g.lo_y, g.hi_y, g.lo_z, g.hi_z, c, box);
+    }
+  }
+  Aabb all;
+  for (const TriRecord& r : b.tris) { all.grow(r.v0); all.grow(r.v1); all.grow(r.v2); }
+  b.bounds = all;
+}
+
 }  // namespace ypt

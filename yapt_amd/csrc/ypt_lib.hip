@@ -160,6 +160,7 @@ struct ResidentScene {
   DevBuf<uint32_t> dTriAlpha;
   uint32_t stackBound2 = BVH_MAX_DEPTH, stackBound8 = 8 * BVH_MAX_DEPTH;
   bool subsurface = false;  // some material has subsurfaceAmount > 0
+  uint32_t refits = 0;      // ypt_refit_scene calls since the build
   size_t device_bytes() const {
     return dPos.n * 4 + dNrm.n * 4 + dTan.n * 4 + dIdx.n * 4 + dTriInst.n * 4 + dInst.n * sizeof(InstanceGPU) +
            dMats.n * sizeof(MaterialGPU) + dLights.n * sizeof(LightGPU) + dNodes2.n * sizeof(Bvh2Node) +
@@ -1617,6 +1618,59 @@ int ypt_scene_info(const ypt_scene* sc, uint64_t* device_bytes, uint32_t* triang
   if (!sc) return fail(YPT_E_INVALID, "null scene");
   if (device_bytes) *device_bytes = sc->r->device_bytes();
   if (triangles) *triangles = (uint32_t)sc->r->scene.triInst.size();
+  return YPT_OK;
+}
+
+// New vertex data for the same triangles: re-prepares the scene, refits its BVH (no rebuild) and
+// re-uploads it in place. The handle must be the scene's only user.
+int ypt_refit_scene(ypt_integrator* h, const ypt_scene_desc* d) {
+  if (!h || !d) return fail(YPT_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (!h->hasScene) return fail(YPT_E_NOSCENE, "refit needs a scene set before");
+    if (h->res.use_count() > 1) return fail(YPT_E_INVALID, "the scene is shared with other handles or jobs; refit a scene of its own");
+    ResidentScene& r = *h->res;
+    if ((size_t)d->vertex_count * 3 != r.scene.pos.size() || (size_t)d->triangle_count != r.scene.triInst.size())
+      return fail(YPT_E_INVALID, "refit: vertex or triangle count differs from the scene's");
+    PreparedScene ps;
+    prepare_scene(*d, host_tables().r2s, ps);
+    if (ps.idx != r.scene.idx || ps.triInst != r.scene.triInst || ps.triAlpha != r.scene.triAlpha || ps.inst.size() != r.scene.inst.size())
+      return fail(YPT_E_INVALID, "refit: triangles, instances or alpha-tested triangles differ from the scene's");
+    HIP_CHECK(hipSetDevice(h->device));
+    HIP_CHECK(hipStreamSynchronize(h->stream));  // nothing in flight reads the buffers being replaced
+    r.scene = std::move(ps);
+    bvh_refit(r.scene.pos.data(), r.scene.idx.data(), r.bvh);
+    r.subsurface = false;
+    for (const MaterialGPU& m : r.scene.mats) r.subsurface = r.subsurface || m.subsurfaceAmount > 0.f;
+    const PreparedScene& q = r.scene;
+    hipStream_t s = h->stream;
+    r.dPos.upload(q.pos.data(), q.pos.size(), s);
+    r.dNrm.upload(q.nrm.data(), q.nrm.size(), s);
+    if (!q.tan.empty()) r.dTan.upload(q.tan.data(), q.tan.size(), s);
+    r.dInst.upload(q.inst.data(), q.inst.size(), s);
+    r.dMats.upload(q.mats.data(), q.mats.size(), s);
+    if (!q.lights.empty()) r.dLights.upload(q.lights.data(), q.lights.size(), s);
+    r.dNodes2.upload(r.bvh.nodes2.data(), r.bvh.nodes2.size(), s);
+    r.dNodes8.upload(r.bvh.nodes8.data(), r.bvh.nodes8.size(), s);
+    r.dTris.upload(r.bvh.tris.data(), r.bvh.tris.size(), s);
+    if (!q.uv.empty()) r.dUv.upload(q.uv.data(), q.uv.size(), s);
+    if (!q.alphaTris.empty()) r.dAlpha.upload(q.alphaTris.data(), q.alphaTris.size(), s);
+    if (!q.envTexels.empty()) r.dEnv.upload(q.envTexels.data(), q.envTexels.size(), s);
+    r.refits++;
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (h->hasFrame) {  // frame constants depend on the scene's bounds and lights
+      prepare_frame(h->frameDesc, h->res->scene, (uint32_t)h->type, h->fc);
+      clear_film(h);
+    }
+    return YPT_OK;
+  });
+}
+
+// BVH builds and refits of the handle's scene since it was created
+int ypt_scene_bvh_counts(ypt_integrator* h, uint32_t* builds, uint32_t* refits) {
+  if (!h || !builds || !refits) return fail(YPT_E_INVALID, "null argument");
+  if (!h->hasScene) return fail(YPT_E_NOSCENE, "no scene");
+  *builds = 1;
+  *refits = h->res->refits;
   return YPT_OK;
 }
 

@@ -26,7 +26,9 @@ Instancing: each render object references a mesh and has its own transform, and 
 one world-space copy of the mesh's triangles per render object (positions by the object's world
 matrix, normals by its inverse transpose, tangents by the matrix with the handedness flipped and
 the winding reversed under a mirroring transform, as the object-space front-face rule keeps them).
-Traversal then runs over one BVH with no per-instance ray transforms.
+Traversal then runs over one BVH with no per-instance ray transforms. When only transforms change,
+the new world-space copy goes to the library as a refit (ypt_refit_scene): the same tree with new
+boxes, no rebuild.
 """
 import enum
 import math
@@ -219,7 +221,7 @@ class SceneObject:
     def __init__(self, scene, name, on_change=None):
         self._scene = scene
         self._name = name
-        self._transform = Transform(on_change or scene._touch)
+        self._transform = Transform(on_change or scene._touch_transform)
 
     def getName(self):
         return self._name
@@ -822,6 +824,9 @@ class Scene:
     def _touch(self):
         self._renderer._touch()
 
+    def _touch_transform(self):
+        self._renderer._touch_transform()
+
     def createRenderObject(self, name):
         o = RenderObject(self, name)
         self._objects.append(o)
@@ -962,16 +967,21 @@ class Renderer:
         self._scene = None
         self._scene_gen = 0
         self._frame_gen = 0
-        self._committed = None     # (pipeline, scene generation, frame generation, denoise features)
+        self._xform_gen = 0
+        self._committed = None     # (pipeline, scene, frame generations, denoise features, transform generation)
         self._integrators = {}
         self._arrays = None
         self._samples = 0
         self._warned_auto = False
 
-    # generation counters: a scene change re-uploads the scene, a frame change re-sets the frame,
-    # and either restarts the accumulation as the reference's resetRenderOutput does
+    # generation counters: a scene change re-uploads the scene, a frame change re-sets the frame, a
+    # transform change of scene objects refits the BVH to the moved triangles (no rebuild), and each
+    # restarts the accumulation as the reference's resetRenderOutput does
     def _touch(self):
         self._scene_gen += 1
+
+    def _touch_transform(self):
+        self._xform_gen += 1
 
     def _touch_frame(self):
         self._frame_gen += 1
@@ -1052,7 +1062,7 @@ class Renderer:
             raise RuntimeError("no scene: create one with Scene(renderer) or Application")
         kind = self.pipeline()
         features = self._var("Denoise.Mode").selected() == "ON"
-        state = (kind, self._scene_gen, self._frame_gen, features)
+        state = (kind, self._scene_gen, self._frame_gen, features, self._xform_gen)
         it = self._integrators.get(kind)
         if it is None:
             it = self._integrators[kind] = _lib.native().Integrator(kind, self._device)
@@ -1061,6 +1071,12 @@ class Renderer:
             if self._committed is None or self._committed[1] != self._scene_gen or self._committed[0] != kind:
                 self._arrays = self._scene.arrays()
                 it.set_scene(self._arrays)
+            elif self._committed[4] != self._xform_gen:
+                self._arrays = self._scene.arrays()
+                try:
+                    it.refit_scene(self._arrays)   # moved scene objects: the BVH is refitted, not rebuilt
+                except RuntimeError:
+                    it.set_scene(self._arrays)     # a mirroring change reversed windings: other triangles
             it.set_option("first_hit_features", 1 if features else 0)
             it.set_frame(self.frame_desc())
             self._committed = state
