@@ -229,7 +229,9 @@ int ypt_get_stats(ypt_integrator* h, double* out16);
  *  "bvh_width" 2 | 8, "block_size" 64..1024 (multiple of 64), "lds_stack_depth" 1..64,
  *  "passes_in_flight" 0..128 (0 = automatic), "persistent" 0/1 (traversal kernels loop over their
  *  queue), "persistent_waves" 1..64 (waves per CU of those launches), "refill_threshold" 1..64
- *  (idle lanes before a wave fetches new rays), "leaf_share" 1..64 (BVH2 persistent traversal: a
+ *  (idle lanes before a wave fetches new rays), "steps_per_fetch" 1..64 (node steps between those
+ *  checks, 8 by default), "wave_ray_below" (queues shorter than this are traced with one wavefront
+ *  per ray; 0 = never), "leaf_share" 1..64 (BVH2 persistent traversal: a
  *  wave intersects its waiting leaves once 1/leaf_share of its active lanes wait), "shade_waves" 0..1024 (0 = one workgroup per queue
  *  entry), "shade_reg_waves" 2 | 4 | 8 (register budget of the shading kernel, in waves per SIMD),
  *  "graph" 0/1 (replay each batch of passes as a captured hipGraph), "count_traversal" 0/1,

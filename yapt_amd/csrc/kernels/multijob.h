@@ -95,14 +95,16 @@ __global__ void k_extend_pm(FusedArgs A, PathSoA P, Queues Q, int qsel, uint32_t
       if (wf.drained) break;
       continue;
     }
-    bool done = false;
-    if (has) {
-      if constexpr (WIDTH == 2) done = step_bvh2_deferred<0, false, ALPHA>(nodes2, tris, r, st, cnt, leafShare);
-      else done = trav_step<WIDTH, 0, false, ALPHA>(nodes2, nodes8, tris, r, st, cnt);
-    }
-    if (done) {
-      Q.hit[slot] = make_float4(r.best, r.b1, r.b2, __uint_as_float(r.bid));
-      has = false;
+    for (uint32_t k = 0; k < Q.stepsPerFetch; ++k) {  // node steps between refill checks, as in k_extend_p
+      bool done = false;
+      if (has) {
+        if constexpr (WIDTH == 2) done = step_bvh2_deferred<0, false, ALPHA>(nodes2, tris, r, st, cnt, leafShare);
+        else done = trav_step<WIDTH, 0, false, ALPHA>(nodes2, nodes8, tris, r, st, cnt);
+      }
+      if (done) {
+        Q.hit[slot] = make_float4(r.best, r.b1, r.b2, __uint_as_float(r.bid));
+        has = false;
+      }
     }
   }
 }
@@ -201,19 +203,21 @@ __global__ void k_shadow_pm(FusedArgs A, PathSoA P, Queues Q, uint32_t ldsDepth,
       if (wf.drained) break;
       continue;
     }
-    bool done = false;
-    if (has) {
-      if constexpr (WIDTH == 2) done = step_bvh2_deferred<1, false, ALPHA>(nodes2, tris, r, st, cnt, leafShare);
-      else done = trav_step<WIDTH, 1, false, ALPHA>(nodes2, nodes8, tris, r, st, cnt);
-    }
-    if (done) {
-      if (!r.occluded) {
-        uint32_t slot = Q.sSlot[cur];
-        float4 c = Q.sContrib[cur];
-        float4 l = P.light[slot];
-        P.light[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w + c.w);
+    for (uint32_t k = 0; k < Q.stepsPerFetch; ++k) {
+      bool done = false;
+      if (has) {
+        if constexpr (WIDTH == 2) done = step_bvh2_deferred<1, false, ALPHA>(nodes2, tris, r, st, cnt, leafShare);
+        else done = trav_step<WIDTH, 1, false, ALPHA>(nodes2, nodes8, tris, r, st, cnt);
       }
-      has = false;
+      if (done) {
+        if (!r.occluded) {
+          uint32_t slot = Q.sSlot[cur];
+          float4 c = Q.sContrib[cur];
+          float4 l = P.light[slot];
+          P.light[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w + c.w);
+        }
+        has = false;
+      }
     }
   }
 }

@@ -44,6 +44,7 @@ struct Queues {
   unsigned long long* workCounters;  // [0] nodes, [1] tris (extension), [2] nodes, [3] tris (shadow)
   uint32_t* sssStack;  // subsurface probe traversal stacks of the shading launch, [depth][threads]
   uint32_t waveBelow;  // ray and shadow queues shorter than this are traced wave-per-ray (k_extend_w, k_shadow_w)
+  uint32_t stepsPerFetch;  // persistent traversal: node steps between a wave's refill checks (>= 1)
 };
 
 struct FilmBuffers {
@@ -398,14 +399,18 @@ __global__ void k_extend_p(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t l
       if (wf.drained) break;
       continue;
     }
-    bool done = false;
-    if (has) {
-      if constexpr (WIDTH == 2) done = step_bvh2_deferred<0, COUNT, ALPHA>(S.nodes2, S.tris, r, st, cnt, leafShare);
-      else done = trav_step<WIDTH, 0, COUNT, ALPHA>(S.nodes2, S.nodes8, S.tris, r, st, cnt);
-    }
-    if (done) {
-      Q.hit[slot] = make_float4(r.best, r.b1, r.b2, __uint_as_float(r.bid));
-      has = false;
+    // several traversal steps between refill checks: the fetch ballots and the loop's own ballot are
+    // scalar work that a single node step does not amortise
+    for (uint32_t k = 0; k < Q.stepsPerFetch; ++k) {
+      bool done = false;
+      if (has) {
+        if constexpr (WIDTH == 2) done = step_bvh2_deferred<0, COUNT, ALPHA>(S.nodes2, S.tris, r, st, cnt, leafShare);
+        else done = trav_step<WIDTH, 0, COUNT, ALPHA>(S.nodes2, S.nodes8, S.tris, r, st, cnt);
+      }
+      if (done) {
+        Q.hit[slot] = make_float4(r.best, r.b1, r.b2, __uint_as_float(r.bid));
+        has = false;
+      }
     }
   }
   if (COUNT) {
@@ -463,24 +468,28 @@ __global__ void k_shadow_p(DevScene S, PathSoA P, Queues Q, uint32_t ldsDepth, u
       if (wf.drained) break;
       continue;
     }
-    bool done = false;
-    if (has) {
-      if constexpr (WIDTH == 2) done = step_bvh2_deferred<1, COUNT, ALPHA>(S.nodes2, S.tris, r, st, cnt, leafShare);
-      else done = trav_step<WIDTH, 1, COUNT, ALPHA>(S.nodes2, S.nodes8, S.tris, r, st, cnt);
-    }
-    if (done) {
-      if (!r.occluded) {
-        uint32_t slot = Q.sSlot[cur];
-        float4 c = Q.sContrib[cur];
-        if (ATOMIC) {
-          float* l = reinterpret_cast<float*>(P.light + slot);
-          atomicAdd(l, c.x); atomicAdd(l + 1, c.y); atomicAdd(l + 2, c.z); atomicAdd(l + 3, c.w);
-        } else {
-          float4 l = P.light[slot];
-          P.light[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w + c.w);
-        }
+    // several traversal steps between refill checks: the fetch ballots and the loop's own ballot are
+    // scalar work that a single node step does not amortise
+    for (uint32_t k = 0; k < Q.stepsPerFetch; ++k) {
+      bool done = false;
+      if (has) {
+        if constexpr (WIDTH == 2) done = step_bvh2_deferred<1, COUNT, ALPHA>(S.nodes2, S.tris, r, st, cnt, leafShare);
+        else done = trav_step<WIDTH, 1, COUNT, ALPHA>(S.nodes2, S.nodes8, S.tris, r, st, cnt);
       }
-      has = false;
+      if (done) {
+        if (!r.occluded) {
+          uint32_t slot = Q.sSlot[cur];
+          float4 c = Q.sContrib[cur];
+          if (ATOMIC) {
+            float* l = reinterpret_cast<float*>(P.light + slot);
+            atomicAdd(l, c.x); atomicAdd(l + 1, c.y); atomicAdd(l + 2, c.z); atomicAdd(l + 3, c.w);
+          } else {
+            float4 l = P.light[slot];
+            P.light[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w + c.w);
+          }
+        }
+        has = false;
+      }
     }
   }
   if (COUNT) {

@@ -261,6 +261,7 @@ struct ypt_integrator {
   int shadeRegWaves = 4;   // register budget of the shading kernel: 2, 4 or 8 waves per SIMD
   int bdptWaves = 4;       // register budget of the camera-subpath kernel: waves per SIMD (1, 2, 3, 4, 6, 8)
   int waveBelow = 0;       // unidirectional bounces: ray and shadow queues shorter than this are traced wave-per-ray
+  int stepsPerFetch = 8;   // persistent traversal: node steps of a wave between its refill checks
   int cus = 256;           // compute units of the device
 
   // stats of the last render
@@ -317,6 +318,7 @@ struct ypt_integrator {
     Q.ray[0] = rayQ0.p; Q.ray[1] = rayQ1.p; Q.counters = counters.p; Q.hit = hit.p;
     Q.sOrgTmax = sOrg.p; Q.sDirTri = sDir.p; Q.sContrib = sContrib.p; Q.sSlot = sSlot.p;
     Q.ovf = ovf.p; Q.ovfStride = ovfStride; Q.workCounters = work.p;
+    Q.stepsPerFetch = (uint32_t)stepsPerFetch;
     Q.sssStack = sssStack.p;
     return Q;
   }
@@ -1134,6 +1136,9 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
   } else if (n == "refill_threshold") {
     if (value < 1 || value > 64) return fail(YPT_E_INVALID, "refill_threshold must be in [1, 64] (lanes of a wave)");
     h->refill = (int)value;
+  } else if (n == "steps_per_fetch") {
+    if (value < 1 || value > 64) return fail(YPT_E_INVALID, "steps_per_fetch must be in [1, 64]");
+    h->stepsPerFetch = (int)value;
   } else if (n == "wave_ray_below") {
     if (value < 0 || value > (1 << 30)) return fail(YPT_E_INVALID, "wave_ray_below must be in [0, 2^30] (0: thread-per-ray only)");
     h->waveBelow = (int)value;

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 
 DEFAULT = {"bvh_width": 2, "block_size": 64, "lds_stack_depth": 16, "passes_in_flight": 0, "persistent": 1,
            "persistent_waves": 32, "refill_threshold": 16, "leaf_share": 2, "shade_waves": 64, "shade_reg_waves": 4,
-           "wave_ray_below": 0}
+           "wave_ray_below": 0, "steps_per_fetch": 8}
 
 
 def _load(path=None):
@@ -59,6 +59,7 @@ SPACE = {
     "shade_waves": (0, 16, 32, 64, 128),
     "shade_reg_waves": (2, 4, 8),
     "wave_ray_below": (0, 2048, 8192, 32768),
+    "steps_per_fetch": (1, 4, 8, 16),
 }
 
 SPACE_BDPT = {
@@ -70,6 +71,7 @@ SPACE_BDPT = {
     "refill_threshold": (8, 16, 32),
     "leaf_share": (1, 2, 4),
     "lds_stack_depth": (12, 16, 24),
+    "steps_per_fetch": (1, 4, 8, 16),
 }
 
 
