@@ -238,7 +238,10 @@ int ypt_get_stats(ypt_integrator* h, double* out16);
  *  "first_hit_features" 0/1, "profile" 0/1.
  *  Bidirectional integrator: "bdpt_wavefront" 0/1 (camera subpaths as queued kernels rather than
  *  one kernel per texel), "bdpt_waves" 1 | 2 | 3 | 4 | 6 | 8 (register budget of the camera kernels;
- *  the queued shading kernel takes 1, 2 or 4).
+ *  the queued shading kernel takes 1, 2 or 4), "bdpt_split_connections" 0 | 1 | 2 | 4 (queued camera
+ *  subpaths: the connections of each shaded vertex in a launch of their own, one thread per light
+ *  vertex, with that launch's register budget in waves per SIMD; 0 = in the shading launch's loop;
+ *  1 by default; scenes with subsurface materials always use the loop).
  *  None of them changes a result bit: the visible connections of a camera vertex are added in the
  *  order they were made whichever "bdpt_wavefront" form runs. (Scenes with subsurface materials add
  *  the two shadow rays of a subsurface event atomically, which can change the last bits of a sum

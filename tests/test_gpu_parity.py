@@ -265,6 +265,41 @@ def test_bdpt_wavefront_matches_megakernel(ylib, shapes_small):
     assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
 
 
+@pytest.mark.parametrize("split", [1, 2, 4])
+def test_bdpt_split_connections_match(ylib, oracle, shapes_small, split):
+    # connections of each camera vertex in a launch of their own, one thread per light vertex: the
+    # checker's per-sample radiance, and the film of the in-loop form bit for bit (records of
+    # connections that carry nothing add +0 between the others)
+    from yapt_amd import camera
+    scene, cam = shapes_small
+    frame = camera.make_frame(cam, 72, 48, max_depth=8, light_path_vertices=8)
+    _compare_render(ylib, oracle, scene, frame, ylib.YPT_INTEGRATOR_BDPT, passes=2, begin=1,
+                    options={"bdpt_wavefront": 1, "bdpt_split_connections": split})
+    out = []
+    for opts in ({"bdpt_split_connections": 0}, {"bdpt_split_connections": split},
+                 {"bdpt_split_connections": split, "bdpt_waves": 2}):
+        it = ylib.Integrator(ylib.YPT_INTEGRATOR_BDPT, 0)
+        it.set_option("bdpt_wavefront", 1)
+        for k, v in opts.items():
+            it.set_option(k, v)
+        it.set_scene(scene)
+        it.set_frame(frame)
+        it.render(3, 2)
+        out.append((it.last_samples(), it.film()))
+        it.close()
+    for la, fa in out[1:]:
+        assert np.array_equal(la.view(np.uint32), out[0][0].view(np.uint32))
+        assert np.array_equal(fa.view(np.uint32), out[0][1].view(np.uint32))
+
+
+def test_bdpt_split_connections_is_validated(ylib):
+    it = ylib.Integrator(ylib.YPT_INTEGRATOR_BDPT, 0)
+    with pytest.raises(Exception):
+        it.set_option("bdpt_split_connections", 3)
+    it.set_option("bdpt_split_connections", 0)
+    it.close()
+
+
 @pytest.mark.gpu
 def test_bdpt_light_stage_overlap_matches_one_stream(ylib, shapes_small):
     # light subpaths of the next pass on a second stream (default) or everything on one stream

@@ -996,6 +996,73 @@ YPT_HD bool connection_prepare(const SceneView& S, const FrameConstants& fc, uin
   return true;
 }
 
+// The light-vertex context that the connection to light vertex j inherits from vertices 0 .. j-1
+// (LightVertexContext, advanced as the connection loop of traceCameraRays walks the subpath).
+struct LightPrefix {
+  LightVertexCtx lv;
+  float deferred;
+  YPT_HD void begin() {
+    lv.riSum = 0.f;
+    lv.prevFlags = 0;
+    lv.originFlags = 0;
+    lv.prevPos = vec3(0.f);
+    lv.prevNrm = vec3(0.f);
+    lv.prevFwdMIS = 0.f;
+    deferred = 0.f;
+  }
+  // past light vertex n (index j): the context of vertex j + 1
+  YPT_HD void advance(const LightNode& n, uint32_t j) {
+    lv.riSum = deferred;
+    deferred = accumulate_ri(deferred, n.pdfBwd, n.pdfFwd, (n.flags & LN_SUBSURFACE_EXIT) == 0);
+    lv.prevNrm = n.nrm;
+    lv.prevPos = n.pos;
+    lv.prevFwdMIS = n.pdfFwd;
+    lv.prevFlags = n.flags;
+    if (j == 0) lv.originFlags = n.flags;
+  }
+};
+
+// One step of the connection loop: camera vertex cv (camera vertex index k, surface ctx, path state
+// rs) to light vertex n = j of the subpath, whose inherited context is lv. The direction cv.wi is set
+// here on the copy. False when the strategy does not contribute or the connection carries nothing.
+template <class Surfaces>
+YPT_HD bool connect_camera_to_node(const SceneView& S, const FrameConstants& fc, uint32_t k, CameraVertexCtx cv,
+                                   const SurfaceContext& ctx, const BdptState& rs, const Lambdas& L, const LightNode& n, uint32_t j,
+                                   const LightVertexCtx& lv, Surfaces& surfs, PendingConnection& pc) {
+  if ((n.flags & LN_DELTA) != 0 || !should_contribute(fc, k + 1u, j + 1u)) return false;
+  if (j == 0 && infinite_node(n.flags)) {
+    cv.wi = normalize(-infinite_node_direction(S, n));
+  } else {
+    vec3 toNode = n.pos - cv.pos;
+    float dist = length(toNode);
+    cv.wi = normalize(toNode / dist);
+  }
+  const LightNodeSurface* ls = j > 0 ? surfs.get(S, n, j, lv.prevPos, L) : nullptr;
+  return connection_prepare(S, fc, k, cv, ctx, n, j, ls, lv, rs, L, pc);
+}
+
+// A camera vertex whose connections a separate launch makes, one thread per light vertex (the
+// bidirectional camera wavefront): the vertex context, its surface and path state as the connection
+// loop would read them, and how many light vertices to connect to (0: none).
+struct DeferredConnections {
+  CameraVertexCtx cv;
+  SurfaceContext ctx;
+  BdptState rs;
+  uint32_t k;
+  uint32_t count;
+};
+
+// where camera_vertex's connections go: made in its own loop (ConnectInLoop) or left, with the
+// vertex context, to a separate launch (ConnectLater)
+struct ConnectInLoop {
+  static constexpr bool later = false;
+  DeferredConnections* d = nullptr;
+};
+struct ConnectLater {
+  static constexpr bool later = true;
+  DeferredConnections* d;
+};
+
 // ---------------------------------------------------------------- camera subpath
 // traceCameraRays (BDPTTraceCameraPath.comp:40-432) as a resumable state machine: camera_begin sets
 // up texel (px, py), camera_vertex consumes the traced ray's hit (or miss) and does one iteration of
@@ -1049,10 +1116,11 @@ YPT_HD void camera_begin(const FrameConstants& fc, const uint32_t* point, uint32
 
 // A vertex is the traced ray's hit or miss, or (c.pendingExit) the exit point of the subsurface
 // event taken at the previous vertex; `probe` traces the subsurface probe rays.
-template <class Sink, class Surfaces, class Probe>
+template <class Sink, class Surfaces, class Probe, class Connect = ConnectInLoop>
 YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const uint32_t* point, CameraPath& c, bool hit,
                           const Hit& h, const LightNodePacked* nodes, uint32_t nodeCount, Surfaces& surfs, Sink& sink,
-                          FirstHitFeatures* fh, Probe& probe) {
+                          FirstHitFeatures* fh, Probe& probe, Connect connect = Connect()) {
+  if constexpr (Connect::later) connect.d->count = 0;
   RandomSampler rand;
   rand.point = point;
   rand.dim = c.dim;
@@ -1115,36 +1183,22 @@ YPT_HD void camera_vertex(const SceneView& S, const FrameConstants& fc, const ui
     c.dim = rand.dim;
     return;
   }
-  if (!delta) {
-    LightVertexCtx lv;
-    lv.riSum = 0.f;
-    lv.prevFlags = 0;
-    lv.originFlags = 0;
-    lv.prevPos = vec3(0.f);
-    lv.prevNrm = vec3(0.f);
-    lv.prevFwdMIS = 0.f;
-    float deferredLight = 0.f;
+  if constexpr (Connect::later) {  // connections made by a separate launch from what the loop would read
+    if (!delta) {
+      connect.d->cv = cv;
+      connect.d->ctx = v.ctx;
+      connect.d->rs = c.rs;
+      connect.d->k = k;
+      connect.d->count = nodeCount;
+    }
+  } else if (!delta) {
+    LightPrefix pre;
+    pre.begin();
     for (uint32_t j = 0; j < nodeCount; ++j) {
       LightNode n = unpack_node(nodes[j]);
-      if ((n.flags & LN_DELTA) == 0 && should_contribute(fc, k + 1u, j + 1u)) {
-        if (j == 0 && infinite_node(n.flags)) {
-          cv.wi = normalize(-infinite_node_direction(S, n));
-        } else {
-          vec3 toNode = n.pos - v.pos;
-          float dist = length(toNode);
-          cv.wi = normalize(toNode / dist);
-        }
-        const LightNodeSurface* ls = j > 0 ? surfs.get(S, n, j, lv.prevPos, L) : nullptr;
-        PendingConnection pc;
-        if (connection_prepare(S, fc, k, cv, v.ctx, n, j, ls, lv, c.rs, L, pc)) sink(c, pc);
-      }
-      lv.riSum = deferredLight;
-      deferredLight = accumulate_ri(deferredLight, n.pdfBwd, n.pdfFwd, (n.flags & LN_SUBSURFACE_EXIT) == 0);
-      lv.prevNrm = n.nrm;
-      lv.prevPos = n.pos;
-      lv.prevFwdMIS = n.pdfFwd;
-      lv.prevFlags = n.flags;
-      if (j == 0) lv.originFlags = n.flags;
+      PendingConnection pc;
+      if (connect_camera_to_node(S, fc, k, cv, v.ctx, c.rs, L, n, j, pre.lv, surfs, pc)) sink(c, pc);
+      pre.advance(n, j);
     }
   }
   c.throughput = cv.throughput;

@@ -310,7 +310,27 @@ struct CamSoA {
   float4* conn;        // connection records: connCap per slot
   uint32_t* pending;   // records of the slot's last shaded vertices not yet added to its total
   uint32_t connCap;
+  DeferredConnections* defer;  // per slot: the shaded vertex whose connections k_bdpt_connect makes
+  uint32_t* connList;          // k_bdpt_connect's work: slot * 16 + light vertex index, length in
+                               // Q.counters[6 + qsel] of the shading launch that wrote it
 };
+
+// Reserve n entries per lane (n may differ per lane, 0 included): one atomic per wave, the lane's
+// first entry from an exclusive scan over the wave.
+__device__ __forceinline__ uint32_t wave_append_n(uint32_t* counter, uint32_t n) {
+  const uint32_t lane = __lane_id();
+  uint32_t incl = n;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t v = (uint32_t)__shfl_up((int)incl, off);
+    if (lane >= (uint32_t)off) incl += v;
+  }
+  const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+  uint32_t base = 0;
+  if (lane == 63 && total > 0) base = atomicAdd(counter, total);
+  base = (uint32_t)__shfl((int)base, 63);
+  return base + incl - n;
+}
 
 // adds the slot's pending connection records to its total, in the order they were queued
 __device__ __forceinline__ void fold_connections(const CamSoA& C, uint32_t s, Spectrum& total) {
@@ -405,7 +425,7 @@ __global__ void k_bdpt_cstart(FrameConstants fc, const uint32_t* point, BdptBuff
   uint32_t W = (uint32_t)fc.targetTexDimensions[0], H = (uint32_t)fc.targetTexDimensions[1];
   uint32_t npix = W * H;
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) { Q.counters[0] = npix; Q.counters[1] = 0; Q.counters[2] = 0; Q.counters[4] = 0; }
+  if (i == 0) { Q.counters[0] = npix; Q.counters[1] = 0; Q.counters[2] = 0; Q.counters[4] = 0; Q.counters[6] = 0; Q.counters[7] = 0; }
   if (i >= npix) return;
   uint32_t dx, dy;
   slot_position(W, H, i, dx, dy);
@@ -461,17 +481,23 @@ struct QueueSink {
 // SSS: the scene has subsurface materials; probes then run here (stacks in Q.sssStack) and the exit
 // vertex of a subsurface event is shaded in the same launch, so a path can queue the connections of
 // two vertices
-template <int MINW, bool SSS = false>
+// SPLIT: the vertex's connections are left to k_bdpt_connect (its context goes to C.defer[slot]).
+template <int MINW, bool SSS = false, bool SPLIT = false>
 __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B,
                                                           CamSoA C, Queues Q, int qsel, int first, float* features) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;
+  // counters[6 + qsel]: connection list of this launch (zero since the launch before the last one);
+  // counters[6 + (qsel ^ 1)]: the next launch's, read by the connection launch before this one
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Q.counters[4] = 0;
+    if constexpr (SPLIT) Q.counters[6 + (qsel ^ 1)] = 0;
+  }
   const uint32_t n = Q.counters[qsel];
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t W = (uint32_t)fc.targetTexDimensions[0];
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool cont = false;
-    uint32_t slot = 0;
+    uint32_t slot = 0, nconn = 0;
     if (i < n) {
       slot = first ? i : Q.ray[qsel][i];
       CameraPath c;
@@ -504,10 +530,14 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
         if (c.pendingExit && c.alive) camera_vertex(S.sv, fc, point, c, false, h, nodes, count, surfs, sink, nullptr, pr);
       } else {
         NoSubsurface none;
-        camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none);  // features written below only when requested
+        if constexpr (SPLIT)
+          camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none, ConnectLater{C.defer + slot});
+        else
+          camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none);  // features written below only when requested
       }
       store_cam(C, slot, c, lp, count, nvolIn > 0 || c.rs.n > 0);
-      C.pending[slot] = sink.count;
+      if constexpr (SPLIT) nconn = C.defer[slot].count;
+      C.pending[slot] = SPLIT ? nconn : sink.count;
       cont = c.alive;
       if (features && first && fh.hit) {
         uint32_t q = slot_texel(fc, B, slot);
@@ -516,6 +546,55 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
     }
     uint32_t ri = wave_append(&Q.counters[qsel ^ 1], cont);
     if (cont) Q.ray[qsel ^ 1][ri] = slot;
+    if constexpr (SPLIT) {
+      uint32_t ci = wave_append_n(&Q.counters[6 + qsel], nconn);
+      for (uint32_t j = 0; j < nconn; ++j) C.connList[ci + j] = slot * BDPT_MAX_LIGHT_VERTICES + j;
+    }
+  }
+}
+
+// The connections of the vertices that k_bdpt_cshade<..., SPLIT> shaded: one thread per entry of
+// its list, (slot, light vertex j) for every j below the light subpath's vertex count, so the
+// connections of one camera vertex run side by side instead of in one thread's loop. Thread j advances the light-vertex context over vertices
+// 0 .. j-1 as the loop does and makes connection j with the loop's own step
+// (connect_camera_to_node). Connection j writes record j of the slot (zero, filled by the occlusion
+// kernel when unoccluded); the next shading launch adds records 0 .. count-1 in order, the same sum
+// as the loop's records with zero terms between them (the totals are never -0, so a +0 term changes
+// no bit).
+template <int MINW>
+__global__ __launch_bounds__(64, MINW) void k_bdpt_connect(FrameConstants fc, DevScene S, BdptBuffers B, CamSoA C, Queues Q, int qsel) {
+  constexpr uint32_t PER = BDPT_MAX_LIGHT_VERTICES;
+  const uint32_t n = Q.counters[6 + qsel];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint32_t t = base + threadIdx.x;
+    bool valid = false;
+    PendingConnection pc;
+    uint32_t rec = 0;
+    if (t < n) {
+      const uint32_t v = C.connList[t];
+      const uint32_t slot = v / PER, j = v % PER;
+      rec = slot * C.connCap + j;
+      C.conn[rec] = make_float4(0.f, 0.f, 0.f, 0.f);
+      const DeferredConnections& d = C.defer[slot];
+      const uint32_t lp = C.st1[slot].z;
+      const LightNodePacked* nodes = B.nodes + (size_t)lp * PER;
+      NodeSurfaceTable surfs;
+      surfs.p = B.surf + (size_t)lp * PER;
+      Lambdas L = spectral_lambdas(C.orgLambda[slot].w);
+      LightPrefix pre;
+      pre.begin();
+      for (uint32_t i = 0; i < j; ++i) pre.advance(unpack_node(nodes[i]), i);
+      LightNode nd = unpack_node(nodes[j]);
+      valid = connect_camera_to_node(S.sv, fc, d.k, d.cv, d.ctx, d.rs, L, nd, j, pre.lv, surfs, pc);
+    }
+    uint32_t si = wave_append(&Q.counters[2], valid);
+    if (valid) {
+      Q.sOrgTmax[si] = make_float4(pc.origin.x, pc.origin.y, pc.origin.z, pc.tmax);
+      Q.sDirTri[si] = make_float4(pc.dir.x, pc.dir.y, pc.dir.z, __uint_as_float(pc.tri));
+      Q.sContrib[si] = make_float4(pc.contrib.s[0], pc.contrib.s[1], pc.contrib.s[2], pc.contrib.s[3]);
+      Q.sSlot[si] = rec;
+    }
   }
 }
 

@@ -74,7 +74,12 @@ static constexpr int YPT_INST_GROUPS = 11;
   EXT __global__ void k_bdpt_cshade<1, false>(YPT_CSHADE_ARGS);        \
   EXT __global__ void k_bdpt_cshade<2, false>(YPT_CSHADE_ARGS);        \
   EXT __global__ void k_bdpt_cshade<4, false>(YPT_CSHADE_ARGS);        \
-  EXT __global__ void k_bdpt_cshade<1, true>(YPT_CSHADE_ARGS);
+  EXT __global__ void k_bdpt_cshade<1, true>(YPT_CSHADE_ARGS);        \
+  EXT __global__ void k_bdpt_cshade<1, false, true>(YPT_CSHADE_ARGS);  \
+  EXT __global__ void k_bdpt_cshade<2, false, true>(YPT_CSHADE_ARGS);  \
+  EXT __global__ void k_bdpt_connect<1>(FrameConstants, DevScene, BdptBuffers, CamSoA, Queues, int); \
+  EXT __global__ void k_bdpt_connect<2>(FrameConstants, DevScene, BdptBuffers, CamSoA, Queues, int); \
+  EXT __global__ void k_bdpt_connect<4>(FrameConstants, DevScene, BdptBuffers, CamSoA, Queues, int);
 
 // kernels of the server's fused steps (several jobs per launch)
 #define YPT_FUSED_TRAV_ARGS FusedArgs, PathSoA, Queues, int, uint32_t, uint32_t, uint32_t

@@ -217,6 +217,8 @@ struct ypt_integrator {
   DevBuf<float> cDeferred;
   DevBuf<uint32_t> cSSlot, cPending;
   DevBuf<float4> cConn;  // connection records, connCap per texel
+  DevBuf<DeferredConnections> cDefer;  // the shaded vertex per texel whose connections the connection launch makes
+  DevBuf<uint32_t> cConnList;          // the connection launch's work list
   int bdptWavefront = 1;
   size_t bSortBytes = 0;
 
@@ -260,6 +262,7 @@ struct ypt_integrator {
   int shadeWaves = 64;     // waves per CU of the shading launch (0: one workgroup per queue entry)
   int shadeRegWaves = 4;   // register budget of the shading kernel: 2, 4 or 8 waves per SIMD
   int bdptWaves = 4;       // register budget of the camera-subpath kernel: waves per SIMD (1, 2, 3, 4, 6, 8)
+  int bdptSplit = 1;       // camera wavefront: connections in their own launch (its waves per SIMD: 1, 2, 4; 0 = in the shading launch)
   int waveBelow = 0;       // unidirectional bounces: ray and shadow queues shorter than this are traced wave-per-ray
   int stepsPerFetch = 8;   // persistent traversal: node steps of a wave between its refill checks
   int cus = 256;           // compute units of the device
@@ -760,6 +763,10 @@ void ensure_bdpt(ypt_integrator* h) {
     size_t nsh = (size_t)npix * h->fc.lightPathVertices * (h->res->subsurface ? 2u : 1u);
     h->cSOrg.ensure(nsh); h->cSDir.ensure(nsh); h->cSContrib.ensure(nsh); h->cSSlot.ensure(nsh);
     h->cConn.ensure(nsh); h->cPending.ensure(npix);
+    if (h->bdptSplit && !h->res->subsurface) {
+      h->cDefer.ensure(npix);
+      h->cConnList.ensure((size_t)npix * h->fc.lightPathVertices);
+    }
     ensure_slots(h, npix);
     ensure_overflow(h);
   }
@@ -778,6 +785,12 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
   C.volIor = h->cVolIor.p; C.volMat = h->cVolMat.p; C.volAbs = h->cVolAbs.p;
   C.conn = h->cConn.p; C.pending = h->cPending.p;
   C.connCap = fc.lightPathVertices * (h->res->subsurface ? 2u : 1u);
+  // connections in a launch of their own, one thread per (vertex, light vertex) (not with subsurface
+  // scenes, whose shading launch can shade two vertices of a path)
+  const bool split = h->bdptSplit != 0 && !h->res->subsurface;
+  C.defer = split ? h->cDefer.p : nullptr;
+  C.connList = split ? h->cConnList.p : nullptr;
+  uint32_t xgrid = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for((size_t)npix * fc.lightPathVertices, 64), (uint32_t)h->cus * 32u));
   PathSoA P;
   std::memset(&P, 0, sizeof(P));
   // the occlusion kernel adds each unoccluded connection to its own record (one ray per record, so
@@ -806,7 +819,18 @@ void bdpt_camera_wavefront(ypt_integrator* h, const DevScene& S, const uint32_t*
     launch_extend_wave<WIDTH>(h, S, P, Q, q);
     if (h->res->subsurface)  // probes in the shading launch, within the stack buffer's 32 workgroups per CU
       hipLaunchKernelGGL((k_bdpt_cshade<1, true>), dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
-    else if (h->bdptWaves == 1)
+    else if (split) {
+      if (h->bdptWaves == 1)
+        hipLaunchKernelGGL((k_bdpt_cshade<1, false, true>), dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
+      else
+        hipLaunchKernelGGL((k_bdpt_cshade<2, false, true>), dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
+      if (h->bdptSplit == 1)
+        hipLaunchKernelGGL(k_bdpt_connect<1>, dim3(xgrid), dim3(64), 0, s, fc, S, B, C, Q, q);
+      else if (h->bdptSplit == 2)
+        hipLaunchKernelGGL(k_bdpt_connect<2>, dim3(xgrid), dim3(64), 0, s, fc, S, B, C, Q, q);
+      else
+        hipLaunchKernelGGL(k_bdpt_connect<4>, dim3(xgrid), dim3(64), 0, s, fc, S, B, C, Q, q);
+    } else if (h->bdptWaves == 1)
       hipLaunchKernelGGL(k_bdpt_cshade<1>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
     else if (h->bdptWaves == 2)
       hipLaunchKernelGGL(k_bdpt_cshade<2>, dim3(cgrid), dim3(64), 0, s, fc, S, point, B, C, Q, q, k == 0 ? 1 : 0, feats);
@@ -1124,6 +1148,9 @@ int ypt_set_option(ypt_integrator* h, const char* name, int64_t value) {
   } else if (n == "shade_reg_waves") {
     if (value != 2 && value != 4 && value != 8) return fail(YPT_E_INVALID, "shade_reg_waves must be 2, 4 or 8");
     h->shadeRegWaves = (int)value;
+  } else if (n == "bdpt_split_connections") {
+    if (value != 0 && value != 1 && value != 2 && value != 4) return fail(YPT_E_INVALID, "bdpt_split_connections must be 0, 1, 2 or 4");
+    h->bdptSplit = (int)value;
   } else if (n == "bdpt_waves") {
     if (value < 1 || value > 8 || value == 5 || value == 7) return fail(YPT_E_INVALID, "bdpt_waves must be 1, 2, 3, 4, 6 or 8");
     h->bdptWaves = (int)value;

@@ -66,6 +66,7 @@ SPACE_BDPT = {
     "bvh_width": (2, 8),
     "bdpt_wavefront": (1, 0),
     "bdpt_waves": (1, 2, 3, 4, 6),
+    "bdpt_split_connections": (0, 1, 2),
     "block_size": (64, 128, 256),
     "persistent_waves": (16, 24, 32),
     "refill_threshold": (8, 16, 32),
