@@ -136,8 +136,8 @@ def dominant_kernel(w, integrator):
     launches = st_prof[10]
     if bdpt:   # camera-subpath stage: traversal bytes of its extension and connection rays
         nbytes = st_cnt[6] * NODE_BYTES[width] + st_cnt[7] * TRI_BYTES
-        kernel = ("camera-subpath stage: k_extend_p<%d> + k_bdpt_cshade + k_shadow_p<%d> (extension and connection rays)"
-                  % (width, width) if w["cfg"].get("bdpt_wavefront", 1) else "k_bdpt_camera<%d>" % width)
+        kernel = ("camera-subpath stage: k_extend_p<%d> + k_bdpt_cshade + k_bdpt_connect + k_shadow_p<%d> (extension and "
+                  "connection rays)" % (width, width) if w["cfg"].get("bdpt_wavefront", 1) else "k_bdpt_camera<%d>" % width)
         key = "camera-bdpt"
     else:
         nbytes = st_cnt[6] * NODE_BYTES[width] + st_cnt[7] * TRI_BYTES + st_cnt[0] * RAY_BYTES
