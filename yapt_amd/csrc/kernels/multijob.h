@@ -109,20 +109,28 @@ __global__ void k_extend_pm(FusedArgs A, PathSoA P, Queues Q, int qsel, uint32_t
   }
 }
 
-// shading of a queue of several jobs' paths (k_shade without subsurface probes or first-hit features)
+// shading of a queue of several jobs' paths (k_shade without subsurface probes or first-hit features);
+// misses are finished during the walk over the queue, hits shaded in full waves (HitCompactor)
 template <int MINW>
 __global__ __launch_bounds__(64, MINW) void k_shade_m(FusedArgs A, PathSoA P, Queues Q, int qsel) {
+  __shared__ uint32_t pend[128];
   if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;  // fetch counter of the next extension launch
-  const uint32_t n = Q.counters[qsel];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    uint32_t i = base + threadIdx.x;
+  HitCompactor hc;
+  hc.n = Q.counters[qsel];
+  hc.stride = gridDim.x * blockDim.x;
+  hc.base = blockIdx.x * blockDim.x;
+  hc.npend = 0;
+  uint32_t slot = 0;
+  bool act = false;
+  auto miss = [&](uint32_t s) {
+    const FusedJob& J = A.jobs[A.job_of(s)];
+    finish_miss(J.S.sv, J.fc, P, s);
+  };
+  while (hc.next(Q, qsel, pend, miss, slot, act)) {
     ShadowRay sr;
     sr.valid = false;
     bool cont = false;
-    uint32_t slot = 0;
-    if (i < n) {
-      slot = Q.ray[qsel][i];
+    if (act) {
       uint32_t j = A.job_of(slot);
       const FusedJob& J = A.jobs[j];
       const uint32_t* point = J.points + (size_t)((slot - A.slotBase[j]) / J.npix) * YPT_SOBOL_POINT_DIMS;
@@ -132,13 +140,9 @@ __global__ __launch_bounds__(64, MINW) void k_shade_m(FusedArgs A, PathSoA P, Qu
       float4 hv = Q.hit[slot];
       Hit h;
       h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.tri = __float_as_uint(hv.w);
-      if (h.tri == NO_HIT_ID) {
-        shade_miss(J.S.sv, J.fc, ps);
-      } else {
-        FirstHitFeatures fh;
-        fh.hit = 0u;
-        shade_hit(J.S.sv, J.fc, point, J.nee != 0u, ps, h, sr, &fh);
-      }
+      FirstHitFeatures fh;
+      fh.hit = 0u;
+      shade_hit(J.S.sv, J.fc, point, J.nee != 0u, ps, h, sr, &fh);
       after_segment(point, ps);
       cont = ps.rayState == RAY_ALIVE && ps.pathLength < J.fc.maxRayDepth;
       store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);

@@ -194,6 +194,64 @@ __global__ void k_extend(DevScene S, PathSoA P, Queues Q, int qsel, uint32_t lds
   }
 }
 
+// A path whose extension ray left the scene: shade_miss and after_segment on the fields they use
+// (hero wavelength, direction, throughput, light, path length, ray state); the rest of the slot is
+// unchanged, as store_path would write it back.
+__device__ __forceinline__ void finish_miss(const SceneView& S, const FrameConstants& fc, const PathSoA& P, uint32_t s) {
+  float4 a = P.orgLambda[s], b = P.dirPdf[s], t = P.thr[s], l = P.light[s];
+  uint4 u0 = P.st0[s], u1 = P.st1[s];
+  PathState ps;
+  ps.lambda = a.w;
+  ps.dir = vec3(b.x, b.y, b.z);
+  ps.throughput = Spectrum(t.x, t.y, t.z, t.w);
+  ps.totalLight = Spectrum(l.x, l.y, l.z, l.w);
+  ps.pathLength = u0.w;
+  ps.rayState = u1.x;
+  shade_miss(S, fc, ps);
+  ps.pathLength += 1;  // after_segment of a finished path
+  P.light[s] = make_float4(ps.totalLight.s[0], ps.totalLight.s[1], ps.totalLight.s[2], ps.totalLight.s[3]);
+  P.st0[s] = make_uint4(u0.x, u0.y, u0.z, ps.pathLength);
+  P.st1[s] = make_uint4(ps.rayState, u1.y, u1.z, 0u);
+}
+
+// Hit compaction for the shading launches. Rays that left the scene cost a few loads and stores,
+// and with them in the same wave most lanes of the surface shading would idle. A wave walks the
+// queue 64 entries at a time, finishes the misses at once, and parks the slots of hits in LDS until
+// 64 of them are waiting; it then shades those 64 with every lane busy. The park buffer holds at
+// most 63 + 64 entries. Each workgroup is one wave, so the barriers only order its own LDS
+// accesses. `miss(slot)` finishes a miss. Returns false when the queue is exhausted and nothing is
+// parked; otherwise `slot` holds this lane's hit and `act` says whether the lane has one (fewer than
+// 64 only at the end).
+struct HitCompactor {
+  uint32_t base, npend, n, stride;
+  template <class Miss>
+  __device__ __forceinline__ bool next(const Queues& Q, int qsel, uint32_t* pend, Miss&& miss, uint32_t& slot, bool& act) {
+    const uint32_t lane = threadIdx.x;
+    while (npend < 64 && base < n) {
+      uint32_t i = base + lane;
+      bool hit = false;
+      uint32_t s = 0;
+      if (i < n) {
+        s = Q.ray[qsel][i];
+        hit = __float_as_uint(Q.hit[s].w) != NO_HIT_ID;
+        if (!hit) miss(s);
+      }
+      unsigned long long m = __ballot(hit ? 1 : 0);
+      if (hit) pend[npend + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = s;
+      npend += (uint32_t)__popcll(m);
+      base += stride;
+    }
+    if (npend == 0) return false;
+    __syncthreads();
+    uint32_t take = npend < 64 ? npend : 64;
+    npend -= take;
+    act = lane < take;
+    slot = act ? pend[npend + lane] : 0u;
+    __syncthreads();
+    return true;
+  }
+};
+
 // Grid-stride over the ray queue: a launch sized for the device, not for the slot count, so the
 // short queues of deep bounces cost little. The loop bound is uniform across each workgroup, so
 // every wave reaches the queue appends with all of its lanes.
@@ -217,21 +275,26 @@ struct GpuProbe {
 // SSS: the scene has subsurface materials. Probe rays then run inside the shading launch, each
 // thread with its own stack column in Q.sssStack, and a path may hand two shadow rays (vertex and
 // subsurface exit) to the shadow queue, which is sized for that and applied atomically.
+// Misses are finished during the walk over the queue (HitCompactor); the loop body shades hits only.
 template <bool NEE, int MINW, bool SSS = false>
 __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene S, const uint32_t* points, uint32_t npix,
                                                     PathSoA P, Queues Q, FilmBuffers F, int qsel, int writeFeatures) {
+  __shared__ uint32_t pend[128];
   if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;  // fetch counter of the next extension launch
-  const uint32_t n = Q.counters[qsel];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    uint32_t i = base + threadIdx.x;
+  HitCompactor hc;
+  hc.n = Q.counters[qsel];
+  hc.stride = gridDim.x * blockDim.x;
+  hc.base = blockIdx.x * blockDim.x;
+  hc.npend = 0;
+  uint32_t slot = 0;
+  bool act = false;
+  auto miss = [&](uint32_t s) { finish_miss(S.sv, fc, P, s); };
+  while (hc.next(Q, qsel, pend, miss, slot, act)) {
     ShadowRay sr, sr2;
     sr.valid = false;
     sr2.valid = false;
     bool cont = false;
-    uint32_t slot = 0;
-    if (i < n) {
-      slot = Q.ray[qsel][i];
+    if (act) {
       const uint32_t* point = points + (size_t)(slot / npix) * YPT_SOBOL_POINT_DIMS;
       PathState ps;
       load_path(P, slot, ps);
@@ -242,13 +305,11 @@ __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene 
       FirstHitFeatures fh;
       fh.hit = 0u;
       bool first = ps.pathLength == 0;
-      if (h.tri == NO_HIT_ID) {
-        shade_miss(S.sv, fc, ps);
-      } else if constexpr (SSS) {
+      if constexpr (SSS) {
         GpuProbe pr;
         pr.nodes2 = S.nodes2; pr.nodes8 = S.nodes8; pr.tris = S.tris;
         pr.st.lds = nullptr; pr.st.ovf = Q.sssStack; pr.st.block = 64; pr.st.lane = threadIdx.x; pr.st.ldsDepth = 0;
-        pr.st.ovfStride = stride; pr.st.gid = blockIdx.x * blockDim.x + threadIdx.x; pr.st.sp = 0;
+        pr.st.ovfStride = hc.stride; pr.st.gid = blockIdx.x * blockDim.x + threadIdx.x; pr.st.sp = 0;
         shade_hit(S.sv, fc, point, NEE, ps, h, sr, &fh, &pr, &sr2);
       } else {
         shade_hit(S.sv, fc, point, NEE, ps, h, sr, &fh);  // features written below only when requested
