@@ -475,7 +475,68 @@ struct QueueSink {
   }
 };
 
-// grid-stride over the ray queue with a loop bound uniform per workgroup (see k_shade);
+// One camera vertex of slot `slot`: the previous vertex's connections added, the hit (or miss) shaded,
+// the state stored. MISS: the caller knows the ray left the scene (the surface code then drops out).
+// Returns whether the path continues; nconn: connections left to k_bdpt_connect (SPLIT).
+template <bool SSS, bool SPLIT, bool MISS>
+__device__ __forceinline__ bool cshade_slot(const FrameConstants& fc, const DevScene& S, const uint32_t* point,
+                                            const BdptBuffers& B, const CamSoA& C, const Queues& Q, uint32_t slot,
+                                            uint32_t stride, float* features, uint32_t& nconn) {
+  const uint32_t W = (uint32_t)fc.targetTexDimensions[0];
+  CameraPath c;
+  uint32_t lp, count;
+  load_cam(C, slot, c, lp, count);
+  fold_connections(C, slot, c.total);  // the previous vertex's visible connections, in queue order
+  uint32_t nvolIn = c.rs.n;
+  Hit h;
+  bool hit = false;
+  if constexpr (MISS) {
+    h.t = 0.f; h.b1 = 0.f; h.b2 = 0.f; h.tri = NO_HIT_ID;
+  } else {
+    float4 hv = Q.hit[slot];
+    h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.tri = __float_as_uint(hv.w);
+    hit = h.tri != NO_HIT_ID;
+  }
+  const LightNodePacked* nodes = lp != 0xFFFFFFFFu ? B.nodes + (size_t)lp * BDPT_MAX_LIGHT_VERTICES : nullptr;
+  QueueSink sink;
+  sink.Q = Q;
+  sink.first = slot * C.connCap;
+  sink.records = C.conn + sink.first;
+  sink.count = 0;
+  NodeSurfaceTable surfs;
+  surfs.p = nodes ? B.surf + (size_t)lp * BDPT_MAX_LIGHT_VERTICES : nullptr;
+  FirstHitFeatures fh;
+  fh.hit = 0u;
+  bool first = c.k == 0;
+  if constexpr (SSS) {
+    GpuProbe pr;
+    pr.nodes2 = S.nodes2; pr.nodes8 = S.nodes8; pr.tris = S.tris;
+    pr.st.lds = nullptr; pr.st.ovf = Q.sssStack; pr.st.block = 64; pr.st.lane = threadIdx.x; pr.st.ldsDepth = 0;
+    pr.st.ovfStride = stride; pr.st.gid = blockIdx.x * blockDim.x + threadIdx.x; pr.st.sp = 0;
+    camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, pr);
+    // the exit point of a subsurface event is the next vertex, with no ray to trace: take it now
+    if (c.pendingExit && c.alive) camera_vertex(S.sv, fc, point, c, false, h, nodes, count, surfs, sink, nullptr, pr);
+  } else {
+    NoSubsurface none;
+    if constexpr (SPLIT)
+      camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none, ConnectLater{C.defer + slot});
+    else
+      camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none);  // features written below only when requested
+  }
+  store_cam(C, slot, c, lp, count, nvolIn > 0 || c.rs.n > 0);
+  nconn = 0;
+  if constexpr (SPLIT) nconn = C.defer[slot].count;
+  C.pending[slot] = SPLIT ? nconn : sink.count;
+  if (features && first && fh.hit) {
+    uint32_t q = slot_texel(fc, B, slot);
+    store_features(features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * FEATURE_STRIDE, fh);
+  }
+  return c.alive;
+}
+
+// Grid-stride over the ray queue, misses finished during the walk and hits shaded in full waves
+// (HitCompactor). A miss has no exit vertex to take: an exit is shaded in the launch that took its
+// subsurface event, and load_cam starts every launch without one.
 // first: the first vertex, whose queue holds every slot; the slots are then shaded in slot (tile)
 // order rather than the screen order of the primary-ray queue
 // SSS: the scene has subsurface materials; probes then run here (stacks in Q.sssStack) and the exit
@@ -485,65 +546,28 @@ struct QueueSink {
 template <int MINW, bool SSS = false, bool SPLIT = false>
 __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B,
                                                           CamSoA C, Queues Q, int qsel, int first, float* features) {
+  __shared__ uint32_t pend[128];
   // counters[6 + qsel]: connection list of this launch (zero since the launch before the last one);
   // counters[6 + (qsel ^ 1)]: the next launch's, read by the connection launch before this one
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     Q.counters[4] = 0;
     if constexpr (SPLIT) Q.counters[6 + (qsel ^ 1)] = 0;
   }
-  const uint32_t n = Q.counters[qsel];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t W = (uint32_t)fc.targetTexDimensions[0];
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    uint32_t i = base + threadIdx.x;
+  HitCompactor hc;
+  hc.init(Q.counters[qsel]);
+  auto take = [&](uint32_t i) {
+    uint32_t s = first ? i : Q.ray[qsel][i];
+    if (__float_as_uint(Q.hit[s].w) != NO_HIT_ID) return s;
+    uint32_t nc;
+    cshade_slot<SSS, SPLIT, true>(fc, S, point, B, C, Q, s, hc.stride, features, nc);
+    return NO_SLOT;
+  };
+  uint32_t slot = 0;
+  bool act = false;
+  while (hc.next(pend, take, slot, act)) {
     bool cont = false;
-    uint32_t slot = 0, nconn = 0;
-    if (i < n) {
-      slot = first ? i : Q.ray[qsel][i];
-      CameraPath c;
-      uint32_t lp, count;
-      load_cam(C, slot, c, lp, count);
-      fold_connections(C, slot, c.total);  // the previous vertex's visible connections, in queue order
-      uint32_t nvolIn = c.rs.n;
-      float4 hv = Q.hit[slot];
-      Hit h;
-      h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.tri = __float_as_uint(hv.w);
-      bool hit = h.tri != NO_HIT_ID;
-      const LightNodePacked* nodes = lp != 0xFFFFFFFFu ? B.nodes + (size_t)lp * BDPT_MAX_LIGHT_VERTICES : nullptr;
-      QueueSink sink;
-      sink.Q = Q;
-      sink.first = slot * C.connCap;
-      sink.records = C.conn + sink.first;
-      sink.count = 0;
-      NodeSurfaceTable surfs;
-      surfs.p = nodes ? B.surf + (size_t)lp * BDPT_MAX_LIGHT_VERTICES : nullptr;
-      FirstHitFeatures fh;
-      fh.hit = 0u;
-      bool first = c.k == 0;
-      if constexpr (SSS) {
-        GpuProbe pr;
-        pr.nodes2 = S.nodes2; pr.nodes8 = S.nodes8; pr.tris = S.tris;
-        pr.st.lds = nullptr; pr.st.ovf = Q.sssStack; pr.st.block = 64; pr.st.lane = threadIdx.x; pr.st.ldsDepth = 0;
-        pr.st.ovfStride = stride; pr.st.gid = blockIdx.x * blockDim.x + threadIdx.x; pr.st.sp = 0;
-        camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, pr);
-        // the exit point of a subsurface event is the next vertex, with no ray to trace: take it now
-        if (c.pendingExit && c.alive) camera_vertex(S.sv, fc, point, c, false, h, nodes, count, surfs, sink, nullptr, pr);
-      } else {
-        NoSubsurface none;
-        if constexpr (SPLIT)
-          camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none, ConnectLater{C.defer + slot});
-        else
-          camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none);  // features written below only when requested
-      }
-      store_cam(C, slot, c, lp, count, nvolIn > 0 || c.rs.n > 0);
-      if constexpr (SPLIT) nconn = C.defer[slot].count;
-      C.pending[slot] = SPLIT ? nconn : sink.count;
-      cont = c.alive;
-      if (features && first && fh.hit) {
-        uint32_t q = slot_texel(fc, B, slot);
-        store_features(features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * FEATURE_STRIDE, fh);
-      }
-    }
+    uint32_t nconn = 0;
+    if (act) cont = cshade_slot<SSS, SPLIT, false>(fc, S, point, B, C, Q, slot, hc.stride, features, nconn);
     uint32_t ri = wave_append(&Q.counters[qsel ^ 1], cont);
     if (cont) Q.ray[qsel ^ 1][ri] = slot;
     if constexpr (SPLIT) {

@@ -116,17 +116,17 @@ __global__ __launch_bounds__(64, MINW) void k_shade_m(FusedArgs A, PathSoA P, Qu
   __shared__ uint32_t pend[128];
   if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;  // fetch counter of the next extension launch
   HitCompactor hc;
-  hc.n = Q.counters[qsel];
-  hc.stride = gridDim.x * blockDim.x;
-  hc.base = blockIdx.x * blockDim.x;
-  hc.npend = 0;
+  hc.init(Q.counters[qsel]);
   uint32_t slot = 0;
   bool act = false;
-  auto miss = [&](uint32_t s) {
+  auto take = [&](uint32_t i) {
+    uint32_t s = Q.ray[qsel][i];
+    if (__float_as_uint(Q.hit[s].w) != NO_HIT_ID) return s;
     const FusedJob& J = A.jobs[A.job_of(s)];
     finish_miss(J.S.sv, J.fc, P, s);
+    return NO_SLOT;
   };
-  while (hc.next(Q, qsel, pend, miss, slot, act)) {
+  while (hc.next(pend, take, slot, act)) {
     ShadowRay sr;
     sr.valid = false;
     bool cont = false;

@@ -218,24 +218,27 @@ __device__ __forceinline__ void finish_miss(const SceneView& S, const FrameConst
 // and with them in the same wave most lanes of the surface shading would idle. A wave walks the
 // queue 64 entries at a time, finishes the misses at once, and parks the slots of hits in LDS until
 // 64 of them are waiting; it then shades those 64 with every lane busy. The park buffer holds at
-// most 63 + 64 entries. Each workgroup is one wave, so the barriers only order its own LDS
-// accesses. `miss(slot)` finishes a miss. Returns false when the queue is exhausted and nothing is
-// parked; otherwise `slot` holds this lane's hit and `act` says whether the lane has one (fewer than
-// 64 only at the end).
+// most 63 + 64 entries. Each workgroup is one wave (64 threads), so the barriers only order its own
+// LDS accesses. `take(i)` returns the slot of queue entry i to shade, or NO_SLOT when it has
+// finished the entry itself (a miss). next() returns false when the queue is exhausted and nothing
+// is parked; otherwise `slot` holds this lane's hit and `act` says whether the lane has one (fewer
+// than 64 only at the end).
+static constexpr uint32_t NO_SLOT = 0xFFFFFFFFu;
 struct HitCompactor {
   uint32_t base, npend, n, stride;
-  template <class Miss>
-  __device__ __forceinline__ bool next(const Queues& Q, int qsel, uint32_t* pend, Miss&& miss, uint32_t& slot, bool& act) {
+  __device__ __forceinline__ void init(uint32_t count) {
+    n = count;
+    stride = gridDim.x * blockDim.x;
+    base = blockIdx.x * blockDim.x;
+    npend = 0;
+  }
+  template <class Take>
+  __device__ __forceinline__ bool next(uint32_t* pend, Take&& take, uint32_t& slot, bool& act) {
     const uint32_t lane = threadIdx.x;
     while (npend < 64 && base < n) {
       uint32_t i = base + lane;
-      bool hit = false;
-      uint32_t s = 0;
-      if (i < n) {
-        s = Q.ray[qsel][i];
-        hit = __float_as_uint(Q.hit[s].w) != NO_HIT_ID;
-        if (!hit) miss(s);
-      }
+      uint32_t s = i < n ? take(i) : NO_SLOT;
+      bool hit = s != NO_SLOT;
       unsigned long long m = __ballot(hit ? 1 : 0);
       if (hit) pend[npend + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = s;
       npend += (uint32_t)__popcll(m);
@@ -243,9 +246,9 @@ struct HitCompactor {
     }
     if (npend == 0) return false;
     __syncthreads();
-    uint32_t take = npend < 64 ? npend : 64;
-    npend -= take;
-    act = lane < take;
+    uint32_t take_n = npend < 64 ? npend : 64;
+    npend -= take_n;
+    act = lane < take_n;
     slot = act ? pend[npend + lane] : 0u;
     __syncthreads();
     return true;
@@ -282,14 +285,16 @@ __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene 
   __shared__ uint32_t pend[128];
   if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;  // fetch counter of the next extension launch
   HitCompactor hc;
-  hc.n = Q.counters[qsel];
-  hc.stride = gridDim.x * blockDim.x;
-  hc.base = blockIdx.x * blockDim.x;
-  hc.npend = 0;
+  hc.init(Q.counters[qsel]);
   uint32_t slot = 0;
   bool act = false;
-  auto miss = [&](uint32_t s) { finish_miss(S.sv, fc, P, s); };
-  while (hc.next(Q, qsel, pend, miss, slot, act)) {
+  auto take = [&](uint32_t i) {
+    uint32_t s = Q.ray[qsel][i];
+    if (__float_as_uint(Q.hit[s].w) != NO_HIT_ID) return s;
+    finish_miss(S.sv, fc, P, s);
+    return NO_SLOT;
+  };
+  while (hc.next(pend, take, slot, act)) {
     ShadowRay sr, sr2;
     sr.valid = false;
     sr2.valid = false;
