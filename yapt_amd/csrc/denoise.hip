@@ -89,6 +89,11 @@ __global__ __launch_bounds__(256, OCC) void k_conv3x3(ConvIn src, int N, int H, 
   constexpr int NIN = (NVI + 255) / 256, NW = (NVW + 255) / 256;
   __shared__ __attribute__((aligned(16))) uint8_t sIn[PH * PW * PIX];
   __shared__ __attribute__((aligned(16))) uint8_t sW[32 * COT * WROW];
+  // the bias, read by every epilogue: staged once (the first step's barrier orders it), so the
+  // epilogue does not wait on global loads between the matrix work and the stores
+  constexpr int BIAS_LDS = 256;
+  __shared__ float sBias[BIAS_LDS];
+  for (int c = threadIdx.x; c < BIAS_LDS; c += 256) sBias[c] = (bias && c < Cout) ? bias[c] : 0.f;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, col = lane & 31, half = lane >> 5;
   const int tilesX = (W + TW - 1) / TW, tiles = tilesX * ((H + TH - 1) / TH);
   const int Cin = src.Ca + src.Cb, nChunks = Cin / CIC;
@@ -228,7 +233,7 @@ __global__ __launch_bounds__(256, OCC) void k_conv3x3(ConvIn src, int N, int H, 
             const int co = cb + 8 * g + 4 * half;
             uint16_t o0[4], o1[4];
             for (int j = 0; j < 4; ++j) {
-              const float bj = bias ? bias[co + j] : 0.f;
+              const float bj = co + j < BIAS_LDS ? sBias[co + j] : (bias ? bias[co + j] : 0.f);
               float v0 = acc[r][t][4 * g + j] + bj, v1 = acc[r + 1][t][4 * g + j] + bj;
               if (relu && v0 < 0.f) v0 = 0.f;
               if (relu && v1 < 0.f) v1 = 0.f;
