@@ -121,7 +121,12 @@ YPT_HD vec3 operator*(vec3 a, vec3 b) { return vec3(a.x * b.x, a.y * b.y, a.z * 
 YPT_HD vec3 operator/(vec3 a, vec3 b) { return vec3(a.x / b.x, a.y / b.y, a.z / b.z); }
 YPT_HD vec3 operator*(vec3 a, float s) { return vec3(a.x * s, a.y * s, a.z * s); }
 YPT_HD vec3 operator*(float s, vec3 a) { return vec3(s * a.x, s * a.y, s * a.z); }
-YPT_HD vec3 operator/(vec3 a, float s) { return vec3(a.x / s, a.y / s, a.z / s); }
+// one correctly rounded reciprocal and three products: within an ulp of three quotients, at a third
+// of their cost on the GPU (both the device code and the CPU checker compile this definition)
+YPT_HD vec3 operator/(vec3 a, float s) {
+  const float r = 1.f / s;
+  return vec3(a.x * r, a.y * r, a.z * r);
+}
 YPT_HD vec3 operator-(vec3 a) { return vec3(-a.x, -a.y, -a.z); }
 YPT_HD float dot(vec3 a, vec3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 YPT_HD vec3 cross(vec3 a, vec3 b) {

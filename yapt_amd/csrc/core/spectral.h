@@ -99,7 +99,10 @@ YPT_HD Spectrum operator/(const Spectrum& a, const Spectrum& b) { return Spectru
 YPT_HD Spectrum operator+(const Spectrum& a, const Spectrum& b) { return Spectrum(a.s[0] + b.s[0], a.s[1] + b.s[1], a.s[2] + b.s[2], a.s[3] + b.s[3]); }
 YPT_HD Spectrum operator-(const Spectrum& a, const Spectrum& b) { return Spectrum(a.s[0] - b.s[0], a.s[1] - b.s[1], a.s[2] - b.s[2], a.s[3] - b.s[3]); }
 YPT_HD Spectrum operator*(const Spectrum& a, float v) { return Spectrum(a.s[0] * v, a.s[1] * v, a.s[2] * v, a.s[3] * v); }
-YPT_HD Spectrum operator/(const Spectrum& a, float v) { return Spectrum(a.s[0] / v, a.s[1] / v, a.s[2] / v, a.s[3] / v); }
+YPT_HD Spectrum operator/(const Spectrum& a, float v) {  // one reciprocal, four products (as vec3 / float)
+  const float r = 1.f / v;
+  return Spectrum(a.s[0] * r, a.s[1] * r, a.s[2] * r, a.s[3] * r);
+}
 YPT_HD Spectrum operator+(const Spectrum& a, float v) { return Spectrum(a.s[0] + v, a.s[1] + v, a.s[2] + v, a.s[3] + v); }
 YPT_HD Spectrum operator-(const Spectrum& a, float v) { return Spectrum(a.s[0] - v, a.s[1] - v, a.s[2] - v, a.s[3] - v); }
 
