@@ -17,7 +17,7 @@ extern "C" {
 #define YDN_E_DEVICE (-2)
 
 /* out = act(conv3x3(in, w) + bias), zero padding, stride 1; act = ReLU when relu != 0.
- * Cin % 16 == 0, Cout % 32 == 0; bias may be NULL. */
+ * Cin % 16 == 0, Cout % 32 == 0, Cout <= 256; bias may be NULL. */
 int ydn_conv3x3(const uint16_t* in, int N, int H, int W, int Cin, const uint16_t* w, const float* bias, int Cout, int relu,
                 uint16_t* out, void* stream);
 /* ydn_conv3x3 that also writes pooled[N][H/2][W/2][Cout] = ydn_pool2(out), from the tile in registers

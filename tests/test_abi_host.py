@@ -108,6 +108,11 @@ def test_denoiser_resampling_rejects_odd_and_empty_shapes():
         assert lib.ydn_conv3x3_up2cat(p, c1, p, c2, 1, H, W, p, None, cout, 1, p, None) == -1
     assert lib.ydn_conv3x3(p, 1, 8, 8, 24, p, None, 32, 1, p, None) == -1
     assert b"multiple of 16" in lib.ydn_last_error()
+    # the bias of one conv is staged in LDS, 256 channels at most
+    assert lib.ydn_conv3x3(p, 1, 8, 8, 32, p, p, 288, 1, p, None) == -1
+    assert b"at most 256" in lib.ydn_last_error()
+    assert lib.ydn_conv3x3_pool2(p, 1, 8, 8, 32, p, p, 288, 1, p, p, None) == -1
+    assert lib.ydn_conv3x3_up2cat(p, 16, p, 16, 1, 8, 8, p, p, 288, 1, p, None) == -1
 
 
 def test_denoiser_library_exports_every_declared_function():

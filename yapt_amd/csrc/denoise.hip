@@ -75,6 +75,8 @@ __device__ __forceinline__ bf16x8 lds8(const uint8_t* p) {
 // registers before the current step's matrix work, so their latency hides behind it. When the input has
 // one chunk, the weights stay in LDS from one job to the next of the same channel block.
 // Requirements: Cin % 16 == 0 (CIC divides Cin), Ca % 8 == 0, Cout % 32 == 0. block 256.
+constexpr int CONV_MAX_COUT = 256;  // output channels of one conv (the bias staged in LDS)
+
 template <int CIC, int COT, int RPW, int OCC>
 __global__ __launch_bounds__(256, OCC) void k_conv3x3(ConvIn src, int N, int H, int W, const uint16_t* __restrict__ w,
                                                     const float* __restrict__ bias, int Cout, int relu,
@@ -89,11 +91,11 @@ __global__ __launch_bounds__(256, OCC) void k_conv3x3(ConvIn src, int N, int H, 
   constexpr int NIN = (NVI + 255) / 256, NW = (NVW + 255) / 256;
   __shared__ __attribute__((aligned(16))) uint8_t sIn[PH * PW * PIX];
   __shared__ __attribute__((aligned(16))) uint8_t sW[32 * COT * WROW];
-  // the bias, read by every epilogue: staged once (the first step's barrier orders it), so the
-  // epilogue does not wait on global loads between the matrix work and the stores
-  constexpr int BIAS_LDS = 256;
-  __shared__ float sBias[BIAS_LDS];
-  for (int c = threadIdx.x; c < BIAS_LDS; c += 256) sBias[c] = (bias && c < Cout) ? bias[c] : 0.f;
+  // the bias, read by every epilogue: staged once (the first step's barrier orders it). The
+  // epilogue then issues no global load, whose wait (the vector-memory counter drains in order)
+  // would also wait for the next step's staging loads issued before the matrix work.
+  __shared__ float sBias[CONV_MAX_COUT];
+  for (int c = threadIdx.x; c < CONV_MAX_COUT; c += 256) sBias[c] = (bias && c < Cout) ? bias[c] : 0.f;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, col = lane & 31, half = lane >> 5;
   const int tilesX = (W + TW - 1) / TW, tiles = tilesX * ((H + TH - 1) / TH);
   const int Cin = src.Ca + src.Cb, nChunks = Cin / CIC;
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(256, OCC) void k_conv3x3(ConvIn src, int N, int H, 
             const int co = cb + 8 * g + 4 * half;
             uint16_t o0[4], o1[4];
             for (int j = 0; j < 4; ++j) {
-              const float bj = co + j < BIAS_LDS ? sBias[co + j] : (bias ? bias[co + j] : 0.f);
+              const float bj = sBias[co + j];
               float v0 = acc[r][t][4 * g + j] + bj, v1 = acc[r + 1][t][4 * g + j] + bj;
               if (relu && v0 < 0.f) v0 = 0.f;
               if (relu && v1 < 0.f) v1 = 0.f;
@@ -536,6 +538,7 @@ int check_conv(int N, int H, int W, int Cin, int Cout) {
   if (N > 65535) return fail(YDN_E_INVALID, "more than 65535 images");
   if (Cin % 16 != 0) return fail(YDN_E_INVALID, "Cin must be a multiple of 16");
   if (Cout % 32 != 0) return fail(YDN_E_INVALID, "Cout must be a multiple of 32");
+  if (Cout > CONV_MAX_COUT) return fail(YDN_E_INVALID, "Cout must be at most 256");
   if ((long)((W + 31) / 32) * ((H + 7) / 8) > 0x7FFFFFFFL) return fail(YDN_E_INVALID, "image too large");
   return YDN_OK;
 }
