@@ -87,6 +87,8 @@ struct GpuTracer {
 struct BdptBuffers {
   LightNodePacked* nodes;  // [path][BDPT_MAX_LIGHT_VERTICES]
   LightNodeSurface* surf;  // [path][BDPT_MAX_LIGHT_VERTICES]: surface of each vertex after the first
+  LightVertexCtx* pre;     // [path][BDPT_MAX_LIGHT_VERTICES]: connection context of each vertex after the
+                           // first (LightPrefix over the vertices before it), for k_bdpt_connect
   uint32_t* counts;        // vertices per light subpath (0: none)
   float* lambdas;          // hero wavelength per light subpath
   uint32_t* headers;       // existing subpaths in index order
@@ -178,6 +180,13 @@ __global__ void k_bdpt_node_surfaces(DevScene S, BdptBuffers B) {
   LightNodeSurface out;
   light_node_surface(S.sv, n, prev.pos, spectral_lambdas(B.lambdas[p]), out);
   B.surf[(size_t)p * BDPT_MAX_LIGHT_VERTICES + j] = out;
+  if (B.pre) {  // the context the connection loop carries into vertex j, made once per vertex
+    LightPrefix pre;
+    pre.begin();
+    for (uint32_t i = 0; i + 1u < j; ++i) pre.advance(unpack_node(nodes[i]), i);
+    pre.advance(prev, j - 1u);
+    B.pre[(size_t)p * BDPT_MAX_LIGHT_VERTICES + j] = pre.lv;
+  }
 }
 #endif
 
@@ -579,9 +588,10 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
 
 // The connections of the vertices that k_bdpt_cshade<..., SPLIT> shaded: one thread per entry of
 // its list, (slot, light vertex j) for every j below the light subpath's vertex count, so the
-// connections of one camera vertex run side by side instead of in one thread's loop. Thread j advances the light-vertex context over vertices
-// 0 .. j-1 as the loop does and makes connection j with the loop's own step
-// (connect_camera_to_node). Connection j writes record j of the slot (zero, filled by the occlusion
+// connections of one camera vertex run side by side instead of in one thread's loop. Thread j reads
+// the light-vertex context that the loop would carry into vertex j (k_bdpt_node_surfaces advances it
+// over vertices 0 .. j-1 once per light vertex, with the loop's own steps) and makes connection j
+// with the loop's own step (connect_camera_to_node). Connection j writes record j of the slot (zero, filled by the occlusion
 // kernel when unoccluded); the next shading launch adds records 0 .. count-1 in order, the same sum
 // as the loop's records with zero terms between them (the totals are never -0, so a +0 term changes
 // no bit).
@@ -608,7 +618,7 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_connect(FrameConstants fc, De
       Lambdas L = spectral_lambdas(C.orgLambda[slot].w);
       LightPrefix pre;
       pre.begin();
-      for (uint32_t i = 0; i < j; ++i) pre.advance(unpack_node(nodes[i]), i);
+      if (j > 0) pre.lv = B.pre[(size_t)lp * PER + j];
       LightNode nd = unpack_node(nodes[j]);
       valid = connect_camera_to_node(S.sv, fc, d.k, d.cv, d.ctx, d.rs, L, nd, j, pre.lv, surfs, pc);
     }

@@ -202,6 +202,7 @@ struct ypt_integrator {
   struct LightSet {
     DevBuf<LightNodePacked> nodes;
     DevBuf<LightNodeSurface> surf;  // surface of each stored light vertex, made once per pass
+    DevBuf<LightVertexCtx> pre;     // connection context of each stored light vertex, made with it
     DevBuf<uint32_t> counts, headers, headerCount, ovf;
     DevBuf<float> lambdas;
   } lset[2];
@@ -728,6 +729,7 @@ void ensure_bdpt(ypt_integrator* h) {
     for (auto& L : h->lset) {
       L.nodes.ensure((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
       L.surf.ensure((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
+      L.pre.ensure((size_t)npaths * BDPT_MAX_LIGHT_VERTICES);
       L.counts.ensure(npaths); L.lambdas.ensure(npaths); L.headers.ensure(npaths); L.headerCount.ensure(1);
     }
     h->bCamXYZ.ensure(npix);
@@ -920,7 +922,7 @@ void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count
       pts[(size_t)k * YPT_SOBOL_POINT_DIMS + dd] = sobol_uint(YPT_SOBOL_MATRICES, sample_begin + k, dd);
   if (h->sobolPoints.n < pts.size()) h->sobolPoints.alloc(pts.size());
   h->sobolPoints.upload(pts.data(), pts.size(), s);
-  BdptBuffers B;
+  BdptBuffers B{};
   B.perm = h->bPerm.p; B.camXYZ = h->bCamXYZ.p;
   B.recKeys = h->bKeys.p; B.recVals = h->bVals.p; B.recXYZ = h->bRecXYZ.p;
   B.ovf = h->bOvf.p; B.ovfStride = h->bOvfStride;
@@ -954,7 +956,7 @@ void render_bdpt(ypt_integrator* h, uint32_t sample_begin, uint32_t sample_count
   auto with_set = [&](uint32_t k, bool lightKernel) {
     ypt_integrator::LightSet& L = h->lset[k & 1u];
     BdptBuffers b = B;
-    b.nodes = L.nodes.p; b.surf = L.surf.p; b.counts = L.counts.p; b.lambdas = L.lambdas.p;
+    b.nodes = L.nodes.p; b.surf = L.surf.p; b.pre = L.pre.p; b.counts = L.counts.p; b.lambdas = L.lambdas.p;
     b.headers = L.headers.p; b.headerCount = L.headerCount.p;
     if (lightKernel) { b.ovf = L.ovf.p; b.ovfStride = b.npaths; }
     return b;
