@@ -57,6 +57,12 @@ struct ConvIn {
   int upA;            // a is at half resolution
 };
 
+// the value of the neighbouring lane (lane ^ 1): a DPP quad permutation [1, 0, 3, 2] in the VALU,
+// where a shuffle would go through the LDS crossbar
+__device__ __forceinline__ float xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
 __device__ __forceinline__ bf16x8 lds8(const uint8_t* p) {
   uint4 v = *reinterpret_cast<const uint4*>(p);
   return __builtin_bit_cast(bf16x8, v);
@@ -254,7 +260,7 @@ __global__ __launch_bounds__(256, OCC) void k_conv3x3(ConvIn src, int N, int H, 
               uint16_t q[4];
               for (int j = 0; j < 4; ++j) {
                 const float x00 = bf2f(o0[j]), x10 = bf2f(o1[j]);
-                const float x01 = __shfl_xor(x00, 1), x11 = __shfl_xor(x10, 1);
+                const float x01 = xor1(x00), x11 = xor1(x10);
                 q[j] = f2bf(0.25f * (((x00 + x01) + x10) + x11));
               }
               if ((col & 1) == 0 && y < H && x < W)
