@@ -395,22 +395,32 @@ YPT_HD void shade_hit(const SceneView& S, const FrameConstants& fc, const uint32
   sr.valid = false;
   Lambdas L = spectral_lambdas(ps.lambda);
   uint32_t instId = S.triInst[h.tri];
+  uint32_t i0 = S.idx[3 * h.tri], i1 = S.idx[3 * h.tri + 1], i2 = S.idx[3 * h.tri + 2];
   const InstanceGPU& inst = S.inst[instId];
   uint32_t matId = inst.material;
   const MaterialGPU& m = S.mats[matId];
+  // The material's texture slots are read here, before the vertex data: the texel fetch below is
+  // their first use, and a GPU waits for loads in the order it issued them, so a slot load issued
+  // after the vertex loads would have its wait cover all of them.
+  const bool textured = (m.tex[0] & m.tex[1] & m.tex[2] & m.tex[3]) >= 0;
   float b0 = 1.f - h.b1 - h.b2;
-  uint32_t i0 = S.idx[3 * h.tri], i1 = S.idx[3 * h.tri + 1], i2 = S.idx[3 * h.tri + 2];
+  // the vertex data of the hit, read together (positions, normals and, when the mesh has them,
+  // tangents) before any of it is used, so that their loads are in flight at once
   vec3 p1 = load3(S.pos, i0), p2 = load3(S.pos, i1), p3 = load3(S.pos, i2);
+  vec3 n0 = load3(S.nrm, i0), n1 = load3(S.nrm, i1), n2 = load3(S.nrm, i2);
+  float tv[3][4] = {};
+  if (S.tan) {
+    const uint32_t vi[3] = {i0, i1, i2};
+    for (int k = 0; k < 3; ++k)
+      for (int c = 0; c < 4; ++c) tv[k][c] = S.tan[4 * vi[k] + c];
+  }
   vec3 geometryNormal = normalize(cross(p2 - p1, p3 - p1));
-  vec3 normal = normalize(b0 * load3(S.nrm, i0) + h.b1 * load3(S.nrm, i1) + h.b2 * load3(S.nrm, i2));
+  vec3 normal = normalize(b0 * n0 + h.b1 * n1 + h.b2 * n2);
   vec3 tangent(1.f, 0.f, 0.f);  // meshHasValidTangents ? fetchMeshTangent : (1, 0, 0)
   if (S.tan) {
-    const float* t0 = S.tan + 4 * i0;
-    const float* t1 = S.tan + 4 * i1;
-    const float* t2 = S.tan + 4 * i2;
-    vec3 a(t0[0] * t0[3], t0[1] * t0[3], t0[2] * t0[3]);
-    vec3 b(t1[0] * t1[3], t1[1] * t1[3], t1[2] * t1[3]);
-    vec3 c(t2[0] * t2[3], t2[1] * t2[3], t2[2] * t2[3]);
+    vec3 a(tv[0][0] * tv[0][3], tv[0][1] * tv[0][3], tv[0][2] * tv[0][3]);
+    vec3 b(tv[1][0] * tv[1][3], tv[1][1] * tv[1][3], tv[1][2] * tv[1][3]);
+    vec3 c(tv[2][0] * tv[2][3], tv[2][1] * tv[2][3], tv[2][2] * tv[2][3]);
     tangent = normalize(b0 * a + h.b1 * b + h.b2 * c);
   }
   vec3 rayDir = normalize(ps.dir);
@@ -419,7 +429,9 @@ YPT_HD void shade_hit(const SceneView& S, const FrameConstants& fc, const uint32
   if (ps.pathLength >= fc.regularizeAfterVertices) ps.flags |= RS_REGULARIZE;
   bool regularizePath = (ps.flags & RS_REGULARIZE) != 0;
   SurfaceTexels tx;
-  sample_surface_texels(S, m, h.tri, h.b1, h.b2, tx);
+  tx.mask = 0;
+  tx.rgb2spec = S.rgb2spec;
+  if (textured) sample_surface_texels(S, m, h.tri, h.b1, h.b2, tx);
   SurfaceDef sd = make_surface(m, geometryNormal, normal, tangent, -rayDir, true, regularizePath, L, S.spec, &tx);
   vec3 hitPos = ps.origin + ps.dir * h.t;
 
