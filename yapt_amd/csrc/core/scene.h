@@ -149,11 +149,13 @@ YPT_HD void tri_vertices(const SceneView& S, uint32_t tri, vec3& p1, vec3& p2, v
 
 // Texture2D.SampleLevel(g_colorSampler, uv, 0): bilinear between texel centres, wrap addressing,
 // sRGB textures decoded per texel before the filter as an _SRGB format is
-YPT_HD void texel_rgba(const SceneView& S, const TextureGPU& t, uint32_t x, uint32_t y, float out[4]) {
-  uint32_t v = S.texels[t.offset + y * t.width + x];
+// (the decoding table exists whenever the scene has textures; it is read for every texel and the
+// value kept for sRGB textures, so that no branch separates the table reads)
+YPT_HD void texel_decode(const SceneView& S, const TextureGPU& t, uint32_t v, float out[4]) {
   for (int k = 0; k < 3; ++k) {
     uint32_t b = (v >> (8 * k)) & 0xFFu;
-    out[k] = t.srgb ? S.srgbLut[b] : (float)b / 255.f;
+    float lin = S.srgbLut[b];
+    out[k] = t.srgb ? lin : (float)b / 255.f;
   }
   out[3] = (float)(v >> 24) / 255.f;
 }
@@ -168,11 +170,15 @@ YPT_HD void sample_texture(const SceneView& S, uint32_t index, float u, float v,
   if (x0 >= t.width) x0 = t.width - 1u;
   if (y0 >= t.height) y0 = t.height - 1u;
   uint32_t x1 = x0 + 1u == t.width ? 0u : x0 + 1u, y1 = y0 + 1u == t.height ? 0u : y0 + 1u;
+  // the four texel words first, so their loads are in flight together, then their decoding
+  const uint32_t* row0 = S.texels + t.offset + y0 * t.width;
+  const uint32_t* row1 = S.texels + t.offset + y1 * t.width;
+  uint32_t w00 = row0[x0], w10 = row0[x1], w01 = row1[x0], w11 = row1[x1];
   float a[4], b[4], c[4], d[4];
-  texel_rgba(S, t, x0, y0, a);
-  texel_rgba(S, t, x1, y0, b);
-  texel_rgba(S, t, x0, y1, c);
-  texel_rgba(S, t, x1, y1, d);
+  texel_decode(S, t, w00, a);
+  texel_decode(S, t, w10, b);
+  texel_decode(S, t, w01, c);
+  texel_decode(S, t, w11, d);
   for (int k = 0; k < 4; ++k) {
     float top = a[k] + ax * (b[k] - a[k]);
     float bot = c[k] + ax * (d[k] - c[k]);
