@@ -157,16 +157,18 @@ def judge(net, W=1920, H=1080, ref_spp=4096, log=lambda m: None):
     return res
 
 
-def forward_roofline(net, W=1920, H=1080, reps=10, seed=0):
+def forward_roofline(net, W=1920, H=1080, reps=20, warmup=5, seed=0):
     """the conv kernels' achieved TFLOP/s over one U-Net forward pass at W x H (padded to the network's
-    multiple), timed with HIP events on the library's stream after a warm-up pass, on a seeded input"""
+    multiple), timed with HIP events on the library's stream over `reps` passes after `warmup` untimed
+    ones (the first passes run while the GPU's clocks ramp up), on a seeded input"""
     import torch
     from . import denoiser
     m = net.multiple
     Hp, Wp = (H + m - 1) // m * m, (W + m - 1) // m * m
     gen = torch.Generator(device="cuda").manual_seed(seed)
     x = denoiser.to_bf16_bits(torch.randn((1, Hp, Wp, denoiser.IN_CHANNELS), device="cuda", generator=gen) * 0.5)
-    net.forward(x)
+    for _ in range(warmup):
+        net.forward(x)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
