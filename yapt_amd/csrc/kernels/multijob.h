@@ -140,9 +140,7 @@ __global__ __launch_bounds__(64, MINW) void k_shade_m(FusedArgs A, PathSoA P, Qu
       float4 hv = Q.hit[slot];
       Hit h;
       h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.tri = __float_as_uint(hv.w);
-      FirstHitFeatures fh;
-      fh.hit = 0u;
-      shade_hit(J.S.sv, J.fc, point, J.nee != 0u, ps, h, sr, &fh);
+      shade_hit(J.S.sv, J.fc, point, J.nee != 0u, ps, h, sr, nullptr);  // fused steps write no features
       after_segment(point, ps);
       cont = ps.rayState == RAY_ALIVE && ps.pathLength < J.fc.maxRayDepth;
       store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);

@@ -283,6 +283,10 @@ template <bool NEE, int MINW, bool SSS = false>
 __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene S, const uint32_t* points, uint32_t npix,
                                                     PathSoA P, Queues Q, FilmBuffers F, int qsel, int writeFeatures) {
   __shared__ uint32_t pend[128];
+  // first-hit features, when requested, go through LDS rather than registers that would stay
+  // allocated across the whole shading step
+  __shared__ __attribute__((aligned(16))) unsigned char fhs[64 * sizeof(FirstHitFeatures)];
+  FirstHitFeatures* fhp = writeFeatures ? reinterpret_cast<FirstHitFeatures*>(fhs) + threadIdx.x : nullptr;
   if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[4] = 0;  // fetch counter of the next extension launch
   HitCompactor hc;
   hc.init(Q.counters[qsel]);
@@ -307,22 +311,20 @@ __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene 
       float4 hv = Q.hit[slot];
       Hit h;
       h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.tri = __float_as_uint(hv.w);
-      FirstHitFeatures fh;
-      fh.hit = 0u;
+      if (fhp) fhp->hit = 0u;
       bool first = ps.pathLength == 0;
       if constexpr (SSS) {
         GpuProbe pr;
         pr.nodes2 = S.nodes2; pr.nodes8 = S.nodes8; pr.tris = S.tris;
         pr.st.lds = nullptr; pr.st.ovf = Q.sssStack; pr.st.block = 64; pr.st.lane = threadIdx.x; pr.st.ldsDepth = 0;
         pr.st.ovfStride = hc.stride; pr.st.gid = blockIdx.x * blockDim.x + threadIdx.x; pr.st.sp = 0;
-        shade_hit(S.sv, fc, point, NEE, ps, h, sr, &fh, &pr, &sr2);
+        shade_hit(S.sv, fc, point, NEE, ps, h, sr, fhp, &pr, &sr2);
       } else {
-        shade_hit(S.sv, fc, point, NEE, ps, h, sr, &fh);  // features written below only when requested
+        shade_hit(S.sv, fc, point, NEE, ps, h, sr, fhp);
       }
       after_segment(point, ps);
-      cont = ps.rayState == RAY_ALIVE && ps.pathLength < fc.maxRayDepth;
-      store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);
-      if (writeFeatures && first && fh.hit) store_features(F.features + (size_t)slot * FEATURE_STRIDE, fh);
+      cont = ps.rayState == RAY_ALIVE && ps.pathLength < fc.maxRayDepth;      store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);
+      if (fhp && first && fhp->hit) store_features(F.features + (size_t)slot * FEATURE_STRIDE, *fhp);
     }
     uint32_t si = wave_append(&Q.counters[2], sr.valid);
     if (sr.valid) {
