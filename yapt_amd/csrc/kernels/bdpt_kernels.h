@@ -487,10 +487,11 @@ struct QueueSink {
 // One camera vertex of slot `slot`: the previous vertex's connections added, the hit (or miss) shaded,
 // the state stored. MISS: the caller knows the ray left the scene (the surface code then drops out).
 // Returns whether the path continues; nconn: connections left to k_bdpt_connect (SPLIT).
+// fhp: this lane's first-hit feature record (in LDS), or null when no features are written.
 template <bool SSS, bool SPLIT, bool MISS>
 __device__ __forceinline__ bool cshade_slot(const FrameConstants& fc, const DevScene& S, const uint32_t* point,
                                             const BdptBuffers& B, const CamSoA& C, const Queues& Q, uint32_t slot,
-                                            uint32_t stride, float* features, uint32_t& nconn) {
+                                            uint32_t stride, float* features, FirstHitFeatures* fhp, uint32_t& nconn) {
   const uint32_t W = (uint32_t)fc.targetTexDimensions[0];
   CameraPath c;
   uint32_t lp, count;
@@ -514,31 +515,30 @@ __device__ __forceinline__ bool cshade_slot(const FrameConstants& fc, const DevS
   sink.count = 0;
   NodeSurfaceTable surfs;
   surfs.p = nodes ? B.surf + (size_t)lp * BDPT_MAX_LIGHT_VERTICES : nullptr;
-  FirstHitFeatures fh;
-  fh.hit = 0u;
+  if (fhp) fhp->hit = 0u;
   bool first = c.k == 0;
   if constexpr (SSS) {
     GpuProbe pr;
     pr.nodes2 = S.nodes2; pr.nodes8 = S.nodes8; pr.tris = S.tris;
     pr.st.lds = nullptr; pr.st.ovf = Q.sssStack; pr.st.block = 64; pr.st.lane = threadIdx.x; pr.st.ldsDepth = 0;
     pr.st.ovfStride = stride; pr.st.gid = blockIdx.x * blockDim.x + threadIdx.x; pr.st.sp = 0;
-    camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, pr);
+    camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, fhp, pr);
     // the exit point of a subsurface event is the next vertex, with no ray to trace: take it now
     if (c.pendingExit && c.alive) camera_vertex(S.sv, fc, point, c, false, h, nodes, count, surfs, sink, nullptr, pr);
   } else {
     NoSubsurface none;
     if constexpr (SPLIT)
-      camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none, ConnectLater{C.defer + slot});
+      camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, fhp, none, ConnectLater{C.defer + slot});
     else
-      camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, &fh, none);  // features written below only when requested
+      camera_vertex(S.sv, fc, point, c, hit, h, nodes, count, surfs, sink, fhp, none);
   }
   store_cam(C, slot, c, lp, count, nvolIn > 0 || c.rs.n > 0);
   nconn = 0;
   if constexpr (SPLIT) nconn = C.defer[slot].count;
   C.pending[slot] = SPLIT ? nconn : sink.count;
-  if (features && first && fh.hit) {
+  if (fhp && first && fhp->hit) {
     uint32_t q = slot_texel(fc, B, slot);
-    store_features(features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * FEATURE_STRIDE, fh);
+    store_features(features + (size_t)((q >> 16) * W + (q & 0xFFFFu)) * FEATURE_STRIDE, *fhp);
   }
   return c.alive;
 }
@@ -556,6 +556,9 @@ template <int MINW, bool SSS = false, bool SPLIT = false>
 __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, DevScene S, const uint32_t* point, BdptBuffers B,
                                                           CamSoA C, Queues Q, int qsel, int first, float* features) {
   __shared__ uint32_t pend[128];
+  // first-hit features, when requested, through LDS (no registers held for them across the step)
+  __shared__ __attribute__((aligned(16))) unsigned char fhs[64 * sizeof(FirstHitFeatures)];
+  FirstHitFeatures* fhp = features ? reinterpret_cast<FirstHitFeatures*>(fhs) + threadIdx.x : nullptr;
   // counters[6 + qsel]: connection list of this launch (zero since the launch before the last one);
   // counters[6 + (qsel ^ 1)]: the next launch's, read by the connection launch before this one
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -568,7 +571,7 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
     uint32_t s = first ? i : Q.ray[qsel][i];
     if (__float_as_uint(Q.hit[s].w) != NO_HIT_ID) return s;
     uint32_t nc;
-    cshade_slot<SSS, SPLIT, true>(fc, S, point, B, C, Q, s, hc.stride, features, nc);
+    cshade_slot<SSS, SPLIT, true>(fc, S, point, B, C, Q, s, hc.stride, features, fhp, nc);
     return NO_SLOT;
   };
   uint32_t slot = 0;
@@ -576,7 +579,7 @@ __global__ __launch_bounds__(64, MINW) void k_bdpt_cshade(FrameConstants fc, Dev
   while (hc.next(pend, take, slot, act)) {
     bool cont = false;
     uint32_t nconn = 0;
-    if (act) cont = cshade_slot<SSS, SPLIT, false>(fc, S, point, B, C, Q, slot, hc.stride, features, nconn);
+    if (act) cont = cshade_slot<SSS, SPLIT, false>(fc, S, point, B, C, Q, slot, hc.stride, features, fhp, nconn);
     uint32_t ri = wave_append(&Q.counters[qsel ^ 1], cont);
     if (cont) Q.ray[qsel ^ 1][ri] = slot;
     if constexpr (SPLIT) {
