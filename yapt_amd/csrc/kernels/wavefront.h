@@ -323,7 +323,8 @@ __global__ __launch_bounds__(64, MINW) void k_shade(FrameConstants fc, DevScene 
         shade_hit(S.sv, fc, point, NEE, ps, h, sr, fhp);
       }
       after_segment(point, ps);
-      cont = ps.rayState == RAY_ALIVE && ps.pathLength < fc.maxRayDepth;      store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);
+      cont = ps.rayState == RAY_ALIVE && ps.pathLength < fc.maxRayDepth;
+      store_path(P, slot, ps, nvolIn > 0 || ps.nvol > 0);
       if (fhp && first && fhp->hit) store_features(F.features + (size_t)slot * FEATURE_STRIDE, *fhp);
     }
     uint32_t si = wave_append(&Q.counters[2], sr.valid);
