@@ -58,7 +58,8 @@ struct ConvIn {
 };
 
 // the value of the neighbouring lane (lane ^ 1): a DPP quad permutation [1, 0, 3, 2] in the VALU,
-// where a shuffle would go through the LDS crossbar
+// where a shuffle would go through the LDS crossbar. Both lanes of the pair must be active (the
+// pooled epilogue runs with the whole wavefront).
 __device__ __forceinline__ float xor1(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
